@@ -1,0 +1,226 @@
+"""ctypes binding of the FP64 CPU restatement in oracle/liboracle.so.
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_L = ctypes.c_long
+
+_SIGS = {
+    "or_collide_cell": (None, [_P, _P, _D, _D, _D]),
+    "or_lbm_step": (None, [_I, _I, _I, _P, _D, _D, _P, _P, _P, _P, _P, _I, _P]),
+    "or_velocity_walls": (None, [_I, _I, _I, _P, _D, _P, _P, _P, _I, _P]),
+    "or_lbm_moments": (None, [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P]),
+    "or_lbm_init_equilibrium": (None, [_L, _P, _P, _P]),
+    "or_lbm_conserved": (None, [_L, _P, _P, _P]),
+    "or_dirac_1d": (_D, [_I, _D]),
+    "or_dirac_support": (_I, [_I]),
+    "or_dirac_half_support": (_D, [_I]),
+    "or_support_base": (_L, [_I, _D]),
+    "or_interpolate": (None, [_I, _I, _I, _I, _P, _P, _L, _P, _P]),
+    "or_spread": (None, [_I, _I, _I, _I, _P, _P, _L, _P, _P]),
+    "or_capsule_forces": (None, [_L, _L, _P, _P, _P, _D, _D, _I, _I, _P]),
+    "or_build_links": (_L, [_L, _L, _P, _P, _P, _P, _P]),
+    "or_build_pairs": (_L, [_L, _L, _P, _P, _L, _P, _P, _P, _P]),
+    "or_rbc_forces": (None, [_L, _L, _P, _P, _L, _P, _P, _L, _P, _P, _D, _P, _D, _D, _P, _P]),
+    "or_particle_update": (None, [_L, _L, _P, _P, _P, _P]),
+    "or_moment_of_inertia": (None, [_L, _L, _P, _P, _P]),
+    "or_ray_hits_z": (_L, [_L, _L, _P, _P, _L, _P]),
+}
+
+
+class RbcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "shear_modulus", "L0", "k_area_global", "k_area_local",
+        "k_volume", "vol_desired", "k_bend", "theta0")]
+
+
+def _load():
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ----------------------------------------------------------------- lattice
+def lbm_run(f, force, phase, mask, periodic, omega_out, omega_in, steps,
+            wall_axis=1, wall_u=None):
+    """Advance AoS populations f (nx,ny,nz,19) by `steps` collide-and-stream
+    steps; returns a new array."""
+    f = _c(f, np.float64).copy()
+    nx, ny, nz = f.shape[:3]
+    tmp = np.empty_like(f)
+    force = _c(force, np.float64)
+    phase = _c(phase, np.float64)
+    mask = _c(mask, np.uint8)
+    per = _c(periodic, np.int32)
+    wu = _c(np.zeros(6) if wall_u is None else wall_u, np.float64)
+    for _ in range(steps):
+        lib.or_lbm_step(nx, ny, nz, _p(per), omega_out, omega_in, _p(f), _p(tmp),
+                        _p(force), _p(phase), _p(mask), wall_axis, _p(wu))
+    return f
+
+
+def lbm_velocity_walls(f, force, mask, periodic, omega_out, wall_axis=1, wall_u=None):
+    f = _c(f, np.float64).copy()
+    nx, ny, nz = f.shape[:3]
+    wu = _c(np.zeros(6) if wall_u is None else wall_u, np.float64)
+    lib.or_velocity_walls(nx, ny, nz, _p(_c(periodic, np.int32)), omega_out, _p(f),
+                          _p(_c(force, np.float64)), _p(_c(mask, np.uint8)), wall_axis, _p(wu))
+    return f
+
+
+def lbm_moments(f, force, mask=None, wall_axis=1, wall_u=None):
+    f = _c(f, np.float64)
+    nx, ny, nz = f.shape[:3]
+    mask = np.zeros((nx, ny, nz), np.uint8) if mask is None else _c(mask, np.uint8)
+    wu = _c(np.zeros(6) if wall_u is None else wall_u, np.float64)
+    rho = np.empty((nx, ny, nz))
+    u = np.empty((nx, ny, nz, 3))
+    lib.or_lbm_moments(nx, ny, nz, _p(f), _p(_c(force, np.float64)), _p(mask), wall_axis,
+                       _p(wu), _p(rho), _p(u))
+    return rho, u
+
+
+def lbm_init_equilibrium(rho, u):
+    rho = _c(rho, np.float64)
+    u = _c(u, np.float64)
+    f = np.empty(rho.shape + (19,))
+    lib.or_lbm_init_equilibrium(rho.size, _p(rho), _p(u), _p(f))
+    return f
+
+
+def lbm_conserved(f, mask=None):
+    f = _c(f, np.float64)
+    ncell = f.size // 19
+    out = np.zeros(4)
+    m = None if mask is None else _c(mask, np.uint8)
+    lib.or_lbm_conserved(ncell, _p(f), None if m is None else _p(m), _p(out))
+    return out
+
+
+def collide_cell(f, F, phase, omega_out, omega_in):
+    g = _c(f, np.float64).copy()
+    lib.or_collide_cell(_p(g), _p(_c(F, np.float64)), phase, omega_out, omega_in)
+    return g
+
+
+# ------------------------------------------------------- immersed boundary
+KERNELS = {"roma": 0, "peskin": 1, "tophat": 2}
+
+
+def dirac(kernel, r):
+    k = KERNELS.get(kernel, kernel)
+    return np.array([lib.or_dirac_1d(k, float(v)) for v in np.ravel(r)]).reshape(np.shape(r))
+
+
+def support_base(kernel, x):
+    k = KERNELS.get(kernel, kernel)
+    return np.array([lib.or_support_base(k, float(v)) for v in np.ravel(x)],
+                    dtype=np.int64).reshape(np.shape(x))
+
+
+def interpolate(kernel, ufield, periodic, pos):
+    k = KERNELS.get(kernel, kernel)
+    u = _c(ufield, np.float64)
+    pos = _c(pos, np.float64)
+    nx, ny, nz = u.shape[:3]
+    vel = np.empty((len(pos), 3))
+    lib.or_interpolate(k, nx, ny, nz, _p(_c(periodic, np.int32)), _p(u), len(pos), _p(pos), _p(vel))
+    return vel
+
+
+def spread(kernel, shape, periodic, pos, fvert, base=None):
+    k = KERNELS.get(kernel, kernel)
+    field = np.zeros(tuple(shape) + (3,)) if base is None else _c(base, np.float64).copy()
+    pos = _c(pos, np.float64)
+    lib.or_spread(k, shape[0], shape[1], shape[2], _p(_c(periodic, np.int32)), _p(field),
+                  len(pos), _p(pos), _p(_c(fvert, np.float64)))
+    return field
+
+
+def capsule_forces(tri, rest_pos, pos, G, C, model="skalak", bug_compatible=True):
+    tri = _c(tri, np.int32)
+    rest_pos = _c(rest_pos, np.float64)
+    pos = _c(pos, np.float64)
+    F = np.zeros_like(pos)
+    lib.or_capsule_forces(len(pos), len(tri), _p(tri), _p(rest_pos), _p(pos), G, C,
+                          0 if model == "skalak" else 1, int(bug_compatible), _p(F))
+    return F
+
+
+def build_topology(tri, rest_pos):
+    tri = _c(tri, np.int32)
+    rest_pos = _c(rest_pos, np.float64)
+    nv, nt = len(rest_pos), len(tri)
+    li = np.zeros((3 * nt, 2), np.int32)
+    ll = np.zeros(3 * nt)
+    lt = np.zeros((3 * nt, 2), np.int32)
+    nl = lib.or_build_links(nv, nt, _p(tri), _p(rest_pos), _p(li), _p(ll), _p(lt))
+    pi = np.zeros((3 * nt, 4), np.int32)
+    pc = np.zeros((3 * nt, 2))
+    npair = lib.or_build_pairs(nv, nt, _p(tri), _p(rest_pos), nl, _p(li), _p(lt), _p(pi), _p(pc))
+    return dict(link_i=li[:nl].copy(), link_len=ll[:nl].copy(), link_t=lt[:nl].copy(),
+                pair_i=pi[:npair].copy(), pair_cs0=pc[:npair].copy())
+
+
+def rbc_forces(tri, tri_area0, topo, area0_total, params, area, volume, pos):
+    tri = _c(tri, np.int32)
+    pos = _c(pos, np.float64)
+    F = np.zeros_like(pos)
+    p = RbcParams(*[float(v) for v in params])
+    li, ll = _c(topo["link_i"], np.int32), _c(topo["link_len"], np.float64)
+    pi, pc = _c(topo["pair_i"], np.int32), _c(topo["pair_cs0"], np.float64)
+    lib.or_rbc_forces(len(pos), len(tri), _p(tri), _p(_c(tri_area0, np.float64)),
+                      len(ll), _p(li), _p(ll), len(pc), _p(pi), _p(pc), area0_total,
+                      ctypes.byref(p), area, volume, _p(pos), _p(F))
+    return F
+
+
+def particle_update(tri, pos):
+    tri = _c(tri, np.int32)
+    pos = _c(pos, np.float64)
+    out = np.zeros(5)
+    bb = np.zeros(6)
+    lib.or_particle_update(len(pos), len(tri), _p(tri), _p(pos), _p(out), _p(bb))
+    return dict(area=out[0], volume=out[1], com=out[2:5].copy(), bbox=bb)
+
+
+def moment_of_inertia(tri, pos):
+    tri = _c(tri, np.int32)
+    pos = _c(pos, np.float64)
+    out = np.zeros(9)
+    lib.or_moment_of_inertia(len(pos), len(tri), _p(tri), _p(pos), _p(out))
+    return out.reshape(3, 3)
+
+
+def ray_hits_z(tri, pos, cap=1 << 20):
+    tri = _c(tri, np.int32)
+    pos = _c(pos, np.float64)
+    hits = np.zeros((cap, 3))
+    n = lib.or_ray_hits_z(len(pos), len(tri), _p(tri), _p(pos), cap, _p(hits))
+    return hits[:min(n, cap)].copy()
