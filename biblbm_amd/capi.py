@@ -1,0 +1,130 @@
+"""ctypes binding of libbiblbm.so (declarations in include/biblbm.h).
+
+The library is required: importing this module without the built library
+raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbiblbm.so")
+
+BL_OK = 0
+STORE = {"f64": 0, "f32": 1, "f16s": 2}
+KERNEL = {"roma": 0, "peskin": 1, "tophat": 2}
+MODEL = {"capsule_skalak": 0, "capsule_neohookean": 1, "rbc": 2}
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PI = ctypes.POINTER(ctypes.c_int)
+_PLL = ctypes.POINTER(ctypes.c_longlong)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes (all return int status, except bl_last_error)
+SIGNATURES = {
+    "bl_init": [_I],
+    "bl_shutdown": [],
+    "bl_device_info": [ctypes.c_char_p, _I, _PI, _PD],
+    "bl_lattice_create": [_I, _I, _I, _P, _I, ctypes.POINTER(_P)],
+    "bl_lattice_destroy": [_P],
+    "bl_lattice_set_dynamics": [_P, _D, _D],
+    "bl_lattice_set_mask": [_P, _P],
+    "bl_lattice_set_velocity_walls": [_P, _I, _P, _P],
+    "bl_lattice_set_body_force": [_P, _P],
+    "bl_lattice_set_force_field": [_P, _P],
+    "bl_lattice_set_phase": [_P, _P],
+    "bl_lattice_init_equilibrium": [_P, _P, _P],
+    "bl_lattice_put_populations": [_P, _P],
+    "bl_lattice_get_populations": [_P, _P],
+    "bl_lattice_initialize": [_P],
+    "bl_collide_stream": [_P, _I],
+    "bl_lattice_get_moments": [_P, _P, _P],
+    "bl_lattice_get_force": [_P, _P],
+    "bl_lattice_conserved": [_P, _P],
+    "bl_lattice_steps": [_P, _PLL],
+    "bl_lattice_set_launch": [_P, _I, _I],
+    "bl_lattice_stream": [_P, ctypes.POINTER(_P)],
+    "bl_lattice_time_collide": [_P, _I, _PD],
+    "bl_ibm_create": [_P, _I, ctypes.POINTER(_P)],
+    "bl_ibm_destroy": [_P],
+    "bl_shape_add": [_P, _I, _I, _P, _P, _PI],
+    "bl_shape_info": [_P, _I, _P],
+    "bl_particle_add": [_P, _I, _I, _P, _I, _P, _I, _PI],
+    "bl_ibm_init": [_P],
+    "bl_ibm_interpolate": [_P],
+    "bl_ibm_compute_forces": [_P],
+    "bl_ibm_spread": [_P],
+    "bl_ibm_move": [_P],
+    "bl_ibm_step": [_P],
+    "bl_coupled_steps": [_P, _I, _I],
+    "bl_ibm_count": [_P, _PI, _PI],
+    "bl_particle_get": [_P, _I, _I, _P],
+    "bl_particle_set_positions": [_P, _I, _P],
+    "bl_particle_observables": [_P, _I, _P],
+    "bl_ibm_get_phase": [_P, _P],
+    "bl_ibm_stage_times": [_P, _P],
+    "bl_ensemble_create": [_I, _P, _P, _I, ctypes.POINTER(_P)],
+    "bl_ensemble_destroy": [_P],
+    "bl_ensemble_lattice": [_P, _I, ctypes.POINTER(_P)],
+    "bl_ensemble_ibm": [_P, _I, ctypes.POINTER(_P)],
+    "bl_ensemble_steps": [_P, _I, _I],
+    "bl_ensemble_time_steps": [_P, _I, _I, _PD],
+    "bl_tune_collide": [_P, _P, _I],
+}
+
+
+class BlError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "biblbm_amd: %s is not built; run `python __graft_entry__.py build` "
+            "(there is no CPU fallback)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.bl_last_error.restype = ctypes.c_char_p
+    lib.bl_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = ctypes.c_int
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def exported_symbols():
+    return [n for n in list(SIGNATURES) + ["bl_last_error"] if hasattr(lib, n)]
+
+
+def check(rc, what=""):
+    if rc != BL_OK:
+        msg = lib.bl_last_error().decode(errors="replace")
+        raise BlError("%s failed (status %d): %s" % (what, rc, msg))
+
+
+def call(name, *args):
+    check(getattr(lib, name)(*args), name)
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)

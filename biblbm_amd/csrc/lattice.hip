@@ -1,0 +1,1214 @@
+// D3Q19 Guo-VOF collide-and-stream for gfx950.
+//
+// One fused pass per time step over structure-of-arrays populations, in place
+// with the AA access pattern:
+//   even step: read f_i(x) from plane i at x, collide, write f*_i(x) to plane
+//              opp(i) at x (purely local);
+//   odd step:  read f_i(x) from plane opp(i) at x - c_i, collide, write
+//              f*_i(x) to plane i at x + c_i.
+// Every slot is read and written by the same thread, so no step needs a
+// second lattice.  Non-periodic faces reflect (a population leaving the box
+// re-enters the same cell in the opposite direction); wall cells (bounce-back,
+// velocity wall) sit inside the box.
+//
+// The collision restates GuoExternalForceVOFDynamics::collide
+// (src/fsi/VariableViscosityDynamics.hh:151-172): fBar moments
+// (src/latticeBoltzmann/momentTemplates3D.h:375-399), u = j/rho + F/2
+// (src/basicDynamics/externalForceDynamics.hh:59-73), phase-dependent omega
+// (src/fsi/VariableViscosityDynamics.hh:131-141), BGK in the fBar form
+// (src/latticeBoltzmann/dynamicsTemplates3D.h:759-838) and the Guo source
+// (src/fsi/VariableViscosityDynamics.hh:47-129).
+#include "lattice.hpp"
+
+#include <cstring>
+#include <vector>
+
+namespace bl {
+
+// ------------------------------------------------------------ storage I/O
+template <class S>
+struct Pop;
+template <>
+struct Pop<double> {
+    using C = double;
+    using F = double;
+    __device__ static C ld(const double* A, unsigned k, int) { return A[k]; }
+    __device__ static void st(double* A, unsigned k, int, C v) { A[k] = v; }
+};
+template <>
+struct Pop<float> {
+    using C = float;
+    using F = float;
+    __device__ static C ld(const float* A, unsigned k, int) { return A[k]; }
+    __device__ static void st(float* A, unsigned k, int, C v) { A[k] = v; }
+};
+template <>
+struct Pop<__half> {
+    using C = float;
+    using F = float;
+    __device__ static C ld(const __half* A, unsigned k, int i)
+    {
+        return __half2float(A[k]) * (float)(tw(i) / kF16Scale);
+    }
+    __device__ static void st(__half* A, unsigned k, int i, C v)
+    {
+        A[k] = __float2half_rn(v * (float)(kF16Scale / tw(i)));
+    }
+};
+
+LatView view_of(const bl_lattice_s* L)
+{
+    LatView v;
+    v.nx = L->nx; v.ny = L->ny; v.nz = L->nz;
+    v.ncell = (unsigned)L->ncell;
+    v.stride = L->stride;
+    for (int d = 0; d < 3; ++d) v.per[d] = L->periodic[d];
+    v.pops = L->pops;
+    v.mask = L->mask;
+    v.phase = L->phase;
+    v.flags = L->tile_flags;
+    v.force = L->force;
+    for (int d = 0; d < 3; ++d) v.g[d] = L->g[d];
+    v.om0 = L->omega_c0;
+    v.om1 = L->omega_c1;
+    v.om_base = L->omega_out;
+    v.ntx = L->ntx; v.nty = L->nty; v.ntz = L->ntz;
+    v.wall_axis = L->wall_axis;
+    for (int k = 0; k < 6; ++k) v.wall_u[k] = L->wall_u[k];
+    v.xcd_remap = L->xcd_remap;
+    return v;
+}
+
+// --------------------------------------------------------- index helpers
+__device__ __forceinline__ int wrapc(int v, int n, int per)
+{
+    if (v < 0) return per ? v + n : -1;
+    if (v >= n) return per ? v - n : -1;
+    return v;
+}
+
+// Workgroups are dealt round-robin over the 8 XCDs; remap so that each XCD
+// walks one contiguous range of cells and neighbouring blocks share its L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nb)
+{
+    const unsigned per = nb >> 3, rem = nb & 7u;
+    const unsigned xcd = bid & 7u, k = bid >> 3;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+struct Nbr {
+    int X[3], Y[3], Z[3];
+    __device__ __forceinline__ void init(const LatView& v, int x, int y, int z)
+    {
+        X[0] = wrapc(x - 1, v.nx, v.per[0]); X[1] = x; X[2] = wrapc(x + 1, v.nx, v.per[0]);
+        Y[0] = wrapc(y - 1, v.ny, v.per[1]); Y[1] = y; Y[2] = wrapc(y + 1, v.ny, v.per[1]);
+        Z[0] = wrapc(z - 1, v.nz, v.per[2]); Z[1] = z; Z[2] = wrapc(z + 1, v.nz, v.per[2]);
+    }
+    // cell index of x + (dx,dy,dz), or -1 outside a non-periodic face
+    __device__ __forceinline__ int at(const LatView& v, int dx, int dy, int dz) const
+    {
+        const int a = X[dx + 1], b = Y[dy + 1], c = Z[dz + 1];
+        if ((a | b | c) < 0) return -1;
+        return (a * v.ny + b) * v.nz + c;
+    }
+};
+
+__device__ __forceinline__ void decode(unsigned c, int ny, int nz, int& x, int& y, int& z)
+{
+    z = (int)(c % (unsigned)nz);
+    const unsigned r = c / (unsigned)nz;
+    y = (int)(r % (unsigned)ny);
+    x = (int)(r / (unsigned)ny);
+}
+
+// Post-stream population f_i(x) after `swapped` (odd number of completed
+// steps) or natural layout: returns (plane, cell).
+__device__ __forceinline__ long post_slot(const LatView& v, const Nbr& n, long c, int i, bool swapped,
+                                         int& plane)
+{
+    if (!swapped) { plane = i; return c; }
+    const long s = n.at(v, -cx(i), -cy(i), -cz(i));
+    if (s < 0) { plane = i; return c; }
+    plane = opp(i);
+    return s;
+}
+
+template <class S>
+__device__ __forceinline__ void load_post(const LatView& v, const Nbr& n, long c, bool swapped,
+                                          typename Pop<S>::C f[Q])
+{
+    const S* A = (const S*)v.pops;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int pl;
+        const long k = post_slot(v, n, c, i, swapped, pl);
+        f[i] = Pop<S>::ld(A + pl * v.stride, (unsigned)k, i);
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void store_post(const LatView& v, const Nbr& n, long c, bool swapped,
+                                           const typename Pop<S>::C f[Q])
+{
+    S* A = (S*)v.pops;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int pl;
+        const long k = post_slot(v, n, c, i, swapped, pl);
+        Pop<S>::st(A + pl * v.stride, (unsigned)k, i, f[i]);
+    }
+}
+
+// External force of cell c.
+template <class FT, class C>
+__device__ __forceinline__ void cell_force(const LatView& v, long c, int x, int y, int z, C F[3])
+{
+    F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2];
+    if (v.flags) {
+        const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
+        if (v.flags[t]) {
+            const FT* Fv = (const FT*)v.force;
+            F[0] = (C)Fv[c];
+            F[1] = (C)Fv[v.stride + c];
+            F[2] = (C)Fv[2 * v.stride + c];
+        }
+    }
+}
+
+// ------------------------------------------------------------- collision
+template <class C>
+__device__ __forceinline__ C sum_rho(const C f[Q])
+{
+    return f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] +
+           f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+}
+
+template <class C>
+__device__ __forceinline__ void sum_j(const C f[Q], C j[3])
+{
+    const C xm = f[1] + f[4] + f[5] + f[6] + f[7];
+    const C xp = f[10] + f[13] + f[14] + f[15] + f[16];
+    const C ym = f[2] + f[4] + f[8] + f[9] + f[14];
+    const C yp = f[5] + f[11] + f[13] + f[17] + f[18];
+    const C zm = f[3] + f[6] + f[8] + f[16] + f[18];
+    const C zp = f[7] + f[9] + f[12] + f[15] + f[17];
+    j[0] = xp - xm;
+    j[1] = yp - ym;
+    j[2] = zp - zm;
+}
+
+template <class C, bool FORCE>
+__device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
+{
+    const C rhoBar = sum_rho(f);
+    C j[3];
+    sum_j(f, j);
+    const C invRho = C(1) / (rhoBar + C(1));
+    C u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = FORCE ? j[d] * invRho + F[d] / C(2) : j[d] * invRho;
+    const C rho = rhoBar + C(1);
+    const C jx = rho * u[0], jy = rho * u[1], jz = rho * u[2];
+
+    const C one_m_omega = C(1) - omega;
+    const C t0_omega = C(1.0 / 3.0) * omega;
+    const C t1_omega = C(1.0 / 18.0) * omega;
+    const C t4_omega = C(1.0 / 36.0) * omega;
+    const C jSqr = jx * jx + jy * jy + jz * jz;
+    const C kx = C(3) * jx, ky = C(3) * jy, kz = C(3) * jz;
+    const C kxSqr_ = invRho / C(2) * kx * kx;
+    const C kySqr_ = invRho / C(2) * ky * ky;
+    const C kzSqr_ = invRho / C(2) * kz * kz;
+    const C kxky_ = invRho * kx * ky;
+    const C kxkz_ = invRho * kx * kz;
+    const C kykz_ = invRho * ky * kz;
+    const C C1 = rhoBar + invRho * C(3) * jSqr;
+    C C2, C3;
+    C3 = -kxSqr_ - kySqr_ - kzSqr_;
+    f[0] = f[0] * one_m_omega + t0_omega * (C1 + C3);
+#define BL_PAIR(a, b, tw_, c2, c3)                       \
+    C2 = (c2);                                           \
+    C3 = (c3);                                           \
+    f[a] = f[a] * one_m_omega + tw_ * (C1 + C2 + C3);    \
+    f[b] = f[b] * one_m_omega + tw_ * (C1 - C2 + C3);
+    BL_PAIR(1, 10, t1_omega, -kx, -kySqr_ - kzSqr_)
+    BL_PAIR(2, 11, t1_omega, -ky, -kxSqr_ - kzSqr_)
+    BL_PAIR(3, 12, t1_omega, -kz, -kxSqr_ - kySqr_)
+    BL_PAIR(4, 13, t4_omega, -kx - ky, kxky_ - kzSqr_)
+    BL_PAIR(5, 14, t4_omega, -kx + ky, -kxky_ - kzSqr_)
+    BL_PAIR(6, 15, t4_omega, -kx - kz, kxkz_ - kySqr_)
+    BL_PAIR(7, 16, t4_omega, -kx + kz, -kxkz_ - kySqr_)
+    BL_PAIR(8, 17, t4_omega, -ky - kz, kykz_ - kxSqr_)
+    BL_PAIR(9, 18, t4_omega, -ky + kz, -kykz_ - kxSqr_)
+#undef BL_PAIR
+
+    if (FORCE) {
+        const C mu = C(1) - omega / C(2);
+        const C o6 = C(1.0 / 6.0), o12 = C(1.0 / 12.0);
+        const C Fx = F[0], Fy = F[1], Fz = F[2];
+        const C ux = u[0], uy = u[1], uz = u[2];
+        f[0] += -mu * (Fx * ux + Fy * uy + Fz * uz);
+        f[1] += o6 * mu * (Fx * (C(-1) + 2 * ux) - Fy * uy - Fz * uz);
+        f[2] += -o6 * mu * (Fx * ux + Fy * (C(1) - 2 * uy) + Fz * uz);
+        f[3] += -o6 * mu * (Fx * ux + Fy * uy + Fz * (C(1) - 2 * uz));
+        f[4] += o12 * mu * (Fx * (C(-1) + 2 * ux + 3 * uy) + Fy * (C(-1) + 2 * uy + 3 * ux) - Fz * uz);
+        f[5] += o12 * mu * (Fx * (C(-1) + 2 * ux - 3 * uy) + Fy * (C(1) + 2 * uy - 3 * ux) - Fz * uz);
+        f[6] += o12 * mu * (Fx * (C(-1) + 2 * ux + 3 * uz) - Fy * uy + Fz * (C(-1) + 2 * uz + 3 * ux));
+        f[7] += o12 * mu * (Fx * (C(-1) + 2 * ux - 3 * uz) - Fy * uy + Fz * (C(1) + 2 * uz - 3 * ux));
+        f[8] += -o12 * mu * (Fx * ux + Fy * (C(1) - 2 * uy - 3 * uz) + Fz * (C(1) - 2 * uz - 3 * uy));
+        f[9] += -o12 * mu * (Fx * ux + Fy * (C(1) - 2 * uy + 3 * uz) + Fz * (C(-1) - 2 * uz + 3 * uy));
+        f[10] += o6 * mu * (Fx * (C(1) + 2 * ux) - Fy * uy - Fz * uz);
+        f[11] += -o6 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy) + Fz * uz);
+        f[12] += -o6 * mu * (Fx * ux + Fy * uy + Fz * (C(-1) - 2 * uz));
+        f[13] += o12 * mu * (Fx * (C(1) + 2 * ux + 3 * uy) + Fy * (C(1) + 2 * uy + 3 * ux) - Fz * uz);
+        f[14] += o12 * mu * (Fx * (C(1) + 2 * ux - 3 * uy) + Fy * (C(-1) + 2 * uy - 3 * ux) - Fz * uz);
+        f[15] += o12 * mu * (Fx * (C(1) + 2 * ux + 3 * uz) - Fy * uy + Fz * (C(1) + 2 * uz + 3 * ux));
+        f[16] += o12 * mu * (Fx * (C(1) + 2 * ux - 3 * uz) - Fy * uy + Fz * (C(-1) + 2 * uz - 3 * ux));
+        f[17] += -o12 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy - 3 * uz) + Fz * (C(-1) - 2 * uz - 3 * uy));
+        f[18] += -o12 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy + 3 * uz) + Fz * (C(1) - 2 * uz + 3 * uy));
+    }
+}
+
+// ------------------------------------------------------ collide & stream
+// FMODE: 0 no force, 1 uniform body force, 2 body force + tile-gated field
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE>
+__global__ void __launch_bounds__(256) k_collide_stream(LatView v)
+{
+    using P = Pop<S>;
+    using C = typename P::C;
+    using FT = typename P::F;
+    unsigned bid = blockIdx.x;
+    if (v.xcd_remap) bid = xcd_block(bid, gridDim.x);
+    const unsigned c = bid * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v.ny, v.nz, x, y, z);
+    S* __restrict__ A = (S*)v.pops;
+    const long st = v.stride;
+
+    Nbr n;
+    if (ODD) n.init(v, x, y, z);
+
+    C f[Q];
+    if (ODD) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int s = n.at(v, -cx(i), -cy(i), -cz(i));
+            f[i] = s >= 0 ? P::ld(A + opp(i) * st, (unsigned)s, i) : P::ld(A + i * st, c, i);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = P::ld(A + i * st, c, i);
+    }
+
+    const unsigned char m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
+    if (m == M_FLUID || m == M_VWALL) {
+        C F[3] = {C(0), C(0), C(0)};
+        if (FMODE == 1) { F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2]; }
+        if (FMODE == 2) cell_force<FT, C>(v, c, x, y, z, F);
+        const C omega = PHASE ? (v.phase[c] ? (C)v.om1 : (C)v.om0) : (C)v.om0;
+        collide<C, FMODE != 0>(f, F, omega);
+    } else if (m == M_BB) {
+#pragma unroll
+        for (int i = 1; i <= 9; ++i) {
+            const C t = f[i];
+            f[i] = f[i + 9];
+            f[i + 9] = t;
+        }
+    }
+
+    if (ODD) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int d = n.at(v, cx(i), cy(i), cz(i));
+            if (d >= 0) P::st(A + i * st, (unsigned)d, i, f[i]);
+            else P::st(A + opp(i) * st, c, i, f[i]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) P::st(A + opp(i) * st, c, i, f[i]);
+    }
+}
+
+template <class S, bool ODD, bool MASK, int FMODE>
+void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
+{
+    if (phase)
+        hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true>), dim3(nblk), dim3(blk), 0, s, v);
+    else
+        hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, false>), dim3(nblk), dim3(blk), 0, s, v);
+}
+
+template <class S, bool ODD, bool MASK>
+void launch_cs2(const LatView& v, int fmode, bool phase, unsigned nblk, int blk, hipStream_t s)
+{
+    if (fmode == 0) launch_cs3<S, ODD, MASK, 0>(v, phase, nblk, blk, s);
+    else if (fmode == 1) launch_cs3<S, ODD, MASK, 1>(v, phase, nblk, blk, s);
+    else launch_cs3<S, ODD, MASK, 2>(v, phase, nblk, blk, s);
+}
+
+template <class S>
+void launch_cs(const LatView& v, bool odd, bool mask, int fmode, bool phase, unsigned nblk, int blk,
+               hipStream_t s)
+{
+    if (odd) {
+        if (mask) launch_cs2<S, true, true>(v, fmode, phase, nblk, blk, s);
+        else launch_cs2<S, true, false>(v, fmode, phase, nblk, blk, s);
+    } else {
+        if (mask) launch_cs2<S, false, true>(v, fmode, phase, nblk, blk, s);
+        else launch_cs2<S, false, false>(v, fmode, phase, nblk, blk, s);
+    }
+}
+
+static int force_mode(const bl_lattice_s* L)
+{
+    if (L->tile_flags && L->force_field) return 2;
+    if (L->g[0] != 0.0 || L->g[1] != 0.0 || L->g[2] != 0.0) return 1;
+    return 0;
+}
+
+int collide_stream_launch(bl_lattice_s* L)
+{
+    LatView v = view_of(L);
+    const bool odd = (L->steps & 1) != 0;
+    const int fm = force_mode(L);
+    if (fm != 2) v.flags = nullptr;
+    const bool phase = L->phase != nullptr && L->omega_c0 != L->omega_c1;
+    const unsigned nblk = (unsigned)((L->ncell + L->block - 1) / L->block);
+    if (L->storage == BL_STORE_F64) launch_cs<double>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
+    else if (L->storage == BL_STORE_F32) launch_cs<float>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
+    else launch_cs<__half>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
+    BL_CHECK_LAUNCH();
+    L->steps += 1;
+    return BL_OK;
+}
+
+// ------------------------------------------------ velocity-wall boundary
+// Regularized finite-difference velocity boundary applied after streaming
+// (PlaneFdBoundaryFunctional3D::processCell,
+// src/boundaryCondition/finiteDifferenceBoundaryProcessor3D.hh:68-115) with
+// the density rule of VelocityDirichletBoundaryDynamics::computeRhoBar
+// (src/boundaryCondition/boundaryDynamics.hh:631-660).  Two launches: the
+// first computes every wall cell's new populations from the post-stream
+// state, the second writes them, so no wall cell reads a value another
+// already replaced.
+template <class S>
+__device__ void cell_velocity_post(const LatView& v, long c, int x, int y, int z, bool swapped, double u[3])
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    const unsigned char m = v.mask ? v.mask[c] : (unsigned char)M_FLUID;
+    if (m == M_VWALL) {
+        const int p[3] = {x, y, z};
+        const int side = p[v.wall_axis] == 0 ? 0 : 1;
+        for (int d = 0; d < 3; ++d) u[d] = v.wall_u[3 * side + d];
+        return;
+    }
+    Nbr n;
+    n.init(v, x, y, z);
+    C f[Q];
+    load_post<S>(v, n, c, swapped, f);
+    C F[3];
+    cell_force<FT, C>(v, c, x, y, z, F);
+    const C rb = sum_rho(f);
+    C j[3];
+    sum_j(f, j);
+    const C invRho = C(1) / (rb + C(1));
+    for (int d = 0; d < 3; ++d) u[d] = (double)(j[d] * invRho + F[d] / C(2));
+}
+
+__device__ __forceinline__ void wall_cell_coords(const LatView& v, long w, int& x, int& y, int& z)
+{
+    const int a = v.wall_axis;
+    const int n[3] = {v.nx, v.ny, v.nz};
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    const long per_plane = (long)n[t1] * n[t2];
+    const int side = (int)(w / per_plane);
+    const long r = w % per_plane;
+    int p[3];
+    p[a] = side == 0 ? 0 : n[a] - 1;
+    p[t1] = (int)(r / n[t2]);
+    p[t2] = (int)(r % n[t2]);
+    x = p[0]; y = p[1]; z = p[2];
+}
+
+template <class S>
+__global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out)
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwall) return;
+    int x, y, z;
+    wall_cell_coords(v, w, x, y, z);
+    const long c = ((long)x * v.ny + y) * v.nz + z;
+    if (!v.mask || v.mask[c] != M_VWALL) return;
+    const int a = v.wall_axis;
+    const int p[3] = {x, y, z};
+    const int side = p[a] == 0 ? 0 : 1;
+    const int orient = side == 0 ? -1 : 1;
+    const double* uw = v.wall_u + 3 * side;
+
+    Nbr n;
+    n.init(v, x, y, z);
+    C fc[Q];
+    load_post<S>(v, n, c, swapped, fc);
+    C Fc[3];
+    cell_force<FT, C>(v, c, x, y, z, Fc);
+    double onWall = 0.0, normal = 0.0;
+    for (int i = 0; i < Q; ++i)
+        if (ci(i, a) == 0) onWall += (double)fc[i];
+    for (int i = 0; i < Q; ++i)
+        if (ci(i, a) == orient) normal += (double)fc[i];
+    const double velNormal = (double)orient * (uw[a] - 0.5 * (double)Fc[a]);
+    const double rb = (2.0 * normal + onWall - 1.0 * velNormal) / (1.0 + velNormal);
+    const double rho = rb + 1.0;
+
+    double du[3][3];
+    const int n3[3] = {v.nx, v.ny, v.nz};
+    for (int d = 0; d < 3; ++d) {
+        if (d == a) {
+            int q[3] = {x, y, z};
+            q[d] -= orient;
+            const long c1 = ((long)q[0] * v.ny + q[1]) * v.nz + q[2];
+            double u1[3];
+            cell_velocity_post<S>(v, c1, q[0], q[1], q[2], swapped, u1);
+            for (int k = 0; k < 3; ++k) du[d][k] = -orient * (-uw[k] + u1[k]);
+        } else {
+            int qp[3] = {x, y, z}, qm[3] = {x, y, z};
+            qp[d] += 1;
+            qm[d] -= 1;
+            if (v.per[d]) {
+                if (qp[d] >= n3[d]) qp[d] -= n3[d];
+                if (qm[d] < 0) qm[d] += n3[d];
+            } else {
+                if (qp[d] >= n3[d]) qp[d] = n3[d] - 1;
+                if (qm[d] < 0) qm[d] = 0;
+            }
+            const long cp = ((long)qp[0] * v.ny + qp[1]) * v.nz + qp[2];
+            const long cm = ((long)qm[0] * v.ny + qm[1]) * v.nz + qm[2];
+            double up[3], um[3];
+            cell_velocity_post<S>(v, cp, qp[0], qp[1], qp[2], swapped, up);
+            cell_velocity_post<S>(v, cm, qm[0], qm[1], qm[2], swapped, um);
+            for (int k = 0; k < 3; ++k) du[d][k] = (up[k] - um[k]) / 2.0;
+        }
+    }
+    const double sToPi = -rho / 3.0 / v.om_base;
+    double pi[6];
+    pi[0] = 2.0 * du[0][0] * sToPi;
+    pi[3] = 2.0 * du[1][1] * sToPi;
+    pi[5] = 2.0 * du[2][2] * sToPi;
+    pi[1] = (du[0][1] + du[1][0]) * sToPi;
+    pi[2] = (du[0][2] + du[2][0]) * sToPi;
+    pi[4] = (du[1][2] + du[2][1]) * sToPi;
+    const double jv[3] = {rho * uw[0], rho * uw[1], rho * uw[2]};
+    const double jSqr = jv[0] * jv[0] + jv[1] * jv[1] + jv[2] * jv[2];
+    const double invRho = 1.0 / (rb + 1.0);
+    for (int i = 0; i < Q; ++i) {
+        const double c_j = cx(i) * jv[0] + cy(i) * jv[1] + cz(i) * jv[2];
+        const double feq = tw(i) * (rb + 3. * c_j + invRho * (4.5 * c_j * c_j - 1.5 * jSqr));
+        // Pi -> fNeq (src/latticeBoltzmann/offEquilibriumTemplates.h:91-115)
+        double fneq = 0.0;
+        int iPi = 0;
+        for (int al = 0; al < 3; ++al) {
+            fneq += pi[iPi] * (ci(i, al) * ci(i, al) - 1.0 / 3.0);
+            ++iPi;
+            for (int be = al + 1; be < 3; ++be) {
+                fneq += pi[iPi] * 2.0 * ci(i, al) * ci(i, be);
+                ++iPi;
+            }
+        }
+        fneq *= tw(i) * 3.0 * 3.0 / 2.0;
+        out[w * Q + i] = feq + fneq;
+    }
+}
+
+template <class S>
+__global__ void k_vwall_store(LatView v, long nwall, bool swapped, const double* in)
+{
+    using C = typename Pop<S>::C;
+    const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwall) return;
+    int x, y, z;
+    wall_cell_coords(v, w, x, y, z);
+    const long c = ((long)x * v.ny + y) * v.nz + z;
+    if (!v.mask || v.mask[c] != M_VWALL) return;
+    Nbr n;
+    n.init(v, x, y, z);
+    C f[Q];
+    for (int i = 0; i < Q; ++i) f[i] = (C)in[w * Q + i];
+    store_post<S>(v, n, c, swapped, f);
+}
+
+int velocity_walls_launch(bl_lattice_s* L)
+{
+    if (!L->has_vwall || !L->mask) return BL_OK;
+    const LatView v = view_of(L);
+    const bool swapped = (L->steps & 1) != 0;
+    const unsigned nb = (unsigned)((L->nwall + 255) / 256);
+    if (L->storage == BL_STORE_F64) {
+        hipLaunchKernelGGL(k_vwall_compute<double>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
+        hipLaunchKernelGGL(k_vwall_store<double>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
+    } else if (L->storage == BL_STORE_F32) {
+        hipLaunchKernelGGL(k_vwall_compute<float>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
+        hipLaunchKernelGGL(k_vwall_store<float>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
+    } else {
+        hipLaunchKernelGGL(k_vwall_compute<__half>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
+        hipLaunchKernelGGL(k_vwall_store<__half>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
+    }
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+// ----------------------------------------------- transfers and moments
+// buf: ncell*19 doubles in cell-major order (host layout).
+template <class S>
+__global__ void k_put(LatView v, bool swapped, const double* buf)
+{
+    using C = typename Pop<S>::C;
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v.ny, v.nz, x, y, z);
+    Nbr n;
+    n.init(v, x, y, z);
+    C f[Q];
+    for (int i = 0; i < Q; ++i) f[i] = (C)buf[(long)c * Q + i];
+    store_post<S>(v, n, c, swapped, f);
+}
+
+template <class S>
+__global__ void k_get(LatView v, bool swapped, double* buf)
+{
+    using C = typename Pop<S>::C;
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v.ny, v.nz, x, y, z);
+    Nbr n;
+    n.init(v, x, y, z);
+    C f[Q];
+    load_post<S>(v, n, c, swapped, f);
+    for (int i = 0; i < Q; ++i) buf[(long)c * Q + i] = (double)f[i];
+}
+
+// rho, u as computeDensity / computeVelocity report them.
+template <class S>
+__global__ void k_moments(LatView v, bool swapped, double* rho, double* u)
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v.ny, v.nz, x, y, z);
+    Nbr n;
+    n.init(v, x, y, z);
+    C f[Q];
+    load_post<S>(v, n, c, swapped, f);
+    C F[3];
+    cell_force<FT, C>(v, c, x, y, z, F);
+    const unsigned char m = v.mask ? v.mask[c] : (unsigned char)M_FLUID;
+    double r, uu[3];
+    if (m == M_VWALL) {
+        const int p[3] = {x, y, z};
+        const int a = v.wall_axis;
+        const int side = p[a] == 0 ? 0 : 1;
+        const int orient = side == 0 ? -1 : 1;
+        const double* uw = v.wall_u + 3 * side;
+        double onWall = 0.0, normal = 0.0;
+        for (int i = 0; i < Q; ++i)
+            if (ci(i, a) == 0) onWall += (double)f[i];
+        for (int i = 0; i < Q; ++i)
+            if (ci(i, a) == orient) normal += (double)f[i];
+        const double velNormal = (double)orient * (uw[a] - 0.5 * (double)F[a]);
+        r = (2.0 * normal + onWall - 1.0 * velNormal) / (1.0 + velNormal) + 1.0;
+        for (int d = 0; d < 3; ++d) uu[d] = uw[d];
+    } else {
+        const C rb = sum_rho(f);
+        C j[3];
+        sum_j(f, j);
+        const C invRho = C(1) / (rb + C(1));
+        r = (double)(rb + C(1));
+        for (int d = 0; d < 3; ++d) uu[d] = (double)(j[d] * invRho + F[d] / C(2));
+    }
+    if (rho) rho[c] = r;
+    if (u) for (int d = 0; d < 3; ++d) u[3 * (long)c + d] = uu[d];
+}
+
+template <class S>
+__global__ void k_get_force(LatView v, double* out)
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v.ny, v.nz, x, y, z);
+    C F[3];
+    cell_force<FT, C>(v, c, x, y, z, F);
+    for (int d = 0; d < 3; ++d) out[3 * (long)c + d] = (double)F[d];
+}
+
+// Equilibrium initialisation (iniCellAtEquilibrium, src/core/cell.hh:124-132).
+template <class S>
+__global__ void k_init_eq(LatView v, bool swapped, const double* rho, const double* u)
+{
+    using C = typename Pop<S>::C;
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v.ny, v.nz, x, y, z);
+    const double r = rho ? rho[c] : 1.0;
+    double j[3] = {0, 0, 0};
+    if (u) for (int d = 0; d < 3; ++d) j[d] = r * u[3 * (long)c + d];
+    const double jSqr = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
+    const double rb = r - 1.0, invRho = 1.0 / r;
+    C f[Q];
+    for (int i = 0; i < Q; ++i) {
+        const double c_j = cx(i) * j[0] + cy(i) * j[1] + cz(i) * j[2];
+        f[i] = (C)(tw(i) * (rb + 3. * c_j + invRho * (4.5 * c_j * c_j - 1.5 * jSqr)));
+    }
+    Nbr n;
+    n.init(v, x, y, z);
+    store_post<S>(v, n, c, swapped, f);
+}
+
+// FP64 sums of mass and momentum over non-bounce-back cells: one partial
+// per block (fixed-order tree in LDS), then one block sums the partials in
+// order, so the result does not depend on scheduling.
+template <class S>
+__global__ void __launch_bounds__(256) k_conserved_partial(LatView v, bool swapped, double* partial)
+{
+    using C = typename Pop<S>::C;
+    __shared__ double sh[4][256];
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    if (c < v.ncell && !(v.mask && v.mask[c] == M_BB)) {
+        int x, y, z;
+        decode(c, v.ny, v.nz, x, y, z);
+        Nbr n;
+        n.init(v, x, y, z);
+        C f[Q];
+        load_post<S>(v, n, c, swapped, f);
+        double fd[Q];
+        for (int i = 0; i < Q; ++i) fd[i] = (double)f[i];
+        double j[3];
+        sum_j(fd, j);
+        acc[0] = 1.0 + sum_rho(fd);
+        acc[1] = j[0]; acc[2] = j[1]; acc[3] = j[2];
+    }
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partial[4 * (long)blockIdx.x + k] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(const double* partial, long n, double* out)
+{
+    __shared__ double sh[4][256];
+    double acc[4] = {0, 0, 0, 0};
+    for (long b = threadIdx.x; b < n; b += blockDim.x)
+        for (int k = 0; k < 4; ++k) acc[k] += partial[4 * b + k];
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) out[k] = sh[k][0];
+}
+
+// Velocity u = j/rho + F/2 (wall velocity on velocity-wall cells) of every
+// cell of the active tiles, into ubuf[3*c + d] (compute precision).  This is
+// the computeVelocity pass of the progs, restricted to the cells the
+// immersed boundary reads.  Persistent: the tile count is read on device.
+template <class S>
+__global__ void __launch_bounds__(64) k_stage_velocity(LatView v, bool swapped, const int* tiles,
+                                                        const int* n_active, typename Pop<S>::C* ubuf)
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    const int na = *n_active;
+    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+        const int t = tiles[a];
+        const int tz = t % v.ntz, r = t / v.ntz, ty = r % v.nty, tx = r / v.nty;
+        const int x = tx * kTile + (threadIdx.x >> 4);
+        const int y = ty * kTile + ((threadIdx.x >> 2) & 3);
+        const int z = tz * kTile + (threadIdx.x & 3);
+        if (x >= v.nx || y >= v.ny || z >= v.nz) continue;
+        const long c = ((long)x * v.ny + y) * v.nz + z;
+        const unsigned char m = v.mask ? v.mask[c] : (unsigned char)M_FLUID;
+        C u[3];
+        if (m == M_VWALL) {
+            const int p[3] = {x, y, z};
+            const int side = p[v.wall_axis] == 0 ? 0 : 1;
+            for (int d = 0; d < 3; ++d) u[d] = (C)v.wall_u[3 * side + d];
+        } else {
+            Nbr n;
+            n.init(v, x, y, z);
+            C f[Q];
+            load_post<S>(v, n, c, swapped, f);
+            C F[3];
+            cell_force<FT, C>(v, c, x, y, z, F);
+            const C rb = sum_rho(f);
+            C j[3];
+            sum_j(f, j);
+            const C invRho = C(1) / (rb + C(1));
+            for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + F[d] / C(2);
+        }
+        for (int d = 0; d < 3; ++d) ubuf[3 * c + d] = u[d];
+    }
+}
+
+int lattice_stage_velocity_tiles(bl_lattice_s* L, const int* active_tiles, const int* n_active,
+                                 void* ubuf, hipStream_t s)
+{
+    LatView v = view_of(L);
+    if (!L->force_field) v.flags = nullptr;
+    const bool swapped = (L->steps & 1) != 0;
+    const unsigned nb = 2048;
+    if (L->storage == BL_STORE_F64)
+        hipLaunchKernelGGL(k_stage_velocity<double>, dim3(nb), dim3(64), 0, s, v, swapped, active_tiles, n_active, (double*)ubuf);
+    else if (L->storage == BL_STORE_F32)
+        hipLaunchKernelGGL(k_stage_velocity<float>, dim3(nb), dim3(64), 0, s, v, swapped, active_tiles, n_active, (float*)ubuf);
+    else
+        hipLaunchKernelGGL(k_stage_velocity<__half>, dim3(nb), dim3(64), 0, s, v, swapped, active_tiles, n_active, (float*)ubuf);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+// ------------------------------------------------------------ fills
+__global__ void k_fill_u8(unsigned char* p, long n, unsigned char val)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+
+template <class FT>
+__global__ void k_put_force(FT* F, long stride, unsigned ncell, const double* buf)
+{
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    for (int d = 0; d < 3; ++d) F[d * stride + c] = (FT)buf[3 * (long)c + d];
+}
+
+int ensure_force_field(bl_lattice_s* L)
+{
+    if (L->force) return BL_OK;
+    L->force_bytes = L->storage == BL_STORE_F64 ? sizeof(double) : sizeof(float);
+    BL_HIP(hipMalloc(&L->force, 3 * L->stride * L->force_bytes));
+    BL_HIP(hipMemsetAsync(L->force, 0, 3 * L->stride * L->force_bytes, L->stream));
+    BL_HIP(hipMalloc(&L->tile_flags, L->ntiles));
+    const unsigned nb = (unsigned)((L->ntiles + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u8, dim3(nb), dim3(256), 0, L->stream, L->tile_flags, L->ntiles, (unsigned char)0);
+    BL_CHECK_LAUNCH();
+    L->force_field = true;
+    return BL_OK;
+}
+
+}  // namespace bl
+
+// =================================================================== C-ABI
+using namespace bl;
+
+namespace {
+
+bool g_inited = false;
+int g_device = 0;
+
+int stage_alloc(bl_lattice L)
+{
+    if (!L->d_stage) BL_HIP(hipMalloc(&L->d_stage, sizeof(double) * Q * L->ncell));
+    return BL_OK;
+}
+
+#define BL_DISPATCH(L, KERNEL, NB, ...)                                                            \
+    do {                                                                                           \
+        if ((L)->storage == BL_STORE_F64)                                                          \
+            hipLaunchKernelGGL(KERNEL<double>, dim3(NB), dim3(256), 0, (L)->stream, __VA_ARGS__);  \
+        else if ((L)->storage == BL_STORE_F32)                                                     \
+            hipLaunchKernelGGL(KERNEL<float>, dim3(NB), dim3(256), 0, (L)->stream, __VA_ARGS__);   \
+        else                                                                                       \
+            hipLaunchKernelGGL(KERNEL<__half>, dim3(NB), dim3(256), 0, (L)->stream, __VA_ARGS__);  \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int bl_init(int device)
+{
+    int n = 0;
+    BL_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(BL_ERR_ARG, "bl_init: no such device");
+    BL_HIP(hipSetDevice(device));
+    g_device = device;
+    g_inited = true;
+    return BL_OK;
+}
+
+int bl_shutdown(void)
+{
+    if (g_inited) BL_HIP(hipDeviceSynchronize());
+    g_inited = false;
+    return BL_OK;
+}
+
+int bl_device_info(char* name, int cap, int* cus, double* hbm)
+{
+    hipDeviceProp_t p;
+    BL_HIP(hipGetDeviceProperties(&p, g_device));
+    if (name && cap > 0) {
+        std::strncpy(name, p.gcnArchName, cap - 1);
+        name[cap - 1] = 0;
+    }
+    if (cus) *cus = p.multiProcessorCount;
+    if (hbm) *hbm = (double)p.totalGlobalMem;
+    return BL_OK;
+}
+
+int bl_lattice_create(int nx, int ny, int nz, const int periodic[3], int storage, bl_lattice* out)
+{
+    if (!out) return fail(BL_ERR_ARG, "bl_lattice_create: null output handle");
+    if (nx < 3 || ny < 3 || nz < 3) return fail(BL_ERR_ARG, "bl_lattice_create: each extent must be >= 3");
+    const long ncell = (long)nx * ny * nz;
+    if (ncell >= (1L << 31)) return fail(BL_ERR_ARG, "bl_lattice_create: more than 2^31 cells");
+    if (storage < BL_STORE_F64 || storage > BL_STORE_F16S) return fail(BL_ERR_ARG, "bl_lattice_create: unknown storage");
+    if (!g_inited) {
+        int rc = bl_init(0);
+        if (rc) return rc;
+    }
+    bl_lattice L = new bl_lattice_s();
+    L->nx = nx; L->ny = ny; L->nz = nz;
+    L->ncell = ncell;
+    L->stride = (ncell + 127) / 128 * 128;
+    for (int d = 0; d < 3; ++d) L->periodic[d] = periodic ? (periodic[d] != 0) : 1;
+    L->storage = storage;
+    L->pop_bytes = storage == BL_STORE_F64 ? 8 : (storage == BL_STORE_F32 ? 4 : 2);
+    L->ntx = (nx + kTile - 1) / kTile;
+    L->nty = (ny + kTile - 1) / kTile;
+    L->ntz = (nz + kTile - 1) / kTile;
+    L->ntiles = (long)L->ntx * L->nty * L->ntz;
+    hipError_t e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&L->pops, Q * L->stride * L->pop_bytes);
+    if (e == hipSuccess) e = hipMalloc(&L->d_partials, sizeof(double) * 4 * ((ncell + 255) / 256 + 1));
+    if (e != hipSuccess) {
+        delete L;
+        return fail(BL_ERR_HIP, std::string("bl_lattice_create: ") + hipGetErrorString(e));
+    }
+    *out = L;
+    return bl_lattice_init_equilibrium(L, nullptr, nullptr);
+}
+
+int bl_lattice_destroy(bl_lattice L)
+{
+    if (!L) return BL_OK;
+    if (L->stream) hipStreamSynchronize(L->stream);
+    hipFree(L->pops);
+    hipFree(L->mask);
+    hipFree(L->phase);
+    hipFree(L->force);
+    hipFree(L->tile_flags);
+    hipFree(L->d_stage);
+    hipFree(L->d_partials);
+    hipFree(L->d_wall);
+    if (L->stream) hipStreamDestroy(L->stream);
+    delete L;
+    return BL_OK;
+}
+
+int bl_lattice_set_dynamics(bl_lattice L, double omega_out, double omega_in)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    if (!(omega_out > 0.0 && omega_out < 2.0 && omega_in > 0.0 && omega_in < 2.0))
+        return fail(BL_ERR_ARG, "bl_lattice_set_dynamics: omega must lie in (0, 2)");
+    L->omega_out = omega_out;
+    L->omega_in = omega_in;
+    // omega(phase) = w0 w1 / (c w0 + (1-c) w1) at c = 0 and c = 1, evaluated
+    // exactly as the reference does (src/fsi/VariableViscosityDynamics.hh:131-141).
+    const double c0 = 0.0, c1 = 1.0;
+    L->omega_c0 = omega_out * omega_in / (c0 * omega_out + (1. - c0) * omega_in);
+    L->omega_c1 = omega_out * omega_in / (c1 * omega_out + (1. - c1) * omega_in);
+    return BL_OK;
+}
+
+int bl_lattice_set_mask(bl_lattice L, const unsigned char* mask)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    BL_HIP(hipStreamSynchronize(L->stream));
+    if (!mask) {
+        hipFree(L->mask);
+        L->mask = nullptr;
+        L->has_vwall = false;
+        return BL_OK;
+    }
+    bool vw = false;
+    for (long c = 0; c < L->ncell; ++c) {
+        if (mask[c] > M_VWALL) return fail(BL_ERR_ARG, "bl_lattice_set_mask: unknown mask code");
+        vw = vw || mask[c] == M_VWALL;
+    }
+    if (!L->mask) BL_HIP(hipMalloc(&L->mask, L->ncell));
+    BL_HIP(hipMemcpy(L->mask, mask, L->ncell, hipMemcpyHostToDevice));
+    L->has_vwall = vw;
+    if (vw) {
+        // velocity-wall cells must lie on the two planes of the wall axis
+        const int n[3] = {L->nx, L->ny, L->nz};
+        const int a = L->wall_axis;
+        for (int x = 0; x < L->nx; ++x)
+            for (int y = 0; y < L->ny; ++y)
+                for (int z = 0; z < L->nz; ++z) {
+                    const int p[3] = {x, y, z};
+                    if (mask[((long)x * L->ny + y) * L->nz + z] == M_VWALL && p[a] != 0 && p[a] != n[a] - 1)
+                        return fail(BL_ERR_ARG, "bl_lattice_set_mask: velocity-wall cells must lie on the planes "
+                                                "0 and n-1 of the wall axis (set it first)");
+                }
+        const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+        const long nwall = 2L * n[t1] * n[t2];
+        if (nwall > L->nwall) {
+            hipFree(L->d_wall);
+            BL_HIP(hipMalloc(&L->d_wall, sizeof(double) * Q * nwall));
+        }
+        L->nwall = nwall;
+    }
+    return BL_OK;
+}
+
+int bl_lattice_set_velocity_walls(bl_lattice L, int axis, const double u_lo[3], const double u_hi[3])
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    if (axis < 0 || axis > 2) return fail(BL_ERR_ARG, "bl_lattice_set_velocity_walls: axis must be 0, 1 or 2");
+    if (L->periodic[axis]) return fail(BL_ERR_ARG, "bl_lattice_set_velocity_walls: wall axis must be non-periodic");
+    L->wall_axis = axis;
+    for (int d = 0; d < 3; ++d) {
+        L->wall_u[d] = u_lo ? u_lo[d] : 0.0;
+        L->wall_u[3 + d] = u_hi ? u_hi[d] : 0.0;
+    }
+    return BL_OK;
+}
+
+int bl_lattice_set_body_force(bl_lattice L, const double g[3])
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    for (int d = 0; d < 3; ++d) L->g[d] = g ? g[d] : 0.0;
+    return BL_OK;
+}
+
+int bl_lattice_set_force_field(bl_lattice L, const double* force)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    if (!force) return fail(BL_ERR_ARG, "bl_lattice_set_force_field: null array");
+    int rc = ensure_force_field(L);
+    if (rc) return rc;
+    double* d = nullptr;
+    BL_HIP(hipMalloc(&d, sizeof(double) * 3 * L->ncell));
+    BL_HIP(hipMemcpy(d, force, sizeof(double) * 3 * L->ncell, hipMemcpyHostToDevice));
+    const unsigned nb = (unsigned)((L->ncell + 255) / 256);
+    if (L->storage == BL_STORE_F64)
+        hipLaunchKernelGGL(k_put_force<double>, dim3(nb), dim3(256), 0, L->stream, (double*)L->force, L->stride, (unsigned)L->ncell, (const double*)d);
+    else
+        hipLaunchKernelGGL(k_put_force<float>, dim3(nb), dim3(256), 0, L->stream, (float*)L->force, L->stride, (unsigned)L->ncell, (const double*)d);
+    const unsigned nt = (unsigned)((L->ntiles + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u8, dim3(nt), dim3(256), 0, L->stream, L->tile_flags, L->ntiles, (unsigned char)1);
+    BL_CHECK_LAUNCH();
+    BL_HIP(hipStreamSynchronize(L->stream));
+    hipFree(d);
+    return BL_OK;
+}
+
+int bl_lattice_set_phase(bl_lattice L, const unsigned char* phase)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    BL_HIP(hipStreamSynchronize(L->stream));
+    if (!phase) {
+        hipFree(L->phase);
+        L->phase = nullptr;
+        return BL_OK;
+    }
+    for (long c = 0; c < L->ncell; ++c)
+        if (phase[c] > 1) return fail(BL_ERR_ARG, "bl_lattice_set_phase: phase must be 0 or 1");
+    if (!L->phase) BL_HIP(hipMalloc(&L->phase, L->ncell));
+    BL_HIP(hipMemcpy(L->phase, phase, L->ncell, hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+int bl_lattice_init_equilibrium(bl_lattice L, const double* rho, const double* u)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    double *dr = nullptr, *du = nullptr;
+    if (rho) {
+        BL_HIP(hipMalloc(&dr, sizeof(double) * L->ncell));
+        BL_HIP(hipMemcpy(dr, rho, sizeof(double) * L->ncell, hipMemcpyHostToDevice));
+    }
+    if (u) {
+        BL_HIP(hipMalloc(&du, sizeof(double) * 3 * L->ncell));
+        BL_HIP(hipMemcpy(du, u, sizeof(double) * 3 * L->ncell, hipMemcpyHostToDevice));
+    }
+    const LatView v = view_of(L);
+    const bool swapped = (L->steps & 1) != 0;
+    const unsigned nb = (unsigned)((L->ncell + 255) / 256);
+    BL_DISPATCH(L, k_init_eq, nb, v, swapped, (const double*)dr, (const double*)du);
+    BL_CHECK_LAUNCH();
+    BL_HIP(hipStreamSynchronize(L->stream));
+    hipFree(dr);
+    hipFree(du);
+    return BL_OK;
+}
+
+int bl_lattice_put_populations(bl_lattice L, const double* buf)
+{
+    if (!L || !buf) return fail(BL_ERR_ARG, "bl_lattice_put_populations: null argument");
+    int rc = stage_alloc(L);
+    if (rc) return rc;
+    BL_HIP(hipMemcpyAsync(L->d_stage, buf, sizeof(double) * Q * L->ncell, hipMemcpyHostToDevice, L->stream));
+    const LatView v = view_of(L);
+    const unsigned nb = (unsigned)((L->ncell + 255) / 256);
+    BL_DISPATCH(L, k_put, nb, v, (L->steps & 1) != 0, (const double*)L->d_stage);
+    BL_CHECK_LAUNCH();
+    BL_HIP(hipStreamSynchronize(L->stream));
+    return BL_OK;
+}
+
+int bl_lattice_get_populations(bl_lattice L, double* buf)
+{
+    if (!L || !buf) return fail(BL_ERR_ARG, "bl_lattice_get_populations: null argument");
+    int rc = stage_alloc(L);
+    if (rc) return rc;
+    const LatView v = view_of(L);
+    const unsigned nb = (unsigned)((L->ncell + 255) / 256);
+    BL_DISPATCH(L, k_get, nb, v, (L->steps & 1) != 0, L->d_stage);
+    BL_CHECK_LAUNCH();
+    BL_HIP(hipMemcpyAsync(buf, L->d_stage, sizeof(double) * Q * L->ncell, hipMemcpyDeviceToHost, L->stream));
+    BL_HIP(hipStreamSynchronize(L->stream));
+    return BL_OK;
+}
+
+int bl_lattice_initialize(bl_lattice L)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    int rc = velocity_walls_launch(L);
+    if (rc) return rc;
+    BL_HIP(hipStreamSynchronize(L->stream));
+    return BL_OK;
+}
+
+int bl_collide_stream(bl_lattice L, int n_steps)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    if (n_steps < 0) return fail(BL_ERR_ARG, "bl_collide_stream: negative step count");
+    for (int s = 0; s < n_steps; ++s) {
+        int rc = collide_stream_launch(L);
+        if (rc) return rc;
+        rc = velocity_walls_launch(L);
+        if (rc) return rc;
+    }
+    return BL_OK;
+}
+
+int bl_lattice_get_moments(bl_lattice L, double* rho, double* u)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    double *dr = nullptr, *du = nullptr;
+    BL_HIP(hipMalloc(&dr, sizeof(double) * L->ncell));
+    BL_HIP(hipMalloc(&du, sizeof(double) * 3 * L->ncell));
+    LatView v = view_of(L);
+    if (!L->force_field) v.flags = nullptr;
+    const unsigned nb = (unsigned)((L->ncell + 255) / 256);
+    BL_DISPATCH(L, k_moments, nb, v, (L->steps & 1) != 0, dr, du);
+    BL_CHECK_LAUNCH();
+    if (rho) BL_HIP(hipMemcpyAsync(rho, dr, sizeof(double) * L->ncell, hipMemcpyDeviceToHost, L->stream));
+    if (u) BL_HIP(hipMemcpyAsync(u, du, sizeof(double) * 3 * L->ncell, hipMemcpyDeviceToHost, L->stream));
+    BL_HIP(hipStreamSynchronize(L->stream));
+    hipFree(dr);
+    hipFree(du);
+    return BL_OK;
+}
+
+int bl_lattice_get_force(bl_lattice L, double* force)
+{
+    if (!L || !force) return fail(BL_ERR_ARG, "bl_lattice_get_force: null argument");
+    double* d = nullptr;
+    BL_HIP(hipMalloc(&d, sizeof(double) * 3 * L->ncell));
+    LatView v = view_of(L);
+    if (!L->force_field) v.flags = nullptr;
+    const unsigned nb = (unsigned)((L->ncell + 255) / 256);
+    BL_DISPATCH(L, k_get_force, nb, v, d);
+    BL_CHECK_LAUNCH();
+    BL_HIP(hipMemcpyAsync(force, d, sizeof(double) * 3 * L->ncell, hipMemcpyDeviceToHost, L->stream));
+    BL_HIP(hipStreamSynchronize(L->stream));
+    hipFree(d);
+    return BL_OK;
+}
+
+int bl_lattice_conserved(bl_lattice L, double out[4])
+{
+    if (!L || !out) return fail(BL_ERR_ARG, "bl_lattice_conserved: null argument");
+    const LatView v = view_of(L);
+    const long nb = (L->ncell + 255) / 256;
+    if (L->storage == BL_STORE_F64)
+        hipLaunchKernelGGL(k_conserved_partial<double>, dim3(nb), dim3(256), 0, L->stream, v, (L->steps & 1) != 0, L->d_partials);
+    else if (L->storage == BL_STORE_F32)
+        hipLaunchKernelGGL(k_conserved_partial<float>, dim3(nb), dim3(256), 0, L->stream, v, (L->steps & 1) != 0, L->d_partials);
+    else
+        hipLaunchKernelGGL(k_conserved_partial<__half>, dim3(nb), dim3(256), 0, L->stream, v, (L->steps & 1) != 0, L->d_partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, L->stream, (const double*)L->d_partials, nb,
+                       L->d_partials + 4 * nb);
+    BL_CHECK_LAUNCH();
+    BL_HIP(hipMemcpyAsync(out, L->d_partials + 4 * nb, sizeof(double) * 4, hipMemcpyDeviceToHost, L->stream));
+    BL_HIP(hipStreamSynchronize(L->stream));
+    return BL_OK;
+}
+
+int bl_lattice_steps(bl_lattice L, long long* steps)
+{
+    if (!L || !steps) return fail(BL_ERR_ARG, "bl_lattice_steps: null argument");
+    *steps = L->steps;
+    return BL_OK;
+}
+
+int bl_lattice_set_launch(bl_lattice L, int block, int xcd_remap)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    // the collide-stream kernels are compiled for at most 256 threads per block
+    if (block < 64 || block > 256 || block % 64) return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be 64, 128, 192 or 256");
+    L->block = block;
+    L->xcd_remap = xcd_remap ? 1 : 0;
+    return BL_OK;
+}
+
+int bl_lattice_stream(bl_lattice L, void** stream)
+{
+    if (!L || !stream) return fail(BL_ERR_ARG, "bl_lattice_stream: null argument");
+    *stream = (void*)L->stream;
+    return BL_OK;
+}
+
+int bl_lattice_time_collide(bl_lattice L, int n_steps, double* ms)
+{
+    if (!L || !ms || n_steps <= 0) return fail(BL_ERR_ARG, "bl_lattice_time_collide: bad argument");
+    hipEvent_t a, b;
+    BL_HIP(hipEventCreate(&a));
+    BL_HIP(hipEventCreate(&b));
+    BL_HIP(hipEventRecord(a, L->stream));
+    for (int s = 0; s < n_steps; ++s) {
+        int rc = collide_stream_launch(L);
+        if (rc) return rc;
+    }
+    BL_HIP(hipEventRecord(b, L->stream));
+    BL_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    BL_HIP(hipEventElapsedTime(&t, a, b));
+    *ms = (double)t;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return BL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Device-resident D3Q19 lattice: structure-of-arrays populations in an
+// in-place AA pattern, cell mask, tile-gated external force, phase.
+#pragma once
+
+#include "common.hpp"
+
+struct bl_lattice_s {
+    int nx = 0, ny = 0, nz = 0;
+    long ncell = 0;
+    long stride = 0;  // elements between population planes (multiple of 64)
+    int periodic[3] = {1, 1, 1};
+    int storage = BL_STORE_F64;
+    size_t pop_bytes = 0;  // bytes per stored population value
+    void* pops = nullptr;  // Q * stride values of the storage type
+
+    unsigned char* mask = nullptr;   // ncell, or null (all fluid)
+    bool has_vwall = false;
+    int wall_axis = 1;
+    double wall_u[6] = {0, 0, 0, 0, 0, 0};
+
+    double omega_out = 1.0, omega_in = 1.0;
+    double omega_c0 = 1.0, omega_c1 = 1.0;  // omega for phase 0 / 1
+    unsigned char* phase = nullptr;         // ncell, or null (phase 0)
+
+    // external force: body force g everywhere except in tiles whose flag is
+    // set, where force[d*stride + c] holds the full value.
+    double g[3] = {0, 0, 0};
+    void* force = nullptr;  // 3 * stride of double (F64) or float
+    size_t force_bytes = 0;
+    unsigned char* tile_flags = nullptr;
+    int ntx = 0, nty = 0, ntz = 0;
+    long ntiles = 0;
+    bool force_field = false;  // tiles may carry a field
+
+    long long steps = 0;  // completed collide-and-stream steps (AA parity)
+    hipStream_t stream = nullptr;
+
+    int block = 256;
+    int xcd_remap = 1;
+
+    // scratch
+    double* d_stage = nullptr;  // ncell * 19 doubles, allocated on demand
+    double* d_partials = nullptr;
+    double* d_wall = nullptr;   // velocity-wall staging
+    long nwall = 0;
+    bl_ibm ibm = nullptr;
+};
+
+namespace bl {
+
+struct LatView {
+    int nx, ny, nz;
+    unsigned ncell;
+    long stride;
+    int per[3];
+    void* pops;
+    const unsigned char* mask;
+    const unsigned char* phase;
+    const unsigned char* flags;
+    const void* force;
+    double g[3];
+    double om0, om1;
+    double om_base;  // omega_out, as Dynamics::getOmega() reports it
+    int ntx, nty, ntz;
+    int wall_axis;
+    double wall_u[6];
+    int xcd_remap;
+};
+
+LatView view_of(const bl_lattice_s* L);
+int ensure_force_field(bl_lattice_s* L);
+int lattice_stage_velocity_tiles(bl_lattice_s* L, const int* active_tiles, const int* n_active,
+                                 void* ubuf, hipStream_t s);
+int collide_stream_launch(bl_lattice_s* L);
+int velocity_walls_launch(bl_lattice_s* L);
+
+}  // namespace bl

@@ -1,0 +1,168 @@
+/*
+ * biblbm.h -- C-ABI of libbiblbm.so, the MI355X (gfx950) implementation of
+ * the D3Q19 Guo-VOF collide-and-stream and the immersed-boundary coupling.
+ *
+ * Conventions
+ *   - Every call returns an int status: BL_OK (0) or a BL_ERR_* code; the
+ *     text of the last error on the calling thread is bl_last_error().
+ *   - Handles are opaque.  Device memory is owned by the library; host
+ *     arrays passed in are borrowed for the duration of the call only.
+ *   - One HIP stream per lattice (default priority).  Calls are ordered on
+ *     that stream; calls that return data to the host synchronize it.
+ *   - Cell index c = (x*ny + y)*nz + z (z fastest).  Populations crossing the
+ *     boundary are fBar_i = f_i - t_i in the Palabos D3Q19 ordering (i and
+ *     i+9 opposite), 19 per cell, cell-major: buf[c*19 + i].
+ *   - Cell mask: 0 fluid (Guo-VOF BGK), 1 bounce-back, 2 no dynamics,
+ *     3 velocity-wall node (collides as fluid, then regularized after
+ *     streaming).
+ *
+ * The reference interface each entry point replaces is cited next to it
+ * (paths relative to the reference checkout).
+ */
+#ifndef BIBLBM_H
+#define BIBLBM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BL_OK 0
+#define BL_ERR_ARG 1
+#define BL_ERR_HIP 2
+#define BL_ERR_STATE 3
+#define BL_ERR_CAPACITY 4
+
+/* storage of the populations */
+#define BL_STORE_F64 0   /* FP64 fBar, FP64 collision                     */
+#define BL_STORE_F32 1   /* FP32 fBar, FP32 collision                     */
+#define BL_STORE_F16S 2  /* FP16 of fBar_i / t_i * 2^10, FP32 collision   */
+
+/* immersed-boundary delta kernels */
+#define BL_KERNEL_ROMA 0    /* 3-point, src/fsi/Dirac.h:74-101   */
+#define BL_KERNEL_PESKIN 1  /* 4-point, src/fsi/Dirac.h:23-44    */
+#define BL_KERNEL_TOPHAT 2  /* 2-point, src/fsi/Dirac.h:51-66    */
+
+/* membrane models */
+#define BL_MODEL_CAPSULE_SKALAK 0      /* src/fsi/DeformableCapsule.hh:140-142 */
+#define BL_MODEL_CAPSULE_NEOHOOKEAN 1  /* src/fsi/DeformableCapsule.hh:143-145 */
+#define BL_MODEL_RBC 2                 /* src/fsi/RBCParticle.hh:140-247      */
+
+typedef struct bl_lattice_s *bl_lattice;
+typedef struct bl_ibm_s *bl_ibm;
+typedef struct bl_ensemble_s *bl_ensemble;
+
+/* ------------------------------------------------------------ runtime */
+/* Replaces plbInit (src/core/plbInit.cpp:38) for the device side. */
+int bl_init(int device);
+int bl_shutdown(void);
+const char *bl_last_error(void);
+/* name (NUL-terminated, <= cap bytes), compute units, HBM bytes */
+int bl_device_info(char *name, int cap, int *compute_units, double *hbm_bytes);
+
+/* ------------------------------------------------------------ lattice */
+/* MultiBlockLattice3D(nx,ny,nz, dynamics) + periodicity().toggle
+ * (src/multiBlock/multiBlockLattice3D.h, src/multiBlock/multiBlock3D.h:77). */
+int bl_lattice_create(int nx, int ny, int nz, const int periodic[3], int storage,
+                      bl_lattice *out);
+int bl_lattice_destroy(bl_lattice lat);
+/* GuoExternalForceVOFDynamics(omega_out, omega_in)
+ * (src/fsi/VariableViscosityDynamics.h:48-54). */
+int bl_lattice_set_dynamics(bl_lattice lat, double omega_out, double omega_in);
+/* defineDynamics(... BounceBack / NoDynamics) and the velocity-boundary cell
+ * set (src/dataProcessors/dataInitializerWrapper3D.h, src/core/dynamics.h:559,670,
+ * src/boundaryCondition/boundaryCondition3D.hh:1053-1066).  NULL = all fluid. */
+int bl_lattice_set_mask(bl_lattice lat, const unsigned char *mask);
+/* setBoundaryVelocity on the two planes normal to `axis` that carry mask 3
+ * (index 0: u_lo, index n-1: u_hi). */
+int bl_lattice_set_velocity_walls(bl_lattice lat, int axis, const double u_lo[3],
+                                  const double u_hi[3]);
+/* setExternalVector(lattice, domain, forceBeginsAt, g)
+ * (src/dataProcessors/dataInitializerWrapper3D.h:301): the force every cell
+ * carries outside IBM-touched regions. */
+int bl_lattice_set_body_force(bl_lattice lat, const double g[3]);
+/* Per-cell external force (ncell*3) and phase (ncell, 0 or 1): the
+ * ForcedPhaseD3Q19 externals (src/fsi/VariableViscosityDynamics.h:25-42). */
+int bl_lattice_set_force_field(bl_lattice lat, const double *force);
+int bl_lattice_set_phase(bl_lattice lat, const unsigned char *phase);
+/* initializeAtEquilibrium (src/core/cell.hh:124-132); NULL rho -> 1, NULL u -> 0 */
+int bl_lattice_init_equilibrium(bl_lattice lat, const double *rho, const double *u);
+/* Raw population transfer (the coprocessor layout of
+ * src/coProcessors/coProcessor3D.h:84-92): buf[c*19+i] = fBar_i. */
+int bl_lattice_put_populations(bl_lattice lat, const double *buf);
+int bl_lattice_get_populations(bl_lattice lat, double *buf);
+/* MultiBlock3D::initialize(): runs the post-stream boundary pass once
+ * (src/multiBlock/multiBlock3D.cpp:330-332). */
+int bl_lattice_initialize(bl_lattice lat);
+/* MultiBlockLattice3D::collideAndStream(), n times
+ * (src/multiBlock/multiBlockLattice3D.hh:307-347). */
+int bl_collide_stream(bl_lattice lat, int n_steps);
+/* computeDensity / computeVelocity (u = j/rho + F/2)
+ * (src/dataProcessors/dataAnalysisFunctional3D.hh:772-784,
+ *  src/basicDynamics/externalForceDynamics.hh:59-73).  Either may be NULL. */
+int bl_lattice_get_moments(bl_lattice lat, double *rho, double *u);
+int bl_lattice_get_force(bl_lattice lat, double *force);
+/* FP64 device sums over non-bounce-back cells: mass, jx, jy, jz. */
+int bl_lattice_conserved(bl_lattice lat, double out[4]);
+int bl_lattice_steps(bl_lattice lat, long long *steps);
+/* Launch shape of the collide-stream kernel: threads per block (64, 128,
+ * 192 or 256) and XCD-aware block remapping on/off. */
+int bl_lattice_set_launch(bl_lattice lat, int block, int xcd_remap);
+/* HIP stream the lattice works on (hipStream_t), for event timing. */
+int bl_lattice_stream(bl_lattice lat, void **stream);
+/* Device time of the last n collide-stream launches, milliseconds. */
+int bl_lattice_time_collide(bl_lattice lat, int n_steps, double *ms);
+
+/* ------------------------------------------------ immersed boundary */
+/* ImmersedBoundaryDynamics3D(lattice, shape_library)
+ * (src/fsi/ImmersedBoundaryDynamics.h:57-58); the delta kernel is the
+ * reference's compile-time Dirac typedef (src/fsi/ImmersedBoundaryDynamics.h:55). */
+int bl_ibm_create(bl_lattice lat, int kernel, bl_ibm *out);
+int bl_ibm_destroy(bl_ibm g);
+/* ParticleShapeLibrary::store_mesh (src/fsi/ParticleShapeFactory.hh:118-129):
+ * rest geometry (links, adjacent pairs, areas, angles) is computed in FP64. */
+int bl_shape_add(bl_ibm g, int nv, int nt, const double *rest_pos, const int *tri,
+                 int *shape_id);
+/* Shape rest quantities: area, volume, center (3), radius. */
+int bl_shape_info(bl_ibm g, int shape_id, double out[6]);
+/* add_particle (src/fsi/ImmersedBoundaryDynamics.hh:508-525).  params:
+ *   capsule models: {G, C}
+ *   RBC: {shear_modulus, L0, k_area_global, k_area_local, k_volume,
+ *         vol_desired, k_bend, theta0}  (src/fsi/RBCParticle.h:22-58)
+ * pos: nv*3 initial vertex positions.  bug_compatible (capsules) keeps the
+ * reference's c = l' sin(phi) / (l' sin(phi0)) term
+ * (src/fsi/DeformableCapsule.hh:96); 0 uses l0' in the denominator. */
+int bl_particle_add(bl_ibm g, int shape_id, int model, const double *params,
+                    int nparams, const double *pos, int bug_compatible, int *pid);
+/* init() (src/fsi/ImmersedBoundaryDynamics.hh:148-152): uploads particles,
+ * maps every centre of mass onto the periodic grid. */
+int bl_ibm_init(bl_ibm g);
+/* The four stages of ImmersedBoundaryWrapperFunctional3D::process
+ * (src/fsi/DataProcessors.hh:94-108), preceded by the computeVelocity pass
+ * and the setExternalVector(force := body force) reset of the progs. */
+int bl_ibm_interpolate(bl_ibm g);
+int bl_ibm_compute_forces(bl_ibm g);
+int bl_ibm_spread(bl_ibm g);
+int bl_ibm_move(bl_ibm g);
+/* All of the above (and the phase voxelization when omega_in != omega_out). */
+int bl_ibm_step(bl_ibm g);
+/* n coupled time steps: IBM step + collide-and-stream each (the loop body of
+ * progs/capsule_in_shear/capsule_in_shear.cpp:187-227).  use_graph captures
+ * one step in a hipGraph and replays it. */
+int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph);
+int bl_ibm_count(bl_ibm g, int *particles, int *vertices);
+/* Vertex arrays of one particle (pid) or of all (pid = -1), nv*3 doubles. */
+int bl_particle_get(bl_ibm g, int pid, int what, double *out); /* what: 0 pos, 1 vel, 2 force */
+int bl_particle_set_positions(bl_ibm g, int pid, const double *pos);
+/* area, volume, centre of mass (3), inertia tensor (9) of the enclosed
+ * volume (src/fsi/DeformableParticle3D.hh:54-82,173-272). */
+int bl_particle_observables(bl_ibm g, int pid, double out[14]);
+/* The phase field of the last voxelization (ncell bytes, 0/1). */
+int bl_ibm_get_phase(bl_ibm g, unsigned char *phase);
+/* Time of each IBM stage over the last bl_ibm_step, ms: interpolate,
+ * forces, spread, move, voxelize. */
+int bl_ibm_stage_times(bl_ibm g, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
