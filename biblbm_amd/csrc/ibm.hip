@@ -1,0 +1,1416 @@
+// Immersed-boundary coupling on gfx950.
+//
+// Per coupled step (the IBM functional of the reference,
+// src/fsi/DataProcessors.hh:94-108, preceded by the progs' computeVelocity
+// and force reset):
+//   bin      every vertex's delta-kernel stencil into 4x4x4 lattice tiles
+//            (counts, exclusive scan, list fill);
+//   stage    u = j/rho + F/2 on the cells of the touched tiles (from the
+//            post-stream populations and the force of the last collision);
+//   interp   U_k = sum phi phi phi u                     (K4)
+//   forces   per element (triangle / link / pair) contributions, then a
+//            per-vertex gather over the CSR incidence lists (K5, no atomics);
+//   spread   one workgroup per touched tile: its vertex list is sorted in
+//            LDS, every cell sums g + sum_k phi F_k in vertex order (K6,
+//            deterministic, no float atomics);
+//   move     x += U, per-particle area / volume / COM / bbox in FP64 with a
+//            fixed reduction order, COM mapped onto the periodic grid (K7);
+//   voxelize z-ray hits per column, sorted, parity fill of the phase (K8),
+//            only when the inner and outer viscosities differ.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ibm.hpp"
+
+namespace bl {
+
+namespace {
+
+constexpr int kMaxList = 1024;     // vertices per tile the spreading kernel sorts in LDS
+constexpr int kScanChunk = 1024;   // items per scan block
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ int find_part(const long* off, int np, long g)
+{
+    int lo = 0, hi = np;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Delta kernels (src/fsi/Dirac.h:23-66,74-101), 1-D factor.
+__device__ __forceinline__ double dirac(int kernel, double x)
+{
+    const double dx = fabs(x);
+    if (kernel == BL_KERNEL_ROMA)
+        return (1.0 / 6.0) * (dx > 0.5 ? (5.0 - 3 * dx - sqrt(-3 * (1 - dx) * (1 - dx) + 1))
+                                       : 2.0 * (1 + sqrt(1 - 3 * dx * dx)));
+    if (kernel == BL_KERNEL_PESKIN)
+        return 0.125 * (dx <= 1 ? (3 - 2 * dx + sqrt(1 + 4 * dx * (1 - dx)))
+                                : (5 - 2 * dx - sqrt(-7 + 4 * dx * (3 - dx))));
+    return 1. - fabs(x);
+}
+
+__device__ __forceinline__ int wrap_node(int i, int n, int per)
+{
+    if (per) {
+        int r = i % n;
+        return r < 0 ? r + n : r;
+    }
+    return (i < 0 || i >= n) ? -1 : i;
+}
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ D3 operator/(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 a, D3 b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double norm(D3 a) { return sqrt(dot(a, a)); }
+__device__ __forceinline__ void st3(double* p, D3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
+
+struct Grid {
+    int n[3];
+    int per[3];
+    int nt[3];
+};
+
+Grid grid_of(const bl_lattice_s* L)
+{
+    Grid g;
+    g.n[0] = L->nx; g.n[1] = L->ny; g.n[2] = L->nz;
+    for (int d = 0; d < 3; ++d) g.per[d] = L->periodic[d];
+    g.nt[0] = L->ntx; g.nt[1] = L->nty; g.nt[2] = L->ntz;
+    return g;
+}
+
+// ------------------------------------------------------------- binning
+// Distinct tiles touched along one axis by nodes base..base+S-1 (wrapped or
+// clipped); returns the count (<= 4).
+__device__ __forceinline__ int axis_tiles(int base, int S, int n, int per, int out[4])
+{
+    int m = 0;
+    for (int k = 0; k < S; ++k) {
+        const int node = wrap_node(base + k, n, per);
+        if (node < 0) continue;
+        const int t = node >> 2;
+        bool seen = false;
+        for (int q = 0; q < m; ++q) seen = seen || out[q] == t;
+        if (!seen) out[m++] = t;
+    }
+    return m;
+}
+
+__global__ void k_bin_count(Grid gd, int kernel, int S, double h, long NV, const double* __restrict__ pos,
+                            int* __restrict__ base, double* __restrict__ W, int* __restrict__ count)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        const double x = pos[3 * v + a];
+        b[a] = (int)ceil(x - h);
+        base[3 * v + a] = b[a];
+        for (int k = 0; k < 4; ++k)
+            W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(b[a] + k) - x) : 0.0;
+    }
+    int tx[4], ty[4], tz[4];
+    const int mx = axis_tiles(b[0], S, gd.n[0], gd.per[0], tx);
+    const int my = axis_tiles(b[1], S, gd.n[1], gd.per[1], ty);
+    const int mz = axis_tiles(b[2], S, gd.n[2], gd.per[2], tz);
+    for (int i = 0; i < mx; ++i)
+        for (int j = 0; j < my; ++j)
+            for (int k = 0; k < mz; ++k) atomicAdd(&count[(tx[i] * gd.nt[1] + ty[j]) * gd.nt[2] + tz[k]], 1);
+}
+
+__global__ void k_bin_fill(Grid gd, int S, long NV, const int* __restrict__ base, const int* __restrict__ off,
+                           int* __restrict__ fill, int* __restrict__ list, long cap, int* __restrict__ status)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    int tx[4], ty[4], tz[4];
+    const int mx = axis_tiles(base[3 * v], S, gd.n[0], gd.per[0], tx);
+    const int my = axis_tiles(base[3 * v + 1], S, gd.n[1], gd.per[1], ty);
+    const int mz = axis_tiles(base[3 * v + 2], S, gd.n[2], gd.per[2], tz);
+    for (int i = 0; i < mx; ++i)
+        for (int j = 0; j < my; ++j)
+            for (int k = 0; k < mz; ++k) {
+                const int t = (tx[i] * gd.nt[1] + ty[j]) * gd.nt[2] + tz[k];
+                const long slot = (long)off[t] + atomicAdd(&fill[t], 1);
+                if (slot < cap) list[slot] = (int)v;
+                else atomicOr(&status[2], 1);
+            }
+}
+
+// --------------------------------------------------------------- scan
+// Exclusive scan of tile counts into offsets, plus compaction of the tiles
+// with a non-zero count into `active`.  Packed 64-bit values carry
+// (count, count > 0) so one scan yields both.
+__device__ __forceinline__ unsigned long long pack_tile(int c)
+{
+    return (unsigned long long)(unsigned)c | ((unsigned long long)(c > 0) << 32);
+}
+
+__global__ void __launch_bounds__(256) k_scan_partial(const int* in, long n, unsigned long long* sums)
+{
+    __shared__ unsigned long long sh[256];
+    const long b0 = (long)blockIdx.x * kScanChunk;
+    unsigned long long acc = 0;
+    for (int k = 0; k < 4; ++k) {
+        const long i = b0 + threadIdx.x * 4 + k;
+        if (i < n) acc += pack_tile(in[i]);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
+}
+
+// exclusive scan of the block sums (single block, any count); writes the
+// totals into status[0] (active tiles) and status[1] (list entries)
+__global__ void __launch_bounds__(256) k_scan_top(unsigned long long* sums, long nb, int* status)
+{
+    __shared__ unsigned long long sh[256];
+    unsigned long long carry = 0;
+    for (long c0 = 0; c0 < nb; c0 += 256) {
+        const long i = c0 + threadIdx.x;
+        const unsigned long long v = i < nb ? sums[i] : 0ull;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const unsigned long long t = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0ull;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) sums[i] = carry + sh[threadIdx.x] - v;
+        const unsigned long long tot = sh[255];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        status[0] = (int)(carry >> 32);
+        status[1] = (int)(carry & 0xffffffffull);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(const int* in, long n, const unsigned long long* sums, int* off,
+                                                    int* active)
+{
+    __shared__ unsigned long long sh[256];
+    const long b0 = (long)blockIdx.x * kScanChunk;
+    unsigned long long loc[4];
+    unsigned long long acc = 0;
+    for (int k = 0; k < 4; ++k) {
+        const long i = b0 + threadIdx.x * 4 + k;
+        loc[k] = i < n ? pack_tile(in[i]) : 0ull;
+        acc += loc[k];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const unsigned long long t = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    unsigned long long run = sums[blockIdx.x] + sh[threadIdx.x] - acc;
+    for (int k = 0; k < 4; ++k) {
+        const long i = b0 + threadIdx.x * 4 + k;
+        if (i < n) {
+            off[i] = (int)(run & 0xffffffffull);
+            if (in[i] > 0) active[run >> 32] = (int)i;
+        }
+        run += loc[k];
+    }
+}
+
+__global__ void k_fill_i32(int* p, long n, int v)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_flags_from_counts(const int* count, long n, unsigned char* flags)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = count[i] > 0 ? 1 : 0;
+}
+
+// -------------------------------------------------------- interpolation
+// GeneralInterpolator (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:44-78):
+// periodic axes wrap node indices, other axes skip nodes outside the box.
+template <class C>
+__global__ void k_interpolate(Grid gd, int S, long NV, const int* __restrict__ base, const double* __restrict__ W,
+                              const C* __restrict__ u, double* __restrict__ vel)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const int b0 = base[3 * v], b1 = base[3 * v + 1], b2 = base[3 * v + 2];
+    const double* w = W + 12 * v;
+    double acc[3] = {0, 0, 0};
+    for (int i = 0; i < S; ++i) {
+        const int x = wrap_node(b0 + i, gd.n[0], gd.per[0]);
+        if (x < 0) continue;
+        const double wx = w[i];
+        for (int j = 0; j < S; ++j) {
+            const int y = wrap_node(b1 + j, gd.n[1], gd.per[1]);
+            if (y < 0) continue;
+            const double wxy = wx * w[4 + j];
+            for (int k = 0; k < S; ++k) {
+                const int z = wrap_node(b2 + k, gd.n[2], gd.per[2]);
+                if (z < 0) continue;
+                const double wxyz = wxy * w[8 + k];
+                const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
+                acc[0] += wxyz * (double)u[3 * c];
+                acc[1] += wxyz * (double)u[3 * c + 1];
+                acc[2] += wxyz * (double)u[3 * c + 2];
+            }
+        }
+    }
+    vel[3 * v] = acc[0];
+    vel[3 * v + 1] = acc[1];
+    vel[3 * v + 2] = acc[2];
+}
+
+// ------------------------------------------------------------ membrane
+struct ForceView {
+    int np;
+    const long *p_voff, *p_toff, *p_loff, *p_poff;
+    const int *p_shape, *p_model, *p_bug;
+    const double* p_params;
+    const double* p_geom;
+    const int *s_voff, *s_toff, *s_loff, *s_poff, *s_vtoff, *s_vloff, *s_vpoff;
+    const int* s_tri;
+    const double *s_tri_area0, *s_tri_geo0;
+    const int* s_link;
+    const double* s_link_len;
+    const int* s_pair;
+    const double* s_pair_cs0;
+    const int *s_vt_head, *s_vt, *s_vl_head, *s_vl, *s_vp_head, *s_vp;
+    const double* s_area0;
+    const double* pos;
+    double *c_tri, *c_link, *c_pair;
+};
+
+// Skalak / NeoHookean in-plane forces of one triangle
+// (src/fsi/DeformableCapsule.hh:60-150), with the reference's
+// c = l' sin(phi) / (l' sin(phi0)) when `bug` is set.
+__device__ void capsule_triangle(D3 p0, D3 p1, D3 p2, const double* geo0, double area0, double G, double Cc,
+                                 bool skalak, bool bug, D3 out[3])
+{
+    const D3 e0 = p1 - p0, e1 = p2 - p0;
+    const double l = norm(e0), lp = norm(e1);
+    const double l0 = geo0[0], l0p = geo0[1], cos_phi0 = geo0[2], sin_phi0 = geo0[3];
+    const double e0e1 = dot(e0, e1);
+    const double sin_phi = norm(cross(e0, e1)) / (l * lp);
+    const double cos_phi = e0e1 / (l * lp);
+    const double a = l / l0;
+    const double b = (lp * cos_phi / l0p - l * cos_phi0 / l0) / sin_phi0;
+    const double c = bug ? lp * sin_phi / (lp * sin_phi0) : lp * sin_phi / (l0p * sin_phi0);
+    const double D11 = (a * a - 1.) / 2.;
+    const double D22 = (b * b + c * c - 1) / 2.;
+    const double D12 = a * b / 2.;
+    const double I1 = 2. * (D11 + D22);
+    const double I2 = I1 + 4 * (D11 * D22 - D12 * D12);
+    D3 gl[3], glp[3], gc[3];
+    gl[0] = e0 / (-l);
+    gl[1] = e0 / l;
+    gl[2] = {0, 0, 0};
+    glp[0] = e1 / (-lp);
+    glp[1] = {0, 0, 0};
+    glp[2] = e1 / lp;
+    gc[0] = (-1. / (l * lp) + e0e1 / (l * l * l * lp)) * e0 + (-1. / (l * lp) + e0e1 / (l * lp * lp * lp)) * e1;
+    gc[1] = (1. / (l * lp)) * e1 - (e0e1 / (l * l * l * lp)) * e0;
+    gc[2] = (1. / (l * lp)) * e0 - (e0e1 / (l * lp * lp * lp)) * e1;
+    const double f1 = l / (l0 * l0);
+    const double f2a = lp / (l0p * l0p) - l * cos_phi * cos_phi0 / (l0 * l0p);
+    const double f2b = l * (cos_phi0 * cos_phi0) / (l0 * l0) - lp * cos_phi * cos_phi0 / (l0p * l0);
+    const double f2c = l * lp * cos_phi0 / (l0 * l0p);
+    const double f2 = 1. / (sin_phi0 * sin_phi0);
+    const double f3 = 1. / (2. * l0p * sin_phi0);
+    const double f3a = lp * cos_phi / l0p - 2. * l * cos_phi0 / l0;
+    const double f3b = l * cos_phi / l0p;
+    const double f3c = l * lp / l0p;
+    for (int i = 0; i < 3; ++i) {
+        const D3 gD11 = f1 * gl[i];
+        const D3 gD22 = f2 * (f2a * glp[i] + f2b * gl[i] - f2c * gc[i]);
+        const D3 gD12 = f3 * (f3a * gl[i] + f3b * glp[i] + f3c * gc[i]);
+        const D3 gI1 = 2. * (gD11 + gD22);
+        const D3 gI2 = (2. + 4. * D22) * gD11 + (2. + 4. * D11) * gD22 - (8. * D12) * gD12;
+        if (skalak)
+            out[i] = (-G / 2. * area0) * ((I1 + 1) * gI1 + (Cc * I2 - 1) * gI2);
+        else
+            out[i] = (-G / 2. * area0) * (gI1 - (1. / ((1 + I2) * (1 + I2))) * gI2);
+    }
+}
+
+// Area (local + global) and volume constraint forces of one triangle
+// (src/fsi/RBCParticle.hh:176-209, src/fsi/TriangleUtils.h:60-129).
+__device__ void area_volume_triangle(D3 v0, D3 v1, D3 v2, double area0, double A0tot, double A, double V,
+                                     double k_ag, double k_al, double k_v, double V0, D3 out[3])
+{
+    const double v01v11 = v0.y - v1.y, v02v22 = v0.z - v2.z, v02v12 = v0.z - v1.z;
+    const double v01v21 = v0.y - v2.y, v00v20 = v0.x - v2.x, v00v10 = v0.x - v1.x;
+    const double a = v01v11 * v02v22 - v02v12 * v01v21;
+    const double b = v02v12 * v00v20 - (v0.x - v1.x) * v02v22;
+    const double c = v00v10 * v01v21 - v01v11 * v00v20;
+    const double la = 0.5 * sqrt(a * a + b * b + c * c);
+    const double denom = 1. / (4. * la);
+    D3 g0, g1, g2;
+    g0.x = (b * (v2.z - v1.z) + c * (v1.y - v2.y)) * denom;
+    g0.y = (a * (v1.z - v2.z) + c * (v2.x - v1.x)) * denom;
+    g0.z = (a * (v2.y - v1.y) + b * (v1.x - v2.x)) * denom;
+    g1.x = (b * v02v22 + c * (v2.y - v0.y)) * denom;
+    g1.y = (a * (v2.z - v0.z) + c * v00v20) * denom;
+    g1.z = (a * v01v21 + b * (v2.x - v0.x)) * denom;
+    g2.x = (b * (v1.z - v0.z) + c * v01v11) * denom;
+    g2.y = (a * (v02v12) + c * (v1.x - v0.x)) * denom;
+    g2.z = (a * (v1.y - v0.y) + b * v00v10) * denom;
+    double df_area = 0.;
+    df_area += k_ag * (A0tot - A) / A0tot;
+    df_area += k_al * (area0 - la) / area0;
+    const double df_vol = k_v * (V0 - V) / V0;
+    out[0] = out[0] + df_area * g0;
+    out[1] = out[1] + df_area * g1;
+    out[2] = out[2] + df_area * g2;
+    out[0] = out[0] + df_vol * (cross(v1, v2) / 6.);
+    out[1] = out[1] + df_vol * (cross(v2, v0) / 6.);
+    out[2] = out[2] + df_vol * (cross(v0, v1) / 6.);
+}
+
+__global__ void k_tri_forces(ForceView f, long NT)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NT) return;
+    const int p = find_part(f.p_toff, f.np, g);
+    const int s = f.p_shape[p];
+    const long lt = g - f.p_toff[p];
+    const long st = f.s_toff[s] + lt;
+    const long vo = f.p_voff[p];
+    const int i0 = f.s_tri[3 * st], i1 = f.s_tri[3 * st + 1], i2 = f.s_tri[3 * st + 2];
+    const D3 v0 = ld3(f.pos + 3 * (vo + i0)), v1 = ld3(f.pos + 3 * (vo + i1)), v2 = ld3(f.pos + 3 * (vo + i2));
+    const int model = f.p_model[p];
+    const double* P = f.p_params + 8 * p;
+    D3 out[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double area0 = f.s_tri_area0[st];
+    if (model == BL_MODEL_CAPSULE_SKALAK || model == BL_MODEL_CAPSULE_NEOHOOKEAN || model == 3)
+        capsule_triangle(v0, v1, v2, f.s_tri_geo0 + 4 * st, area0, P[0], P[1], model != BL_MODEL_CAPSULE_NEOHOOKEAN,
+                         f.p_bug[p] != 0, out);
+    if (model == BL_MODEL_RBC || model == 3) {
+        const double A = f.p_geom[11 * p], V = f.p_geom[11 * p + 1];
+        area_volume_triangle(v0, v1, v2, area0, f.s_area0[s], A, V, P[2], P[3], P[4], P[5], out);
+    }
+    for (int k = 0; k < 3; ++k) st3(f.c_tri + 9 * g + 3 * k, out[k]);
+}
+
+// WLC-POW link springs (src/fsi/RBCParticle.hh:27-44,149-171).
+__global__ void k_link_forces(ForceView f, long NL)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NL) return;
+    const int p = find_part(f.p_loff, f.np, g);
+    if (f.p_model[p] != BL_MODEL_RBC) return;
+    const int s = f.p_shape[p];
+    const long sl = f.s_loff[s] + (g - f.p_loff[p]);
+    const long vo = f.p_voff[p];
+    const double* P = f.p_params + 8 * p;
+    const double m = 2;
+    const double x0 = 1. / 2.2;
+    const double ks_coeff1 = 0.25 * (6. * x0 + (3 * x0 * x0 - 2 * x0 * x0 * x0) / ((1 - x0) * (1 - x0)));
+    const double ks_coeff2 = x0 / (2. * (1 - x0) * (1 - x0) * (1 - x0)) + 0.25 * (1. - 1. / ((1 - x0) * (1 - x0)));
+    const double l0 = f.s_link_len[sl];
+    const double kp_coeff1 = -1. / pow(l0, m);
+    const double kp_coeff2 = (m + 1) / pow(l0, m);
+    const double G_lhs = 4. * l0 * P[0] / sqrt(3.0);
+    const double ks = -kp_coeff1 * G_lhs / (ks_coeff1 * kp_coeff2 - ks_coeff2 * kp_coeff1);
+    const double kp = ks_coeff1 * G_lhs / (ks_coeff1 * kp_coeff2 - ks_coeff2 * kp_coeff1);
+    const int a = f.s_link[2 * sl], b = f.s_link[2 * sl + 1];
+    const D3 dv = ld3(f.pos + 3 * (vo + a)) - ld3(f.pos + 3 * (vo + b));
+    const double L = norm(dv);
+    const double x = L / (l0 * 2.2);
+    const double df = -0.25 * ks * (6. * x + (3. * x * x - 2. * x * x * x) / ((1 - x) * (1 - x))) + kp / pow(L, m);
+    const D3 fa = (df / L) * dv;
+    st3(f.c_link + 6 * g, fa);
+    st3(f.c_link + 6 * g + 3, {-fa.x, -fa.y, -fa.z});
+}
+
+// Dihedral bending of one adjacent pair (src/fsi/RBCParticle.hh:214-246,
+// src/fsi/TriangleUtils.h:148-192).
+__global__ void k_pair_forces(ForceView f, long NP)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NP) return;
+    const int p = find_part(f.p_poff, f.np, g);
+    const int model = f.p_model[p];
+    if (model != BL_MODEL_RBC && model != 3) return;
+    const int s = f.p_shape[p];
+    const long sp = f.s_poff[s] + (g - f.p_poff[p]);
+    const long vo = f.p_voff[p];
+    const double kb = f.p_params[8 * p + 6];
+    const int* q = f.s_pair + 4 * sp;
+    const D3 v0 = ld3(f.pos + 3 * (vo + q[0])), v1 = ld3(f.pos + 3 * (vo + q[1]));
+    const D3 v2 = ld3(f.pos + 3 * (vo + q[2])), v3 = ld3(f.pos + 3 * (vo + q[3]));
+    const D3 e0 = v1 - v0, e1 = v2 - v0, e2 = v3 - v0;
+    const double e1n = norm(e1);
+    const D3 e1u = e1 / e1n;
+    const D3 n1 = cross(e0, e1), n2 = cross(e1, e2);
+    const double n1n = norm(n1), n2n = norm(n2);
+    const D3 n1u = n1 / n1n, n2u = n2 / n2n;
+    const double cs = dot(n1u, n2u);
+    const double sn = ((dot(n1 - n2, v1 - v3) > 0) ? 1.0 : -1.0) * norm(cross(n1u, n2u));
+    const double pref = kb * (-f.s_pair_cs0[2 * sp] * sn + f.s_pair_cs0[2 * sp + 1] * cs);
+    const D3 g0 = (dot(e1u, e1 - e2) / n2n) * n2u + (dot(e1u, e1 - e0) / n1n) * n1u;
+    const D3 g1 = (-e1n / n1n) * n1u;
+    const D3 g2 = (dot(e1u, e2) / n2n) * n2u + (dot(e1u, e0) / n1n) * n1u;
+    const D3 g3 = (-e1n / n2n) * n2u;
+    st3(f.c_pair + 12 * g, pref * g0);
+    st3(f.c_pair + 12 * g + 3, pref * g1);
+    st3(f.c_pair + 12 * g + 6, pref * g2);
+    st3(f.c_pair + 12 * g + 9, pref * g3);
+}
+
+// Per-vertex sum of its element contributions, in the order the reference
+// accumulates them: links, triangles, pairs (element order within each).
+__global__ void k_gather_forces(ForceView f, long NV, double* __restrict__ force)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const int p = find_part(f.p_voff, f.np, v);
+    const int s = f.p_shape[p];
+    const int model = f.p_model[p];
+    const long lv = v - f.p_voff[p];
+    const long sv = f.s_voff[s] + lv;
+    double F[3] = {0, 0, 0};
+    if (model == BL_MODEL_RBC) {
+        const int* head = f.s_vl_head + sv + s;  // heads are stored per shape with one extra entry
+        for (int e = head[0]; e < head[1]; ++e) {
+            const int code = f.s_vl[f.s_vloff[s] + e];
+            const double* c = f.c_link + 6 * (f.p_loff[p] + (code >> 2)) + 3 * (code & 3);
+            F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
+        }
+    }
+    {
+        const int* head = f.s_vt_head + sv + s;
+        for (int e = head[0]; e < head[1]; ++e) {
+            const int code = f.s_vt[f.s_vtoff[s] + e];
+            const double* c = f.c_tri + 9 * (f.p_toff[p] + (code >> 2)) + 3 * (code & 3);
+            F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
+        }
+    }
+    if (model == BL_MODEL_RBC || model == 3) {
+        const int* head = f.s_vp_head + sv + s;
+        for (int e = head[0]; e < head[1]; ++e) {
+            const int code = f.s_vp[f.s_vpoff[s] + e];
+            const double* c = f.c_pair + 12 * (f.p_poff[p] + (code >> 2)) + 3 * (code & 3);
+            F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
+        }
+    }
+    force[3 * v] = F[0];
+    force[3 * v + 1] = F[1];
+    force[3 * v + 2] = F[2];
+}
+
+// ------------------------------------------------------------ spreading
+// One 64-thread workgroup per touched tile (4x4x4 cells).  The tile's
+// vertex list is sorted by vertex id in LDS (bitonic), so every cell adds
+// its contributions in the same order as the reference's vertex loop
+// (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85) whatever order the
+// binning wrote them in.
+template <class FT>
+__global__ void __launch_bounds__(64) k_spread(Grid gd, int S, const int* __restrict__ active,
+                                               int* __restrict__ status, const int* __restrict__ count,
+                                               const int* __restrict__ off, const int* __restrict__ list,
+                                               const int* __restrict__ base, const double* __restrict__ W,
+                                               const double* __restrict__ vforce, double gx, double gy,
+                                               double gz, FT* __restrict__ F, long stride)
+{
+    __shared__ int keys[kMaxList];
+    const int na = status[0];
+    const int lane = threadIdx.x;
+    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+        const int t = active[a];
+        int n = count[t];
+        if (n > kMaxList) {  // reported by the host check; the tile keeps only the first entries
+            if (lane == 0) atomicOr(&status[2], 4);
+            n = kMaxList;
+        }
+        const int o = off[t];
+        int n2 = 1;
+        while (n2 < n) n2 <<= 1;
+        for (int k = lane; k < n2; k += 64) keys[k] = k < n ? list[o + k] : 0x7fffffff;
+        __syncthreads();
+        for (int size = 2; size <= n2; size <<= 1)
+            for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
+                for (int k = lane; k < n2; k += 64) {
+                    const int partner = k ^ stride2;
+                    if (partner > k) {
+                        const bool up = (k & size) == 0;
+                        const int a0 = keys[k], a1 = keys[partner];
+                        if ((a0 > a1) == up) {
+                            keys[k] = a1;
+                            keys[partner] = a0;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        const int tz = t % gd.nt[2], r = t / gd.nt[2], ty = r % gd.nt[1], tx = r / gd.nt[1];
+        const int x = tx * kTile + (lane >> 4), y = ty * kTile + ((lane >> 2) & 3), z = tz * kTile + (lane & 3);
+        const bool inside = x < gd.n[0] && y < gd.n[1] && z < gd.n[2];
+        double acc[3] = {gx, gy, gz};
+        if (inside) {
+            for (int k = 0; k < n; ++k) {
+                const int v = keys[k];
+                const int* b = base + 3 * v;
+                const double* w = W + 12 * v;
+                double wx = 0, wy = 0, wz = 0;
+                bool hx = false, hy = false, hz = false;
+                for (int q = 0; q < S; ++q) {
+                    if (wrap_node(b[0] + q, gd.n[0], gd.per[0]) == x) { wx = hx ? wx + w[q] : w[q]; hx = true; }
+                    if (wrap_node(b[1] + q, gd.n[1], gd.per[1]) == y) { wy = hy ? wy + w[4 + q] : w[4 + q]; hy = true; }
+                    if (wrap_node(b[2] + q, gd.n[2], gd.per[2]) == z) { wz = hz ? wz + w[8 + q] : w[8 + q]; hz = true; }
+                }
+                if (!(hx && hy && hz)) continue;
+                const double wxyz = wx * wy * wz;
+                acc[0] += wxyz * vforce[3 * v];
+                acc[1] += wxyz * vforce[3 * v + 1];
+                acc[2] += wxyz * vforce[3 * v + 2];
+            }
+            const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
+            F[c] = (FT)acc[0];
+            F[stride + c] = (FT)acc[1];
+            F[2 * stride + c] = (FT)acc[2];
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- motion
+__global__ void k_move(long NV, double* __restrict__ pos, const double* __restrict__ vel)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 3 * NV) return;
+    pos[v] += vel[v];
+}
+
+// One workgroup per particle: area, signed volume, area-weighted centroid
+// (src/fsi/DeformableParticle3D.hh:54-82) and bounding box
+// (src/fsi/ParticleBase.hh:51-69), reduced in a fixed order; then the
+// centre of mass is mapped onto the periodic grid
+// (src/fsi/ParticleBase.hh:113-136, src/fsi/Periodicity.h:83-87) and the
+// shift recorded for k_apply_shift.
+__global__ void __launch_bounds__(256) k_update_particles(ForceView f, Grid gd, double* geom, double* delta,
+                                                          int remap)
+{
+    __shared__ double sh[5][256];
+    __shared__ double bb[6][256];
+    const int p = blockIdx.x;
+    const int s = f.p_shape[p];
+    const long vo = f.p_voff[p], nv = f.p_voff[p + 1] - vo;
+    const long nt = f.p_toff[p + 1] - f.p_toff[p];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (long t = threadIdx.x; t < nt; t += blockDim.x) {
+        const int* tri = f.s_tri + 3 * (f.s_toff[s] + t);
+        const D3 v0 = ld3(f.pos + 3 * (vo + tri[0])), v1 = ld3(f.pos + 3 * (vo + tri[1])), v2 = ld3(f.pos + 3 * (vo + tri[2]));
+        const double dA = 0.5 * norm(cross(v1 - v0, v2 - v0));
+        acc[0] += dA;
+        acc[1] += dot(v0, cross(v1, v2)) / 6.;
+        acc[2] += ((v0.x + v1.x + v2.x) / 3.0) * dA;
+        acc[3] += ((v0.y + v1.y + v2.y) / 3.0) * dA;
+        acc[4] += ((v0.z + v1.z + v2.z) / 3.0) * dA;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (long v = threadIdx.x; v < nv; v += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            const double x = f.pos[3 * (vo + v) + d];
+            lo[d] = fmin(lo[d], x);
+            hi[d] = fmax(hi[d], x);
+        }
+    for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+    for (int d = 0; d < 3; ++d) {
+        bb[2 * d][threadIdx.x] = lo[d];
+        bb[2 * d + 1][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+            for (int d = 0; d < 3; ++d) {
+                bb[2 * d][threadIdx.x] = fmin(bb[2 * d][threadIdx.x], bb[2 * d][threadIdx.x + w]);
+                bb[2 * d + 1][threadIdx.x] = fmax(bb[2 * d + 1][threadIdx.x], bb[2 * d + 1][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* G = geom + 11 * p;
+        const double area = sh[0][0];
+        double com[3] = {sh[2][0] / area, sh[3][0] / area, sh[4][0] / area};
+        double sft[3] = {0, 0, 0};
+        if (remap) {
+            for (int d = 0; d < 3; ++d) {
+                if (!gd.per[d]) continue;
+                const double L = (double)gd.n[d];
+                const double x0 = -0.5;
+                const double r = com[d] - L * floor((com[d] - x0) * (1. / L));
+                sft[d] = r - com[d];
+                com[d] = r;
+            }
+        }
+        G[0] = area;
+        G[1] = sh[1][0];
+        for (int d = 0; d < 3; ++d) {
+            G[2 + d] = com[d];
+            G[5 + 2 * d] = bb[2 * d][0] + sft[d];
+            G[6 + 2 * d] = bb[2 * d + 1][0] + sft[d];
+            delta[3 * p + d] = sft[d];
+        }
+    }
+}
+
+__global__ void k_apply_shift(const long* p_voff, int np, long NV, const double* delta, double* pos)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const int p = find_part(p_voff, np, v);
+    for (int d = 0; d < 3; ++d) pos[3 * v + d] += delta[3 * p + d];
+}
+
+// --------------------------------------------------------- voxelization
+// z-rays through integer (x, y) columns (RayTracer::find_intersections_z,
+// src/fsi/RayTracer.hh:19-39,80-108).  Pass 0 counts hits per column,
+// pass 1 writes them (z mapped into [-0.5, nz - 0.5) on a periodic z axis,
+// as ZBuffer::put does, src/fsi/ZBuffer.h:39-49).
+__global__ void k_ray_hits(ForceView f, Grid gd, long NT, int pass, int* col_count, const int* col_off,
+                           int* col_fill, double* col_z, long cap, int* status)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NT) return;
+    const int p = find_part(f.p_toff, f.np, g);
+    const int s = f.p_shape[p];
+    const long vo = f.p_voff[p];
+    const int* tri = f.s_tri + 3 * (f.s_toff[s] + (g - f.p_toff[p]));
+    const D3 a = ld3(f.pos + 3 * (vo + tri[0])), b = ld3(f.pos + 3 * (vo + tri[1])), c = ld3(f.pos + 3 * (vo + tri[2]));
+    const long x0 = (long)ceil(fmin(a.x, fmin(b.x, c.x))), x1 = (long)floor(fmax(a.x, fmax(b.x, c.x)));
+    const long y0 = (long)ceil(fmin(a.y, fmin(b.y, c.y))), y1 = (long)floor(fmax(a.y, fmax(b.y, c.y)));
+    const D3 e1 = b - a, e2 = c - a;
+    const D3 dir = {0, 0, 1};
+    const D3 q = cross(dir, e2);
+    const double det = dot(e1, q);
+    if (fabs(det) < 2.220446049250313e-16) return;
+    const double fi = 1 / det;
+    for (long i = x0; i <= x1; ++i)
+        for (long j = y0; j <= y1; ++j) {
+            const D3 o = {(double)i, (double)j, 0.};
+            const D3 sv = o - a;
+            const double u = fi * dot(sv, q);
+            if (u < 0.0) continue;
+            const D3 r = cross(sv, e1);
+            const double v = fi * dot(dir, r);
+            if (v < 0.0 || u + v > 1.0) continue;
+            const int cx_ = wrap_node((int)i, gd.n[0], gd.per[0]);
+            const int cy_ = wrap_node((int)j, gd.n[1], gd.per[1]);
+            if (cx_ < 0 || cy_ < 0) continue;
+            const long col = (long)cx_ * gd.n[1] + cy_;
+            if (pass == 0) {
+                atomicAdd(&col_count[col], 1);
+            } else {
+                double z = fi * dot(e2, r);
+                if (gd.per[2]) {
+                    const double L = (double)gd.n[2];
+                    z = z - L * floor((z + 0.5) * (1. / L));
+                }
+                const long slot = (long)col_off[col] + atomicAdd(&col_fill[col], 1);
+                if (slot < cap) col_z[slot] = z;
+                else atomicOr(&status[2], 2);
+            }
+        }
+}
+
+// Per column: sort the hits and fill the phase as apply_voxelization does
+// (src/fsi/ImmersedBoundaryDynamics_voxelization.hh:121-165): alternate
+// outside/inside between consecutive hits, later segments overwrite the
+// shared end node.
+__global__ void k_fill_columns(Grid gd, const int* col_count, const int* col_off, double* col_z, long cap,
+                               unsigned char* phase)
+{
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long)gd.n[0] * gd.n[1]) return;
+    const int nz = gd.n[2];
+    unsigned char* ph = phase + col * nz;
+    int n = col_count[col];
+    const long o = col_off[col];
+    if (o + n > cap) n = (int)(cap > o ? cap - o : 0);
+    double* zs = col_z + o;
+    for (int i = 1; i < n; ++i) {  // insertion sort (few hits per column)
+        const double key = zs[i];
+        int j = i - 1;
+        while (j >= 0 && zs[j] > key) {
+            zs[j + 1] = zs[j];
+            --j;
+        }
+        zs[j + 1] = key;
+    }
+    bool inside = false;
+    double z_last = -10000;
+    for (int k = 0; k < n; ++k) {
+        const long zlo = max((long)ceil(z_last), 0L), zhi = min((long)floor(zs[k]), (long)nz - 1);
+        for (long z = zlo; z <= zhi; ++z) ph[z] = inside ? 1 : 0;
+        z_last = zs[k];
+        inside = !inside;
+    }
+    if (ceil(z_last) < nz - 1) {
+        const long zlo = max((long)ceil(z_last), 0L);
+        for (long z = zlo; z <= nz - 1; ++z) ph[z] = inside ? 1 : 0;
+    }
+}
+
+__global__ void k_scan_cols(const int* in, long n, int* off, int* total)
+{
+    // single-workgroup exclusive scan (columns are few: nx*ny)
+    __shared__ int sh[256];
+    int carry = 0;
+    for (long c0 = 0; c0 < n; c0 += 256) {
+        const long i = c0 + threadIdx.x;
+        const int v = i < n ? in[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int t = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) off[i] = carry + sh[threadIdx.x] - v;
+        const int tot = sh[255];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+}  // namespace
+
+// ======================================================== host helpers
+namespace {
+
+template <class T>
+int upload(T*& dst, const std::vector<T>& src)
+{
+    hipFree(dst);
+    dst = nullptr;
+    const size_t n = std::max<size_t>(src.size(), 1);
+    BL_HIP(hipMalloc(&dst, n * sizeof(T)));
+    if (!src.empty()) BL_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+template <class T>
+int alloc(T*& dst, size_t n)
+{
+    hipFree(dst);
+    dst = nullptr;
+    BL_HIP(hipMalloc(&dst, std::max<size_t>(n, 1) * sizeof(T)));
+    return BL_OK;
+}
+
+ForceView force_view(bl_ibm g)
+{
+    ForceView f;
+    f.np = g->np;
+    f.p_voff = g->p_voff; f.p_toff = g->p_toff; f.p_loff = g->p_loff; f.p_poff = g->p_poff;
+    f.p_shape = g->p_shape; f.p_model = g->p_model; f.p_bug = g->p_bug;
+    f.p_params = g->p_params; f.p_geom = g->p_geom;
+    f.s_voff = g->s_voff; f.s_toff = g->s_toff; f.s_loff = g->s_loff; f.s_poff = g->s_poff;
+    f.s_vtoff = g->s_vtoff; f.s_vloff = g->s_vloff; f.s_vpoff = g->s_vpoff;
+    f.s_tri = g->s_tri; f.s_tri_area0 = g->s_tri_area0; f.s_tri_geo0 = g->s_tri_geo0;
+    f.s_link = g->s_link; f.s_link_len = g->s_link_len;
+    f.s_pair = g->s_pair; f.s_pair_cs0 = g->s_pair_cs0;
+    f.s_vt_head = g->s_vt_head; f.s_vt = g->s_vt;
+    f.s_vl_head = g->s_vl_head; f.s_vl = g->s_vl;
+    f.s_vp_head = g->s_vp_head; f.s_vp = g->s_vp;
+    f.s_area0 = g->s_area0;
+    f.pos = g->pos;
+    f.c_tri = g->c_tri; f.c_link = g->c_link; f.c_pair = g->c_pair;
+    return f;
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+int need_init(bl_ibm g)
+{
+    if (!g) return fail(BL_ERR_ARG, "null ibm handle");
+    if (!g->inited) return fail(BL_ERR_STATE, "immersed boundary not initialised (call bl_ibm_init)");
+    return BL_OK;
+}
+
+void mark(bl_ibm g, int k)
+{
+    if (g->timing) hipEventRecord(g->ev[k], g->lat->stream);
+}
+
+int update_particles(bl_ibm g, int remap)
+{
+    if (g->np == 0) return BL_OK;
+    hipStream_t s = g->lat->stream;
+    const ForceView f = force_view(g);
+    const Grid gd = grid_of(g->lat);
+    hipLaunchKernelGGL(k_update_particles, dim3(g->np), dim3(256), 0, s, f, gd, g->p_geom, g->p_delta, remap);
+    if (remap)
+        hipLaunchKernelGGL(k_apply_shift, dim3(nblk(g->NV)), dim3(256), 0, s, (const long*)g->p_voff, g->np, g->NV,
+                           (const double*)g->p_delta, g->pos);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+// bin + stage + interpolate
+int do_interpolate(bl_ibm g)
+{
+    bl_lattice L = g->lat;
+    hipStream_t s = L->stream;
+    const Grid gd = grid_of(L);
+    if (g->NV == 0) return BL_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, g->n_active + 2, 1L, 0);  // overflow flags
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, L->ntiles, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_fill, L->ntiles, 0);
+    hipLaunchKernelGGL(k_bin_count, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
+                       (const double*)g->pos, g->v_base, g->v_w, g->tile_count);
+    const long nb = (L->ntiles + kScanChunk - 1) / kScanChunk;
+    unsigned long long* sums = (unsigned long long*)g->scan_blocks;
+    hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, sums, nb, g->n_active);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles,
+                       (const unsigned long long*)sums, g->tile_off, g->active);
+    hipLaunchKernelGGL(k_bin_fill, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV, (const int*)g->v_base,
+                       (const int*)g->tile_off, g->tile_fill, g->tile_list, g->list_cap, g->n_active);
+    BL_CHECK_LAUNCH();
+    int rc = lattice_stage_velocity_tiles(L, g->active, g->n_active, g->ubuf, s);
+    if (rc) return rc;
+    if (L->storage == BL_STORE_F64)
+        hipLaunchKernelGGL(k_interpolate<double>, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV,
+                           (const int*)g->v_base, (const double*)g->v_w, (const double*)g->ubuf, g->vel);
+    else
+        hipLaunchKernelGGL(k_interpolate<float>, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV,
+                           (const int*)g->v_base, (const double*)g->v_w, (const float*)g->ubuf, g->vel);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+int do_forces(bl_ibm g)
+{
+    hipStream_t s = g->lat->stream;
+    ForceView f = force_view(g);
+    if (g->NT) hipLaunchKernelGGL(k_tri_forces, dim3(nblk(g->NT)), dim3(256), 0, s, f, g->NT);
+    if (g->NL) hipLaunchKernelGGL(k_link_forces, dim3(nblk(g->NL)), dim3(256), 0, s, f, g->NL);
+    if (g->NP) hipLaunchKernelGGL(k_pair_forces, dim3(nblk(g->NP)), dim3(256), 0, s, f, g->NP);
+    if (g->NV) hipLaunchKernelGGL(k_gather_forces, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, g->force);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+int do_spread(bl_ibm g)
+{
+    bl_lattice L = g->lat;
+    hipStream_t s = L->stream;
+    const Grid gd = grid_of(L);
+    hipLaunchKernelGGL(k_flags_from_counts, dim3(nblk(L->ntiles)), dim3(256), 0, s, (const int*)g->tile_count,
+                       L->ntiles, L->tile_flags);
+    if (g->NV) {
+        if (L->storage == BL_STORE_F64)
+            hipLaunchKernelGGL(k_spread<double>, dim3(2048), dim3(64), 0, s, gd, g->S, (const int*)g->active,
+                               g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
+                               (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
+                               (const double*)g->force, L->g[0], L->g[1], L->g[2], (double*)L->force, L->stride);
+        else
+            hipLaunchKernelGGL(k_spread<float>, dim3(2048), dim3(64), 0, s, gd, g->S, (const int*)g->active,
+                               g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
+                               (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
+                               (const double*)g->force, L->g[0], L->g[1], L->g[2], (float*)L->force, L->stride);
+    }
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+int do_move(bl_ibm g)
+{
+    hipStream_t s = g->lat->stream;
+    if (g->NV) hipLaunchKernelGGL(k_move, dim3(nblk(3 * g->NV)), dim3(256), 0, s, g->NV, g->pos, (const double*)g->vel);
+    BL_CHECK_LAUNCH();
+    return update_particles(g, 1);
+}
+
+int do_voxelize(bl_ibm g)
+{
+    bl_lattice L = g->lat;
+    if (L->omega_c0 == L->omega_c1 || g->NT == 0) return BL_OK;
+    hipStream_t s = L->stream;
+    const Grid gd = grid_of(L);
+    const long ncol = (long)L->nx * L->ny;
+    const ForceView f = force_view(g);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ncol)), dim3(256), 0, s, g->col_count, ncol, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ncol)), dim3(256), 0, s, g->col_fill, ncol, 0);
+    hipLaunchKernelGGL(k_ray_hits, dim3(nblk(g->NT)), dim3(256), 0, s, f, gd, g->NT, 0, g->col_count,
+                       (const int*)g->col_off, g->col_fill, g->col_z, g->col_cap, g->n_active);
+    hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(256), 0, s, (const int*)g->col_count, ncol, g->col_off, g->n_hits);
+    hipLaunchKernelGGL(k_ray_hits, dim3(nblk(g->NT)), dim3(256), 0, s, f, gd, g->NT, 1, g->col_count,
+                       (const int*)g->col_off, g->col_fill, g->col_z, g->col_cap, g->n_active);
+    hipLaunchKernelGGL(k_fill_columns, dim3(nblk(ncol)), dim3(256), 0, s, gd, (const int*)g->col_count,
+                       (const int*)g->col_off, g->col_z, g->col_cap, g->phase);
+    BL_CHECK_LAUNCH();
+    L->phase = g->phase;
+    return BL_OK;
+}
+
+int check_overflow(bl_ibm g)
+{
+    int st[3] = {0, 0, 0};
+    BL_HIP(hipMemcpyAsync(st, g->n_active, sizeof(st), hipMemcpyDeviceToHost, g->lat->stream));
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    if (st[2] & 1) return fail(BL_ERR_CAPACITY, "immersed boundary: tile list capacity exceeded");
+    if (st[2] & 2) return fail(BL_ERR_CAPACITY, "voxelization: hit buffer capacity exceeded");
+    if (st[2] & 4) return fail(BL_ERR_CAPACITY, "immersed boundary: more than 1024 vertices touch one 4x4x4 tile");
+    return BL_OK;
+}
+
+int step_body(bl_ibm g)
+{
+    int rc;
+    mark(g, 0);
+    if ((rc = do_interpolate(g))) return rc;
+    mark(g, 1);
+    if ((rc = do_forces(g))) return rc;
+    mark(g, 2);
+    if ((rc = do_spread(g))) return rc;
+    mark(g, 3);
+    if ((rc = do_move(g))) return rc;
+    mark(g, 4);
+    if ((rc = do_voxelize(g))) return rc;
+    mark(g, 5);
+    return BL_OK;
+}
+
+}  // namespace
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" {
+
+int bl_ibm_create(bl_lattice lat, int kernel, bl_ibm* out)
+{
+    if (!lat || !out) return fail(BL_ERR_ARG, "bl_ibm_create: null argument");
+    if (kernel < BL_KERNEL_ROMA || kernel > BL_KERNEL_TOPHAT) return fail(BL_ERR_ARG, "bl_ibm_create: unknown kernel");
+    if (lat->ibm) return fail(BL_ERR_STATE, "bl_ibm_create: lattice already has an immersed boundary");
+    bl_ibm g = new bl_ibm_s();
+    g->lat = lat;
+    g->kernel = kernel;
+    g->S = kernel == BL_KERNEL_ROMA ? 3 : (kernel == BL_KERNEL_PESKIN ? 4 : 2);
+    g->h = kernel == BL_KERNEL_ROMA ? 1.5 : (kernel == BL_KERNEL_PESKIN ? 2.0 : 1.0);
+    lat->ibm = g;
+    *out = g;
+    return BL_OK;
+}
+
+int bl_ibm_destroy(bl_ibm g)
+{
+    if (!g) return BL_OK;
+    if (g->lat) {
+        hipStreamSynchronize(g->lat->stream);
+        if (g->lat->phase == g->phase) g->lat->phase = nullptr;
+        g->lat->ibm = nullptr;
+    }
+    void* ptrs[] = {g->s_voff, g->s_toff, g->s_loff, g->s_poff, g->s_vtoff, g->s_vloff, g->s_vpoff, g->s_tri,
+                    g->s_tri_area0, g->s_tri_geo0, g->s_rest, g->s_link, g->s_link_len, g->s_pair, g->s_pair_cs0,
+                    g->s_vt_head, g->s_vt, g->s_vl_head, g->s_vl, g->s_vp_head, g->s_vp, g->s_area0, g->p_shape,
+                    g->p_model, g->p_bug, g->p_params, g->p_voff, g->p_toff, g->p_loff, g->p_poff, g->p_geom,
+                    g->p_delta, g->pos, g->vel, g->force, g->c_tri, g->c_link, g->c_pair, g->v_base, g->v_w,
+                    g->tile_count, g->tile_off, g->tile_fill, g->tile_list, g->active, g->n_active, g->scan_blocks,
+                    g->ubuf, g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase};
+    for (void* p : ptrs) hipFree(p);
+    for (auto& e : g->ev)
+        if (e) hipEventDestroy(e);
+    if (g->graph2) hipGraphExecDestroy(g->graph2);
+    delete g;
+    return BL_OK;
+}
+
+int bl_shape_add(bl_ibm g, int nv, int nt, const double* rest_pos, const int* tri, int* shape_id)
+{
+    if (!g || !rest_pos || !tri || nv < 3 || nt < 1) return fail(BL_ERR_ARG, "bl_shape_add: bad argument");
+    if (g->inited) return fail(BL_ERR_STATE, "bl_shape_add: shapes must be added before bl_ibm_init");
+    ShapeHost sh;
+    std::string err;
+    if (!build_shape(nv, nt, rest_pos, tri, sh, &err)) return fail(BL_ERR_ARG, "bl_shape_add: " + err);
+    g->shapes.push_back(std::move(sh));
+    if (shape_id) *shape_id = (int)g->shapes.size() - 1;
+    return BL_OK;
+}
+
+int bl_shape_info(bl_ibm g, int shape_id, double out[6])
+{
+    if (!g || !out || shape_id < 0 || shape_id >= (int)g->shapes.size()) return fail(BL_ERR_ARG, "bl_shape_info: bad argument");
+    const ShapeHost& s = g->shapes[shape_id];
+    out[0] = s.area;
+    out[1] = s.volume;
+    for (int d = 0; d < 3; ++d) out[2 + d] = s.center[d];
+    out[5] = s.radius;
+    return BL_OK;
+}
+
+int bl_particle_add(bl_ibm g, int shape_id, int model, const double* params, int nparams, const double* pos,
+                    int bug_compatible, int* pid)
+{
+    if (!g || !params || !pos) return fail(BL_ERR_ARG, "bl_particle_add: null argument");
+    if (g->inited) return fail(BL_ERR_STATE, "bl_particle_add: particles must be added before bl_ibm_init");
+    if (shape_id < 0 || shape_id >= (int)g->shapes.size()) return fail(BL_ERR_ARG, "bl_particle_add: unknown shape");
+    if (model < 0 || model > 3) return fail(BL_ERR_ARG, "bl_particle_add: unknown model");
+    const int need = model <= 1 ? 2 : 7;
+    if (nparams < need || nparams > 8) return fail(BL_ERR_ARG, "bl_particle_add: wrong number of parameters");
+    bl_ibm_s::PartHost p;
+    p.shape = shape_id;
+    p.model = model;
+    p.bug = bug_compatible ? 1 : 0;
+    for (int k = 0; k < nparams; ++k) p.params[k] = params[k];
+    if ((model == BL_MODEL_RBC || model == 3) && !(p.params[5] > 0))
+        return fail(BL_ERR_ARG, "bl_particle_add: vol_desired must be positive");
+    const int nv = g->shapes[shape_id].nv;
+    p.pos.assign(pos, pos + 3 * (size_t)nv);
+    g->parts.push_back(std::move(p));
+    if (pid) *pid = (int)g->parts.size() - 1;
+    return BL_OK;
+}
+
+int bl_ibm_init(bl_ibm g)
+{
+    if (!g) return fail(BL_ERR_ARG, "null ibm handle");
+    if (g->inited) return fail(BL_ERR_STATE, "bl_ibm_init: already initialised");
+    bl_lattice L = g->lat;
+    int rc = ensure_force_field(L);
+    if (rc) return rc;
+    // shapes
+    std::vector<int> s_voff{0}, s_toff{0}, s_loff{0}, s_poff{0}, s_vtoff{0}, s_vloff{0}, s_vpoff{0};
+    std::vector<int> s_tri, s_link, s_pair, vt_head, vt, vl_head, vl, vp_head, vp;
+    std::vector<double> s_area0v, s_geo, s_rest, s_len, s_cs;
+    std::vector<double> s_tarea;
+    for (const ShapeHost& s : g->shapes) {
+        s_voff.push_back(s_voff.back() + s.nv);
+        s_toff.push_back(s_toff.back() + s.nt);
+        s_loff.push_back(s_loff.back() + s.nl);
+        s_poff.push_back(s_poff.back() + s.np);
+        s_vtoff.push_back(s_vtoff.back() + (int)s.v_tri.size());
+        s_vloff.push_back(s_vloff.back() + (int)s.v_link.size());
+        s_vpoff.push_back(s_vpoff.back() + (int)s.v_pair.size());
+        s_tri.insert(s_tri.end(), s.tri.begin(), s.tri.end());
+        s_tarea.insert(s_tarea.end(), s.tri_area0.begin(), s.tri_area0.end());
+        s_geo.insert(s_geo.end(), s.tri_geo0.begin(), s.tri_geo0.end());
+        s_rest.insert(s_rest.end(), s.rest.begin(), s.rest.end());
+        s_link.insert(s_link.end(), s.link.begin(), s.link.end());
+        s_len.insert(s_len.end(), s.link_len.begin(), s.link_len.end());
+        s_pair.insert(s_pair.end(), s.pair.begin(), s.pair.end());
+        s_cs.insert(s_cs.end(), s.pair_cs0.begin(), s.pair_cs0.end());
+        vt_head.insert(vt_head.end(), s.v_tri_off.begin(), s.v_tri_off.end());
+        vt.insert(vt.end(), s.v_tri.begin(), s.v_tri.end());
+        vl_head.insert(vl_head.end(), s.v_link_off.begin(), s.v_link_off.end());
+        vl.insert(vl.end(), s.v_link.begin(), s.v_link.end());
+        vp_head.insert(vp_head.end(), s.v_pair_off.begin(), s.v_pair_off.end());
+        vp.insert(vp.end(), s.v_pair.begin(), s.v_pair.end());
+        s_area0v.push_back(s.area);
+    }
+    g->nshape = (int)g->shapes.size();
+    if ((rc = upload(g->s_voff, s_voff)) || (rc = upload(g->s_toff, s_toff)) || (rc = upload(g->s_loff, s_loff)) ||
+        (rc = upload(g->s_poff, s_poff)) || (rc = upload(g->s_vtoff, s_vtoff)) || (rc = upload(g->s_vloff, s_vloff)) ||
+        (rc = upload(g->s_vpoff, s_vpoff)) || (rc = upload(g->s_tri, s_tri)) || (rc = upload(g->s_tri_area0, s_tarea)) ||
+        (rc = upload(g->s_tri_geo0, s_geo)) || (rc = upload(g->s_rest, s_rest)) || (rc = upload(g->s_link, s_link)) ||
+        (rc = upload(g->s_link_len, s_len)) || (rc = upload(g->s_pair, s_pair)) || (rc = upload(g->s_pair_cs0, s_cs)) ||
+        (rc = upload(g->s_vt_head, vt_head)) || (rc = upload(g->s_vt, vt)) || (rc = upload(g->s_vl_head, vl_head)) ||
+        (rc = upload(g->s_vl, vl)) || (rc = upload(g->s_vp_head, vp_head)) || (rc = upload(g->s_vp, vp)) ||
+        (rc = upload(g->s_area0, s_area0v)))
+        return rc;
+    // particles
+    g->np = (int)g->parts.size();
+    std::vector<long> voff{0}, toff{0}, loff{0}, poff{0};
+    std::vector<int> pshape, pmodel, pbug;
+    std::vector<double> params, pos;
+    for (const auto& p : g->parts) {
+        const ShapeHost& s = g->shapes[p.shape];
+        voff.push_back(voff.back() + s.nv);
+        toff.push_back(toff.back() + s.nt);
+        loff.push_back(loff.back() + s.nl);
+        poff.push_back(poff.back() + s.np);
+        pshape.push_back(p.shape);
+        pmodel.push_back(p.model);
+        pbug.push_back(p.bug);
+        params.insert(params.end(), p.params, p.params + 8);
+        pos.insert(pos.end(), p.pos.begin(), p.pos.end());
+    }
+    g->NV = voff.back();
+    g->NT = toff.back();
+    g->NL = loff.back();
+    g->NP = poff.back();
+    if ((rc = upload(g->p_voff, voff)) || (rc = upload(g->p_toff, toff)) || (rc = upload(g->p_loff, loff)) ||
+        (rc = upload(g->p_poff, poff)) || (rc = upload(g->p_shape, pshape)) || (rc = upload(g->p_model, pmodel)) ||
+        (rc = upload(g->p_bug, pbug)) || (rc = upload(g->p_params, params)) || (rc = upload(g->pos, pos)) ||
+        (rc = alloc(g->vel, 3 * g->NV)) || (rc = alloc(g->force, 3 * g->NV)) || (rc = alloc(g->p_geom, 11 * (size_t)g->np)) ||
+        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) || (rc = alloc(g->c_tri, 9 * g->NT)) ||
+        (rc = alloc(g->c_link, 6 * g->NL)) || (rc = alloc(g->c_pair, 12 * g->NP)) ||
+        (rc = alloc(g->v_base, 3 * g->NV)) || (rc = alloc(g->v_w, 12 * g->NV)))
+        return rc;
+    BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
+    BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
+    // tiles
+    g->list_cap = 27 * std::max<long>(g->NV, 1);
+    g->nscan_blocks = (L->ntiles + kScanChunk - 1) / kScanChunk;
+    if ((rc = alloc(g->tile_count, L->ntiles)) || (rc = alloc(g->tile_off, L->ntiles)) ||
+        (rc = alloc(g->tile_fill, L->ntiles)) || (rc = alloc(g->tile_list, g->list_cap)) ||
+        (rc = alloc(g->active, L->ntiles)) || (rc = alloc(g->n_active, 4)) ||
+        (rc = alloc(g->scan_blocks, 2 * (g->nscan_blocks + 1))))
+        return rc;
+    BL_HIP(hipMemset(g->n_active, 0, 4 * sizeof(int)));
+    BL_HIP(hipMemset(g->tile_count, 0, L->ntiles * sizeof(int)));
+    const size_t usz = L->storage == BL_STORE_F64 ? sizeof(double) : sizeof(float);
+    hipFree(g->ubuf);
+    BL_HIP(hipMalloc(&g->ubuf, 3 * L->ncell * usz));
+    // voxelization
+    const long ncol = (long)L->nx * L->ny;
+    g->col_cap = std::max<long>(64 * g->NT, 1024);
+    if ((rc = alloc(g->col_count, ncol)) || (rc = alloc(g->col_off, ncol)) || (rc = alloc(g->col_fill, ncol)) ||
+        (rc = alloc(g->col_z, g->col_cap)) || (rc = alloc(g->n_hits, 1)) || (rc = alloc(g->phase, L->ncell)))
+        return rc;
+    BL_HIP(hipMemset(g->phase, 0, L->ncell));
+    for (auto& e : g->ev) BL_HIP(hipEventCreate(&e));
+    g->inited = true;
+    // update() of every particle and the centre-of-mass mapping of add_particle
+    rc = update_particles(g, 1);
+    if (rc) return rc;
+    BL_HIP(hipStreamSynchronize(L->stream));
+    return BL_OK;
+}
+
+int bl_ibm_interpolate(bl_ibm g)
+{
+    int rc = need_init(g);
+    return rc ? rc : do_interpolate(g);
+}
+
+int bl_ibm_compute_forces(bl_ibm g)
+{
+    int rc = need_init(g);
+    return rc ? rc : do_forces(g);
+}
+
+int bl_ibm_spread(bl_ibm g)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if ((rc = do_spread(g))) return rc;
+    return check_overflow(g);
+}
+
+int bl_ibm_move(bl_ibm g)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if ((rc = do_move(g))) return rc;
+    return do_voxelize(g);
+}
+
+int bl_ibm_step(bl_ibm g)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    g->timing = true;
+    rc = step_body(g);
+    g->timing = false;
+    if (rc) return rc;
+    BL_HIP(hipEventSynchronize(g->ev[5]));
+    for (int k = 0; k < 5; ++k) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, g->ev[k], g->ev[k + 1]);
+        g->stage_ms[k] = ms;
+    }
+    return check_overflow(g);
+}
+
+int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    bl_lattice L = g->lat;
+    if (n_steps < 0) return fail(BL_ERR_ARG, "bl_coupled_steps: negative step count");
+    int done = 0;
+    if (use_graph && n_steps >= 2) {
+        // Two consecutive steps (the two AA parities) captured once and
+        // replayed; the captured pair depends on the lattice's step parity.
+        const long long par = L->steps & 1;
+        if (!g->graph2 || g->graph_parity != par) {
+            if (g->graph2) {
+                hipGraphExecDestroy(g->graph2);
+                g->graph2 = nullptr;
+            }
+            hipGraph_t graph;
+            BL_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+            const long long s0 = L->steps;
+            for (int k = 0; k < 2 && rc == BL_OK; ++k) {
+                rc = step_body(g);
+                if (!rc) rc = collide_stream_launch(L);
+                if (!rc) rc = velocity_walls_launch(L);
+            }
+            hipError_t e = hipStreamEndCapture(L->stream, &graph);
+            L->steps = s0;  // capture did not execute anything
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&g->graph2, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            g->graph_parity = par;
+        }
+        for (; done + 2 <= n_steps; done += 2) {
+            BL_HIP(hipGraphLaunch(g->graph2, L->stream));
+            L->steps += 2;
+        }
+    }
+    for (; done < n_steps; ++done) {
+        if ((rc = step_body(g))) return rc;
+        if ((rc = collide_stream_launch(L))) return rc;
+        if ((rc = velocity_walls_launch(L))) return rc;
+    }
+    return check_overflow(g);
+}
+
+int bl_ibm_count(bl_ibm g, int* particles, int* vertices)
+{
+    if (!g) return fail(BL_ERR_ARG, "null ibm handle");
+    if (particles) *particles = (int)g->parts.size();
+    long nv = 0;
+    for (const auto& p : g->parts) nv += g->shapes[p.shape].nv;
+    if (vertices) *vertices = (int)nv;
+    return BL_OK;
+}
+
+int bl_particle_get(bl_ibm g, int pid, int what, double* out)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!out || what < 0 || what > 2 || pid < -1 || pid >= g->np) return fail(BL_ERR_ARG, "bl_particle_get: bad argument");
+    std::vector<long> voff{0};
+    for (const auto& p : g->parts) voff.push_back(voff.back() + g->shapes[p.shape].nv);
+    const long v0 = pid < 0 ? 0 : voff[pid], v1 = pid < 0 ? voff.back() : voff[pid + 1];
+    const double* src = what == 0 ? g->pos : (what == 1 ? g->vel : g->force);
+    BL_HIP(hipMemcpyAsync(out, src + 3 * v0, sizeof(double) * 3 * (v1 - v0), hipMemcpyDeviceToHost, g->lat->stream));
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    return BL_OK;
+}
+
+int bl_particle_set_positions(bl_ibm g, int pid, const double* pos)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!pos || pid < -1 || pid >= g->np) return fail(BL_ERR_ARG, "bl_particle_set_positions: bad argument");
+    std::vector<long> voff{0};
+    for (const auto& p : g->parts) voff.push_back(voff.back() + g->shapes[p.shape].nv);
+    const long v0 = pid < 0 ? 0 : voff[pid], v1 = pid < 0 ? voff.back() : voff[pid + 1];
+    BL_HIP(hipMemcpyAsync(g->pos + 3 * v0, pos, sizeof(double) * 3 * (v1 - v0), hipMemcpyHostToDevice, g->lat->stream));
+    rc = update_particles(g, 0);
+    if (rc) return rc;
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    return BL_OK;
+}
+
+int bl_particle_observables(bl_ibm g, int pid, double out[14])
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!out || pid < 0 || pid >= g->np) return fail(BL_ERR_ARG, "bl_particle_observables: bad argument");
+    double geom[11];
+    BL_HIP(hipMemcpyAsync(geom, g->p_geom + 11 * pid, sizeof(geom), hipMemcpyDeviceToHost, g->lat->stream));
+    const int nv = g->shapes[g->parts[pid].shape].nv;
+    std::vector<double> pos(3 * (size_t)nv);
+    rc = bl_particle_get(g, pid, 0, pos.data());
+    if (rc) return rc;
+    // Moment of inertia of the enclosed volume about its centroid
+    // (src/fsi/DeformableParticle3D.hh:173-272), FP64 on the host.
+    const ShapeHost& s = g->shapes[g->parts[pid].shape];
+    double in[10] = {0};
+    for (int t = 0; t < s.nt; ++t) {
+        const double* v[3] = {&pos[3 * s.tri[3 * t]], &pos[3 * s.tri[3 * t + 1]], &pos[3 * s.tri[3 * t + 2]]};
+        const double a[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+        const double b[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+        const double N[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        double f1[3], f2[3], f3[3], g0[3], g1[3], g2[3];
+        for (int d = 0; d < 3; ++d) {
+            const double tmp0 = v[0][d] + v[1][d];
+            f1[d] = tmp0 + v[2][d];
+            const double tmp1 = v[0][d] * v[0][d];
+            const double tmp2 = tmp1 + v[1][d] * tmp0;
+            f2[d] = tmp2 + v[2][d] * f1[d];
+            f3[d] = v[0][d] * tmp1 + v[1][d] * tmp2 + v[2][d] * f2[d];
+            g0[d] = f2[d] + v[0][d] * (f1[d] + v[0][d]);
+            g1[d] = f2[d] + v[1][d] * (f1[d] + v[1][d]);
+            g2[d] = f2[d] + v[2][d] * (f1[d] + v[2][d]);
+        }
+        in[0] += N[0] * f1[0];
+        in[1] += N[0] * f2[0];
+        in[2] += N[1] * f2[1];
+        in[3] += N[2] * f2[2];
+        in[4] += N[0] * f3[0];
+        in[5] += N[1] * f3[1];
+        in[6] += N[2] * f3[2];
+        in[7] += N[0] * (v[0][1] * g0[0] + v[1][1] * g1[0] + v[2][1] * g2[0]);
+        in[8] += N[1] * (v[0][2] * g0[1] + v[1][2] * g1[1] + v[2][2] * g2[1]);
+        in[9] += N[2] * (v[0][0] * g0[2] + v[1][0] * g1[2] + v[2][0] * g2[2]);
+    }
+    in[0] *= 1.0 / 6.0;
+    for (int k = 1; k <= 3; ++k) in[k] *= 1.0 / 24.0;
+    for (int k = 4; k <= 6; ++k) in[k] *= 1.0 / 60.0;
+    for (int k = 7; k <= 9; ++k) in[k] *= 1.0 / 120.0;
+    const double V = in[0];
+    const double c[3] = {in[1] / V, in[2] / V, in[3] / V};
+    double* I = out + 5;
+    I[0] = in[5] + in[6] - V * (c[1] * c[1] + c[2] * c[2]);
+    I[1] = -in[7] + V * c[0] * c[1];
+    I[2] = -in[9] + V * c[2] * c[0];
+    I[3] = I[1];
+    I[4] = in[4] + in[6] - V * (c[2] * c[2] + c[0] * c[0]);
+    I[5] = -in[8] + V * c[1] * c[2];
+    I[6] = I[2];
+    I[7] = I[5];
+    I[8] = in[4] + in[5] - V * (c[0] * c[0] + c[1] * c[1]);
+    for (int k = 0; k < 5; ++k) out[k] = geom[k];
+    return BL_OK;
+}
+
+int bl_ibm_get_phase(bl_ibm g, unsigned char* phase)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!phase) return fail(BL_ERR_ARG, "bl_ibm_get_phase: null argument");
+    BL_HIP(hipMemcpyAsync(phase, g->phase, g->lat->ncell, hipMemcpyDeviceToHost, g->lat->stream));
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    return BL_OK;
+}
+
+int bl_ibm_stage_times(bl_ibm g, double out[5])
+{
+    if (!g || !out) return fail(BL_ERR_ARG, "bl_ibm_stage_times: null argument");
+    for (int k = 0; k < 5; ++k) out[k] = g->stage_ms[k];
+    return BL_OK;
+}
+
+}  // extern "C"

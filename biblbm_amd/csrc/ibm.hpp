@@ -1,0 +1,102 @@
+// Immersed-boundary state: shapes (rest geometry), particles, vertex arrays,
+// tile binning of the delta-kernel stencils, voxelization buffers.
+#pragma once
+
+#include <vector>
+
+#include "lattice.hpp"
+#include "mesh.hpp"
+
+struct bl_ibm_s {
+    bl_lattice lat = nullptr;
+    int kernel = BL_KERNEL_ROMA;
+    int S = 3;        // stencil width
+    double h = 1.5;   // half support
+
+    std::vector<bl::ShapeHost> shapes;
+    struct PartHost {
+        int shape = 0, model = 0, bug = 1;
+        double params[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        std::vector<double> pos;
+    };
+    std::vector<PartHost> parts;
+    bool inited = false;
+
+    int np = 0;
+    long NV = 0, NT = 0, NL = 0, NP = 0;  // totals over particles
+    int nshape = 0;
+
+    // ---- shape data (device, concatenated over shapes) ----
+    int* s_voff = nullptr;  // nshape+1 (rest vertices, CSR heads)
+    int* s_toff = nullptr;  // nshape+1
+    int* s_loff = nullptr;
+    int* s_poff = nullptr;
+    int* s_vtoff = nullptr;  // nshape+1 offsets into CSR entry arrays
+    int* s_vloff = nullptr;
+    int* s_vpoff = nullptr;
+    int* s_tri = nullptr;        // 3 per tri (local ids)
+    double* s_tri_area0 = nullptr;
+    double* s_tri_geo0 = nullptr;  // 4 per tri
+    double* s_rest = nullptr;      // 3 per vertex
+    int* s_link = nullptr;         // 2 per link
+    double* s_link_len = nullptr;
+    int* s_pair = nullptr;         // 4 per pair
+    double* s_pair_cs0 = nullptr;  // 2 per pair
+    int* s_vt_head = nullptr;      // per shape vertex: nv+1 heads (relative)
+    int* s_vt = nullptr;
+    int* s_vl_head = nullptr;
+    int* s_vl = nullptr;
+    int* s_vp_head = nullptr;
+    int* s_vp = nullptr;
+    double* s_area0 = nullptr;  // per shape total rest area
+
+    // ---- particles ----
+    int* p_shape = nullptr;
+    int* p_model = nullptr;
+    int* p_bug = nullptr;
+    double* p_params = nullptr;  // 8 per particle
+    long* p_voff = nullptr;      // np+1
+    long* p_toff = nullptr;
+    long* p_loff = nullptr;
+    long* p_poff = nullptr;
+    double* p_geom = nullptr;    // 11 per particle: area, volume, com(3), bbox(6)
+    double* p_delta = nullptr;   // 3 per particle (COM remap shift)
+
+    // ---- vertices ----
+    double* pos = nullptr;
+    double* vel = nullptr;
+    double* force = nullptr;
+    double* c_tri = nullptr;   // 9 per triangle (3 vertices x 3)
+    double* c_link = nullptr;  // 6 per link
+    double* c_pair = nullptr;  // 12 per pair
+
+    // ---- stencil / tiles ----
+    int* v_base = nullptr;    // 3 per vertex
+    double* v_w = nullptr;    // 12 per vertex (3 axes x up to 4 weights)
+    int* tile_count = nullptr;
+    int* tile_off = nullptr;
+    int* tile_fill = nullptr;
+    int* tile_list = nullptr;  // capacity: list_cap
+    long list_cap = 0;
+    int* active = nullptr;     // active tile ids
+    int* n_active = nullptr;   // device scalar [0] = active tiles, [1] = total pairs, [2] = overflow
+    int* scan_blocks = nullptr;
+    long nscan_blocks = 0;
+    void* ubuf = nullptr;      // 3 per cell, lattice compute precision
+
+    // ---- voxelization ----
+    int* col_count = nullptr;
+    int* col_off = nullptr;
+    int* col_fill = nullptr;
+    double* col_z = nullptr;
+    long col_cap = 0;
+    int* n_hits = nullptr;
+    unsigned char* phase = nullptr;
+
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double stage_ms[5] = {0, 0, 0, 0, 0};
+    bool timing = false;
+
+    hipGraphExec_t graph2 = nullptr;  // captured pair of coupled steps
+    long long graph_parity = -1;
+};

@@ -550,3 +550,62 @@ long or_ray_hits_z(long nv, long nt, const int *tri, const double *pos, long cap
     }
     return n;
 }
+
+/* Column fill of the phase field from z-ray hits, as apply_voxelization
+ * does (src/fsi/ImmersedBoundaryDynamics_voxelization.hh:121-165): hits are
+ * mapped into [-0.5, nz-0.5) on a periodic z axis (src/fsi/ZBuffer.h:39-49),
+ * columns wrap on periodic x / y (src/fsi/ZBuffer.h:211-213), each column's
+ * hits are sorted and the nodes between consecutive hits alternate
+ * outside / inside, a later segment overwriting a shared end node. */
+static int cmp_double(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+void or_voxel_fill(int nx, int ny, int nz, const int periodic[3], long nh,
+                   const double *hits, unsigned char *phase)
+{
+    long ncol = (long)nx * ny;
+    long *cnt = (long *)calloc(ncol + 1, sizeof(long));
+    double *zs = (double *)malloc(sizeof(double) * (nh > 0 ? nh : 1));
+    long *col_of = (long *)malloc(sizeof(long) * (nh > 0 ? nh : 1));
+    for (long k = 0; k < nh; ++k) {
+        long i = (long)hits[3 * k], j = (long)hits[3 * k + 1];
+        if (periodic[0]) i = ((i % nx) + nx) % nx;
+        if (periodic[1]) j = ((j % ny) + ny) % ny;
+        if (i < 0 || i >= nx || j < 0 || j >= ny) { col_of[k] = -1; continue; }
+        col_of[k] = i * ny + j;
+        cnt[col_of[k] + 1]++;
+    }
+    for (long c = 0; c < ncol; ++c) cnt[c + 1] += cnt[c];
+    long *fill = (long *)calloc(ncol, sizeof(long));
+    for (long k = 0; k < nh; ++k) {
+        if (col_of[k] < 0) continue;
+        double z = hits[3 * k + 2];
+        if (periodic[2]) z = z - (double)nz * floor((z + 0.5) * (1.0 / nz));
+        zs[cnt[col_of[k]] + fill[col_of[k]]++] = z;
+    }
+    for (long c = 0; c < ncol; ++c) {
+        long n = cnt[c + 1] - cnt[c];
+        double *z = zs + cnt[c];
+        qsort(z, n, sizeof(double), cmp_double);
+        unsigned char *ph = phase + c * nz;
+        int inside = 0;
+        double z_last = -10000;
+        for (long k = 0; k < n; ++k) {
+            long lo = (long)ceil(z_last), hi = (long)floor(z[k]);
+            if (lo < 0) lo = 0;
+            if (hi > nz - 1) hi = nz - 1;
+            for (long q = lo; q <= hi; ++q) ph[q] = inside ? 1 : 0;
+            z_last = z[k];
+            inside = !inside;
+        }
+        if (ceil(z_last) < nz - 1) {
+            long lo = (long)ceil(z_last);
+            if (lo < 0) lo = 0;
+            for (long q = lo; q <= nz - 1; ++q) ph[q] = inside ? 1 : 0;
+        }
+    }
+    free(cnt); free(zs); free(col_of); free(fill);
+}

@@ -124,6 +124,10 @@ void or_moment_of_inertia(long nv, long nt, const int *tri, const double *pos,
 long or_ray_hits_z(long nv, long nt, const int *tri, const double *pos,
                    long cap, double *hits);
 
+/* Phase field (ncell bytes) from z-ray hits: column sort + parity fill. */
+void or_voxel_fill(int nx, int ny, int nz, const int periodic[3], long nh,
+                   const double *hits, unsigned char *phase);
+
 #ifdef __cplusplus
 }
 #endif

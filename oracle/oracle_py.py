@@ -34,6 +34,7 @@ _SIGS = {
     "or_particle_update": (None, [_L, _L, _P, _P, _P, _P]),
     "or_moment_of_inertia": (None, [_L, _L, _P, _P, _P]),
     "or_ray_hits_z": (_L, [_L, _L, _P, _P, _L, _P]),
+    "or_voxel_fill": (None, [_I, _I, _I, _P, _L, _P, _P]),
 }
 
 
@@ -224,3 +225,83 @@ def ray_hits_z(tri, pos, cap=1 << 20):
     hits = np.zeros((cap, 3))
     n = lib.or_ray_hits_z(len(pos), len(tri), _p(tri), _p(pos), cap, _p(hits))
     return hits[:min(n, cap)].copy()
+
+
+def voxel_fill(shape, periodic, hits):
+    hits = _c(hits, np.float64).reshape(-1, 3)
+    phase = np.zeros(tuple(shape), np.uint8)
+    lib.or_voxel_fill(shape[0], shape[1], shape[2], _p(_c(periodic, np.int32)), len(hits), _p(hits),
+                      _p(phase))
+    return phase
+
+
+# ------------------------------------------------------- composed steps
+class Particle:
+    """Oracle-side particle: rest mesh, model, parameters, vertex state."""
+
+    def __init__(self, rest, tri, model, params, pos, bug_compatible=True):
+        self.rest = _c(rest, np.float64)
+        self.tri = _c(tri, np.int32)
+        self.model = model
+        self.params = np.asarray(params, float)
+        self.pos = _c(pos, np.float64).copy()
+        self.bug = bug_compatible
+        self.topo = build_topology(self.tri, self.rest)
+        a, b, c = self.rest[self.tri[:, 0]], self.rest[self.tri[:, 1]], self.rest[self.tri[:, 2]]
+        self.tri_area0 = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+        self.area0 = self.tri_area0.sum()
+        g = particle_update(self.tri, self.pos)
+        self.area, self.volume, self.com = g["area"], g["volume"], g["com"]
+
+    def forces(self):
+        m, P = self.model, self.params
+        F = np.zeros_like(self.pos)
+        if m in (0, 1, 3):
+            F += capsule_forces(self.tri, self.rest, self.pos, P[0], P[1],
+                                "neohookean" if m == 1 else "skalak", self.bug)
+        if m == 2:
+            F += rbc_forces(self.tri, self.tri_area0, self.topo, self.area0, P, self.area, self.volume,
+                            self.pos)
+        if m == 3:
+            # area / volume / bending terms of the RBC model without its links
+            topo = dict(self.topo)
+            topo["link_i"] = np.zeros((0, 2), np.int32)
+            topo["link_len"] = np.zeros(0)
+            rp = np.array([1.0, 1.0, P[2], P[3], P[4], P[5], P[6], 0.0])
+            F += rbc_forces(self.tri, self.tri_area0, topo, self.area0, rp, self.area, self.volume,
+                            self.pos)
+        return F
+
+    def move(self, vel, shape, periodic):
+        self.pos += vel
+        g = particle_update(self.tri, self.pos)
+        self.area, self.volume = g["area"], g["volume"]
+        com = g["com"].copy()
+        for d in range(3):
+            if periodic[d]:
+                L = float(shape[d])
+                r = com[d] - L * np.floor((com[d] + 0.5) * (1.0 / L))
+                self.pos[:, d] += r - com[d]
+                com[d] = r
+        self.com = com
+
+
+def coupled_step(f, force, phase, mask, periodic, omega, omega_in, body, particles, kernel,
+                 wall_axis=1, wall_u=None):
+    """One time step of the progs' loop, composed from the oracle's pieces:
+    computeVelocity (u = j/rho + F/2 with the last collision's force), force
+    reset to the body force, interpolation, membrane forces, spreading,
+    vertex advection (+ COM remap), then collide-and-stream with the new
+    force.  Returns (f, force, per-particle velocities)."""
+    shape = f.shape[:3]
+    _, u = lbm_moments(f, force, mask, wall_axis, wall_u)
+    new_force = np.broadcast_to(np.asarray(body, float), tuple(shape) + (3,)).copy()
+    vels = []
+    for p in particles:
+        vels.append(interpolate(kernel, u, periodic, p.pos))
+    for p in particles:
+        new_force = spread(kernel, shape, periodic, p.pos, p.forces(), base=new_force)
+    for p, v in zip(particles, vels):
+        p.move(v, shape, periodic)
+    f = lbm_run(f, new_force, phase, mask, periodic, omega, omega_in, 1, wall_axis, wall_u)
+    return f, new_force, vels

@@ -1,0 +1,283 @@
+"""Immersed-boundary stages on the GPU (through the C-ABI) against the
+reference's golden force vectors and the FP64 oracle.
+
+Bar (FP64 storage): membrane forces within 1e-12 (relative to the largest
+force) of the reference's own compute_forces output; interpolated velocities,
+spread force field and advected vertices within 1e-12 of the oracle; coupled
+trajectories within relative L2 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (40, 36, 44)
+PERIODIC = (1, 1, 1)
+
+
+def golden_mesh(name):
+    g = lambda n: load_golden("fsi", name, n + ".npy")
+    return dict(rest=g("rest_pos"), tri=g("tri"), pos=g("pos"), rbc_params=g("rbc_params"),
+                neo=g("force_neohookean"), skalak=g("force_skalak"), rbc=g("force_rbc"),
+                geometry=g("geometry"), inertia=g("inertia"), hits=g("ray_hits_z"))
+
+
+def random_flow(shape, seed=3, amp=0.02):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.99, 1.01, shape)
+    u = rng.uniform(-amp, amp, shape + (3,))
+    f0 = O.lbm_init_equilibrium(rho, u)
+    F = rng.uniform(-1e-4, 1e-4, shape + (3,))
+    return f0, F
+
+
+def make(kernel="roma", storage="f64", shape=SHAPE, periodic=PERIODIC, omega=1 / 0.9, omega_in=None,
+         seed=3):
+    from biblbm_amd.lattice import Lattice
+    from biblbm_amd.ibm import ImmersedBoundary
+    L = Lattice(*shape, periodic=periodic, storage=storage, omega=omega, omega_in=omega_in)
+    f0, F = random_flow(shape, seed)
+    L.put_populations(f0)
+    L.set_force_field(F)
+    ib = ImmersedBoundary(L, kernel)
+    return L, ib, f0, F
+
+
+MODELS = {"neo": (1, lambda m: [0.02, 10.0]), "skalak": (0, lambda m: [0.02, 10.0]),
+          "rbc": (2, lambda m: list(m["rbc_params"]))}
+
+
+@pytest.mark.parametrize("mesh", ["sphere626", "ellipsoid414"])
+@pytest.mark.parametrize("model", ["neo", "skalak", "rbc"])
+def test_forces_match_reference(device, mesh, model):
+    m = golden_mesh(mesh)
+    L, ib, _, _ = make()
+    sid = ib.add_shape(m["rest"], m["tri"])
+    code, params = MODELS[model]
+    ib.add_particle(sid, code, params(m), m["pos"])
+    ib.init()
+    ib.compute_forces()
+    F = ib.forces()
+    ref = m[model]
+    err = np.abs(F - ref).max() / np.abs(ref).max()
+    print(mesh, model, "force rel err vs reference", err)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("mesh", ["sphere626", "ellipsoid414"])
+def test_shape_and_geometry_match_reference(device, mesh):
+    m = golden_mesh(mesh)
+    L, ib, _, _ = make()
+    sid = ib.add_shape(m["rest"], m["tri"])
+    info = ib.shape_info(sid)
+    sc = load_golden("fsi", mesh, "shape_scalars.npy")
+    assert abs(info["area"] - sc[0]) <= 1e-12 * sc[0]
+    assert abs(info["volume"] - sc[1]) <= 1e-12 * sc[1]
+    assert np.allclose(info["center"], sc[2:5], atol=1e-12)
+    assert abs(info["radius"] - sc[5]) <= 1e-13
+    ib.add_particle(sid, 2, m["rbc_params"], m["pos"])
+    ib.init()
+    obs = ib.observables(0)
+    g = m["geometry"]
+    assert abs(obs["area"] - g[0]) <= 1e-12 * g[0]
+    assert abs(obs["volume"] - g[1]) <= 1e-12 * g[1]
+    assert np.allclose(obs["com"], g[2:5], rtol=0, atol=1e-12)
+    assert np.allclose(obs["inertia"], m["inertia"], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("kernel", ["roma", "peskin", "tophat"])
+def test_interpolation_matches_oracle(device, kernel):
+    m = golden_mesh("sphere626")
+    L, ib, f0, F = make(kernel)
+    sid = ib.add_shape(m["rest"], m["tri"])
+    ib.add_particle(sid, 1, [0.02, 1.0], m["pos"])
+    ib.init()
+    ib.interpolate()
+    vel = ib.velocities()
+    _, u = O.lbm_moments(f0, F)
+    ref = O.interpolate(kernel, u, PERIODIC, m["pos"])
+    assert np.abs(vel - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("kernel", ["roma", "peskin"])
+def test_spreading_matches_oracle(device, kernel):
+    m = golden_mesh("ellipsoid414")
+    L, ib, f0, F = make(kernel)
+    g = np.array([1e-6, -2e-6, 0.5e-6])
+    L.set_body_force(g)
+    sid = ib.add_shape(m["rest"], m["tri"])
+    ib.add_particle(sid, 0, [0.02, 10.0], m["pos"])
+    ib.init()
+    ib.interpolate()
+    ib.compute_forces()
+    ib.spread()
+    got = L.force()
+    base = np.broadcast_to(g, SHAPE + (3,))
+    ref = O.spread(kernel, SHAPE, PERIODIC, m["pos"], m["skalak"], base=base)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref - base).max()
+    # total spread force equals the sum of vertex forces (partition of unity)
+    assert np.allclose((got - base).reshape(-1, 3).sum(0), m["skalak"].sum(0), atol=1e-13)
+
+
+def test_move_and_periodic_com_remap(device):
+    m = golden_mesh("sphere626")
+    L, ib, f0, F = make()
+    sid = ib.add_shape(m["rest"], m["tri"])
+    # start near the x = 0 face so that the advection crosses it
+    pos = m["pos"] - [20.3 + 0.45, 0, 0]
+    ib.add_particle(sid, 1, [0.02, 1.0], pos)
+    ib.init()
+    p0 = ib.positions()
+    ib.interpolate()
+    vel = ib.velocities()
+    ib.move()
+    p1 = ib.positions()
+    cand = p0 + vel
+    g = O.particle_update(m["tri"], cand)
+    com = g["com"].copy()
+    shift = np.zeros(3)
+    for d in range(3):
+        Ld = float(SHAPE[d])
+        r = com[d] - Ld * np.floor((com[d] + 0.5) / Ld)
+        shift[d] = r - com[d]
+    assert np.abs(p1 - (cand + shift)).max() <= 1e-12
+    obs = ib.observables(0)
+    assert abs(obs["area"] - g["area"]) <= 1e-12 * g["area"]
+    assert abs(obs["volume"] - g["volume"]) <= 1e-12 * abs(g["volume"])
+
+
+def _two_particles():
+    a = golden_mesh("sphere626")
+    b = golden_mesh("ellipsoid414")
+    # ellipsoid414 is 24 lu long in x; place both inside the box apart
+    pa = a["pos"] - [0, 0, 12.0]
+    pb = b["pos"] + [0, 0, 10.0] - [0, 0, 0]
+    return a, b, pa, pb
+
+
+def _build_pair(kernel, storage="f64", shape=SHAPE, periodic=PERIODIC, mask=None, body=(2e-6, 0, 0),
+                omega_in=None):
+    a, b, pa, pb = _two_particles()
+    L, ib, f0, F = make(kernel, storage, shape, periodic, omega_in=omega_in)
+    if mask is not None:
+        L.set_mask(mask)
+    L.set_body_force(body)
+    sa = ib.add_shape(a["rest"], a["tri"])
+    sb = ib.add_shape(b["rest"], b["tri"])
+    ib.add_particle(sa, 2, a["rbc_params"], pa)
+    ib.add_particle(sb, 0, [0.02, 10.0], pb)
+    ib.init()
+    parts = [O.Particle(a["rest"], a["tri"], 2, a["rbc_params"], pa),
+             O.Particle(b["rest"], b["tri"], 0, [0.02, 10.0], pb)]
+    return L, ib, f0, F, parts
+
+
+@pytest.mark.parametrize("kernel", ["roma", "peskin"])
+def test_coupled_steps_match_oracle(device, kernel):
+    """Five full time steps (interpolate, forces, spread, move, collide and
+    stream) of an RBC-model particle and a Skalak capsule of different mesh
+    sizes, against the same steps composed from the oracle."""
+    L, ib, f0, F, parts = _build_pair(kernel)
+    steps = 5
+    ib.coupled_steps(steps, use_graph=False)
+    f, force = f0, F
+    mask = np.zeros(SHAPE, np.uint8)
+    for _ in range(steps):
+        f, force, _ = O.coupled_step(f, force, np.zeros(SHAPE), mask, PERIODIC, 1 / 0.9, 1 / 0.9,
+                                     (2e-6, 0, 0), parts, kernel)
+    pos = ib.positions()
+    ref = np.concatenate([p.pos for p in parts])
+    disp = ref - np.concatenate([p.pos for p in _build_pair_ref_start()])
+    err = np.linalg.norm(pos - ref) / np.linalg.norm(ref)
+    err_disp = np.linalg.norm(pos - ref) / np.linalg.norm(disp)
+    print(kernel, "position rel L2", err, "relative to displacement", err_disp)
+    assert err <= 1e-10
+    assert err_disp <= 1e-8
+    _, u = L.moments()
+    _, uref = O.lbm_moments(f, force)
+    assert np.linalg.norm(u - uref) <= 1e-10 * np.linalg.norm(uref)
+
+
+def _build_pair_ref_start():
+    a, b, pa, pb = _two_particles()
+    return [O.Particle(a["rest"], a["tri"], 2, a["rbc_params"], pa),
+            O.Particle(b["rest"], b["tri"], 0, [0.02, 10.0], pb)]
+
+
+def test_graph_replay_equals_eager(device):
+    L1, ib1, *_ = _build_pair("roma")
+    L2, ib2, *_ = _build_pair("roma")
+    ib1.coupled_steps(6, use_graph=False)
+    ib2.coupled_steps(6, use_graph=True)
+    assert np.array_equal(ib1.positions(), ib2.positions())
+    assert np.array_equal(L1.get_populations(), L2.get_populations())
+    # more replays of the same captured graph, with eager work in between
+    for _ in range(3):
+        ib1.coupled_steps(4, use_graph=False)
+        ib2.coupled_steps(4, use_graph=True)
+        L2.moments()
+        assert np.array_equal(ib1.positions(), ib2.positions())
+
+
+def test_deterministic(device):
+    L1, ib1, *_ = _build_pair("peskin")
+    L2, ib2, *_ = _build_pair("peskin")
+    ib1.coupled_steps(7, use_graph=True)
+    ib2.coupled_steps(7, use_graph=True)
+    assert np.array_equal(ib1.positions(), ib2.positions())
+    assert np.array_equal(L1.force(), L2.force())
+
+
+def test_walls_and_body_force_coupled(device):
+    """Bounce-back planes at y = 0, ny-1 (non-periodic y), body force."""
+    mask = np.zeros(SHAPE, np.uint8)
+    mask[:, 0, :] = mask[:, -1, :] = 1
+    L, ib, f0, F, parts = _build_pair("roma", periodic=(1, 0, 1), mask=mask)
+    steps = 3
+    ib.coupled_steps(steps, use_graph=False)
+    f, force = f0, F
+    for _ in range(steps):
+        f, force, _ = O.coupled_step(f, force, np.zeros(SHAPE), mask, (1, 0, 1), 1 / 0.9, 1 / 0.9,
+                                     (2e-6, 0, 0), parts, "roma")
+    ref = np.concatenate([p.pos for p in parts])
+    assert np.linalg.norm(ib.positions() - ref) <= 1e-10 * np.linalg.norm(ref)
+
+
+def test_voxelization_matches_oracle_and_sphere(device):
+    from biblbm_amd import meshes
+    v, t = meshes.icosphere(3, 8.0)
+    c = np.array([20.3, 18.6, 21.2])
+    L, ib, f0, F = make(omega=1 / 0.8, omega_in=1 / 2.0)
+    sid = ib.add_shape(v, t)
+    ib.add_particle(sid, 1, [1e-3, 1.0], v + c)
+    ib.init()
+    ib.step()
+    pos = ib.positions()
+    ph = ib.phase()
+    ref = O.voxel_fill(SHAPE, PERIODIC, O.ray_hits_z(t, pos))
+    assert np.array_equal(ph, ref)
+    com = ib.observables(0)["com"]
+    X, Y, Z = np.meshgrid(*[np.arange(n) for n in SHAPE], indexing="ij")
+    r = np.sqrt((X - com[0]) ** 2 + (Y - com[1]) ** 2 + (Z - com[2]) ** 2)
+    far = np.abs(r - 8.0) > 0.3
+    assert np.array_equal(ph[far], (r[far] < 8.0).astype(np.uint8))
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16s", 3e-3)])
+def test_reduced_storage_coupled_error(device, storage, tol):
+    """FP32 / FP16-shifted storage with the FP64 membrane: vertex
+    displacement error against the FP64 run after 8 steps, relative to the
+    displacement itself."""
+    L1, ib1, *_ = _build_pair("roma", "f64")
+    L2, ib2, *_ = _build_pair("roma", storage)
+    p0 = ib1.positions()
+    ib1.coupled_steps(8, use_graph=True)
+    ib2.coupled_steps(8, use_graph=True)
+    d1 = ib1.positions() - p0
+    d2 = ib2.positions() - p0
+    err = np.linalg.norm(d2 - d1) / np.linalg.norm(d1)
+    print(storage, "displacement rel L2 vs FP64", err)
+    assert err <= tol
