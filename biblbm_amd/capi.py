@@ -48,6 +48,8 @@ SIGNATURES = {
     "bl_lattice_set_launch": [_P, _I, _I],
     "bl_lattice_stream": [_P, ctypes.POINTER(_P)],
     "bl_lattice_time_collide": [_P, _I, _PD],
+    "bl_sync": [],
+    "bl_device_copy": [ctypes.c_longlong, _I, _PD],
     "bl_ibm_create": [_P, _I, ctypes.POINTER(_P)],
     "bl_ibm_destroy": [_P],
     "bl_shape_add": [_P, _I, _I, _P, _P, _PI],
@@ -66,6 +68,7 @@ SIGNATURES = {
     "bl_particle_observables": [_P, _I, _P],
     "bl_ibm_get_phase": [_P, _P],
     "bl_ibm_stage_times": [_P, _P],
+    "bl_ibm_tile_stats": [_P, _P],
     "bl_ensemble_create": [_I, _P, _P, _I, ctypes.POINTER(_P)],
     "bl_ensemble_destroy": [_P],
     "bl_ensemble_lattice": [_P, _I, ctypes.POINTER(_P)],

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "ibm.hpp"
+#include "lattice_dev.hpp"
 
 namespace bl {
 
@@ -98,13 +99,22 @@ Grid grid_of(const bl_lattice_s* L)
 }
 
 // ------------------------------------------------------------- binning
-// Distinct tiles touched along one axis by nodes base..base+S-1 (wrapped or
-// clipped); returns the count (<= 4).
-__device__ __forceinline__ int axis_tiles(int base, int S, int n, int per, int out[4])
+// Stencil bases are stored wrapped into [0, n) on periodic axes (raw on the
+// others), so node k of a stencil is base + k minus n at most once.
+__device__ __forceinline__ int node_at(int bw, int k, int n, int per)
+{
+    int v = bw + k;
+    if (per) return v >= n ? v - n : v;
+    return (v < 0 || v >= n) ? -1 : v;
+}
+
+// Distinct tiles touched along one axis by the S nodes of a stencil
+// (at most 3: a partial last tile plus a wrap).
+__device__ __forceinline__ int axis_tiles(int bw, int S, int n, int per, int out[4])
 {
     int m = 0;
     for (int k = 0; k < S; ++k) {
-        const int node = wrap_node(base + k, n, per);
+        const int node = node_at(bw, k, n, per);
         if (node < 0) continue;
         const int t = node >> 2;
         bool seen = false;
@@ -114,45 +124,73 @@ __device__ __forceinline__ int axis_tiles(int base, int S, int n, int per, int o
     return m;
 }
 
-__global__ void k_bin_count(Grid gd, int kernel, int S, double h, long NV, const double* __restrict__ pos,
-                            int* __restrict__ base, double* __restrict__ W, int* __restrict__ count)
+__device__ __forceinline__ int vertex_tiles(const Grid& gd, int S, const int* b, int t[27])
+{
+    int tx[4], ty[4], tz[4];
+    const int mx = axis_tiles(b[0], S, gd.n[0], gd.per[0], tx);
+    const int my = axis_tiles(b[1], S, gd.n[1], gd.per[1], ty);
+    const int mz = axis_tiles(b[2], S, gd.n[2], gd.per[2], tz);
+    int m = 0;
+    for (int i = 0; i < mx; ++i)
+        for (int j = 0; j < my; ++j)
+            for (int k = 0; k < mz; ++k) t[m++] = (tx[i] * gd.nt[1] + ty[j]) * gd.nt[2] + tz[k];
+    return m;
+}
+
+// Stencil base, 1-D weights and the tiles the stencil touches; counts the
+// (tile, vertex) pairs per tile.
+__global__ void __launch_bounds__(256) k_bin_count(Grid gd, int kernel, int S, double h, long NV,
+                                                   const double* __restrict__ pos, int* __restrict__ base,
+                                                   double* __restrict__ W, unsigned char* __restrict__ vnt,
+                                                   int* __restrict__ count)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= NV) return;
     int b[3];
     for (int a = 0; a < 3; ++a) {
         const double x = pos[3 * v + a];
-        b[a] = (int)ceil(x - h);
-        base[3 * v + a] = b[a];
-        for (int k = 0; k < 4; ++k)
-            W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(b[a] + k) - x) : 0.0;
+        const int raw = (int)ceil(x - h);
+        for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(raw + k) - x) : 0.0;
+        int bw = raw;
+        if (gd.per[a]) {
+            bw = raw % gd.n[a];
+            if (bw < 0) bw += gd.n[a];
+        }
+        b[a] = bw;
+        base[3 * v + a] = bw;
     }
-    int tx[4], ty[4], tz[4];
-    const int mx = axis_tiles(b[0], S, gd.n[0], gd.per[0], tx);
-    const int my = axis_tiles(b[1], S, gd.n[1], gd.per[1], ty);
-    const int mz = axis_tiles(b[2], S, gd.n[2], gd.per[2], tz);
-    for (int i = 0; i < mx; ++i)
-        for (int j = 0; j < my; ++j)
-            for (int k = 0; k < mz; ++k) atomicAdd(&count[(tx[i] * gd.nt[1] + ty[j]) * gd.nt[2] + tz[k]], 1);
+    int t[27];
+    const int m = vertex_tiles(gd, S, b, t);
+    vnt[v] = (unsigned char)m;
+    for (int q = 0; q < m; ++q) atomicAdd(&count[t[q]], 1);
 }
 
-__global__ void k_bin_fill(Grid gd, int S, long NV, const int* __restrict__ base, const int* __restrict__ off,
-                           int* __restrict__ fill, int* __restrict__ list, long cap, int* __restrict__ status)
+// Writes each (tile, vertex) pair into its tile's list as v*32 + q, q being
+// the tile's index among the vertex's tiles.  The order within a tile is
+// whatever the atomics give; every consumer sorts or indexes by the entry.
+__global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const int* __restrict__ base,
+                                                  const int* __restrict__ off, int* __restrict__ fill,
+                                                  int* __restrict__ list, long cap, int* __restrict__ status)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= NV) return;
-    int tx[4], ty[4], tz[4];
-    const int mx = axis_tiles(base[3 * v], S, gd.n[0], gd.per[0], tx);
-    const int my = axis_tiles(base[3 * v + 1], S, gd.n[1], gd.per[1], ty);
-    const int mz = axis_tiles(base[3 * v + 2], S, gd.n[2], gd.per[2], tz);
-    for (int i = 0; i < mx; ++i)
-        for (int j = 0; j < my; ++j)
-            for (int k = 0; k < mz; ++k) {
-                const int t = (tx[i] * gd.nt[1] + ty[j]) * gd.nt[2] + tz[k];
-                const long slot = (long)off[t] + atomicAdd(&fill[t], 1);
-                if (slot < cap) list[slot] = (int)v;
-                else atomicOr(&status[2], 1);
-            }
+    int t[27];
+    const int m = vertex_tiles(gd, S, base + 3 * v, t);
+    for (int q = 0; q < m; ++q) {
+        const long slot = (long)off[t[q]] + atomicAdd(&fill[t[q]], 1);
+        if (slot < cap) list[slot] = (int)(v * 32 + q);
+        else atomicOr(&status[2], 1);
+    }
+}
+
+__global__ void k_clear_bins(int* count, int* fill, long n, int* status)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        count[i] = 0;
+        fill[i] = 0;
+    }
+    if (i == 0) status[2] = 0;
 }
 
 // --------------------------------------------------------------- scan
@@ -211,7 +249,7 @@ __global__ void __launch_bounds__(256) k_scan_top(unsigned long long* sums, long
 }
 
 __global__ void __launch_bounds__(256) k_scan_apply(const int* in, long n, const unsigned long long* sums, int* off,
-                                                    int* active)
+                                                    int* active, unsigned char* flags)
 {
     __shared__ unsigned long long sh[256];
     const long b0 = (long)blockIdx.x * kScanChunk;
@@ -236,6 +274,7 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int* in, long n, const
         if (i < n) {
             off[i] = (int)(run & 0xffffffffull);
             if (in[i] > 0) active[run >> 32] = (int)i;
+            flags[i] = in[i] > 0 ? 1 : 0;
         }
         run += loc[k];
     }
@@ -247,47 +286,6 @@ __global__ void k_fill_i32(int* p, long n, int v)
     if (i < n) p[i] = v;
 }
 
-__global__ void k_flags_from_counts(const int* count, long n, unsigned char* flags)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = count[i] > 0 ? 1 : 0;
-}
-
-// -------------------------------------------------------- interpolation
-// GeneralInterpolator (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:44-78):
-// periodic axes wrap node indices, other axes skip nodes outside the box.
-template <class C>
-__global__ void k_interpolate(Grid gd, int S, long NV, const int* __restrict__ base, const double* __restrict__ W,
-                              const C* __restrict__ u, double* __restrict__ vel)
-{
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const int b0 = base[3 * v], b1 = base[3 * v + 1], b2 = base[3 * v + 2];
-    const double* w = W + 12 * v;
-    double acc[3] = {0, 0, 0};
-    for (int i = 0; i < S; ++i) {
-        const int x = wrap_node(b0 + i, gd.n[0], gd.per[0]);
-        if (x < 0) continue;
-        const double wx = w[i];
-        for (int j = 0; j < S; ++j) {
-            const int y = wrap_node(b1 + j, gd.n[1], gd.per[1]);
-            if (y < 0) continue;
-            const double wxy = wx * w[4 + j];
-            for (int k = 0; k < S; ++k) {
-                const int z = wrap_node(b2 + k, gd.n[2], gd.per[2]);
-                if (z < 0) continue;
-                const double wxyz = wxy * w[8 + k];
-                const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
-                acc[0] += wxyz * (double)u[3 * c];
-                acc[1] += wxyz * (double)u[3 * c + 1];
-                acc[2] += wxyz * (double)u[3 * c + 2];
-            }
-        }
-    }
-    vel[3 * v] = acc[0];
-    vel[3 * v + 1] = acc[1];
-    vel[3 * v + 2] = acc[2];
-}
 
 // ------------------------------------------------------------ membrane
 struct ForceView {
@@ -306,14 +304,13 @@ struct ForceView {
     const int *s_vt_head, *s_vt, *s_vl_head, *s_vl, *s_vp_head, *s_vp;
     const double* s_area0;
     const double* pos;
-    double *c_tri, *c_link, *c_pair;
 };
 
 // Skalak / NeoHookean in-plane forces of one triangle
 // (src/fsi/DeformableCapsule.hh:60-150), with the reference's
 // c = l' sin(phi) / (l' sin(phi0)) when `bug` is set.
 __device__ void capsule_triangle(D3 p0, D3 p1, D3 p2, const double* geo0, double area0, double G, double Cc,
-                                 bool skalak, bool bug, D3 out[3])
+                                 bool skalak, bool bug, int only, D3 out[3])
 {
     const D3 e0 = p1 - p0, e1 = p2 - p0;
     const double l = norm(e0), lp = norm(e1);
@@ -349,6 +346,7 @@ __device__ void capsule_triangle(D3 p0, D3 p1, D3 p2, const double* geo0, double
     const double f3b = l * cos_phi / l0p;
     const double f3c = l * lp / l0p;
     for (int i = 0; i < 3; ++i) {
+        if (only >= 0 && i != only) continue;
         const D3 gD11 = f1 * gl[i];
         const D3 gD22 = f2 * (f2a * glp[i] + f2b * gl[i] - f2c * gc[i]);
         const D3 gD12 = f3 * (f3a * gl[i] + f3b * glp[i] + f3c * gc[i]);
@@ -395,46 +393,15 @@ __device__ void area_volume_triangle(D3 v0, D3 v1, D3 v2, double area0, double A
     out[2] = out[2] + df_vol * (cross(v0, v1) / 6.);
 }
 
-__global__ void k_tri_forces(ForceView f, long NT)
+// Contribution of one incident element to one vertex (the element's slot
+// `k` for that vertex).
+__device__ D3 link_contrib(const ForceView& f, int s, long vo, const double* P, int code)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NT) return;
-    const int p = find_part(f.p_toff, f.np, g);
-    const int s = f.p_shape[p];
-    const long lt = g - f.p_toff[p];
-    const long st = f.s_toff[s] + lt;
-    const long vo = f.p_voff[p];
-    const int i0 = f.s_tri[3 * st], i1 = f.s_tri[3 * st + 1], i2 = f.s_tri[3 * st + 2];
-    const D3 v0 = ld3(f.pos + 3 * (vo + i0)), v1 = ld3(f.pos + 3 * (vo + i1)), v2 = ld3(f.pos + 3 * (vo + i2));
-    const int model = f.p_model[p];
-    const double* P = f.p_params + 8 * p;
-    D3 out[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    const double area0 = f.s_tri_area0[st];
-    if (model == BL_MODEL_CAPSULE_SKALAK || model == BL_MODEL_CAPSULE_NEOHOOKEAN || model == 3)
-        capsule_triangle(v0, v1, v2, f.s_tri_geo0 + 4 * st, area0, P[0], P[1], model != BL_MODEL_CAPSULE_NEOHOOKEAN,
-                         f.p_bug[p] != 0, out);
-    if (model == BL_MODEL_RBC || model == 3) {
-        const double A = f.p_geom[11 * p], V = f.p_geom[11 * p + 1];
-        area_volume_triangle(v0, v1, v2, area0, f.s_area0[s], A, V, P[2], P[3], P[4], P[5], out);
-    }
-    for (int k = 0; k < 3; ++k) st3(f.c_tri + 9 * g + 3 * k, out[k]);
-}
-
-// WLC-POW link springs (src/fsi/RBCParticle.hh:27-44,149-171).
-__global__ void k_link_forces(ForceView f, long NL)
-{
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NL) return;
-    const int p = find_part(f.p_loff, f.np, g);
-    if (f.p_model[p] != BL_MODEL_RBC) return;
-    const int s = f.p_shape[p];
-    const long sl = f.s_loff[s] + (g - f.p_loff[p]);
-    const long vo = f.p_voff[p];
-    const double* P = f.p_params + 8 * p;
     const double m = 2;
     const double x0 = 1. / 2.2;
     const double ks_coeff1 = 0.25 * (6. * x0 + (3 * x0 * x0 - 2 * x0 * x0 * x0) / ((1 - x0) * (1 - x0)));
     const double ks_coeff2 = x0 / (2. * (1 - x0) * (1 - x0) * (1 - x0)) + 0.25 * (1. - 1. / ((1 - x0) * (1 - x0)));
+    const long sl = f.s_loff[s] + (code >> 2);
     const double l0 = f.s_link_len[sl];
     const double kp_coeff1 = -1. / pow(l0, m);
     const double kp_coeff2 = (m + 1) / pow(l0, m);
@@ -447,23 +414,30 @@ __global__ void k_link_forces(ForceView f, long NL)
     const double x = L / (l0 * 2.2);
     const double df = -0.25 * ks * (6. * x + (3. * x * x - 2. * x * x * x) / ((1 - x) * (1 - x))) + kp / pow(L, m);
     const D3 fa = (df / L) * dv;
-    st3(f.c_link + 6 * g, fa);
-    st3(f.c_link + 6 * g + 3, {-fa.x, -fa.y, -fa.z});
+    return (code & 3) == 0 ? fa : D3{-fa.x, -fa.y, -fa.z};
 }
 
-// Dihedral bending of one adjacent pair (src/fsi/RBCParticle.hh:214-246,
-// src/fsi/TriangleUtils.h:148-192).
-__global__ void k_pair_forces(ForceView f, long NP)
+__device__ D3 tri_contrib(const ForceView& f, int p, int s, int model, long vo, const double* P, int code)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NP) return;
-    const int p = find_part(f.p_poff, f.np, g);
-    const int model = f.p_model[p];
-    if (model != BL_MODEL_RBC && model != 3) return;
-    const int s = f.p_shape[p];
-    const long sp = f.s_poff[s] + (g - f.p_poff[p]);
-    const long vo = f.p_voff[p];
-    const double kb = f.p_params[8 * p + 6];
+    const int k = code & 3;
+    const long st = f.s_toff[s] + (code >> 2);
+    const int i0 = f.s_tri[3 * st], i1 = f.s_tri[3 * st + 1], i2 = f.s_tri[3 * st + 2];
+    const D3 v0 = ld3(f.pos + 3 * (vo + i0)), v1 = ld3(f.pos + 3 * (vo + i1)), v2 = ld3(f.pos + 3 * (vo + i2));
+    D3 out[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double area0 = f.s_tri_area0[st];
+    if (model == BL_MODEL_CAPSULE_SKALAK || model == BL_MODEL_CAPSULE_NEOHOOKEAN || model == 3)
+        capsule_triangle(v0, v1, v2, f.s_tri_geo0 + 4 * st, area0, P[0], P[1], model != BL_MODEL_CAPSULE_NEOHOOKEAN,
+                         f.p_bug[p] != 0, k, out);
+    if (model == BL_MODEL_RBC || model == 3)
+        area_volume_triangle(v0, v1, v2, area0, f.s_area0[s], f.p_geom[11 * p], f.p_geom[11 * p + 1], P[2], P[3],
+                             P[4], P[5], out);
+    return out[k];
+}
+
+__device__ D3 pair_contrib(const ForceView& f, int s, long vo, const double* P, int code)
+{
+    const int k = code & 3;
+    const long sp = f.s_poff[s] + (code >> 2);
     const int* q = f.s_pair + 4 * sp;
     const D3 v0 = ld3(f.pos + 3 * (vo + q[0])), v1 = ld3(f.pos + 3 * (vo + q[1]));
     const D3 v2 = ld3(f.pos + 3 * (vo + q[2])), v3 = ld3(f.pos + 3 * (vo + q[3]));
@@ -475,73 +449,131 @@ __global__ void k_pair_forces(ForceView f, long NP)
     const D3 n1u = n1 / n1n, n2u = n2 / n2n;
     const double cs = dot(n1u, n2u);
     const double sn = ((dot(n1 - n2, v1 - v3) > 0) ? 1.0 : -1.0) * norm(cross(n1u, n2u));
-    const double pref = kb * (-f.s_pair_cs0[2 * sp] * sn + f.s_pair_cs0[2 * sp + 1] * cs);
-    const D3 g0 = (dot(e1u, e1 - e2) / n2n) * n2u + (dot(e1u, e1 - e0) / n1n) * n1u;
-    const D3 g1 = (-e1n / n1n) * n1u;
-    const D3 g2 = (dot(e1u, e2) / n2n) * n2u + (dot(e1u, e0) / n1n) * n1u;
-    const D3 g3 = (-e1n / n2n) * n2u;
-    st3(f.c_pair + 12 * g, pref * g0);
-    st3(f.c_pair + 12 * g + 3, pref * g1);
-    st3(f.c_pair + 12 * g + 6, pref * g2);
-    st3(f.c_pair + 12 * g + 9, pref * g3);
+    const double pref = P[6] * (-f.s_pair_cs0[2 * sp] * sn + f.s_pair_cs0[2 * sp + 1] * cs);
+    D3 gk;
+    if (k == 0) gk = (dot(e1u, e1 - e2) / n2n) * n2u + (dot(e1u, e1 - e0) / n1n) * n1u;
+    else if (k == 1) gk = (-e1n / n1n) * n1u;
+    else if (k == 2) gk = (dot(e1u, e2) / n2n) * n2u + (dot(e1u, e0) / n1n) * n1u;
+    else gk = (-e1n / n2n) * n2u;
+    return pref * gk;
 }
 
-// Per-vertex sum of its element contributions, in the order the reference
-// accumulates them: links, triangles, pairs (element order within each).
-__global__ void k_gather_forces(ForceView f, long NV, double* __restrict__ force)
+// Force on every vertex: LPV lanes per vertex, lane e evaluates the vertex's
+// e-th incident element in the reference's accumulation order (links
+// src/fsi/RBCParticle.hh:149-171, triangles src/fsi/DeformableCapsule.hh:60-150
+// and src/fsi/RBCParticle.hh:176-209, pairs src/fsi/RBCParticle.hh:214-246),
+// and the lanes' contributions are summed by a fixed butterfly.  No atomics
+// and no per-element buffer.
+template <int LPV>
+__global__ void __launch_bounds__(256) k_vertex_forces(ForceView f, long NV, double* __restrict__ force)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const int p = find_part(f.p_voff, f.np, v);
-    const int s = f.p_shape[p];
-    const int model = f.p_model[p];
-    const long lv = v - f.p_voff[p];
-    const long sv = f.s_voff[s] + lv;
-    double F[3] = {0, 0, 0};
-    if (model == BL_MODEL_RBC) {
-        const int* head = f.s_vl_head + sv + s;  // heads are stored per shape with one extra entry
-        for (int e = head[0]; e < head[1]; ++e) {
-            const int code = f.s_vl[f.s_vloff[s] + e];
-            const double* c = f.c_link + 6 * (f.p_loff[p] + (code >> 2)) + 3 * (code & 3);
-            F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
-        }
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long v = gid / LPV;
+    const int e = (int)(gid % LPV);
+    D3 c = {0, 0, 0};
+    if (v < NV) {
+        const int p = find_part(f.p_voff, f.np, v);
+        const int s = f.p_shape[p];
+        const int model = f.p_model[p];
+        const long vo = f.p_voff[p];
+        const long sv = f.s_voff[s] + (v - vo);
+        const double* P = f.p_params + 8 * p;
+        const int* hl = f.s_vl_head + sv + s;
+        const int* ht = f.s_vt_head + sv + s;
+        const int* hp = f.s_vp_head + sv + s;
+        const int nl = model == BL_MODEL_RBC ? hl[1] - hl[0] : 0;
+        const int nt = ht[1] - ht[0];
+        const int np_ = (model == BL_MODEL_RBC || model == 3) ? hp[1] - hp[0] : 0;
+        if (e < nl) c = link_contrib(f, s, vo, P, f.s_vl[f.s_vloff[s] + hl[0] + e]);
+        else if (e < nl + nt) c = tri_contrib(f, p, s, model, vo, P, f.s_vt[f.s_vtoff[s] + ht[0] + e - nl]);
+        else if (e < nl + nt + np_) c = pair_contrib(f, s, vo, P, f.s_vp[f.s_vpoff[s] + hp[0] + e - nl - nt]);
     }
-    {
-        const int* head = f.s_vt_head + sv + s;
-        for (int e = head[0]; e < head[1]; ++e) {
-            const int code = f.s_vt[f.s_vtoff[s] + e];
-            const double* c = f.c_tri + 9 * (f.p_toff[p] + (code >> 2)) + 3 * (code & 3);
-            F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
-        }
+#pragma unroll
+    for (int o = LPV / 2; o > 0; o >>= 1) {
+        c.x += __shfl_xor(c.x, o);
+        c.y += __shfl_xor(c.y, o);
+        c.z += __shfl_xor(c.z, o);
     }
-    if (model == BL_MODEL_RBC || model == 3) {
-        const int* head = f.s_vp_head + sv + s;
-        for (int e = head[0]; e < head[1]; ++e) {
-            const int code = f.s_vp[f.s_vpoff[s] + e];
-            const double* c = f.c_pair + 12 * (f.p_poff[p] + (code >> 2)) + 3 * (code & 3);
-            F[0] += c[0]; F[1] += c[1]; F[2] += c[2];
-        }
-    }
-    force[3 * v] = F[0];
-    force[3 * v + 1] = F[1];
-    force[3 * v + 2] = F[2];
+    if (e == 0 && v < NV) st3(force + 3 * v, c);
 }
 
-// ------------------------------------------------------------ spreading
-// One 64-thread workgroup per touched tile (4x4x4 cells).  The tile's
-// vertex list is sorted by vertex id in LDS (bitonic), so every cell adds
-// its contributions in the same order as the reference's vertex loop
-// (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85) whatever order the
-// binning wrote them in.
-template <class FT>
-__global__ void __launch_bounds__(64) k_spread(Grid gd, int S, const int* __restrict__ active,
-                                               int* __restrict__ status, const int* __restrict__ count,
-                                               const int* __restrict__ off, const int* __restrict__ list,
-                                               const int* __restrict__ base, const double* __restrict__ W,
-                                               const double* __restrict__ vforce, double gx, double gy,
-                                               double gz, FT* __restrict__ F, long stride)
+// ------------------------------------------------- velocity staging
+// The progs' computeVelocity pass (u = j/rho + F/2 from the post-stream
+// populations and the last collision's force,
+// src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
+// bounce-back / no-dynamics cells, the wall velocity on velocity-wall
+// cells), restricted to the cells of the tiles this step's stencils touch.
+// One thread per cell in z-fastest order, so population reads coalesce as
+// in the collide kernel; u goes to a tile-major buffer (64 cells x 3 per
+// tile) that the tile kernel reads contiguously.
+template <class S>
+__global__ void __launch_bounds__(256) k_stage_u(LatView lv, bool swapped, const int* __restrict__ count,
+                                                 typename Pop<S>::C* __restrict__ ut)
 {
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= lv.ncell) return;
+    int x, y, z;
+    decode(c, lv.ny, lv.nz, x, y, z);
+    const long t = ((long)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
+    if (count[t] == 0) return;
+    const int local = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+    const unsigned char m = lv.mask ? lv.mask[c] : (unsigned char)M_FLUID;
+    C u[3] = {C(0), C(0), C(0)};
+    if (m == M_VWALL) {
+        const int pp[3] = {x, y, z};
+        const int side = pp[lv.wall_axis] == 0 ? 0 : 1;
+        for (int d = 0; d < 3; ++d) u[d] = (C)lv.wall_u[3 * side + d];
+    } else if (m == M_FLUID) {
+        Nbr nb;
+        if (swapped) nb.init(lv, x, y, z);
+        C f[Q];
+        load_post<S>(lv, nb, c, swapped, f);
+        C Fc[3];
+        cell_force<FT, C>(lv, c, x, y, z, Fc);
+        const C rb = sum_rho(f);
+        C j[3];
+        sum_j(f, j);
+        const C invRho = C(1) / (rb + C(1));
+        for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
+    }
+    C* dst = ut + 3 * (64 * t + local);
+    dst[0] = u[0];
+    dst[1] = u[1];
+    dst[2] = u[2];
+}
+
+// ----------------------------------------------------- tile kernel (K4+K6)
+// One 64-thread workgroup per touched 4x4x4 tile (one cell per lane):
+//   * INTERP: the cell's velocity u = j/rho + F/2 is formed from its
+//     post-stream populations and the force of the last collision (the
+//     progs' computeVelocity pass, src/dataProcessors/dataAnalysisFunctional3D.hh:772-784;
+//     zero on bounce-back / no-dynamics cells, the wall velocity on
+//     velocity-wall cells), then for every vertex of the tile's list the
+//     workgroup sums phi phi phi u over its cells with a fixed butterfly
+//     and writes the partial to the vertex's slot q;
+//   * SPREAD: every cell sums g + phi phi phi F_k over the tile's vertices
+//     in increasing vertex order (the list is sorted in LDS first), the
+//     order of the reference's vertex loop
+//     (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes its
+//     force.
+// Vertex stencil data are staged in LDS 64 vertices at a time.
+template <class S, bool INTERP, bool SPREAD>
+__global__ void __launch_bounds__(64) k_tile(LatView lv, long NV, Grid gd, int Sw, const int* __restrict__ active,
+                                             int* __restrict__ status, const int* __restrict__ count,
+                                             const int* __restrict__ off, const int* __restrict__ list,
+                                             const int* __restrict__ base, const double* __restrict__ W,
+                                             const double* __restrict__ vforce, double* __restrict__ part,
+                                             const typename Pop<S>::C* __restrict__ ut,
+                                             typename Pop<S>::F* __restrict__ Fout)
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
     __shared__ int keys[kMaxList];
+    __shared__ int sb[3][64];
+    __shared__ double sw[12][64];
+    __shared__ double sf[3][64];
     const int na = status[0];
     const int lane = threadIdx.x;
     for (int a = blockIdx.x; a < na; a += gridDim.x) {
@@ -556,49 +588,118 @@ __global__ void __launch_bounds__(64) k_spread(Grid gd, int S, const int* __rest
         while (n2 < n) n2 <<= 1;
         for (int k = lane; k < n2; k += 64) keys[k] = k < n ? list[o + k] : 0x7fffffff;
         __syncthreads();
-        for (int size = 2; size <= n2; size <<= 1)
-            for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
-                for (int k = lane; k < n2; k += 64) {
-                    const int partner = k ^ stride2;
-                    if (partner > k) {
-                        const bool up = (k & size) == 0;
-                        const int a0 = keys[k], a1 = keys[partner];
-                        if ((a0 > a1) == up) {
-                            keys[k] = a1;
-                            keys[partner] = a0;
+        if (SPREAD) {
+            for (int size = 2; size <= n2; size <<= 1)
+                for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
+                    for (int k = lane; k < n2; k += 64) {
+                        const int partner = k ^ stride2;
+                        if (partner > k) {
+                            const bool up = (k & size) == 0;
+                            const int a0 = keys[k], a1 = keys[partner];
+                            if ((a0 > a1) == up) {
+                                keys[k] = a1;
+                                keys[partner] = a0;
+                            }
                         }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
+        }
         const int tz = t % gd.nt[2], r = t / gd.nt[2], ty = r % gd.nt[1], tx = r / gd.nt[1];
         const int x = tx * kTile + (lane >> 4), y = ty * kTile + ((lane >> 2) & 3), z = tz * kTile + (lane & 3);
         const bool inside = x < gd.n[0] && y < gd.n[1] && z < gd.n[2];
-        double acc[3] = {gx, gy, gz};
-        if (inside) {
-            for (int k = 0; k < n; ++k) {
-                const int v = keys[k];
-                const int* b = base + 3 * v;
-                const double* w = W + 12 * v;
-                double wx = 0, wy = 0, wz = 0;
-                bool hx = false, hy = false, hz = false;
-                for (int q = 0; q < S; ++q) {
-                    if (wrap_node(b[0] + q, gd.n[0], gd.per[0]) == x) { wx = hx ? wx + w[q] : w[q]; hx = true; }
-                    if (wrap_node(b[1] + q, gd.n[1], gd.per[1]) == y) { wy = hy ? wy + w[4 + q] : w[4 + q]; hy = true; }
-                    if (wrap_node(b[2] + q, gd.n[2], gd.per[2]) == z) { wz = hz ? wz + w[8 + q] : w[8 + q]; hz = true; }
-                }
-                if (!(hx && hy && hz)) continue;
-                const double wxyz = wx * wy * wz;
-                acc[0] += wxyz * vforce[3 * v];
-                acc[1] += wxyz * vforce[3 * v + 1];
-                acc[2] += wxyz * vforce[3 * v + 2];
-            }
-            const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
-            F[c] = (FT)acc[0];
-            F[stride + c] = (FT)acc[1];
-            F[2 * stride + c] = (FT)acc[2];
+        const long c = inside ? ((long)x * gd.n[1] + y) * gd.n[2] + z : 0;
+        double u0 = 0, u1 = 0, u2 = 0;
+        if (INTERP) {
+            const auto* up = ut + 3 * (64 * (long)t + lane);
+            u0 = (double)up[0];
+            u1 = (double)up[1];
+            u2 = (double)up[2];
         }
-        __syncthreads();
+        double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int jv = c0 + lane;
+            if (jv < n) {
+                const int v = keys[jv] >> 5;
+                sb[0][lane] = base[3 * v];
+                sb[1][lane] = base[3 * v + 1];
+                sb[2][lane] = base[3 * v + 2];
+                for (int q = 0; q < 12; ++q) sw[q][lane] = W[12 * (long)v + q];
+                if (SPREAD) {
+                    sf[0][lane] = vforce[3 * (long)v];
+                    sf[1][lane] = vforce[3 * (long)v + 1];
+                    sf[2][lane] = vforce[3 * (long)v + 2];
+                }
+            }
+            __syncthreads();
+            const int cnt = min(64, n - c0);
+            for (int jj = 0; jj < cnt; ++jj) {
+                int dx = x - sb[0][jj];
+                if (gd.per[0] && dx < 0) dx += gd.n[0];
+                int dy = y - sb[1][jj];
+                if (gd.per[1] && dy < 0) dy += gd.n[1];
+                int dz = z - sb[2][jj];
+                if (gd.per[2] && dz < 0) dz += gd.n[2];
+                const bool hit = inside && (unsigned)dx < (unsigned)Sw && (unsigned)dy < (unsigned)Sw &&
+                                 (unsigned)dz < (unsigned)Sw;
+                const double w = hit ? (sw[dx][jj] * sw[4 + dy][jj]) * sw[8 + dz][jj] : 0.0;
+                if (INTERP) {
+                    double p0 = w * u0, p1 = w * u1, p2 = w * u2;
+#pragma unroll
+                    for (int ofs = 32; ofs > 0; ofs >>= 1) {
+                        p0 += __shfl_xor(p0, ofs);
+                        p1 += __shfl_xor(p1, ofs);
+                        p2 += __shfl_xor(p2, ofs);
+                    }
+                    if (lane == 0) {
+                        const int key = keys[c0 + jj];
+                        double* dst = part + 3 * ((long)(key & 31) * NV + (key >> 5));
+                        dst[0] = p0;
+                        dst[1] = p1;
+                        dst[2] = p2;
+                    }
+                }
+                if (SPREAD && hit) {
+                    acc0 += w * sf[0][jj];
+                    acc1 += w * sf[1][jj];
+                    acc2 += w * sf[2][jj];
+                }
+            }
+            __syncthreads();
+        }
+        if (SPREAD && inside) {
+            Fout[c] = (FT)acc0;
+            Fout[lv.stride + c] = (FT)acc1;
+            Fout[2 * lv.stride + c] = (FT)acc2;
+        }
+    }
+}
+
+// Vertex velocity: sum of its per-tile partials in tile order; with MOVE
+// the vertex is advanced, x += U (explicit Euler,
+// src/fsi/DeformableParticle3D.hh:26-36).
+template <bool MOVE>
+__global__ void __launch_bounds__(256) k_vertex_velocity(long NV, const unsigned char* __restrict__ vnt,
+                                                         const double* __restrict__ part, double* __restrict__ vel,
+                                                         double* __restrict__ pos)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const int m = vnt[v];
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int q = 0; q < m; ++q) {
+        const double* pp = part + 3 * ((long)q * NV + v);
+        a0 += pp[0];
+        a1 += pp[1];
+        a2 += pp[2];
+    }
+    vel[3 * v] = a0;
+    vel[3 * v + 1] = a1;
+    vel[3 * v + 2] = a2;
+    if (MOVE) {
+        pos[3 * v] += a0;
+        pos[3 * v + 1] += a1;
+        pos[3 * v + 2] += a2;
     }
 }
 
@@ -610,78 +711,102 @@ __global__ void k_move(long NV, double* __restrict__ pos, const double* __restri
     pos[v] += vel[v];
 }
 
-// One workgroup per particle: area, signed volume, area-weighted centroid
+// Per-particle area, signed volume, area-weighted centroid
 // (src/fsi/DeformableParticle3D.hh:54-82) and bounding box
-// (src/fsi/ParticleBase.hh:51-69), reduced in a fixed order; then the
-// centre of mass is mapped onto the periodic grid
-// (src/fsi/ParticleBase.hh:113-136, src/fsi/Periodicity.h:83-87) and the
-// shift recorded for k_apply_shift.
-__global__ void __launch_bounds__(256) k_update_particles(ForceView f, Grid gd, double* geom, double* delta,
-                                                          int remap)
+// (src/fsi/ParticleBase.hh:51-69) in FP64: each particle's triangles are cut
+// into chunks of 256 (one workgroup each, fixed-order tree in LDS), the
+// chunk partials are then summed in chunk order, so the result does not
+// depend on scheduling.
+__global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int* __restrict__ ch_part,
+                                                      const int* __restrict__ ch_idx,
+                                                      const int* __restrict__ p_choff, double* __restrict__ partial)
 {
-    __shared__ double sh[5][256];
-    __shared__ double bb[6][256];
-    const int p = blockIdx.x;
+    __shared__ double sh[11][256];
+    const int c = blockIdx.x;
+    const int p = ch_part[c], k = ch_idx[c];
+    const int nch = p_choff[p + 1] - p_choff[p];
     const int s = f.p_shape[p];
     const long vo = f.p_voff[p], nv = f.p_voff[p + 1] - vo;
     const long nt = f.p_toff[p + 1] - f.p_toff[p];
     double acc[5] = {0, 0, 0, 0, 0};
-    for (long t = threadIdx.x; t < nt; t += blockDim.x) {
+    const long t = (long)k * 256 + threadIdx.x;
+    if (t < nt) {
         const int* tri = f.s_tri + 3 * (f.s_toff[s] + t);
-        const D3 v0 = ld3(f.pos + 3 * (vo + tri[0])), v1 = ld3(f.pos + 3 * (vo + tri[1])), v2 = ld3(f.pos + 3 * (vo + tri[2]));
+        const D3 v0 = ld3(f.pos + 3 * (vo + tri[0])), v1 = ld3(f.pos + 3 * (vo + tri[1])),
+                 v2 = ld3(f.pos + 3 * (vo + tri[2]));
         const double dA = 0.5 * norm(cross(v1 - v0, v2 - v0));
-        acc[0] += dA;
-        acc[1] += dot(v0, cross(v1, v2)) / 6.;
-        acc[2] += ((v0.x + v1.x + v2.x) / 3.0) * dA;
-        acc[3] += ((v0.y + v1.y + v2.y) / 3.0) * dA;
-        acc[4] += ((v0.z + v1.z + v2.z) / 3.0) * dA;
+        acc[0] = dA;
+        acc[1] = dot(v0, cross(v1, v2)) / 6.;
+        acc[2] = ((v0.x + v1.x + v2.x) / 3.0) * dA;
+        acc[3] = ((v0.y + v1.y + v2.y) / 3.0) * dA;
+        acc[4] = ((v0.z + v1.z + v2.z) / 3.0) * dA;
     }
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (long v = threadIdx.x; v < nv; v += blockDim.x)
+    const long v0i = nv * k / nch, v1i = nv * (k + 1) / nch;
+    for (long v = v0i + threadIdx.x; v < v1i; v += blockDim.x)
         for (int d = 0; d < 3; ++d) {
             const double x = f.pos[3 * (vo + v) + d];
             lo[d] = fmin(lo[d], x);
             hi[d] = fmax(hi[d], x);
         }
-    for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+    for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = acc[q];
     for (int d = 0; d < 3; ++d) {
-        bb[2 * d][threadIdx.x] = lo[d];
-        bb[2 * d + 1][threadIdx.x] = hi[d];
+        sh[5 + 2 * d][threadIdx.x] = lo[d];
+        sh[6 + 2 * d][threadIdx.x] = hi[d];
     }
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
-            for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+            for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
             for (int d = 0; d < 3; ++d) {
-                bb[2 * d][threadIdx.x] = fmin(bb[2 * d][threadIdx.x], bb[2 * d][threadIdx.x + w]);
-                bb[2 * d + 1][threadIdx.x] = fmax(bb[2 * d + 1][threadIdx.x], bb[2 * d + 1][threadIdx.x + w]);
+                sh[5 + 2 * d][threadIdx.x] = fmin(sh[5 + 2 * d][threadIdx.x], sh[5 + 2 * d][threadIdx.x + w]);
+                sh[6 + 2 * d][threadIdx.x] = fmax(sh[6 + 2 * d][threadIdx.x], sh[6 + 2 * d][threadIdx.x + w]);
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double* G = geom + 11 * p;
-        const double area = sh[0][0];
-        double com[3] = {sh[2][0] / area, sh[3][0] / area, sh[4][0] / area};
-        double sft[3] = {0, 0, 0};
-        if (remap) {
-            for (int d = 0; d < 3; ++d) {
-                if (!gd.per[d]) continue;
-                const double L = (double)gd.n[d];
-                const double x0 = -0.5;
-                const double r = com[d] - L * floor((com[d] - x0) * (1. / L));
-                sft[d] = r - com[d];
-                com[d] = r;
-            }
-        }
-        G[0] = area;
-        G[1] = sh[1][0];
+    if (threadIdx.x < 11) partial[11 * (long)c + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Chunk partials -> geometry; the centre of mass is then mapped onto the
+// periodic grid (src/fsi/ParticleBase.hh:113-136, src/fsi/Periodicity.h:83-87)
+// and the shift recorded for k_apply_shift.
+__global__ void k_geom_final(int np, Grid gd, const int* __restrict__ p_choff, const double* __restrict__ partial,
+                             double* __restrict__ geom, double* __restrict__ delta, int remap)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double acc[5] = {0, 0, 0, 0, 0};
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int c = p_choff[p]; c < p_choff[p + 1]; ++c) {
+        const double* P = partial + 11 * (long)c;
+        for (int q = 0; q < 5; ++q) acc[q] += P[q];
         for (int d = 0; d < 3; ++d) {
-            G[2 + d] = com[d];
-            G[5 + 2 * d] = bb[2 * d][0] + sft[d];
-            G[6 + 2 * d] = bb[2 * d + 1][0] + sft[d];
-            delta[3 * p + d] = sft[d];
+            lo[d] = fmin(lo[d], P[5 + 2 * d]);
+            hi[d] = fmax(hi[d], P[6 + 2 * d]);
         }
+    }
+    double* G = geom + 11 * p;
+    const double area = acc[0];
+    double com[3] = {acc[2] / area, acc[3] / area, acc[4] / area};
+    double sft[3] = {0, 0, 0};
+    if (remap) {
+        for (int d = 0; d < 3; ++d) {
+            if (!gd.per[d]) continue;
+            const double L = (double)gd.n[d];
+            const double x0 = -0.5;
+            const double r = com[d] - L * floor((com[d] - x0) * (1. / L));
+            sft[d] = r - com[d];
+            com[d] = r;
+        }
+    }
+    G[0] = area;
+    G[1] = acc[1];
+    for (int d = 0; d < 3; ++d) {
+        G[2 + d] = com[d];
+        G[5 + 2 * d] = lo[d] + sft[d];
+        G[6 + 2 * d] = hi[d] + sft[d];
+        delta[3 * p + d] = sft[d];
     }
 }
 
@@ -848,7 +973,6 @@ ForceView force_view(bl_ibm g)
     f.s_vp_head = g->s_vp_head; f.s_vp = g->s_vp;
     f.s_area0 = g->s_area0;
     f.pos = g->pos;
-    f.c_tri = g->c_tri; f.c_link = g->c_link; f.c_pair = g->c_pair;
     return f;
 }
 
@@ -872,7 +996,10 @@ int update_particles(bl_ibm g, int remap)
     hipStream_t s = g->lat->stream;
     const ForceView f = force_view(g);
     const Grid gd = grid_of(g->lat);
-    hipLaunchKernelGGL(k_update_particles, dim3(g->np), dim3(256), 0, s, f, gd, g->p_geom, g->p_delta, remap);
+    hipLaunchKernelGGL(k_geom_partial, dim3(g->nchunk), dim3(256), 0, s, f, (const int*)g->ch_part,
+                       (const int*)g->ch_idx, (const int*)g->p_choff, g->ch_partial);
+    hipLaunchKernelGGL(k_geom_final, dim3(nblk(g->np, 64)), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
+                       (const double*)g->ch_partial, g->p_geom, g->p_delta, remap);
     if (remap)
         hipLaunchKernelGGL(k_apply_shift, dim3(nblk(g->NV)), dim3(256), 0, s, (const long*)g->p_voff, g->np, g->NV,
                            (const double*)g->p_delta, g->pos);
@@ -880,72 +1007,110 @@ int update_particles(bl_ibm g, int remap)
     return BL_OK;
 }
 
-// bin + stage + interpolate
-int do_interpolate(bl_ibm g)
+// stencil binning: counts, scan (+ next tile flags), list fill
+int do_bin(bl_ibm g)
 {
     bl_lattice L = g->lat;
     hipStream_t s = L->stream;
     const Grid gd = grid_of(L);
-    if (g->NV == 0) return BL_OK;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, g->n_active + 2, 1L, 0);  // overflow flags
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, L->ntiles, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_fill, L->ntiles, 0);
-    hipLaunchKernelGGL(k_bin_count, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
-                       (const double*)g->pos, g->v_base, g->v_w, g->tile_count);
+    hipLaunchKernelGGL(k_clear_bins, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, g->tile_fill, L->ntiles,
+                       g->n_active);
+    if (g->NV)
+        hipLaunchKernelGGL(k_bin_count, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
+                           (const double*)g->pos, g->v_base, g->v_w, g->v_nt, g->tile_count);
     const long nb = (L->ntiles + kScanChunk - 1) / kScanChunk;
     unsigned long long* sums = (unsigned long long*)g->scan_blocks;
     hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles, sums);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, sums, nb, g->n_active);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles,
-                       (const unsigned long long*)sums, g->tile_off, g->active);
-    hipLaunchKernelGGL(k_bin_fill, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV, (const int*)g->v_base,
-                       (const int*)g->tile_off, g->tile_fill, g->tile_list, g->list_cap, g->n_active);
+                       (const unsigned long long*)sums, g->tile_off, g->active, g->flags_alt);
+    if (g->NV)
+        hipLaunchKernelGGL(k_bin_fill, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV, (const int*)g->v_base,
+                           (const int*)g->tile_off, g->tile_fill, g->tile_list, g->list_cap, g->n_active);
     BL_CHECK_LAUNCH();
-    int rc = lattice_stage_velocity_tiles(L, g->active, g->n_active, g->ubuf, s);
-    if (rc) return rc;
-    if (L->storage == BL_STORE_F64)
-        hipLaunchKernelGGL(k_interpolate<double>, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV,
-                           (const int*)g->v_base, (const double*)g->v_w, (const double*)g->ubuf, g->vel);
+    g->binned = true;
+    return BL_OK;
+}
+
+template <class S, bool I, bool SP>
+void launch_tile_t(bl_ibm g, const LatView& lv, bool swapped, unsigned grid)
+{
+    bl_lattice L = g->lat;
+    using C = typename Pop<S>::C;
+    if (I)
+        hipLaunchKernelGGL(k_stage_u<S>, dim3((unsigned)((L->ncell + 255) / 256)), dim3(256), 0, L->stream, lv,
+                           swapped, (const int*)g->tile_count, (C*)g->ut);
+    hipLaunchKernelGGL((k_tile<S, I, SP>), dim3(grid), dim3(64), 0, L->stream, lv, g->NV, grid_of(L), g->S,
+                       (const int*)g->active, g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
+                       (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
+                       (const double*)g->force, g->part, (const C*)g->ut, (typename Pop<S>::F*)L->force);
+}
+
+template <class S>
+void launch_tile_s(bl_ibm g, const LatView& lv, bool swapped, unsigned grid, bool interp, bool spread)
+{
+    if (interp && spread) launch_tile_t<S, true, true>(g, lv, swapped, grid);
+    else if (interp) launch_tile_t<S, true, false>(g, lv, swapped, grid);
+    else launch_tile_t<S, false, true>(g, lv, swapped, grid);
+}
+
+int launch_tile(bl_ibm g, bool interp, bool spread)
+{
+    bl_lattice L = g->lat;
+    if (g->NV == 0) return BL_OK;
+    LatView lv = view_of(L);
+    if (!L->force_field) lv.flags = nullptr;
+    const bool swapped = (L->steps & 1) != 0;
+    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 32768));
+    if (L->storage == BL_STORE_F64) launch_tile_s<double>(g, lv, swapped, grid, interp, spread);
+    else if (L->storage == BL_STORE_F32) launch_tile_s<float>(g, lv, swapped, grid, interp, spread);
+    else launch_tile_s<__half>(g, lv, swapped, grid, interp, spread);
+    BL_CHECK_LAUNCH();
+    if (spread) std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
+    return BL_OK;
+}
+
+int vertex_velocity(bl_ibm g, bool move)
+{
+    hipStream_t s = g->lat->stream;
+    if (!g->NV) return BL_OK;
+    if (move)
+        hipLaunchKernelGGL(k_vertex_velocity<true>, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV,
+                           (const unsigned char*)g->v_nt, (const double*)g->part, g->vel, g->pos);
     else
-        hipLaunchKernelGGL(k_interpolate<float>, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV,
-                           (const int*)g->v_base, (const double*)g->v_w, (const float*)g->ubuf, g->vel);
+        hipLaunchKernelGGL(k_vertex_velocity<false>, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV,
+                           (const unsigned char*)g->v_nt, (const double*)g->part, g->vel, g->pos);
     BL_CHECK_LAUNCH();
     return BL_OK;
+}
+
+int do_interpolate(bl_ibm g)
+{
+    int rc;
+    if ((rc = do_bin(g))) return rc;
+    if ((rc = launch_tile(g, true, false))) return rc;
+    return vertex_velocity(g, false);
 }
 
 int do_forces(bl_ibm g)
 {
     hipStream_t s = g->lat->stream;
-    ForceView f = force_view(g);
-    if (g->NT) hipLaunchKernelGGL(k_tri_forces, dim3(nblk(g->NT)), dim3(256), 0, s, f, g->NT);
-    if (g->NL) hipLaunchKernelGGL(k_link_forces, dim3(nblk(g->NL)), dim3(256), 0, s, f, g->NL);
-    if (g->NP) hipLaunchKernelGGL(k_pair_forces, dim3(nblk(g->NP)), dim3(256), 0, s, f, g->NP);
-    if (g->NV) hipLaunchKernelGGL(k_gather_forces, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, g->force);
+    const ForceView f = force_view(g);
+    if (g->NV) {
+        if (g->max_incident <= 32)
+            hipLaunchKernelGGL(k_vertex_forces<32>, dim3(nblk(32 * g->NV)), dim3(256), 0, s, f, g->NV, g->force);
+        else
+            hipLaunchKernelGGL(k_vertex_forces<64>, dim3(nblk(64 * g->NV)), dim3(256), 0, s, f, g->NV, g->force);
+    }
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
 int do_spread(bl_ibm g)
 {
-    bl_lattice L = g->lat;
-    hipStream_t s = L->stream;
-    const Grid gd = grid_of(L);
-    hipLaunchKernelGGL(k_flags_from_counts, dim3(nblk(L->ntiles)), dim3(256), 0, s, (const int*)g->tile_count,
-                       L->ntiles, L->tile_flags);
-    if (g->NV) {
-        if (L->storage == BL_STORE_F64)
-            hipLaunchKernelGGL(k_spread<double>, dim3(2048), dim3(64), 0, s, gd, g->S, (const int*)g->active,
-                               g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
-                               (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
-                               (const double*)g->force, L->g[0], L->g[1], L->g[2], (double*)L->force, L->stride);
-        else
-            hipLaunchKernelGGL(k_spread<float>, dim3(2048), dim3(64), 0, s, gd, g->S, (const int*)g->active,
-                               g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
-                               (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
-                               (const double*)g->force, L->g[0], L->g[1], L->g[2], (float*)L->force, L->stride);
-    }
-    BL_CHECK_LAUNCH();
-    return BL_OK;
+    int rc;
+    if (!g->binned && (rc = do_bin(g))) return rc;
+    return launch_tile(g, false, true);
 }
 
 int do_move(bl_ibm g)
@@ -953,6 +1118,7 @@ int do_move(bl_ibm g)
     hipStream_t s = g->lat->stream;
     if (g->NV) hipLaunchKernelGGL(k_move, dim3(nblk(3 * g->NV)), dim3(256), 0, s, g->NV, g->pos, (const double*)g->vel);
     BL_CHECK_LAUNCH();
+    g->binned = false;
     return update_particles(g, 1);
 }
 
@@ -993,13 +1159,15 @@ int step_body(bl_ibm g)
 {
     int rc;
     mark(g, 0);
-    if ((rc = do_interpolate(g))) return rc;
+    if ((rc = do_bin(g))) return rc;
     mark(g, 1);
     if ((rc = do_forces(g))) return rc;
     mark(g, 2);
-    if ((rc = do_spread(g))) return rc;
+    if ((rc = launch_tile(g, true, true))) return rc;
     mark(g, 3);
-    if ((rc = do_move(g))) return rc;
+    if ((rc = vertex_velocity(g, true))) return rc;
+    g->binned = false;
+    if ((rc = update_particles(g, 1))) return rc;
     mark(g, 4);
     if ((rc = do_voxelize(g))) return rc;
     mark(g, 5);
@@ -1018,6 +1186,10 @@ int bl_ibm_create(bl_lattice lat, int kernel, bl_ibm* out)
     if (!lat || !out) return fail(BL_ERR_ARG, "bl_ibm_create: null argument");
     if (kernel < BL_KERNEL_ROMA || kernel > BL_KERNEL_TOPHAT) return fail(BL_ERR_ARG, "bl_ibm_create: unknown kernel");
     if (lat->ibm) return fail(BL_ERR_STATE, "bl_ibm_create: lattice already has an immersed boundary");
+    const int n3[3] = {lat->nx, lat->ny, lat->nz};
+    for (int d = 0; d < 3; ++d)
+        if (lat->periodic[d] && n3[d] < 4)
+            return fail(BL_ERR_ARG, "bl_ibm_create: periodic extents must be at least 4 (one stencil width)");
     bl_ibm g = new bl_ibm_s();
     g->lat = lat;
     g->kernel = kernel;
@@ -1040,9 +1212,10 @@ int bl_ibm_destroy(bl_ibm g)
                     g->s_tri_area0, g->s_tri_geo0, g->s_rest, g->s_link, g->s_link_len, g->s_pair, g->s_pair_cs0,
                     g->s_vt_head, g->s_vt, g->s_vl_head, g->s_vl, g->s_vp_head, g->s_vp, g->s_area0, g->p_shape,
                     g->p_model, g->p_bug, g->p_params, g->p_voff, g->p_toff, g->p_loff, g->p_poff, g->p_geom,
-                    g->p_delta, g->pos, g->vel, g->force, g->c_tri, g->c_link, g->c_pair, g->v_base, g->v_w,
+                    g->p_delta, g->pos, g->vel, g->force, g->v_nt, g->part, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_off, g->tile_fill, g->tile_list, g->active, g->n_active, g->scan_blocks,
-                    g->ubuf, g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase};
+                    g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
+                    g->flags_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial};
     for (void* p : ptrs) hipFree(p);
     for (auto& e : g->ev)
         if (e) hipEventDestroy(e);
@@ -1134,6 +1307,18 @@ int bl_ibm_init(bl_ibm g)
         s_area0v.push_back(s.area);
     }
     g->nshape = (int)g->shapes.size();
+    g->max_incident = 0;
+    for (const auto& p : g->parts) {
+        const ShapeHost& sh = g->shapes[p.shape];
+        for (int v = 0; v < sh.nv; ++v) {
+            int n = sh.v_tri_off[v + 1] - sh.v_tri_off[v];
+            if (p.model == BL_MODEL_RBC) n += sh.v_link_off[v + 1] - sh.v_link_off[v];
+            if (p.model == BL_MODEL_RBC || p.model == 3) n += sh.v_pair_off[v + 1] - sh.v_pair_off[v];
+            g->max_incident = std::max(g->max_incident, n);
+        }
+    }
+    if (g->max_incident > 64)
+        return fail(BL_ERR_CAPACITY, "bl_ibm_init: a vertex has more than 64 incident elements");
     if ((rc = upload(g->s_voff, s_voff)) || (rc = upload(g->s_toff, s_toff)) || (rc = upload(g->s_loff, s_loff)) ||
         (rc = upload(g->s_poff, s_poff)) || (rc = upload(g->s_vtoff, s_vtoff)) || (rc = upload(g->s_vloff, s_vloff)) ||
         (rc = upload(g->s_vpoff, s_vpoff)) || (rc = upload(g->s_tri, s_tri)) || (rc = upload(g->s_tri_area0, s_tarea)) ||
@@ -1168,8 +1353,9 @@ int bl_ibm_init(bl_ibm g)
         (rc = upload(g->p_poff, poff)) || (rc = upload(g->p_shape, pshape)) || (rc = upload(g->p_model, pmodel)) ||
         (rc = upload(g->p_bug, pbug)) || (rc = upload(g->p_params, params)) || (rc = upload(g->pos, pos)) ||
         (rc = alloc(g->vel, 3 * g->NV)) || (rc = alloc(g->force, 3 * g->NV)) || (rc = alloc(g->p_geom, 11 * (size_t)g->np)) ||
-        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) || (rc = alloc(g->c_tri, 9 * g->NT)) ||
-        (rc = alloc(g->c_link, 6 * g->NL)) || (rc = alloc(g->c_pair, 12 * g->NP)) ||
+        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) || (rc = alloc(g->v_nt, g->NV)) ||
+        (rc = alloc(g->part, 3 * 27 * g->NV)) ||
+        (rc = alloc(g->ut, 3 * 64 * L->ntiles)) ||
         (rc = alloc(g->v_base, 3 * g->NV)) || (rc = alloc(g->v_w, 12 * g->NV)))
         return rc;
     BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
@@ -1183,10 +1369,25 @@ int bl_ibm_init(bl_ibm g)
         (rc = alloc(g->scan_blocks, 2 * (g->nscan_blocks + 1))))
         return rc;
     BL_HIP(hipMemset(g->n_active, 0, 4 * sizeof(int)));
+    if ((rc = alloc(g->flags_alt, L->ntiles))) return rc;
+    BL_HIP(hipMemset(g->flags_alt, 0, L->ntiles));
+    {
+        std::vector<int> chp, chi, choff{0};
+        for (int p = 0; p < g->np; ++p) {
+            const long nt = toff[p + 1] - toff[p];
+            const int nch = (int)std::max<long>(1, (nt + 255) / 256);
+            for (int k = 0; k < nch; ++k) {
+                chp.push_back(p);
+                chi.push_back(k);
+            }
+            choff.push_back(choff.back() + nch);
+        }
+        g->nchunk = (int)chp.size();
+        if ((rc = upload(g->ch_part, chp)) || (rc = upload(g->ch_idx, chi)) || (rc = upload(g->p_choff, choff)) ||
+            (rc = alloc(g->ch_partial, 11 * (size_t)std::max(g->nchunk, 1))))
+            return rc;
+    }
     BL_HIP(hipMemset(g->tile_count, 0, L->ntiles * sizeof(int)));
-    const size_t usz = L->storage == BL_STORE_F64 ? sizeof(double) : sizeof(float);
-    hipFree(g->ubuf);
-    BL_HIP(hipMalloc(&g->ubuf, 3 * L->ncell * usz));
     // voxelization
     const long ncol = (long)L->nx * L->ny;
     g->col_cap = std::max<long>(64 * g->NT, 1024);
@@ -1403,6 +1604,19 @@ int bl_ibm_get_phase(bl_ibm g, unsigned char* phase)
     if (!phase) return fail(BL_ERR_ARG, "bl_ibm_get_phase: null argument");
     BL_HIP(hipMemcpyAsync(phase, g->phase, g->lat->ncell, hipMemcpyDeviceToHost, g->lat->stream));
     BL_HIP(hipStreamSynchronize(g->lat->stream));
+    return BL_OK;
+}
+
+int bl_ibm_tile_stats(bl_ibm g, long long out[2])
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!out) return fail(BL_ERR_ARG, "bl_ibm_tile_stats: null argument");
+    int st[2] = {0, 0};
+    BL_HIP(hipMemcpyAsync(st, g->n_active, sizeof(st), hipMemcpyDeviceToHost, g->lat->stream));
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    out[0] = st[0];
+    out[1] = st[1];
     return BL_OK;
 }
 

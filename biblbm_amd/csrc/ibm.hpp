@@ -25,6 +25,7 @@ struct bl_ibm_s {
     int np = 0;
     long NV = 0, NT = 0, NL = 0, NP = 0;  // totals over particles
     int nshape = 0;
+    int max_incident = 0;  // largest number of elements touching one vertex
 
     // ---- shape data (device, concatenated over shapes) ----
     int* s_voff = nullptr;  // nshape+1 (rest vertices, CSR heads)
@@ -66,9 +67,10 @@ struct bl_ibm_s {
     double* pos = nullptr;
     double* vel = nullptr;
     double* force = nullptr;
-    double* c_tri = nullptr;   // 9 per triangle (3 vertices x 3)
-    double* c_link = nullptr;  // 6 per link
-    double* c_pair = nullptr;  // 12 per pair
+    unsigned char* v_nt = nullptr;  // tiles touched by each vertex's stencil (<= 27)
+    double* part = nullptr;         // 27 x 3 per vertex, [q][v][3]: per-tile interpolation partials
+    double* ut = nullptr;           // staged velocity, tile-major: 64 cells x 3 per tile (FP64 / FP32 by storage)
+    bool binned = false;            // tile lists match the current positions
 
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
@@ -81,8 +83,13 @@ struct bl_ibm_s {
     int* active = nullptr;     // active tile ids
     int* n_active = nullptr;   // device scalar [0] = active tiles, [1] = total pairs, [2] = overflow
     int* scan_blocks = nullptr;
+    unsigned char* flags_alt = nullptr;  // next step's tile flags (swapped with the lattice's)
+    int* ch_part = nullptr;              // per geometry chunk: particle
+    int* ch_idx = nullptr;               //                     chunk index within the particle
+    int* p_choff = nullptr;              // np+1 chunk offsets
+    int nchunk = 0;
+    double* ch_partial = nullptr;        // 11 per chunk
     long nscan_blocks = 0;
-    void* ubuf = nullptr;      // 3 per cell, lattice compute precision
 
     // ---- voxelization ----
     int* col_count = nullptr;

@@ -69,8 +69,6 @@ struct LatView {
 
 LatView view_of(const bl_lattice_s* L);
 int ensure_force_field(bl_lattice_s* L);
-int lattice_stage_velocity_tiles(bl_lattice_s* L, const int* active_tiles, const int* n_active,
-                                 void* ubuf, hipStream_t s);
 int collide_stream_launch(bl_lattice_s* L);
 int velocity_walls_launch(bl_lattice_s* L);
 

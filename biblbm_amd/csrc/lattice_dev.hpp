@@ -1,0 +1,231 @@
+// Device-side helpers shared by the lattice and immersed-boundary kernels:
+// storage conversion, AA-pattern slot mapping, D3Q19 moments and the
+// Guo-VOF BGK collision.
+#pragma once
+
+#include "lattice.hpp"
+
+namespace bl {
+
+// ------------------------------------------------------------ storage I/O
+template <class S>
+struct Pop;
+template <>
+struct Pop<double> {
+    using C = double;
+    using F = double;
+    __device__ static C ld(const double* A, unsigned k, int) { return A[k]; }
+    __device__ static void st(double* A, unsigned k, int, C v) { A[k] = v; }
+};
+template <>
+struct Pop<float> {
+    using C = float;
+    using F = float;
+    __device__ static C ld(const float* A, unsigned k, int) { return A[k]; }
+    __device__ static void st(float* A, unsigned k, int, C v) { A[k] = v; }
+};
+template <>
+struct Pop<__half> {
+    using C = float;
+    using F = float;
+    __device__ static C ld(const __half* A, unsigned k, int i)
+    {
+        return __half2float(A[k]) * (float)(tw(i) / kF16Scale);
+    }
+    __device__ static void st(__half* A, unsigned k, int i, C v)
+    {
+        A[k] = __float2half_rn(v * (float)(kF16Scale / tw(i)));
+    }
+};
+
+// --------------------------------------------------------- index helpers
+__device__ __forceinline__ int wrapc(int v, int n, int per)
+{
+    if (v < 0) return per ? v + n : -1;
+    if (v >= n) return per ? v - n : -1;
+    return v;
+}
+
+// Workgroups are dealt round-robin over the 8 XCDs; remap so that each XCD
+// walks one contiguous range of cells and neighbouring blocks share its L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nb)
+{
+    const unsigned per = nb >> 3, rem = nb & 7u;
+    const unsigned xcd = bid & 7u, k = bid >> 3;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+struct Nbr {
+    int X[3], Y[3], Z[3];
+    __device__ __forceinline__ void init(const LatView& v, int x, int y, int z)
+    {
+        X[0] = wrapc(x - 1, v.nx, v.per[0]); X[1] = x; X[2] = wrapc(x + 1, v.nx, v.per[0]);
+        Y[0] = wrapc(y - 1, v.ny, v.per[1]); Y[1] = y; Y[2] = wrapc(y + 1, v.ny, v.per[1]);
+        Z[0] = wrapc(z - 1, v.nz, v.per[2]); Z[1] = z; Z[2] = wrapc(z + 1, v.nz, v.per[2]);
+    }
+    // cell index of x + (dx,dy,dz), or -1 outside a non-periodic face
+    __device__ __forceinline__ int at(const LatView& v, int dx, int dy, int dz) const
+    {
+        const int a = X[dx + 1], b = Y[dy + 1], c = Z[dz + 1];
+        if ((a | b | c) < 0) return -1;
+        return (a * v.ny + b) * v.nz + c;
+    }
+};
+
+__device__ __forceinline__ void decode(unsigned c, int ny, int nz, int& x, int& y, int& z)
+{
+    z = (int)(c % (unsigned)nz);
+    const unsigned r = c / (unsigned)nz;
+    y = (int)(r % (unsigned)ny);
+    x = (int)(r / (unsigned)ny);
+}
+
+// Post-stream population f_i(x) after `swapped` (odd number of completed
+// steps) or natural layout: returns (plane, cell).
+__device__ __forceinline__ long post_slot(const LatView& v, const Nbr& n, long c, int i, bool swapped,
+                                         int& plane)
+{
+    if (!swapped) { plane = i; return c; }
+    const long s = n.at(v, -cx(i), -cy(i), -cz(i));
+    if (s < 0) { plane = i; return c; }
+    plane = opp(i);
+    return s;
+}
+
+template <class S>
+__device__ __forceinline__ void load_post(const LatView& v, const Nbr& n, long c, bool swapped,
+                                          typename Pop<S>::C f[Q])
+{
+    const S* A = (const S*)v.pops;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int pl;
+        const long k = post_slot(v, n, c, i, swapped, pl);
+        f[i] = Pop<S>::ld(A + pl * v.stride, (unsigned)k, i);
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void store_post(const LatView& v, const Nbr& n, long c, bool swapped,
+                                           const typename Pop<S>::C f[Q])
+{
+    S* A = (S*)v.pops;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int pl;
+        const long k = post_slot(v, n, c, i, swapped, pl);
+        Pop<S>::st(A + pl * v.stride, (unsigned)k, i, f[i]);
+    }
+}
+
+// External force of cell c.
+template <class FT, class C>
+__device__ __forceinline__ void cell_force(const LatView& v, long c, int x, int y, int z, C F[3])
+{
+    F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2];
+    if (v.flags) {
+        const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
+        if (v.flags[t]) {
+            const FT* Fv = (const FT*)v.force;
+            F[0] = (C)Fv[c];
+            F[1] = (C)Fv[v.stride + c];
+            F[2] = (C)Fv[2 * v.stride + c];
+        }
+    }
+}
+
+// ------------------------------------------------------------- collision
+template <class C>
+__device__ __forceinline__ C sum_rho(const C f[Q])
+{
+    return f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] +
+           f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+}
+
+template <class C>
+__device__ __forceinline__ void sum_j(const C f[Q], C j[3])
+{
+    const C xm = f[1] + f[4] + f[5] + f[6] + f[7];
+    const C xp = f[10] + f[13] + f[14] + f[15] + f[16];
+    const C ym = f[2] + f[4] + f[8] + f[9] + f[14];
+    const C yp = f[5] + f[11] + f[13] + f[17] + f[18];
+    const C zm = f[3] + f[6] + f[8] + f[16] + f[18];
+    const C zp = f[7] + f[9] + f[12] + f[15] + f[17];
+    j[0] = xp - xm;
+    j[1] = yp - ym;
+    j[2] = zp - zm;
+}
+
+template <class C, bool FORCE>
+__device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
+{
+    const C rhoBar = sum_rho(f);
+    C j[3];
+    sum_j(f, j);
+    const C invRho = C(1) / (rhoBar + C(1));
+    C u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = FORCE ? j[d] * invRho + F[d] / C(2) : j[d] * invRho;
+    const C rho = rhoBar + C(1);
+    const C jx = rho * u[0], jy = rho * u[1], jz = rho * u[2];
+
+    const C one_m_omega = C(1) - omega;
+    const C t0_omega = C(1.0 / 3.0) * omega;
+    const C t1_omega = C(1.0 / 18.0) * omega;
+    const C t4_omega = C(1.0 / 36.0) * omega;
+    const C jSqr = jx * jx + jy * jy + jz * jz;
+    const C kx = C(3) * jx, ky = C(3) * jy, kz = C(3) * jz;
+    const C kxSqr_ = invRho / C(2) * kx * kx;
+    const C kySqr_ = invRho / C(2) * ky * ky;
+    const C kzSqr_ = invRho / C(2) * kz * kz;
+    const C kxky_ = invRho * kx * ky;
+    const C kxkz_ = invRho * kx * kz;
+    const C kykz_ = invRho * ky * kz;
+    const C C1 = rhoBar + invRho * C(3) * jSqr;
+    C C2, C3;
+    C3 = -kxSqr_ - kySqr_ - kzSqr_;
+    f[0] = f[0] * one_m_omega + t0_omega * (C1 + C3);
+#define BL_PAIR(a, b, tw_, c2, c3)                       \
+    C2 = (c2);                                           \
+    C3 = (c3);                                           \
+    f[a] = f[a] * one_m_omega + tw_ * (C1 + C2 + C3);    \
+    f[b] = f[b] * one_m_omega + tw_ * (C1 - C2 + C3);
+    BL_PAIR(1, 10, t1_omega, -kx, -kySqr_ - kzSqr_)
+    BL_PAIR(2, 11, t1_omega, -ky, -kxSqr_ - kzSqr_)
+    BL_PAIR(3, 12, t1_omega, -kz, -kxSqr_ - kySqr_)
+    BL_PAIR(4, 13, t4_omega, -kx - ky, kxky_ - kzSqr_)
+    BL_PAIR(5, 14, t4_omega, -kx + ky, -kxky_ - kzSqr_)
+    BL_PAIR(6, 15, t4_omega, -kx - kz, kxkz_ - kySqr_)
+    BL_PAIR(7, 16, t4_omega, -kx + kz, -kxkz_ - kySqr_)
+    BL_PAIR(8, 17, t4_omega, -ky - kz, kykz_ - kxSqr_)
+    BL_PAIR(9, 18, t4_omega, -ky + kz, -kykz_ - kxSqr_)
+#undef BL_PAIR
+
+    if (FORCE) {
+        const C mu = C(1) - omega / C(2);
+        const C o6 = C(1.0 / 6.0), o12 = C(1.0 / 12.0);
+        const C Fx = F[0], Fy = F[1], Fz = F[2];
+        const C ux = u[0], uy = u[1], uz = u[2];
+        f[0] += -mu * (Fx * ux + Fy * uy + Fz * uz);
+        f[1] += o6 * mu * (Fx * (C(-1) + 2 * ux) - Fy * uy - Fz * uz);
+        f[2] += -o6 * mu * (Fx * ux + Fy * (C(1) - 2 * uy) + Fz * uz);
+        f[3] += -o6 * mu * (Fx * ux + Fy * uy + Fz * (C(1) - 2 * uz));
+        f[4] += o12 * mu * (Fx * (C(-1) + 2 * ux + 3 * uy) + Fy * (C(-1) + 2 * uy + 3 * ux) - Fz * uz);
+        f[5] += o12 * mu * (Fx * (C(-1) + 2 * ux - 3 * uy) + Fy * (C(1) + 2 * uy - 3 * ux) - Fz * uz);
+        f[6] += o12 * mu * (Fx * (C(-1) + 2 * ux + 3 * uz) - Fy * uy + Fz * (C(-1) + 2 * uz + 3 * ux));
+        f[7] += o12 * mu * (Fx * (C(-1) + 2 * ux - 3 * uz) - Fy * uy + Fz * (C(1) + 2 * uz - 3 * ux));
+        f[8] += -o12 * mu * (Fx * ux + Fy * (C(1) - 2 * uy - 3 * uz) + Fz * (C(1) - 2 * uz - 3 * uy));
+        f[9] += -o12 * mu * (Fx * ux + Fy * (C(1) - 2 * uy + 3 * uz) + Fz * (C(-1) - 2 * uz + 3 * uy));
+        f[10] += o6 * mu * (Fx * (C(1) + 2 * ux) - Fy * uy - Fz * uz);
+        f[11] += -o6 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy) + Fz * uz);
+        f[12] += -o6 * mu * (Fx * ux + Fy * uy + Fz * (C(-1) - 2 * uz));
+        f[13] += o12 * mu * (Fx * (C(1) + 2 * ux + 3 * uy) + Fy * (C(1) + 2 * uy + 3 * ux) - Fz * uz);
+        f[14] += o12 * mu * (Fx * (C(1) + 2 * ux - 3 * uy) + Fy * (C(-1) + 2 * uy - 3 * ux) - Fz * uz);
+        f[15] += o12 * mu * (Fx * (C(1) + 2 * ux + 3 * uz) - Fy * uy + Fz * (C(1) + 2 * uz + 3 * ux));
+        f[16] += o12 * mu * (Fx * (C(1) + 2 * ux - 3 * uz) - Fy * uy + Fz * (C(-1) + 2 * uz - 3 * ux));
+        f[17] += -o12 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy - 3 * uz) + Fz * (C(-1) - 2 * uz - 3 * uy));
+        f[18] += -o12 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy + 3 * uz) + Fz * (C(1) - 2 * uz + 3 * uy));
+    }
+}
+
+}  // namespace bl

@@ -107,10 +107,15 @@ class ImmersedBoundary:
         call("bl_ibm_get_phase", self.handle, ptr(out))
         return out
 
+    def tile_stats(self):
+        out = np.zeros(2, np.int64)
+        call("bl_ibm_tile_stats", self.handle, ptr(out))
+        return dict(active_tiles=int(out[0]), list_entries=int(out[1]))
+
     def stage_times(self):
         out = np.zeros(5)
         call("bl_ibm_stage_times", self.handle, ptr(out))
-        return dict(zip(["interpolate", "compute_forces", "spread", "move", "voxelize"], out))
+        return dict(zip(["bin", "compute_forces", "interpolate_spread", "move", "voxelize"], out))
 
     def close(self):
         if getattr(self, "handle", None):

@@ -109,8 +109,16 @@ int bl_lattice_steps(bl_lattice lat, long long *steps);
 int bl_lattice_set_launch(bl_lattice lat, int block, int xcd_remap);
 /* HIP stream the lattice works on (hipStream_t), for event timing. */
 int bl_lattice_stream(bl_lattice lat, void **stream);
-/* Device time of the last n collide-stream launches, milliseconds. */
+/* Runs n collide-stream launches (no boundary pass) and returns their total
+ * device time in milliseconds, measured with HIP events on the lattice
+ * stream. */
 int bl_lattice_time_collide(bl_lattice lat, int n_steps, double *ms);
+/* Waits for all work on the device (the reference is synchronous; this has
+ * no counterpart there). */
+int bl_sync(void);
+/* Plain device copy of n_bytes, repeated n_iter times: bandwidth reference
+ * and counter calibration (no counterpart in the reference).  *ms = total. */
+int bl_device_copy(long long n_bytes, int n_iter, double *ms);
 
 /* ------------------------------------------------ immersed boundary */
 /* ImmersedBoundaryDynamics3D(lattice, shape_library)
@@ -158,9 +166,13 @@ int bl_particle_set_positions(bl_ibm g, int pid, const double *pos);
 int bl_particle_observables(bl_ibm g, int pid, double out[14]);
 /* The phase field of the last voxelization (ncell bytes, 0/1). */
 int bl_ibm_get_phase(bl_ibm g, unsigned char *phase);
-/* Time of each IBM stage over the last bl_ibm_step, ms: interpolate,
- * forces, spread, move, voxelize. */
+/* Time of each IBM stage over the last bl_ibm_step, ms: stencil binning,
+ * membrane forces, fused interpolation + spreading, advection + geometry,
+ * voxelization. */
 int bl_ibm_stage_times(bl_ibm g, double out[5]);
+/* Tiles touched by the last binning and (tile, vertex) list entries
+ * (diagnostics for the traffic model; no counterpart in the reference). */
+int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
 
 #ifdef __cplusplus
 }

@@ -178,12 +178,18 @@ static inline long idx(int nx, int ny, int nz, int x, int y, int z)
 
 static inline int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-/* Velocity as Cell::computeVelocity reports it. */
+/* Velocity as Cell::computeVelocity reports it: stored wall velocity on
+ * velocity-wall cells, zero on bounce-back and no-dynamics cells
+ * (src/core/dynamics.hh:796-801,970-975). */
 static void cell_velocity(const double *f, const double *F, unsigned char m,
                           const double *uw, double u[3])
 {
     if (m == 3) {
         u[0] = uw[0]; u[1] = uw[1]; u[2] = uw[2];
+        return;
+    }
+    if (m == 1 || m == 2) {
+        u[0] = u[1] = u[2] = 0.0;
         return;
     }
     double rb = rho_bar(f), j[3];
@@ -335,6 +341,11 @@ void or_lbm_moments(int nx, int ny, int nz, const double *f,
                     const double *uw = wall_u + 3 * side;
                     rho[c] = wall_rho_bar(f + 19 * c, force + 3 * c, wall_axis, side ? 1 : -1, uw) + 1.0;
                     for (int d = 0; d < 3; ++d) u[3 * c + d] = uw[d];
+                } else if (m == 1 || m == 2) {
+                    /* default-constructed BounceBack / NoDynamics report rho = 0
+                     * (src/core/dynamics.h:564,673, src/core/dynamics.hh:786-788) */
+                    rho[c] = 0.0;
+                    u[3 * c] = u[3 * c + 1] = u[3 * c + 2] = 0.0;
                 } else {
                     rho[c] = rho_bar(f + 19 * c) + 1.0;
                     cell_velocity(f + 19 * c, force + 3 * c, 0, 0, u + 3 * c);
