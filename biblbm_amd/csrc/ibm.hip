@@ -4,16 +4,18 @@
 // src/fsi/DataProcessors.hh:94-108, preceded by the progs' computeVelocity
 // and force reset):
 //   bin      every vertex's delta-kernel stencil into 4x4x4 lattice tiles
-//            (counts, exclusive scan, list fill);
-//   stage    u = j/rho + F/2 on the cells of the touched tiles (from the
-//            post-stream populations and the force of the last collision);
-//   interp   U_k = sum phi phi phi u                     (K4)
+//            (counts, exclusive scan, list fill; atomics aggregated per
+//            workgroup in LDS);
 //   forces   per element (triangle / link / pair) contributions, then a
 //            per-vertex gather over the CSR incidence lists (K5, no atomics);
+//   stage    u = j/rho + F/2 on the cells of the touched tiles (from the
+//            post-stream populations and the force of the last collision);
 //   spread   one workgroup per touched tile: its vertex list is sorted in
 //            LDS, every cell sums g + sum_k phi F_k in vertex order (K6,
 //            deterministic, no float atomics);
-//   move     x += U, per-particle area / volume / COM / bbox in FP64 with a
+//   interp   U_k = sum phi phi phi u, one thread per vertex in the
+//            reference's node order (K4), fused with the advection x += U;
+//   move     per-particle area / volume / COM / bbox in FP64 with a
 //            fixed reduction order, COM mapped onto the periodic grid (K7);
 //   voxelize z-ray hits per column, sorted, parity fill of the phase (K8),
 //            only when the inner and outer viscosities differ.
@@ -137,47 +139,118 @@ __device__ __forceinline__ int vertex_tiles(const Grid& gd, int S, const int* b,
     return m;
 }
 
+// Neighbouring vertices of a membrane touch the same few tiles, so a
+// workgroup first aggregates its (tile, vertex) pairs in an LDS hash table
+// (open addressing, linear probing) and then issues one global atomic per
+// distinct tile.  A pair whose tile finds the table full goes straight to
+// the global counter; counts stay exact either way.
+constexpr int kHashBits = 11;
+constexpr int kHash = 1 << kHashBits;
+
+__device__ __forceinline__ int hash_insert(int* keys, int t)
+{
+    unsigned h = ((unsigned)t * 2654435761u) >> (32 - kHashBits);
+    for (int probe = 0; probe < kHash; ++probe) {
+        const int old = atomicCAS(&keys[h], -1, t);
+        if (old == -1 || old == t) return (int)h;
+        h = (h + 1) & (kHash - 1);
+    }
+    return -1;
+}
+
+// slot of a tile inserted earlier in the same workgroup, -1 if its insert
+// found the table full (then it was never inserted)
+__device__ __forceinline__ int hash_find(const int* keys, int t)
+{
+    unsigned h = ((unsigned)t * 2654435761u) >> (32 - kHashBits);
+    for (int probe = 0; probe < kHash; ++probe) {
+        const int k = keys[h];
+        if (k == t) return (int)h;
+        if (k == -1) return -1;
+        h = (h + 1) & (kHash - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void hash_clear(int* keys, int* a, int* b)
+{
+    for (int i = threadIdx.x; i < kHash; i += blockDim.x) {
+        keys[i] = -1;
+        a[i] = 0;
+        if (b) b[i] = 0;
+    }
+}
+
 // Stencil base, 1-D weights and the tiles the stencil touches; counts the
 // (tile, vertex) pairs per tile.
 __global__ void __launch_bounds__(256) k_bin_count(Grid gd, int kernel, int S, double h, long NV,
                                                    const double* __restrict__ pos, int* __restrict__ base,
-                                                   double* __restrict__ W, unsigned char* __restrict__ vnt,
-                                                   int* __restrict__ count)
+                                                   double* __restrict__ W, int* __restrict__ count)
 {
+    __shared__ int keys[kHash];
+    __shared__ int cnt[kHash];
+    hash_clear(keys, cnt, nullptr);
+    __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    int b[3];
-    for (int a = 0; a < 3; ++a) {
-        const double x = pos[3 * v + a];
-        const int raw = (int)ceil(x - h);
-        for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(raw + k) - x) : 0.0;
-        int bw = raw;
-        if (gd.per[a]) {
-            bw = raw % gd.n[a];
-            if (bw < 0) bw += gd.n[a];
-        }
-        b[a] = bw;
-        base[3 * v + a] = bw;
-    }
     int t[27];
-    const int m = vertex_tiles(gd, S, b, t);
-    vnt[v] = (unsigned char)m;
-    for (int q = 0; q < m; ++q) atomicAdd(&count[t[q]], 1);
+    int m = 0;
+    if (v < NV) {
+        int b[3];
+        for (int a = 0; a < 3; ++a) {
+            const double x = pos[3 * v + a];
+            const int raw = (int)ceil(x - h);
+            for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(raw + k) - x) : 0.0;
+            int bw = raw;
+            if (gd.per[a]) {
+                bw = raw % gd.n[a];
+                if (bw < 0) bw += gd.n[a];
+            }
+            b[a] = bw;
+            base[3 * v + a] = bw;
+        }
+        m = vertex_tiles(gd, S, b, t);
+    }
+    for (int q = 0; q < m; ++q) {
+        const int s = hash_insert(keys, t[q]);
+        if (s >= 0) atomicAdd(&cnt[s], 1);
+        else atomicAdd(&count[t[q]], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHash; i += blockDim.x)
+        if (cnt[i] > 0) atomicAdd(&count[keys[i]], cnt[i]);
 }
 
 // Writes each (tile, vertex) pair into its tile's list as v*32 + q, q being
-// the tile's index among the vertex's tiles.  The order within a tile is
-// whatever the atomics give; every consumer sorts or indexes by the entry.
+// the tile's index among the vertex's tiles.  Each workgroup reserves one
+// range per distinct tile and hands out slots inside it from LDS.  The order
+// within a tile is whatever the atomics give; the spreading kernel sorts.
 __global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const int* __restrict__ base,
                                                   const int* __restrict__ off, int* __restrict__ fill,
                                                   int* __restrict__ list, long cap, int* __restrict__ status)
 {
+    __shared__ int keys[kHash];
+    __shared__ int cnt[kHash];  // pairs per slot, then the slot's first list index
+    __shared__ int rank[kHash];
+    hash_clear(keys, cnt, rank);
+    __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
     int t[27];
-    const int m = vertex_tiles(gd, S, base + 3 * v, t);
+    int m = 0;
+    if (v < NV) m = vertex_tiles(gd, S, base + 3 * v, t);
     for (int q = 0; q < m; ++q) {
-        const long slot = (long)off[t[q]] + atomicAdd(&fill[t[q]], 1);
+        const int s = hash_insert(keys, t[q]);
+        if (s >= 0) atomicAdd(&cnt[s], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHash; i += blockDim.x) {
+        const int c = cnt[i];
+        if (c > 0) cnt[i] = off[keys[i]] + atomicAdd(&fill[keys[i]], c);
+    }
+    __syncthreads();
+    for (int q = 0; q < m; ++q) {
+        const int s = hash_find(keys, t[q]);
+        const long slot = s >= 0 ? (long)cnt[s] + atomicAdd(&rank[s], 1)
+                                 : (long)off[t[q]] + atomicAdd(&fill[t[q]], 1);
         if (slot < cap) list[slot] = (int)(v * 32 + q);
         else atomicOr(&status[2], 1);
     }
@@ -393,15 +466,15 @@ __device__ void area_volume_triangle(D3 v0, D3 v1, D3 v2, double area0, double A
     out[2] = out[2] + df_vol * (cross(v0, v1) / 6.);
 }
 
-// Contribution of one incident element to one vertex (the element's slot
-// `k` for that vertex).
-__device__ D3 link_contrib(const ForceView& f, int s, long vo, const double* P, int code)
+// Force of one link on its first vertex (the second gets the negative):
+// worm-like chain + power-law repulsion (src/fsi/RBCParticle.hh:149-171).
+__device__ D3 link_force(const ForceView& f, int s, long vo, const double* P, long link)
 {
     const double m = 2;
     const double x0 = 1. / 2.2;
     const double ks_coeff1 = 0.25 * (6. * x0 + (3 * x0 * x0 - 2 * x0 * x0 * x0) / ((1 - x0) * (1 - x0)));
     const double ks_coeff2 = x0 / (2. * (1 - x0) * (1 - x0) * (1 - x0)) + 0.25 * (1. - 1. / ((1 - x0) * (1 - x0)));
-    const long sl = f.s_loff[s] + (code >> 2);
+    const long sl = f.s_loff[s] + link;
     const double l0 = f.s_link_len[sl];
     const double kp_coeff1 = -1. / pow(l0, m);
     const double kp_coeff2 = (m + 1) / pow(l0, m);
@@ -413,31 +486,30 @@ __device__ D3 link_contrib(const ForceView& f, int s, long vo, const double* P, 
     const double L = norm(dv);
     const double x = L / (l0 * 2.2);
     const double df = -0.25 * ks * (6. * x + (3. * x * x - 2. * x * x * x) / ((1 - x) * (1 - x))) + kp / pow(L, m);
-    const D3 fa = (df / L) * dv;
-    return (code & 3) == 0 ? fa : D3{-fa.x, -fa.y, -fa.z};
+    return (df / L) * dv;
 }
 
-__device__ D3 tri_contrib(const ForceView& f, int p, int s, int model, long vo, const double* P, int code)
+// Forces of one triangle on its three vertices.
+__device__ void tri_forces(const ForceView& f, int p, int s, int model, long vo, const double* P, long tri,
+                           D3 out[3])
 {
-    const int k = code & 3;
-    const long st = f.s_toff[s] + (code >> 2);
+    const long st = f.s_toff[s] + tri;
     const int i0 = f.s_tri[3 * st], i1 = f.s_tri[3 * st + 1], i2 = f.s_tri[3 * st + 2];
     const D3 v0 = ld3(f.pos + 3 * (vo + i0)), v1 = ld3(f.pos + 3 * (vo + i1)), v2 = ld3(f.pos + 3 * (vo + i2));
-    D3 out[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    out[0] = out[1] = out[2] = D3{0, 0, 0};
     const double area0 = f.s_tri_area0[st];
     if (model == BL_MODEL_CAPSULE_SKALAK || model == BL_MODEL_CAPSULE_NEOHOOKEAN || model == 3)
         capsule_triangle(v0, v1, v2, f.s_tri_geo0 + 4 * st, area0, P[0], P[1], model != BL_MODEL_CAPSULE_NEOHOOKEAN,
-                         f.p_bug[p] != 0, k, out);
+                         f.p_bug[p] != 0, -1, out);
     if (model == BL_MODEL_RBC || model == 3)
         area_volume_triangle(v0, v1, v2, area0, f.s_area0[s], f.p_geom[11 * p], f.p_geom[11 * p + 1], P[2], P[3],
                              P[4], P[5], out);
-    return out[k];
 }
 
-__device__ D3 pair_contrib(const ForceView& f, int s, long vo, const double* P, int code)
+// Bending forces of one dihedral pair on its four vertices.
+__device__ void pair_forces(const ForceView& f, int s, long vo, const double* P, long pair, D3 out[4])
 {
-    const int k = code & 3;
-    const long sp = f.s_poff[s] + (code >> 2);
+    const long sp = f.s_poff[s] + pair;
     const int* q = f.s_pair + 4 * sp;
     const D3 v0 = ld3(f.pos + 3 * (vo + q[0])), v1 = ld3(f.pos + 3 * (vo + q[1]));
     const D3 v2 = ld3(f.pos + 3 * (vo + q[2])), v3 = ld3(f.pos + 3 * (vo + q[3]));
@@ -450,51 +522,96 @@ __device__ D3 pair_contrib(const ForceView& f, int s, long vo, const double* P, 
     const double cs = dot(n1u, n2u);
     const double sn = ((dot(n1 - n2, v1 - v3) > 0) ? 1.0 : -1.0) * norm(cross(n1u, n2u));
     const double pref = P[6] * (-f.s_pair_cs0[2 * sp] * sn + f.s_pair_cs0[2 * sp + 1] * cs);
-    D3 gk;
-    if (k == 0) gk = (dot(e1u, e1 - e2) / n2n) * n2u + (dot(e1u, e1 - e0) / n1n) * n1u;
-    else if (k == 1) gk = (-e1n / n1n) * n1u;
-    else if (k == 2) gk = (dot(e1u, e2) / n2n) * n2u + (dot(e1u, e0) / n1n) * n1u;
-    else gk = (-e1n / n2n) * n2u;
-    return pref * gk;
+    out[0] = pref * ((dot(e1u, e1 - e2) / n2n) * n2u + (dot(e1u, e1 - e0) / n1n) * n1u);
+    out[1] = pref * ((-e1n / n1n) * n1u);
+    out[2] = pref * ((dot(e1u, e2) / n2n) * n2u + (dot(e1u, e0) / n1n) * n1u);
+    out[3] = pref * ((-e1n / n2n) * n2u);
 }
 
-// Force on every vertex: LPV lanes per vertex, lane e evaluates the vertex's
-// e-th incident element in the reference's accumulation order (links
-// src/fsi/RBCParticle.hh:149-171, triangles src/fsi/DeformableCapsule.hh:60-150
-// and src/fsi/RBCParticle.hh:176-209, pairs src/fsi/RBCParticle.hh:214-246),
-// and the lanes' contributions are summed by a fixed butterfly.  No atomics
-// and no per-element buffer.
-template <int LPV>
-__global__ void __launch_bounds__(256) k_vertex_forces(ForceView f, long NV, double* __restrict__ force)
+// One thread per element, over the concatenation [triangles | pairs | links]
+// of all particles: the element's forces on its vertices go to its own slots
+// of `ec` (9 doubles per triangle, 12 per pair, 3 per link).  Elements that
+// the particle's model does not use are skipped.
+__global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long NP, long NL, long ec_pair,
+                                                     long ec_link, double* __restrict__ ec)
 {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long v = gid / LPV;
-    const int e = (int)(gid % LPV);
-    D3 c = {0, 0, 0};
-    if (v < NV) {
-        const int p = find_part(f.p_voff, f.np, v);
-        const int s = f.p_shape[p];
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < NT) {
+        const int p = find_part(f.p_toff, f.np, e);
+        D3 out[3];
+        tri_forces(f, p, f.p_shape[p], f.p_model[p], f.p_voff[p], f.p_params + 8 * p, e - f.p_toff[p], out);
+        for (int k = 0; k < 3; ++k) st3(ec + 9 * e + 3 * k, out[k]);
+    } else if (e < NT + NP) {
+        const long g = e - NT;
+        const int p = find_part(f.p_poff, f.np, g);
         const int model = f.p_model[p];
-        const long vo = f.p_voff[p];
-        const long sv = f.s_voff[s] + (v - vo);
-        const double* P = f.p_params + 8 * p;
-        const int* hl = f.s_vl_head + sv + s;
-        const int* ht = f.s_vt_head + sv + s;
-        const int* hp = f.s_vp_head + sv + s;
-        const int nl = model == BL_MODEL_RBC ? hl[1] - hl[0] : 0;
-        const int nt = ht[1] - ht[0];
-        const int np_ = (model == BL_MODEL_RBC || model == 3) ? hp[1] - hp[0] : 0;
-        if (e < nl) c = link_contrib(f, s, vo, P, f.s_vl[f.s_vloff[s] + hl[0] + e]);
-        else if (e < nl + nt) c = tri_contrib(f, p, s, model, vo, P, f.s_vt[f.s_vtoff[s] + ht[0] + e - nl]);
-        else if (e < nl + nt + np_) c = pair_contrib(f, s, vo, P, f.s_vp[f.s_vpoff[s] + hp[0] + e - nl - nt]);
+        if (model != BL_MODEL_RBC && model != 3) return;
+        D3 out[4];
+        pair_forces(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_poff[p], out);
+        for (int k = 0; k < 4; ++k) st3(ec + ec_pair + 12 * g + 3 * k, out[k]);
+    } else if (e < NT + NP + NL) {
+        const long g = e - NT - NP;
+        const int p = find_part(f.p_loff, f.np, g);
+        if (f.p_model[p] != BL_MODEL_RBC) return;
+        st3(ec + ec_link + 3 * g, link_force(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_loff[p]));
     }
-#pragma unroll
-    for (int o = LPV / 2; o > 0; o >>= 1) {
-        c.x += __shfl_xor(c.x, o);
-        c.y += __shfl_xor(c.y, o);
-        c.z += __shfl_xor(c.z, o);
+}
+
+// Force on every vertex: the contributions of its incident elements summed
+// in the reference's accumulation order -- links
+// (src/fsi/RBCParticle.hh:149-171), then triangles in triangle order
+// (src/fsi/DeformableCapsule.hh:60-150, src/fsi/RBCParticle.hh:176-209),
+// then pairs in pair order (src/fsi/RBCParticle.hh:214-246).  The incidence
+// lists hold element * 4 + the vertex's slot in the element, sorted by
+// element.  No atomics.
+__global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, long ec_pair, long ec_link,
+                                                       const double* __restrict__ ec, double* __restrict__ force)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const int p = find_part(f.p_voff, f.np, v);
+    const int s = f.p_shape[p];
+    const int model = f.p_model[p];
+    const long vo = f.p_voff[p];
+    const long sv = f.s_voff[s] + (v - vo);
+    double c0 = 0, c1 = 0, c2 = 0;
+    if (model == BL_MODEL_RBC) {
+        const int* h = f.s_vl_head + sv + s;
+        const int* L = f.s_vl + f.s_vloff[s];
+        const double* E = ec + ec_link + 3 * f.p_loff[p];
+        for (int k = h[0]; k < h[1]; ++k) {
+            const int code = L[k];
+            const double* x = E + 3 * (code >> 2);
+            if ((code & 3) == 0) {
+                c0 += x[0]; c1 += x[1]; c2 += x[2];
+            } else {
+                c0 += -x[0]; c1 += -x[1]; c2 += -x[2];
+            }
+        }
     }
-    if (e == 0 && v < NV) st3(force + 3 * v, c);
+    {
+        const int* h = f.s_vt_head + sv + s;
+        const int* L = f.s_vt + f.s_vtoff[s];
+        const double* E = ec + 9 * f.p_toff[p];
+        for (int k = h[0]; k < h[1]; ++k) {
+            const int code = L[k];
+            const double* x = E + 9 * (code >> 2) + 3 * (code & 3);
+            c0 += x[0]; c1 += x[1]; c2 += x[2];
+        }
+    }
+    if (model == BL_MODEL_RBC || model == 3) {
+        const int* h = f.s_vp_head + sv + s;
+        const int* L = f.s_vp + f.s_vpoff[s];
+        const double* E = ec + ec_pair + 12 * f.p_poff[p];
+        for (int k = h[0]; k < h[1]; ++k) {
+            const int code = L[k];
+            const double* x = E + 12 * (code >> 2) + 3 * (code & 3);
+            c0 += x[0]; c1 += x[1]; c2 += x[2];
+        }
+    }
+    force[3 * v] = c0;
+    force[3 * v + 1] = c1;
+    force[3 * v + 2] = c2;
 }
 
 // ------------------------------------------------- velocity staging
@@ -504,10 +621,9 @@ __global__ void __launch_bounds__(256) k_vertex_forces(ForceView f, long NV, dou
 // bounce-back / no-dynamics cells, the wall velocity on velocity-wall
 // cells), restricted to the cells of the tiles this step's stencils touch.
 // One thread per cell in z-fastest order, so population reads coalesce as
-// in the collide kernel; u goes to a tile-major buffer (64 cells x 3 per
-// tile) that the tile kernel reads contiguously.
-template <class S>
-__global__ void __launch_bounds__(256) k_stage_u(LatView lv, bool swapped, const int* __restrict__ count,
+// in the collide kernel and the three u planes are written contiguously.
+template <class S, bool SWAPPED>
+__global__ void __launch_bounds__(256) k_stage_u(LatView lv, const int* __restrict__ count,
                                                  typename Pop<S>::C* __restrict__ ut)
 {
     using C = typename Pop<S>::C;
@@ -516,9 +632,8 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, bool swapped, const
     if (c >= lv.ncell) return;
     int x, y, z;
     decode(c, lv.ny, lv.nz, x, y, z);
-    const long t = ((long)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
+    const unsigned t = ((unsigned)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
     if (count[t] == 0) return;
-    const int local = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
     const unsigned char m = lv.mask ? lv.mask[c] : (unsigned char)M_FLUID;
     C u[3] = {C(0), C(0), C(0)};
     if (m == M_VWALL) {
@@ -527,9 +642,9 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, bool swapped, const
         for (int d = 0; d < 3; ++d) u[d] = (C)lv.wall_u[3 * side + d];
     } else if (m == M_FLUID) {
         Nbr nb;
-        if (swapped) nb.init(lv, x, y, z);
+        if (SWAPPED) nb.init(lv, x, y, z);
         C f[Q];
-        load_post<S>(lv, nb, c, swapped, f);
+        load_post<S>(lv, nb, c, SWAPPED, f);
         C Fc[3];
         cell_force<FT, C>(lv, c, x, y, z, Fc);
         const C rb = sum_rho(f);
@@ -538,37 +653,80 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, bool swapped, const
         const C invRho = C(1) / (rb + C(1));
         for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
     }
-    C* dst = ut + 3 * (64 * t + local);
-    dst[0] = u[0];
-    dst[1] = u[1];
-    dst[2] = u[2];
+    ut[c] = u[0];
+    ut[lv.ncell + c] = u[1];
+    ut[2 * (size_t)lv.ncell + c] = u[2];
 }
 
-// ----------------------------------------------------- tile kernel (K4+K6)
-// One 64-thread workgroup per touched 4x4x4 tile (one cell per lane):
-//   * INTERP: the cell's velocity u = j/rho + F/2 is formed from its
-//     post-stream populations and the force of the last collision (the
-//     progs' computeVelocity pass, src/dataProcessors/dataAnalysisFunctional3D.hh:772-784;
-//     zero on bounce-back / no-dynamics cells, the wall velocity on
-//     velocity-wall cells), then for every vertex of the tile's list the
-//     workgroup sums phi phi phi u over its cells with a fixed butterfly
-//     and writes the partial to the vertex's slot q;
-//   * SPREAD: every cell sums g + phi phi phi F_k over the tile's vertices
-//     in increasing vertex order (the list is sorted in LDS first), the
-//     order of the reference's vertex loop
-//     (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes its
-//     force.
-// Vertex stencil data are staged in LDS 64 vertices at a time.
-template <class S, bool INTERP, bool SPREAD>
-__global__ void __launch_bounds__(64) k_tile(LatView lv, long NV, Grid gd, int Sw, const int* __restrict__ active,
-                                             int* __restrict__ status, const int* __restrict__ count,
-                                             const int* __restrict__ off, const int* __restrict__ list,
-                                             const int* __restrict__ base, const double* __restrict__ W,
-                                             const double* __restrict__ vforce, double* __restrict__ part,
-                                             const typename Pop<S>::C* __restrict__ ut,
-                                             typename Pop<S>::F* __restrict__ Fout)
+// ------------------------------------------------------- interpolation
+// U_k = sum over the stencil nodes of phi(dx) phi(dy) phi(dz) u, one thread
+// per vertex, nodes visited in the reference's loop order (x outer, z
+// inner; nodes outside a non-periodic face skipped), FP64 accumulation
+// (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:14-78).  With MOVE the vertex is
+// then advanced, x += U (explicit Euler, src/fsi/DeformableParticle3D.hh:26-36).
+template <class C, int SW, bool MOVE>
+__global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __restrict__ base,
+                                                const double* __restrict__ W, const C* __restrict__ ut,
+                                                double* __restrict__ vel, double* __restrict__ pos)
 {
-    using C = typename Pop<S>::C;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const size_t plane = (size_t)gd.n[0] * gd.n[1] * gd.n[2];
+    const int bx = base[3 * v], by = base[3 * v + 1], bz = base[3 * v + 2];
+    double w[3][SW];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < SW; ++k) w[a][k] = W[12 * v + 4 * a + k];
+    int zs[SW];
+#pragma unroll
+    for (int k = 0; k < SW; ++k) zs[k] = node_at(bz, k, gd.n[2], gd.per[2]);
+    double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+    for (int i = 0; i < SW; ++i) {
+        const int xi = node_at(bx, i, gd.n[0], gd.per[0]);
+        if (xi < 0) continue;
+#pragma unroll
+        for (int j = 0; j < SW; ++j) {
+            const int yj = node_at(by, j, gd.n[1], gd.per[1]);
+            if (yj < 0) continue;
+            const double wxy = w[0][i] * w[1][j];
+            const size_t row = ((size_t)xi * gd.n[1] + yj) * gd.n[2];
+#pragma unroll
+            for (int k = 0; k < SW; ++k) {
+                if (zs[k] < 0) continue;
+                const double wk = wxy * w[2][k];
+                const size_t c = row + zs[k];
+                a0 += wk * (double)ut[c];
+                a1 += wk * (double)ut[plane + c];
+                a2 += wk * (double)ut[2 * plane + c];
+            }
+        }
+    }
+    vel[3 * v] = a0;
+    vel[3 * v + 1] = a1;
+    vel[3 * v + 2] = a2;
+    if (MOVE) {
+        pos[3 * v] += a0;
+        pos[3 * v + 1] += a1;
+        pos[3 * v + 2] += a2;
+    }
+}
+
+// ----------------------------------------------------------- spreading
+// One 64-thread workgroup per touched 4x4x4 tile, one cell per lane: every
+// cell sums g + phi phi phi F_k over the tile's vertices in increasing vertex
+// order (the list is sorted in LDS first), the order of the reference's
+// vertex loop (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes
+// its force.  Vertex stencil data are staged in LDS 64 vertices at a time.
+template <class S>
+__global__ void __launch_bounds__(64) k_spread(LatView lv, Grid gd, int Sw, const int* __restrict__ active,
+                                               int* __restrict__ status, const int* __restrict__ count,
+                                               const int* __restrict__ off, const int* __restrict__ list,
+                                               const int* __restrict__ base, const double* __restrict__ W,
+                                               const double* __restrict__ vforce,
+                                               typename Pop<S>::F* __restrict__ Fout)
+{
     using FT = typename Pop<S>::F;
     __shared__ int keys[kMaxList];
     __shared__ int sb[3][64];
@@ -588,34 +746,25 @@ __global__ void __launch_bounds__(64) k_tile(LatView lv, long NV, Grid gd, int S
         while (n2 < n) n2 <<= 1;
         for (int k = lane; k < n2; k += 64) keys[k] = k < n ? list[o + k] : 0x7fffffff;
         __syncthreads();
-        if (SPREAD) {
-            for (int size = 2; size <= n2; size <<= 1)
-                for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
-                    for (int k = lane; k < n2; k += 64) {
-                        const int partner = k ^ stride2;
-                        if (partner > k) {
-                            const bool up = (k & size) == 0;
-                            const int a0 = keys[k], a1 = keys[partner];
-                            if ((a0 > a1) == up) {
-                                keys[k] = a1;
-                                keys[partner] = a0;
-                            }
+        for (int size = 2; size <= n2; size <<= 1)
+            for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
+                for (int k = lane; k < n2; k += 64) {
+                    const int partner = k ^ stride2;
+                    if (partner > k) {
+                        const bool up = (k & size) == 0;
+                        const int a0 = keys[k], a1 = keys[partner];
+                        if ((a0 > a1) == up) {
+                            keys[k] = a1;
+                            keys[partner] = a0;
                         }
                     }
-                    __syncthreads();
                 }
-        }
+                __syncthreads();
+            }
         const int tz = t % gd.nt[2], r = t / gd.nt[2], ty = r % gd.nt[1], tx = r / gd.nt[1];
         const int x = tx * kTile + (lane >> 4), y = ty * kTile + ((lane >> 2) & 3), z = tz * kTile + (lane & 3);
         const bool inside = x < gd.n[0] && y < gd.n[1] && z < gd.n[2];
         const long c = inside ? ((long)x * gd.n[1] + y) * gd.n[2] + z : 0;
-        double u0 = 0, u1 = 0, u2 = 0;
-        if (INTERP) {
-            const auto* up = ut + 3 * (64 * (long)t + lane);
-            u0 = (double)up[0];
-            u1 = (double)up[1];
-            u2 = (double)up[2];
-        }
         double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int jv = c0 + lane;
@@ -625,11 +774,9 @@ __global__ void __launch_bounds__(64) k_tile(LatView lv, long NV, Grid gd, int S
                 sb[1][lane] = base[3 * v + 1];
                 sb[2][lane] = base[3 * v + 2];
                 for (int q = 0; q < 12; ++q) sw[q][lane] = W[12 * (long)v + q];
-                if (SPREAD) {
-                    sf[0][lane] = vforce[3 * (long)v];
-                    sf[1][lane] = vforce[3 * (long)v + 1];
-                    sf[2][lane] = vforce[3 * (long)v + 2];
-                }
+                sf[0][lane] = vforce[3 * (long)v];
+                sf[1][lane] = vforce[3 * (long)v + 1];
+                sf[2][lane] = vforce[3 * (long)v + 2];
             }
             __syncthreads();
             const int cnt = min(64, n - c0);
@@ -642,24 +789,8 @@ __global__ void __launch_bounds__(64) k_tile(LatView lv, long NV, Grid gd, int S
                 if (gd.per[2] && dz < 0) dz += gd.n[2];
                 const bool hit = inside && (unsigned)dx < (unsigned)Sw && (unsigned)dy < (unsigned)Sw &&
                                  (unsigned)dz < (unsigned)Sw;
-                const double w = hit ? (sw[dx][jj] * sw[4 + dy][jj]) * sw[8 + dz][jj] : 0.0;
-                if (INTERP) {
-                    double p0 = w * u0, p1 = w * u1, p2 = w * u2;
-#pragma unroll
-                    for (int ofs = 32; ofs > 0; ofs >>= 1) {
-                        p0 += __shfl_xor(p0, ofs);
-                        p1 += __shfl_xor(p1, ofs);
-                        p2 += __shfl_xor(p2, ofs);
-                    }
-                    if (lane == 0) {
-                        const int key = keys[c0 + jj];
-                        double* dst = part + 3 * ((long)(key & 31) * NV + (key >> 5));
-                        dst[0] = p0;
-                        dst[1] = p1;
-                        dst[2] = p2;
-                    }
-                }
-                if (SPREAD && hit) {
+                if (hit) {
+                    const double w = (sw[dx][jj] * sw[4 + dy][jj]) * sw[8 + dz][jj];
                     acc0 += w * sf[0][jj];
                     acc1 += w * sf[1][jj];
                     acc2 += w * sf[2][jj];
@@ -667,39 +798,11 @@ __global__ void __launch_bounds__(64) k_tile(LatView lv, long NV, Grid gd, int S
             }
             __syncthreads();
         }
-        if (SPREAD && inside) {
+        if (inside) {
             Fout[c] = (FT)acc0;
             Fout[lv.stride + c] = (FT)acc1;
             Fout[2 * lv.stride + c] = (FT)acc2;
         }
-    }
-}
-
-// Vertex velocity: sum of its per-tile partials in tile order; with MOVE
-// the vertex is advanced, x += U (explicit Euler,
-// src/fsi/DeformableParticle3D.hh:26-36).
-template <bool MOVE>
-__global__ void __launch_bounds__(256) k_vertex_velocity(long NV, const unsigned char* __restrict__ vnt,
-                                                         const double* __restrict__ part, double* __restrict__ vel,
-                                                         double* __restrict__ pos)
-{
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const int m = vnt[v];
-    double a0 = 0, a1 = 0, a2 = 0;
-    for (int q = 0; q < m; ++q) {
-        const double* pp = part + 3 * ((long)q * NV + v);
-        a0 += pp[0];
-        a1 += pp[1];
-        a2 += pp[2];
-    }
-    vel[3 * v] = a0;
-    vel[3 * v + 1] = a1;
-    vel[3 * v + 2] = a2;
-    if (MOVE) {
-        pos[3 * v] += a0;
-        pos[3 * v + 1] += a1;
-        pos[3 * v + 2] += a2;
     }
 }
 
@@ -1017,7 +1120,7 @@ int do_bin(bl_ibm g)
                        g->n_active);
     if (g->NV)
         hipLaunchKernelGGL(k_bin_count, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
-                           (const double*)g->pos, g->v_base, g->v_w, g->v_nt, g->tile_count);
+                           (const double*)g->pos, g->v_base, g->v_w, g->tile_count);
     const long nb = (L->ntiles + kScanChunk - 1) / kScanChunk;
     unsigned long long* sums = (unsigned long long*)g->scan_blocks;
     hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles, sums);
@@ -1032,55 +1135,94 @@ int do_bin(bl_ibm g)
     return BL_OK;
 }
 
-template <class S, bool I, bool SP>
-void launch_tile_t(bl_ibm g, const LatView& lv, bool swapped, unsigned grid)
+// u on the cells of the touched tiles (before spreading overwrites the
+// force field this reads)
+template <class S>
+void launch_stage_s(bl_ibm g, const LatView& lv, bool swapped)
 {
     bl_lattice L = g->lat;
     using C = typename Pop<S>::C;
-    if (I)
-        hipLaunchKernelGGL(k_stage_u<S>, dim3((unsigned)((L->ncell + 255) / 256)), dim3(256), 0, L->stream, lv,
-                           swapped, (const int*)g->tile_count, (C*)g->ut);
-    hipLaunchKernelGGL((k_tile<S, I, SP>), dim3(grid), dim3(64), 0, L->stream, lv, g->NV, grid_of(L), g->S,
-                       (const int*)g->active, g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
-                       (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
-                       (const double*)g->force, g->part, (const C*)g->ut, (typename Pop<S>::F*)L->force);
+    const dim3 grid((unsigned)((L->ncell + 255) / 256));
+    if (swapped)
+        hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, L->stream, lv, (const int*)g->tile_count,
+                           (C*)g->ut);
+    else
+        hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, L->stream, lv, (const int*)g->tile_count,
+                           (C*)g->ut);
 }
 
-template <class S>
-void launch_tile_s(bl_ibm g, const LatView& lv, bool swapped, unsigned grid, bool interp, bool spread)
+LatView ibm_view(bl_ibm g)
 {
-    if (interp && spread) launch_tile_t<S, true, true>(g, lv, swapped, grid);
-    else if (interp) launch_tile_t<S, true, false>(g, lv, swapped, grid);
-    else launch_tile_t<S, false, true>(g, lv, swapped, grid);
+    LatView lv = view_of(g->lat);
+    if (!g->lat->force_field) lv.flags = nullptr;
+    return lv;
 }
 
-int launch_tile(bl_ibm g, bool interp, bool spread)
+int launch_stage(bl_ibm g)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
-    LatView lv = view_of(L);
-    if (!L->force_field) lv.flags = nullptr;
+    const LatView lv = ibm_view(g);
     const bool swapped = (L->steps & 1) != 0;
-    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 32768));
-    if (L->storage == BL_STORE_F64) launch_tile_s<double>(g, lv, swapped, grid, interp, spread);
-    else if (L->storage == BL_STORE_F32) launch_tile_s<float>(g, lv, swapped, grid, interp, spread);
-    else launch_tile_s<__half>(g, lv, swapped, grid, interp, spread);
+    if (L->storage == BL_STORE_F64) launch_stage_s<double>(g, lv, swapped);
+    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(g, lv, swapped);
+    else launch_stage_s<__half>(g, lv, swapped);
     BL_CHECK_LAUNCH();
-    if (spread) std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
     return BL_OK;
 }
 
-int vertex_velocity(bl_ibm g, bool move)
+template <class C, bool MOVE>
+void launch_interp_c(bl_ibm g)
 {
     hipStream_t s = g->lat->stream;
-    if (!g->NV) return BL_OK;
-    if (move)
-        hipLaunchKernelGGL(k_vertex_velocity<true>, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV,
-                           (const unsigned char*)g->v_nt, (const double*)g->part, g->vel, g->pos);
+    const Grid gd = grid_of(g->lat);
+    const dim3 grid(nblk(g->NV));
+    const C* ut = (const C*)g->ut;
+    if (g->S == 4)
+        hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
+                           (const double*)g->v_w, ut, g->vel, g->pos);
+    else if (g->S == 3)
+        hipLaunchKernelGGL((k_interp<C, 3, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
+                           (const double*)g->v_w, ut, g->vel, g->pos);
     else
-        hipLaunchKernelGGL(k_vertex_velocity<false>, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV,
-                           (const unsigned char*)g->v_nt, (const double*)g->part, g->vel, g->pos);
+        hipLaunchKernelGGL((k_interp<C, 2, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
+                           (const double*)g->v_w, ut, g->vel, g->pos);
+}
+
+// vertex velocities from the staged u; with `move` the vertices advance
+int launch_interp(bl_ibm g, bool move)
+{
+    if (g->NV == 0) return BL_OK;
+    const bool f64 = g->lat->storage == BL_STORE_F64;
+    if (f64 && move) launch_interp_c<double, true>(g);
+    else if (f64) launch_interp_c<double, false>(g);
+    else if (move) launch_interp_c<float, true>(g);
+    else launch_interp_c<float, false>(g);
     BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+template <class S>
+void launch_spread_s(bl_ibm g, const LatView& lv, unsigned grid)
+{
+    bl_lattice L = g->lat;
+    hipLaunchKernelGGL(k_spread<S>, dim3(grid), dim3(64), 0, L->stream, lv, grid_of(L), g->S,
+                       (const int*)g->active, g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
+                       (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
+                       (const double*)g->force, (typename Pop<S>::F*)L->force);
+}
+
+int launch_spread(bl_ibm g)
+{
+    bl_lattice L = g->lat;
+    if (g->NV == 0) return BL_OK;
+    const LatView lv = ibm_view(g);
+    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 32768));
+    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv, grid);
+    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, grid);
+    else launch_spread_s<__half>(g, lv, grid);
+    BL_CHECK_LAUNCH();
+    std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
     return BL_OK;
 }
 
@@ -1088,19 +1230,20 @@ int do_interpolate(bl_ibm g)
 {
     int rc;
     if ((rc = do_bin(g))) return rc;
-    if ((rc = launch_tile(g, true, false))) return rc;
-    return vertex_velocity(g, false);
+    if ((rc = launch_stage(g))) return rc;
+    return launch_interp(g, false);
 }
 
 int do_forces(bl_ibm g)
 {
     hipStream_t s = g->lat->stream;
     const ForceView f = force_view(g);
+    const long ne = g->NT + g->NP + g->NL;
     if (g->NV) {
-        if (g->max_incident <= 32)
-            hipLaunchKernelGGL(k_vertex_forces<32>, dim3(nblk(32 * g->NV)), dim3(256), 0, s, f, g->NV, g->force);
-        else
-            hipLaunchKernelGGL(k_vertex_forces<64>, dim3(nblk(64 * g->NV)), dim3(256), 0, s, f, g->NV, g->force);
+        hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair,
+                           g->ec_link, g->ec);
+        hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair, g->ec_link,
+                           (const double*)g->ec, g->force);
     }
     BL_CHECK_LAUNCH();
     return BL_OK;
@@ -1110,7 +1253,7 @@ int do_spread(bl_ibm g)
 {
     int rc;
     if (!g->binned && (rc = do_bin(g))) return rc;
-    return launch_tile(g, false, true);
+    return launch_spread(g);
 }
 
 int do_move(bl_ibm g)
@@ -1163,9 +1306,10 @@ int step_body(bl_ibm g)
     mark(g, 1);
     if ((rc = do_forces(g))) return rc;
     mark(g, 2);
-    if ((rc = launch_tile(g, true, true))) return rc;
+    if ((rc = launch_stage(g))) return rc;
+    if ((rc = launch_spread(g))) return rc;
+    if ((rc = launch_interp(g, true))) return rc;
     mark(g, 3);
-    if ((rc = vertex_velocity(g, true))) return rc;
     g->binned = false;
     if ((rc = update_particles(g, 1))) return rc;
     mark(g, 4);
@@ -1212,7 +1356,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->s_tri_area0, g->s_tri_geo0, g->s_rest, g->s_link, g->s_link_len, g->s_pair, g->s_pair_cs0,
                     g->s_vt_head, g->s_vt, g->s_vl_head, g->s_vl, g->s_vp_head, g->s_vp, g->s_area0, g->p_shape,
                     g->p_model, g->p_bug, g->p_params, g->p_voff, g->p_toff, g->p_loff, g->p_poff, g->p_geom,
-                    g->p_delta, g->pos, g->vel, g->force, g->v_nt, g->part, g->ut, g->v_base, g->v_w,
+                    g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_off, g->tile_fill, g->tile_list, g->active, g->n_active, g->scan_blocks,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
                     g->flags_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial};
@@ -1307,18 +1451,6 @@ int bl_ibm_init(bl_ibm g)
         s_area0v.push_back(s.area);
     }
     g->nshape = (int)g->shapes.size();
-    g->max_incident = 0;
-    for (const auto& p : g->parts) {
-        const ShapeHost& sh = g->shapes[p.shape];
-        for (int v = 0; v < sh.nv; ++v) {
-            int n = sh.v_tri_off[v + 1] - sh.v_tri_off[v];
-            if (p.model == BL_MODEL_RBC) n += sh.v_link_off[v + 1] - sh.v_link_off[v];
-            if (p.model == BL_MODEL_RBC || p.model == 3) n += sh.v_pair_off[v + 1] - sh.v_pair_off[v];
-            g->max_incident = std::max(g->max_incident, n);
-        }
-    }
-    if (g->max_incident > 64)
-        return fail(BL_ERR_CAPACITY, "bl_ibm_init: a vertex has more than 64 incident elements");
     if ((rc = upload(g->s_voff, s_voff)) || (rc = upload(g->s_toff, s_toff)) || (rc = upload(g->s_loff, s_loff)) ||
         (rc = upload(g->s_poff, s_poff)) || (rc = upload(g->s_vtoff, s_vtoff)) || (rc = upload(g->s_vloff, s_vloff)) ||
         (rc = upload(g->s_vpoff, s_vpoff)) || (rc = upload(g->s_tri, s_tri)) || (rc = upload(g->s_tri_area0, s_tarea)) ||
@@ -1353,11 +1485,18 @@ int bl_ibm_init(bl_ibm g)
         (rc = upload(g->p_poff, poff)) || (rc = upload(g->p_shape, pshape)) || (rc = upload(g->p_model, pmodel)) ||
         (rc = upload(g->p_bug, pbug)) || (rc = upload(g->p_params, params)) || (rc = upload(g->pos, pos)) ||
         (rc = alloc(g->vel, 3 * g->NV)) || (rc = alloc(g->force, 3 * g->NV)) || (rc = alloc(g->p_geom, 11 * (size_t)g->np)) ||
-        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) || (rc = alloc(g->v_nt, g->NV)) ||
-        (rc = alloc(g->part, 3 * 27 * g->NV)) ||
-        (rc = alloc(g->ut, 3 * 64 * L->ntiles)) ||
+        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) ||
+        (rc = alloc(g->ec, 9 * (size_t)g->NT + 12 * (size_t)g->NP + 3 * (size_t)g->NL)) ||
         (rc = alloc(g->v_base, 3 * g->NV)) || (rc = alloc(g->v_w, 12 * g->NV)))
         return rc;
+    g->ec_pair = 9 * g->NT;
+    g->ec_link = g->ec_pair + 12 * g->NP;
+    {
+        const size_t csz = L->storage == BL_STORE_F64 ? sizeof(double) : sizeof(float);
+        g->ut = nullptr;
+        BL_HIP(hipMalloc(&g->ut, 3 * csz * (size_t)L->ncell));
+        BL_HIP(hipMemset(g->ut, 0, 3 * csz * (size_t)L->ncell));
+    }
     BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     // tiles

@@ -25,7 +25,6 @@ struct bl_ibm_s {
     int np = 0;
     long NV = 0, NT = 0, NL = 0, NP = 0;  // totals over particles
     int nshape = 0;
-    int max_incident = 0;  // largest number of elements touching one vertex
 
     // ---- shape data (device, concatenated over shapes) ----
     int* s_voff = nullptr;  // nshape+1 (rest vertices, CSR heads)
@@ -67,10 +66,10 @@ struct bl_ibm_s {
     double* pos = nullptr;
     double* vel = nullptr;
     double* force = nullptr;
-    unsigned char* v_nt = nullptr;  // tiles touched by each vertex's stencil (<= 27)
-    double* part = nullptr;         // 27 x 3 per vertex, [q][v][3]: per-tile interpolation partials
-    double* ut = nullptr;           // staged velocity, tile-major: 64 cells x 3 per tile (FP64 / FP32 by storage)
-    bool binned = false;            // tile lists match the current positions
+    double* ec = nullptr;  // per-element force contributions: 9 per triangle, 12 per pair, 3 per link
+    long ec_pair = 0, ec_link = 0;  // offsets of the pair and link sections in ec
+    void* ut = nullptr;    // staged velocity, 3 planes of ncell (FP64 for FP64 storage, else FP32)
+    bool binned = false;   // tile lists match the current positions
 
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
