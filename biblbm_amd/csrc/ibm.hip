@@ -36,17 +36,6 @@ constexpr int kMaxList = 1024;     // vertices per tile the spreading kernel sor
 constexpr int kScanChunk = 1024;   // items per scan block
 
 // ------------------------------------------------------------- helpers
-__device__ __forceinline__ int find_part(const long* off, int np, long g)
-{
-    int lo = 0, hi = np;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Delta kernels (src/fsi/Dirac.h:23-66,74-101), 1-D factor.
 __device__ __forceinline__ double dirac(int kernel, double x)
 {
@@ -377,6 +366,7 @@ struct ForceView {
     const int *s_vt_head, *s_vt, *s_vl_head, *s_vl, *s_vp_head, *s_vp;
     const double* s_area0;
     const double* pos;
+    const int *v_part, *t_part, *r_part, *l_part;  // particle of each vertex / triangle / pair / link
 };
 
 // Skalak / NeoHookean in-plane forces of one triangle
@@ -537,13 +527,13 @@ __global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long 
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < NT) {
-        const int p = find_part(f.p_toff, f.np, e);
+        const int p = f.t_part[e];
         D3 out[3];
         tri_forces(f, p, f.p_shape[p], f.p_model[p], f.p_voff[p], f.p_params + 8 * p, e - f.p_toff[p], out);
         for (int k = 0; k < 3; ++k) st3(ec + 9 * e + 3 * k, out[k]);
     } else if (e < NT + NP) {
         const long g = e - NT;
-        const int p = find_part(f.p_poff, f.np, g);
+        const int p = f.r_part[g];
         const int model = f.p_model[p];
         if (model != BL_MODEL_RBC && model != 3) return;
         D3 out[4];
@@ -551,87 +541,107 @@ __global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long 
         for (int k = 0; k < 4; ++k) st3(ec + ec_pair + 12 * g + 3 * k, out[k]);
     } else if (e < NT + NP + NL) {
         const long g = e - NT - NP;
-        const int p = find_part(f.p_loff, f.np, g);
+        const int p = f.l_part[g];
         if (f.p_model[p] != BL_MODEL_RBC) return;
         st3(ec + ec_link + 3 * g, link_force(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_loff[p]));
     }
 }
 
-// Force on every vertex: the contributions of its incident elements summed
-// in the reference's accumulation order -- links
-// (src/fsi/RBCParticle.hh:149-171), then triangles in triangle order
-// (src/fsi/DeformableCapsule.hh:60-150, src/fsi/RBCParticle.hh:176-209),
-// then pairs in pair order (src/fsi/RBCParticle.hh:214-246).  The incidence
-// lists hold element * 4 + the vertex's slot in the element, sorted by
-// element.  No atomics.
+// Force on every vertex: the contributions of its incident elements --
+// links (src/fsi/RBCParticle.hh:149-171), triangles
+// (src/fsi/DeformableCapsule.hh:60-150, src/fsi/RBCParticle.hh:176-209) and
+// pairs (src/fsi/RBCParticle.hh:214-246) -- from the per-element buffer.
+// The incidence lists hold element * 4 + the vertex's slot in the element.
+// Four lanes per vertex: lane l sums entries l, l+4, ... of the vertex's
+// list [links | triangles | pairs] in order, then a fixed butterfly adds the
+// lanes, so the result does not depend on scheduling.  No atomics.
+constexpr int kGatherLanes = 4;
+
 __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, long ec_pair, long ec_link,
                                                        const double* __restrict__ ec, double* __restrict__ force)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const int p = find_part(f.p_voff, f.np, v);
-    const int s = f.p_shape[p];
-    const int model = f.p_model[p];
-    const long vo = f.p_voff[p];
-    const long sv = f.s_voff[s] + (v - vo);
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long v = gid / kGatherLanes;
+    const int lane = (int)(gid % kGatherLanes);
     double c0 = 0, c1 = 0, c2 = 0;
-    if (model == BL_MODEL_RBC) {
-        const int* h = f.s_vl_head + sv + s;
-        const int* L = f.s_vl + f.s_vloff[s];
-        const double* E = ec + ec_link + 3 * f.p_loff[p];
-        for (int k = h[0]; k < h[1]; ++k) {
-            const int code = L[k];
-            const double* x = E + 3 * (code >> 2);
-            if ((code & 3) == 0) {
-                c0 += x[0]; c1 += x[1]; c2 += x[2];
+    if (v < NV) {
+        const int p = f.v_part[v];
+        const int s = f.p_shape[p];
+        const int model = f.p_model[p];
+        const long sv = f.s_voff[s] + (v - f.p_voff[p]);
+        const int* hl = f.s_vl_head + sv + s;
+        const int* ht = f.s_vt_head + sv + s;
+        const int* hp = f.s_vp_head + sv + s;
+        const int nl = model == BL_MODEL_RBC ? hl[1] - hl[0] : 0;
+        const int nt = ht[1] - ht[0];
+        const int npr = (model == BL_MODEL_RBC || model == 3) ? hp[1] - hp[0] : 0;
+        const int* Ll = f.s_vl + f.s_vloff[s] + hl[0];
+        const int* Lt = f.s_vt + f.s_vtoff[s] + ht[0];
+        const int* Lp = f.s_vp + f.s_vpoff[s] + hp[0];
+        const double* El = ec + ec_link + 3 * f.p_loff[p];
+        const double* Et = ec + 9 * f.p_toff[p];
+        const double* Ep = ec + ec_pair + 12 * f.p_poff[p];
+        for (int e = lane; e < nl + nt + npr; e += kGatherLanes) {
+            if (e < nl) {
+                const int code = Ll[e];
+                const double* x = El + 3 * (code >> 2);
+                const double sg = (code & 3) == 0 ? 1.0 : -1.0;
+                c0 += sg * x[0];
+                c1 += sg * x[1];
+                c2 += sg * x[2];
+            } else if (e < nl + nt) {
+                const int code = Lt[e - nl];
+                const double* x = Et + 9 * (code >> 2) + 3 * (code & 3);
+                c0 += x[0];
+                c1 += x[1];
+                c2 += x[2];
             } else {
-                c0 += -x[0]; c1 += -x[1]; c2 += -x[2];
+                const int code = Lp[e - nl - nt];
+                const double* x = Ep + 12 * (code >> 2) + 3 * (code & 3);
+                c0 += x[0];
+                c1 += x[1];
+                c2 += x[2];
             }
         }
     }
-    {
-        const int* h = f.s_vt_head + sv + s;
-        const int* L = f.s_vt + f.s_vtoff[s];
-        const double* E = ec + 9 * f.p_toff[p];
-        for (int k = h[0]; k < h[1]; ++k) {
-            const int code = L[k];
-            const double* x = E + 9 * (code >> 2) + 3 * (code & 3);
-            c0 += x[0]; c1 += x[1]; c2 += x[2];
-        }
+#pragma unroll
+    for (int o = 1; o < kGatherLanes; o <<= 1) {
+        c0 += __shfl_xor(c0, o);
+        c1 += __shfl_xor(c1, o);
+        c2 += __shfl_xor(c2, o);
     }
-    if (model == BL_MODEL_RBC || model == 3) {
-        const int* h = f.s_vp_head + sv + s;
-        const int* L = f.s_vp + f.s_vpoff[s];
-        const double* E = ec + ec_pair + 12 * f.p_poff[p];
-        for (int k = h[0]; k < h[1]; ++k) {
-            const int code = L[k];
-            const double* x = E + 12 * (code >> 2) + 3 * (code & 3);
-            c0 += x[0]; c1 += x[1]; c2 += x[2];
-        }
+    if (lane == 0 && v < NV) {
+        force[3 * v] = c0;
+        force[3 * v + 1] = c1;
+        force[3 * v + 2] = c2;
     }
-    force[3 * v] = c0;
-    force[3 * v + 1] = c1;
-    force[3 * v + 2] = c2;
 }
 
 // ------------------------------------------------- velocity staging
+// Staged velocity: one aligned 4-vector per cell (u, 0) so that a z-row of
+// a stencil is one contiguous run of memory.
+template <class C>
+struct alignas(4 * sizeof(C)) Vel4 {
+    C x, y, z, w;
+};
+
 // The progs' computeVelocity pass (u = j/rho + F/2 from the post-stream
 // populations and the last collision's force,
 // src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
 // bounce-back / no-dynamics cells, the wall velocity on velocity-wall
 // cells), restricted to the cells of the tiles this step's stencils touch.
 // One thread per cell in z-fastest order, so population reads coalesce as
-// in the collide kernel and the three u planes are written contiguously.
+// in the collide kernel and the staged vectors are written contiguously.
 template <class S, bool SWAPPED>
 __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const int* __restrict__ count,
-                                                 typename Pop<S>::C* __restrict__ ut)
+                                                 Vel4<typename Pop<S>::C>* __restrict__ ut)
 {
     using C = typename Pop<S>::C;
     using FT = typename Pop<S>::F;
     const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= lv.ncell) return;
     int x, y, z;
-    decode(c, lv.ny, lv.nz, x, y, z);
+    decode(c, lv, x, y, z);
     const unsigned t = ((unsigned)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
     if (count[t] == 0) return;
     const unsigned char m = lv.mask ? lv.mask[c] : (unsigned char)M_FLUID;
@@ -653,88 +663,112 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const int* __restri
         const C invRho = C(1) / (rb + C(1));
         for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
     }
-    ut[c] = u[0];
-    ut[lv.ncell + c] = u[1];
-    ut[2 * (size_t)lv.ncell + c] = u[2];
+    ut[c] = Vel4<C>{u[0], u[1], u[2], C(0)};
 }
 
 // ------------------------------------------------------- interpolation
-// U_k = sum over the stencil nodes of phi(dx) phi(dy) phi(dz) u, one thread
-// per vertex, nodes visited in the reference's loop order (x outer, z
-// inner; nodes outside a non-periodic face skipped), FP64 accumulation
-// (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:14-78).  With MOVE the vertex is
-// then advanced, x += U (explicit Euler, src/fsi/DeformableParticle3D.hh:26-36).
+// U_k = sum over the stencil nodes of phi(dx) phi(dy) phi(dz) u
+// (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:14-78): a group of
+// lanes per vertex, lane (j, k) walks the x-line of its (y, z) node pair
+// (nodes outside a non-periodic face skipped, weight (phi_x phi_y) phi_z as
+// the reference forms it), then the lanes are added by a fixed butterfly,
+// all in FP64.  The lanes of one x-step read consecutive cells of a few
+// z-rows.  With MOVE the vertex is then advanced, x += U (explicit Euler,
+// src/fsi/DeformableParticle3D.hh:26-36).
 template <class C, int SW, bool MOVE>
 __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __restrict__ base,
-                                                const double* __restrict__ W, const C* __restrict__ ut,
+                                                const double* __restrict__ W, const Vel4<C>* __restrict__ ut,
                                                 double* __restrict__ vel, double* __restrict__ pos)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const size_t plane = (size_t)gd.n[0] * gd.n[1] * gd.n[2];
-    const int bx = base[3 * v], by = base[3 * v + 1], bz = base[3 * v + 2];
-    double w[3][SW];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int k = 0; k < SW; ++k) w[a][k] = W[12 * v + 4 * a + k];
-    int zs[SW];
-#pragma unroll
-    for (int k = 0; k < SW; ++k) zs[k] = node_at(bz, k, gd.n[2], gd.per[2]);
+    constexpr int LA = SW == 2 ? 2 : 4;
+    constexpr int LPV = LA * LA;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long v = gid / LPV;
+    const int r = (int)(gid % LPV);
+    const int j = r / LA, k = r % LA;
     double a0 = 0, a1 = 0, a2 = 0;
+    if (v < NV && j < SW && k < SW) {
+        const int yj = node_at(base[3 * v + 1], j, gd.n[1], gd.per[1]);
+        const int zk = node_at(base[3 * v + 2], k, gd.n[2], gd.per[2]);
+        if (yj >= 0 && zk >= 0) {
+            const double* Wv = W + 12 * v;
+            const double wy = Wv[4 + j], wz = Wv[8 + k];
+            const int bx = base[3 * v];
 #pragma unroll
-    for (int i = 0; i < SW; ++i) {
-        const int xi = node_at(bx, i, gd.n[0], gd.per[0]);
-        if (xi < 0) continue;
-#pragma unroll
-        for (int j = 0; j < SW; ++j) {
-            const int yj = node_at(by, j, gd.n[1], gd.per[1]);
-            if (yj < 0) continue;
-            const double wxy = w[0][i] * w[1][j];
-            const size_t row = ((size_t)xi * gd.n[1] + yj) * gd.n[2];
-#pragma unroll
-            for (int k = 0; k < SW; ++k) {
-                if (zs[k] < 0) continue;
-                const double wk = wxy * w[2][k];
-                const size_t c = row + zs[k];
-                a0 += wk * (double)ut[c];
-                a1 += wk * (double)ut[plane + c];
-                a2 += wk * (double)ut[2 * plane + c];
+            for (int i = 0; i < SW; ++i) {
+                const int xi = node_at(bx, i, gd.n[0], gd.per[0]);
+                if (xi < 0) continue;
+                const double w = (Wv[i] * wy) * wz;
+                const Vel4<C> u = ut[((size_t)xi * gd.n[1] + yj) * gd.n[2] + zk];
+                a0 += w * (double)u.x;
+                a1 += w * (double)u.y;
+                a2 += w * (double)u.z;
             }
         }
     }
-    vel[3 * v] = a0;
-    vel[3 * v + 1] = a1;
-    vel[3 * v + 2] = a2;
-    if (MOVE) {
-        pos[3 * v] += a0;
-        pos[3 * v + 1] += a1;
-        pos[3 * v + 2] += a2;
+#pragma unroll
+    for (int o = 1; o < LPV; o <<= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+        a2 += __shfl_xor(a2, o);
+    }
+    if (r == 0 && v < NV) {
+        vel[3 * v] = a0;
+        vel[3 * v + 1] = a1;
+        vel[3 * v + 2] = a2;
+        if (MOVE) {
+            pos[3 * v] += a0;
+            pos[3 * v + 1] += a1;
+            pos[3 * v + 2] += a2;
+        }
     }
 }
 
 // ----------------------------------------------------------- spreading
-// One 64-thread workgroup per touched 4x4x4 tile, one cell per lane: every
-// cell sums g + phi phi phi F_k over the tile's vertices in increasing vertex
-// order (the list is sorted in LDS first), the order of the reference's
-// vertex loop (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes
-// its force.  Vertex stencil data are staged in LDS 64 vertices at a time.
+// One 64-lane wave per touched 4x4x4 tile, one cell per lane: every cell
+// sums g + phi phi phi F_k over the tile's vertices in increasing vertex
+// order, the order of the reference's vertex loop
+// (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes its force.
+// The tile's list is sorted first (in registers when it has at most 64
+// entries, in LDS otherwise).  Then, kChunk entries at a time, lane e forms
+// entry e's weights on the tile's own 4 cells per axis (zero outside the
+// stencil) and puts them with the entry's force into LDS; every cell then
+// walks the chunk in order with broadcast LDS reads.
+constexpr int kChunk = 32;
+constexpr int kSpreadWaves = 4;  // waves per workgroup, each on its own tiles
+
+// Orders the LDS accesses of one wave (a wave's LDS operations complete in
+// issue order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class S>
-__global__ void __launch_bounds__(64) k_spread(LatView lv, Grid gd, int Sw, const int* __restrict__ active,
-                                               int* __restrict__ status, const int* __restrict__ count,
-                                               const int* __restrict__ off, const int* __restrict__ list,
-                                               const int* __restrict__ base, const double* __restrict__ W,
-                                               const double* __restrict__ vforce,
-                                               typename Pop<S>::F* __restrict__ Fout)
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid gd, int Sw,
+                                                              const int* __restrict__ active,
+                                                              int* __restrict__ status, const int* __restrict__ count,
+                                                              const int* __restrict__ off,
+                                                              const int* __restrict__ list,
+                                                              const int* __restrict__ base,
+                                                              const double* __restrict__ W,
+                                                              const double* __restrict__ vforce,
+                                                              typename Pop<S>::F* __restrict__ Fout)
 {
     using FT = typename Pop<S>::F;
-    __shared__ int keys[kMaxList];
-    __shared__ int sb[3][64];
-    __shared__ double sw[12][64];
-    __shared__ double sf[3][64];
+    __shared__ int keys_all[kSpreadWaves][kMaxList];
+    // per entry e: ew[0..11][e] the x, y, z weights on the tile's 4 cells per
+    // axis, ew[12..14][e] the force (rows padded against bank conflicts)
+    __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
+    const int wv = threadIdx.x >> 6;
+    int* keys = keys_all[wv];
+    double(*ew)[kChunk + 1] = ew_all[wv];
     const int na = status[0];
-    const int lane = threadIdx.x;
-    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+    for (int a = blockIdx.x * kSpreadWaves + wv; a < na; a += gridDim.x * kSpreadWaves) {
         const int t = active[a];
         int n = count[t];
         if (n > kMaxList) {  // reported by the host check; the tile keeps only the first entries
@@ -742,63 +776,73 @@ __global__ void __launch_bounds__(64) k_spread(LatView lv, Grid gd, int Sw, cons
             n = kMaxList;
         }
         const int o = off[t];
-        int n2 = 1;
-        while (n2 < n) n2 <<= 1;
-        for (int k = lane; k < n2; k += 64) keys[k] = k < n ? list[o + k] : 0x7fffffff;
-        __syncthreads();
-        for (int size = 2; size <= n2; size <<= 1)
-            for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
-                for (int k = lane; k < n2; k += 64) {
-                    const int partner = k ^ stride2;
-                    if (partner > k) {
-                        const bool up = (k & size) == 0;
-                        const int a0 = keys[k], a1 = keys[partner];
-                        if ((a0 > a1) == up) {
-                            keys[k] = a1;
-                            keys[partner] = a0;
+        int key = lane < n ? list[o + lane] : 0x7fffffff;
+        if (n <= 64) {
+#pragma unroll
+            for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    const int other = __shfl_xor(key, stride);
+                    const bool up = (lane & size) == 0;
+                    const bool lower = (lane & stride) == 0;
+                    key = (lower == up) ? min(key, other) : max(key, other);
+                }
+        } else {
+            int n2 = 1;
+            while (n2 < n) n2 <<= 1;
+            for (int k = lane; k < n2; k += 64) keys[k] = k < n ? list[o + k] : 0x7fffffff;
+            wave_sync();
+            for (int size = 2; size <= n2; size <<= 1)
+                for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
+                    for (int k = lane; k < n2; k += 64) {
+                        const int partner = k ^ stride2;
+                        if (partner > k) {
+                            const bool up = (k & size) == 0;
+                            const int a0 = keys[k], a1 = keys[partner];
+                            if ((a0 > a1) == up) {
+                                keys[k] = a1;
+                                keys[partner] = a0;
+                            }
                         }
                     }
+                    wave_sync();
                 }
-                __syncthreads();
-            }
-        const int tz = t % gd.nt[2], r = t / gd.nt[2], ty = r % gd.nt[1], tx = r / gd.nt[1];
-        const int x = tx * kTile + (lane >> 4), y = ty * kTile + ((lane >> 2) & 3), z = tz * kTile + (lane & 3);
-        const bool inside = x < gd.n[0] && y < gd.n[1] && z < gd.n[2];
-        const long c = inside ? ((long)x * gd.n[1] + y) * gd.n[2] + z : 0;
-        double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
-        for (int c0 = 0; c0 < n; c0 += 64) {
-            const int jv = c0 + lane;
-            if (jv < n) {
-                const int v = keys[jv] >> 5;
-                sb[0][lane] = base[3 * v];
-                sb[1][lane] = base[3 * v + 1];
-                sb[2][lane] = base[3 * v + 2];
-                for (int q = 0; q < 12; ++q) sw[q][lane] = W[12 * (long)v + q];
-                sf[0][lane] = vforce[3 * (long)v];
-                sf[1][lane] = vforce[3 * (long)v + 1];
-                sf[2][lane] = vforce[3 * (long)v + 2];
-            }
-            __syncthreads();
-            const int cnt = min(64, n - c0);
-            for (int jj = 0; jj < cnt; ++jj) {
-                int dx = x - sb[0][jj];
-                if (gd.per[0] && dx < 0) dx += gd.n[0];
-                int dy = y - sb[1][jj];
-                if (gd.per[1] && dy < 0) dy += gd.n[1];
-                int dz = z - sb[2][jj];
-                if (gd.per[2] && dz < 0) dz += gd.n[2];
-                const bool hit = inside && (unsigned)dx < (unsigned)Sw && (unsigned)dy < (unsigned)Sw &&
-                                 (unsigned)dz < (unsigned)Sw;
-                if (hit) {
-                    const double w = (sw[dx][jj] * sw[4 + dy][jj]) * sw[8 + dz][jj];
-                    acc0 += w * sf[0][jj];
-                    acc1 += w * sf[1][jj];
-                    acc2 += w * sf[2][jj];
-                }
-            }
-            __syncthreads();
         }
-        if (inside) {
+        const int tz = t % gd.nt[2], rr = t / gd.nt[2], ty = rr % gd.nt[1], tx = rr / gd.nt[1];
+        const int org[3] = {tx * kTile, ty * kTile, tz * kTile};
+        double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
+        for (int c0 = 0; c0 < n; c0 += kChunk) {
+            const int jv = c0 + lane;
+            const int kk = n <= 64 ? __shfl(key, jv & 63) : keys[min(jv, n - 1)];
+            if (lane < kChunk && jv < n) {
+                const long v = kk >> 5;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int b = base[3 * v + ax];
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        int d = org[ax] + l - b;
+                        if (gd.per[ax] && d < 0) d += gd.n[ax];
+                        ew[4 * ax + l][lane] = (unsigned)d < (unsigned)Sw ? W[12 * v + 4 * ax + d] : 0.0;
+                    }
+                }
+                ew[12][lane] = vforce[3 * v];
+                ew[13][lane] = vforce[3 * v + 1];
+                ew[14][lane] = vforce[3 * v + 2];
+            }
+            wave_sync();
+            const int cnt = min(kChunk, n - c0);
+            for (int e = 0; e < cnt; ++e) {
+                const double w = (ew[lx][e] * ew[4 + ly][e]) * ew[8 + lz][e];
+                acc0 += w * ew[12][e];
+                acc1 += w * ew[13][e];
+                acc2 += w * ew[14][e];
+            }
+            wave_sync();
+        }
+        const int x = org[0] + lx, y = org[1] + ly, z = org[2] + lz;
+        if (x < gd.n[0] && y < gd.n[1] && z < gd.n[2]) {
+            const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
             Fout[c] = (FT)acc0;
             Fout[lv.stride + c] = (FT)acc1;
             Fout[2 * lv.stride + c] = (FT)acc2;
@@ -913,11 +957,12 @@ __global__ void k_geom_final(int np, Grid gd, const int* __restrict__ p_choff, c
     }
 }
 
-__global__ void k_apply_shift(const long* p_voff, int np, long NV, const double* delta, double* pos)
+__global__ void k_apply_shift(const int* __restrict__ v_part, long NV, const double* __restrict__ delta,
+                              double* __restrict__ pos)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= NV) return;
-    const int p = find_part(p_voff, np, v);
+    const int p = v_part[v];
     for (int d = 0; d < 3; ++d) pos[3 * v + d] += delta[3 * p + d];
 }
 
@@ -931,7 +976,7 @@ __global__ void k_ray_hits(ForceView f, Grid gd, long NT, int pass, int* col_cou
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= NT) return;
-    const int p = find_part(f.p_toff, f.np, g);
+    const int p = f.t_part[g];
     const int s = f.p_shape[p];
     const long vo = f.p_voff[p];
     const int* tri = f.s_tri + 3 * (f.s_toff[s] + (g - f.p_toff[p]));
@@ -1076,6 +1121,7 @@ ForceView force_view(bl_ibm g)
     f.s_vp_head = g->s_vp_head; f.s_vp = g->s_vp;
     f.s_area0 = g->s_area0;
     f.pos = g->pos;
+    f.v_part = g->v_part; f.t_part = g->t_part; f.r_part = g->r_part; f.l_part = g->l_part;
     return f;
 }
 
@@ -1104,7 +1150,7 @@ int update_particles(bl_ibm g, int remap)
     hipLaunchKernelGGL(k_geom_final, dim3(nblk(g->np, 64)), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
                        (const double*)g->ch_partial, g->p_geom, g->p_delta, remap);
     if (remap)
-        hipLaunchKernelGGL(k_apply_shift, dim3(nblk(g->NV)), dim3(256), 0, s, (const long*)g->p_voff, g->np, g->NV,
+        hipLaunchKernelGGL(k_apply_shift, dim3(nblk(g->NV)), dim3(256), 0, s, (const int*)g->v_part, g->NV,
                            (const double*)g->p_delta, g->pos);
     BL_CHECK_LAUNCH();
     return BL_OK;
@@ -1141,14 +1187,14 @@ template <class S>
 void launch_stage_s(bl_ibm g, const LatView& lv, bool swapped)
 {
     bl_lattice L = g->lat;
-    using C = typename Pop<S>::C;
+    using V = Vel4<typename Pop<S>::C>;
     const dim3 grid((unsigned)((L->ncell + 255) / 256));
     if (swapped)
         hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, L->stream, lv, (const int*)g->tile_count,
-                           (C*)g->ut);
+                           (V*)g->ut);
     else
         hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, L->stream, lv, (const int*)g->tile_count,
-                           (C*)g->ut);
+                           (V*)g->ut);
 }
 
 LatView ibm_view(bl_ibm g)
@@ -1176,8 +1222,8 @@ void launch_interp_c(bl_ibm g)
 {
     hipStream_t s = g->lat->stream;
     const Grid gd = grid_of(g->lat);
-    const dim3 grid(nblk(g->NV));
-    const C* ut = (const C*)g->ut;
+    const dim3 grid(nblk((g->S == 2 ? 4 : 16) * g->NV));
+    const Vel4<C>* ut = (const Vel4<C>*)g->ut;
     if (g->S == 4)
         hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
                            (const double*)g->v_w, ut, g->vel, g->pos);
@@ -1203,10 +1249,12 @@ int launch_interp(bl_ibm g, bool move)
 }
 
 template <class S>
-void launch_spread_s(bl_ibm g, const LatView& lv, unsigned grid)
+void launch_spread_s(bl_ibm g, const LatView& lv)
 {
     bl_lattice L = g->lat;
-    hipLaunchKernelGGL(k_spread<S>, dim3(grid), dim3(64), 0, L->stream, lv, grid_of(L), g->S,
+    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves,
+                                                                      8192));
+    hipLaunchKernelGGL(k_spread<S>, dim3(grid), dim3(64 * kSpreadWaves), 0, L->stream, lv, grid_of(L), g->S,
                        (const int*)g->active, g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
                        (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
                        (const double*)g->force, (typename Pop<S>::F*)L->force);
@@ -1217,10 +1265,9 @@ int launch_spread(bl_ibm g)
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
     const LatView lv = ibm_view(g);
-    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 32768));
-    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv, grid);
-    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, grid);
-    else launch_spread_s<__half>(g, lv, grid);
+    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv);
+    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv);
+    else launch_spread_s<__half>(g, lv);
     BL_CHECK_LAUNCH();
     std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
     return BL_OK;
@@ -1242,7 +1289,8 @@ int do_forces(bl_ibm g)
     if (g->NV) {
         hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair,
                            g->ec_link, g->ec);
-        hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair, g->ec_link,
+        hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
+                           g->ec_link,
                            (const double*)g->ec, g->force);
     }
     BL_CHECK_LAUNCH();
@@ -1359,7 +1407,8 @@ int bl_ibm_destroy(bl_ibm g)
                     g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_off, g->tile_fill, g->tile_list, g->active, g->n_active, g->scan_blocks,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
-                    g->flags_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial};
+                    g->flags_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_part, g->t_part, g->r_part,
+                    g->l_part};
     for (void* p : ptrs) hipFree(p);
     for (auto& e : g->ev)
         if (e) hipEventDestroy(e);
@@ -1481,6 +1530,22 @@ int bl_ibm_init(bl_ibm g)
     g->NT = toff.back();
     g->NL = loff.back();
     g->NP = poff.back();
+    {
+        std::vector<int> vp, tp, rp, lp;
+        vp.reserve(g->NV);
+        tp.reserve(g->NT);
+        rp.reserve(g->NP);
+        lp.reserve(g->NL);
+        for (int p = 0; p < g->np; ++p) {
+            vp.insert(vp.end(), voff[p + 1] - voff[p], p);
+            tp.insert(tp.end(), toff[p + 1] - toff[p], p);
+            rp.insert(rp.end(), poff[p + 1] - poff[p], p);
+            lp.insert(lp.end(), loff[p + 1] - loff[p], p);
+        }
+        if ((rc = upload(g->v_part, vp)) || (rc = upload(g->t_part, tp)) || (rc = upload(g->r_part, rp)) ||
+            (rc = upload(g->l_part, lp)))
+            return rc;
+    }
     if ((rc = upload(g->p_voff, voff)) || (rc = upload(g->p_toff, toff)) || (rc = upload(g->p_loff, loff)) ||
         (rc = upload(g->p_poff, poff)) || (rc = upload(g->p_shape, pshape)) || (rc = upload(g->p_model, pmodel)) ||
         (rc = upload(g->p_bug, pbug)) || (rc = upload(g->p_params, params)) || (rc = upload(g->pos, pos)) ||
@@ -1492,10 +1557,10 @@ int bl_ibm_init(bl_ibm g)
     g->ec_pair = 9 * g->NT;
     g->ec_link = g->ec_pair + 12 * g->NP;
     {
-        const size_t csz = L->storage == BL_STORE_F64 ? sizeof(double) : sizeof(float);
+        const size_t csz = L->storage == BL_STORE_F64 ? sizeof(Vel4<double>) : sizeof(Vel4<float>);
         g->ut = nullptr;
-        BL_HIP(hipMalloc(&g->ut, 3 * csz * (size_t)L->ncell));
-        BL_HIP(hipMemset(g->ut, 0, 3 * csz * (size_t)L->ncell));
+        BL_HIP(hipMalloc(&g->ut, csz * (size_t)L->ncell));
+        BL_HIP(hipMemset(g->ut, 0, csz * (size_t)L->ncell));
     }
     BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));

@@ -61,6 +61,10 @@ struct bl_ibm_s {
     long* p_poff = nullptr;
     double* p_geom = nullptr;    // 11 per particle: area, volume, com(3), bbox(6)
     double* p_delta = nullptr;   // 3 per particle (COM remap shift)
+    int* v_part = nullptr;       // particle of each vertex, triangle, pair, link
+    int* t_part = nullptr;
+    int* r_part = nullptr;
+    int* l_part = nullptr;
 
     // ---- vertices ----
     double* pos = nullptr;
@@ -68,7 +72,7 @@ struct bl_ibm_s {
     double* force = nullptr;
     double* ec = nullptr;  // per-element force contributions: 9 per triangle, 12 per pair, 3 per link
     long ec_pair = 0, ec_link = 0;  // offsets of the pair and link sections in ec
-    void* ut = nullptr;    // staged velocity, 3 planes of ncell (FP64 for FP64 storage, else FP32)
+    void* ut = nullptr;    // staged velocity, (u, 0) per cell (FP64 for FP64 storage, else FP32)
     bool binned = false;   // tile lists match the current positions
 
     // ---- stencil / tiles ----

@@ -26,10 +26,20 @@
 
 namespace bl {
 
+FastDiv make_fastdiv(unsigned d)
+{
+    FastDiv f;
+    while ((1ull << f.l) < d) ++f.l;
+    f.m = (unsigned)((((unsigned long long)1 << 32) * ((1ull << f.l) - d)) / d + 1);
+    return f;
+}
+
 LatView view_of(const bl_lattice_s* L)
 {
     LatView v;
     v.nx = L->nx; v.ny = L->ny; v.nz = L->nz;
+    v.dy = make_fastdiv((unsigned)L->ny);
+    v.dz = make_fastdiv((unsigned)L->nz);
     v.ncell = (unsigned)L->ncell;
     v.stride = L->stride;
     for (int d = 0; d < 3; ++d) v.per[d] = L->periodic[d];
@@ -62,7 +72,7 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
     const unsigned c = bid * blockDim.x + threadIdx.x;
     if (c >= v.ncell) return;
     int x, y, z;
-    decode(c, v.ny, v.nz, x, y, z);
+    decode(c, v, x, y, z);
     S* __restrict__ A = (S*)v.pops;
     const long st = v.stride;
 

@@ -48,8 +48,18 @@ struct bl_lattice_s {
 
 namespace bl {
 
+// n / d for 0 <= n < 2^31 without a hardware divide: q = (t + ((n - t) >> 1))
+// >> (l - 1) with t = mulhi(m, n), l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1 (Granlund & Montgomery, PLDI 1994, sec. 4).
+struct FastDiv {
+    unsigned m = 0;
+    int l = 0;
+};
+FastDiv make_fastdiv(unsigned d);
+
 struct LatView {
     int nx, ny, nz;
+    FastDiv dy, dz;  // division by ny and by nz
     unsigned ncell;
     long stride;
     int per[3];

@@ -72,6 +72,22 @@ struct Nbr {
     }
 };
 
+__device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f)
+{
+    if (f.l == 0) return n;
+    const unsigned t = __umulhi(f.m, n);
+    return (t + ((n - t) >> 1)) >> (f.l - 1);
+}
+
+__device__ __forceinline__ void decode(unsigned c, const LatView& v, int& x, int& y, int& z)
+{
+    const unsigned r = fdiv(c, v.dz);
+    z = (int)(c - r * (unsigned)v.nz);
+    const unsigned q = fdiv(r, v.dy);
+    y = (int)(r - q * (unsigned)v.ny);
+    x = (int)q;
+}
+
 __device__ __forceinline__ void decode(unsigned c, int ny, int nz, int& x, int& y, int& z)
 {
     z = (int)(c % (unsigned)nz);
