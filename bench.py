@@ -189,6 +189,7 @@ def main():
         "roofline": rl,
         "ibm_stage_ms": stages,
         "fluid_mlups": round(W.fluid_cells * args.steps / dt / 1e6, 2),
+        "vertex_updates_per_s": round(W.vertices * args.steps / dt, 1),
     }
     W.ibm.close()
     W.lattice.close()

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libbiblbm.so")
 BL_OK = 0
 STORE = {"f64": 0, "f32": 1, "f16s": 2}
 KERNEL = {"roma": 0, "peskin": 1, "tophat": 2}
-MODEL = {"capsule_skalak": 0, "capsule_neohookean": 1, "rbc": 2}
+MODEL = {"capsule_skalak": 0, "capsule_neohookean": 1, "rbc": 2, "skalak_rbc": 3}
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int

@@ -1,0 +1,131 @@
+"""Workloads for kernel profiles, and summaries of rocprofv3 output.
+
+Run under rocprofv3 (kernel trace or counters), for example
+
+    rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o trace -- \
+        python3 -m biblbm_amd.profile_kernels collide rbc_in_pipe f64 40
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT_F -o pmc -- \
+        python3 -m biblbm_amd.profile_kernels collide rbc_in_pipe f64 10 copy 268435456 5
+
+and summarise with
+
+    python3 -m biblbm_amd.profile_kernels stats OUT
+    python3 -m biblbm_amd.profile_kernels traffic OUT_F OUT_W 268435456 k_collide_stream
+
+`collide W S N` builds workload W (rbc_in_pipe, suspension, taylor_green or
+capsule_in_shear) with storage S, runs two coupled steps so that the force
+tiles are those of a running simulation, then N collide-and-stream launches.
+`coupled W S N` runs N eager coupled steps instead.  `copy B N` runs N
+launches of a plain 16-byte-per-thread device copy of B bytes (the
+calibration kernel for the HBM byte counters).
+"""
+import csv
+import glob
+import os
+import sys
+
+
+def _workload(name, storage):
+    from . import workloads
+    return getattr(workloads, name)(storage)
+
+
+def run(argv):
+    from . import capi
+    capi.call("bl_init", 0)
+    i = 0
+    while i < len(argv):
+        cmd = argv[i]
+        if cmd in ("collide", "coupled"):
+            W = _workload(argv[i + 1], argv[i + 2])
+            n = int(argv[i + 3])
+            W.ibm.coupled_steps(2, use_graph=False)
+            capi.call("bl_sync")
+            if cmd == "collide":
+                ms = W.lattice.time_collide(n)
+            else:
+                import time
+                t0 = time.perf_counter()
+                W.ibm.coupled_steps(n, use_graph=False)
+                capi.call("bl_sync")
+                ms = (time.perf_counter() - t0) * 1e3
+            print("%s %s %s: %d launches, %.4f ms each" % (cmd, argv[i + 1], argv[i + 2], n, ms / n), flush=True)
+            W.ibm.close()
+            W.lattice.close()
+            i += 4
+        elif cmd == "copy":
+            import ctypes
+            nbytes, n = int(argv[i + 1]), int(argv[i + 2])
+            ms = ctypes.c_double()
+            capi.call("bl_device_copy", nbytes, n, ctypes.byref(ms))
+            print("copy %d bytes: %.4f ms each, %.1f GB/s (read + write)" %
+                  (nbytes, ms.value / n, 2 * nbytes / (ms.value / n * 1e-3) / 1e9), flush=True)
+            i += 3
+        else:
+            raise SystemExit("unknown command %r" % cmd)
+
+
+def _csv_rows(outdir, suffix):
+    rows = []
+    for f in sorted(glob.glob(os.path.join(outdir, "**", "*" + suffix), recursive=True)):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def stats(outdir, top=25):
+    """Kernel table from a --kernel-trace --stats run (kernel_stats.csv)."""
+    rows = _csv_rows(outdir, "kernel_stats.csv")
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print("| kernel | calls | total us | avg us | min us | max us | % |")
+    print("|---|---|---|---|---|---|---|")
+    for r in rows[:top]:
+        print("| %s | %s | %.1f | %.3f | %.3f | %.3f | %.2f |" % (
+            r["Name"][:90].replace("|", "/"), r["Calls"], float(r["TotalDurationNs"]) / 1e3,
+            float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+
+
+def _counter_per_dispatch(outdir, counter, kernel_substr):
+    vals = {}
+    for r in _csv_rows(outdir, "counter_collection.csv"):
+        if r["Counter_Name"] != counter or kernel_substr not in r["Kernel_Name"]:
+            continue
+        key = r["Dispatch_Id"]
+        vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
+    return list(vals.values())
+
+
+def traffic(fetch_dir, write_dir, copy_bytes, kernel_substr):
+    """HBM bytes per launch of `kernel_substr` from FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch), each scaled by what the same counter reads for the
+    plain copy kernel of known size."""
+    cb = float(copy_bytes)
+    out = {}
+    for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        copy = _counter_per_dispatch(d, name, "k_copy16")
+        kern = _counter_per_dispatch(d, name, kernel_substr)
+        if not copy or not kern:
+            raise SystemExit("no %s samples for k_copy16 / %s in %s" % (name, kernel_substr, d))
+        copy_kib = sorted(copy)[len(copy) // 2]
+        factor = cb / (copy_kib * 1024.0)
+        kern_kib = sorted(kern)[len(kern) // 2]
+        out[name] = (kern_kib, copy_kib, factor, kern_kib * 1024.0 * factor)
+    for name, (k, c, f, b) in out.items():
+        print("%s: kernel %.1f KiB/dispatch (median of dispatches), copy kernel %.1f KiB for %d bytes, "
+              "factor %.4f -> %.4e bytes" % (name, k, c, int(cb), f, b))
+    total = out["FETCH_SIZE"][3] + out["WRITE_SIZE"][3]
+    print("traffic_bytes_per_launch: %.6e" % total)
+    return total
+
+
+def main(argv):
+    if argv and argv[0] == "stats":
+        stats(argv[1], int(argv[2]) if len(argv) > 2 else 25)
+    elif argv and argv[0] == "traffic":
+        traffic(argv[1], argv[2], argv[3], argv[4])
+    else:
+        run(argv)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -46,6 +46,12 @@ extern "C" {
 #define BL_MODEL_CAPSULE_SKALAK 0      /* src/fsi/DeformableCapsule.hh:140-142 */
 #define BL_MODEL_CAPSULE_NEOHOOKEAN 1  /* src/fsi/DeformableCapsule.hh:143-145 */
 #define BL_MODEL_RBC 2                 /* src/fsi/RBCParticle.hh:140-247      */
+/* Skalak in-plane law of the capsule (src/fsi/DeformableCapsule.hh:60-150)
+ * with the RBC model's local + global area, volume and bending terms
+ * (src/fsi/RBCParticle.hh:176-246) and without its WLC-POW links: the
+ * "Skalak + bending + area + volume" membrane of the red-cell workloads.
+ * params: {G, C, k_area_global, k_area_local, k_volume, vol_desired, k_bend} */
+#define BL_MODEL_SKALAK_RBC 3
 
 typedef struct bl_lattice_s *bl_lattice;
 typedef struct bl_ibm_s *bl_ibm;
@@ -167,8 +173,8 @@ int bl_particle_observables(bl_ibm g, int pid, double out[14]);
 /* The phase field of the last voxelization (ncell bytes, 0/1). */
 int bl_ibm_get_phase(bl_ibm g, unsigned char *phase);
 /* Time of each IBM stage over the last bl_ibm_step, ms: stencil binning,
- * membrane forces, fused interpolation + spreading, advection + geometry,
- * voxelization. */
+ * membrane forces, velocity staging + spreading + interpolation with
+ * advection, particle geometry + centre-of-mass remap, voxelization. */
 int bl_ibm_stage_times(bl_ibm g, double out[5]);
 /* Tiles touched by the last binning and (tile, vertex) list entries
  * (diagnostics for the traffic model; no counterpart in the reference). */
