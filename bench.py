@@ -20,6 +20,8 @@ bounding box).  Extra objects on the same JSON line:
   fp32          the same workload with FP32 storage
   suspension    the 400-cell channel: FP16-shifted vs FP32 storage speed and
                 velocity error against the FP64 run
+  ensemble      64 shear boxes of uneven sizes advanced by one hipGraph
+                launch, against the same boxes stepped one after another
 """
 import argparse
 import json
@@ -151,6 +153,46 @@ def suspension_leg(steps, warmup):
     return out
 
 
+def ensemble_leg(steps, warmup, n_boxes=64):
+    """64 independent shear boxes (sides from {32..64}^3, one 642-vertex cell
+    each) advanced by one hipGraph launch per two steps, against the same
+    boxes stepped one after another (each with its own captured graph)."""
+    import numpy as np
+    from biblbm_amd import workloads
+    from biblbm_amd.ensemble import Ensemble, tune_collide
+    from biblbm_amd.ibm import taylor_deformation
+    ws = workloads.shear_ensemble("f64", n_boxes=n_boxes)
+    cells = sum(w.cells for w in ws)
+    tuned = tune_collide(ws[0].lattice)["best"]
+    for w in ws:
+        w.lattice.set_launch(tuned["block"], tuned["xcd_remap"])
+    ens = Ensemble([(w.lattice, w.ibm) for w in ws])
+    ens.steps(warmup)
+    ms = ens.time_steps(steps)
+    from biblbm_amd import capi
+    for w in ws:
+        w.ibm.coupled_steps(warmup, use_graph=True)
+    capi.call("bl_sync")
+    t0 = time.perf_counter()
+    for w in ws:
+        w.ibm.coupled_steps(steps, use_graph=True)
+    capi.call("bl_sync")
+    loop_s = time.perf_counter() - t0
+    D = np.array([taylor_deformation(w.ibm.observables(0)["inertia"]) for w in ws])
+    out = dict(mlups=round(cells * steps / (ms * 1e-3) / 1e6, 1), ms_per_step=round(ms / steps, 4),
+               per_box_loop_mlups=round(cells * steps / loop_s / 1e6, 1),
+               deformation_index=dict(min=float(D.min()), median=float(np.median(D)), max=float(D.max())),
+               tuned_launch=tuned,
+               workload="%d shear boxes, %d cells, sides %s, one 642-vertex biconcave cell each, FP64, "
+                        "steps %d after %d warmup" % (n_boxes, cells, sorted({w.params["s"] for w in ws}), steps,
+                                                      warmup))
+    ens.close()
+    for w in ws:
+        w.ibm.close()
+        w.lattice.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +244,7 @@ def main():
         W32.ibm.close()
         W32.lattice.close()
         line["suspension"] = suspension_leg(max(50, args.steps // 10), 20)
+        line["ensemble"] = ensemble_leg(max(50, args.steps // 10), 10)
     if not args.no_cpu_baseline:
         cb, cpu_pos = cpu_baseline(args.cpu_steps, args.cpu_budget)
         # the same number of coupled steps on the GPU from the same state: parity check

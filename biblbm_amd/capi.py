@@ -71,11 +71,9 @@ SIGNATURES = {
     "bl_ibm_tile_stats": [_P, _P],
     "bl_ensemble_create": [_I, _P, _P, _I, ctypes.POINTER(_P)],
     "bl_ensemble_destroy": [_P],
-    "bl_ensemble_lattice": [_P, _I, ctypes.POINTER(_P)],
-    "bl_ensemble_ibm": [_P, _I, ctypes.POINTER(_P)],
     "bl_ensemble_steps": [_P, _I, _I],
     "bl_ensemble_time_steps": [_P, _I, _I, _PD],
-    "bl_tune_collide": [_P, _P, _I],
+    "bl_tune_collide": [_P, ctypes.c_char_p, _I],
 }
 
 

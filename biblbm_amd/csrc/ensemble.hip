@@ -1,0 +1,279 @@
+// Ensembles of independent boxes advanced together, and the launch-shape
+// tuner of the collide-and-stream kernel.
+//
+// An ensemble groups lattices (each optionally with its immersed boundary)
+// created through the ordinary entry points.  One hipGraph holds two
+// coupled steps (both AA parities) of every box, each box a parallel branch
+// forked from and joined into the ensemble's stream, so the small kernels of
+// different boxes run side by side and the whole ensemble step is one graph
+// launch.  The reference runs such boxes as separate programs.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "ibm.hpp"
+#include "lattice.hpp"
+
+struct bl_ensemble_s {
+    std::vector<bl_lattice> lats;
+    std::vector<bl_ibm> ibms;  // nullptr entries: lattice-only boxes
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t graph = nullptr;
+    std::vector<long long> parity;  // each box's AA parity the graph was captured for
+};
+
+namespace bl {
+namespace {
+
+int box_step(bl_ensemble e, size_t i)
+{
+    if (e->ibms[i]) return ibm_step_launch(e->ibms[i]);
+    int rc = collide_stream_launch(e->lats[i]);
+    return rc ? rc : velocity_walls_launch(e->lats[i]);
+}
+
+// the ensemble stream waits for each box's pending work, and the box
+// streams wait for the ensemble stream afterwards
+int order_before(bl_ensemble e)
+{
+    for (size_t i = 0; i < e->lats.size(); ++i) {
+        BL_HIP(hipEventRecord(e->join[i], e->lats[i]->stream));
+        BL_HIP(hipStreamWaitEvent(e->stream, e->join[i], 0));
+    }
+    return BL_OK;
+}
+
+int order_after(bl_ensemble e)
+{
+    BL_HIP(hipEventRecord(e->done, e->stream));
+    for (bl_lattice L : e->lats) BL_HIP(hipStreamWaitEvent(L->stream, e->done, 0));
+    return BL_OK;
+}
+
+int capture(bl_ensemble e)
+{
+    if (e->graph) {
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    std::vector<long long> s0(e->lats.size());
+    for (size_t i = 0; i < e->lats.size(); ++i) s0[i] = e->lats[i]->steps;
+    BL_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = BL_OK;
+    hipError_t he = hipEventRecord(e->fork, e->stream);
+    for (size_t i = 0; i < e->lats.size() && rc == BL_OK && he == hipSuccess; ++i) {
+        hipStream_t s = e->lats[i]->stream;
+        he = hipStreamWaitEvent(s, e->fork, 0);
+        for (int k = 0; k < 2 && rc == BL_OK && he == hipSuccess; ++k) rc = box_step(e, i);
+        if (rc == BL_OK && he == hipSuccess) he = hipEventRecord(e->join[i], s);
+        if (rc == BL_OK && he == hipSuccess) he = hipStreamWaitEvent(e->stream, e->join[i], 0);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(e->stream, &graph);
+    for (size_t i = 0; i < e->lats.size(); ++i) e->lats[i]->steps = s0[i];  // capture ran nothing
+    if (rc != BL_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (he != hipSuccess || ee != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(BL_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(he != hipSuccess ? he : ee));
+    }
+    const hipError_t ie = hipGraphInstantiate(&e->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(BL_ERR_HIP, std::string("ensemble graph: ") + hipGetErrorString(ie));
+    e->parity.resize(e->lats.size());
+    for (size_t i = 0; i < e->lats.size(); ++i) e->parity[i] = e->lats[i]->steps & 1;
+    return BL_OK;
+}
+
+bool parity_matches(bl_ensemble e)
+{
+    if (!e->graph || e->parity.size() != e->lats.size()) return false;
+    for (size_t i = 0; i < e->lats.size(); ++i)
+        if ((e->lats[i]->steps & 1) != e->parity[i]) return false;
+    return true;
+}
+
+int run_steps(bl_ensemble e, int n, int use_graph)
+{
+    int rc;
+    int done = 0;
+    if (use_graph && n >= 2) {
+        if (!parity_matches(e) && (rc = capture(e))) return rc;
+        if ((rc = order_before(e))) return rc;
+        for (; done + 2 <= n; done += 2) {
+            BL_HIP(hipGraphLaunch(e->graph, e->stream));
+            for (bl_lattice L : e->lats) L->steps += 2;
+        }
+        if ((rc = order_after(e))) return rc;
+    }
+    for (; done < n; ++done)
+        for (size_t i = 0; i < e->lats.size(); ++i)
+            if ((rc = box_step(e, i))) return rc;
+    return BL_OK;
+}
+
+}  // namespace
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" {
+
+int bl_ensemble_create(int n_boxes, const bl_lattice* lats, const bl_ibm* ibms, int flags, bl_ensemble* out)
+{
+    if (n_boxes <= 0 || !lats || !out) return fail(BL_ERR_ARG, "bl_ensemble_create: bad argument");
+    if (flags != 0) return fail(BL_ERR_ARG, "bl_ensemble_create: flags must be 0");
+    for (int i = 0; i < n_boxes; ++i) {
+        if (!lats[i]) return fail(BL_ERR_ARG, "bl_ensemble_create: null lattice");
+        for (int j = 0; j < i; ++j)
+            if (lats[j] == lats[i]) return fail(BL_ERR_ARG, "bl_ensemble_create: a lattice appears twice");
+        if (ibms && ibms[i]) {
+            if (ibms[i]->lat != lats[i])
+                return fail(BL_ERR_ARG, "bl_ensemble_create: immersed boundary belongs to another lattice");
+            if (!ibms[i]->inited) return fail(BL_ERR_STATE, "bl_ensemble_create: immersed boundary not initialised");
+        }
+    }
+    bl_ensemble e = new bl_ensemble_s();
+    e->lats.assign(lats, lats + n_boxes);
+    e->ibms.assign((size_t)n_boxes, nullptr);
+    if (ibms)
+        for (int i = 0; i < n_boxes; ++i) e->ibms[i] = ibms[i];
+    e->join.assign((size_t)n_boxes, nullptr);
+    hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->done, hipEventDisableTiming);
+    for (int i = 0; i < n_boxes && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming);
+    if (he != hipSuccess) {
+        bl_ensemble_destroy(e);
+        return fail(BL_ERR_HIP, std::string("bl_ensemble_create: ") + hipGetErrorString(he));
+    }
+    *out = e;
+    return BL_OK;
+}
+
+int bl_ensemble_destroy(bl_ensemble e)
+{
+    if (!e) return BL_OK;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->graph) hipGraphExecDestroy(e->graph);
+    for (hipEvent_t ev : e->join)
+        if (ev) hipEventDestroy(ev);
+    if (e->fork) hipEventDestroy(e->fork);
+    if (e->done) hipEventDestroy(e->done);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return BL_OK;
+}
+
+int bl_ensemble_steps(bl_ensemble e, int n_steps, int use_graph)
+{
+    if (!e || n_steps < 0) return fail(BL_ERR_ARG, "bl_ensemble_steps: bad argument");
+    int rc = run_steps(e, n_steps, use_graph);
+    if (rc) return rc;
+    for (bl_ibm g : e->ibms)
+        if (g && (rc = ibm_check_overflow(g))) return rc;
+    return BL_OK;
+}
+
+int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double* ms)
+{
+    if (!e || n_steps <= 0 || !ms) return fail(BL_ERR_ARG, "bl_ensemble_time_steps: bad argument");
+    int rc;
+    hipEvent_t a, b;
+    BL_HIP(hipEventCreate(&a));
+    BL_HIP(hipEventCreate(&b));
+    if ((rc = order_before(e))) return rc;
+    BL_HIP(hipEventRecord(a, e->stream));
+    if ((rc = order_after(e))) return rc;
+    if ((rc = run_steps(e, n_steps, use_graph))) return rc;
+    if ((rc = order_before(e))) return rc;
+    BL_HIP(hipEventRecord(b, e->stream));
+    BL_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    BL_HIP(hipEventElapsedTime(&t, a, b));
+    *ms = (double)t;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    for (bl_ibm g : e->ibms)
+        if (g && (rc = ibm_check_overflow(g))) return rc;
+    return BL_OK;
+}
+
+int bl_tune_collide(bl_lattice L, char* report, int cap)
+{
+    if (!L) return fail(BL_ERR_ARG, "bl_tune_collide: null lattice");
+    static const int blocks[] = {64, 128, 192, 256};
+    const size_t sz = L->storage == BL_STORE_F64 ? 8 : (L->storage == BL_STORE_F32 ? 4 : 2);
+    const size_t bytes = (size_t)Q * L->stride * sz;
+    void* saved = nullptr;
+    BL_HIP(hipMalloc(&saved, bytes));
+    BL_HIP(hipMemcpyAsync(saved, L->pops, bytes, hipMemcpyDeviceToDevice, L->stream));
+    const long long steps0 = L->steps;
+    const int block0 = L->block, remap0 = L->xcd_remap;
+    hipEvent_t a, b;
+    BL_HIP(hipEventCreate(&a));
+    BL_HIP(hipEventCreate(&b));
+    std::string rep = "{\"trials\": [";
+    double best = 1e30;
+    int best_block = block0, best_remap = remap0;
+    int rc = BL_OK;
+    bool first = true;
+    for (int remap = 0; remap < 2 && rc == BL_OK; ++remap)
+        for (int bi = 0; bi < 4 && rc == BL_OK; ++bi) {
+            L->block = blocks[bi];
+            L->xcd_remap = remap;
+            for (int k = 0; k < 2 && rc == BL_OK; ++k) rc = collide_stream_launch(L);
+            if (rc) break;
+            const int n = 10;
+            hipEventRecord(a, L->stream);
+            for (int k = 0; k < n && rc == BL_OK; ++k) rc = collide_stream_launch(L);
+            hipEventRecord(b, L->stream);
+            if (rc) break;
+            hipEventSynchronize(b);
+            float t = 0.f;
+            hipEventElapsedTime(&t, a, b);
+            const double ms = (double)t / n;
+            char buf[128];
+            snprintf(buf, sizeof(buf), "%s{\"block\": %d, \"xcd_remap\": %d, \"ms\": %.6f}", first ? "" : ", ",
+                     blocks[bi], remap, ms);
+            rep += buf;
+            first = false;
+            if (ms < best) {
+                best = ms;
+                best_block = blocks[bi];
+                best_remap = remap;
+            }
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    // the trials advanced the state: put the populations and the step count back
+    hipMemcpyAsync(L->pops, saved, bytes, hipMemcpyDeviceToDevice, L->stream);
+    hipStreamSynchronize(L->stream);
+    hipFree(saved);
+    L->steps = steps0;
+    if (rc) {
+        L->block = block0;
+        L->xcd_remap = remap0;
+        return rc;
+    }
+    L->block = best_block;
+    L->xcd_remap = best_remap;
+    if (L->ibm) L->ibm->graph_parity = -1;  // a captured step holds the old launch shape
+    char buf[128];
+    snprintf(buf, sizeof(buf), "], \"best\": {\"block\": %d, \"xcd_remap\": %d, \"ms\": %.6f}}", best_block,
+             best_remap, best);
+    rep += buf;
+    if (report && cap > 0) {
+        const size_t n = std::min<size_t>(rep.size(), (size_t)cap - 1);
+        rep.copy(report, n);
+        report[n] = '\0';
+    }
+    return BL_OK;
+}
+
+}  // extern "C"

@@ -1367,6 +1367,17 @@ int step_body(bl_ibm g)
 }
 
 }  // namespace
+
+int ibm_step_launch(bl_ibm g)
+{
+    int rc;
+    if ((rc = step_body(g))) return rc;
+    if ((rc = collide_stream_launch(g->lat))) return rc;
+    return velocity_walls_launch(g->lat);
+}
+
+int ibm_check_overflow(bl_ibm g) { return check_overflow(g); }
+
 }  // namespace bl
 
 using namespace bl;

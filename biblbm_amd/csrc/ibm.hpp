@@ -110,3 +110,11 @@ struct bl_ibm_s {
     hipGraphExec_t graph2 = nullptr;  // captured pair of coupled steps
     long long graph_parity = -1;
 };
+
+namespace bl {
+// One coupled step's launches on the lattice stream, without host
+// synchronisation: IBM stages, collide-and-stream, velocity-wall pass.
+int ibm_step_launch(bl_ibm g);
+// Reads the overflow flags of the last steps (synchronises the stream).
+int ibm_check_overflow(bl_ibm g);
+}  // namespace bl

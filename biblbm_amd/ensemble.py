@@ -1,0 +1,53 @@
+"""Ensembles of independent boxes over the C-ABI (bl_ensemble_*), and the
+collide launch-shape tuner (bl_tune_collide).
+
+No counterpart in the reference, which runs each box (one rbc_in_shear
+setup) as its own program: here every box keeps its own Lattice /
+ImmersedBoundary objects and one hipGraph launch advances all of them.
+"""
+import ctypes
+import json
+
+from . import capi
+from .capi import call
+
+
+class Ensemble:
+    def __init__(self, boxes):
+        """boxes: list of (Lattice, ImmersedBoundary or None); the objects
+        must stay alive as long as the ensemble."""
+        self.boxes = list(boxes)
+        n = len(self.boxes)
+        lats = (ctypes.c_void_p * n)(*[b[0].handle.value for b in self.boxes])
+        ibms = (ctypes.c_void_p * n)(*[(b[1].handle.value if b[1] is not None else None) for b in self.boxes])
+        h = ctypes.c_void_p()
+        call("bl_ensemble_create", n, ctypes.cast(lats, ctypes.c_void_p), ctypes.cast(ibms, ctypes.c_void_p), 0,
+             ctypes.byref(h))
+        self.handle = h
+
+    def steps(self, n, use_graph=True):
+        call("bl_ensemble_steps", self.handle, int(n), int(bool(use_graph)))
+
+    def time_steps(self, n, use_graph=True):
+        ms = ctypes.c_double()
+        call("bl_ensemble_time_steps", self.handle, int(n), int(bool(use_graph)), ctypes.byref(ms))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            capi.lib.bl_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_collide(lattice):
+    """Times every launch shape of the lattice's collide kernel, keeps the
+    fastest (the lattice state is left as it was); returns the report."""
+    buf = ctypes.create_string_buffer(4096)
+    call("bl_tune_collide", lattice.handle, buf, len(buf))
+    return json.loads(buf.value.decode())

@@ -17,6 +17,11 @@
                      discs jams a little below 20 %).
   taylor_green       384^3 periodic Taylor-Green (u0 0.03, tau 0.8) with 16
                      Skalak capsules (642 vertices) at seeded positions.
+  shear_ensemble     64 independent shear boxes, sides drawn from
+                     {32, 40, 48, 56, 64}, velocity walls y = 0 / s-1, tau
+                     uniform in [0.7, 1.2], one 642-vertex biconcave cell
+                     (radius 0.16 s, shear modulus 0.002) per box, the wall
+                     speed setting Ca log-uniform in [0.05, 0.5]; Peskin.
 
 The red-cell constraint parameters follow the reference's create_rbc_params
 (progs/common/calibrate_rbc_shape.h:75-112): area modulus split 0.8 / 0.2
@@ -238,3 +243,50 @@ def taylor_green(storage="f64", n=384, u0=0.03, tau=0.8, n_caps=16, radius=8.0, 
     ib.init()
     return Workload("taylor_green", L, ib, n ** 3, n ** 3, n_caps * len(v),
                     dict(n=n, u0=u0, tau=tau, G=G, kernel=kernel, seed=seed))
+
+
+def shear_box(storage, s, tau, Ca, G=0.002, level=3, C=10.0, kernel="peskin", radius=None, seed=0):
+    """One rbc_in_shear-style box: s^3 lattice, periodic x / z, regularized
+    velocity walls at y = 0 and s - 1, linear shear start, one biconcave
+    cell (Skalak + bending + area + volume, shear modulus G) at the centre
+    with a seeded orientation.  The shear rate sets the capillary number:
+    gamma = Ca G / (mu R), walls at -/+ gamma (s - 1) / 2 (below 0.05 for
+    the ensemble's draws; a fixed G keeps the explicit membrane update
+    stable for every draw)."""
+    R = 0.16 * s if radius is None else radius
+    mu = (tau - 0.5) / 3.0
+    gamma = Ca * G / (mu * R)
+    wall_u = gamma * (s - 1) / 2.0
+    L = Lattice(s, s, s, periodic=(1, 0, 1), storage=storage, omega=1 / tau)
+    L.set_velocity_walls(1, [-wall_u, 0, 0], [wall_u, 0, 0])
+    mask = np.zeros((s, s, s), np.uint8)
+    mask[:, 0, :] = mask[:, -1, :] = VELOCITY_WALL
+    L.set_mask(mask)
+    y = np.arange(s)
+    u = np.zeros((s, s, s, 3))
+    u[..., 0] = (wall_u * (2.0 * y / (s - 1) - 1.0))[None, :, None]
+    L.init_equilibrium(None, u)
+    L.initialize()
+    v, t = meshes.biconcave(level, R)
+    rng = np.random.default_rng(seed)
+    axis = rng.normal(size=3)
+    rot = meshes.rotation_matrix(axis / np.linalg.norm(axis), rng.uniform(0, np.pi))
+    ib = ImmersedBoundary(L, kernel)
+    sid = ib.add_shape(v, t)
+    ib.add_particle(sid, SKALAK_RBC, skalak_rbc_params(G, C, R, _shape_volume(v, t), 0.02 * G * R * R),
+                    v @ rot.T + [s / 2.0, (s - 1) / 2.0, s / 2.0])
+    ib.init()
+    return Workload("shear_box", L, ib, s ** 3, s ** 3, len(v),
+                    dict(s=s, tau=tau, Ca=Ca, G=G, radius=R, wall_u=wall_u, shear_rate=gamma))
+
+
+def shear_ensemble(storage="f64", n_boxes=64, seed=11, sides=(32, 40, 48, 56, 64), **kw):
+    """The boxes of the batched-ensemble configuration (seeded draws)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n_boxes):
+        s = int(rng.choice(sides))
+        tau = float(rng.uniform(0.7, 1.2))
+        Ca = float(np.exp(rng.uniform(np.log(0.05), np.log(0.5))))
+        out.append(shear_box(storage, s, tau, Ca, seed=seed * 1000 + b, **kw))
+    return out

@@ -180,6 +180,28 @@ int bl_ibm_stage_times(bl_ibm g, double out[5]);
  * (diagnostics for the traffic model; no counterpart in the reference). */
 int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
 
+/* ----------------------------------------------------------- ensembles */
+/* No counterpart in the reference, which runs independent boxes as separate
+ * programs (one rbc_in_shear run per box).  Groups n_boxes lattices (each
+ * with its immersed boundary, or NULL / ibms == NULL for lattice-only boxes)
+ * created and initialised through the entry points above; flags must be 0.
+ * The handles stay owned by the caller and must outlive the ensemble. */
+int bl_ensemble_create(int n_boxes, const bl_lattice *lattices, const bl_ibm *ibms, int flags,
+                       bl_ensemble *out);
+int bl_ensemble_destroy(bl_ensemble e);
+/* n coupled steps of every box.  use_graph: two steps of all boxes are
+ * captured once in one hipGraph (each box a parallel branch) and replayed. */
+int bl_ensemble_steps(bl_ensemble e, int n_steps, int use_graph);
+/* The same, returning the device time of the n steps in milliseconds. */
+int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms);
+
+/* ------------------------------------------------------------- tuning */
+/* No counterpart in the reference.  Times the collide-and-stream kernel of
+ * this lattice for every launch shape bl_lattice_set_launch accepts, keeps
+ * the fastest, and restores the populations and step count the trials
+ * advanced.  report (cap bytes, may be NULL) receives a JSON summary. */
+int bl_tune_collide(bl_lattice lat, char *report, int cap);
+
 #ifdef __cplusplus
 }
 #endif
