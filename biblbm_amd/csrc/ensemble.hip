@@ -30,7 +30,7 @@ namespace {
 
 int box_step(bl_ensemble e, size_t i)
 {
-    if (e->ibms[i]) return ibm_step_launch(e->ibms[i]);
+    if (e->ibms[i]) return ibm_step_launch(e->ibms[i], false);  // the boxes are the parallel branches
     int rc = collide_stream_launch(e->lats[i]);
     return rc ? rc : velocity_walls_launch(e->lats[i]);
 }

@@ -99,34 +99,47 @@ __device__ __forceinline__ int node_at(int bw, int k, int n, int per)
     return (v < 0 || v >= n) ? -1 : v;
 }
 
-// Distinct tiles touched along one axis by the S nodes of a stencil
-// (at most 3: a partial last tile plus a wrap).
-__device__ __forceinline__ int axis_tiles(int bw, int S, int n, int per, int out[4])
+// Distinct tiles touched along one axis by the S (<= 4) nodes of a stencil:
+// at most 3 (two for the run up to the face, one more after a periodic
+// wrap), kept in registers.
+struct AxisTiles {
+    int t0 = -1, t1 = -1, t2 = -1, m = 0;
+    __device__ __forceinline__ int get(int i) const { return i == 0 ? t0 : (i == 1 ? t1 : t2); }
+};
+
+__device__ __forceinline__ AxisTiles axis_tiles(int bw, int S, int n, int per)
 {
-    int m = 0;
-    for (int k = 0; k < S; ++k) {
-        const int node = node_at(bw, k, n, per);
-        if (node < 0) continue;
+    AxisTiles r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int node = k < S ? node_at(bw, k, n, per) : -1;
         const int t = node >> 2;
-        bool seen = false;
-        for (int q = 0; q < m; ++q) seen = seen || out[q] == t;
-        if (!seen) out[m++] = t;
+        const bool fresh = node >= 0 && t != r.t0 && t != r.t1 && t != r.t2;
+        if (fresh) {
+            if (r.m == 0) r.t0 = t;
+            else if (r.m == 1) r.t1 = t;
+            else r.t2 = t;
+            ++r.m;
+        }
     }
-    return m;
+    return r;
 }
 
-__device__ __forceinline__ int vertex_tiles(const Grid& gd, int S, const int* b, int t[27])
-{
-    int tx[4], ty[4], tz[4];
-    const int mx = axis_tiles(b[0], S, gd.n[0], gd.per[0], tx);
-    const int my = axis_tiles(b[1], S, gd.n[1], gd.per[1], ty);
-    const int mz = axis_tiles(b[2], S, gd.n[2], gd.per[2], tz);
-    int m = 0;
-    for (int i = 0; i < mx; ++i)
-        for (int j = 0; j < my; ++j)
-            for (int k = 0; k < mz; ++k) t[m++] = (tx[i] * gd.nt[1] + ty[j]) * gd.nt[2] + tz[k];
-    return m;
-}
+// Tiles touched by a vertex's stencil: the product of the per-axis sets.
+struct StencilTiles {
+    AxisTiles ax[3];
+    __device__ __forceinline__ void init(const Grid& gd, int S, const int* b)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ax[a] = axis_tiles(b[a], S, gd.n[a], gd.per[a]);
+    }
+    // tile (i, j, k) of the product, or -1 past the per-axis counts
+    __device__ __forceinline__ int at(const Grid& gd, int i, int j, int k) const
+    {
+        if (i >= ax[0].m || j >= ax[1].m || k >= ax[2].m) return -1;
+        return (ax[0].get(i) * gd.nt[1] + ax[1].get(j)) * gd.nt[2] + ax[2].get(k);
+    }
+};
 
 // Neighbouring vertices of a membrane touch the same few tiles, so a
 // workgroup first aggregates its (tile, vertex) pairs in an LDS hash table
@@ -181,8 +194,6 @@ __global__ void __launch_bounds__(256) k_bin_count(Grid gd, int kernel, int S, d
     hash_clear(keys, cnt, nullptr);
     __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int t[27];
-    int m = 0;
     if (v < NV) {
         int b[3];
         for (int a = 0; a < 3; ++a) {
@@ -197,22 +208,30 @@ __global__ void __launch_bounds__(256) k_bin_count(Grid gd, int kernel, int S, d
             b[a] = bw;
             base[3 * v + a] = bw;
         }
-        m = vertex_tiles(gd, S, b, t);
-    }
-    for (int q = 0; q < m; ++q) {
-        const int s = hash_insert(keys, t[q]);
-        if (s >= 0) atomicAdd(&cnt[s], 1);
-        else atomicAdd(&count[t[q]], 1);
+        StencilTiles st;
+        st.init(gd, S, b);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int tt = st.at(gd, i, j, k);
+                    if (tt < 0) continue;
+                    const int sl = hash_insert(keys, tt);
+                    if (sl >= 0) atomicAdd(&cnt[sl], 1);
+                    else atomicAdd(&count[tt], 1);
+                }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kHash; i += blockDim.x)
         if (cnt[i] > 0) atomicAdd(&count[keys[i]], cnt[i]);
 }
 
-// Writes each (tile, vertex) pair into its tile's list as v*32 + q, q being
-// the tile's index among the vertex's tiles.  Each workgroup reserves one
-// range per distinct tile and hands out slots inside it from LDS.  The order
-// within a tile is whatever the atomics give; the spreading kernel sorts.
+// Writes each (tile, vertex) pair into its tile's list (entry = vertex
+// index).  Each workgroup reserves one range per distinct tile and hands
+// out slots inside it from LDS.  The order within a tile is whatever the
+// atomics give; the spreading kernel sorts.
 __global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const int* __restrict__ base,
                                                   const int* __restrict__ off, int* __restrict__ fill,
                                                   int* __restrict__ list, long cap, int* __restrict__ status)
@@ -223,12 +242,20 @@ __global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const
     hash_clear(keys, cnt, rank);
     __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int t[27];
-    int m = 0;
-    if (v < NV) m = vertex_tiles(gd, S, base + 3 * v, t);
-    for (int q = 0; q < m; ++q) {
-        const int s = hash_insert(keys, t[q]);
-        if (s >= 0) atomicAdd(&cnt[s], 1);
+    StencilTiles st;
+    if (v < NV) {
+        st.init(gd, S, base + 3 * v);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int tt = st.at(gd, i, j, k);
+                    if (tt < 0) continue;
+                    const int sl = hash_insert(keys, tt);
+                    if (sl >= 0) atomicAdd(&cnt[sl], 1);
+                }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kHash; i += blockDim.x) {
@@ -236,23 +263,31 @@ __global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const
         if (c > 0) cnt[i] = off[keys[i]] + atomicAdd(&fill[keys[i]], c);
     }
     __syncthreads();
-    for (int q = 0; q < m; ++q) {
-        const int s = hash_find(keys, t[q]);
-        const long slot = s >= 0 ? (long)cnt[s] + atomicAdd(&rank[s], 1)
-                                 : (long)off[t[q]] + atomicAdd(&fill[t[q]], 1);
-        if (slot < cap) list[slot] = (int)(v * 32 + q);
-        else atomicOr(&status[2], 1);
+    if (v < NV) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int tt = st.at(gd, i, j, k);
+                    if (tt < 0) continue;
+                    const int sl = hash_find(keys, tt);
+                    const long slot = sl >= 0 ? (long)cnt[sl] + atomicAdd(&rank[sl], 1)
+                                              : (long)off[tt] + atomicAdd(&fill[tt], 1);
+                    if (slot < cap) list[slot] = (int)v;
+                    else atomicOr(&status[2], 1);
+                }
     }
 }
 
-__global__ void k_clear_bins(int* count, int* fill, long n, int* status)
+__global__ void k_clear_bins(int* count, int* fill, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         count[i] = 0;
         fill[i] = 0;
     }
-    if (i == 0) status[2] = 0;
 }
 
 // --------------------------------------------------------------- scan
@@ -339,6 +374,64 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int* in, long n, const
             flags[i] = in[i] > 0 ? 1 : 0;
         }
         run += loc[k];
+    }
+}
+
+// The same scan in one workgroup, for lattices of at most kScanOneBlock
+// tiles (one launch instead of three): 1024 threads, 4 consecutive tiles
+// each per round, a running carry between rounds.
+constexpr long kScanOneBlock = 1L << 16;
+
+__global__ void __launch_bounds__(1024) k_scan_one(const int* __restrict__ in, long n, int* __restrict__ off,
+                                                  int* __restrict__ active, unsigned char* __restrict__ flags,
+                                                  int* __restrict__ status)
+{
+    __shared__ unsigned long long wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long carry = 0;
+    for (long r0 = 0; r0 < n; r0 += 4096) {
+        unsigned long long loc[4];
+        unsigned long long acc = 0;
+        for (int k = 0; k < 4; ++k) {
+            const long i = r0 + threadIdx.x * 4 + k;
+            loc[k] = i < n ? pack_tile(in[i]) : 0ull;
+            acc += loc[k];
+        }
+        unsigned long long inc = acc;  // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long t = __shfl_up(inc, o);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            unsigned long long x = lane < 16 ? wsum[lane] : 0ull;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const unsigned long long t = __shfl_up(x, o);
+                if (lane >= o) x += t;
+            }
+            if (lane < 16) wsum[lane] = x;  // inclusive over waves
+        }
+        __syncthreads();
+        unsigned long long run = carry + (w > 0 ? wsum[w - 1] : 0ull) + inc - acc;
+        const unsigned long long tot = wsum[15];
+        for (int k = 0; k < 4; ++k) {
+            const long i = r0 + threadIdx.x * 4 + k;
+            if (i < n) {
+                off[i] = (int)(run & 0xffffffffull);
+                if (loc[k] >> 32) active[run >> 32] = (int)i;
+                flags[i] = (loc[k] >> 32) ? 1 : 0;
+            }
+            run += loc[k];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        status[0] = (int)(carry >> 32);
+        status[1] = (int)(carry & 0xffffffffull);
     }
 }
 
@@ -725,17 +818,26 @@ __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __r
 }
 
 // ----------------------------------------------------------- spreading
-// One 64-lane wave per touched 4x4x4 tile, one cell per lane: every cell
-// sums g + phi phi phi F_k over the tile's vertices in increasing vertex
-// order, the order of the reference's vertex loop
-// (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes its force.
-// The tile's list is sorted first (in registers when it has at most 64
-// entries, in LDS otherwise).  Then, kChunk entries at a time, lane e forms
-// entry e's weights on the tile's own 4 cells per axis (zero outside the
-// stencil) and puts them with the entry's force into LDS; every cell then
-// walks the chunk in order with broadcast LDS reads.
+// Every cell of a touched 4x4x4 tile sums g + phi phi phi F_k over the
+// tile's vertices in increasing vertex order, the order of the reference's
+// vertex loop (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes
+// its force; one lane per cell.  The tile's list is sorted first.  Then,
+// kChunk entries at a time, lane e forms entry e's weights on the tile's own
+// 4 cells per axis (zero outside the stencil) and puts them with the
+// entry's force into LDS; the cells walk the chunk in order with broadcast
+// LDS reads.
+//   k_spread_small: tiles with at most 64 entries, one wave per tile (four
+//                   independent waves per workgroup), keys sorted in
+//                   registers, one running sum from g in vertex order;
+//   k_spread_big:   the other tiles, one workgroup of four waves per tile,
+//                   keys sorted in LDS; wave w sums the w-th contiguous
+//                   quarter of the sorted list and the four partial sums are
+//                   added to g in wave order.
+// The two take disjoint tiles and may run concurrently.  Both clear the
+// tile's count and fill counters once done, so the next binning starts
+// from zero without a clearing pass.
 constexpr int kChunk = 32;
-constexpr int kSpreadWaves = 4;  // waves per workgroup, each on its own tiles
+constexpr int kSpreadWaves = 4;
 
 // Orders the LDS accesses of one wave (a wave's LDS operations complete in
 // issue order; this keeps the compiler from moving them across).
@@ -746,38 +848,142 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <class S>
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid gd, int Sw,
-                                                              const int* __restrict__ active,
-                                                              int* __restrict__ status, const int* __restrict__ count,
-                                                              const int* __restrict__ off,
-                                                              const int* __restrict__ list,
-                                                              const int* __restrict__ base,
-                                                              const double* __restrict__ W,
-                                                              const double* __restrict__ vforce,
-                                                              typename Pop<S>::F* __restrict__ Fout)
+// Lane `lane` (< kChunk) stages entry v (tile-local weights, force) into
+// ew[.][lane].
+__device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int org[3], long v,
+                                            const int* __restrict__ base, const double* __restrict__ W,
+                                            const double* __restrict__ vforce, double (*ew)[kChunk + 1], int lane)
 {
-    using FT = typename Pop<S>::F;
-    __shared__ int keys_all[kSpreadWaves][kMaxList];
-    // per entry e: ew[0..11][e] the x, y, z weights on the tile's 4 cells per
-    // axis, ew[12..14][e] the force (rows padded against bank conflicts)
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int b = base[3 * v + ax];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            int d = org[ax] + l - b;
+            if (gd.per[ax] && d < 0) d += gd.n[ax];
+            ew[4 * ax + l][lane] = (unsigned)d < (unsigned)Sw ? W[12 * v + 4 * ax + d] : 0.0;
+        }
+    }
+    ew[12][lane] = vforce[3 * v];
+    ew[13][lane] = vforce[3 * v + 1];
+    ew[14][lane] = vforce[3 * v + 2];
+}
+
+__device__ __forceinline__ void chunk_sum(const double (*ew)[kChunk + 1], int cnt, int lx, int ly, int lz,
+                                          double& a0, double& a1, double& a2)
+{
+    for (int e = 0; e < cnt; ++e) {
+        const double w = (ew[lx][e] * ew[4 + ly][e]) * ew[8 + lz][e];
+        a0 += w * ew[12][e];
+        a1 += w * ew[13][e];
+        a2 += w * ew[14][e];
+    }
+}
+
+__device__ __forceinline__ void tile_origin(const Grid& gd, int t, int org[3])
+{
+    const int tz = t % gd.nt[2], rr = t / gd.nt[2], ty = rr % gd.nt[1], tx = rr / gd.nt[1];
+    org[0] = tx * kTile;
+    org[1] = ty * kTile;
+    org[2] = tz * kTile;
+}
+
+template <class FT>
+__device__ __forceinline__ void store_cell(const LatView& lv, const Grid& gd, const int org[3], int lx, int ly,
+                                           int lz, double a0, double a1, double a2, FT* __restrict__ Fout)
+{
+    const int x = org[0] + lx, y = org[1] + ly, z = org[2] + lz;
+    if (x < gd.n[0] && y < gd.n[1] && z < gd.n[2]) {
+        const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
+        Fout[c] = (FT)a0;
+        Fout[lv.stride + c] = (FT)a1;
+        Fout[2 * lv.stride + c] = (FT)a2;
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_small(LatView lv, Grid gd, int Sw,
+                                                                    const int* __restrict__ active,
+                                                                    const int* __restrict__ status,
+                                                                    int* __restrict__ count, int* __restrict__ fill,
+                                                                    const int* __restrict__ off,
+                                                                    const int* __restrict__ list,
+                                                                    const int* __restrict__ base,
+                                                                    const double* __restrict__ W,
+                                                                    const double* __restrict__ vforce,
+                                                                    typename Pop<S>::F* __restrict__ Fout)
+{
     __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
     const int wv = threadIdx.x >> 6;
-    int* keys = keys_all[wv];
     double(*ew)[kChunk + 1] = ew_all[wv];
     const int na = status[0];
     const int lane = threadIdx.x & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     for (int a = blockIdx.x * kSpreadWaves + wv; a < na; a += gridDim.x * kSpreadWaves) {
         const int t = active[a];
+        const int n = count[t];
+        if (n > 64) continue;
+        int key = lane < n ? list[off[t] + lane] : 0x7fffffff;
+#pragma unroll
+        for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const int other = __shfl_xor(key, stride);
+                const bool up = (lane & size) == 0;
+                const bool lower = (lane & stride) == 0;
+                key = (lower == up) ? min(key, other) : max(key, other);
+            }
+        int org[3];
+        tile_origin(gd, t, org);
+        double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
+        for (int c0 = 0; c0 < n; c0 += kChunk) {
+            const int kk = __shfl(key, (c0 + lane) & 63);
+            if (lane < kChunk && c0 + lane < n) stage_entry(gd, Sw, org, kk, base, W, vforce, ew, lane);
+            wave_sync();
+            chunk_sum(ew, min(kChunk, n - c0), lx, ly, lz, acc0, acc1, acc2);
+            wave_sync();
+        }
+        store_cell(lv, gd, org, lx, ly, lz, acc0, acc1, acc2, Fout);
+        if (lane == 0) {
+            count[t] = 0;
+            fill[t] = 0;
+        }
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Grid gd, int Sw,
+                                                              const int* __restrict__ active,
+                                                              int* __restrict__ status, int* __restrict__ count,
+                                                              int* __restrict__ fill, const int* __restrict__ off,
+                                                              const int* __restrict__ list,
+                                                              const int* __restrict__ base,
+                                                              const double* __restrict__ W,
+                                                              const double* __restrict__ vforce,
+                                                              typename Pop<S>::F* __restrict__ Fout)
+{
+    __shared__ int keys[kMaxList];
+    __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
+    __shared__ double part[kSpreadWaves][3][64];
+    const int wv = threadIdx.x >> 6;
+    double(*ew)[kChunk + 1] = ew_all[wv];
+    const int na = status[0];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+        const int t = active[a];
         int n = count[t];
+        if (n <= 64) continue;
         if (n > kMaxList) {  // reported by the host check; the tile keeps only the first entries
-            if (lane == 0) atomicOr(&status[2], 4);
+            if (tid == 0) atomicOr(&status[2], 4);
             n = kMaxList;
         }
         const int o = off[t];
+        // sorted keys: in registers of every wave for n <= 64, else in LDS
         int key = lane < n ? list[o + lane] : 0x7fffffff;
-        if (n <= 64) {
+        const bool small = n <= 64;
+        if (small) {
 #pragma unroll
             for (int size = 2; size <= 64; size <<= 1)
 #pragma unroll
@@ -790,11 +996,11 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid g
         } else {
             int n2 = 1;
             while (n2 < n) n2 <<= 1;
-            for (int k = lane; k < n2; k += 64) keys[k] = k < n ? list[o + k] : 0x7fffffff;
-            wave_sync();
+            for (int k = tid; k < n2; k += blockDim.x) keys[k] = k < n ? list[o + k] : 0x7fffffff;
+            __syncthreads();
             for (int size = 2; size <= n2; size <<= 1)
                 for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
-                    for (int k = lane; k < n2; k += 64) {
+                    for (int k = tid; k < n2; k += blockDim.x) {
                         const int partner = k ^ stride2;
                         if (partner > k) {
                             const bool up = (k & size) == 0;
@@ -805,48 +1011,41 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid g
                             }
                         }
                     }
-                    wave_sync();
+                    __syncthreads();
                 }
         }
-        const int tz = t % gd.nt[2], rr = t / gd.nt[2], ty = rr % gd.nt[1], tx = rr / gd.nt[1];
-        const int org[3] = {tx * kTile, ty * kTile, tz * kTile};
-        double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
-        for (int c0 = 0; c0 < n; c0 += kChunk) {
-            const int jv = c0 + lane;
-            const int kk = n <= 64 ? __shfl(key, jv & 63) : keys[min(jv, n - 1)];
-            if (lane < kChunk && jv < n) {
-                const long v = kk >> 5;
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    const int b = base[3 * v + ax];
-#pragma unroll
-                    for (int l = 0; l < 4; ++l) {
-                        int d = org[ax] + l - b;
-                        if (gd.per[ax] && d < 0) d += gd.n[ax];
-                        ew[4 * ax + l][lane] = (unsigned)d < (unsigned)Sw ? W[12 * v + 4 * ax + d] : 0.0;
-                    }
-                }
-                ew[12][lane] = vforce[3 * v];
-                ew[13][lane] = vforce[3 * v + 1];
-                ew[14][lane] = vforce[3 * v + 2];
-            }
+        int org[3];
+        tile_origin(gd, t, org);
+        // wave w sums the contiguous range [lo, hi) of the sorted entries
+        const int q = (n + kSpreadWaves - 1) / kSpreadWaves;
+        const int lo = min(n, wv * q), hi = min(n, lo + q);
+        double acc0 = 0, acc1 = 0, acc2 = 0;
+        for (int c0 = lo; c0 < hi; c0 += kChunk) {
+            const int e = c0 + lane;
+            const int kk = small ? __shfl(key, e & 63) : keys[min(e, n - 1)];
+            if (lane < kChunk && e < hi) stage_entry(gd, Sw, org, kk, base, W, vforce, ew, lane);
             wave_sync();
-            const int cnt = min(kChunk, n - c0);
-            for (int e = 0; e < cnt; ++e) {
-                const double w = (ew[lx][e] * ew[4 + ly][e]) * ew[8 + lz][e];
-                acc0 += w * ew[12][e];
-                acc1 += w * ew[13][e];
-                acc2 += w * ew[14][e];
-            }
+            chunk_sum(ew, min(kChunk, hi - c0), lx, ly, lz, acc0, acc1, acc2);
             wave_sync();
         }
-        const int x = org[0] + lx, y = org[1] + ly, z = org[2] + lz;
-        if (x < gd.n[0] && y < gd.n[1] && z < gd.n[2]) {
-            const long c = ((long)x * gd.n[1] + y) * gd.n[2] + z;
-            Fout[c] = (FT)acc0;
-            Fout[lv.stride + c] = (FT)acc1;
-            Fout[2 * lv.stride + c] = (FT)acc2;
+        part[wv][0][lane] = acc0;
+        part[wv][1][lane] = acc1;
+        part[wv][2][lane] = acc2;
+        __syncthreads();
+        if (wv == 0) {
+            double s0 = lv.g[0], s1 = lv.g[1], s2 = lv.g[2];
+            for (int w = 0; w < kSpreadWaves; ++w) {
+                s0 += part[w][0][lane];
+                s1 += part[w][1][lane];
+                s2 += part[w][2][lane];
+            }
+            store_cell(lv, gd, org, lx, ly, lz, s0, s1, s2, Fout);
+            if (lane == 0) {
+                count[t] = 0;
+                fill[t] = 0;
+            }
         }
+        __syncthreads();  // keys and partial sums are rewritten for the next tile
     }
 }
 
@@ -917,53 +1116,54 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int* __
 
 // Chunk partials -> geometry; the centre of mass is then mapped onto the
 // periodic grid (src/fsi/ParticleBase.hh:113-136, src/fsi/Periodicity.h:83-87)
-// and the shift recorded for k_apply_shift.
-__global__ void k_geom_final(int np, Grid gd, const int* __restrict__ p_choff, const double* __restrict__ partial,
-                             double* __restrict__ geom, double* __restrict__ delta, int remap)
+// and the whole particle shifted by the same amount.
+__global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* __restrict__ p_choff,
+                                                  const double* __restrict__ partial, const long* __restrict__ p_voff,
+                                                  double* __restrict__ geom, double* __restrict__ delta,
+                                                  double* __restrict__ pos, int remap)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    double acc[5] = {0, 0, 0, 0, 0};
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int c = p_choff[p]; c < p_choff[p + 1]; ++c) {
-        const double* P = partial + 11 * (long)c;
-        for (int q = 0; q < 5; ++q) acc[q] += P[q];
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], P[5 + 2 * d]);
-            hi[d] = fmax(hi[d], P[6 + 2 * d]);
+    __shared__ double sft[3];
+    const int p = blockIdx.x;
+    if (threadIdx.x == 0) {
+        double acc[5] = {0, 0, 0, 0, 0};
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int c = p_choff[p]; c < p_choff[p + 1]; ++c) {
+            const double* P = partial + 11 * (long)c;
+            for (int q = 0; q < 5; ++q) acc[q] += P[q];
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], P[5 + 2 * d]);
+                hi[d] = fmax(hi[d], P[6 + 2 * d]);
+            }
         }
-    }
-    double* G = geom + 11 * p;
-    const double area = acc[0];
-    double com[3] = {acc[2] / area, acc[3] / area, acc[4] / area};
-    double sft[3] = {0, 0, 0};
-    if (remap) {
+        double* G = geom + 11 * p;
+        const double area = acc[0];
+        double com[3] = {acc[2] / area, acc[3] / area, acc[4] / area};
         for (int d = 0; d < 3; ++d) {
-            if (!gd.per[d]) continue;
+            sft[d] = 0.0;
+            if (!remap || !gd.per[d]) continue;
             const double L = (double)gd.n[d];
             const double x0 = -0.5;
             const double r = com[d] - L * floor((com[d] - x0) * (1. / L));
             sft[d] = r - com[d];
             com[d] = r;
         }
+        G[0] = area;
+        G[1] = acc[1];
+        for (int d = 0; d < 3; ++d) {
+            G[2 + d] = com[d];
+            G[5 + 2 * d] = lo[d] + sft[d];
+            G[6 + 2 * d] = hi[d] + sft[d];
+            delta[3 * p + d] = sft[d];
+        }
     }
-    G[0] = area;
-    G[1] = acc[1];
-    for (int d = 0; d < 3; ++d) {
-        G[2 + d] = com[d];
-        G[5 + 2 * d] = lo[d] + sft[d];
-        G[6 + 2 * d] = hi[d] + sft[d];
-        delta[3 * p + d] = sft[d];
+    __syncthreads();
+    const double s0 = sft[0], s1 = sft[1], s2 = sft[2];
+    if (s0 == 0.0 && s1 == 0.0 && s2 == 0.0) return;
+    for (long v = p_voff[p] + threadIdx.x; v < p_voff[p + 1]; v += blockDim.x) {
+        pos[3 * v] += s0;
+        pos[3 * v + 1] += s1;
+        pos[3 * v + 2] += s2;
     }
-}
-
-__global__ void k_apply_shift(const int* __restrict__ v_part, long NV, const double* __restrict__ delta,
-                              double* __restrict__ pos)
-{
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const int p = v_part[v];
-    for (int d = 0; d < 3; ++d) pos[3 * v + d] += delta[3 * p + d];
 }
 
 // --------------------------------------------------------- voxelization
@@ -1139,62 +1339,70 @@ void mark(bl_ibm g, int k)
     if (g->timing) hipEventRecord(g->ev[k], g->lat->stream);
 }
 
-int update_particles(bl_ibm g, int remap)
+int update_particles(bl_ibm g, int remap, hipStream_t s)
 {
     if (g->np == 0) return BL_OK;
-    hipStream_t s = g->lat->stream;
     const ForceView f = force_view(g);
     const Grid gd = grid_of(g->lat);
     hipLaunchKernelGGL(k_geom_partial, dim3(g->nchunk), dim3(256), 0, s, f, (const int*)g->ch_part,
                        (const int*)g->ch_idx, (const int*)g->p_choff, g->ch_partial);
-    hipLaunchKernelGGL(k_geom_final, dim3(nblk(g->np, 64)), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
-                       (const double*)g->ch_partial, g->p_geom, g->p_delta, remap);
-    if (remap)
-        hipLaunchKernelGGL(k_apply_shift, dim3(nblk(g->NV)), dim3(256), 0, s, (const int*)g->v_part, g->NV,
-                           (const double*)g->p_delta, g->pos);
+    hipLaunchKernelGGL(k_geom_final, dim3(g->np), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
+                       (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, g->pos, remap);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
-// stencil binning: counts, scan (+ next tile flags), list fill
-int do_bin(bl_ibm g)
+int update_particles(bl_ibm g, int remap) { return update_particles(g, remap, g->lat->stream); }
+
+// stencil binning: counts, scan (+ next tile flags), list fill.  The
+// spreading kernels clear the counters of the tiles they consume; a binning
+// that no spreading consumed leaves them set, and the next one clears first.
+// With `scanned` the event is recorded once the scan is done (the staging
+// pass needs the counts, not the lists).
+int do_bin(bl_ibm g, hipEvent_t scanned = nullptr)
 {
     bl_lattice L = g->lat;
     hipStream_t s = L->stream;
     const Grid gd = grid_of(L);
-    hipLaunchKernelGGL(k_clear_bins, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, g->tile_fill, L->ntiles,
-                       g->n_active);
+    if (g->bins_dirty)
+        hipLaunchKernelGGL(k_clear_bins, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, g->tile_fill,
+                           L->ntiles);
     if (g->NV)
         hipLaunchKernelGGL(k_bin_count, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
                            (const double*)g->pos, g->v_base, g->v_w, g->tile_count);
-    const long nb = (L->ntiles + kScanChunk - 1) / kScanChunk;
-    unsigned long long* sums = (unsigned long long*)g->scan_blocks;
-    hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles, sums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, sums, nb, g->n_active);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles,
-                       (const unsigned long long*)sums, g->tile_off, g->active, g->flags_alt);
+    if (L->ntiles <= kScanOneBlock && g->scan_one) {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, (const int*)g->tile_count, L->ntiles, g->tile_off,
+                           g->active, g->flags_alt, g->n_active);
+    } else {
+        const long nb = (L->ntiles + kScanChunk - 1) / kScanChunk;
+        unsigned long long* sums = (unsigned long long*)g->scan_blocks;
+        hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles, sums);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, sums, nb, g->n_active);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles,
+                           (const unsigned long long*)sums, g->tile_off, g->active, g->flags_alt);
+    }
+    if (scanned) BL_HIP(hipEventRecord(scanned, s));
     if (g->NV)
         hipLaunchKernelGGL(k_bin_fill, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV, (const int*)g->v_base,
                            (const int*)g->tile_off, g->tile_fill, g->tile_list, g->list_cap, g->n_active);
     BL_CHECK_LAUNCH();
     g->binned = true;
+    g->bins_dirty = true;
     return BL_OK;
 }
 
 // u on the cells of the touched tiles (before spreading overwrites the
 // force field this reads)
 template <class S>
-void launch_stage_s(bl_ibm g, const LatView& lv, bool swapped)
+void launch_stage_s(bl_ibm g, const LatView& lv, bool swapped, hipStream_t s)
 {
     bl_lattice L = g->lat;
     using V = Vel4<typename Pop<S>::C>;
     const dim3 grid((unsigned)((L->ncell + 255) / 256));
     if (swapped)
-        hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, L->stream, lv, (const int*)g->tile_count,
-                           (V*)g->ut);
+        hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, s, lv, (const int*)g->tile_count, (V*)g->ut);
     else
-        hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, L->stream, lv, (const int*)g->tile_count,
-                           (V*)g->ut);
+        hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, s, lv, (const int*)g->tile_count, (V*)g->ut);
 }
 
 LatView ibm_view(bl_ibm g)
@@ -1204,23 +1412,22 @@ LatView ibm_view(bl_ibm g)
     return lv;
 }
 
-int launch_stage(bl_ibm g)
+int launch_stage(bl_ibm g, hipStream_t s)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
     const LatView lv = ibm_view(g);
     const bool swapped = (L->steps & 1) != 0;
-    if (L->storage == BL_STORE_F64) launch_stage_s<double>(g, lv, swapped);
-    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(g, lv, swapped);
-    else launch_stage_s<__half>(g, lv, swapped);
+    if (L->storage == BL_STORE_F64) launch_stage_s<double>(g, lv, swapped, s);
+    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(g, lv, swapped, s);
+    else launch_stage_s<__half>(g, lv, swapped, s);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
 template <class C, bool MOVE>
-void launch_interp_c(bl_ibm g)
+void launch_interp_c(bl_ibm g, hipStream_t s)
 {
-    hipStream_t s = g->lat->stream;
     const Grid gd = grid_of(g->lat);
     const dim3 grid(nblk((g->S == 2 ? 4 : 16) * g->NV));
     const Vel4<C>* ut = (const Vel4<C>*)g->ut;
@@ -1236,40 +1443,50 @@ void launch_interp_c(bl_ibm g)
 }
 
 // vertex velocities from the staged u; with `move` the vertices advance
-int launch_interp(bl_ibm g, bool move)
+int launch_interp(bl_ibm g, bool move, hipStream_t s)
 {
     if (g->NV == 0) return BL_OK;
     const bool f64 = g->lat->storage == BL_STORE_F64;
-    if (f64 && move) launch_interp_c<double, true>(g);
-    else if (f64) launch_interp_c<double, false>(g);
-    else if (move) launch_interp_c<float, true>(g);
-    else launch_interp_c<float, false>(g);
+    if (f64 && move) launch_interp_c<double, true>(g, s);
+    else if (f64) launch_interp_c<double, false>(g, s);
+    else if (move) launch_interp_c<float, true>(g, s);
+    else launch_interp_c<float, false>(g, s);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
 template <class S>
-void launch_spread_s(bl_ibm g, const LatView& lv)
+void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s_small, hipStream_t s_big)
 {
     bl_lattice L = g->lat;
-    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves,
-                                                                      8192));
-    hipLaunchKernelGGL(k_spread<S>, dim3(grid), dim3(64 * kSpreadWaves), 0, L->stream, lv, grid_of(L), g->S,
-                       (const int*)g->active, g->n_active, (const int*)g->tile_count, (const int*)g->tile_off,
+    using FT = typename Pop<S>::F;
+    const Grid gd = grid_of(L);
+    const unsigned gs = (unsigned)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves,
+                                                                    8192));
+    hipLaunchKernelGGL(k_spread_small<S>, dim3(gs), dim3(64 * kSpreadWaves), 0, s_small, lv, gd, g->S,
+                       (const int*)g->active, (const int*)g->n_active, g->tile_count, g->tile_fill,
+                       (const int*)g->tile_off, (const int*)g->tile_list, (const int*)g->v_base,
+                       (const double*)g->v_w, (const double*)g->force, (FT*)L->force);
+    const unsigned gb = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 4096));
+    hipLaunchKernelGGL(k_spread_big<S>, dim3(gb), dim3(64 * kSpreadWaves), 0, s_big, lv, gd, g->S,
+                       (const int*)g->active, g->n_active, g->tile_count, g->tile_fill, (const int*)g->tile_off,
                        (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
-                       (const double*)g->force, (typename Pop<S>::F*)L->force);
+                       (const double*)g->force, (FT*)L->force);
 }
 
-int launch_spread(bl_ibm g)
+// The two spreading kernels take disjoint tiles; s_big may be another
+// stream than s_small (the caller joins it back).
+int launch_spread(bl_ibm g, hipStream_t s_small, hipStream_t s_big)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
     const LatView lv = ibm_view(g);
-    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv);
-    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv);
-    else launch_spread_s<__half>(g, lv);
+    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv, s_small, s_big);
+    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, s_small, s_big);
+    else launch_spread_s<__half>(g, lv, s_small, s_big);
     BL_CHECK_LAUNCH();
     std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
+    g->bins_dirty = false;
     return BL_OK;
 }
 
@@ -1277,21 +1494,19 @@ int do_interpolate(bl_ibm g)
 {
     int rc;
     if ((rc = do_bin(g))) return rc;
-    if ((rc = launch_stage(g))) return rc;
-    return launch_interp(g, false);
+    if ((rc = launch_stage(g, g->lat->stream))) return rc;
+    return launch_interp(g, false, g->lat->stream);
 }
 
-int do_forces(bl_ibm g)
+int do_forces(bl_ibm g, hipStream_t s)
 {
-    hipStream_t s = g->lat->stream;
     const ForceView f = force_view(g);
     const long ne = g->NT + g->NP + g->NL;
     if (g->NV) {
         hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair,
                            g->ec_link, g->ec);
         hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
-                           g->ec_link,
-                           (const double*)g->ec, g->force);
+                           g->ec_link, (const double*)g->ec, g->force);
     }
     BL_CHECK_LAUNCH();
     return BL_OK;
@@ -1301,7 +1516,7 @@ int do_spread(bl_ibm g)
 {
     int rc;
     if (!g->binned && (rc = do_bin(g))) return rc;
-    return launch_spread(g);
+    return launch_spread(g, g->lat->stream, g->lat->stream);
 }
 
 int do_move(bl_ibm g)
@@ -1313,11 +1528,10 @@ int do_move(bl_ibm g)
     return update_particles(g, 1);
 }
 
-int do_voxelize(bl_ibm g)
+int do_voxelize(bl_ibm g, hipStream_t s)
 {
     bl_lattice L = g->lat;
     if (L->omega_c0 == L->omega_c1 || g->NT == 0) return BL_OK;
-    hipStream_t s = L->stream;
     const Grid gd = grid_of(L);
     const long ncol = (long)L->nx * L->ny;
     const ForceView f = force_view(g);
@@ -1335,43 +1549,94 @@ int do_voxelize(bl_ibm g)
     return BL_OK;
 }
 
+// Overflow flags accumulate on the device over any number of steps; this
+// reads and clears them.
 int check_overflow(bl_ibm g)
 {
     int st[3] = {0, 0, 0};
     BL_HIP(hipMemcpyAsync(st, g->n_active, sizeof(st), hipMemcpyDeviceToHost, g->lat->stream));
     BL_HIP(hipStreamSynchronize(g->lat->stream));
+    if (st[2]) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, g->lat->stream, g->n_active + 2, 1L, 0);
+        BL_CHECK_LAUNCH();
+    }
     if (st[2] & 1) return fail(BL_ERR_CAPACITY, "immersed boundary: tile list capacity exceeded");
     if (st[2] & 2) return fail(BL_ERR_CAPACITY, "voxelization: hit buffer capacity exceeded");
     if (st[2] & 4) return fail(BL_ERR_CAPACITY, "immersed boundary: more than 1024 vertices touch one 4x4x4 tile");
     return BL_OK;
 }
 
-int step_body(bl_ibm g)
+// One IBM step, in one stream (with the stage timing events) ...
+int step_serial(bl_ibm g)
 {
     int rc;
+    hipStream_t s = g->lat->stream;
     mark(g, 0);
     if ((rc = do_bin(g))) return rc;
     mark(g, 1);
-    if ((rc = do_forces(g))) return rc;
+    if ((rc = do_forces(g, s))) return rc;
     mark(g, 2);
-    if ((rc = launch_stage(g))) return rc;
-    if ((rc = launch_spread(g))) return rc;
-    if ((rc = launch_interp(g, true))) return rc;
+    if ((rc = launch_stage(g, s))) return rc;
+    if ((rc = launch_spread(g, s, s))) return rc;
+    if ((rc = launch_interp(g, true, s))) return rc;
     mark(g, 3);
     g->binned = false;
-    if ((rc = update_particles(g, 1))) return rc;
+    if ((rc = update_particles(g, 1, s))) return rc;
     mark(g, 4);
-    if ((rc = do_voxelize(g))) return rc;
+    if ((rc = do_voxelize(g, s))) return rc;
     mark(g, 5);
     return BL_OK;
 }
 
-}  // namespace
-
-int ibm_step_launch(bl_ibm g)
+// ... or forked over the lattice stream M and two more, A and B, so that
+// independent stages overlap:
+//   A: membrane forces (read the positions before the move)
+//   M: binning; once scanned -> B: velocity staging
+//   M: list fill; after forces and staging: small-tile spreading,
+//      A: big-tile spreading
+//   B: (after forces) interpolation + advection, geometry, voxelization
+//   M: joins A and B (the collision reads the force field and the phase;
+//      the next binning reads the moved positions).
+// Staging reads the force field before spreading rewrites it, and the
+// interpolation reads the base / weights the binning wrote.
+int step_forked(bl_ibm g)
 {
     int rc;
-    if ((rc = step_body(g))) return rc;
+    hipStream_t M = g->lat->stream, A = g->aux[0], B = g->aux[1];
+    hipEvent_t* e = g->fev;
+    BL_HIP(hipEventRecord(e[0], M));
+    BL_HIP(hipStreamWaitEvent(A, e[0], 0));
+    if ((rc = do_forces(g, A))) return rc;
+    BL_HIP(hipEventRecord(e[1], A));
+    if ((rc = do_bin(g, e[2]))) return rc;
+    BL_HIP(hipStreamWaitEvent(B, e[2], 0));
+    if ((rc = launch_stage(g, B))) return rc;
+    BL_HIP(hipEventRecord(e[3], B));
+    BL_HIP(hipStreamWaitEvent(M, e[1], 0));
+    BL_HIP(hipStreamWaitEvent(M, e[3], 0));
+    BL_HIP(hipEventRecord(e[4], M));
+    BL_HIP(hipStreamWaitEvent(A, e[4], 0));
+    if ((rc = launch_spread(g, M, A))) return rc;
+    BL_HIP(hipEventRecord(e[5], A));
+    BL_HIP(hipStreamWaitEvent(B, e[1], 0));
+    if ((rc = launch_interp(g, true, B))) return rc;
+    g->binned = false;
+    if ((rc = update_particles(g, 1, B))) return rc;
+    if ((rc = do_voxelize(g, B))) return rc;
+    BL_HIP(hipEventRecord(e[6], B));
+    BL_HIP(hipStreamWaitEvent(M, e[5], 0));
+    BL_HIP(hipStreamWaitEvent(M, e[6], 0));
+    return BL_OK;
+}
+
+int step_body(bl_ibm g) { return g->timing ? step_serial(g) : step_forked(g); }
+
+}  // namespace
+
+int ibm_step_launch(bl_ibm g, bool forked)
+{
+    int rc;
+    if ((rc = forked ? step_body(g) : step_serial(g))) return rc;
     if ((rc = collide_stream_launch(g->lat))) return rc;
     return velocity_walls_launch(g->lat);
 }
@@ -1423,6 +1688,13 @@ int bl_ibm_destroy(bl_ibm g)
     for (void* p : ptrs) hipFree(p);
     for (auto& e : g->ev)
         if (e) hipEventDestroy(e);
+    for (auto& e : g->fev)
+        if (e) hipEventDestroy(e);
+    for (auto& st : g->aux)
+        if (st) {
+            hipStreamSynchronize(st);
+            hipStreamDestroy(st);
+        }
     if (g->graph2) hipGraphExecDestroy(g->graph2);
     delete g;
     return BL_OK;
@@ -1603,6 +1875,7 @@ int bl_ibm_init(bl_ibm g)
             return rc;
     }
     BL_HIP(hipMemset(g->tile_count, 0, L->ntiles * sizeof(int)));
+    BL_HIP(hipMemset(g->tile_fill, 0, L->ntiles * sizeof(int)));
     // voxelization
     const long ncol = (long)L->nx * L->ny;
     g->col_cap = std::max<long>(64 * g->NT, 1024);
@@ -1611,6 +1884,8 @@ int bl_ibm_init(bl_ibm g)
         return rc;
     BL_HIP(hipMemset(g->phase, 0, L->ncell));
     for (auto& e : g->ev) BL_HIP(hipEventCreate(&e));
+    for (auto& st : g->aux) BL_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (auto& e : g->fev) BL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     g->inited = true;
     // update() of every particle and the centre-of-mass mapping of add_particle
     rc = update_particles(g, 1);
@@ -1628,7 +1903,7 @@ int bl_ibm_interpolate(bl_ibm g)
 int bl_ibm_compute_forces(bl_ibm g)
 {
     int rc = need_init(g);
-    return rc ? rc : do_forces(g);
+    return rc ? rc : do_forces(g, g->lat->stream);
 }
 
 int bl_ibm_spread(bl_ibm g)
@@ -1644,7 +1919,7 @@ int bl_ibm_move(bl_ibm g)
     int rc = need_init(g);
     if (rc) return rc;
     if ((rc = do_move(g))) return rc;
-    return do_voxelize(g);
+    return do_voxelize(g, g->lat->stream);
 }
 
 int bl_ibm_step(bl_ibm g)

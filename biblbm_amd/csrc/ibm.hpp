@@ -74,6 +74,8 @@ struct bl_ibm_s {
     long ec_pair = 0, ec_link = 0;  // offsets of the pair and link sections in ec
     void* ut = nullptr;    // staged velocity, (u, 0) per cell (FP64 for FP64 storage, else FP32)
     bool binned = false;   // tile lists match the current positions
+    bool bins_dirty = false;  // tile counters hold a binning no spreading has consumed
+    bool scan_one = false;    // one-workgroup tile scan instead of the three-kernel one
 
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
@@ -107,6 +109,9 @@ struct bl_ibm_s {
     double stage_ms[5] = {0, 0, 0, 0, 0};
     bool timing = false;
 
+    hipStream_t aux[2] = {nullptr, nullptr};  // forked stages of a step (default priority)
+    hipEvent_t fev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
     hipGraphExec_t graph2 = nullptr;  // captured pair of coupled steps
     long long graph_parity = -1;
 };
@@ -114,7 +119,9 @@ struct bl_ibm_s {
 namespace bl {
 // One coupled step's launches on the lattice stream, without host
 // synchronisation: IBM stages, collide-and-stream, velocity-wall pass.
-int ibm_step_launch(bl_ibm g);
+// `forked` lets independent IBM stages run on the boundary's own extra
+// streams; otherwise everything stays on the lattice stream.
+int ibm_step_launch(bl_ibm g, bool forked);
 // Reads the overflow flags of the last steps (synchronises the stream).
 int ibm_check_overflow(bl_ibm g);
 }  // namespace bl
