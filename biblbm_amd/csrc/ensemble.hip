@@ -53,6 +53,21 @@ int order_after(bl_ensemble e)
     return BL_OK;
 }
 
+// what a captured step depends on: the AA parity of the lattice and the
+// tile-list buffer of its immersed boundary
+long long capture_key(bl_ensemble e, size_t i)
+{
+    return (e->lats[i]->steps & 1) | (e->ibms[i] ? (long long)e->ibms[i]->alt << 1 : 0);
+}
+
+bool parity_matches(bl_ensemble e)
+{
+    if (!e->graph || e->parity.size() != e->lats.size()) return false;
+    for (size_t i = 0; i < e->lats.size(); ++i)
+        if (capture_key(e, i) != e->parity[i]) return false;
+    return true;
+}
+
 int capture(bl_ensemble e)
 {
     if (e->graph) {
@@ -86,23 +101,26 @@ int capture(bl_ensemble e)
     hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(BL_ERR_HIP, std::string("ensemble graph: ") + hipGetErrorString(ie));
     e->parity.resize(e->lats.size());
-    for (size_t i = 0; i < e->lats.size(); ++i) e->parity[i] = e->lats[i]->steps & 1;
+    for (size_t i = 0; i < e->lats.size(); ++i) e->parity[i] = capture_key(e, i);
     return BL_OK;
 }
 
-bool parity_matches(bl_ensemble e)
+bool needs_settle(bl_ensemble e)
 {
-    if (!e->graph || e->parity.size() != e->lats.size()) return false;
-    for (size_t i = 0; i < e->lats.size(); ++i)
-        if ((e->lats[i]->steps & 1) != e->parity[i]) return false;
-    return true;
+    for (bl_ibm g : e->ibms)
+        if (g && ibm_needs_settle(g)) return true;
+    return false;
 }
 
 int run_steps(bl_ensemble e, int n, int use_graph)
 {
     int rc;
     int done = 0;
-    if (use_graph && n >= 2) {
+    // force fields loaded into a lattice are consumed over two eager steps
+    for (; done < n && needs_settle(e); ++done)
+        for (size_t i = 0; i < e->lats.size(); ++i)
+            if ((rc = box_step(e, i))) return rc;
+    if (use_graph && n - done >= 2) {
         if (!parity_matches(e) && (rc = capture(e))) return rc;
         if ((rc = order_before(e))) return rc;
         for (; done + 2 <= n; done += 2) {

@@ -32,8 +32,8 @@ namespace bl {
 
 namespace {
 
-constexpr int kMaxList = 1024;     // vertices per tile the spreading kernel sorts in LDS
-constexpr int kScanChunk = 1024;   // items per scan block
+constexpr int kTileCap = 512;      // list entries per 4x4x4 tile (stencils touching it)
+constexpr int kMaxList = kTileCap; // entries the spreading kernel sorts in LDS
 
 // ------------------------------------------------------------- helpers
 // Delta kernels (src/fsi/Dirac.h:23-66,74-101), 1-D factor.
@@ -183,17 +183,63 @@ __device__ __forceinline__ void hash_clear(int* keys, int* a, int* b)
     }
 }
 
-// Stencil base, 1-D weights and the tiles the stencil touches; counts the
-// (tile, vertex) pairs per tile.
-__global__ void __launch_bounds__(256) k_bin_count(Grid gd, int kernel, int S, double h, long NV,
-                                                   const double* __restrict__ pos, int* __restrict__ base,
-                                                   double* __restrict__ W, int* __restrict__ count)
+// Starts a binning into list `act` / flag buffer `flags`: clears the flags
+// of the tiles this list held (two steps back), then the last workgroup to
+// finish empties the list (every workgroup has read its length by then).
+// st[0] list length, st[1] pair total, st[2] arrival counter.
+__global__ void __launch_bounds__(256) k_bin_prep(const int* __restrict__ act, int* __restrict__ na,
+                                                 unsigned char* __restrict__ flags, int* __restrict__ entries,
+                                                 int* __restrict__ arrive)
+{
+    __shared__ int n;
+    if (threadIdx.x == 0) n = *na;
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) flags[act[i]] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(arrive, 1) == (int)gridDim.x - 1) {
+            *na = 0;
+            *entries = 0;
+            *arrive = 0;
+        }
+    }
+}
+
+// The first pair that reaches a tile (outside a workgroup's table) lists it
+// as touched and flags it as carrying spread force.
+__device__ __forceinline__ void touch_tile(int t, int* act, int* na, unsigned char* flags)
+{
+    act[atomicAdd(na, 1)] = t;
+    flags[t] = 1;
+}
+
+// Binning in one pass: stencil base and 1-D weights per vertex, the tiles
+// its stencil touches, and every (tile, vertex) pair written into the
+// tile's fixed-capacity list (entry = vertex index).  Per workgroup the
+// pairs are counted per distinct tile in LDS, one global atomic per tile
+// reserves the workgroup's run, and the slots inside the run come from LDS.
+// The order within a list is whatever the atomics give; spreading sorts.
+__global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double h, long NV,
+                                             const double* __restrict__ pos, int* __restrict__ base,
+                                             double* __restrict__ W, int* __restrict__ count,
+                                             int* __restrict__ list, int cap, int* __restrict__ act,
+                                             int* __restrict__ na, unsigned char* __restrict__ flags,
+                                             int* __restrict__ status)
 {
     __shared__ int keys[kHash];
-    __shared__ int cnt[kHash];
-    hash_clear(keys, cnt, nullptr);
+    __shared__ int cnt[kHash];  // pairs per slot, then the slot's first index in the tile's list
+    __shared__ int rank[kHash];
+    __shared__ int fresh[kHash];  // tiles this workgroup reached first
+    __shared__ int n_fresh, n_pairs, fresh_base;
+    hash_clear(keys, cnt, rank);
+    if (threadIdx.x == 0) {
+        n_fresh = 0;
+        n_pairs = 0;
+    }
     __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    StencilTiles st;
     if (v < NV) {
         int b[3];
         for (int a = 0; a < 3; ++a) {
@@ -208,43 +254,7 @@ __global__ void __launch_bounds__(256) k_bin_count(Grid gd, int kernel, int S, d
             b[a] = bw;
             base[3 * v + a] = bw;
         }
-        StencilTiles st;
         st.init(gd, S, b);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int tt = st.at(gd, i, j, k);
-                    if (tt < 0) continue;
-                    const int sl = hash_insert(keys, tt);
-                    if (sl >= 0) atomicAdd(&cnt[sl], 1);
-                    else atomicAdd(&count[tt], 1);
-                }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHash; i += blockDim.x)
-        if (cnt[i] > 0) atomicAdd(&count[keys[i]], cnt[i]);
-}
-
-// Writes each (tile, vertex) pair into its tile's list (entry = vertex
-// index).  Each workgroup reserves one range per distinct tile and hands
-// out slots inside it from LDS.  The order within a tile is whatever the
-// atomics give; the spreading kernel sorts.
-__global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const int* __restrict__ base,
-                                                  const int* __restrict__ off, int* __restrict__ fill,
-                                                  int* __restrict__ list, long cap, int* __restrict__ status)
-{
-    __shared__ int keys[kHash];
-    __shared__ int cnt[kHash];  // pairs per slot, then the slot's first list index
-    __shared__ int rank[kHash];
-    hash_clear(keys, cnt, rank);
-    __syncthreads();
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    StencilTiles st;
-    if (v < NV) {
-        st.init(gd, S, base + 3 * v);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -260,9 +270,25 @@ __global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const
     __syncthreads();
     for (int i = threadIdx.x; i < kHash; i += blockDim.x) {
         const int c = cnt[i];
-        if (c > 0) cnt[i] = off[keys[i]] + atomicAdd(&fill[keys[i]], c);
+        if (c > 0) {
+            const int t = keys[i];
+            const int old = atomicAdd(&count[t], c);
+            cnt[i] = old;
+            if (old == 0) fresh[atomicAdd(&n_fresh, 1)] = t;
+            atomicAdd(&n_pairs, c);
+            if (old + c > cap) atomicOr(&status[2], 1);
+        }
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        fresh_base = n_fresh ? atomicAdd(na, n_fresh) : 0;
+        if (n_pairs) atomicAdd(&status[3], n_pairs);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_fresh; i += blockDim.x) {
+        act[fresh_base + i] = fresh[i];
+        flags[fresh[i]] = 1;
+    }
     if (v < NV) {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -273,166 +299,30 @@ __global__ void __launch_bounds__(256) k_bin_fill(Grid gd, int S, long NV, const
                     const int tt = st.at(gd, i, j, k);
                     if (tt < 0) continue;
                     const int sl = hash_find(keys, tt);
-                    const long slot = sl >= 0 ? (long)cnt[sl] + atomicAdd(&rank[sl], 1)
-                                              : (long)off[tt] + atomicAdd(&fill[tt], 1);
-                    if (slot < cap) list[slot] = (int)v;
-                    else atomicOr(&status[2], 1);
+                    int slot;
+                    if (sl >= 0) {
+                        slot = cnt[sl] + atomicAdd(&rank[sl], 1);
+                    } else {  // the workgroup's table was full for this tile
+                        slot = atomicAdd(&count[tt], 1);
+                        if (slot == 0) touch_tile(tt, act, na, flags);
+                        atomicAdd(&status[3], 1);
+                        if (slot >= cap) atomicOr(&status[2], 1);
+                    }
+                    if (slot < cap) list[(long)tt * cap + slot] = (int)v;
                 }
     }
 }
 
-__global__ void k_clear_bins(int* count, int* fill, long n)
+__global__ void k_clear_bins(int* count, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        count[i] = 0;
-        fill[i] = 0;
-    }
+    if (i < n) count[i] = 0;
 }
 
-// --------------------------------------------------------------- scan
-// Exclusive scan of tile counts into offsets, plus compaction of the tiles
-// with a non-zero count into `active`.  Packed 64-bit values carry
-// (count, count > 0) so one scan yields both.
-__device__ __forceinline__ unsigned long long pack_tile(int c)
+__global__ void k_zero_u8(unsigned char* p, long n)
 {
-    return (unsigned long long)(unsigned)c | ((unsigned long long)(c > 0) << 32);
-}
-
-__global__ void __launch_bounds__(256) k_scan_partial(const int* in, long n, unsigned long long* sums)
-{
-    __shared__ unsigned long long sh[256];
-    const long b0 = (long)blockIdx.x * kScanChunk;
-    unsigned long long acc = 0;
-    for (int k = 0; k < 4; ++k) {
-        const long i = b0 + threadIdx.x * 4 + k;
-        if (i < n) acc += pack_tile(in[i]);
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
-}
-
-// exclusive scan of the block sums (single block, any count); writes the
-// totals into status[0] (active tiles) and status[1] (list entries)
-__global__ void __launch_bounds__(256) k_scan_top(unsigned long long* sums, long nb, int* status)
-{
-    __shared__ unsigned long long sh[256];
-    unsigned long long carry = 0;
-    for (long c0 = 0; c0 < nb; c0 += 256) {
-        const long i = c0 + threadIdx.x;
-        const unsigned long long v = i < nb ? sums[i] : 0ull;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const unsigned long long t = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0ull;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < nb) sums[i] = carry + sh[threadIdx.x] - v;
-        const unsigned long long tot = sh[255];
-        __syncthreads();
-        carry += tot;
-    }
-    if (threadIdx.x == 0) {
-        status[0] = (int)(carry >> 32);
-        status[1] = (int)(carry & 0xffffffffull);
-    }
-}
-
-__global__ void __launch_bounds__(256) k_scan_apply(const int* in, long n, const unsigned long long* sums, int* off,
-                                                    int* active, unsigned char* flags)
-{
-    __shared__ unsigned long long sh[256];
-    const long b0 = (long)blockIdx.x * kScanChunk;
-    unsigned long long loc[4];
-    unsigned long long acc = 0;
-    for (int k = 0; k < 4; ++k) {
-        const long i = b0 + threadIdx.x * 4 + k;
-        loc[k] = i < n ? pack_tile(in[i]) : 0ull;
-        acc += loc[k];
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const unsigned long long t = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0ull;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
-    }
-    unsigned long long run = sums[blockIdx.x] + sh[threadIdx.x] - acc;
-    for (int k = 0; k < 4; ++k) {
-        const long i = b0 + threadIdx.x * 4 + k;
-        if (i < n) {
-            off[i] = (int)(run & 0xffffffffull);
-            if (in[i] > 0) active[run >> 32] = (int)i;
-            flags[i] = in[i] > 0 ? 1 : 0;
-        }
-        run += loc[k];
-    }
-}
-
-// The same scan in one workgroup, for lattices of at most kScanOneBlock
-// tiles (one launch instead of three): 1024 threads, 4 consecutive tiles
-// each per round, a running carry between rounds.
-constexpr long kScanOneBlock = 1L << 16;
-
-__global__ void __launch_bounds__(1024) k_scan_one(const int* __restrict__ in, long n, int* __restrict__ off,
-                                                  int* __restrict__ active, unsigned char* __restrict__ flags,
-                                                  int* __restrict__ status)
-{
-    __shared__ unsigned long long wsum[16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned long long carry = 0;
-    for (long r0 = 0; r0 < n; r0 += 4096) {
-        unsigned long long loc[4];
-        unsigned long long acc = 0;
-        for (int k = 0; k < 4; ++k) {
-            const long i = r0 + threadIdx.x * 4 + k;
-            loc[k] = i < n ? pack_tile(in[i]) : 0ull;
-            acc += loc[k];
-        }
-        unsigned long long inc = acc;  // inclusive scan over the wave
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long t = __shfl_up(inc, o);
-            if (lane >= o) inc += t;
-        }
-        if (lane == 63) wsum[w] = inc;
-        __syncthreads();
-        if (w == 0) {
-            unsigned long long x = lane < 16 ? wsum[lane] : 0ull;
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                const unsigned long long t = __shfl_up(x, o);
-                if (lane >= o) x += t;
-            }
-            if (lane < 16) wsum[lane] = x;  // inclusive over waves
-        }
-        __syncthreads();
-        unsigned long long run = carry + (w > 0 ? wsum[w - 1] : 0ull) + inc - acc;
-        const unsigned long long tot = wsum[15];
-        for (int k = 0; k < 4; ++k) {
-            const long i = r0 + threadIdx.x * 4 + k;
-            if (i < n) {
-                off[i] = (int)(run & 0xffffffffull);
-                if (loc[k] >> 32) active[run >> 32] = (int)i;
-                flags[i] = (loc[k] >> 32) ? 1 : 0;
-            }
-            run += loc[k];
-        }
-        carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        status[0] = (int)(carry >> 32);
-        status[1] = (int)(carry & 0xffffffffull);
-    }
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 __global__ void k_fill_i32(int* p, long n, int v)
@@ -834,8 +724,8 @@ __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __r
 //                   quarter of the sorted list and the four partial sums are
 //                   added to g in wave order.
 // The two take disjoint tiles and may run concurrently.  Both clear the
-// tile's count and fill counters once done, so the next binning starts
-// from zero without a clearing pass.
+// tile's count once done, so the next binning starts from zero without a
+// clearing pass.
 constexpr int kChunk = 32;
 constexpr int kSpreadWaves = 4;
 
@@ -904,9 +794,8 @@ __device__ __forceinline__ void store_cell(const LatView& lv, const Grid& gd, co
 template <class S>
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_small(LatView lv, Grid gd, int Sw,
                                                                     const int* __restrict__ active,
-                                                                    const int* __restrict__ status,
-                                                                    int* __restrict__ count, int* __restrict__ fill,
-                                                                    const int* __restrict__ off,
+                                                                    const int* __restrict__ nactive,
+                                                                    int* __restrict__ count, int cap,
                                                                     const int* __restrict__ list,
                                                                     const int* __restrict__ base,
                                                                     const double* __restrict__ W,
@@ -916,14 +805,14 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_small(LatView lv, 
     __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
-    const int na = status[0];
+    const int na = *nactive;
     const int lane = threadIdx.x & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     for (int a = blockIdx.x * kSpreadWaves + wv; a < na; a += gridDim.x * kSpreadWaves) {
         const int t = active[a];
         const int n = count[t];
         if (n > 64) continue;
-        int key = lane < n ? list[off[t] + lane] : 0x7fffffff;
+        int key = lane < n ? list[(long)t * cap + lane] : 0x7fffffff;
 #pragma unroll
         for (int size = 2; size <= 64; size <<= 1)
 #pragma unroll
@@ -944,19 +833,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_small(LatView lv, 
             wave_sync();
         }
         store_cell(lv, gd, org, lx, ly, lz, acc0, acc1, acc2, Fout);
-        if (lane == 0) {
-            count[t] = 0;
-            fill[t] = 0;
-        }
+        if (lane == 0) count[t] = 0;
     }
 }
 
 template <class S>
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Grid gd, int Sw,
-                                                              const int* __restrict__ active,
-                                                              int* __restrict__ status, int* __restrict__ count,
-                                                              int* __restrict__ fill, const int* __restrict__ off,
-                                                              const int* __restrict__ list,
+                                                                  const int* __restrict__ active,
+                                                                  const int* __restrict__ nactive,
+                                                                  int* __restrict__ count, int cap,
+                                                                  const int* __restrict__ list,
                                                               const int* __restrict__ base,
                                                               const double* __restrict__ W,
                                                               const double* __restrict__ vforce,
@@ -967,7 +853,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Gr
     __shared__ double part[kSpreadWaves][3][64];
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
-    const int na = status[0];
+    const int na = *nactive;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
@@ -975,11 +861,8 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Gr
         const int t = active[a];
         int n = count[t];
         if (n <= 64) continue;
-        if (n > kMaxList) {  // reported by the host check; the tile keeps only the first entries
-            if (tid == 0) atomicOr(&status[2], 4);
-            n = kMaxList;
-        }
-        const int o = off[t];
+        n = min(n, cap);  // a longer list was cut at binning and reported there
+        const long o = (long)t * cap;
         // sorted keys: in registers of every wave for n <= 64, else in LDS
         int key = lane < n ? list[o + lane] : 0x7fffffff;
         const bool small = n <= 64;
@@ -1040,10 +923,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Gr
                 s2 += part[w][2][lane];
             }
             store_cell(lv, gd, org, lx, ly, lz, s0, s1, s2, Fout);
-            if (lane == 0) {
-                count[t] = 0;
-                fill[t] = 0;
-            }
+            if (lane == 0) count[t] = 0;
         }
         __syncthreads();  // keys and partial sums are rewritten for the next tile
     }
@@ -1331,6 +1211,7 @@ int need_init(bl_ibm g)
 {
     if (!g) return fail(BL_ERR_ARG, "null ibm handle");
     if (!g->inited) return fail(BL_ERR_STATE, "immersed boundary not initialised (call bl_ibm_init)");
+    if (!g->lat) return fail(BL_ERR_STATE, "the lattice of this immersed boundary was destroyed");
     return BL_OK;
 }
 
@@ -1354,40 +1235,34 @@ int update_particles(bl_ibm g, int remap, hipStream_t s)
 
 int update_particles(bl_ibm g, int remap) { return update_particles(g, remap, g->lat->stream); }
 
-// stencil binning: counts, scan (+ next tile flags), list fill.  The
-// spreading kernels clear the counters of the tiles they consume; a binning
-// that no spreading consumed leaves them set, and the next one clears first.
-// With `scanned` the event is recorded once the scan is done (the staging
-// pass needs the counts, not the lists).
-int do_bin(bl_ibm g, hipEvent_t scanned = nullptr)
+// Stencil binning into the tile lists, the touched-tile list and the force
+// flags of this step (buffer g->alt).  The spreading kernels clear the
+// counts of the tiles they consume; a binning that no spreading consumed
+// leaves them set, and the next one clears first.  `done` (optional) is
+// recorded once the lists are complete.
+int do_bin(bl_ibm g, hipEvent_t done = nullptr)
 {
     bl_lattice L = g->lat;
     hipStream_t s = L->stream;
     const Grid gd = grid_of(L);
+    int* act = g->active + (long)g->alt * L->ntiles;
     if (g->bins_dirty)
-        hipLaunchKernelGGL(k_clear_bins, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, g->tile_fill,
-                           L->ntiles);
-    if (g->NV)
-        hipLaunchKernelGGL(k_bin_count, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
-                           (const double*)g->pos, g->v_base, g->v_w, g->tile_count);
-    if (L->ntiles <= kScanOneBlock && g->scan_one) {
-        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, (const int*)g->tile_count, L->ntiles, g->tile_off,
-                           g->active, g->flags_alt, g->n_active);
-    } else {
-        const long nb = (L->ntiles + kScanChunk - 1) / kScanChunk;
-        unsigned long long* sums = (unsigned long long*)g->scan_blocks;
-        hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles, sums);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, sums, nb, g->n_active);
-        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, (const int*)g->tile_count, L->ntiles,
-                           (const unsigned long long*)sums, g->tile_off, g->active, g->flags_alt);
+        hipLaunchKernelGGL(k_clear_bins, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, L->ntiles);
+    if (g->alt_full) {
+        hipLaunchKernelGGL(k_zero_u8, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->flags_alt, L->ntiles);
+        g->alt_full = false;
     }
-    if (scanned) BL_HIP(hipEventRecord(scanned, s));
+    hipLaunchKernelGGL(k_bin_prep, dim3((unsigned)std::min<long>(nblk(L->ntiles), 256)), dim3(256), 0, s,
+                       (const int*)act, g->n_active + g->alt, g->flags_alt, g->n_active + 3, g->n_active + 4);
     if (g->NV)
-        hipLaunchKernelGGL(k_bin_fill, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->S, g->NV, (const int*)g->v_base,
-                           (const int*)g->tile_off, g->tile_fill, g->tile_list, g->list_cap, g->n_active);
+        hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
+                           (const double*)g->pos, g->v_base, g->v_w, g->tile_count, g->tile_list, kTileCap, act,
+                           g->n_active + g->alt, g->flags_alt, g->n_active);
+    if (done) BL_HIP(hipEventRecord(done, s));
     BL_CHECK_LAUNCH();
     g->binned = true;
     g->bins_dirty = true;
+    g->last_list = g->alt;
     return BL_OK;
 }
 
@@ -1461,17 +1336,17 @@ void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s_small, hipStream
     bl_lattice L = g->lat;
     using FT = typename Pop<S>::F;
     const Grid gd = grid_of(L);
+    const int* act = g->active + (long)g->alt * L->ntiles;
+    const int* na = g->n_active + g->alt;
     const unsigned gs = (unsigned)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves,
                                                                     8192));
-    hipLaunchKernelGGL(k_spread_small<S>, dim3(gs), dim3(64 * kSpreadWaves), 0, s_small, lv, gd, g->S,
-                       (const int*)g->active, (const int*)g->n_active, g->tile_count, g->tile_fill,
-                       (const int*)g->tile_off, (const int*)g->tile_list, (const int*)g->v_base,
+    hipLaunchKernelGGL(k_spread_small<S>, dim3(gs), dim3(64 * kSpreadWaves), 0, s_small, lv, gd, g->S, act, na,
+                       g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
                        (const double*)g->v_w, (const double*)g->force, (FT*)L->force);
     const unsigned gb = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 4096));
-    hipLaunchKernelGGL(k_spread_big<S>, dim3(gb), dim3(64 * kSpreadWaves), 0, s_big, lv, gd, g->S,
-                       (const int*)g->active, g->n_active, g->tile_count, g->tile_fill, (const int*)g->tile_off,
-                       (const int*)g->tile_list, (const int*)g->v_base, (const double*)g->v_w,
-                       (const double*)g->force, (FT*)L->force);
+    hipLaunchKernelGGL(k_spread_big<S>, dim3(gb), dim3(64 * kSpreadWaves), 0, s_big, lv, gd, g->S, act, na,
+                       g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
+                       (const double*)g->v_w, (const double*)g->force, (FT*)L->force);
 }
 
 // The two spreading kernels take disjoint tiles; s_big may be another
@@ -1485,7 +1360,12 @@ int launch_spread(bl_ibm g, hipStream_t s_small, hipStream_t s_big)
     else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, s_small, s_big);
     else launch_spread_s<__half>(g, lv, s_small, s_big);
     BL_CHECK_LAUNCH();
+    if (L->flags_full) {  // the buffer leaving the lattice has every tile set
+        g->alt_full = true;
+        L->flags_full = false;
+    }
     std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
+    g->alt ^= 1;
     g->bins_dirty = false;
     return BL_OK;
 }
@@ -1560,9 +1440,10 @@ int check_overflow(bl_ibm g)
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, g->lat->stream, g->n_active + 2, 1L, 0);
         BL_CHECK_LAUNCH();
     }
-    if (st[2] & 1) return fail(BL_ERR_CAPACITY, "immersed boundary: tile list capacity exceeded");
+    if (st[2] & 1)
+        return fail(BL_ERR_CAPACITY, "immersed boundary: more than " + std::to_string(kTileCap) +
+                                         " stencils touch one 4x4x4 tile");
     if (st[2] & 2) return fail(BL_ERR_CAPACITY, "voxelization: hit buffer capacity exceeded");
-    if (st[2] & 4) return fail(BL_ERR_CAPACITY, "immersed boundary: more than 1024 vertices touch one 4x4x4 tile");
     return BL_OK;
 }
 
@@ -1608,7 +1489,7 @@ int step_forked(bl_ibm g)
     BL_HIP(hipStreamWaitEvent(A, e[0], 0));
     if ((rc = do_forces(g, A))) return rc;
     BL_HIP(hipEventRecord(e[1], A));
-    if ((rc = do_bin(g, e[2]))) return rc;
+    if ((rc = do_bin(g, e[2]))) return rc;  // the staging reads the counts
     BL_HIP(hipStreamWaitEvent(B, e[2], 0));
     if ((rc = launch_stage(g, B))) return rc;
     BL_HIP(hipEventRecord(e[3], B));
@@ -1643,6 +1524,15 @@ int ibm_step_launch(bl_ibm g, bool forked)
 
 int ibm_check_overflow(bl_ibm g) { return check_overflow(g); }
 
+void ibm_detach(bl_ibm g)
+{
+    for (auto& st : g->aux)
+        if (st) hipStreamSynchronize(st);
+    g->lat = nullptr;
+}
+
+bool ibm_needs_settle(bl_ibm g) { return g->lat && (g->lat->flags_full || g->alt_full); }
+
 }  // namespace bl
 
 using namespace bl;
@@ -1673,7 +1563,7 @@ int bl_ibm_destroy(bl_ibm g)
     if (!g) return BL_OK;
     if (g->lat) {
         hipStreamSynchronize(g->lat->stream);
-        if (g->lat->phase == g->phase) g->lat->phase = nullptr;
+        if (g->lat->phase == g->phase) g->lat->phase = g->lat->own_phase;
         g->lat->ibm = nullptr;
     }
     void* ptrs[] = {g->s_voff, g->s_toff, g->s_loff, g->s_poff, g->s_vtoff, g->s_vloff, g->s_vpoff, g->s_tri,
@@ -1681,7 +1571,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->s_vt_head, g->s_vt, g->s_vl_head, g->s_vl, g->s_vp_head, g->s_vp, g->s_area0, g->p_shape,
                     g->p_model, g->p_bug, g->p_params, g->p_voff, g->p_toff, g->p_loff, g->p_poff, g->p_geom,
                     g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
-                    g->tile_count, g->tile_off, g->tile_fill, g->tile_list, g->active, g->n_active, g->scan_blocks,
+                    g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
                     g->flags_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_part, g->t_part, g->r_part,
                     g->l_part};
@@ -1848,14 +1738,10 @@ int bl_ibm_init(bl_ibm g)
     BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     // tiles
-    g->list_cap = 27 * std::max<long>(g->NV, 1);
-    g->nscan_blocks = (L->ntiles + kScanChunk - 1) / kScanChunk;
-    if ((rc = alloc(g->tile_count, L->ntiles)) || (rc = alloc(g->tile_off, L->ntiles)) ||
-        (rc = alloc(g->tile_fill, L->ntiles)) || (rc = alloc(g->tile_list, g->list_cap)) ||
-        (rc = alloc(g->active, L->ntiles)) || (rc = alloc(g->n_active, 4)) ||
-        (rc = alloc(g->scan_blocks, 2 * (g->nscan_blocks + 1))))
+    if ((rc = alloc(g->tile_count, L->ntiles)) || (rc = alloc(g->tile_list, (size_t)L->ntiles * kTileCap)) ||
+        (rc = alloc(g->active, 2 * (size_t)L->ntiles)) || (rc = alloc(g->n_active, 8)))
         return rc;
-    BL_HIP(hipMemset(g->n_active, 0, 4 * sizeof(int)));
+    BL_HIP(hipMemset(g->n_active, 0, 8 * sizeof(int)));
     if ((rc = alloc(g->flags_alt, L->ntiles))) return rc;
     BL_HIP(hipMemset(g->flags_alt, 0, L->ntiles));
     {
@@ -1875,7 +1761,6 @@ int bl_ibm_init(bl_ibm g)
             return rc;
     }
     BL_HIP(hipMemset(g->tile_count, 0, L->ntiles * sizeof(int)));
-    BL_HIP(hipMemset(g->tile_fill, 0, L->ntiles * sizeof(int)));
     // voxelization
     const long ncol = (long)L->nx * L->ny;
     g->col_cap = std::max<long>(64 * g->NT, 1024);
@@ -1946,10 +1831,17 @@ int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph)
     bl_lattice L = g->lat;
     if (n_steps < 0) return fail(BL_ERR_ARG, "bl_coupled_steps: negative step count");
     int done = 0;
-    if (use_graph && n_steps >= 2) {
+    // a force field loaded into the lattice is consumed over the next two
+    // steps; those run eagerly, so no captured step depends on it
+    for (; done < n_steps && ibm_needs_settle(g); ++done) {
+        if ((rc = step_body(g))) return rc;
+        if ((rc = collide_stream_launch(L))) return rc;
+        if ((rc = velocity_walls_launch(L))) return rc;
+    }
+    if (use_graph && n_steps - done >= 2) {
         // Two consecutive steps (the two AA parities) captured once and
         // replayed; the captured pair depends on the lattice's step parity.
-        const long long par = L->steps & 1;
+        const long long par = (L->steps & 1) | ((long long)g->alt << 1);
         if (!g->graph2 || g->graph_parity != par) {
             if (g->graph2) {
                 hipGraphExecDestroy(g->graph2);
@@ -2102,11 +1994,11 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2])
     int rc = need_init(g);
     if (rc) return rc;
     if (!out) return fail(BL_ERR_ARG, "bl_ibm_tile_stats: null argument");
-    int st[2] = {0, 0};
+    int st[4] = {0, 0, 0, 0};
     BL_HIP(hipMemcpyAsync(st, g->n_active, sizeof(st), hipMemcpyDeviceToHost, g->lat->stream));
     BL_HIP(hipStreamSynchronize(g->lat->stream));
-    out[0] = st[0];
-    out[1] = st[1];
+    out[0] = st[g->last_list];
+    out[1] = st[3];
     return BL_OK;
 }
 

@@ -75,26 +75,24 @@ struct bl_ibm_s {
     void* ut = nullptr;    // staged velocity, (u, 0) per cell (FP64 for FP64 storage, else FP32)
     bool binned = false;   // tile lists match the current positions
     bool bins_dirty = false;  // tile counters hold a binning no spreading has consumed
-    bool scan_one = false;    // one-workgroup tile scan instead of the three-kernel one
 
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
     double* v_w = nullptr;    // 12 per vertex (3 axes x up to 4 weights)
-    int* tile_count = nullptr;
-    int* tile_off = nullptr;
-    int* tile_fill = nullptr;
-    int* tile_list = nullptr;  // capacity: list_cap
-    long list_cap = 0;
-    int* active = nullptr;     // active tile ids
-    int* n_active = nullptr;   // device scalar [0] = active tiles, [1] = total pairs, [2] = overflow
-    int* scan_blocks = nullptr;
+    int* tile_count = nullptr;  // entries per tile (cleared by the spreading that consumes them)
+    int* tile_list = nullptr;   // per tile a fixed-capacity list of the vertices whose stencil touches it
+    int* active = nullptr;      // two lists of touched tiles, one per force-flag buffer (ntiles each)
+    int* n_active = nullptr;    // [0], [1] lengths of the two lists, [2] overflow flags, [3] pairs of the last
+                                // binning, [4] arrival counter of the list reset
+    int alt = 0;                // list and flag buffer the next binning writes (flags_alt)
+    int last_list = 0;          // list of the last binning
+    bool alt_full = false;      // flags_alt has every tile set: the next binning clears all of it
     unsigned char* flags_alt = nullptr;  // next step's tile flags (swapped with the lattice's)
     int* ch_part = nullptr;              // per geometry chunk: particle
     int* ch_idx = nullptr;               //                     chunk index within the particle
     int* p_choff = nullptr;              // np+1 chunk offsets
     int nchunk = 0;
     double* ch_partial = nullptr;        // 11 per chunk
-    long nscan_blocks = 0;
 
     // ---- voxelization ----
     int* col_count = nullptr;
@@ -124,4 +122,10 @@ namespace bl {
 int ibm_step_launch(bl_ibm g, bool forked);
 // Reads the overflow flags of the last steps (synchronises the stream).
 int ibm_check_overflow(bl_ibm g);
+// The lattice is going away: forget it (the boundary's buffers stay valid
+// until bl_ibm_destroy).
+void ibm_detach(bl_ibm g);
+// A full force field was loaded into the lattice (every tile flagged):
+// steps must run eagerly until the flags are consistent again.
+bool ibm_needs_settle(bl_ibm g);
 }  // namespace bl

@@ -18,6 +18,7 @@
 // (src/fsi/VariableViscosityDynamics.hh:131-141), BGK in the fBar form
 // (src/latticeBoltzmann/dynamicsTemplates3D.h:759-838) and the Guo source
 // (src/fsi/VariableViscosityDynamics.hh:47-129).
+#include "ibm.hpp"
 #include "lattice_dev.hpp"
 
 #include <algorithm>
@@ -657,9 +658,10 @@ int bl_lattice_destroy(bl_lattice L)
 {
     if (!L) return BL_OK;
     if (L->stream) hipStreamSynchronize(L->stream);
+    if (L->ibm) bl::ibm_detach(L->ibm);  // the boundary keeps its own buffers; it cannot step any more
     hipFree(L->pops);
     hipFree(L->mask);
-    hipFree(L->phase);
+    hipFree(L->own_phase);
     hipFree(L->force);
     hipFree(L->tile_flags);
     hipFree(L->d_stage);
@@ -763,6 +765,7 @@ int bl_lattice_set_force_field(bl_lattice L, const double* force)
     const unsigned nt = (unsigned)((L->ntiles + 255) / 256);
     hipLaunchKernelGGL(k_fill_u8, dim3(nt), dim3(256), 0, L->stream, L->tile_flags, L->ntiles, (unsigned char)1);
     BL_CHECK_LAUNCH();
+    L->flags_full = true;
     BL_HIP(hipStreamSynchronize(L->stream));
     hipFree(d);
     return BL_OK;
@@ -773,14 +776,16 @@ int bl_lattice_set_phase(bl_lattice L, const unsigned char* phase)
     if (!L) return fail(BL_ERR_ARG, "null lattice");
     BL_HIP(hipStreamSynchronize(L->stream));
     if (!phase) {
-        hipFree(L->phase);
+        hipFree(L->own_phase);
+        L->own_phase = nullptr;
         L->phase = nullptr;
         return BL_OK;
     }
     for (long c = 0; c < L->ncell; ++c)
         if (phase[c] > 1) return fail(BL_ERR_ARG, "bl_lattice_set_phase: phase must be 0 or 1");
-    if (!L->phase) BL_HIP(hipMalloc(&L->phase, L->ncell));
-    BL_HIP(hipMemcpy(L->phase, phase, L->ncell, hipMemcpyHostToDevice));
+    if (!L->own_phase) BL_HIP(hipMalloc(&L->own_phase, L->ncell));
+    BL_HIP(hipMemcpy(L->own_phase, phase, L->ncell, hipMemcpyHostToDevice));
+    L->phase = L->own_phase;
     return BL_OK;
 }
 

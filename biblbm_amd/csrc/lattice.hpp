@@ -20,7 +20,8 @@ struct bl_lattice_s {
 
     double omega_out = 1.0, omega_in = 1.0;
     double omega_c0 = 1.0, omega_c1 = 1.0;  // omega for phase 0 / 1
-    unsigned char* phase = nullptr;         // ncell, or null (phase 0)
+    unsigned char* phase = nullptr;         // ncell, or null (phase 0): own_phase or an IBM's voxelization
+    unsigned char* own_phase = nullptr;     // the phase set through the lattice (owned here)
 
     // external force: body force g everywhere except in tiles whose flag is
     // set, where force[d*stride + c] holds the full value.
@@ -31,6 +32,7 @@ struct bl_lattice_s {
     int ntx = 0, nty = 0, ntz = 0;
     long ntiles = 0;
     bool force_field = false;  // tiles may carry a field
+    bool flags_full = false;   // every tile flag was set (a full field was loaded); cleared by its consumer
 
     long long steps = 0;  // completed collide-and-stream steps (AA parity)
     hipStream_t stream = nullptr;

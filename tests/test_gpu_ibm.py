@@ -281,3 +281,30 @@ def test_reduced_storage_coupled_error(device, storage, tol):
     err = np.linalg.norm(d2 - d1) / np.linalg.norm(d1)
     print(storage, "displacement rel L2 vs FP64", err)
     assert err <= tol
+
+
+def test_force_field_reload_between_graph_replays(device):
+    """A full force field loaded between captured steps is used exactly as
+    in eager stepping (its tiles are all flagged once, then cleared)."""
+    L1, ib1, *_ = _build_pair("roma")
+    L2, ib2, *_ = _build_pair("roma")
+    ib1.coupled_steps(4, use_graph=True)
+    ib2.coupled_steps(4, use_graph=False)
+    F = np.random.default_rng(9).uniform(-1e-4, 1e-4, SHAPE + (3,))
+    L1.set_force_field(F)
+    L2.set_force_field(F)
+    ib1.coupled_steps(6, use_graph=True)
+    ib2.coupled_steps(6, use_graph=False)
+    assert np.array_equal(ib1.positions(), ib2.positions())
+    assert np.array_equal(L1.get_populations(), L2.get_populations())
+    assert np.array_equal(L1.force(), L2.force())
+
+
+def test_lattice_destroyed_before_boundary(device):
+    from biblbm_amd import capi
+    L, ib, *_ = _build_pair("roma")
+    ib.coupled_steps(2, use_graph=True)
+    L.close()
+    with pytest.raises(capi.BlError):
+        ib.coupled_steps(1)
+    ib.close()
