@@ -58,6 +58,25 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_examples()
+
+
+EXAMPLES = ["capsule_in_shear"]
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_examples():
+    """C++ progs over include/biblbm_plb.h, linked against the library
+    (found next to the binary's folder at run time)."""
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name + ".cpp")
+        out = os.path.join(HERE, "examples", name)
+        rocm_lib = "/opt/rocm/lib"
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, src, "-L" + HERE, "-lbiblbm", "-L" + rocm_lib,
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + rocm_lib, "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("example build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     return LIB
 
 
