@@ -281,7 +281,7 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
     }
     L->block = best_block;
     L->xcd_remap = best_remap;
-    if (L->ibm) L->ibm->graph_parity = -1;  // a captured step holds the old launch shape
+    if (L->ibm) L->ibm->graph_parity[0] = L->ibm->graph_parity[1] = -1;  // captured steps hold the old launch shape
     char buf[128];
     snprintf(buf, sizeof(buf), "], \"best\": {\"block\": %d, \"xcd_remap\": %d, \"ms\": %.6f}}", best_block,
              best_remap, best);

@@ -183,28 +183,40 @@ __device__ __forceinline__ void hash_clear(int* keys, int* a, int* b)
     }
 }
 
-// Starts a binning into list `act` / flag buffer `flags`: clears the flags
-// of the tiles this list held (two steps back), then the last workgroup to
-// finish empties the list (every workgroup has read its length by then).
-// st[0] list length, st[1] pair total, st[2] arrival counter.
-__global__ void __launch_bounds__(256) k_bin_prep(const int* __restrict__ act, int* __restrict__ na,
-                                                 unsigned char* __restrict__ flags, int* __restrict__ entries,
-                                                 int* __restrict__ arrive)
+// Readies list `act` (length *na) / flag buffer `flags` for a binning:
+// clears the flags of the tiles the list held (two steps back), then the
+// last workgroup to finish empties the list and its pair count.  The first
+// kResetBlocks workgroups of the grid that carries it take part (all their
+// threads); they have read the length before they arrive, so no fence is
+// needed, and a bounded number of them keeps the arrival atomic cheap.
+constexpr int kResetBlocks = 256;
+struct ListReset {
+    const int* act = nullptr;  // null: nothing to reset
+    int* na = nullptr;
+    unsigned char* flags = nullptr;
+    int* pairs = nullptr;
+    int* arrive = nullptr;
+};
+
+__device__ __forceinline__ void list_reset(const ListReset& lr)
 {
+    const int nb = min((int)gridDim.x, kResetBlocks);
+    if ((int)blockIdx.x >= nb) return;
     __shared__ int n;
-    if (threadIdx.x == 0) n = *na;
+    if (threadIdx.x == 0) n = *lr.na;
     __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) flags[act[i]] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) lr.flags[lr.act[i]] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(arrive, 1) == (int)gridDim.x - 1) {
-            *na = 0;
-            *entries = 0;
-            *arrive = 0;
+        if (atomicAdd(lr.arrive, 1) == nb - 1) {
+            *lr.na = 0;
+            *lr.pairs = 0;
+            *lr.arrive = 0;
         }
     }
 }
+
+__global__ void __launch_bounds__(256) k_bin_prep(ListReset lr) { list_reset(lr); }
 
 // The first pair that reaches a tile (outside a workgroup's table) lists it
 // as touched and flags it as carrying spread force.
@@ -225,17 +237,19 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
                                              double* __restrict__ W, int* __restrict__ count,
                                              int* __restrict__ list, int cap, int* __restrict__ act,
                                              int* __restrict__ na, unsigned char* __restrict__ flags,
-                                             int* __restrict__ status)
+                                             int* __restrict__ status, int* __restrict__ pairs)
 {
     __shared__ int keys[kHash];
     __shared__ int cnt[kHash];  // pairs per slot, then the slot's first index in the tile's list
     __shared__ int rank[kHash];
     __shared__ int fresh[kHash];  // tiles this workgroup reached first
-    __shared__ int n_fresh, n_pairs, fresh_base;
+    __shared__ int occ[kHash];    // occupied slots, compacted
+    __shared__ int n_fresh, n_pairs, fresh_base, n_occ;
     hash_clear(keys, cnt, rank);
     if (threadIdx.x == 0) {
         n_fresh = 0;
         n_pairs = 0;
+        n_occ = 0;
     }
     __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,6 +259,7 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
         for (int a = 0; a < 3; ++a) {
             const double x = pos[3 * v + a];
             const int raw = (int)ceil(x - h);
+#pragma unroll 1
             for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(raw + k) - x) : 0.0;
             int bw = raw;
             if (gd.per[a]) {
@@ -255,34 +270,38 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
             base[3 * v + a] = bw;
         }
         st.init(gd, S, b);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int tt = st.at(gd, i, j, k);
-                    if (tt < 0) continue;
-                    const int sl = hash_insert(keys, tt);
+        // runtime loops: the kernel runs once per launch on few workgroups,
+        // so compact code (instruction fetch) beats unrolling
+#pragma unroll 1
+        for (int i = 0; i < st.ax[0].m; ++i)
+#pragma unroll 1
+            for (int j = 0; j < st.ax[1].m; ++j)
+#pragma unroll 1
+                for (int k = 0; k < st.ax[2].m; ++k) {
+                    const int sl = hash_insert(keys, st.at(gd, i, j, k));
                     if (sl >= 0) atomicAdd(&cnt[sl], 1);
                 }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kHash; i += blockDim.x) {
+    // one global atomic per distinct tile; the occupied slots are compacted
+    // first so that each lane waits for one atomic, not one per slot it scans
+    for (int i = threadIdx.x; i < kHash; i += blockDim.x)
+        if (cnt[i] > 0) occ[atomicAdd(&n_occ, 1)] = i;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n_occ; k += blockDim.x) {
+        const int i = occ[k];
         const int c = cnt[i];
-        if (c > 0) {
-            const int t = keys[i];
-            const int old = atomicAdd(&count[t], c);
-            cnt[i] = old;
-            if (old == 0) fresh[atomicAdd(&n_fresh, 1)] = t;
-            atomicAdd(&n_pairs, c);
-            if (old + c > cap) atomicOr(&status[2], 1);
-        }
+        const int t = keys[i];
+        const int old = atomicAdd(&count[t], c);
+        cnt[i] = old;
+        if (old == 0) fresh[atomicAdd(&n_fresh, 1)] = t;
+        atomicAdd(&n_pairs, c);
+        if (old + c > cap) atomicOr(&status[2], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         fresh_base = n_fresh ? atomicAdd(na, n_fresh) : 0;
-        if (n_pairs) atomicAdd(&status[3], n_pairs);
+        if (n_pairs) atomicAdd(pairs, n_pairs);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n_fresh; i += blockDim.x) {
@@ -290,14 +309,13 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
         flags[fresh[i]] = 1;
     }
     if (v < NV) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
+#pragma unroll 1
+        for (int i = 0; i < st.ax[0].m; ++i)
+#pragma unroll 1
+            for (int j = 0; j < st.ax[1].m; ++j)
+#pragma unroll 1
+                for (int k = 0; k < st.ax[2].m; ++k) {
                     const int tt = st.at(gd, i, j, k);
-                    if (tt < 0) continue;
                     const int sl = hash_find(keys, tt);
                     int slot;
                     if (sl >= 0) {
@@ -305,7 +323,7 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
                     } else {  // the workgroup's table was full for this tile
                         slot = atomicAdd(&count[tt], 1);
                         if (slot == 0) touch_tile(tt, act, na, flags);
-                        atomicAdd(&status[3], 1);
+                        atomicAdd(pairs, 1);
                         if (slot >= cap) atomicOr(&status[2], 1);
                     }
                     if (slot < cap) list[(long)tt * cap + slot] = (int)v;
@@ -612,11 +630,13 @@ struct alignas(4 * sizeof(C)) Vel4 {
 // populations and the last collision's force,
 // src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
 // bounce-back / no-dynamics cells, the wall velocity on velocity-wall
-// cells), restricted to the cells of the tiles this step's stencils touch.
-// One thread per cell in z-fastest order, so population reads coalesce as
-// in the collide kernel and the staged vectors are written contiguously.
+// cells), restricted to the cells of the tiles this step's stencils touch
+// (`touched`, the step's tile flags).  `lv` carries the force field and
+// flags of the last collision.  One thread per cell in z-fastest order, so
+// population reads coalesce as in the collide kernel and the staged vectors
+// are written contiguously.
 template <class S, bool SWAPPED>
-__global__ void __launch_bounds__(256) k_stage_u(LatView lv, const int* __restrict__ count,
+__global__ void __launch_bounds__(256) k_stage_u(LatView lv, const unsigned char* __restrict__ touched,
                                                  Vel4<typename Pop<S>::C>* __restrict__ ut)
 {
     using C = typename Pop<S>::C;
@@ -626,7 +646,7 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const int* __restri
     int x, y, z;
     decode(c, lv, x, y, z);
     const unsigned t = ((unsigned)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
-    if (count[t] == 0) return;
+    if (!touched[t]) return;
     const unsigned char m = lv.mask ? lv.mask[c] : (unsigned char)M_FLUID;
     C u[3] = {C(0), C(0), C(0)};
     if (m == M_VWALL) {
@@ -661,8 +681,9 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const int* __restri
 template <class C, int SW, bool MOVE>
 __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __restrict__ base,
                                                 const double* __restrict__ W, const Vel4<C>* __restrict__ ut,
-                                                double* __restrict__ vel, double* __restrict__ pos)
+                                                double* __restrict__ vel, double* __restrict__ pos, ListReset lr)
 {
+    if (lr.act) list_reset(lr);  // the next binning's list, carried by this launch
     constexpr int LA = SW == 2 ? 2 : 4;
     constexpr int LPV = LA * LA;
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -716,16 +737,16 @@ __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __r
 // 4 cells per axis (zero outside the stencil) and puts them with the
 // entry's force into LDS; the cells walk the chunk in order with broadcast
 // LDS reads.
-//   k_spread_small: tiles with at most 64 entries, one wave per tile (four
-//                   independent waves per workgroup), keys sorted in
-//                   registers, one running sum from g in vertex order;
-//   k_spread_big:   the other tiles, one workgroup of four waves per tile,
-//                   keys sorted in LDS; wave w sums the w-th contiguous
-//                   quarter of the sorted list and the four partial sums are
-//                   added to g in wave order.
-// The two take disjoint tiles and may run concurrently.  Both clear the
-// tile's count once done, so the next binning starts from zero without a
-// clearing pass.
+// One launch (k_spread) holds two kinds of workgroups:
+//   small: tiles with at most 64 entries, one wave per tile (four
+//          independent waves per workgroup), keys sorted in registers, one
+//          running sum from g in vertex order;
+//   big:   the other tiles, one workgroup of four waves per tile, keys
+//          sorted in LDS; wave w sums the w-th contiguous quarter of the
+//          sorted list and the four partial sums are added to g in wave
+//          order.
+// Each clears the tile's count once done, so the next binning starts from
+// zero without a clearing pass.
 constexpr int kChunk = 32;
 constexpr int kSpreadWaves = 4;
 
@@ -740,10 +761,20 @@ __device__ __forceinline__ void wave_sync()
 
 // Lane `lane` (< kChunk) stages entry v (tile-local weights, force) into
 // ew[.][lane].
+// The vertex's 12 weights are loaded whole (not indexed by the stencil
+// offset) so that no load waits for the base.
 __device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int org[3], long v,
                                             const int* __restrict__ base, const double* __restrict__ W,
                                             const double* __restrict__ vforce, double (*ew)[kChunk + 1], int lane)
 {
+    const double2* W2 = (const double2*)(W + 12 * v);
+    double w[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double2 p = W2[k];
+        w[2 * k] = p.x;
+        w[2 * k + 1] = p.y;
+    }
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const int b = base[3 * v + ax];
@@ -751,7 +782,9 @@ __device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int or
         for (int l = 0; l < 4; ++l) {
             int d = org[ax] + l - b;
             if (gd.per[ax] && d < 0) d += gd.n[ax];
-            ew[4 * ax + l][lane] = (unsigned)d < (unsigned)Sw ? W[12 * v + 4 * ax + d] : 0.0;
+            const double* wa = w + 4 * ax;
+            const double x = d == 0 ? wa[0] : (d == 1 ? wa[1] : (d == 2 ? wa[2] : wa[3]));
+            ew[4 * ax + l][lane] = (unsigned)d < (unsigned)Sw ? x : 0.0;
         }
     }
     ew[12][lane] = vforce[3 * v];
@@ -792,23 +825,19 @@ __device__ __forceinline__ void store_cell(const LatView& lv, const Grid& gd, co
 }
 
 template <class S>
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_small(LatView lv, Grid gd, int Sw,
-                                                                    const int* __restrict__ active,
-                                                                    const int* __restrict__ nactive,
-                                                                    int* __restrict__ count, int cap,
-                                                                    const int* __restrict__ list,
-                                                                    const int* __restrict__ base,
-                                                                    const double* __restrict__ W,
-                                                                    const double* __restrict__ vforce,
-                                                                    typename Pop<S>::F* __restrict__ Fout)
+__device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[15][kChunk + 1], const LatView& lv,
+                                             const Grid& gd, int Sw, const int* __restrict__ active,
+                                             const int* __restrict__ nactive, int* __restrict__ count, int cap,
+                                             const int* __restrict__ list, const int* __restrict__ base,
+                                             const double* __restrict__ W, const double* __restrict__ vforce,
+                                             typename Pop<S>::F* __restrict__ Fout)
 {
-    __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
     const int na = *nactive;
     const int lane = threadIdx.x & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
-    for (int a = blockIdx.x * kSpreadWaves + wv; a < na; a += gridDim.x * kSpreadWaves) {
+    for (int a = blk * kSpreadWaves + wv; a < na; a += nb * kSpreadWaves) {
         const int t = active[a];
         const int n = count[t];
         if (n > 64) continue;
@@ -838,26 +867,20 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_small(LatView lv, 
 }
 
 template <class S>
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Grid gd, int Sw,
-                                                                  const int* __restrict__ active,
-                                                                  const int* __restrict__ nactive,
-                                                                  int* __restrict__ count, int cap,
-                                                                  const int* __restrict__ list,
-                                                              const int* __restrict__ base,
-                                                              const double* __restrict__ W,
-                                                              const double* __restrict__ vforce,
-                                                              typename Pop<S>::F* __restrict__ Fout)
+__device__ __forceinline__ void spread_big(int blk, int nb, int* keys, double (*ew_all)[15][kChunk + 1],
+                                           double (*part)[3][64], const LatView& lv, const Grid& gd, int Sw,
+                                           const int* __restrict__ active, const int* __restrict__ nactive,
+                                           int* __restrict__ count, int cap, const int* __restrict__ list,
+                                           const int* __restrict__ base, const double* __restrict__ W,
+                                           const double* __restrict__ vforce, typename Pop<S>::F* __restrict__ Fout)
 {
-    __shared__ int keys[kMaxList];
-    __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
-    __shared__ double part[kSpreadWaves][3][64];
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
     const int na = *nactive;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
-    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+    for (int a = blk; a < na; a += nb) {
         const int t = active[a];
         int n = count[t];
         if (n <= 64) continue;
@@ -927,6 +950,30 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_big(LatView lv, Gr
         }
         __syncthreads();  // keys and partial sums are rewritten for the next tile
     }
+}
+
+// Both kinds in one launch: workgroups [0, n_small) take the small tiles,
+// the others the big ones.
+template <class S>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid gd, int Sw, int n_small,
+                                                              const int* __restrict__ active,
+                                                              const int* __restrict__ nactive,
+                                                              int* __restrict__ count, int cap,
+                                                              const int* __restrict__ list,
+                                                              const int* __restrict__ base,
+                                                              const double* __restrict__ W,
+                                                              const double* __restrict__ vforce,
+                                                              typename Pop<S>::F* __restrict__ Fout)
+{
+    __shared__ int keys[kMaxList];
+    __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
+    __shared__ double part[kSpreadWaves][3][64];
+    const int b = (int)blockIdx.x;
+    if (b < n_small)
+        spread_small<S>(b, n_small, ew_all, lv, gd, Sw, active, nactive, count, cap, list, base, W, vforce, Fout);
+    else
+        spread_big<S>(b - n_small, (int)gridDim.x - n_small, keys, ew_all, part, lv, gd, Sw, active, nactive, count,
+                      cap, list, base, W, vforce, Fout);
 }
 
 // --------------------------------------------------------------- motion
@@ -1236,28 +1283,63 @@ int update_particles(bl_ibm g, int remap, hipStream_t s)
 int update_particles(bl_ibm g, int remap) { return update_particles(g, remap, g->lat->stream); }
 
 // Stencil binning into the tile lists, the touched-tile list and the force
-// flags of this step (buffer g->alt).  The spreading kernels clear the
-// counts of the tiles they consume; a binning that no spreading consumed
-// leaves them set, and the next one clears first.  `done` (optional) is
-// recorded once the lists are complete.
-int do_bin(bl_ibm g, hipEvent_t done = nullptr)
+// flags of this step (buffer g->alt), on stream s.  The spreading kernels
+// clear the counts of the tiles they consume; a binning that no spreading
+// consumed leaves them set, and the next one clears first.  `done`
+// (optional) is recorded once the lists are complete.
+//
+// Before it, list g->alt and flag buffer flags_alt are readied: counters a
+// binning left unconsumed are cleared, the flags of the tiles the list held
+// (two steps back) are cleared and the list emptied.  That may run as soon
+// as the last staging has read those flags, carried by the interpolation
+// launch of the step before (prep_rare + list_reset_of), or here.
+ListReset list_reset_of(bl_ibm g)
 {
     bl_lattice L = g->lat;
-    hipStream_t s = L->stream;
-    const Grid gd = grid_of(L);
-    int* act = g->active + (long)g->alt * L->ntiles;
-    if (g->bins_dirty)
+    ListReset lr;
+    lr.act = g->active + (long)g->alt * L->ntiles;
+    lr.na = g->n_active + g->alt;
+    lr.flags = g->flags_alt;
+    lr.pairs = g->n_active + 5 + g->alt;
+    lr.arrive = g->n_active + 4;
+    return lr;
+}
+
+void prep_rare(bl_ibm g, hipStream_t s)
+{
+    bl_lattice L = g->lat;
+    if (g->bins_dirty) {
         hipLaunchKernelGGL(k_clear_bins, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->tile_count, L->ntiles);
+        g->bins_dirty = false;
+    }
     if (g->alt_full) {
         hipLaunchKernelGGL(k_zero_u8, dim3(nblk(L->ntiles)), dim3(256), 0, s, g->flags_alt, L->ntiles);
         g->alt_full = false;
     }
-    hipLaunchKernelGGL(k_bin_prep, dim3((unsigned)std::min<long>(nblk(L->ntiles), 256)), dim3(256), 0, s,
-                       (const int*)act, g->n_active + g->alt, g->flags_alt, g->n_active + 3, g->n_active + 4);
+}
+
+int prep_bin(bl_ibm g, hipStream_t s)
+{
+    prep_rare(g, s);
+    hipLaunchKernelGGL(k_bin_prep, dim3((unsigned)std::min<long>(nblk(g->lat->ntiles), kResetBlocks)), dim3(256), 0, s,
+                       list_reset_of(g));
+    BL_CHECK_LAUNCH();
+    g->prepped = true;
+    return BL_OK;
+}
+
+int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
+{
+    bl_lattice L = g->lat;
+    const Grid gd = grid_of(L);
+    int* act = g->active + (long)g->alt * L->ntiles;
+    int rc;
+    if (!g->prepped && (rc = prep_bin(g, s))) return rc;
+    g->prepped = false;
     if (g->NV)
         hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
                            (const double*)g->pos, g->v_base, g->v_w, g->tile_count, g->tile_list, kTileCap, act,
-                           g->n_active + g->alt, g->flags_alt, g->n_active);
+                           g->n_active + g->alt, g->flags_alt, g->n_active, g->n_active + 5 + g->alt);
     if (done) BL_HIP(hipEventRecord(done, s));
     BL_CHECK_LAUNCH();
     g->binned = true;
@@ -1266,18 +1348,17 @@ int do_bin(bl_ibm g, hipEvent_t done = nullptr)
     return BL_OK;
 }
 
-// u on the cells of the touched tiles (before spreading overwrites the
-// force field this reads)
+// u on the cells of the touched tiles
 template <class S>
-void launch_stage_s(bl_ibm g, const LatView& lv, bool swapped, hipStream_t s)
+void launch_stage_s(bl_ibm g, const LatView& lv, const unsigned char* touched, bool swapped, hipStream_t s)
 {
     bl_lattice L = g->lat;
     using V = Vel4<typename Pop<S>::C>;
     const dim3 grid((unsigned)((L->ncell + 255) / 256));
     if (swapped)
-        hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, s, lv, (const int*)g->tile_count, (V*)g->ut);
+        hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, s, lv, touched, (V*)g->ut);
     else
-        hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, s, lv, (const int*)g->tile_count, (V*)g->ut);
+        hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, s, lv, touched, (V*)g->ut);
 }
 
 LatView ibm_view(bl_ibm g)
@@ -1287,84 +1368,101 @@ LatView ibm_view(bl_ibm g)
     return lv;
 }
 
-int launch_stage(bl_ibm g, hipStream_t s)
+// Velocity staging of the step whose binning is the last one.  Before that
+// binning's spreading (`spread` false) the last collision's force field and
+// flags are the lattice's, and the binning's flags are in flags_alt; after
+// it (`spread` true) the two have been swapped.
+int launch_stage(bl_ibm g, hipStream_t s, bool spread)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
-    const LatView lv = ibm_view(g);
+    LatView lv = ibm_view(g);
+    const unsigned char* touched = g->flags_alt;
+    if (spread) {
+        lv.force = g->force_alt;
+        if (lv.flags) lv.flags = g->flags_alt;
+        touched = L->tile_flags;
+    }
     const bool swapped = (L->steps & 1) != 0;
-    if (L->storage == BL_STORE_F64) launch_stage_s<double>(g, lv, swapped, s);
-    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(g, lv, swapped, s);
-    else launch_stage_s<__half>(g, lv, swapped, s);
+    if (L->storage == BL_STORE_F64) launch_stage_s<double>(g, lv, touched, swapped, s);
+    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(g, lv, touched, swapped, s);
+    else launch_stage_s<__half>(g, lv, touched, swapped, s);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
 template <class C, bool MOVE>
-void launch_interp_c(bl_ibm g, hipStream_t s)
+void launch_interp_c(bl_ibm g, hipStream_t s, const ListReset& lr)
 {
     const Grid gd = grid_of(g->lat);
     const dim3 grid(nblk((g->S == 2 ? 4 : 16) * g->NV));
     const Vel4<C>* ut = (const Vel4<C>*)g->ut;
     if (g->S == 4)
         hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos);
+                           (const double*)g->v_w, ut, g->vel, g->pos, lr);
     else if (g->S == 3)
         hipLaunchKernelGGL((k_interp<C, 3, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos);
+                           (const double*)g->v_w, ut, g->vel, g->pos, lr);
     else
         hipLaunchKernelGGL((k_interp<C, 2, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos);
+                           (const double*)g->v_w, ut, g->vel, g->pos, lr);
 }
 
-// vertex velocities from the staged u; with `move` the vertices advance
-int launch_interp(bl_ibm g, bool move, hipStream_t s)
+// vertex velocities from the staged u; with `move` the vertices advance.
+// `reset_next` (after this step's spreading, once its staging is done):
+// the launch also readies the list of the next binning.
+int launch_interp(bl_ibm g, bool move, hipStream_t s, bool reset_next = false)
 {
     if (g->NV == 0) return BL_OK;
+    ListReset lr;
+    if (reset_next) {
+        prep_rare(g, s);
+        lr = list_reset_of(g);
+        g->prepped = true;
+    }
     const bool f64 = g->lat->storage == BL_STORE_F64;
-    if (f64 && move) launch_interp_c<double, true>(g, s);
-    else if (f64) launch_interp_c<double, false>(g, s);
-    else if (move) launch_interp_c<float, true>(g, s);
-    else launch_interp_c<float, false>(g, s);
+    if (f64 && move) launch_interp_c<double, true>(g, s, lr);
+    else if (f64) launch_interp_c<double, false>(g, s, lr);
+    else if (move) launch_interp_c<float, true>(g, s, lr);
+    else launch_interp_c<float, false>(g, s, lr);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
 template <class S>
-void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s_small, hipStream_t s_big)
+void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s)
 {
     bl_lattice L = g->lat;
     using FT = typename Pop<S>::F;
     const Grid gd = grid_of(L);
     const int* act = g->active + (long)g->alt * L->ntiles;
     const int* na = g->n_active + g->alt;
-    const unsigned gs = (unsigned)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves,
-                                                                    8192));
-    hipLaunchKernelGGL(k_spread_small<S>, dim3(gs), dim3(64 * kSpreadWaves), 0, s_small, lv, gd, g->S, act, na,
+    const int gs = (int)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves, 4096));
+    const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, 2048));
+    hipLaunchKernelGGL(k_spread<S>, dim3(gs + gb), dim3(64 * kSpreadWaves), 0, s, lv, gd, g->S, gs, act, na,
                        g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
-                       (const double*)g->v_w, (const double*)g->force, (FT*)L->force);
-    const unsigned gb = (unsigned)std::max<long>(1, std::min<long>(L->ntiles, 4096));
-    hipLaunchKernelGGL(k_spread_big<S>, dim3(gb), dim3(64 * kSpreadWaves), 0, s_big, lv, gd, g->S, act, na,
-                       g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
-                       (const double*)g->v_w, (const double*)g->force, (FT*)L->force);
+                       (const double*)g->v_w, (const double*)g->force, (FT*)g->force_alt);
 }
 
-// The two spreading kernels take disjoint tiles; s_big may be another
-// stream than s_small (the caller joins it back).
-int launch_spread(bl_ibm g, hipStream_t s_small, hipStream_t s_big)
+// Spreading writes the force field and flags of the next collision into
+// the boundary's second buffers, which then trade places with the
+// lattice's: the last collision's field stays readable (velocity staging)
+// while the next one is formed.
+int launch_spread(bl_ibm g, hipStream_t s)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
     const LatView lv = ibm_view(g);
-    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv, s_small, s_big);
-    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, s_small, s_big);
-    else launch_spread_s<__half>(g, lv, s_small, s_big);
+    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv, s);
+    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, s);
+    else launch_spread_s<__half>(g, lv, s);
     BL_CHECK_LAUNCH();
     if (L->flags_full) {  // the buffer leaving the lattice has every tile set
         g->alt_full = true;
         L->flags_full = false;
     }
     std::swap(L->tile_flags, g->flags_alt);  // this step's tiles carry the force now
+    std::swap(L->force, g->force_alt);
     g->alt ^= 1;
     g->bins_dirty = false;
     return BL_OK;
@@ -1373,8 +1471,8 @@ int launch_spread(bl_ibm g, hipStream_t s_small, hipStream_t s_big)
 int do_interpolate(bl_ibm g)
 {
     int rc;
-    if ((rc = do_bin(g))) return rc;
-    if ((rc = launch_stage(g, g->lat->stream))) return rc;
+    if ((rc = do_bin(g, g->lat->stream))) return rc;
+    if ((rc = launch_stage(g, g->lat->stream, false))) return rc;
     return launch_interp(g, false, g->lat->stream);
 }
 
@@ -1395,8 +1493,8 @@ int do_forces(bl_ibm g, hipStream_t s)
 int do_spread(bl_ibm g)
 {
     int rc;
-    if (!g->binned && (rc = do_bin(g))) return rc;
-    return launch_spread(g, g->lat->stream, g->lat->stream);
+    if (!g->binned && (rc = do_bin(g, g->lat->stream))) return rc;
+    return launch_spread(g, g->lat->stream);
 }
 
 int do_move(bl_ibm g)
@@ -1453,13 +1551,13 @@ int step_serial(bl_ibm g)
     int rc;
     hipStream_t s = g->lat->stream;
     mark(g, 0);
-    if ((rc = do_bin(g))) return rc;
+    if ((rc = do_bin(g, s))) return rc;
     mark(g, 1);
     if ((rc = do_forces(g, s))) return rc;
     mark(g, 2);
-    if ((rc = launch_stage(g, s))) return rc;
-    if ((rc = launch_spread(g, s, s))) return rc;
-    if ((rc = launch_interp(g, true, s))) return rc;
+    if ((rc = launch_stage(g, s, false))) return rc;
+    if ((rc = launch_spread(g, s))) return rc;
+    if ((rc = launch_interp(g, true, s, true))) return rc;
     mark(g, 3);
     g->binned = false;
     if ((rc = update_particles(g, 1, s))) return rc;
@@ -1469,17 +1567,18 @@ int step_serial(bl_ibm g)
     return BL_OK;
 }
 
-// ... or forked over the lattice stream M and two more, A and B, so that
-// independent stages overlap:
+// ... or forked over the lattice stream M and the boundary's streams A and
+// B, so that independent stages overlap:
 //   A: membrane forces (read the positions before the move)
-//   M: binning; once scanned -> B: velocity staging
-//   M: list fill; after forces and staging: small-tile spreading,
-//      A: big-tile spreading
-//   B: (after forces) interpolation + advection, geometry, voxelization
+//   M: binning, then velocity staging (the last collision's force field
+//      and the new tile flags)
+//   A: after the binning: spreading into the second force buffer (it needs
+//      the lists and the forces, not the staging)
+//   B: after the staging and the forces: interpolation + advection (the
+//      launch also readies the next binning's list: the flags and list two
+//      steps back are free once staged), geometry, voxelization
 //   M: joins A and B (the collision reads the force field and the phase;
 //      the next binning reads the moved positions).
-// Staging reads the force field before spreading rewrites it, and the
-// interpolation reads the base / weights the binning wrote.
 int step_forked(bl_ibm g)
 {
     int rc;
@@ -1489,35 +1588,43 @@ int step_forked(bl_ibm g)
     BL_HIP(hipStreamWaitEvent(A, e[0], 0));
     if ((rc = do_forces(g, A))) return rc;
     BL_HIP(hipEventRecord(e[1], A));
-    if ((rc = do_bin(g, e[2]))) return rc;  // the staging reads the counts
-    BL_HIP(hipStreamWaitEvent(B, e[2], 0));
-    if ((rc = launch_stage(g, B))) return rc;
-    BL_HIP(hipEventRecord(e[3], B));
-    BL_HIP(hipStreamWaitEvent(M, e[1], 0));
-    BL_HIP(hipStreamWaitEvent(M, e[3], 0));
-    BL_HIP(hipEventRecord(e[4], M));
-    BL_HIP(hipStreamWaitEvent(A, e[4], 0));
-    if ((rc = launch_spread(g, M, A))) return rc;
-    BL_HIP(hipEventRecord(e[5], A));
+    if ((rc = do_bin(g, M, e[2]))) return rc;
+    if ((rc = launch_stage(g, M, false))) return rc;
+    BL_HIP(hipEventRecord(e[3], M));
+    BL_HIP(hipStreamWaitEvent(A, e[2], 0));
+    if ((rc = launch_spread(g, A))) return rc;
+    BL_HIP(hipEventRecord(e[4], A));
+    BL_HIP(hipStreamWaitEvent(B, e[3], 0));
     BL_HIP(hipStreamWaitEvent(B, e[1], 0));
-    if ((rc = launch_interp(g, true, B))) return rc;
+    if ((rc = launch_interp(g, true, B, true))) return rc;
     g->binned = false;
     if ((rc = update_particles(g, 1, B))) return rc;
     if ((rc = do_voxelize(g, B))) return rc;
-    BL_HIP(hipEventRecord(e[6], B));
+    BL_HIP(hipEventRecord(e[5], B));
+    BL_HIP(hipStreamWaitEvent(M, e[4], 0));
     BL_HIP(hipStreamWaitEvent(M, e[5], 0));
-    BL_HIP(hipStreamWaitEvent(M, e[6], 0));
     return BL_OK;
 }
 
-int step_body(bl_ibm g) { return g->timing ? step_serial(g) : step_forked(g); }
+// n coupled steps enqueued on the lattice stream (forked IBM stages)
+int steps_forked(bl_ibm g, int n)
+{
+    int rc;
+    for (int k = 0; k < n; ++k) {
+        if ((rc = step_forked(g))) return rc;
+        if ((rc = collide_stream_launch(g->lat))) return rc;
+        if ((rc = velocity_walls_launch(g->lat))) return rc;
+    }
+    return BL_OK;
+}
 
 }  // namespace
 
 int ibm_step_launch(bl_ibm g, bool forked)
 {
     int rc;
-    if ((rc = forked ? step_body(g) : step_serial(g))) return rc;
+    if (forked) return steps_forked(g, 1);
+    if ((rc = step_serial(g))) return rc;
     if ((rc = collide_stream_launch(g->lat))) return rc;
     return velocity_walls_launch(g->lat);
 }
@@ -1573,7 +1680,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
-                    g->flags_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_part, g->t_part, g->r_part,
+                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_part, g->t_part, g->r_part,
                     g->l_part};
     for (void* p : ptrs) hipFree(p);
     for (auto& e : g->ev)
@@ -1585,7 +1692,8 @@ int bl_ibm_destroy(bl_ibm g)
             hipStreamSynchronize(st);
             hipStreamDestroy(st);
         }
-    if (g->graph2) hipGraphExecDestroy(g->graph2);
+    for (auto& ge : g->graph)
+        if (ge) hipGraphExecDestroy(ge);
     delete g;
     return BL_OK;
 }
@@ -1744,6 +1852,8 @@ int bl_ibm_init(bl_ibm g)
     BL_HIP(hipMemset(g->n_active, 0, 8 * sizeof(int)));
     if ((rc = alloc(g->flags_alt, L->ntiles))) return rc;
     BL_HIP(hipMemset(g->flags_alt, 0, L->ntiles));
+    BL_HIP(hipMalloc(&g->force_alt, 3 * L->stride * L->force_bytes));
+    BL_HIP(hipMemset(g->force_alt, 0, 3 * L->stride * L->force_bytes));
     {
         std::vector<int> chp, chi, choff{0};
         for (int p = 0; p < g->np; ++p) {
@@ -1812,7 +1922,7 @@ int bl_ibm_step(bl_ibm g)
     int rc = need_init(g);
     if (rc) return rc;
     g->timing = true;
-    rc = step_body(g);
+    rc = step_serial(g);
     g->timing = false;
     if (rc) return rc;
     BL_HIP(hipEventSynchronize(g->ev[5]));
@@ -1824,6 +1934,45 @@ int bl_ibm_step(bl_ibm g)
     return check_overflow(g);
 }
 
+namespace {
+constexpr int kLongRun = 32;  // steps per replay of the long captured run
+
+// Captures `n` (even) coupled steps into graph slot `slot` unless the slot
+// holds a capture made in the same step parity / buffer / list-reset state.
+int ensure_graph(bl_ibm g, int slot, int n)
+{
+    bl_lattice L = g->lat;
+    const long long par = (L->steps & 1) | ((long long)g->alt << 1) | ((long long)g->prepped << 2);
+    if (g->graph[slot] && g->graph_parity[slot] == par) return BL_OK;
+    if (g->graph[slot]) {
+        hipGraphExecDestroy(g->graph[slot]);
+        g->graph[slot] = nullptr;
+    }
+    hipGraph_t graph;
+    BL_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+    const long long s0 = L->steps;
+    int rc = steps_forked(g, n);
+    hipError_t e = hipStreamEndCapture(L->stream, &graph);
+    L->steps = s0;  // capture did not execute anything
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g->graph[slot], graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    g->graph_parity[slot] = par;
+    return BL_OK;
+}
+
+int replay(bl_ibm g, int slot, int n, int reps)
+{
+    for (int r = 0; r < reps; ++r) {
+        BL_HIP(hipGraphLaunch(g->graph[slot], g->lat->stream));
+        g->lat->steps += n;
+    }
+    return BL_OK;
+}
+}  // namespace
+
 int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph)
 {
     int rc = need_init(g);
@@ -1833,47 +1982,25 @@ int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph)
     int done = 0;
     // a force field loaded into the lattice is consumed over the next two
     // steps; those run eagerly, so no captured step depends on it
-    for (; done < n_steps && ibm_needs_settle(g); ++done) {
-        if ((rc = step_body(g))) return rc;
-        if ((rc = collide_stream_launch(L))) return rc;
-        if ((rc = velocity_walls_launch(L))) return rc;
-    }
-    if (use_graph && n_steps - done >= 2) {
-        // Two consecutive steps (the two AA parities) captured once and
-        // replayed; the captured pair depends on the lattice's step parity.
-        const long long par = (L->steps & 1) | ((long long)g->alt << 1);
-        if (!g->graph2 || g->graph_parity != par) {
-            if (g->graph2) {
-                hipGraphExecDestroy(g->graph2);
-                g->graph2 = nullptr;
-            }
-            hipGraph_t graph;
-            BL_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
-            const long long s0 = L->steps;
-            for (int k = 0; k < 2 && rc == BL_OK; ++k) {
-                rc = step_body(g);
-                if (!rc) rc = collide_stream_launch(L);
-                if (!rc) rc = velocity_walls_launch(L);
-            }
-            hipError_t e = hipStreamEndCapture(L->stream, &graph);
-            L->steps = s0;  // capture did not execute anything
-            if (rc) return rc;
-            if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&g->graph2, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-            g->graph_parity = par;
+    for (; done < n_steps && ibm_needs_settle(g); ++done)
+        if ((rc = steps_forked(g, 1))) return rc;
+    if (use_graph) {
+        // Runs of kLongRun steps, then pairs (the two AA parities), each
+        // captured once per step parity and replayed.
+        const int left = n_steps - done;
+        if (left >= kLongRun) {
+            if ((rc = ensure_graph(g, 1, kLongRun))) return rc;
+            if ((rc = replay(g, 1, kLongRun, left / kLongRun))) return rc;
+            done += left / kLongRun * kLongRun;
         }
-        for (; done + 2 <= n_steps; done += 2) {
-            BL_HIP(hipGraphLaunch(g->graph2, L->stream));
-            L->steps += 2;
+        if (n_steps - done >= 2) {
+            if ((rc = ensure_graph(g, 0, 2))) return rc;
+            const int reps = (n_steps - done) / 2;
+            if ((rc = replay(g, 0, 2, reps))) return rc;
+            done += 2 * reps;
         }
     }
-    for (; done < n_steps; ++done) {
-        if ((rc = step_body(g))) return rc;
-        if ((rc = collide_stream_launch(L))) return rc;
-        if ((rc = velocity_walls_launch(L))) return rc;
-    }
+    if (done < n_steps && (rc = steps_forked(g, n_steps - done))) return rc;
     return check_overflow(g);
 }
 
@@ -1994,11 +2121,11 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2])
     int rc = need_init(g);
     if (rc) return rc;
     if (!out) return fail(BL_ERR_ARG, "bl_ibm_tile_stats: null argument");
-    int st[4] = {0, 0, 0, 0};
+    int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     BL_HIP(hipMemcpyAsync(st, g->n_active, sizeof(st), hipMemcpyDeviceToHost, g->lat->stream));
     BL_HIP(hipStreamSynchronize(g->lat->stream));
     out[0] = st[g->last_list];
-    out[1] = st[3];
+    out[1] = st[5 + g->last_list];
     return BL_OK;
 }
 

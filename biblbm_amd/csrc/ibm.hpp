@@ -75,6 +75,7 @@ struct bl_ibm_s {
     void* ut = nullptr;    // staged velocity, (u, 0) per cell (FP64 for FP64 storage, else FP32)
     bool binned = false;   // tile lists match the current positions
     bool bins_dirty = false;  // tile counters hold a binning no spreading has consumed
+    bool prepped = false;     // the next binning's list reset is already enqueued
 
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
@@ -82,12 +83,13 @@ struct bl_ibm_s {
     int* tile_count = nullptr;  // entries per tile (cleared by the spreading that consumes them)
     int* tile_list = nullptr;   // per tile a fixed-capacity list of the vertices whose stencil touches it
     int* active = nullptr;      // two lists of touched tiles, one per force-flag buffer (ntiles each)
-    int* n_active = nullptr;    // [0], [1] lengths of the two lists, [2] overflow flags, [3] pairs of the last
-                                // binning, [4] arrival counter of the list reset
+    int* n_active = nullptr;    // [0], [1] lengths of the two lists, [2] overflow flags, [4] arrival counter
+                                // of the list reset, [5], [6] (tile, vertex) pairs of the two lists
     int alt = 0;                // list and flag buffer the next binning writes (flags_alt)
     int last_list = 0;          // list of the last binning
     bool alt_full = false;      // flags_alt has every tile set: the next binning clears all of it
     unsigned char* flags_alt = nullptr;  // next step's tile flags (swapped with the lattice's)
+    void* force_alt = nullptr;           // next step's force field (swapped with the lattice's)
     int* ch_part = nullptr;              // per geometry chunk: particle
     int* ch_idx = nullptr;               //                     chunk index within the particle
     int* p_choff = nullptr;              // np+1 chunk offsets
@@ -108,10 +110,12 @@ struct bl_ibm_s {
     bool timing = false;
 
     hipStream_t aux[2] = {nullptr, nullptr};  // forked stages of a step (default priority)
-    hipEvent_t fev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
-    hipGraphExec_t graph2 = nullptr;  // captured pair of coupled steps
-    long long graph_parity = -1;
+    // captured runs of coupled steps: [0] two steps, [1] kLongRun steps,
+    // each with the step parity / buffer state it was captured in
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    long long graph_parity[2] = {-1, -1};
 };
 
 namespace bl {

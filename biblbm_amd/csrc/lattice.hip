@@ -62,7 +62,23 @@ LatView view_of(const bl_lattice_s* L)
 
 // ------------------------------------------------------ collide & stream
 // FMODE: 0 no force, 1 uniform body force, 2 body force + tile-gated field
-template <class S, bool ODD, bool MASK, int FMODE, bool PHASE>
+// ODD steps address the neighbours' slots: element offsets from the start
+// of the population buffer, 32-bit (`unsigned`) unless the buffer holds
+// 2^32 values or more (`size_t`).  A slot past a non-periodic face is
+// replaced by the cell's own slot of the opposite population (reflection),
+// chosen per lane without branching.  Waves whose lanes all sit one cell or
+// more inside the x and y faces (nearly all of them) take the x/y part of
+// every neighbour offset as a wave-uniform value; only z is per lane.
+template <class K>
+__device__ __forceinline__ K odd_slot(const NbrOff& nb, bool inner, int sx, int sy, unsigned c, long st,
+                                      int dx, int dy, int dz, int p_valid, int p_face)
+{
+    bool valid;
+    const int d = inner ? nb.off_inner(dx, dy, dz, sx, sy, valid) : nb.off(dx, dy, dz, valid);
+    return valid ? (K)p_valid * (K)st + (K)(c + d) : (K)p_face * (K)st + (K)c;
+}
+
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K>
 __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 {
     using P = Pop<S>;
@@ -77,15 +93,24 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
     S* __restrict__ A = (S*)v.pops;
     const long st = v.stride;
 
-    Nbr n;
-    if (ODD) n.init(v, x, y, z);
+    NbrOff nb;
+    bool inner = false;
+    const int sx = v.ny * v.nz, sy = v.nz;
+    if (ODD) {
+        nb.init(v, x, y, z);
+        inner = __all(x > 0 && x < v.nx - 1 && y > 0 && y < v.ny - 1) != 0;
+    }
 
     C f[Q];
     if (ODD) {
+        if (inner) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const int s = n.at(v, -cx(i), -cy(i), -cz(i));
-            f[i] = s >= 0 ? P::ld(A + opp(i) * st, (unsigned)s, i) : P::ld(A + i * st, c, i);
+            for (int i = 0; i < Q; ++i)
+                f[i] = P::ld(A, odd_slot<K>(nb, true, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i), i);
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                f[i] = P::ld(A, odd_slot<K>(nb, false, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i), i);
         }
     } else {
 #pragma unroll
@@ -109,11 +134,14 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
     }
 
     if (ODD) {
+        if (inner) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const int d = n.at(v, cx(i), cy(i), cz(i));
-            if (d >= 0) P::st(A + i * st, (unsigned)d, i, f[i]);
-            else P::st(A + opp(i) * st, c, i, f[i]);
+            for (int i = 0; i < Q; ++i)
+                P::st(A, odd_slot<K>(nb, true, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i)), i, f[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                P::st(A, odd_slot<K>(nb, false, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i)), i, f[i]);
         }
     } else {
 #pragma unroll
@@ -124,10 +152,18 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 template <class S, bool ODD, bool MASK, int FMODE>
 void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
 {
-    if (phase)
-        hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true>), dim3(nblk), dim3(blk), 0, s, v);
-    else
-        hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, false>), dim3(nblk), dim3(blk), 0, s, v);
+    const bool big = ODD && (size_t)Q * (size_t)v.stride > 0xffffffffull;
+    if (big) {
+        if (phase)
+            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true, size_t>), dim3(nblk), dim3(blk), 0, s, v);
+        else
+            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, false, size_t>), dim3(nblk), dim3(blk), 0, s, v);
+    } else {
+        if (phase)
+            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true, unsigned>), dim3(nblk), dim3(blk), 0, s, v);
+        else
+            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, false, unsigned>), dim3(nblk), dim3(blk), 0, s, v);
+    }
 }
 
 template <class S, bool ODD, bool MASK>

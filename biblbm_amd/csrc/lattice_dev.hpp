@@ -14,25 +14,31 @@ template <>
 struct Pop<double> {
     using C = double;
     using F = double;
-    __device__ static C ld(const double* A, unsigned k, int) { return A[k]; }
-    __device__ static void st(double* A, unsigned k, int, C v) { A[k] = v; }
+    template <class K>
+    __device__ static C ld(const double* A, K k, int) { return A[k]; }
+    template <class K>
+    __device__ static void st(double* A, K k, int, C v) { A[k] = v; }
 };
 template <>
 struct Pop<float> {
     using C = float;
     using F = float;
-    __device__ static C ld(const float* A, unsigned k, int) { return A[k]; }
-    __device__ static void st(float* A, unsigned k, int, C v) { A[k] = v; }
+    template <class K>
+    __device__ static C ld(const float* A, K k, int) { return A[k]; }
+    template <class K>
+    __device__ static void st(float* A, K k, int, C v) { A[k] = v; }
 };
 template <>
 struct Pop<__half> {
     using C = float;
     using F = float;
-    __device__ static C ld(const __half* A, unsigned k, int i)
+    template <class K>
+    __device__ static C ld(const __half* A, K k, int i)
     {
         return __half2float(A[k]) * (float)(tw(i) / kF16Scale);
     }
-    __device__ static void st(__half* A, unsigned k, int i, C v)
+    template <class K>
+    __device__ static void st(__half* A, K k, int i, C v)
     {
         A[k] = __float2half_rn(v * (float)(kF16Scale / tw(i)));
     }
@@ -69,6 +75,42 @@ struct Nbr {
         const int a = X[dx + 1], b = Y[dy + 1], c = Z[dz + 1];
         if ((a | b | c) < 0) return -1;
         return (a * v.ny + b) * v.nz + c;
+    }
+};
+
+// Neighbour addressing of the AA odd step without branches: per axis the
+// element offset of the -1 / +1 neighbour (wrapped across a periodic face)
+// and whether that neighbour exists (not past a non-periodic face).
+struct NbrOff {
+    int o[3][3];     // [axis][d + 1]
+    unsigned ok[3];  // bit d + 1: the neighbour at d exists
+    __device__ __forceinline__ void axis(int a, int p, int n, int per, int s)
+    {
+        const bool lo = p > 0, hi = p < n - 1;
+        o[a][0] = lo ? -s : (n - 1) * s;
+        o[a][1] = 0;
+        o[a][2] = hi ? s : -(n - 1) * s;
+        ok[a] = 2u | ((lo || per) ? 1u : 0u) | ((hi || per) ? 4u : 0u);
+    }
+    __device__ __forceinline__ void init(const LatView& v, int x, int y, int z)
+    {
+        axis(0, x, v.nx, v.per[0], v.ny * v.nz);
+        axis(1, y, v.ny, v.per[1], v.nz);
+        axis(2, z, v.nz, v.per[2], 1);
+    }
+    // element offset of the cell x + (dx, dy, dz); `valid` false past a face
+    __device__ __forceinline__ int off(int dx, int dy, int dz, bool& valid) const
+    {
+        valid = ((ok[0] >> (dx + 1)) & (ok[1] >> (dy + 1)) & (ok[2] >> (dz + 1)) & 1u) != 0;
+        return o[0][dx + 1] + o[1][dy + 1] + o[2][dz + 1];
+    }
+    // the same where x and y are at least one cell away from their faces in
+    // every lane of the wave: their part of the offset, dx*sx + dy*sy, is
+    // then the same in every lane (sx, sy wave-uniform)
+    __device__ __forceinline__ int off_inner(int dx, int dy, int dz, int sx, int sy, bool& valid) const
+    {
+        valid = ((ok[2] >> (dz + 1)) & 1u) != 0;
+        return dx * sx + dy * sy + o[2][dz + 1];
     }
 };
 
