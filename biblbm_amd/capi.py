@@ -74,6 +74,15 @@ SIGNATURES = {
     "bl_ensemble_steps": [_P, _I, _I],
     "bl_ensemble_time_steps": [_P, _I, _I, _PD],
     "bl_tune_collide": [_P, ctypes.c_char_p, _I],
+    "bl_guo_wall_create": [_P, _P, ctypes.POINTER(_P)],
+    "bl_guo_wall_destroy": [_P],
+    "bl_guo_wall_count": [_P, _PLL],
+    "bl_guo_wall_nodes": [_P, _P, _P, _P, _P],
+    "bl_guo_wall_set_velocity": [_P, _P],
+    "bl_guo_wall_set_node_velocities": [_P, _P],
+    "bl_guo_wall_insert": [_P],
+    "bl_boundary_outside_mask": [_P, _I, _I, _I, _D, _P, ctypes.c_ubyte],
+    "bl_ibm_set_boundary": [_P, _P],
 }
 
 

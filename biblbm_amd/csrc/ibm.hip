@@ -27,6 +27,7 @@
 
 #include "ibm.hpp"
 #include "lattice_dev.hpp"
+#include "walls.hpp"
 
 namespace bl {
 
@@ -218,6 +219,26 @@ __device__ __forceinline__ void list_reset(const ListReset& lr)
 
 __global__ void __launch_bounds__(256) k_bin_prep(ListReset lr) { list_reset(lr); }
 
+// Near-wall vertices (a boundary is set and the vertex is nearer to it than
+// the kernel's half support + 1, src/fsi/ImmersedBoundaryDynamics.hh:
+// 346-357) spread after every other vertex, the kernel part (force scaled by
+// the blend weight w) before the top-hat part (force scaled by 1 - w), as
+// src/fsi/ImmersedBoundaryDynamics_forces.hh:193-237,259-277 orders them.
+// List entries carry that order in their value: v for a bulk vertex,
+// kNearKey | 2v for the kernel part and kNearKey | 2v + 1 for the top-hat
+// part of a near-wall vertex, so sorting a tile's entries gives the
+// reference's summation order.
+constexpr int kNearKey = 1 << 30;
+
+struct NearView {
+    bl_boundary b;
+    int on = 0;                      // a boundary is set
+    unsigned char* near = nullptr;   // per vertex: near-wall
+    double* blend = nullptr;         // per vertex: distance - half support
+    int* base_t = nullptr;           // per vertex: top-hat stencil base (3)
+    double* W_t = nullptr;           // per vertex: top-hat weights (2 per axis)
+};
+
 // The first pair that reaches a tile (outside a workgroup's table) lists it
 // as touched and flags it as carrying spread force.
 __device__ __forceinline__ void touch_tile(int t, int* act, int* na, unsigned char* flags)
@@ -226,18 +247,73 @@ __device__ __forceinline__ void touch_tile(int t, int* act, int* na, unsigned ch
     flags[t] = 1;
 }
 
+__device__ __forceinline__ void count_tiles(const StencilTiles& st, const Grid& gd, int* keys, int* cnt)
+{
+#pragma unroll 1
+    for (int i = 0; i < st.ax[0].m; ++i)
+#pragma unroll 1
+        for (int j = 0; j < st.ax[1].m; ++j)
+#pragma unroll 1
+            for (int k = 0; k < st.ax[2].m; ++k) {
+                const int sl = hash_insert(keys, st.at(gd, i, j, k));
+                if (sl >= 0) atomicAdd(&cnt[sl], 1);
+            }
+}
+
+__device__ __forceinline__ void fill_tiles(const StencilTiles& st, const Grid& gd, int entry, const int* keys,
+                                           const int* cnt, int* rank, int* count, int* list, int cap, int* act,
+                                           int* na, unsigned char* flags, int* status, int* pairs)
+{
+#pragma unroll 1
+    for (int i = 0; i < st.ax[0].m; ++i)
+#pragma unroll 1
+        for (int j = 0; j < st.ax[1].m; ++j)
+#pragma unroll 1
+            for (int k = 0; k < st.ax[2].m; ++k) {
+                const int tt = st.at(gd, i, j, k);
+                const int sl = hash_find(keys, tt);
+                int slot;
+                if (sl >= 0) {
+                    slot = cnt[sl] + atomicAdd(&rank[sl], 1);
+                } else {  // the workgroup's table was full for this tile
+                    slot = atomicAdd(&count[tt], 1);
+                    if (slot == 0) touch_tile(tt, act, na, flags);
+                    atomicAdd(pairs, 1);
+                    if (slot >= cap) atomicOr(&status[2], 1);
+                }
+                if (slot < cap) list[(long)tt * cap + slot] = entry;
+            }
+}
+
+// Support-box base (wrapped on periodic axes) and the S weights of one axis.
+__device__ __forceinline__ int axis_stencil(int kernel, int S, double h, double x, int n, int per, double* w)
+{
+    const int raw = (int)ceil(x - h);
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k)
+        if (k < S) w[k] = dirac(kernel, (double)(raw + k) - x);
+    int bw = raw;
+    if (per) {
+        bw = raw % n;
+        if (bw < 0) bw += n;
+    }
+    return bw;
+}
+
 // Binning in one pass: stencil base and 1-D weights per vertex, the tiles
 // its stencil touches, and every (tile, vertex) pair written into the
-// tile's fixed-capacity list (entry = vertex index).  Per workgroup the
-// pairs are counted per distinct tile in LDS, one global atomic per tile
-// reserves the workgroup's run, and the slots inside the run come from LDS.
-// The order within a list is whatever the atomics give; spreading sorts.
+// tile's fixed-capacity list (entry values as described at kNearKey; a
+// near-wall vertex also gets its blend weight, top-hat base and weights and
+// the tiles of its top-hat stencil).  Per workgroup the pairs are counted
+// per distinct tile in LDS, one global atomic per tile reserves the
+// workgroup's run, and the slots inside the run come from LDS.  The order
+// within a list is whatever the atomics give; spreading sorts.
 __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double h, long NV,
                                              const double* __restrict__ pos, int* __restrict__ base,
                                              double* __restrict__ W, int* __restrict__ count,
                                              int* __restrict__ list, int cap, int* __restrict__ act,
                                              int* __restrict__ na, unsigned char* __restrict__ flags,
-                                             int* __restrict__ status, int* __restrict__ pairs)
+                                             int* __restrict__ status, int* __restrict__ pairs, NearView nw)
 {
     __shared__ int keys[kHash];
     __shared__ int cnt[kHash];  // pairs per slot, then the slot's first index in the tile's list
@@ -253,34 +329,44 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
     }
     __syncthreads();
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    StencilTiles st;
+    StencilTiles st, stt;
+    int e0 = -1, e1 = -1;  // list entries of the kernel part and the top-hat part (-1: none)
     if (v < NV) {
         int b[3];
         for (int a = 0; a < 3; ++a) {
-            const double x = pos[3 * v + a];
-            const int raw = (int)ceil(x - h);
-#pragma unroll 1
-            for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = k < S ? dirac(kernel, (double)(raw + k) - x) : 0.0;
-            int bw = raw;
-            if (gd.per[a]) {
-                bw = raw % gd.n[a];
-                if (bw < 0) bw += gd.n[a];
-            }
-            b[a] = bw;
-            base[3 * v + a] = bw;
+            double w[4] = {0.0, 0.0, 0.0, 0.0};
+            b[a] = axis_stencil(kernel, S, h, pos[3 * v + a], gd.n[a], gd.per[a], w);
+            base[3 * v + a] = b[a];
+            for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = w[k];
         }
         st.init(gd, S, b);
+        e0 = (int)v;
+        if (nw.on) {
+            const double p[3] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]};
+            const bool nr = bnd_less_than(nw.b, p, h + 1.);
+            const double wb = bnd_distance(nw.b, p) - h;
+            nw.near[v] = nr;
+            nw.blend[v] = wb;
+            if (nr) {
+                e0 = wb > 0 ? (kNearKey | (int)(2 * v)) : -1;
+                if (wb < 1) {
+                    int bt[3];
+                    for (int a = 0; a < 3; ++a) {
+                        double w[4];
+                        bt[a] = axis_stencil(BL_KERNEL_TOPHAT, 2, 1.0, p[a], gd.n[a], gd.per[a], w);
+                        nw.base_t[3 * v + a] = bt[a];
+                        nw.W_t[6 * v + 2 * a] = w[0];
+                        nw.W_t[6 * v + 2 * a + 1] = w[1];
+                    }
+                    stt.init(gd, 2, bt);
+                    e1 = kNearKey | (int)(2 * v + 1);
+                }
+            }
+        }
         // runtime loops: the kernel runs once per launch on few workgroups,
         // so compact code (instruction fetch) beats unrolling
-#pragma unroll 1
-        for (int i = 0; i < st.ax[0].m; ++i)
-#pragma unroll 1
-            for (int j = 0; j < st.ax[1].m; ++j)
-#pragma unroll 1
-                for (int k = 0; k < st.ax[2].m; ++k) {
-                    const int sl = hash_insert(keys, st.at(gd, i, j, k));
-                    if (sl >= 0) atomicAdd(&cnt[sl], 1);
-                }
+        if (e0 >= 0) count_tiles(st, gd, keys, cnt);
+        if (e1 >= 0) count_tiles(stt, gd, keys, cnt);
     }
     __syncthreads();
     // one global atomic per distinct tile; the occupied slots are compacted
@@ -308,27 +394,8 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
         act[fresh_base + i] = fresh[i];
         flags[fresh[i]] = 1;
     }
-    if (v < NV) {
-#pragma unroll 1
-        for (int i = 0; i < st.ax[0].m; ++i)
-#pragma unroll 1
-            for (int j = 0; j < st.ax[1].m; ++j)
-#pragma unroll 1
-                for (int k = 0; k < st.ax[2].m; ++k) {
-                    const int tt = st.at(gd, i, j, k);
-                    const int sl = hash_find(keys, tt);
-                    int slot;
-                    if (sl >= 0) {
-                        slot = cnt[sl] + atomicAdd(&rank[sl], 1);
-                    } else {  // the workgroup's table was full for this tile
-                        slot = atomicAdd(&count[tt], 1);
-                        if (slot == 0) touch_tile(tt, act, na, flags);
-                        atomicAdd(pairs, 1);
-                        if (slot >= cap) atomicOr(&status[2], 1);
-                    }
-                    if (slot < cap) list[(long)tt * cap + slot] = (int)v;
-                }
-    }
+    if (e0 >= 0) fill_tiles(st, gd, e0, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs);
+    if (e1 >= 0) fill_tiles(stt, gd, e1, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs);
 }
 
 __global__ void k_clear_bins(int* count, long n)
@@ -678,10 +745,36 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const unsigned char
 // all in FP64.  The lanes of one x-step read consecutive cells of a few
 // z-rows.  With MOVE the vertex is then advanced, x += U (explicit Euler,
 // src/fsi/DeformableParticle3D.hh:26-36).
+// Explicit Euler step that reflects off the boundary
+// (DeformableParticle3D::move_vertices, src/fsi/DeformableParticle3D.hh:26-50).
+__device__ __forceinline__ void move_reflect(const bl_boundary& b, double* p, const double* u)
+{
+#pragma clang fp contract(off)
+    const double old[3] = {p[0], p[1], p[2]};
+    const double nw[3] = {old[0] + u[0], old[1] + u[1], old[2] + u[2]};
+    double t;
+    if (!bnd_trace_ray(b, old, nw, t)) {
+        p[0] = nw[0];
+        p[1] = nw[1];
+        p[2] = nw[2];
+        return;
+    }
+    double q[3] = {old[0] + t * u[0], old[1] + t * u[1], old[2] + t * u[2]};
+    double n[3];
+    bnd_normal(b, q, n);
+    const double vn = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    for (int a = 0; a < 3; ++a) p[a] = q[a] + (u[a] - 2. * vn * n[a]) * (1. - t);
+}
+
+// A near-wall vertex (blend weight w < 1) also interpolates with the top
+// hat, lanes (j, k) < 2 over its 2x2x2 stencil, and takes (1 - w) u_tophat
+// + w u_kernel, or u_tophat alone for w <= 0
+// (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:182-230).
 template <class C, int SW, bool MOVE>
 __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __restrict__ base,
                                                 const double* __restrict__ W, const Vel4<C>* __restrict__ ut,
-                                                double* __restrict__ vel, double* __restrict__ pos, ListReset lr)
+                                                double* __restrict__ vel, double* __restrict__ pos, ListReset lr,
+                                                NearView nw)
 {
     if (lr.act) list_reset(lr);  // the next binning's list, carried by this launch
     constexpr int LA = SW == 2 ? 2 : 4;
@@ -716,14 +809,56 @@ __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __r
         a1 += __shfl_xor(a1, o);
         a2 += __shfl_xor(a2, o);
     }
+    if (nw.on && v < NV && nw.near[v] && nw.blend[v] < 1) {  // the same for all lanes of the vertex
+        const double wb = nw.blend[v];
+        double t0 = 0, t1 = 0, t2 = 0;
+        if (j < 2 && k < 2) {
+            const int* bt = nw.base_t + 3 * v;
+            const double* Wt = nw.W_t + 6 * v;
+            const int yj = node_at(bt[1], j, gd.n[1], gd.per[1]);
+            const int zk = node_at(bt[2], k, gd.n[2], gd.per[2]);
+            if (yj >= 0 && zk >= 0) {
+                const double wy = Wt[2 + j], wz = Wt[4 + k];
+                for (int i = 0; i < 2; ++i) {
+                    const int xi = node_at(bt[0], i, gd.n[0], gd.per[0]);
+                    if (xi < 0) continue;
+                    const double w = (Wt[i] * wy) * wz;
+                    const Vel4<C> u = ut[((size_t)xi * gd.n[1] + yj) * gd.n[2] + zk];
+                    t0 += w * (double)u.x;
+                    t1 += w * (double)u.y;
+                    t2 += w * (double)u.z;
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < LPV; o <<= 1) {
+            t0 += __shfl_xor(t0, o);
+            t1 += __shfl_xor(t1, o);
+            t2 += __shfl_xor(t2, o);
+        }
+        if (wb <= 0) {
+            a0 = t0;
+            a1 = t1;
+            a2 = t2;
+        } else {
+            a0 = (1. - wb) * t0 + wb * a0;
+            a1 = (1. - wb) * t1 + wb * a1;
+            a2 = (1. - wb) * t2 + wb * a2;
+        }
+    }
     if (r == 0 && v < NV) {
         vel[3 * v] = a0;
         vel[3 * v + 1] = a1;
         vel[3 * v + 2] = a2;
         if (MOVE) {
-            pos[3 * v] += a0;
-            pos[3 * v + 1] += a1;
-            pos[3 * v + 2] += a2;
+            if (nw.on) {
+                const double u[3] = {a0, a1, a2};
+                move_reflect(nw.b, pos + 3 * v, u);
+            } else {
+                pos[3 * v] += a0;
+                pos[3 * v + 1] += a1;
+                pos[3 * v + 2] += a2;
+            }
         }
     }
 }
@@ -759,37 +894,70 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Lane `lane` (< kChunk) stages entry v (tile-local weights, force) into
-// ew[.][lane].
-// The vertex's 12 weights are loaded whole (not indexed by the stencil
-// offset) so that no load waits for the base.
-__device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int org[3], long v,
+// Lane `lane` (< kChunk) stages list entry `e` (tile-local weights, force)
+// into ew[.][lane].  The vertex's weights are loaded whole (not indexed by
+// the stencil offset) so that no load waits for the base.  A near-wall
+// vertex's kernel part carries the force times w (w < 1), its top-hat part
+// the force times 1 - w (w > 0), the top hat's two weights per axis.
+__device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int org[3], int e,
                                             const int* __restrict__ base, const double* __restrict__ W,
-                                            const double* __restrict__ vforce, double (*ew)[kChunk + 1], int lane)
+                                            const double* __restrict__ vforce, const NearView& nw,
+                                            double (*ew)[kChunk + 1], int lane)
 {
-    const double2* W2 = (const double2*)(W + 12 * v);
+    const bool nr = (e & kNearKey) != 0;
+    const long v = nr ? (long)((e & ~kNearKey) >> 1) : (long)e;
+    const bool top = nr && (e & 1);
     double w[12];
+    const int* bv;
+    int sw;
+    if (top) {
+        const double2* W2 = (const double2*)(nw.W_t + 6 * v);
+        for (int k = 0; k < 3; ++k) {
+            const double2 p = W2[k];
+            w[4 * k] = p.x;
+            w[4 * k + 1] = p.y;
+            w[4 * k + 2] = 0.0;
+            w[4 * k + 3] = 0.0;
+        }
+        bv = nw.base_t + 3 * v;
+        sw = 2;
+    } else {
+        const double2* W2 = (const double2*)(W + 12 * v);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const double2 p = W2[k];
-        w[2 * k] = p.x;
-        w[2 * k + 1] = p.y;
+        for (int k = 0; k < 6; ++k) {
+            const double2 p = W2[k];
+            w[2 * k] = p.x;
+            w[2 * k + 1] = p.y;
+        }
+        bv = base + 3 * v;
+        sw = Sw;
     }
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const int b = base[3 * v + ax];
+        const int b = bv[ax];
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
             int d = org[ax] + l - b;
             if (gd.per[ax] && d < 0) d += gd.n[ax];
             const double* wa = w + 4 * ax;
             const double x = d == 0 ? wa[0] : (d == 1 ? wa[1] : (d == 2 ? wa[2] : wa[3]));
-            ew[4 * ax + l][lane] = (unsigned)d < (unsigned)Sw ? x : 0.0;
+            ew[4 * ax + l][lane] = (unsigned)d < (unsigned)sw ? x : 0.0;
         }
     }
-    ew[12][lane] = vforce[3 * v];
-    ew[13][lane] = vforce[3 * v + 1];
-    ew[14][lane] = vforce[3 * v + 2];
+    double F0 = vforce[3 * v], F1 = vforce[3 * v + 1], F2 = vforce[3 * v + 2];
+    if (nr) {
+        const double wb = nw.blend[v];
+        const bool scale = top ? wb > 0 : wb < 1;
+        const double s = top ? 1. - wb : wb;
+        if (scale) {
+            F0 *= s;
+            F1 *= s;
+            F2 *= s;
+        }
+    }
+    ew[12][lane] = F0;
+    ew[13][lane] = F1;
+    ew[14][lane] = F2;
 }
 
 __device__ __forceinline__ void chunk_sum(const double (*ew)[kChunk + 1], int cnt, int lx, int ly, int lz,
@@ -830,7 +998,7 @@ __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[1
                                              const int* __restrict__ nactive, int* __restrict__ count, int cap,
                                              const int* __restrict__ list, const int* __restrict__ base,
                                              const double* __restrict__ W, const double* __restrict__ vforce,
-                                             typename Pop<S>::F* __restrict__ Fout)
+                                             const NearView& nw, typename Pop<S>::F* __restrict__ Fout)
 {
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
@@ -856,7 +1024,7 @@ __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[1
         double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
         for (int c0 = 0; c0 < n; c0 += kChunk) {
             const int kk = __shfl(key, (c0 + lane) & 63);
-            if (lane < kChunk && c0 + lane < n) stage_entry(gd, Sw, org, kk, base, W, vforce, ew, lane);
+            if (lane < kChunk && c0 + lane < n) stage_entry(gd, Sw, org, kk, base, W, vforce, nw, ew, lane);
             wave_sync();
             chunk_sum(ew, min(kChunk, n - c0), lx, ly, lz, acc0, acc1, acc2);
             wave_sync();
@@ -872,7 +1040,8 @@ __device__ __forceinline__ void spread_big(int blk, int nb, int* keys, double (*
                                            const int* __restrict__ active, const int* __restrict__ nactive,
                                            int* __restrict__ count, int cap, const int* __restrict__ list,
                                            const int* __restrict__ base, const double* __restrict__ W,
-                                           const double* __restrict__ vforce, typename Pop<S>::F* __restrict__ Fout)
+                                           const double* __restrict__ vforce, const NearView& nw,
+                                           typename Pop<S>::F* __restrict__ Fout)
 {
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
@@ -929,7 +1098,7 @@ __device__ __forceinline__ void spread_big(int blk, int nb, int* keys, double (*
         for (int c0 = lo; c0 < hi; c0 += kChunk) {
             const int e = c0 + lane;
             const int kk = small ? __shfl(key, e & 63) : keys[min(e, n - 1)];
-            if (lane < kChunk && e < hi) stage_entry(gd, Sw, org, kk, base, W, vforce, ew, lane);
+            if (lane < kChunk && e < hi) stage_entry(gd, Sw, org, kk, base, W, vforce, nw, ew, lane);
             wave_sync();
             chunk_sum(ew, min(kChunk, hi - c0), lx, ly, lz, acc0, acc1, acc2);
             wave_sync();
@@ -962,7 +1131,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid g
                                                               const int* __restrict__ list,
                                                               const int* __restrict__ base,
                                                               const double* __restrict__ W,
-                                                              const double* __restrict__ vforce,
+                                                              const double* __restrict__ vforce, NearView nw,
                                                               typename Pop<S>::F* __restrict__ Fout)
 {
     __shared__ int keys[kMaxList];
@@ -970,10 +1139,11 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid g
     __shared__ double part[kSpreadWaves][3][64];
     const int b = (int)blockIdx.x;
     if (b < n_small)
-        spread_small<S>(b, n_small, ew_all, lv, gd, Sw, active, nactive, count, cap, list, base, W, vforce, Fout);
+        spread_small<S>(b, n_small, ew_all, lv, gd, Sw, active, nactive, count, cap, list, base, W, vforce, nw,
+                        Fout);
     else
         spread_big<S>(b - n_small, (int)gridDim.x - n_small, keys, ew_all, part, lv, gd, Sw, active, nactive, count,
-                      cap, list, base, W, vforce, Fout);
+                      cap, list, base, W, vforce, nw, Fout);
 }
 
 // --------------------------------------------------------------- motion
@@ -1252,6 +1422,31 @@ ForceView force_view(bl_ibm g)
     return f;
 }
 
+NearView near_view(bl_ibm g)
+{
+    NearView nw;
+    nw.b = g->boundary;
+    nw.on = g->has_boundary && g->v_near ? 1 : 0;
+    nw.near = g->v_near;
+    nw.blend = g->v_blend;
+    nw.base_t = g->v_base_t;
+    nw.W_t = g->v_w_t;
+    return nw;
+}
+
+// Per-vertex near-wall buffers, once both a boundary and the vertices exist.
+int alloc_near(bl_ibm g)
+{
+    if (!g->has_boundary || !g->inited || g->v_near) return BL_OK;
+    const size_t nv = (size_t)std::max<long>(g->NV, 1);
+    hipError_t e = hipMalloc(&g->v_near, nv);
+    if (e == hipSuccess) e = hipMalloc(&g->v_blend, sizeof(double) * nv);
+    if (e == hipSuccess) e = hipMalloc(&g->v_base_t, sizeof(int) * 3 * nv);
+    if (e == hipSuccess) e = hipMalloc(&g->v_w_t, sizeof(double) * 6 * nv);
+    if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("near-wall buffers: ") + hipGetErrorString(e));
+    return BL_OK;
+}
+
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 int need_init(bl_ibm g)
@@ -1339,7 +1534,7 @@ int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
     if (g->NV)
         hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
                            (const double*)g->pos, g->v_base, g->v_w, g->tile_count, g->tile_list, kTileCap, act,
-                           g->n_active + g->alt, g->flags_alt, g->n_active, g->n_active + 5 + g->alt);
+                           g->n_active + g->alt, g->flags_alt, g->n_active, g->n_active + 5 + g->alt, near_view(g));
     if (done) BL_HIP(hipEventRecord(done, s));
     BL_CHECK_LAUNCH();
     g->binned = true;
@@ -1399,13 +1594,13 @@ void launch_interp_c(bl_ibm g, hipStream_t s, const ListReset& lr)
     const Vel4<C>* ut = (const Vel4<C>*)g->ut;
     if (g->S == 4)
         hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos, lr);
+                           (const double*)g->v_w, ut, g->vel, g->pos, lr, near_view(g));
     else if (g->S == 3)
         hipLaunchKernelGGL((k_interp<C, 3, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos, lr);
+                           (const double*)g->v_w, ut, g->vel, g->pos, lr, near_view(g));
     else
         hipLaunchKernelGGL((k_interp<C, 2, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos, lr);
+                           (const double*)g->v_w, ut, g->vel, g->pos, lr, near_view(g));
 }
 
 // vertex velocities from the staged u; with `move` the vertices advance.
@@ -1441,7 +1636,7 @@ void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s)
     const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, 2048));
     hipLaunchKernelGGL(k_spread<S>, dim3(gs + gb), dim3(64 * kSpreadWaves), 0, s, lv, gd, g->S, gs, act, na,
                        g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
-                       (const double*)g->v_w, (const double*)g->force, (FT*)g->force_alt);
+                       (const double*)g->v_w, (const double*)g->force, near_view(g), (FT*)g->force_alt);
 }
 
 // Spreading writes the force field and flags of the next collision into
@@ -1638,6 +1833,8 @@ void ibm_detach(bl_ibm g)
     g->lat = nullptr;
 }
 
+void ibm_invalidate_graphs(bl_ibm g) { g->graph_parity[0] = g->graph_parity[1] = -1; }
+
 bool ibm_needs_settle(bl_ibm g) { return g->lat && (g->lat->flags_full || g->alt_full); }
 
 }  // namespace bl
@@ -1680,7 +1877,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
-                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_part, g->t_part, g->r_part,
+                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
                     g->l_part};
     for (void* p : ptrs) hipFree(p);
     for (auto& e : g->ev)
@@ -1808,6 +2005,7 @@ int bl_ibm_init(bl_ibm g)
         pos.insert(pos.end(), p.pos.begin(), p.pos.end());
     }
     g->NV = voff.back();
+    if (g->NV >= (1L << 29)) return fail(BL_ERR_CAPACITY, "bl_ibm_init: at most 2^29 vertices");
     g->NT = toff.back();
     g->NL = loff.back();
     g->NP = poff.back();
@@ -1882,11 +2080,24 @@ int bl_ibm_init(bl_ibm g)
     for (auto& st : g->aux) BL_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     for (auto& e : g->fev) BL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     g->inited = true;
+    if ((rc = alloc_near(g))) return rc;
     // update() of every particle and the centre-of-mass mapping of add_particle
     rc = update_particles(g, 1);
     if (rc) return rc;
     BL_HIP(hipStreamSynchronize(L->stream));
     return BL_OK;
+}
+
+int bl_ibm_set_boundary(bl_ibm g, const bl_boundary* b)
+{
+    if (!g) return fail(BL_ERR_ARG, "null ibm handle");
+    if (b && b->kind != BL_BOUNDARY_PIPE && b->kind != BL_BOUNDARY_PLATES)
+        return fail(BL_ERR_ARG, "bl_ibm_set_boundary: unknown boundary kind");
+    if (g->lat) BL_HIP(hipStreamSynchronize(g->lat->stream));
+    g->has_boundary = b != nullptr;
+    if (b) g->boundary = *b;
+    ibm_invalidate_graphs(g);
+    return alloc_near(g);
 }
 
 int bl_ibm_interpolate(bl_ibm g)

@@ -77,6 +77,14 @@ struct bl_ibm_s {
     bool bins_dirty = false;  // tile counters hold a binning no spreading has consumed
     bool prepped = false;     // the next binning's list reset is already enqueued
 
+    // ---- near-wall blending and reflection (a boundary is set) ----
+    bool has_boundary = false;
+    bl_boundary boundary{};
+    unsigned char* v_near = nullptr;  // per vertex: nearer than half support + 1
+    double* v_blend = nullptr;        // per vertex: distance - half support
+    int* v_base_t = nullptr;          // per vertex: top-hat stencil base (3)
+    double* v_w_t = nullptr;          // per vertex: top-hat weights (2 per axis)
+
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
     double* v_w = nullptr;    // 12 per vertex (3 axes x up to 4 weights)
@@ -129,6 +137,7 @@ int ibm_check_overflow(bl_ibm g);
 // The lattice is going away: forget it (the boundary's buffers stay valid
 // until bl_ibm_destroy).
 void ibm_detach(bl_ibm g);
+void ibm_invalidate_graphs(bl_ibm g);  // captured steps no longer match the lattice's passes
 // A full force field was loaded into the lattice (every tile flagged):
 // steps must run eagerly until the flags are consistent again.
 bool ibm_needs_settle(bl_ibm g);

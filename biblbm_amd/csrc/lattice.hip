@@ -373,7 +373,7 @@ __global__ void k_vwall_store(LatView v, long nwall, bool swapped, const double*
 
 int velocity_walls_launch(bl_lattice_s* L)
 {
-    if (!L->has_vwall || !L->mask) return BL_OK;
+    if (!L->has_vwall || !L->mask) return guo_walls_launch(L);
     const LatView v = view_of(L);
     const bool swapped = (L->steps & 1) != 0;
     const unsigned nb = (unsigned)((L->nwall + 255) / 256);
@@ -388,7 +388,7 @@ int velocity_walls_launch(bl_lattice_s* L)
         hipLaunchKernelGGL(k_vwall_store<__half>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
     }
     BL_CHECK_LAUNCH();
-    return BL_OK;
+    return guo_walls_launch(L);
 }
 
 // ----------------------------------------------- transfers and moments
@@ -695,6 +695,7 @@ int bl_lattice_destroy(bl_lattice L)
     if (!L) return BL_OK;
     if (L->stream) hipStreamSynchronize(L->stream);
     if (L->ibm) bl::ibm_detach(L->ibm);  // the boundary keeps its own buffers; it cannot step any more
+    bl::guo_walls_detach(L);
     hipFree(L->pops);
     hipFree(L->mask);
     hipFree(L->own_phase);

@@ -2,6 +2,8 @@
 // in-place AA pattern, cell mask, tile-gated external force, phase.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 struct bl_lattice_s {
@@ -46,6 +48,7 @@ struct bl_lattice_s {
     double* d_wall = nullptr;   // velocity-wall staging
     long nwall = 0;
     bl_ibm ibm = nullptr;
+    std::vector<bl_guo_wall> guo;  // inserted Guo curved walls (not owned)
 };
 
 namespace bl {
@@ -82,6 +85,8 @@ struct LatView {
 LatView view_of(const bl_lattice_s* L);
 int ensure_force_field(bl_lattice_s* L);
 int collide_stream_launch(bl_lattice_s* L);
-int velocity_walls_launch(bl_lattice_s* L);
+int velocity_walls_launch(bl_lattice_s* L);  // post-stream passes: velocity walls, then Guo walls
+int guo_walls_launch(bl_lattice_s* L);
+void guo_walls_detach(bl_lattice_s* L);
 
 }  // namespace bl

@@ -54,6 +54,11 @@ class ImmersedBoundary:
     def init(self):
         call("bl_ibm_init", self.handle)
 
+    def set_boundary(self, boundary):
+        """set_boundary (src/fsi/ImmersedBoundaryDynamics.h:93): near-wall
+        kernel blending and vertex reflection; None removes it."""
+        call("bl_ibm_set_boundary", self.handle, None if boundary is None else ctypes.byref(boundary))
+
     # --- stages ------------------------------------------------------------
     def interpolate(self):
         call("bl_ibm_interpolate", self.handle)

@@ -56,6 +56,17 @@ extern "C" {
 typedef struct bl_lattice_s *bl_lattice;
 typedef struct bl_ibm_s *bl_ibm;
 typedef struct bl_ensemble_s *bl_ensemble;
+typedef struct bl_guo_wall_s *bl_guo_wall;
+
+/* Analytic boundaries (src/fsi/Boundary.h): PipeBoundary(y0, z0, radius,
+ * length) along x (:132-186), ParallelPlatesBoundary(y0, y1) normal to y
+ * (:305-365).  Lattice coordinates: cell (x, y, z) sits at (x, y, z). */
+#define BL_BOUNDARY_PIPE 0    /* p = {y0, z0, radius, length} */
+#define BL_BOUNDARY_PLATES 1  /* p = {y0, y1, -, -}           */
+typedef struct {
+    int kind;
+    double p[4];
+} bl_boundary;
 
 /* ------------------------------------------------------------ runtime */
 /* Replaces plbInit (src/core/plbInit.cpp:38) for the device side. */
@@ -179,6 +190,44 @@ int bl_ibm_stage_times(bl_ibm g, double out[5]);
 /* Tiles touched by the last binning and (tile, vertex) list entries
  * (diagnostics for the traffic model; no counterpart in the reference). */
 int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
+
+/* ---------------------------------------------------- curved walls */
+/* GuoRigidWallBoundary(boundary, lattice) (src/fsi/GuoBoundaryCondition.hh:
+ * 17-22, instantiator :62-135): the wall nodes are the cells outside the
+ * boundary with a lattice link that crosses it; each keeps the link most
+ * opposed to the wall normal (direction index 1..18 towards the fluid) and
+ * delta = 1 - t of its crossing.  Fails (BL_ERR_ARG) if a node's two fluid
+ * neighbours along that link leave a non-periodic axis or are wall nodes
+ * themselves. */
+int bl_guo_wall_create(bl_lattice lat, const bl_boundary *b, bl_guo_wall *out);
+int bl_guo_wall_destroy(bl_guo_wall w);
+int bl_guo_wall_count(bl_guo_wall w, long long *n);
+/* The node list in instantiation order (x, y, z loops); any pointer may be
+ * NULL.  cell: (x*ny + y)*nz + z; wall_pos: 3 per node. */
+int bl_guo_wall_nodes(bl_guo_wall w, long long *cell, int *dir, double *delta, double *wall_pos);
+/* set_wall_velocity(Array) / set_wall_velocity(function) (:44-59): one
+ * velocity for every node, or 3 per node (e.g. evaluated at wall_pos). */
+int bl_guo_wall_set_velocity(bl_guo_wall w, const double vel[3]);
+int bl_guo_wall_set_node_velocities(bl_guo_wall w, const double *vel);
+/* insert() (:24-28): the extrapolation (GuoRigidWallBoundaryFunctional,
+ * :138-199, second-order branch) runs after every collide-and-stream of the
+ * lattice, and once in bl_lattice_initialize, as an integrated processor
+ * does. */
+int bl_guo_wall_insert(bl_guo_wall w);
+/* defineDynamics(..., BoundaryOutside(boundary, margin), value)
+ * (src/fsi/Boundary.h:114-128): mask[c] = value for every cell with
+ * !contains(c, -margin).  The reference passes sqrt(2) to an integer
+ * margin, i.e. 1.  Host only. */
+int bl_boundary_outside_mask(const bl_boundary *b, int nx, int ny, int nz, double margin,
+                             unsigned char *mask, unsigned char value);
+/* ImmersedBoundaryDynamics3D::set_boundary (src/fsi/ImmersedBoundaryDynamics.h:93):
+ * vertices nearer the boundary than the kernel's half support + 1 blend the
+ * kernel with the top hat in interpolation and spreading
+ * (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:182-230,
+ * src/fsi/ImmersedBoundaryDynamics_forces.hh:193-237) and are spread after
+ * every other vertex; advected vertices reflect off the boundary
+ * (src/fsi/DeformableParticle3D.hh:26-50).  NULL: no boundary. */
+int bl_ibm_set_boundary(bl_ibm g, const bl_boundary *b);
 
 /* ----------------------------------------------------------- ensembles */
 /* No counterpart in the reference, which runs independent boxes as separate

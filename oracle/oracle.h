@@ -128,6 +128,42 @@ long or_ray_hits_z(long nv, long nt, const int *tri, const double *pos,
 void or_voxel_fill(int nx, int ny, int nz, const int periodic[3], long nh,
                    const double *hits, unsigned char *phase);
 
+/* ---- curved walls ------------------------------------------------- */
+#define OR_KERNEL_TOPHAT 2
+#define OR_BOUNDARY_PIPE 0   /* along x: p = {y0, z0, radius, length} */
+#define OR_BOUNDARY_PLATES 1 /* y0 < y < y1: p = {y0, y1} */
+typedef struct {
+    int kind;
+    double p[4];
+} or_boundary;
+
+int    or_boundary_contains(const or_boundary *b, const double p[3], double margin);
+int    or_boundary_less_than(const or_boundary *b, const double p[3], double dist);
+double or_boundary_distance(const or_boundary *b, const double p[3]);
+int    or_boundary_trace_ray(const or_boundary *b, const double x0[3], const double x1[3], double *t);
+void   or_boundary_normal(const or_boundary *b, const double p[3], double n[3]);
+
+/* Guo wall nodes of an nx*ny*nz lattice at integer positions: cells outside
+ * the boundary with a lattice link crossing it.  Returns the count (arrays
+ * filled up to `cap`): cell index, lattice direction (1..18) towards the
+ * fluid, delta = 1 - t, wall position. */
+long or_guo_wall_nodes(int nx, int ny, int nz, const or_boundary *b, long cap, long *cell, int *dir,
+                       double *delta, double *wall_pos);
+
+/* The Guo extrapolation over the wall nodes, in node order, on post-stream
+ * populations f (cell-major, fBar).  vel[3*k]: wall velocity of node k.
+ * mask (may be NULL: all fluid) gives each wall node's dynamics. */
+void or_guo_wall_apply(int nx, int ny, int nz, const int periodic[3], double *f, const unsigned char *mask,
+                       long n, const long *cell, const int *dir, const double *delta, const double *vel);
+
+/* Near-wall interpolation / spreading (b may be NULL: plain kernel) and
+ * vertex advection with reflection off the boundary. */
+void or_interpolate_near(int kernel, int nx, int ny, int nz, const int periodic[3], const double *ufield,
+                         long nv, const double *pos, const or_boundary *b, double *vel);
+void or_spread_near(int kernel, int nx, int ny, int nz, const int periodic[3], double *ffield, long nv,
+                    const double *pos, const double *fvert, const or_boundary *b);
+void or_move_reflect(long nv, double *pos, const double *vel, const or_boundary *b);
+
 #ifdef __cplusplus
 }
 #endif

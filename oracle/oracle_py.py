@@ -35,6 +35,16 @@ _SIGS = {
     "or_moment_of_inertia": (None, [_L, _L, _P, _P, _P]),
     "or_ray_hits_z": (_L, [_L, _L, _P, _P, _L, _P]),
     "or_voxel_fill": (None, [_I, _I, _I, _P, _L, _P, _P]),
+    "or_boundary_contains": (_I, [_P, _P, _D]),
+    "or_boundary_less_than": (_I, [_P, _P, _D]),
+    "or_boundary_distance": (_D, [_P, _P]),
+    "or_boundary_trace_ray": (_I, [_P, _P, _P, _P]),
+    "or_boundary_normal": (None, [_P, _P, _P]),
+    "or_guo_wall_nodes": (_L, [_I, _I, _I, _P, _L, _P, _P, _P, _P]),
+    "or_guo_wall_apply": (None, [_I, _I, _I, _P, _P, _P, _L, _P, _P, _P, _P]),
+    "or_interpolate_near": (None, [_I, _I, _I, _I, _P, _P, _L, _P, _P, _P]),
+    "or_spread_near": (None, [_I, _I, _I, _I, _P, _P, _L, _P, _P, _P]),
+    "or_move_reflect": (None, [_L, _P, _P, _P]),
 }
 
 
@@ -69,9 +79,15 @@ def _c(a, dtype):
 
 # ----------------------------------------------------------------- lattice
 def lbm_run(f, force, phase, mask, periodic, omega_out, omega_in, steps,
-            wall_axis=1, wall_u=None):
+            wall_axis=1, wall_u=None, guo=None):
     """Advance AoS populations f (nx,ny,nz,19) by `steps` collide-and-stream
-    steps; returns a new array."""
+    steps; returns a new array.  `guo`: (wall nodes, wall velocities) of a
+    Guo curved wall applied after every step."""
+    if guo is not None:
+        for _ in range(steps):
+            f = lbm_run(f, force, phase, mask, periodic, omega_out, omega_in, 1, wall_axis, wall_u)
+            f = guo_wall_apply(f, periodic, guo[0], guo[1], mask)
+        return f
     f = _c(f, np.float64).copy()
     nx, ny, nz = f.shape[:3]
     tmp = np.empty_like(f)
@@ -83,6 +99,44 @@ def lbm_run(f, force, phase, mask, periodic, omega_out, omega_in, steps,
     for _ in range(steps):
         lib.or_lbm_step(nx, ny, nz, _p(per), omega_out, omega_in, _p(f), _p(tmp),
                         _p(force), _p(phase), _p(mask), wall_axis, _p(wu))
+    return f
+
+
+class Boundary(ctypes.Structure):
+    """Analytic boundary: kind 0 pipe along x (y0, z0, radius, length),
+    kind 1 parallel plates y0 < y < y1."""
+    _fields_ = [("kind", ctypes.c_int), ("p", ctypes.c_double * 4)]
+
+    @classmethod
+    def pipe(cls, y0, z0, radius, length=0.0):
+        return cls(0, (ctypes.c_double * 4)(y0, z0, radius, length))
+
+    @classmethod
+    def plates(cls, y0, y1):
+        return cls(1, (ctypes.c_double * 4)(y0, y1, 0.0, 0.0))
+
+
+def guo_wall_nodes(shape, boundary):
+    """Wall nodes of the Guo curved wall: dict(cell, dir, delta, wall_pos)."""
+    nx, ny, nz = shape
+    n = lib.or_guo_wall_nodes(nx, ny, nz, ctypes.byref(boundary), 0, None, None, None, None)
+    cell = np.empty(n, np.int64)
+    d = np.empty(n, np.int32)
+    delta = np.empty(n)
+    wp = np.empty((n, 3))
+    lib.or_guo_wall_nodes(nx, ny, nz, ctypes.byref(boundary), n, _p(cell), _p(d), _p(delta), _p(wp))
+    return dict(cell=cell, dir=d, delta=delta, wall_pos=wp)
+
+
+def guo_wall_apply(f, periodic, walls, vel, mask=None):
+    """The Guo extrapolation over the wall nodes (in place on a copy)."""
+    f = _c(f, np.float64).copy()
+    nx, ny, nz = f.shape[:3]
+    vel = _c(np.broadcast_to(np.asarray(vel, float), (len(walls["cell"]), 3)), np.float64)
+    m = None if mask is None else _p(_c(mask, np.uint8))
+    lib.or_guo_wall_apply(nx, ny, nz, _p(_c(periodic, np.int32)), _p(f), m, len(walls["cell"]),
+                          _p(_c(walls["cell"], np.int64)), _p(_c(walls["dir"], np.int32)),
+                          _p(_c(walls["delta"], np.float64)), _p(vel))
     return f
 
 
@@ -162,6 +216,36 @@ def spread(kernel, shape, periodic, pos, fvert, base=None):
     lib.or_spread(k, shape[0], shape[1], shape[2], _p(_c(periodic, np.int32)), _p(field),
                   len(pos), _p(pos), _p(_c(fvert, np.float64)))
     return field
+
+
+def interpolate_near(kernel, ufield, periodic, pos, boundary):
+    """Interpolation with the near-wall blend (boundary None: plain)."""
+    k = KERNELS.get(kernel, kernel)
+    u = _c(ufield, np.float64)
+    pos = _c(pos, np.float64)
+    nx, ny, nz = u.shape[:3]
+    vel = np.empty((len(pos), 3))
+    b = None if boundary is None else ctypes.byref(boundary)
+    lib.or_interpolate_near(k, nx, ny, nz, _p(_c(periodic, np.int32)), _p(u), len(pos), _p(pos), b, _p(vel))
+    return vel
+
+
+def spread_near(kernel, shape, periodic, pos, fvert, boundary, base=None):
+    """Spreading of all vertices, bulk vertices first, then the near-wall
+    ones with the blend (boundary None: plain, in vertex order)."""
+    k = KERNELS.get(kernel, kernel)
+    field = np.zeros(tuple(shape) + (3,)) if base is None else _c(base, np.float64).copy()
+    pos = _c(pos, np.float64)
+    b = None if boundary is None else ctypes.byref(boundary)
+    lib.or_spread_near(k, shape[0], shape[1], shape[2], _p(_c(periodic, np.int32)), _p(field), len(pos),
+                       _p(pos), _p(_c(fvert, np.float64)), b)
+    return field
+
+
+def move_reflect(pos, vel, boundary):
+    pos = _c(pos, np.float64).copy()
+    lib.or_move_reflect(len(pos), _p(pos), _p(_c(vel, np.float64)), ctypes.byref(boundary))
+    return pos
 
 
 def capsule_forces(tri, rest_pos, pos, G, C, model="skalak", bug_compatible=True):
@@ -272,8 +356,11 @@ class Particle:
                             self.pos)
         return F
 
-    def move(self, vel, shape, periodic):
-        self.pos += vel
+    def move(self, vel, shape, periodic, boundary=None):
+        if boundary is None:
+            self.pos += vel
+        else:
+            self.pos = move_reflect(self.pos, vel, boundary)
         g = particle_update(self.tri, self.pos)
         self.area, self.volume = g["area"], g["volume"]
         com = g["com"].copy()
@@ -287,21 +374,33 @@ class Particle:
 
 
 def coupled_step(f, force, phase, mask, periodic, omega, omega_in, body, particles, kernel,
-                 wall_axis=1, wall_u=None):
+                 wall_axis=1, wall_u=None, boundary=None, guo=None):
     """One time step of the progs' loop, composed from the oracle's pieces:
     computeVelocity (u = j/rho + F/2 with the last collision's force), force
     reset to the body force, interpolation, membrane forces, spreading,
     vertex advection (+ COM remap), then collide-and-stream with the new
-    force.  Returns (f, force, per-particle velocities)."""
+    force.  With a `boundary` (set_boundary), near-wall vertices blend with
+    the top hat and are spread after all others, and advected vertices
+    reflect off it; `guo` = (wall nodes, velocities) of a Guo curved wall run
+    after the collide-and-stream.  Returns (f, force, per-particle
+    velocities)."""
     shape = f.shape[:3]
     _, u = lbm_moments(f, force, mask, wall_axis, wall_u)
     new_force = np.broadcast_to(np.asarray(body, float), tuple(shape) + (3,)).copy()
-    vels = []
-    for p in particles:
-        vels.append(interpolate(kernel, u, periodic, p.pos))
-    for p in particles:
-        new_force = spread(kernel, shape, periodic, p.pos, p.forces(), base=new_force)
+    if boundary is None:
+        vels = []
+        for p in particles:
+            vels.append(interpolate(kernel, u, periodic, p.pos))
+        for p in particles:
+            new_force = spread(kernel, shape, periodic, p.pos, p.forces(), base=new_force)
+    else:
+        allpos = np.concatenate([p.pos for p in particles])
+        allv = interpolate_near(kernel, u, periodic, allpos, boundary)
+        forces = np.concatenate([p.forces() for p in particles])
+        new_force = spread_near(kernel, shape, periodic, allpos, forces, boundary, base=new_force)
+        cuts = np.cumsum([len(p.pos) for p in particles])[:-1]
+        vels = np.split(allv, cuts)
     for p, v in zip(particles, vels):
-        p.move(v, shape, periodic)
-    f = lbm_run(f, new_force, phase, mask, periodic, omega, omega_in, 1, wall_axis, wall_u)
+        p.move(v, shape, periodic, boundary)
+    f = lbm_run(f, new_force, phase, mask, periodic, omega, omega_in, 1, wall_axis, wall_u, guo=guo)
     return f, new_force, vels
