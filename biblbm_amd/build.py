@@ -61,7 +61,7 @@ def build(verbose=False, jobs=None):
     build_examples()
 
 
-EXAMPLES = ["capsule_in_shear"]
+EXAMPLES = ["capsule_in_shear", "rbc_in_pipe"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 

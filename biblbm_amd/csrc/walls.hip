@@ -6,7 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
-#include <unordered_set>
+#include <unordered_map>
 #include <vector>
 
 #include "ibm.hpp"
@@ -25,21 +25,34 @@ struct bl_guo_wall_s {
     int* d_dir = nullptr;
     double* d_delta = nullptr;
     double* d_vel = nullptr;
+    // Nodes whose link reads another wall node (lattice nodes on the
+    // boundary surface) run after the others, one at a time in node order,
+    // as the reference's sequential functional does; a read of a node that
+    // comes later in that order sees its value from before the pass
+    // (snapshot).
+    long n_indep = 0, n_dep = 0, n_snap = 0;
+    int* d_indep = nullptr;         // node indices
+    int* d_dep = nullptr;           // node indices, ascending
+    int* d_src = nullptr;           // per dependent node and neighbour: -1 lattice, else snapshot slot
+    unsigned* d_snap_cell = nullptr;
+    double* d_snap = nullptr;       // 19 per snapshot slot
     bool inserted = false;
 };
 
 namespace bl {
 namespace {
 
-// One thread per wall node (GuoRigidWallBoundaryFunctional::process,
-// src/fsi/GuoBoundaryCondition.hh:138-199, depth 2): moments and
-// non-equilibrium parts of the two fluid cells along the node's link,
-// extrapolated wall momentum and non-equilibrium part, and the node's
-// post-stream populations rewritten as feq(rhoBar_1, j_w) + fNeq_w.  The
-// equilibrium comes from the node's own dynamics: zero on no-dynamics and
-// bounce-back cells (src/core/dynamics.hh:773-777,946-950).  Nodes never
-// read each other (checked when the wall is created), so they run in
-// parallel.  Arithmetic in FP64 whatever the storage.
+// GuoRigidWallBoundaryFunctional::process (src/fsi/GuoBoundaryCondition.hh:
+// 138-199, depth 2): moments and non-equilibrium parts of the two cells
+// along the node's link, extrapolated wall momentum and non-equilibrium
+// part, and the node's post-stream populations rewritten as
+// feq(rhoBar_1, j_w) + fNeq_w.  The equilibrium comes from the node's own
+// dynamics: zero on no-dynamics and bounce-back cells
+// (src/core/dynamics.hh:773-777,946-950).  The reference runs the nodes one
+// after the other in place; nodes that read only non-node cells are
+// independent and run one thread each, the few that read other nodes keep
+// the sequential semantics (see bl_guo_wall_s).  Arithmetic in FP64
+// whatever the storage.
 __device__ __forceinline__ double feq_d(int i, double rhoBar, const double* j, double jSqr)
 {
     const double invRho = 1.0 / (rhoBar + 1.0);
@@ -49,39 +62,49 @@ __device__ __forceinline__ double feq_d(int i, double rhoBar, const double* j, d
 
 __device__ __forceinline__ int wrap_axis(int q, int n, int per) { return per ? (q < 0 ? q + n : (q >= n ? q - n : q)) : q; }
 
+struct GuoNodes {
+    long n;
+    const unsigned* cell;
+    const int* dir;
+    const double* delta;
+    const double* vel;
+};
+
+__device__ __forceinline__ long nbr_cell(const LatView& v, int x, int y, int z, int i, int m, int& qx, int& qy,
+                                         int& qz)
+{
+    qx = wrap_axis(x + m * cx(i), v.nx, v.per[0]);
+    qy = wrap_axis(y + m * cy(i), v.ny, v.per[1]);
+    qz = wrap_axis(z + m * cz(i), v.nz, v.per[2]);
+    return ((long)qx * v.ny + qy) * v.nz + qz;
+}
+
 template <class S>
-__global__ void __launch_bounds__(64) k_guo_wall(LatView v, long n, const unsigned* __restrict__ cell,
-                                                 const int* __restrict__ dir, const double* __restrict__ delta,
-                                                 const double* __restrict__ vel, bool swapped)
+__device__ __forceinline__ void load_cell(const LatView& v, long q, int qx, int qy, int qz, bool swapped, double* f)
 {
     using C = typename Pop<S>::C;
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const unsigned c = cell[k];
-    int x, y, z;
-    decode(c, v.ny, v.nz, x, y, z);
-    const int i = dir[k];
-    double f1[Q], f2[Q];
-    for (int m = 1; m <= 2; ++m) {
-        const int qx = wrap_axis(x + m * cx(i), v.nx, v.per[0]);
-        const int qy = wrap_axis(y + m * cy(i), v.ny, v.per[1]);
-        const int qz = wrap_axis(z + m * cz(i), v.nz, v.per[2]);
-        const long q = ((long)qx * v.ny + qy) * v.nz + qz;
-        Nbr nb;
-        nb.init(v, qx, qy, qz);
-        C g[Q];
-        load_post<S>(v, nb, q, swapped, g);
-        double* f = m == 1 ? f1 : f2;
-        for (int p = 0; p < Q; ++p) f[p] = (double)g[p];
-    }
+    Nbr nb;
+    nb.init(v, qx, qy, qz);
+    C g[Q];
+    load_post<S>(v, nb, q, swapped, g);
+    for (int p = 0; p < Q; ++p) f[p] = (double)g[p];
+}
+
+// The extrapolation for node k from its two neighbours' populations f1, f2.
+template <class S>
+__device__ void guo_node(const LatView& v, const GuoNodes& g, long k, int x, int y, int z, const double* f1,
+                         const double* f2, bool swapped)
+{
+    using C = typename Pop<S>::C;
+    const unsigned c = g.cell[k];
     double j1[3], j2[3];
     const double rb1 = sum_rho(f1), rb2 = sum_rho(f2);
     sum_j(f1, j1);
     sum_j(f2, j2);
     const double j1sqr = j1[0] * j1[0] + j1[1] * j1[1] + j1[2] * j1[2];
     const double j2sqr = j2[0] * j2[0] + j2[1] * j2[1] + j2[2] * j2[2];
-    const double dl = delta[k];
-    const double* u = vel + 3 * k;
+    const double dl = g.delta[k];
+    const double* u = g.vel + 3 * k;
     const double wj[3] = {(rb1 + 1.0) * u[0], (rb1 + 1.0) * u[1], (rb1 + 1.0) * u[2]};
     double jw[3];
     if (dl < 0.75) {
@@ -104,12 +127,78 @@ __global__ void __launch_bounds__(64) k_guo_wall(LatView v, long n, const unsign
     store_post<S>(v, nb, c, swapped, fw);
 }
 
+// the nodes that read only fluid cells, one thread each
+template <class S>
+__global__ void __launch_bounds__(64) k_guo_wall(LatView v, GuoNodes g, long n, const int* __restrict__ list,
+                                                 bool swapped)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long k = list[t];
+    int x, y, z;
+    decode(g.cell[k], v.ny, v.nz, x, y, z);
+    const int i = g.dir[k];
+    double f1[Q], f2[Q];
+    int qx, qy, qz;
+    long q = nbr_cell(v, x, y, z, i, 1, qx, qy, qz);
+    load_cell<S>(v, q, qx, qy, qz, swapped, f1);
+    q = nbr_cell(v, x, y, z, i, 2, qx, qy, qz);
+    load_cell<S>(v, q, qx, qy, qz, swapped, f2);
+    guo_node<S>(v, g, k, x, y, z, f1, f2, swapped);
+}
+
+// populations of the wall nodes that a dependent node reads before their
+// own turn, taken before the pass changes them
+template <class S>
+__global__ void __launch_bounds__(64) k_guo_snapshot(LatView v, long n, const unsigned* __restrict__ cell,
+                                                     double* __restrict__ snap, bool swapped)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int x, y, z;
+    decode(cell[s], v.ny, v.nz, x, y, z);
+    load_cell<S>(v, cell[s], x, y, z, swapped, snap + (long)Q * s);
+}
+
+// the dependent nodes, one at a time in node order (a single thread)
+template <class S>
+__global__ void __launch_bounds__(64) k_guo_dep(LatView v, GuoNodes g, long n, const int* __restrict__ dep,
+                                                const int* __restrict__ src, const double* __restrict__ snap,
+                                                bool swapped)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (long t = 0; t < n; ++t) {
+        const long k = dep[t];
+        int x, y, z;
+        decode(g.cell[k], v.ny, v.nz, x, y, z);
+        double f[2][Q];
+        for (int m = 0; m < 2; ++m) {
+            const int sl = src[2 * t + m];
+            if (sl >= 0) {
+                for (int p = 0; p < Q; ++p) f[m][p] = snap[(long)Q * sl + p];
+            } else {
+                int qx, qy, qz;
+                const long q = nbr_cell(v, x, y, z, g.dir[k], m + 1, qx, qy, qz);
+                load_cell<S>(v, q, qx, qy, qz, swapped, f[m]);
+            }
+        }
+        guo_node<S>(v, g, k, x, y, z, f[0], f[1], swapped);
+    }
+}
+
 template <class S>
 void launch_guo(bl_guo_wall w, const LatView& v, bool swapped, hipStream_t s)
 {
-    const unsigned nb = (unsigned)((w->n + 63) / 64);
-    hipLaunchKernelGGL(k_guo_wall<S>, dim3(nb), dim3(64), 0, s, v, w->n, (const unsigned*)w->d_cell,
-                       (const int*)w->d_dir, (const double*)w->d_delta, (const double*)w->d_vel, swapped);
+    const GuoNodes g{w->n, w->d_cell, w->d_dir, w->d_delta, w->d_vel};
+    if (w->n_snap)
+        hipLaunchKernelGGL(k_guo_snapshot<S>, dim3((unsigned)((w->n_snap + 63) / 64)), dim3(64), 0, s, v, w->n_snap,
+                           (const unsigned*)w->d_snap_cell, w->d_snap, swapped);
+    if (w->n_indep)
+        hipLaunchKernelGGL(k_guo_wall<S>, dim3((unsigned)((w->n_indep + 63) / 64)), dim3(64), 0, s, v, g, w->n_indep,
+                           (const int*)w->d_indep, swapped);
+    if (w->n_dep)
+        hipLaunchKernelGGL(k_guo_dep<S>, dim3(1), dim3(64), 0, s, v, g, w->n_dep, (const int*)w->d_dep,
+                           (const int*)w->d_src, (const double*)w->d_snap, swapped);
 }
 
 int upload_vel(bl_guo_wall w)
@@ -204,12 +293,18 @@ int bl_guo_wall_create(bl_lattice lat, const bl_boundary* b, bl_guo_wall* out)
             }
     w->n = (long)w->cell.size();
     w->vel.assign(3 * w->n, 0.0);
-    // the two neighbours along each node's link: inside the lattice, not nodes
-    std::unordered_set<long long> nodes(w->cell.begin(), w->cell.end());
+    // the two neighbours along each node's link must lie in the lattice;
+    // those that are wall nodes themselves make the reading node dependent
+    std::unordered_map<long long, int> node_of;
+    for (long k = 0; k < w->n; ++k) node_of[w->cell[k]] = (int)k;
+    std::vector<int> indep, dep, src, snap_node;
+    std::unordered_map<int, int> snap_slot;
     for (long k = 0; k < w->n; ++k) {
         const long long c = w->cell[k];
         const int p[3] = {(int)(c / ((long long)n3[1] * n3[2])), (int)((c / n3[2]) % n3[1]), (int)(c % n3[2])};
         const int d[3] = {cx(w->dir[k]), cy(w->dir[k]), cz(w->dir[k])};
+        int s2[2] = {-1, -1};
+        bool is_dep = false;
         for (int m = 1; m <= 2; ++m) {
             int q[3];
             bool inside = true;
@@ -220,26 +315,59 @@ int bl_guo_wall_create(bl_lattice lat, const bl_boundary* b, bl_guo_wall* out)
                     else inside = false;
                 }
             }
-            const long long qc = ((long long)q[0] * n3[1] + q[1]) * n3[2] + q[2];
-            if (!inside || nodes.count(qc)) {
+            if (!inside) {
                 delete w;
                 return fail(BL_ERR_ARG, "bl_guo_wall_create: wall node (" + std::to_string(p[0]) + ", " +
                                             std::to_string(p[1]) + ", " + std::to_string(p[2]) +
-                                            ") has a neighbour along its link " +
-                                            (inside ? "that is a wall node itself" : "outside the lattice"));
+                                            ") has a neighbour along its link outside the lattice");
+            }
+            const auto it = node_of.find(((long long)q[0] * n3[1] + q[1]) * n3[2] + q[2]);
+            if (it == node_of.end()) continue;
+            is_dep = true;
+            if (it->second > (int)k) {  // read before its own turn
+                auto sl = snap_slot.find(it->second);
+                if (sl == snap_slot.end()) {
+                    sl = snap_slot.emplace(it->second, (int)snap_node.size()).first;
+                    snap_node.push_back(it->second);
+                }
+                s2[m - 1] = sl->second;
             }
         }
+        if (is_dep) {
+            dep.push_back((int)k);
+            src.push_back(s2[0]);
+            src.push_back(s2[1]);
+        } else {
+            indep.push_back((int)k);
+        }
     }
+    w->n_indep = (long)indep.size();
+    w->n_dep = (long)dep.size();
+    w->n_snap = (long)snap_node.size();
     if (w->n) {
-        std::vector<unsigned> cu(w->cell.begin(), w->cell.end());
+        std::vector<unsigned> cu(w->cell.begin(), w->cell.end()), snap_cell;
+        for (int j : snap_node) snap_cell.push_back((unsigned)w->cell[j]);
         hipError_t e = hipMalloc(&w->d_cell, sizeof(unsigned) * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_dir, sizeof(int) * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_delta, sizeof(double) * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_vel, sizeof(double) * 3 * w->n);
+        if (e == hipSuccess) e = hipMalloc(&w->d_indep, sizeof(int) * std::max<size_t>(indep.size(), 1));
+        if (e == hipSuccess) e = hipMalloc(&w->d_dep, sizeof(int) * std::max<size_t>(dep.size(), 1));
+        if (e == hipSuccess) e = hipMalloc(&w->d_src, sizeof(int) * std::max<size_t>(src.size(), 1));
+        if (e == hipSuccess) e = hipMalloc(&w->d_snap_cell, sizeof(unsigned) * std::max<size_t>(snap_cell.size(), 1));
+        if (e == hipSuccess) e = hipMalloc(&w->d_snap, sizeof(double) * Q * std::max<size_t>(snap_cell.size(), 1));
         if (e == hipSuccess) e = hipMemcpy(w->d_cell, cu.data(), sizeof(unsigned) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w->d_dir, w->dir.data(), sizeof(int) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w->d_delta, w->delta.data(), sizeof(double) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(w->d_vel, 0, sizeof(double) * 3 * w->n);
+        if (e == hipSuccess && !indep.empty())
+            e = hipMemcpy(w->d_indep, indep.data(), sizeof(int) * indep.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !dep.empty())
+            e = hipMemcpy(w->d_dep, dep.data(), sizeof(int) * dep.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !src.empty())
+            e = hipMemcpy(w->d_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !snap_cell.empty())
+            e = hipMemcpy(w->d_snap_cell, snap_cell.data(), sizeof(unsigned) * snap_cell.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             bl_guo_wall_destroy(w);
             return fail(BL_ERR_HIP, std::string("bl_guo_wall_create: ") + hipGetErrorString(e));
@@ -261,6 +389,11 @@ int bl_guo_wall_destroy(bl_guo_wall w)
     hipFree(w->d_dir);
     hipFree(w->d_delta);
     hipFree(w->d_vel);
+    hipFree(w->d_indep);
+    hipFree(w->d_dep);
+    hipFree(w->d_src);
+    hipFree(w->d_snap_cell);
+    hipFree(w->d_snap);
     delete w;
     return BL_OK;
 }

@@ -196,9 +196,10 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
  * 17-22, instantiator :62-135): the wall nodes are the cells outside the
  * boundary with a lattice link that crosses it; each keeps the link most
  * opposed to the wall normal (direction index 1..18 towards the fluid) and
- * delta = 1 - t of its crossing.  Fails (BL_ERR_ARG) if a node's two fluid
- * neighbours along that link leave a non-periodic axis or are wall nodes
- * themselves. */
+ * delta = 1 - t of its crossing.  Fails (BL_ERR_ARG) if a node's two
+ * neighbours along that link leave the lattice across a non-periodic face.
+ * Nodes whose neighbours are wall nodes themselves (lattice nodes on the
+ * boundary surface) keep the reference's one-after-the-other order. */
 int bl_guo_wall_create(bl_lattice lat, const bl_boundary *b, bl_guo_wall *out);
 int bl_guo_wall_destroy(bl_guo_wall w);
 int bl_guo_wall_count(bl_guo_wall w, long long *n);

@@ -21,8 +21,13 @@
  *   fsi::ParticleShapeLibrary, DeformableCapsuleParticle, RBCParticle
  *                                     src/fsi/ParticleShapeFactory.hh:32-129,
  *                                     src/fsi/DeformableCapsule.h:22-64, src/fsi/RBCParticle.h:22-58
+ *   fsi::PipeBoundary, fsi::ParallelPlatesBoundary, fsi::BoundaryOutside,
+ *   fsi::GuoRigidWallBoundary (insert, set_wall_velocity)
+ *                                     src/fsi/Boundary.h:114-365,
+ *                                     src/fsi/GuoBoundaryCondition.h:19-52
  *   fsi::ImmersedBoundaryDynamics3D: add_particle, init, get_particle,
- *   compute_total_volume, write_particles_as_vtk, and the per-step functional
+ *   set_boundary, compute_total_volume, write_particles_as_vtk, and the
+ *   per-step functional
  *   applyProcessingFunctional(wrap_ibm_dynamics3D(ibm), domain, lattice, u)
  *                                     src/fsi/ImmersedBoundaryDynamics.h:57-114,
  *                                     src/fsi/DataProcessors.hh:94-131
@@ -264,6 +269,15 @@ public:
                     mask_[(x * ny_ + y) * nz_ + z] = code;
         if (h_) pushMask();
     }
+    template <class Pred>
+    void setMaskWhere(const Box3D& b, Pred&& pred, unsigned char code)
+    {
+        for (plint x = std::max<plint>(b.x0, 0); x <= std::min(b.x1, nx_ - 1); ++x)
+            for (plint y = std::max<plint>(b.y0, 0); y <= std::min(b.y1, ny_ - 1); ++y)
+                for (plint z = std::max<plint>(b.z0, 0); z <= std::min(b.z1, nz_ - 1); ++z)
+                    if (pred(x, y, z)) mask_[(x * ny_ + y) * nz_ + z] = code;
+        if (h_) pushMask();
+    }
     void setWall(int axis, bool hi, const Array<T, 3>& u)
     {
         if (wall_axis_ >= 0 && wall_axis_ != axis)
@@ -340,6 +354,17 @@ template <typename T, template <typename U> class Descriptor>
 void defineDynamics(MultiBlockLattice3D<T, Descriptor>& lattice, const Box3D& box, Dynamics<T, Descriptor>* dyn)
 {
     lattice.setMask(box, dyn->maskCode());
+    delete dyn;
+}
+
+// defineDynamics over the cells of the box where a domain functional holds
+// (src/dataProcessors/dataInitializerWrapper3D.hh); takes ownership of both.
+template <typename T, template <typename U> class Descriptor, class DomainFunctional>
+void defineDynamics(MultiBlockLattice3D<T, Descriptor>& lattice, const Box3D& box, DomainFunctional* domain,
+                    Dynamics<T, Descriptor>* dyn)
+{
+    std::unique_ptr<DomainFunctional> own(domain);
+    lattice.setMaskWhere(box, [&](plint x, plint y, plint z) { return (*own)(x, y, z); }, dyn->maskCode());
     delete dyn;
 }
 
@@ -428,6 +453,136 @@ void setBoundaryVelocity(MultiBlockLattice3D<T, Descriptor>& lattice, const Box3
 
 // ================================================================== fsi
 namespace fsi {
+
+// Analytic boundaries (src/fsi/Boundary.h): the same expressions as the
+// device code uses; desc() is what crosses the C-ABI.
+template <class T>
+class Boundary {
+public:
+    virtual ~Boundary() {}
+    virtual bool contains(const Array<T, 3>& p, T margin = 0) const = 0;
+    virtual Box3D get_bounding_box(plint margin) const = 0;
+    virtual bl_boundary desc() const = 0;
+};
+
+template <class T>
+class PipeBoundary : public Boundary<T> {
+public:
+    PipeBoundary() {}
+    PipeBoundary(T y0_, T z0_, T radius_, T length_) : y0(y0_), z0(z0_), radius(radius_), length(length_) {}
+    Box3D get_bounding_box(plint margin) const override
+    {
+        return Box3D((plint)std::floor(x0), (plint)std::ceil(x0 + std::floor(length)),
+                     (plint)std::ceil(y0 - radius) - margin, (plint)std::floor(y0 + radius) + margin,
+                     (plint)std::ceil(z0 - radius) - margin, (plint)std::floor(z0 + radius) + margin);
+    }
+    bool contains(const Array<T, 3>& p, T margin = 0) const override
+    {
+        return ((p[1] - y0) * (p[1] - y0) + (p[2] - z0) * (p[2] - z0)) < (radius - margin) * (radius - margin);
+    }
+    bl_boundary desc() const override { return bl_boundary{BL_BOUNDARY_PIPE, {y0, z0, radius, length}}; }
+    T x0 = 0, y0 = 0, z0 = 0, radius = 0, length = 0;
+};
+
+template <class T>
+class ParallelPlatesBoundary : public Boundary<T> {
+public:
+    ParallelPlatesBoundary(T y0_, T y1_) : y0(y0_), y1(y1_) {}
+    bool contains(const Array<T, 3>& p, T margin = 0) const override { return p[1] > y0 + margin && p[1] < y1 - margin; }
+    Box3D get_bounding_box(plint) const override { return Box3D(0, 0, (plint)y0, (plint)y1, 0, 0); }
+    bl_boundary desc() const override { return bl_boundary{BL_BOUNDARY_PLATES, {y0, y1, 0, 0}}; }
+
+private:
+    T y0, y1;
+};
+
+// Cells farther outside than `margin` (src/fsi/Boundary.h:114-128).  The
+// margin is an integer, as in the reference (whose progs pass sqrt(2): 1).
+template <class T>
+class BoundaryOutside {
+public:
+    BoundaryOutside(const Boundary<T>& b, plint margin) : b_(b), margin_((T)margin) {}
+    bool operator()(plint x, plint y, plint z) const { return !b_.contains(Array<T, 3>(x, y, z), -margin_); }
+
+private:
+    const Boundary<T>& b_;
+    T margin_;
+};
+
+// GuoRigidWallBoundary(boundary, lattice) (src/fsi/GuoBoundaryCondition.h:
+// 19-52): the wall nodes are found when it is constructed (the lattice is
+// created then); insert() makes the extrapolation run after every
+// collideAndStream and in initialize().
+template <typename T, template <typename U> class Descriptor>
+class GuoRigidWallBoundary {
+public:
+    struct NodeInfo {
+        Array<plint, 3> location;
+        T delta;
+        Array<plint, 3> direction;
+        Array<T, 3> wall_position;
+        Array<T, 3> velocity;
+    };
+    typedef typename std::vector<NodeInfo>::const_iterator const_iterator;
+
+    GuoRigidWallBoundary(const Boundary<T>& boundary, MultiBlockLattice3D<T, Descriptor>& lattice)
+        : boundary_(boundary), lattice_(lattice)
+    {
+        const bl_boundary b = boundary.desc();
+        blCheck(bl_guo_wall_create(lattice.handle(), &b, &w_), "GuoRigidWallBoundary");
+        long long n = 0;
+        blCheck(bl_guo_wall_count(w_, &n), "GuoRigidWallBoundary");
+        std::vector<long long> cell(n);
+        std::vector<int> dir(n);
+        std::vector<double> delta(n), wp(3 * n);
+        blCheck(bl_guo_wall_nodes(w_, cell.data(), dir.data(), delta.data(), wp.data()), "GuoRigidWallBoundary");
+        static const int C[19][3] = {{0, 0, 0},  {-1, 0, 0}, {0, -1, 0},  {0, 0, -1}, {-1, -1, 0}, {-1, 1, 0}, {-1, 0, -1},
+                                     {-1, 0, 1}, {0, -1, -1}, {0, -1, 1}, {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
+                                     {1, -1, 0}, {1, 0, 1},  {1, 0, -1}, {0, 1, 1},   {0, 1, -1}};
+        const plint ny = lattice.getNy(), nz = lattice.getNz();
+        nodes_.resize(n);
+        for (long long k = 0; k < n; ++k) {
+            NodeInfo& i = nodes_[k];
+            i.location = Array<plint, 3>(cell[k] / (ny * nz), (cell[k] / nz) % ny, cell[k] % nz);
+            i.direction = Array<plint, 3>(C[dir[k]][0], C[dir[k]][1], C[dir[k]][2]);
+            i.delta = delta[k];
+            i.wall_position = Array<T, 3>(wp[3 * k], wp[3 * k + 1], wp[3 * k + 2]);
+        }
+    }
+    ~GuoRigidWallBoundary()
+    {
+        if (w_) bl_guo_wall_destroy(w_);
+    }
+    GuoRigidWallBoundary(const GuoRigidWallBoundary&) = delete;
+    GuoRigidWallBoundary& operator=(const GuoRigidWallBoundary&) = delete;
+
+    const Boundary<T>& boundary() const { return boundary_; }
+    void insert() { blCheck(bl_guo_wall_insert(w_), "GuoRigidWallBoundary::insert"); }
+    void set_wall_velocity(const Array<T, 3>& velocity)
+    {
+        const double v[3] = {velocity[0], velocity[1], velocity[2]};
+        blCheck(bl_guo_wall_set_velocity(w_, v), "set_wall_velocity");
+        for (NodeInfo& i : nodes_) i.velocity = velocity;
+    }
+    template <class VelocityFunction>
+    void set_wall_velocity(VelocityFunction velocity_function)
+    {
+        std::vector<double> v;
+        for (NodeInfo& i : nodes_) {
+            i.velocity = velocity_function(i.wall_position);
+            v.insert(v.end(), {i.velocity[0], i.velocity[1], i.velocity[2]});
+        }
+        blCheck(bl_guo_wall_set_node_velocities(w_, v.data()), "set_wall_velocity");
+    }
+    const_iterator begin() const { return nodes_.begin(); }
+    const_iterator end() const { return nodes_.end(); }
+
+private:
+    const Boundary<T>& boundary_;
+    MultiBlockLattice3D<T, Descriptor>& lattice_;
+    std::vector<NodeInfo> nodes_;
+    bl_guo_wall w_ = nullptr;
+};
 
 template <class T>
 class ParticleShape {
@@ -719,8 +874,24 @@ public:
                     "add_particle");
         }
         blCheck(bl_ibm_init(g_), "init");
+        if (boundary_) set_boundary(boundary_);
     }
     bl_ibm handle() const { return g_; }
+
+    // set_boundary (src/fsi/ImmersedBoundaryDynamics.h:93): near-wall kernel
+    // blending and vertex reflection; the boundary must outlive this object
+    void set_boundary(const Boundary<T>* b)
+    {
+        boundary_ = b;
+        if (!g_) return;
+        lattice_.flushPendingBoundaryStep();
+        if (b) {
+            const bl_boundary d = b->desc();
+            blCheck(bl_ibm_set_boundary(g_, &d), "set_boundary");
+        } else {
+            blCheck(bl_ibm_set_boundary(g_, nullptr), "set_boundary");
+        }
+    }
 
     // current vertices of particle i into a host copy (src/fsi/ImmersedBoundaryDynamics.h:95)
     bool get_particle(plint i, std::vector<Array<T, 3>>& pos)
@@ -787,6 +958,7 @@ private:
     MultiBlockLattice3D<T, Descriptor>& lattice_;
     int kernel_;
     std::vector<const ParticleBase3D<T>*> parts_;
+    const Boundary<T>* boundary_ = nullptr;
     bl_ibm g_ = nullptr;
 };
 

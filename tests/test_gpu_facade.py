@@ -38,3 +38,55 @@ def test_capsule_in_shear_prog_matches_python_path(device, tmp_path):
     assert deform.shape[0] == steps // 10 + 1 and np.all(np.isfinite(deform))
     W.ibm.close()
     W.lattice.close()
+
+
+def test_rbc_in_pipe_prog_matches_python_path(device, tmp_path):
+    """The C++ rbc_in_pipe prog with the Guo curved wall, no dynamics beyond
+    the pipe and the pipe as the immersed boundary's wall, against the same
+    set-up built from the Python objects (GuoWall, outside_mask,
+    set_boundary): vertex positions and the velocity profile after the same
+    number of coupled steps."""
+    from biblbm_amd import meshes
+    from biblbm_amd.capi import call
+    from biblbm_amd.ibm import ImmersedBoundary
+    from biblbm_amd.lattice import Lattice
+    from biblbm_amd.walls import Boundary, GuoWall, outside_mask
+    R, length, steps = 8.0, 16.0, 12
+    v, t = meshes.biconcave(2, 3.5)
+    msh = tmp_path / "rbc.msh"
+    meshes.write_gmsh(str(msh), v, t)
+    exe = os.path.join(ROOT, "biblbm_amd", "examples", "rbc_in_pipe")
+    out = str(tmp_path / "run")
+    r = subprocess.run([exe, str(msh), str(R), str(length), str(steps), out, "1", "1"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    pos = np.loadtxt(out + "_final.txt")
+    prof = np.loadtxt(out + "_profile.txt")
+
+    c = R + 2
+    shape = (17, 21, 21)
+    b = Boundary.pipe(c, c, R, length)
+    L = Lattice(*shape, periodic=(1, 0, 0), omega=1.0)
+    W = GuoWall(L, b)
+    W.insert()
+    L.set_mask(outside_mask(b, shape, 1.0))
+    nu = 0.5 / 3
+    L.set_body_force([4 * nu * 0.02 / (R * R), 0, 0])
+    L.init_equilibrium(np.ones(shape), np.zeros(shape + (3,)))
+    call("bl_lattice_initialize", L.handle)
+    vol = np.sum(np.einsum("ij,ij->i", v[t[:, 0]], np.cross(v[t[:, 1]], v[t[:, 2]]))) / 6
+    ib = ImmersedBoundary(L, "peskin")
+    sid = ib.add_shape(v, t)
+    start = v + [length / 2, c, c]
+    ib.add_particle(sid, 2, [0.0015, 1.0, 0.5, 0.005, 0.5, vol, 0.0005, 0.0], start)
+    ib.init()
+    ib.set_boundary(b)
+    ib.coupled_steps(steps, use_graph=False)
+    ref = ib.positions()
+    assert np.linalg.norm(pos - ref) <= 1e-8 * np.linalg.norm(ref - start)
+    _, u = L.moments()
+    assert np.abs(prof[:, 1] - u[shape[0] // 2, :, int(c), 0]).max() <= 1e-12
+    ib.close()
+    W.close()
+    L.close()

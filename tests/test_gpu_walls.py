@@ -111,9 +111,56 @@ def test_guo_wall_rejects_a_lattice_too_thin_for_the_extrapolation(device):
     from biblbm_amd.capi import BlError
     from biblbm_amd.lattice import Lattice
     from biblbm_amd.walls import Boundary, GuoWall
-    L = Lattice(4, 5, 4, periodic=(True, False, True), omega=1.0)
+    L = Lattice(4, 3, 4, periodic=(True, False, True), omega=1.0)
     with pytest.raises(BlError):
-        GuoWall(L, Boundary.plates(0.5, 1.6))
+        GuoWall(L, Boundary.plates(1.5, 2.5))
+    L.close()
+
+
+def test_guo_wall_nodes_on_the_surface_keep_the_sequential_order(device):
+    """Integer centre and radius put lattice nodes exactly on the pipe
+    surface: some wall nodes then read other wall nodes.  The pass keeps the
+    reference's node-by-node order (the oracle runs it in place, in node
+    order); populations after several steps agree to round-off."""
+    from biblbm_amd.lattice import Lattice
+    from biblbm_amd.walls import Boundary, GuoWall, outside_mask
+    from oracle import oracle_py as O
+    shape, per = (6, 21, 21), (1, 0, 0)
+    b, ob = Boundary.pipe(10.0, 10.0, 8.0), O.Boundary.pipe(10.0, 10.0, 8.0)
+    mask = outside_mask(b, shape, 1.0)
+    rng = np.random.default_rng(12)
+    u = rng.uniform(-0.02, 0.02, shape + (3,))
+    f0 = O.lbm_init_equilibrium(rng.uniform(0.99, 1.01, shape), u)
+    w = O.guo_wall_nodes(shape, ob)
+    c = np.unravel_index(w["cell"], shape)
+    node = np.zeros(shape, bool)
+    node[c] = True
+    reads_node = False
+    C = np.array([[0, 0, 0], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1],
+                            [-1, 0, 1], [0, -1, -1], [0, -1, 1], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0],
+                            [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]])
+    for k in range(len(w["cell"])):
+        p = np.array([c[0][k], c[1][k], c[2][k]])
+        for m in (1, 2):
+            q = p + m * C[w["dir"][k]]
+            q[0] %= shape[0]
+            reads_node = reads_node or bool(node[tuple(q)])
+    assert reads_node
+    vel = np.tile([0.01, 0.0, 0.0], (len(w["cell"]), 1))
+    L = Lattice(*shape, periodic=per, omega=1 / 0.8)
+    L.set_mask(mask)
+    L.put_populations(f0)
+    W = GuoWall(L, b)
+    W.set_wall_velocity([0.01, 0.0, 0.0])
+    W.insert()
+    steps = 6
+    L.collide_and_stream(steps)
+    ref = O.lbm_run(f0, np.zeros(shape + (3,)), np.zeros(shape), mask, per, 1 / 0.8, 1 / 0.8, steps, guo=(w, vel))
+    chk = (mask == 0) | node
+    err = np.abs(L.get_populations() - ref)[chk].max() / np.abs(ref[chk]).max()
+    print("surface nodes: rel err", err)
+    assert err <= 1e-12
+    W.close()
     L.close()
 
 
