@@ -22,6 +22,8 @@ bounding box).  Extra objects on the same JSON line:
                 velocity error against the FP64 run
   ensemble      64 shear boxes of uneven sizes advanced by one hipGraph
                 launch, against the same boxes stepped one after another
+  second_order  rbc_in_pipe with the reference prog's default wall: the Guo
+                curved wall on the pipe surface, no dynamics beyond it
 """
 import argparse
 import json
@@ -243,6 +245,14 @@ def main():
                             roofline=roofline(W32, "f32"))
         W32.ibm.close()
         W32.lattice.close()
+        W2 = workloads.rbc_in_pipe("f64", second_order=True)
+        dt2 = time_coupled(W2, args.steps // 2, args.warmup // 2)
+        line["second_order"] = dict(value=round(W2.cells * (args.steps // 2) / dt2 / 1e6, 2),
+                                    ms_per_step=round(dt2 / (args.steps // 2) * 1e3, 5),
+                                    wall_nodes=W2.params["wall_nodes"], fluid_cells=W2.fluid_cells)
+        W2.ibm.close()
+        W2.guo.close()
+        W2.lattice.close()
         line["suspension"] = suspension_leg(max(50, args.steps // 10), 20)
         line["ensemble"] = ensemble_leg(max(50, args.steps // 10), 10)
     if not args.no_cpu_baseline:

@@ -25,15 +25,17 @@ struct bl_guo_wall_s {
     int* d_dir = nullptr;
     double* d_delta = nullptr;
     double* d_vel = nullptr;
-    // Nodes whose link reads another wall node (lattice nodes on the
-    // boundary surface) run after the others, one at a time in node order,
-    // as the reference's sequential functional does; a read of a node that
-    // comes later in that order sees its value from before the pass
-    // (snapshot).
-    long n_indep = 0, n_dep = 0, n_snap = 0;
-    int* d_indep = nullptr;         // node indices
-    int* d_dep = nullptr;           // node indices, ascending
-    int* d_src = nullptr;           // per dependent node and neighbour: -1 lattice, else snapshot slot
+    // The reference runs the nodes one after the other in place, so a node
+    // whose link reads another wall node (lattice nodes on the boundary
+    // surface) sees it updated if it comes earlier in node order and not
+    // yet updated if it comes later.  Here the later ones are read from a
+    // snapshot taken before the pass, and the nodes run in levels: a node's
+    // level is one more than the highest level among the earlier nodes it
+    // reads, and each level is one parallel launch.
+    long n_snap = 0;
+    std::vector<long> level_off;    // entries of level l: [level_off[l], level_off[l+1])
+    int* d_order = nullptr;         // node indices by level, ascending within a level
+    int* d_src = nullptr;           // per entry and neighbour: -1 the lattice, else a snapshot slot
     unsigned* d_snap_cell = nullptr;
     double* d_snap = nullptr;       // 19 per snapshot slot
     bool inserted = false;
@@ -48,10 +50,9 @@ namespace {
 // part, and the node's post-stream populations rewritten as
 // feq(rhoBar_1, j_w) + fNeq_w.  The equilibrium comes from the node's own
 // dynamics: zero on no-dynamics and bounce-back cells
-// (src/core/dynamics.hh:773-777,946-950).  The reference runs the nodes one
-// after the other in place; nodes that read only non-node cells are
-// independent and run one thread each, the few that read other nodes keep
-// the sequential semantics (see bl_guo_wall_s).  Arithmetic in FP64
+// (src/core/dynamics.hh:773-777,946-950).  One thread per node, in the
+// levels and with the snapshot described at bl_guo_wall_s, which give the
+// result of the reference's node-by-node pass.  Arithmetic in FP64
 // whatever the storage.
 __device__ __forceinline__ double feq_d(int i, double rhoBar, const double* j, double jSqr)
 {
@@ -127,28 +128,33 @@ __device__ void guo_node(const LatView& v, const GuoNodes& g, long k, int x, int
     store_post<S>(v, nb, c, swapped, fw);
 }
 
-// the nodes that read only fluid cells, one thread each
+// one level of nodes, one thread each
 template <class S>
-__global__ void __launch_bounds__(64) k_guo_wall(LatView v, GuoNodes g, long n, const int* __restrict__ list,
+__global__ void __launch_bounds__(64) k_guo_wall(LatView v, GuoNodes g, long n, const int* __restrict__ order,
+                                                 const int* __restrict__ src, const double* __restrict__ snap,
                                                  bool swapped)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const long k = list[t];
+    const long k = order[t];
     int x, y, z;
     decode(g.cell[k], v.ny, v.nz, x, y, z);
-    const int i = g.dir[k];
-    double f1[Q], f2[Q];
-    int qx, qy, qz;
-    long q = nbr_cell(v, x, y, z, i, 1, qx, qy, qz);
-    load_cell<S>(v, q, qx, qy, qz, swapped, f1);
-    q = nbr_cell(v, x, y, z, i, 2, qx, qy, qz);
-    load_cell<S>(v, q, qx, qy, qz, swapped, f2);
-    guo_node<S>(v, g, k, x, y, z, f1, f2, swapped);
+    double f[2][Q];
+    for (int m = 0; m < 2; ++m) {
+        const int sl = src[2 * t + m];
+        if (sl >= 0) {
+            for (int p = 0; p < Q; ++p) f[m][p] = snap[(long)Q * sl + p];
+        } else {
+            int qx, qy, qz;
+            const long q = nbr_cell(v, x, y, z, g.dir[k], m + 1, qx, qy, qz);
+            load_cell<S>(v, q, qx, qy, qz, swapped, f[m]);
+        }
+    }
+    guo_node<S>(v, g, k, x, y, z, f[0], f[1], swapped);
 }
 
-// populations of the wall nodes that a dependent node reads before their
-// own turn, taken before the pass changes them
+// populations of the wall nodes that earlier nodes read, taken before the
+// pass changes them
 template <class S>
 __global__ void __launch_bounds__(64) k_guo_snapshot(LatView v, long n, const unsigned* __restrict__ cell,
                                                      double* __restrict__ snap, bool swapped)
@@ -160,32 +166,6 @@ __global__ void __launch_bounds__(64) k_guo_snapshot(LatView v, long n, const un
     load_cell<S>(v, cell[s], x, y, z, swapped, snap + (long)Q * s);
 }
 
-// the dependent nodes, one at a time in node order (a single thread)
-template <class S>
-__global__ void __launch_bounds__(64) k_guo_dep(LatView v, GuoNodes g, long n, const int* __restrict__ dep,
-                                                const int* __restrict__ src, const double* __restrict__ snap,
-                                                bool swapped)
-{
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (long t = 0; t < n; ++t) {
-        const long k = dep[t];
-        int x, y, z;
-        decode(g.cell[k], v.ny, v.nz, x, y, z);
-        double f[2][Q];
-        for (int m = 0; m < 2; ++m) {
-            const int sl = src[2 * t + m];
-            if (sl >= 0) {
-                for (int p = 0; p < Q; ++p) f[m][p] = snap[(long)Q * sl + p];
-            } else {
-                int qx, qy, qz;
-                const long q = nbr_cell(v, x, y, z, g.dir[k], m + 1, qx, qy, qz);
-                load_cell<S>(v, q, qx, qy, qz, swapped, f[m]);
-            }
-        }
-        guo_node<S>(v, g, k, x, y, z, f[0], f[1], swapped);
-    }
-}
-
 template <class S>
 void launch_guo(bl_guo_wall w, const LatView& v, bool swapped, hipStream_t s)
 {
@@ -193,12 +173,12 @@ void launch_guo(bl_guo_wall w, const LatView& v, bool swapped, hipStream_t s)
     if (w->n_snap)
         hipLaunchKernelGGL(k_guo_snapshot<S>, dim3((unsigned)((w->n_snap + 63) / 64)), dim3(64), 0, s, v, w->n_snap,
                            (const unsigned*)w->d_snap_cell, w->d_snap, swapped);
-    if (w->n_indep)
-        hipLaunchKernelGGL(k_guo_wall<S>, dim3((unsigned)((w->n_indep + 63) / 64)), dim3(64), 0, s, v, g, w->n_indep,
-                           (const int*)w->d_indep, swapped);
-    if (w->n_dep)
-        hipLaunchKernelGGL(k_guo_dep<S>, dim3(1), dim3(64), 0, s, v, g, w->n_dep, (const int*)w->d_dep,
-                           (const int*)w->d_src, (const double*)w->d_snap, swapped);
+    for (size_t l = 0; l + 1 < w->level_off.size(); ++l) {
+        const long a = w->level_off[l], n = w->level_off[l + 1] - a;
+        hipLaunchKernelGGL(k_guo_wall<S>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, v, g, n,
+                           (const int*)w->d_order + a, (const int*)w->d_src + 2 * a, (const double*)w->d_snap,
+                           swapped);
+    }
 }
 
 int upload_vel(bl_guo_wall w)
@@ -294,17 +274,15 @@ int bl_guo_wall_create(bl_lattice lat, const bl_boundary* b, bl_guo_wall* out)
     w->n = (long)w->cell.size();
     w->vel.assign(3 * w->n, 0.0);
     // the two neighbours along each node's link must lie in the lattice;
-    // those that are wall nodes themselves make the reading node dependent
+    // reads of wall nodes decide the snapshot and the levels
     std::unordered_map<long long, int> node_of;
     for (long k = 0; k < w->n; ++k) node_of[w->cell[k]] = (int)k;
-    std::vector<int> indep, dep, src, snap_node;
+    std::vector<int> level(w->n, 0), src(2 * w->n, -1), snap_node;
     std::unordered_map<int, int> snap_slot;
     for (long k = 0; k < w->n; ++k) {
         const long long c = w->cell[k];
         const int p[3] = {(int)(c / ((long long)n3[1] * n3[2])), (int)((c / n3[2]) % n3[1]), (int)(c % n3[2])};
         const int d[3] = {cx(w->dir[k]), cy(w->dir[k]), cz(w->dir[k])};
-        int s2[2] = {-1, -1};
-        bool is_dep = false;
         for (int m = 1; m <= 2; ++m) {
             int q[3];
             bool inside = true;
@@ -323,26 +301,31 @@ int bl_guo_wall_create(bl_lattice lat, const bl_boundary* b, bl_guo_wall* out)
             }
             const auto it = node_of.find(((long long)q[0] * n3[1] + q[1]) * n3[2] + q[2]);
             if (it == node_of.end()) continue;
-            is_dep = true;
-            if (it->second > (int)k) {  // read before its own turn
-                auto sl = snap_slot.find(it->second);
+            const int j = it->second;
+            if (j < (int)k) {  // already updated when k runs
+                level[k] = std::max(level[k], level[j] + 1);
+            } else {           // not yet: its value from before the pass
+                auto sl = snap_slot.find(j);
                 if (sl == snap_slot.end()) {
-                    sl = snap_slot.emplace(it->second, (int)snap_node.size()).first;
-                    snap_node.push_back(it->second);
+                    sl = snap_slot.emplace(j, (int)snap_node.size()).first;
+                    snap_node.push_back(j);
                 }
-                s2[m - 1] = sl->second;
+                src[2 * k + m - 1] = sl->second;
             }
         }
-        if (is_dep) {
-            dep.push_back((int)k);
-            src.push_back(s2[0]);
-            src.push_back(s2[1]);
-        } else {
-            indep.push_back((int)k);
-        }
     }
-    w->n_indep = (long)indep.size();
-    w->n_dep = (long)dep.size();
+    const int nlev = w->n ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+    std::vector<int> order, osrc;
+    w->level_off.assign(1, 0);
+    for (int l = 0; l < nlev; ++l) {
+        for (long k = 0; k < w->n; ++k)
+            if (level[k] == l) {
+                order.push_back((int)k);
+                osrc.push_back(src[2 * k]);
+                osrc.push_back(src[2 * k + 1]);
+            }
+        w->level_off.push_back((long)order.size());
+    }
     w->n_snap = (long)snap_node.size();
     if (w->n) {
         std::vector<unsigned> cu(w->cell.begin(), w->cell.end()), snap_cell;
@@ -351,21 +334,16 @@ int bl_guo_wall_create(bl_lattice lat, const bl_boundary* b, bl_guo_wall* out)
         if (e == hipSuccess) e = hipMalloc(&w->d_dir, sizeof(int) * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_delta, sizeof(double) * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_vel, sizeof(double) * 3 * w->n);
-        if (e == hipSuccess) e = hipMalloc(&w->d_indep, sizeof(int) * std::max<size_t>(indep.size(), 1));
-        if (e == hipSuccess) e = hipMalloc(&w->d_dep, sizeof(int) * std::max<size_t>(dep.size(), 1));
-        if (e == hipSuccess) e = hipMalloc(&w->d_src, sizeof(int) * std::max<size_t>(src.size(), 1));
+        if (e == hipSuccess) e = hipMalloc(&w->d_order, sizeof(int) * w->n);
+        if (e == hipSuccess) e = hipMalloc(&w->d_src, sizeof(int) * 2 * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_snap_cell, sizeof(unsigned) * std::max<size_t>(snap_cell.size(), 1));
         if (e == hipSuccess) e = hipMalloc(&w->d_snap, sizeof(double) * Q * std::max<size_t>(snap_cell.size(), 1));
         if (e == hipSuccess) e = hipMemcpy(w->d_cell, cu.data(), sizeof(unsigned) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w->d_dir, w->dir.data(), sizeof(int) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w->d_delta, w->delta.data(), sizeof(double) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(w->d_vel, 0, sizeof(double) * 3 * w->n);
-        if (e == hipSuccess && !indep.empty())
-            e = hipMemcpy(w->d_indep, indep.data(), sizeof(int) * indep.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess && !dep.empty())
-            e = hipMemcpy(w->d_dep, dep.data(), sizeof(int) * dep.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess && !src.empty())
-            e = hipMemcpy(w->d_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w->d_order, order.data(), sizeof(int) * w->n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w->d_src, osrc.data(), sizeof(int) * 2 * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess && !snap_cell.empty())
             e = hipMemcpy(w->d_snap_cell, snap_cell.data(), sizeof(unsigned) * snap_cell.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -389,8 +367,7 @@ int bl_guo_wall_destroy(bl_guo_wall w)
     hipFree(w->d_dir);
     hipFree(w->d_delta);
     hipFree(w->d_vel);
-    hipFree(w->d_indep);
-    hipFree(w->d_dep);
+    hipFree(w->d_order);
     hipFree(w->d_src);
     hipFree(w->d_snap_cell);
     hipFree(w->d_snap);

@@ -4,7 +4,9 @@
                      and 63 moving at -/+0.01, tau 1.0, one Skalak capsule
                      (642-vertex icosphere, radius 8, Ca 0.1, C 10), Peskin.
   rbc_in_pipe        pipe radius 32, length 257 (bbox 257 x 67 x 67,
-                     staircase bounce-back), body force for centreline
+                     staircase bounce-back, or with second_order the Guo
+                     curved wall and no dynamics beyond it, the reference
+                     prog's default), body force for centreline
                      velocity 0.02, tau 0.9; one Evans-Fung red cell on the
                      2562-vertex icosphere, radius 12: Skalak in-plane +
                      bending kb = 0.02 G R^2 + global area and volume
@@ -34,6 +36,7 @@ import numpy as np
 from . import meshes
 from .ibm import ImmersedBoundary, CAPSULE_SKALAK, SKALAK_RBC
 from .lattice import Lattice, BOUNCE_BACK, VELOCITY_WALL
+from .walls import Boundary, GuoWall, outside_mask
 
 
 @dataclass
@@ -45,6 +48,7 @@ class Workload:
     fluid_cells: int
     vertices: int
     params: dict = field(default_factory=dict)
+    guo: object = None  # a Guo curved wall the lattice runs (kept alive with it)
 
 
 def rbc_constraint_params(G, radius, volume, K_area_over_G=20.0, kb=0.0):
@@ -100,10 +104,20 @@ def pipe_mask(R, length):
 
 
 def rbc_in_pipe(storage="f64", R=32, length=257, u_c=0.02, tau=0.9, level=4, cell_radius=12.0, Ca=0.1,
-                C=10.0, kernel="peskin"):
+                C=10.0, kernel="peskin", second_order=False):
+    """second_order: the reference prog's default wall
+    (progs/rbc_in_pipe/rbc_in_pipe.cpp:302-318): Guo curved wall on the pipe
+    surface, no dynamics beyond it (margin 1); otherwise bounce-back on every
+    cell outside the pipe."""
     mask, r2, c = pipe_mask(R, length)
     ny = nz = mask.shape[1]
     L = Lattice(length, ny, nz, periodic=(1, 0, 0), storage=storage, omega=1 / tau)
+    guo = None
+    if second_order:
+        b = Boundary.pipe(c, c, R, float(length))
+        guo = GuoWall(L, b)
+        guo.insert()
+        mask = outside_mask(b, mask.shape, 1.0)
     L.set_mask(mask)
     nu = (tau - 0.5) / 3.0
     g = 4.0 * nu * u_c / (R * R)
@@ -112,6 +126,8 @@ def rbc_in_pipe(storage="f64", R=32, length=257, u_c=0.02, tau=0.9, level=4, cel
     u = np.zeros((length, ny, nz, 3))
     u[..., 0] = prof[None, :, :]
     L.init_equilibrium(None, u)
+    if guo is not None:
+        L.initialize()
     G = nu * (u_c / R) * cell_radius / Ca
     kb = 0.02 * G * cell_radius ** 2
     k_bend = 2.0 * kb / np.sqrt(3.0)
@@ -126,7 +142,8 @@ def rbc_in_pipe(storage="f64", R=32, length=257, u_c=0.02, tau=0.9, level=4, cel
     fluid = int((mask == 0).sum())
     return Workload("rbc_in_pipe", L, ib, mask.size, fluid, len(v),
                     dict(R=R, length=length, u_c=u_c, tau=tau, G=G, kb=kb, k_bend=k_bend, g=g,
-                         cell_radius=cell_radius, kernel=kernel))
+                         cell_radius=cell_radius, kernel=kernel, second_order=second_order,
+                         wall_nodes=0 if guo is None else guo.n), guo)
 
 
 def _solid_samples(v):

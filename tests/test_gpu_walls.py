@@ -301,3 +301,46 @@ def test_pipe_guo_wall_with_near_wall_capsule(device):
     print("pipe: position error / displacement", err, "population rel err", err_f)
     assert err <= 1e-9 and err_f <= 1e-11
     W.close()
+
+
+def test_second_order_pipe_workload_matches_oracle(device):
+    """The benchmark's pipe with the reference prog's default wall (Guo
+    curved wall, no dynamics beyond it) at a reduced size: three coupled
+    steps of the red cell (Skalak + bending + area + volume) against the
+    oracle started from the same state."""
+    from biblbm_amd import workloads
+    from oracle import oracle_py as O
+    W = workloads.rbc_in_pipe("f64", R=8, length=20, level=2, cell_radius=3.5, second_order=True)
+    L, ib = W.lattice, W.ibm
+    f0 = L.get_populations()
+    pos0 = ib.positions()
+    p = W.params
+    c = (2 * 8 + 3 - 1) / 2.0
+    ob = O.Boundary.pipe(c, c, 8.0, 20.0)
+    shape = L.shape
+    w = O.guo_wall_nodes(shape, ob)
+    mask = np.zeros(shape, np.uint8)
+    from biblbm_amd.walls import Boundary, outside_mask
+    mask = outside_mask(Boundary.pipe(c, c, 8.0, 20.0), shape, 1.0)
+    from biblbm_amd import meshes
+    v, t = meshes.biconcave(2, 3.5)
+    vol = meshes.signed_volume(v, t)
+    params = workloads.skalak_rbc_params(p["G"], 10.0, 3.5, vol, p["k_bend"])
+    parts = [O.Particle(v, t, 3, params, pos0)]
+    steps = 3
+    ib.coupled_steps(steps, use_graph=False)
+    f = f0
+    body = (p["g"], 0.0, 0.0)
+    force = np.broadcast_to(np.asarray(body), shape + (3,)).copy()
+    for _ in range(steps):
+        f, force, _ = O.coupled_step(f, force, np.zeros(shape), mask, (1, 0, 0), 1 / 0.9, 1 / 0.9, body, parts,
+                                     "peskin", guo=(w, np.zeros((len(w["cell"]), 3))))
+    ref = parts[0].pos
+    err = np.linalg.norm(ib.positions() - ref) / np.linalg.norm(ref - pos0)
+    fluid = mask == 0
+    err_f = np.abs(L.get_populations() - f)[fluid].max() / np.abs(f[fluid]).max()
+    print("second-order pipe: position error / displacement", err, "population rel err", err_f)
+    assert err <= 1e-9 and err_f <= 1e-11
+    ib.close()
+    W.guo.close()
+    L.close()
