@@ -83,6 +83,10 @@ SIGNATURES = {
     "bl_guo_wall_insert": [_P],
     "bl_boundary_outside_mask": [_P, _I, _I, _I, _D, _P, ctypes.c_ubyte],
     "bl_ibm_set_boundary": [_P, _P],
+    "bl_potential_eval": [_P, _D, _PD, _PD],
+    "bl_ibm_add_wall_interaction": [_P, _P, _P],
+    "bl_ibm_add_pp_force": [_P, _P],
+    "bl_ibm_compute_collision_forces": [_P, _P, _P],
 }
 
 

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "ibm.hpp"
+#include "interactions.hpp"
 #include "lattice_dev.hpp"
 #include "walls.hpp"
 
@@ -623,15 +624,26 @@ __global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long 
 // Four lanes per vertex: lane l sums entries l, l+4, ... of the vertex's
 // list [links | triangles | pairs] in order, then a fixed butterfly adds the
 // lanes, so the result does not depend on scheduling.  No atomics.
+// Lane 0 starts from the wall-interaction decorators' forces, which the
+// reference applies before the membrane forces
+// (src/fsi/ImmersedBoundaryDynamics_forces.hh:293-296).
 constexpr int kGatherLanes = 4;
 
 __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, long ec_pair, long ec_link,
-                                                       const double* __restrict__ ec, double* __restrict__ force)
+                                                       const double* __restrict__ ec, double* __restrict__ force,
+                                                       DecoSet deco)
 {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long v = gid / kGatherLanes;
     const int lane = (int)(gid % kGatherLanes);
     double c0 = 0, c1 = 0, c2 = 0;
+    if (deco.n && lane == 0 && v < NV) {
+        double w[3] = {0, 0, 0};
+        deco_apply(deco, f.pos + 3 * v, w);
+        c0 = w[0];
+        c1 = w[1];
+        c2 = w[2];
+    }
     if (v < NV) {
         const int p = f.v_part[v];
         const int s = f.p_shape[p];
@@ -1679,10 +1691,10 @@ int do_forces(bl_ibm g, hipStream_t s)
         hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair,
                            g->ec_link, g->ec);
         hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
-                           g->ec_link, (const double*)g->ec, g->force);
+                           g->ec_link, (const double*)g->ec, g->force, deco_set(g));
     }
     BL_CHECK_LAUNCH();
-    return BL_OK;
+    return interactions_launch(g, s);
 }
 
 int do_spread(bl_ibm g)
@@ -1880,6 +1892,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
                     g->l_part};
     for (void* p : ptrs) hipFree(p);
+    interactions_free(g);
     for (auto& e : g->ev)
         if (e) hipEventDestroy(e);
     for (auto& e : g->fev)
@@ -2081,6 +2094,7 @@ int bl_ibm_init(bl_ibm g)
     for (auto& e : g->fev) BL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     g->inited = true;
     if ((rc = alloc_near(g))) return rc;
+    if (!g->pp.empty() && (rc = interactions_alloc(g))) return rc;
     // update() of every particle and the centre-of-mass mapping of add_particle
     rc = update_particles(g, 1);
     if (rc) return rc;

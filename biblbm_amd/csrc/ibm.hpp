@@ -85,6 +85,21 @@ struct bl_ibm_s {
     int* v_base_t = nullptr;          // per vertex: top-hat stencil base (3)
     double* v_w_t = nullptr;          // per vertex: top-hat weights (2 per axis)
 
+    // ---- interaction forces ----
+    struct WallDeco {
+        bl_boundary b;
+        bl_potential u;
+    };
+    std::vector<WallDeco> wall_deco;  // wall-interaction decorators, in order
+    std::vector<bl_potential> pp;     // particle-particle potentials, in order
+    int* pp_head = nullptr;   // per collision cell: the last vertex pushed onto its list, -1 if empty
+    int2* pp_info = nullptr;  // per collision cell: (first slot, vertex count), (0, 0) if empty
+    int* pp_next = nullptr;   // per vertex: next vertex of its cell's list
+    int* pp_cell = nullptr;   // per vertex: collision cell, -1 if the vertex does not take part
+    int* pp_slot = nullptr;   // per slot: vertex, grouped by cell, ascending within a cell
+    int* pp_ctr = nullptr;    // slots handed out
+    long pp_ncell = 0;
+
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
     double* v_w = nullptr;    // 12 per vertex (3 axes x up to 4 weights)
@@ -141,4 +156,11 @@ void ibm_invalidate_graphs(bl_ibm g);  // captured steps no longer match the lat
 // A full force field was loaded into the lattice (every tile flagged):
 // steps must run eagerly until the flags are consistent again.
 bool ibm_needs_settle(bl_ibm g);
+
+// Interaction forces (interactions.hip).
+struct DecoSet;
+DecoSet deco_set(bl_ibm g);                       // the wall-interaction decorators
+int interactions_alloc(bl_ibm g);                 // collision grid for the current vertices
+void interactions_free(bl_ibm g);
+int interactions_launch(bl_ibm g, hipStream_t s);  // particle-particle forces, added to the vertex forces
 }  // namespace bl

@@ -8,13 +8,17 @@
 // setExternalVector -> applyProcessingFunctional(wrap_ibm_dynamics3D(fsi))
 // -> collideAndStream.
 //
-// usage: rbc_in_pipe mesh.msh R L steps out_prefix [second_order] [ibm_boundary] [storage]
+// usage: rbc_in_pipe mesh.msh R L steps out_prefix [second_order] [ibm_boundary] [storage] [interactions]
 //   R, L          pipe radius and length (lattice units)
 //   second_order  1 (default): Guo curved wall; 0: bounce-back surface
 //   ibm_boundary  1: the immersed boundary blends and reflects at the pipe
 //                 wall (set_boundary, as the rbc_in_pipe_distribution prog
 //                 does); 0 (default): not, as rbc_in_pipe itself
 //   storage       f64 (default), f32 or f16s
+//   interactions  1: the prog's Morse wall interaction (r_min 2, beta 2,
+//                 strength 1e-4) and Morse particle-particle force (r_min 1,
+//                 beta 1, strength 1e-3), the values of its input.xml
+//                 (read_parameters, :150-178, and main, :735-741); 0 (default): none
 // Writes <out_prefix>_final.txt (vertex positions) and
 // <out_prefix>_profile.txt (u_x across the pipe at mid-length).
 #include <cmath>
@@ -44,7 +48,8 @@ private:
 int main(int argc, char** argv)
 {
     if (argc < 6) {
-        std::cerr << "usage: rbc_in_pipe mesh.msh R L steps out_prefix [second_order] [ibm_boundary] [storage]\n";
+        std::cerr << "usage: rbc_in_pipe mesh.msh R L steps out_prefix [second_order] [ibm_boundary] [storage] "
+                     "[interactions]\n";
         return 2;
     }
     try {
@@ -56,6 +61,7 @@ int main(int argc, char** argv)
         const bool second_order = argc > 6 ? std::atoi(argv[6]) != 0 : true;
         const bool ibm_boundary = argc > 7 ? std::atoi(argv[7]) != 0 : false;
         const std::string storage = argc > 8 ? argv[8] : "f64";
+        const bool interactions = argc > 9 ? std::atoi(argv[9]) != 0 : false;
 
         blCheck(bl_init(0), "bl_init");
         const T tau = 1.0, u_max = 0.02;
@@ -119,6 +125,12 @@ int main(int argc, char** argv)
 
         ImmersedBoundaryDynamics3D<T, DESCRIPTOR> fsi(lattice, shape_library, BL_KERNEL_PESKIN);
         if (ibm_boundary) fsi.set_boundary(&pipe_boundary);
+        WallInteraction<T, MorsePotential<T>> wall_interaction(pipe_boundary, MorsePotential<T>(2., 2., 1e-4));
+        PPPotentialForce<T, MorsePotential<T>> pp_interaction(MorsePotential<T>(1., 1., 1e-3));
+        if (interactions) {
+            fsi.add_force_decorator(&wall_interaction);
+            fsi.add_pp_force(&pp_interaction);
+        }
         fsi.add_particle(&rbc);
         fsi.init();
         std::unique_ptr<MultiTensorField3D<T, 3>> velocity = generateMultiTensorField<T, 3>(domain);

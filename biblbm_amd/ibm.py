@@ -59,6 +59,26 @@ class ImmersedBoundary:
         kernel blending and vertex reflection; None removes it."""
         call("bl_ibm_set_boundary", self.handle, None if boundary is None else ctypes.byref(boundary))
 
+    # --- interaction forces -------------------------------------------------
+    def add_wall_interaction(self, boundary, potential):
+        """add_force_decorator(new WallInteraction(boundary, potential))
+        (src/fsi/ImmersedBoundaryDynamics.h:78, src/fsi/ForceDecorator.hh:27-35)."""
+        call("bl_ibm_add_wall_interaction", self.handle, ctypes.byref(boundary), ctypes.byref(potential))
+
+    add_force_decorator = add_wall_interaction
+
+    def add_pp_force(self, potential):
+        """add_pp_force(new PPPotentialForce(potential))
+        (src/fsi/ImmersedBoundaryDynamics.h:79)."""
+        call("bl_ibm_add_pp_force", self.handle, ctypes.byref(potential))
+
+    def compute_collision_forces(self, potential, boundary=None):
+        """compute_collision_forces(potential, boundary)
+        (src/fsi/ImmersedBoundaryDynamics_forces.hh:505-521): vertex forces
+        := collision-handler forces alone."""
+        call("bl_ibm_compute_collision_forces", self.handle, ctypes.byref(potential),
+             None if boundary is None else ctypes.byref(boundary))
+
     # --- stages ------------------------------------------------------------
     def interpolate(self):
         call("bl_ibm_interpolate", self.handle)

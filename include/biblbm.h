@@ -230,6 +230,44 @@ int bl_boundary_outside_mask(const bl_boundary *b, int nx, int ny, int nz, doubl
  * (src/fsi/DeformableParticle3D.hh:26-50).  NULL: no boundary. */
 int bl_ibm_set_boundary(bl_ibm g, const bl_boundary *b);
 
+/* ------------------------------------------------ interaction forces */
+/* Interaction potentials (src/fsi/Potentials.h): MorsePotential(r_min,
+ * beta, strength), force 2 beta strength x (x - 1) with
+ * x = exp(beta (r_min - max(d, 0))), cut-off r_min (:28-56);
+ * SpringPotential(min_dist, strength), force (min_dist - d) strength,
+ * cut-off min_dist (:58-72). */
+#define BL_POTENTIAL_MORSE 0  /* p = {r_min, beta, strength} */
+#define BL_POTENTIAL_SPRING 1 /* p = {min_dist, strength, -}  */
+typedef struct {
+    int kind;
+    double p[3];
+} bl_potential;
+/* The potential's force at distance d and its cut-off (host only). */
+int bl_potential_eval(const bl_potential *u, double d, double *force, double *cutoff);
+/* add_force_decorator(new WallInteraction(boundary, potential))
+ * (src/fsi/ImmersedBoundaryDynamics.h:78, src/fsi/ForceDecorator.hh:27-35):
+ * before the membrane forces of every step, each vertex at distance
+ * d <= cut-off from the boundary gets force -= F(d) n.  Decorators apply
+ * in the order added; at most 4. */
+int bl_ibm_add_wall_interaction(bl_ibm g, const bl_boundary *b, const bl_potential *u);
+/* add_pp_force(new PPPotentialForce(potential))
+ * (src/fsi/ImmersedBoundaryDynamics.h:79, src/fsi/ImmersedBoundaryDynamics_forces.hh:
+ * 287-302,332-375): after the membrane forces, vertices of different
+ * particles interact through the collision handler
+ * (src/fsi/CollisionForces.h:84-161) over the lattice box: unit cells,
+ * pairs within the cell and with its 7 forward neighbours, and the
+ * handler's face / edge / corner pairs across periodic axes.  Vertices
+ * outside the box enlarged by the cut-off do not take part.  The cut-off
+ * must not exceed the kernel's half support - 0.5 (BL_ERR_ARG, where the
+ * reference exits).  At most 4. */
+int bl_ibm_add_pp_force(bl_ibm g, const bl_potential *u);
+/* compute_collision_forces(potential, boundary)
+ * (src/fsi/ImmersedBoundaryDynamics_forces.hh:505-521): vertex forces :=
+ * the collision handler's pair forces alone, plus, with a boundary (NULL:
+ * none), its wall forces (src/fsi/CollisionForces.h:167-187) on vertices
+ * in unit cells that the boundary, widened by the cut-off, crosses. */
+int bl_ibm_compute_collision_forces(bl_ibm g, const bl_potential *u, const bl_boundary *b);
+
 /* ----------------------------------------------------------- ensembles */
 /* No counterpart in the reference, which runs independent boxes as separate
  * programs (one rbc_in_shear run per box).  Groups n_boxes lattices (each

@@ -25,8 +25,13 @@
  *   fsi::GuoRigidWallBoundary (insert, set_wall_velocity)
  *                                     src/fsi/Boundary.h:114-365,
  *                                     src/fsi/GuoBoundaryCondition.h:19-52
+ *   fsi::MorsePotential, fsi::SpringPotential, fsi::PPPotentialForce,
+ *   fsi::WallInteraction          src/fsi/Potentials.h:28-72,
+ *                                     src/fsi/ParticleParticleInteraction.h:17-40,
+ *                                     src/fsi/ForceDecorator.hh:18-35
  *   fsi::ImmersedBoundaryDynamics3D: add_particle, init, get_particle,
- *   set_boundary, compute_total_volume, write_particles_as_vtk, and the
+ *   set_boundary, add_force_decorator, add_pp_force,
+ *   compute_collision_forces, compute_total_volume, write_particles_as_vtk, and the
  *   per-step functional
  *   applyProcessingFunctional(wrap_ibm_dynamics3D(ibm), domain, lattice, u)
  *                                     src/fsi/ImmersedBoundaryDynamics.h:57-114,
@@ -509,6 +514,98 @@ private:
     T margin_;
 };
 
+// Interaction potentials (src/fsi/Potentials.h:28-72): the force at a
+// distance and the cut-off; desc() is what crosses the C-ABI.
+template <class T>
+class MorsePotential {
+public:
+    MorsePotential(T r_min_, T beta_, T strength_) : r_min(r_min_), beta(beta_), strength(strength_) {}
+    T operator()(T dist) const
+    {
+        double f = 0;
+        const bl_potential u = desc();
+        blCheck(bl_potential_eval(&u, dist, &f, nullptr), "MorsePotential");
+        return (T)f;
+    }
+    T get_cutoff_distance() const { return r_min; }
+    T get_repulsion_distance() const { return r_min; }
+    bl_potential desc() const { return bl_potential{BL_POTENTIAL_MORSE, {r_min, beta, strength}}; }
+
+private:
+    T r_min, beta, strength;
+};
+
+template <class T>
+class SpringPotential {
+public:
+    SpringPotential(T min_dist_, T strength_) : min_dist(min_dist_), strength(strength_) {}
+    T operator()(T dist) const { return (min_dist - dist) * strength; }
+    T get_cutoff_distance() const { return min_dist; }
+    T get_repulsion_distance() const { return min_dist; }
+    bl_potential desc() const { return bl_potential{BL_POTENTIAL_SPRING, {min_dist, strength, 0}}; }
+
+private:
+    T min_dist, strength;
+};
+
+// Particle-particle forces (src/fsi/ParticleParticleInteraction.h:17-40).
+template <class T>
+struct PPInteractionForce {
+    virtual ~PPInteractionForce() {}
+    T operator()(T dist) const { return eval(dist); }
+    virtual T eval(T dist) const = 0;
+    virtual T get_cutoff_distance() const = 0;
+    virtual bl_potential desc() const = 0;
+};
+
+template <class T, class Interaction>
+class PPPotentialForce : public PPInteractionForce<T> {
+public:
+    explicit PPPotentialForce(const Interaction& potential) : potential_(potential) {}
+    T eval(T dist) const override { return potential_(dist); }
+    T get_cutoff_distance() const override { return potential_.get_cutoff_distance(); }
+    bl_potential desc() const override { return potential_.desc(); }
+
+private:
+    Interaction potential_;
+};
+
+// Force decorators (src/fsi/ForceDecorator.h:12-27): applied to every
+// particle before its membrane forces.  The device runs the wall
+// interaction; a decorator without a device form is refused when added.
+template <class T>
+struct ForceDecorator {
+    virtual ~ForceDecorator() {}
+    virtual bool device_form(bl_boundary& b, bl_potential& u) const
+    {
+        (void)b;
+        (void)u;
+        return false;
+    }
+};
+
+// WallInteraction(boundary, interaction) (src/fsi/ForceDecorator.hh:18-35):
+// force -= F(d) n on vertices within the cut-off of the boundary; the
+// boundary must outlive the decorator.
+template <class T, class Interaction>
+class WallInteraction : public ForceDecorator<T> {
+public:
+    WallInteraction(const Boundary<T>& boundary, const Interaction& interaction)
+        : boundary_(&boundary), interaction_(interaction)
+    {
+    }
+    bool device_form(bl_boundary& b, bl_potential& u) const override
+    {
+        b = boundary_->desc();
+        u = interaction_.desc();
+        return true;
+    }
+
+private:
+    const Boundary<T>* boundary_;
+    Interaction interaction_;
+};
+
 // GuoRigidWallBoundary(boundary, lattice) (src/fsi/GuoBoundaryCondition.h:
 // 19-52): the wall nodes are found when it is constructed (the lattice is
 // created then); insert() makes the extrapolation run after every
@@ -875,8 +972,51 @@ public:
         }
         blCheck(bl_ibm_init(g_), "init");
         if (boundary_) set_boundary(boundary_);
+        for (const ForceDecorator<T>* d : decorators_) push_decorator(d);
+        for (const PPInteractionForce<T>* f : pp_forces_) push_pp_force(f);
     }
     bl_ibm handle() const { return g_; }
+
+    // add_force_decorator / add_pp_force (src/fsi/ImmersedBoundaryDynamics.h:
+    // 78-79): applied in every step's force computation, in the order added;
+    // the objects must outlive this one
+    void add_force_decorator(ForceDecorator<T>* d)
+    {
+        decorators_.push_back(d);
+        if (g_) push_decorator(d);
+    }
+    void add_pp_force(PPInteractionForce<T>* f)
+    {
+        pp_forces_.push_back(f);
+        if (g_) push_pp_force(f);
+    }
+    // compute_collision_forces(interaction, boundary)
+    // (src/fsi/ImmersedBoundaryDynamics_forces.hh:505-521): vertex forces :=
+    // the collision handler's forces alone (read them with get_forces)
+    template <class InteractionFunctional>
+    void compute_collision_forces(const InteractionFunctional& interaction, const Boundary<T>* boundary = 0)
+    {
+        if (!g_) throw std::runtime_error("compute_collision_forces: call init() first");
+        lattice_.flushPendingBoundaryStep();
+        const bl_potential u = interaction.desc();
+        if (boundary) {
+            const bl_boundary b = boundary->desc();
+            blCheck(bl_ibm_compute_collision_forces(g_, &u, &b), "compute_collision_forces");
+        } else {
+            blCheck(bl_ibm_compute_collision_forces(g_, &u, nullptr), "compute_collision_forces");
+        }
+    }
+    // current vertex forces of particle i into a host copy
+    bool get_forces(plint i, std::vector<Array<T, 3>>& f)
+    {
+        if (i < 0 || i >= (plint)parts_.size()) return false;
+        lattice_.flushPendingBoundaryStep();
+        std::vector<double> buf(3 * parts_[i]->count_nodes());
+        blCheck(bl_particle_get(g_, (int)i, 2, buf.data()), "get_forces");
+        f.resize(parts_[i]->count_nodes());
+        for (size_t k = 0; k < f.size(); ++k) f[k] = Array<T, 3>(buf[3 * k], buf[3 * k + 1], buf[3 * k + 2]);
+        return true;
+    }
 
     // set_boundary (src/fsi/ImmersedBoundaryDynamics.h:93): near-wall kernel
     // blending and vertex reflection; the boundary must outlive this object
@@ -957,7 +1097,25 @@ public:
 private:
     MultiBlockLattice3D<T, Descriptor>& lattice_;
     int kernel_;
+    void push_decorator(const ForceDecorator<T>* d)
+    {
+        bl_boundary b;
+        bl_potential u;
+        if (!d->device_form(b, u))
+            throw std::runtime_error("add_force_decorator: only wall-interaction decorators run on the device");
+        lattice_.flushPendingBoundaryStep();
+        blCheck(bl_ibm_add_wall_interaction(g_, &b, &u), "add_force_decorator");
+    }
+    void push_pp_force(const PPInteractionForce<T>* f)
+    {
+        const bl_potential u = f->desc();
+        lattice_.flushPendingBoundaryStep();
+        blCheck(bl_ibm_add_pp_force(g_, &u), "add_pp_force");
+    }
+
     std::vector<const ParticleBase3D<T>*> parts_;
+    std::vector<const ForceDecorator<T>*> decorators_;
+    std::vector<const PPInteractionForce<T>*> pp_forces_;
     const Boundary<T>* boundary_ = nullptr;
     bl_ibm g_ = nullptr;
 };

@@ -164,6 +164,41 @@ void or_spread_near(int kernel, int nx, int ny, int nz, const int periodic[3], d
                     const double *pos, const double *fvert, const or_boundary *b);
 void or_move_reflect(long nv, double *pos, const double *vel, const or_boundary *b);
 
+/* ---- interaction forces ------------------------------------------- */
+#define OR_POTENTIAL_MORSE 0  /* p = {r_min, beta, strength} */
+#define OR_POTENTIAL_SPRING 1 /* p = {min_dist, strength} */
+typedef struct {
+    int kind;
+    double p[3];
+} or_potential;
+
+double or_potential_force(const or_potential *u, double dist);
+double or_potential_cutoff(const or_potential *u);
+
+/* Wall interaction decorator: force -= F(d) n for every vertex at
+ * distance d <= cutoff from the boundary. */
+void or_wall_interaction(const or_boundary *b, const or_potential *u, long nv, const double *pos, double *force);
+
+/* Collision-handler grid over an nx*ny*nz lattice box: bounding box
+ * [-0.5, n - 0.5 - eps], cells of size ~1 (num per axis) plus a one-cell
+ * envelope, nodes taken if strictly inside the box enlarged by the
+ * cut-off. */
+typedef struct {
+    double bb_lo[3], bb_hi[3], cs[3], x0g[3], lo[3], hi[3], L[3], Linv[3];
+    long num[3];
+} or_collision_grid_t;
+void or_collision_grid(int nx, int ny, int nz, double cutoff, or_collision_grid_t *g);
+/* cell index k + (num_z+2)*(j + (num_y+2)*i) of a node, or -1 if not taken */
+long or_collision_cell(const or_collision_grid_t *g, const double p[3]);
+int or_boundary_intersects_rect(const or_boundary *b, const double r[6], double margin);
+
+/* Particle-particle interaction forces of nv nodes (obj: particle of each
+ * node; nodes of one particle do not interact), ADDED to force, in the
+ * reference's pair order; wall (may be NULL) adds the handler's wall
+ * collision forces afterwards.  Returns the number of interacting pairs. */
+long or_collision_forces(int nx, int ny, int nz, const int periodic[3], const or_potential *u, long nv,
+                         const double *pos, const int *obj, const or_boundary *wall, double *force);
+
 #ifdef __cplusplus
 }
 #endif

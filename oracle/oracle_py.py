@@ -45,6 +45,11 @@ _SIGS = {
     "or_interpolate_near": (None, [_I, _I, _I, _I, _P, _P, _L, _P, _P, _P]),
     "or_spread_near": (None, [_I, _I, _I, _I, _P, _P, _L, _P, _P, _P]),
     "or_move_reflect": (None, [_L, _P, _P, _P]),
+    "or_potential_force": (_D, [_P, _D]),
+    "or_potential_cutoff": (_D, [_P]),
+    "or_wall_interaction": (None, [_P, _P, _L, _P, _P]),
+    "or_collision_forces": (_L, [_I, _I, _I, _P, _P, _L, _P, _P, _P, _P]),
+    "or_boundary_intersects_rect": (_I, [_P, _P, _D]),
 }
 
 
@@ -242,6 +247,43 @@ def spread_near(kernel, shape, periodic, pos, fvert, boundary, base=None):
     return field
 
 
+class Potential(ctypes.Structure):
+    """Interaction potential: kind 0 Morse (r_min, beta, strength), kind 1
+    spring (min_dist, strength)."""
+    _fields_ = [("kind", ctypes.c_int), ("p", ctypes.c_double * 3)]
+
+    @classmethod
+    def morse(cls, r_min, beta, strength):
+        return cls(0, (ctypes.c_double * 3)(r_min, beta, strength))
+
+    @classmethod
+    def spring(cls, min_dist, strength):
+        return cls(1, (ctypes.c_double * 3)(min_dist, strength, 0.0))
+
+
+def potential_force(u, dist):
+    return lib.or_potential_force(ctypes.byref(u), float(dist))
+
+
+def wall_interaction(boundary, u, pos, force=None):
+    """Wall interaction decorator: returns force - F(d) n (per vertex)."""
+    pos = _c(pos, np.float64)
+    out = np.zeros_like(pos) if force is None else _c(force, np.float64).copy()
+    lib.or_wall_interaction(ctypes.byref(boundary), ctypes.byref(u), len(pos), _p(pos), _p(out))
+    return out
+
+
+def collision_forces(shape, periodic, u, pos, obj, wall=None, force=None):
+    """Particle-particle interaction forces (collision handler over the
+    lattice box), added to `force`; returns (force, interacting pairs)."""
+    pos = _c(pos, np.float64)
+    out = np.zeros_like(pos) if force is None else _c(force, np.float64).copy()
+    w = None if wall is None else ctypes.byref(wall)
+    n = lib.or_collision_forces(shape[0], shape[1], shape[2], _p(_c(periodic, np.int32)), ctypes.byref(u),
+                                len(pos), _p(pos), _p(_c(obj, np.int32)), w, _p(out))
+    return out, n
+
+
 def move_reflect(pos, vel, boundary):
     pos = _c(pos, np.float64).copy()
     lib.or_move_reflect(len(pos), _p(pos), _p(_c(vel, np.float64)), ctypes.byref(boundary))
@@ -373,30 +415,55 @@ class Particle:
         self.com = com
 
 
+def vertex_forces(particles, shape, periodic, wall_interactions=(), pp_forces=()):
+    """compute_forces of the immersed boundary: per particle the wall
+    decorators (from zero) then the membrane forces, then the
+    particle-particle forces over all vertices (particle index as the
+    owner).  Returns one (nv, 3) array per particle."""
+    forces = []
+    for p in particles:
+        F = p.forces()
+        if wall_interactions:
+            fw = np.zeros_like(p.pos)
+            for b, u in wall_interactions:
+                fw = wall_interaction(b, u, p.pos, fw)
+            F = fw + F
+        forces.append(F)
+    if pp_forces and particles:
+        allpos = np.concatenate([p.pos for p in particles])
+        obj = np.concatenate([np.full(len(p.pos), i, np.int32) for i, p in enumerate(particles)])
+        allF = np.concatenate(forces)
+        for u in pp_forces:
+            allF, _ = collision_forces(shape, periodic, u, allpos, obj, None, allF)
+        forces = np.split(allF, np.cumsum([len(p.pos) for p in particles])[:-1])
+    return forces
+
+
 def coupled_step(f, force, phase, mask, periodic, omega, omega_in, body, particles, kernel,
-                 wall_axis=1, wall_u=None, boundary=None, guo=None):
+                 wall_axis=1, wall_u=None, boundary=None, guo=None, wall_interactions=(), pp_forces=()):
     """One time step of the progs' loop, composed from the oracle's pieces:
     computeVelocity (u = j/rho + F/2 with the last collision's force), force
-    reset to the body force, interpolation, membrane forces, spreading,
-    vertex advection (+ COM remap), then collide-and-stream with the new
-    force.  With a `boundary` (set_boundary), near-wall vertices blend with
-    the top hat and are spread after all others, and advected vertices
-    reflect off it; `guo` = (wall nodes, velocities) of a Guo curved wall run
-    after the collide-and-stream.  Returns (f, force, per-particle
-    velocities)."""
+    reset to the body force, interpolation, vertex forces (wall decorators,
+    membrane, particle-particle), spreading, vertex advection (+ COM
+    remap), then collide-and-stream with the new force.  With a `boundary`
+    (set_boundary), near-wall vertices blend with the top hat and are spread
+    after all others, and advected vertices reflect off it; `guo` = (wall
+    nodes, velocities) of a Guo curved wall run after the
+    collide-and-stream.  Returns (f, force, per-particle velocities)."""
     shape = f.shape[:3]
     _, u = lbm_moments(f, force, mask, wall_axis, wall_u)
     new_force = np.broadcast_to(np.asarray(body, float), tuple(shape) + (3,)).copy()
+    pforces = vertex_forces(particles, shape, periodic, wall_interactions, pp_forces)
     if boundary is None:
         vels = []
         for p in particles:
             vels.append(interpolate(kernel, u, periodic, p.pos))
-        for p in particles:
-            new_force = spread(kernel, shape, periodic, p.pos, p.forces(), base=new_force)
+        for p, F in zip(particles, pforces):
+            new_force = spread(kernel, shape, periodic, p.pos, F, base=new_force)
     else:
         allpos = np.concatenate([p.pos for p in particles])
         allv = interpolate_near(kernel, u, periodic, allpos, boundary)
-        forces = np.concatenate([p.forces() for p in particles])
+        forces = np.concatenate(pforces)
         new_force = spread_near(kernel, shape, periodic, allpos, forces, boundary, base=new_force)
         cuts = np.cumsum([len(p.pos) for p in particles])[:-1]
         vels = np.split(allv, cuts)

@@ -90,3 +90,57 @@ def test_rbc_in_pipe_prog_matches_python_path(device, tmp_path):
     ib.close()
     W.close()
     L.close()
+
+
+def test_rbc_in_pipe_prog_with_interactions(device, tmp_path):
+    """The rbc_in_pipe prog with its input.xml interactions (Morse wall
+    decorator on the pipe, Morse particle-particle force) and a cell close
+    enough to the wall for the decorator to act, against the Python path
+    with the same decorator and force."""
+    from biblbm_amd import meshes
+    from biblbm_amd.capi import call
+    from biblbm_amd.ibm import ImmersedBoundary
+    from biblbm_amd.lattice import Lattice
+    from biblbm_amd.potentials import MorsePotential
+    from biblbm_amd.walls import Boundary, GuoWall, outside_mask
+    R, length, steps = 6.0, 16.0, 10
+    v, t = meshes.biconcave(2, 4.2)
+    msh = tmp_path / "rbc.msh"
+    meshes.write_gmsh(str(msh), v, t)
+    exe = os.path.join(ROOT, "biblbm_amd", "examples", "rbc_in_pipe")
+    out = str(tmp_path / "run")
+    r = subprocess.run([exe, str(msh), str(R), str(length), str(steps), out, "1", "1", "f64", "1"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    pos = np.loadtxt(out + "_final.txt")
+
+    c = R + 2
+    shape = (17, 17, 17)
+    b = Boundary.pipe(c, c, R, length)
+    L = Lattice(*shape, periodic=(1, 0, 0), omega=1.0)
+    W = GuoWall(L, b)
+    W.insert()
+    L.set_mask(outside_mask(b, shape, 1.0))
+    nu = 0.5 / 3
+    L.set_body_force([4 * nu * 0.02 / (R * R), 0, 0])
+    L.init_equilibrium(np.ones(shape), np.zeros(shape + (3,)))
+    call("bl_lattice_initialize", L.handle)
+    vol = np.sum(np.einsum("ij,ij->i", v[t[:, 0]], np.cross(v[t[:, 1]], v[t[:, 2]]))) / 6
+    ib = ImmersedBoundary(L, "peskin")
+    sid = ib.add_shape(v, t)
+    start = v + [length / 2, c, c]
+    ib.add_particle(sid, 2, [0.0015, 1.0, 0.5, 0.005, 0.5, vol, 0.0005, 0.0], start)
+    ib.init()
+    ib.set_boundary(b)
+    ib.add_wall_interaction(b, MorsePotential(2.0, 2.0, 1e-4))
+    ib.add_pp_force(MorsePotential(1.0, 1.0, 1e-3))
+    ib.coupled_steps(steps, use_graph=False)
+    ref = ib.positions()
+    assert np.linalg.norm(pos - ref) <= 1e-8 * np.linalg.norm(ref - start)
+    # the decorator acts on the rim
+    dist = R - np.hypot(start[:, 1] - c, start[:, 2] - c)
+    assert (dist < 2.0).sum() > 10
+    ib.close()
+    W.close()
+    L.close()
