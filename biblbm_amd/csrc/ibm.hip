@@ -625,13 +625,14 @@ __global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long 
 // list [links | triangles | pairs] in order, then a fixed butterfly adds the
 // lanes, so the result does not depend on scheduling.  No atomics.
 // Lane 0 starts from the wall-interaction decorators' forces, which the
-// reference applies before the membrane forces
-// (src/fsi/ImmersedBoundaryDynamics_forces.hh:293-296).
+// reference applies before the membrane forces, and the sum ends with the
+// particle-particle forces (`fpp`), which it adds after them
+// (src/fsi/ImmersedBoundaryDynamics_forces.hh:293-301).
 constexpr int kGatherLanes = 4;
 
 __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, long ec_pair, long ec_link,
                                                        const double* __restrict__ ec, double* __restrict__ force,
-                                                       DecoSet deco)
+                                                       DecoSet deco, const double* __restrict__ fpp)
 {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long v = gid / kGatherLanes;
@@ -691,6 +692,11 @@ __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, lon
         c2 += __shfl_xor(c2, o);
     }
     if (lane == 0 && v < NV) {
+        if (fpp) {
+            c0 += fpp[3 * v];
+            c1 += fpp[3 * v + 1];
+            c2 += fpp[3 * v + 2];
+        }
         force[3 * v] = c0;
         force[3 * v + 1] = c1;
         force[3 * v + 2] = c2;
@@ -1683,18 +1689,24 @@ int do_interpolate(bl_ibm g)
     return launch_interp(g, false, g->lat->stream);
 }
 
+// Vertex forces on stream s: particle-particle forces into their own
+// buffer, per-element membrane forces, then the per-vertex sum.
 int do_forces(bl_ibm g, hipStream_t s)
 {
+    int rc;
     const ForceView f = force_view(g);
     const long ne = g->NT + g->NP + g->NL;
+    const bool pp = !g->pp.empty() && g->NV;
+    if (pp && (rc = interactions_launch(g, s))) return rc;
     if (g->NV) {
         hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair,
                            g->ec_link, g->ec);
         hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
-                           g->ec_link, (const double*)g->ec, g->force, deco_set(g));
+                           g->ec_link, (const double*)g->ec, g->force, deco_set(g),
+                           pp ? (const double*)g->pp_force : nullptr);
     }
     BL_CHECK_LAUNCH();
-    return interactions_launch(g, s);
+    return BL_OK;
 }
 
 int do_spread(bl_ibm g)
@@ -1776,7 +1788,8 @@ int step_serial(bl_ibm g)
 
 // ... or forked over the lattice stream M and the boundary's streams A and
 // B, so that independent stages overlap:
-//   A: membrane forces (read the positions before the move)
+//   A: particle-particle forces (when there are any) and membrane forces
+//      (read the positions before the move)
 //   M: binning, then velocity staging (the last collision's force field
 //      and the new tile flags)
 //   A: after the binning: spreading into the second force buffer (it needs

@@ -7,6 +7,10 @@
 #include "lattice.hpp"
 #include "mesh.hpp"
 
+namespace bl {
+struct PPRec;
+}
+
 struct bl_ibm_s {
     bl_lattice lat = nullptr;
     int kernel = BL_KERNEL_ROMA;
@@ -93,11 +97,11 @@ struct bl_ibm_s {
     std::vector<WallDeco> wall_deco;  // wall-interaction decorators, in order
     std::vector<bl_potential> pp;     // particle-particle potentials, in order
     int* pp_head = nullptr;   // per collision cell: the last vertex pushed onto its list, -1 if empty
-    int2* pp_info = nullptr;  // per collision cell: (first slot, vertex count), (0, 0) if empty
+    int* pp_first = nullptr;  // per collision cell: its smallest vertex id, -1 if empty
     int* pp_next = nullptr;   // per vertex: next vertex of its cell's list
     int* pp_cell = nullptr;   // per vertex: collision cell, -1 if the vertex does not take part
-    int* pp_slot = nullptr;   // per slot: vertex, grouped by cell, ascending within a cell
-    int* pp_ctr = nullptr;    // slots handed out
+    bl::PPRec* pp_rec = nullptr;  // per vertex: position, owner, successor in its cell (ascending id)
+    double* pp_force = nullptr;   // per vertex: the step's particle-particle forces (3)
     long pp_ncell = 0;
 
     // ---- stencil / tiles ----
@@ -162,5 +166,5 @@ struct DecoSet;
 DecoSet deco_set(bl_ibm g);                       // the wall-interaction decorators
 int interactions_alloc(bl_ibm g);                 // collision grid for the current vertices
 void interactions_free(bl_ibm g);
-int interactions_launch(bl_ibm g, hipStream_t s);  // particle-particle forces, added to the vertex forces
+int interactions_launch(bl_ibm g, hipStream_t s);  // particle-particle forces into pp_force
 }  // namespace bl

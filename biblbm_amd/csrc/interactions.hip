@@ -10,16 +10,20 @@
 // node and -df to its second.  Here every vertex gathers its own share of
 // exactly those pairs, so no atomics touch the forces:
 //   k_pp_bin    cell of each vertex; vertices pushed onto per-cell lists
-//   k_pp_pack   one vertex per occupied cell copies its list into a run of
-//               slots, sorted by vertex id (the reference's insertion order)
-//   k_pp_force  per vertex: the cells it pairs with under the handler's
-//               rules, each with the role (first / second node) the vertex
-//               has in those pairs; the pair force is evaluated with the
-//               reference's operand order for that role
+//   k_pp_link   every vertex of a cell finds its successor in ascending id
+//               order (the reference's insertion order) and writes its record
+//               (position, owner, successor); the cell's smallest id becomes
+//               the cell's first vertex
+//   k_pp_force  per vertex, four lanes: the cells it pairs with under the
+//               handler's rules, each with the role (first / second node) the
+//               vertex has in those pairs; the pair force is evaluated with
+//               the reference's operand order for that role
 //   k_pp_clear  empties the touched cells for the next evaluation
-// A vertex's sum runs over its partner cells in a fixed rule order and over
-// each cell's slots in vertex order, so results do not depend on
-// scheduling.
+// A vertex's sum runs over its partner cells in a fixed rule order (dealt
+// round-robin to four lanes, added by a fixed butterfly) and over each
+// cell's vertices in id order, so results do not depend on scheduling.  No
+// counter is shared between cells: same-address atomics serialise across
+// the chip's dies.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -44,30 +48,32 @@ __global__ void __launch_bounds__(256) k_pp_bin(long NV, const double* __restric
     if (c >= 0) next[v] = atomicExch(head + c, (int)v);
 }
 
-__global__ void __launch_bounds__(256) k_pp_pack(long NV, const int* __restrict__ cell,
+// Every listed vertex: its successor in its cell (the smallest larger id)
+// and, for the smallest id, the cell's first vertex.
+__global__ void __launch_bounds__(256) k_pp_link(long NV, const int* __restrict__ cell,
                                                  const int* __restrict__ head, const int* __restrict__ next,
-                                                 int* __restrict__ ctr, int* __restrict__ slot,
-                                                 int2* __restrict__ info)
+                                                 const double* __restrict__ pos, const int* __restrict__ vpart,
+                                                 PPRec* __restrict__ rec, int* __restrict__ first)
 {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= NV) return;
-    const int c = cell[v];
-    if (c < 0 || head[c] != (int)v) return;  // one vertex per occupied cell packs it
-    int n = 0;
-    for (int w = (int)v; w >= 0; w = next[w]) ++n;
-    const int base = atomicAdd(ctr, n);
-    int k = 0;
-    for (int w = (int)v; w >= 0; w = next[w]) {
-        // insertion into the sorted run slot[base .. base+k)
-        int j = k;
-        while (j > 0 && slot[base + j - 1] > w) {
-            slot[base + j] = slot[base + j - 1];
-            --j;
-        }
-        slot[base + j] = w;
-        ++k;
+    const long vl = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vl >= NV) return;
+    const int c = cell[vl];
+    if (c < 0) return;
+    const int v = (int)vl;
+    int succ = -1;
+    bool smallest = true;
+    for (int w = head[c]; w >= 0; w = next[w]) {
+        if (w > v && (succ < 0 || w < succ)) succ = w;
+        if (w < v) smallest = false;
     }
-    info[c] = make_int2(base, n);
+    PPRec r;
+    r.x = pos[3 * vl];
+    r.y = pos[3 * vl + 1];
+    r.z = pos[3 * vl + 2];
+    r.obj = vpart[v];
+    r.succ = succ;
+    rec[v] = r;
+    if (smallest) first[c] = v;
 }
 
 // minimal-image difference x - y along axis d (src/fsi/Periodicity.h:27-35,91-99)
@@ -79,102 +85,141 @@ __device__ inline double mm_diff(const CollGrid& g, int d, double x, double y)
     return v - floor(v * g.Linv[d] + 0.5) * g.L[d];
 }
 
-struct PPAcc {
-    double f[3];
-};
+// The partner cells of a vertex in cell (i, j, k), as 22 entries in the
+// handler's order: the cell itself and its 7 forward neighbours when it is a
+// bulk cell (src/fsi/CollisionForces.h:89-117), the 7 cells it is a forward
+// neighbour of, then the face / edge / corner pairs across periodic axes
+// (:119-160: first cell at the low bulk index of the axes involved, second
+// at the high one).  Returns the partner cell or -1, and the role: 0 the
+// vertex is the first node of every pair, 1 the second, 2 (its own cell)
+// first when it precedes the other vertex.
+constexpr int kPartners = 22;
 
-// The pairs of vertex v with the vertices of cell pc.  role 0: v is the
-// first node of every pair; 1: the second; 2 (its own cell): first if it
-// precedes the other vertex in insertion order, else second.
-__device__ inline void pp_cell(const CollGrid& g, const bl_potential& u, const double* __restrict__ pos,
-                               const int* __restrict__ vpart, const int2* __restrict__ info,
-                               const int* __restrict__ slot, int v, int ov, const double pv[3], int pc, int role,
-                               PPAcc& a)
+__device__ inline int partner(const CollGrid& g, int t, int i, int j, int k, int& role)
+{
+    const int NZ = g.num[2] + 2, NY = g.num[1] + 2;
+    const int X1 = g.num[0], Y1 = g.num[1], Z1 = g.num[2];
+    auto bulk = [&](int a, int b, int e) { return a >= 1 && a <= X1 && b >= 1 && b <= Y1 && e >= 1 && e <= Z1; };
+    auto at = [&](int a, int b, int e) { return e + NZ * (b + NY * a); };
+    // forward offsets, bits (dx << 2) | (dy << 1) | dz, in the handler's order
+    // (0,1,0) (0,1,1) (0,0,1) (1,0,0) (1,1,0) (1,1,1) (1,0,1)
+    constexpr unsigned kFwd = 2u | (3u << 3) | (1u << 6) | (4u << 9) | (6u << 12) | (7u << 15) | (5u << 18);
+    if (t == 0) {
+        role = 2;
+        return bulk(i, j, k) ? at(i, j, k) : -1;
+    }
+    if (t <= 14) {
+        const unsigned o = (kFwd >> (3 * ((t - 1) % 7))) & 7u;
+        const int dx = (o >> 2) & 1, dy = (o >> 1) & 1, dz = o & 1;
+        if (t <= 7) {
+            role = 0;
+            return bulk(i, j, k) ? at(i + dx, j + dy, k + dz) : -1;
+        }
+        role = 1;
+        return bulk(i - dx, j - dy, k - dz) ? at(i - dx, j - dy, k - dz) : -1;
+    }
+    // periodic rules over the axis sets x, y, z, xy, xz, yz, xyz
+    constexpr unsigned char kAxes[7] = {1, 2, 4, 3, 5, 6, 7};
+    const unsigned m = kAxes[t - 15];
+    const int ix[3] = {i, j, k}, hi[3] = {X1, Y1, Z1};
+    bool low = true, high = true;
+    for (int d = 0; d < 3; ++d) {
+        const bool in_bulk = ix[d] >= 1 && ix[d] <= hi[d];
+        if (m & (1u << d)) {
+            if (!g.per[d]) return -1;
+            low = low && ix[d] == 1;
+            high = high && ix[d] == hi[d];
+        } else if (!in_bulk) {
+            return -1;
+        }
+    }
+    int q[3] = {i, j, k};
+    if (low) {
+        role = 0;
+        for (int d = 0; d < 3; ++d)
+            if (m & (1u << d)) q[d] = hi[d];
+        return at(q[0], q[1], q[2]);
+    }
+    if (high) {
+        role = 1;
+        for (int d = 0; d < 3; ++d)
+            if (m & (1u << d)) q[d] = 1;
+        return at(q[0], q[1], q[2]);
+    }
+    return -1;
+}
+
+// Four lanes per vertex: lane l takes partner entries l, l+4, ...; a fixed
+// butterfly adds the lanes' sums to the vertex force.
+constexpr int kPPLanes = 4;
+
+// out[v] = in[v] (0 without `in`) + the vertex's pair forces, for every
+// vertex (those outside the handler get in[v]).
+__global__ void __launch_bounds__(256) k_pp_force(long NV, CollGrid g, bl_potential u, int has_wall, bl_boundary wall,
+                                                  const int* __restrict__ cell, const int* __restrict__ first,
+                                                  const PPRec* __restrict__ rec, const double* in,
+                                                  double* out)
 {
 #pragma clang fp contract(off)
-    const int2 in = info[pc];
-    for (int s = in.x; s < in.x + in.y; ++s) {
-        const int w = slot[s];
-        if (vpart[w] == ov) continue;
-        const bool first = role == 0 || (role == 2 && v < w);
-        const double pw[3] = {pos[3 * (long)w], pos[3 * (long)w + 1], pos[3 * (long)w + 2]};
-        double dr[3];
-        for (int d = 0; d < 3; ++d) dr[d] = first ? mm_diff(g, d, pw[d], pv[d]) : mm_diff(g, d, pv[d], pw[d]);
-        const double d2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-        if (d2 <= g.cut2) {
-            const double dist = sqrt(d2);
-            const double sc = -pot_force(u, dist) / dist;
-            for (int d = 0; d < 3; ++d) {
-                const double df = sc * dr[d];
-                a.f[d] = first ? a.f[d] + df : a.f[d] - df;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long vl = gid / kPPLanes;
+    const int lane = (int)(gid % kPPLanes);
+    const int c = vl < NV ? cell[vl] : -1;
+    double f[3] = {0, 0, 0};
+    if (in && lane == 0 && vl < NV) {
+        f[0] = in[3 * vl];
+        f[1] = in[3 * vl + 1];
+        f[2] = in[3 * vl + 2];
+    }
+    double pv[3] = {0, 0, 0};
+    const int NZ = g.num[2] + 2, NY = g.num[1] + 2;
+    int i = 0, j = 0, k = 0;
+    if (c >= 0) {
+        const int v = (int)vl;
+        i = c / (NY * NZ);
+        j = (c / NZ) % NY;
+        k = c % NZ;
+        const PPRec me = rec[v];
+        pv[0] = me.x;
+        pv[1] = me.y;
+        pv[2] = me.z;
+#pragma unroll 1
+        for (int t = lane; t < kPartners; t += kPPLanes) {
+            int role = 0;
+            const int pc = partner(g, t, i, j, k, role);
+            if (pc < 0) continue;
+            for (int w = first[pc]; w >= 0;) {
+                const PPRec r = rec[w];
+                if (r.obj != me.obj) {
+                    const bool fst = role == 0 || (role == 2 && v < w);
+                    const double pw[3] = {r.x, r.y, r.z};
+                    double dr[3];
+                    for (int d = 0; d < 3; ++d)
+                        dr[d] = fst ? mm_diff(g, d, pw[d], pv[d]) : mm_diff(g, d, pv[d], pw[d]);
+                    const double d2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                    if (d2 <= g.cut2) {
+                        const double dist = sqrt(d2);
+                        const double sc = -pot_force(u, dist) / dist;
+                        for (int d = 0; d < 3; ++d) {
+                            const double df = sc * dr[d];
+                            f[d] = fst ? f[d] + df : f[d] - df;
+                        }
+                    }
+                }
+                w = r.succ;
             }
         }
     }
-}
-
-__global__ void __launch_bounds__(256) k_pp_force(long NV, CollGrid g, bl_potential u, int has_wall, bl_boundary wall,
-                                                  const double* __restrict__ pos, const int* __restrict__ vpart,
-                                                  const int* __restrict__ cell, const int2* __restrict__ info,
-                                                  const int* __restrict__ slot, double* __restrict__ force)
-{
-#pragma clang fp contract(off)
-    const long vl = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (vl >= NV) return;
-    const int c = cell[vl];
-    if (c < 0) return;
-    const int v = (int)vl;
-    const int NZ = g.num[2] + 2, NY = g.num[1] + 2;
-    const int i = c / (NY * NZ), j = (c / NZ) % NY, k = c % NZ;
-    const int X1 = g.num[0], Y1 = g.num[1], Z1 = g.num[2];
-    auto at = [&](int a, int b, int e) { return e + NZ * (b + NY * a); };
-    auto bulk = [&](int a, int b, int e) { return a >= 1 && a <= X1 && b >= 1 && b <= Y1 && e >= 1 && e <= Z1; };
-    const double pv[3] = {pos[3 * vl], pos[3 * vl + 1], pos[3 * vl + 2]};
-    const int ov = vpart[v];
-    PPAcc a{{force[3 * vl], force[3 * vl + 1], force[3 * vl + 2]}};
-    // forward neighbours of the bulk loop (src/fsi/CollisionForces.h:96-104)
-    constexpr int off[7][3] = {{0, 1, 0}, {0, 1, 1}, {0, 0, 1}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}};
-    if (bulk(i, j, k)) {
-        pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, c, 2, a);
-        for (int o = 0; o < 7; ++o)
-            pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i + off[o][0], j + off[o][1], k + off[o][2]), 0, a);
+#pragma unroll
+    for (int o = 1; o < kPPLanes; o <<= 1) {
+        f[0] += __shfl_xor(f[0], o);
+        f[1] += __shfl_xor(f[1], o);
+        f[2] += __shfl_xor(f[2], o);
     }
-    for (int o = 0; o < 7; ++o) {
-        const int a0 = i - off[o][0], b0 = j - off[o][1], e0 = k - off[o][2];
-        if (bulk(a0, b0, e0)) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(a0, b0, e0), 1, a);
-    }
-    // cell pairs across periodic axes (:119-160): first cell at the low
-    // bulk index of the axes involved, second at the high one
-    const bool bi = i >= 1 && i <= X1, bj = j >= 1 && j <= Y1, bk = k >= 1 && k <= Z1;
-    if (g.per[0] && bj && bk) {
-        if (i == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(X1, j, k), 0, a);
-        if (i == X1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(1, j, k), 1, a);
-    }
-    if (g.per[1] && bi && bk) {
-        if (j == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i, Y1, k), 0, a);
-        if (j == Y1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i, 1, k), 1, a);
-    }
-    if (g.per[2] && bi && bj) {
-        if (k == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i, j, Z1), 0, a);
-        if (k == Z1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i, j, 1), 1, a);
-    }
-    if (g.per[0] && g.per[1] && bk) {
-        if (i == 1 && j == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(X1, Y1, k), 0, a);
-        if (i == X1 && j == Y1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(1, 1, k), 1, a);
-    }
-    if (g.per[0] && g.per[2] && bj) {
-        if (i == 1 && k == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(X1, j, Z1), 0, a);
-        if (i == X1 && k == Z1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(1, j, 1), 1, a);
-    }
-    if (g.per[1] && g.per[2] && bi) {
-        if (j == 1 && k == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i, Y1, Z1), 0, a);
-        if (j == Y1 && k == Z1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(i, 1, 1), 1, a);
-    }
-    if (g.per[0] && g.per[1] && g.per[2]) {
-        if (i == 1 && j == 1 && k == 1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(X1, Y1, Z1), 0, a);
-        if (i == X1 && j == Y1 && k == Z1) pp_cell(g, u, pos, vpart, info, slot, v, ov, pv, at(1, 1, 1), 1, a);
-    }
+    if (vl >= NV || lane != 0) return;
     // the handler's wall forces (:167-187), after every pair
-    if (has_wall && bulk(i, j, k)) {
+    const int X1 = g.num[0], Y1 = g.num[1], Z1 = g.num[2];
+    if (c >= 0 && has_wall && i >= 1 && i <= X1 && j >= 1 && j <= Y1 && k >= 1 && k <= Z1) {
         const double x0 = g.bb_lo[0] + (double)(i - 1) * g.cs[0];
         const double y0 = g.bb_lo[1] + (double)(j - 1) * g.cs[1];
         const double z0 = g.bb_lo[2] + (double)(k - 1) * g.cs[2];
@@ -185,41 +230,34 @@ __global__ void __launch_bounds__(256) k_pp_force(long NV, CollGrid g, bl_potent
                 const double F = pot_force(u, d);
                 double n[3];
                 bnd_normal(wall, pv, n);
-                for (int q = 0; q < 3; ++q) a.f[q] -= F * n[q];
+                for (int q = 0; q < 3; ++q) f[q] -= F * n[q];
             }
         }
     }
-    force[3 * vl] = a.f[0];
-    force[3 * vl + 1] = a.f[1];
-    force[3 * vl + 2] = a.f[2];
+    out[3 * vl] = f[0];
+    out[3 * vl + 1] = f[1];
+    out[3 * vl + 2] = f[2];
 }
 
 __global__ void __launch_bounds__(256) k_pp_clear(long NV, const int* __restrict__ cell, int* __restrict__ head,
-                                                  int2* __restrict__ info, int* __restrict__ ctr)
+                                                  int* __restrict__ first)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= NV) return;
-    if (v == 0) *ctr = 0;
     const int c = cell[v];
     if (c >= 0) {
         head[c] = -1;
-        info[c] = make_int2(0, 0);
+        first[c] = -1;
     }
 }
 
-__global__ void k_pp_init(long n, int* __restrict__ head, int2* __restrict__ info)
+__global__ void k_pp_init(long n, int* __restrict__ head, int* __restrict__ first)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c < n) {
         head[c] = -1;
-        info[c] = make_int2(0, 0);
+        first[c] = -1;
     }
-}
-
-__global__ void k_zero_f64(long n, double* __restrict__ p)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 CollGrid grid_for(bl_ibm g, const bl_potential& u)
@@ -240,9 +278,9 @@ int check_potential(bl_ibm g, const bl_potential* u, const char* who, bool pp)
     return BL_OK;
 }
 
-// One collision-handler pass: pair forces (and the handler's wall forces)
-// added to the vertex forces.
-int pp_pass(bl_ibm g, const bl_potential& u, const bl_boundary* wall, hipStream_t s)
+// One collision-handler pass: out := in (0 without `in`) + pair forces (+
+// the handler's wall forces) of every vertex.
+int pp_pass(bl_ibm g, const bl_potential& u, const bl_boundary* wall, const double* in, double* out, hipStream_t s)
 {
     const long NV = g->NV;
     if (NV == 0) return BL_OK;
@@ -250,13 +288,13 @@ int pp_pass(bl_ibm g, const bl_potential& u, const bl_boundary* wall, hipStream_
     const bl_boundary w = wall ? *wall : bl_boundary{};
     hipLaunchKernelGGL(k_pp_bin, dim3(nblk(NV)), dim3(256), 0, s, NV, (const double*)g->pos, cg, g->pp_head,
                        g->pp_next, g->pp_cell);
-    hipLaunchKernelGGL(k_pp_pack, dim3(nblk(NV)), dim3(256), 0, s, NV, (const int*)g->pp_cell,
-                       (const int*)g->pp_head, (const int*)g->pp_next, g->pp_ctr, g->pp_slot, g->pp_info);
-    hipLaunchKernelGGL(k_pp_force, dim3(nblk(NV)), dim3(256), 0, s, NV, cg, u, wall ? 1 : 0, w,
-                       (const double*)g->pos, (const int*)g->v_part, (const int*)g->pp_cell,
-                       (const int2*)g->pp_info, (const int*)g->pp_slot, g->force);
+    hipLaunchKernelGGL(k_pp_link, dim3(nblk(NV)), dim3(256), 0, s, NV, (const int*)g->pp_cell,
+                       (const int*)g->pp_head, (const int*)g->pp_next, (const double*)g->pos,
+                       (const int*)g->v_part, g->pp_rec, g->pp_first);
+    hipLaunchKernelGGL(k_pp_force, dim3(nblk(kPPLanes * NV)), dim3(256), 0, s, NV, cg, u, wall ? 1 : 0, w,
+                       (const int*)g->pp_cell, (const int*)g->pp_first, (const PPRec*)g->pp_rec, in, out);
     hipLaunchKernelGGL(k_pp_clear, dim3(nblk(NV)), dim3(256), 0, s, NV, (const int*)g->pp_cell, g->pp_head,
-                       g->pp_info, g->pp_ctr);
+                       g->pp_first);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
@@ -283,42 +321,40 @@ int interactions_alloc(bl_ibm g)
     const long ncell = coll_ncell(grid_for(g, any));
     const size_t nv = (size_t)std::max<long>(g->NV, 1);
     if (g->pp_head && g->pp_ncell == ncell) return BL_OK;
-    void* old[] = {g->pp_head, g->pp_info, g->pp_next, g->pp_cell, g->pp_slot, g->pp_ctr};
-    for (void* p : old) hipFree(p);
-    g->pp_head = g->pp_next = g->pp_cell = g->pp_slot = g->pp_ctr = nullptr;
-    g->pp_info = nullptr;
+    interactions_free(g);
     hipError_t e = hipMalloc(&g->pp_head, sizeof(int) * ncell);
-    if (e == hipSuccess) e = hipMalloc(&g->pp_info, sizeof(int2) * ncell);
+    if (e == hipSuccess) e = hipMalloc(&g->pp_first, sizeof(int) * ncell);
     if (e == hipSuccess) e = hipMalloc(&g->pp_next, sizeof(int) * nv);
     if (e == hipSuccess) e = hipMalloc(&g->pp_cell, sizeof(int) * nv);
-    if (e == hipSuccess) e = hipMalloc(&g->pp_slot, sizeof(int) * nv);
-    if (e == hipSuccess) e = hipMalloc(&g->pp_ctr, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&g->pp_rec, sizeof(PPRec) * nv);
+    if (e == hipSuccess) e = hipMalloc(&g->pp_force, sizeof(double) * 3 * nv);
     if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("collision grid: ") + hipGetErrorString(e));
     g->pp_ncell = ncell;
     hipStream_t s = g->lat->stream;
-    hipLaunchKernelGGL(k_pp_init, dim3(nblk(ncell)), dim3(256), 0, s, ncell, g->pp_head, g->pp_info);
+    hipLaunchKernelGGL(k_pp_init, dim3(nblk(ncell)), dim3(256), 0, s, ncell, g->pp_head, g->pp_first);
     BL_CHECK_LAUNCH();
-    BL_HIP(hipMemsetAsync(g->pp_ctr, 0, sizeof(int), s));
     BL_HIP(hipStreamSynchronize(s));
     return BL_OK;
 }
 
 void interactions_free(bl_ibm g)
 {
-    void* p[] = {g->pp_head, g->pp_info, g->pp_next, g->pp_cell, g->pp_slot, g->pp_ctr};
-    for (void* q : p) hipFree(q);
-    g->pp_head = g->pp_next = g->pp_cell = g->pp_slot = g->pp_ctr = nullptr;
-    g->pp_info = nullptr;
+    void* p[] = {g->pp_head, g->pp_first, g->pp_next, g->pp_cell, g->pp_rec, g->pp_force};
+    for (void* q : p) (void)hipFree(q);
+    g->pp_head = g->pp_first = g->pp_next = g->pp_cell = nullptr;
+    g->pp_rec = nullptr;
+    g->pp_force = nullptr;
     g->pp_ncell = 0;
 }
 
-// The particle-particle forces of a step, after the membrane forces
-// (src/fsi/ImmersedBoundaryDynamics_forces.hh:299-301).
+// The particle-particle forces of a step into their own per-vertex buffer;
+// the membrane-force gather adds it last, as the reference adds them after
+// the membrane forces (src/fsi/ImmersedBoundaryDynamics_forces.hh:299-301).
 int interactions_launch(bl_ibm g, hipStream_t s)
 {
     int rc;
-    for (const bl_potential& u : g->pp)
-        if ((rc = pp_pass(g, u, nullptr, s))) return rc;
+    for (size_t k = 0; k < g->pp.size(); ++k)
+        if ((rc = pp_pass(g, g->pp[k], nullptr, k ? g->pp_force : nullptr, g->pp_force, s))) return rc;
     return BL_OK;
 }
 
@@ -378,10 +414,8 @@ int bl_ibm_compute_collision_forces(bl_ibm g, const bl_potential* u, const bl_bo
     if (b && b->kind != BL_BOUNDARY_PIPE && b->kind != BL_BOUNDARY_PLATES)
         return fail(BL_ERR_ARG, "bl_ibm_compute_collision_forces: unknown boundary kind");
     if ((rc = interactions_alloc(g))) return rc;
-    hipStream_t s = g->lat->stream;
-    if (g->NV) hipLaunchKernelGGL(k_zero_f64, dim3(nblk(3 * g->NV)), dim3(256), 0, s, 3 * g->NV, g->force);
-    BL_CHECK_LAUNCH();
-    return pp_pass(g, *u, b, s);
+    // vertex forces := the handler's forces alone (set_forces_to_zero first)
+    return pp_pass(g, *u, b, nullptr, g->force, g->lat->stream);
 }
 
 }  // extern "C"

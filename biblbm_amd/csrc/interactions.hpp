@@ -14,6 +14,13 @@ namespace bl {
 constexpr int kMaxWallDeco = 4;
 constexpr int kMaxPPForce = 4;
 
+// A vertex of the collision handler's cell lists: position, owner and the
+// next larger vertex id in its cell (-1 at the end).
+struct alignas(32) PPRec {
+    double x, y, z;
+    int obj, succ;
+};
+
 __host__ __device__ inline double pot_force(const bl_potential& u, double dist)
 {
 #pragma clang fp contract(off)

@@ -36,6 +36,7 @@ import numpy as np
 from . import meshes
 from .ibm import ImmersedBoundary, CAPSULE_SKALAK, SKALAK_RBC
 from .lattice import Lattice, BOUNCE_BACK, VELOCITY_WALL
+from .potentials import MorsePotential
 from .walls import Boundary, GuoWall, outside_mask
 
 
@@ -198,7 +199,7 @@ def place_cells(shape, kinds, seed, periodic=(1, 0, 1), margin=1.5, y_margin=2.0
 
 
 def suspension(storage="f32", nx=512, ny=128, nz=128, tau=1.0, u_c=0.02, n642=300, n2562=80, n162=20,
-               cell_radius=13.5, Ca=0.1, C=10.0, kernel="peskin", seed=7):
+               cell_radius=13.5, Ca=0.1, C=10.0, kernel="peskin", seed=7, interactions=False):
     L = Lattice(nx, ny, nz, periodic=(1, 0, 1), storage=storage, omega=1 / tau)
     mask = np.zeros((nx, ny, nz), np.uint8)
     mask[:, 0, :] = mask[:, -1, :] = BOUNCE_BACK
@@ -231,10 +232,21 @@ def suspension(storage="f32", nx=512, ny=128, nz=128, tau=1.0, u_c=0.02, n642=30
             ib.add_particle(sids[kind], SKALAK_RBC, skalak_rbc_params(G, C, cell_radius, vols[kind], k_bend), pos)
         nv += len(pos)
     ib.init()
+    if interactions:
+        # the rbc_in_pipe prog's input.xml values: Morse wall interaction on
+        # the bounce-back half-way planes, Morse particle-particle force
+        ib.add_wall_interaction(Boundary.plates(0.5, ny - 1.5), MorsePotential(2.0, 2.0, 1e-4))
+        ib.add_pp_force(MorsePotential(1.0, 1.0, 1e-3))
     vf = (n642 * vols["rbc642"] + n2562 * vols["rbc2562"] + n162 * vols["platelet"]) / (nx * (ny - 2) * nz)
     return Workload("suspension", L, ib, nx * ny * nz, nx * (ny - 2) * nz, nv,
                     dict(nx=nx, ny=ny, nz=nz, tau=tau, G=G, cells=len(placed), volume_fraction=vf,
                          kernel=kernel, seed=seed))
+
+
+def suspension_interactions(storage="f32", **kw):
+    """The suspension with the wall-interaction decorator and the
+    particle-particle force switched on."""
+    return suspension(storage, interactions=True, **kw)
 
 
 def taylor_green(storage="f64", n=384, u0=0.03, tau=0.8, n_caps=16, radius=8.0, level=3, Ca=0.1, C=10.0,

@@ -197,3 +197,40 @@ def test_coupled_steps_with_wall_and_pp_forces(device, use_graph):
     assert np.count_nonzero(np.abs(d_wall_pp).sum(1) > 1e-9) > 10
     ib.close()
     L.close()
+
+
+def test_compute_forces_stage_with_interactions(device):
+    """The force stage alone (decorators, membrane, particle-particle) on a
+    suspension of touching capsules near plates, against the oracle's
+    compute_forces."""
+    from biblbm_amd.lattice import Lattice
+    from biblbm_amd.ibm import ImmersedBoundary
+    from biblbm_amd.potentials import MorsePotential, SpringPotential
+    from biblbm_amd.walls import Boundary
+    from oracle import oracle_py as O
+    box = 48
+    rest, tri, pos = _suspension(n_caps=30, box=box, seed=5)
+    pos = [np.clip(p, 0.5, box - 1.5) for p in pos]  # some vertices pressed onto the plates
+    L = Lattice(box, box, box, periodic=(1, 0, 1), omega=1.0)
+    ib = ImmersedBoundary(L, "peskin")
+    sid = ib.add_shape(rest, tri)
+    for p in pos:
+        ib.add_particle(sid, 0, [0.02, 10.0], p)
+    ib.init()
+    ib.set_positions(np.vstack(pos))
+    ib.add_wall_interaction(Boundary.plates(0.5, box - 1.5), MorsePotential(2.0, 2.0, 1e-4))
+    ib.add_pp_force(MorsePotential(1.0, 1.0, 1e-3))
+    ib.add_pp_force(SpringPotential(0.8, 0.05))
+    ib.compute_forces()
+    f = ib.forces()
+    parts = [O.Particle(rest, tri, 0, [0.02, 10.0], p) for p in pos]
+    wi = [(O.Boundary.plates(0.5, box - 1.5), O.Potential.morse(2.0, 2.0, 1e-4))]
+    pp = [O.Potential.morse(1.0, 1.0, 1e-3), O.Potential.spring(0.8, 0.05)]
+    ref = np.vstack(O.vertex_forces(parts, (box, box, box), (1, 0, 1), wi, pp))
+    bare = np.vstack(O.vertex_forces(parts, (box, box, box), (1, 0, 1)))
+    err = np.abs(f - ref).max() / np.abs(ref).max()
+    print("rel err", err, "vertices changed by the interactions", np.count_nonzero(np.abs(ref - bare).sum(1) > 0))
+    assert np.count_nonzero(np.abs(ref - bare).sum(1) > 0) > 100
+    assert err <= 1e-12
+    ib.close()
+    L.close()
