@@ -24,6 +24,9 @@ bounding box).  Extra objects on the same JSON line:
                 launch, against the same boxes stepped one after another
   second_order  rbc_in_pipe with the reference prog's default wall: the Guo
                 curved wall on the pipe surface, no dynamics beyond it
+  interactions  the suspension with the rbc_in_pipe prog's Morse wall and
+                particle-particle interactions, and one collision-handler
+                pass alone
 """
 import argparse
 import json
@@ -155,6 +158,32 @@ def suspension_leg(steps, warmup):
     return out
 
 
+def interactions_leg(steps, warmup, f32_ms=None):
+    """The 400-cell suspension (FP32 storage) with the rbc_in_pipe prog's
+    interactions: Morse wall decorator on the channel walls, Morse
+    particle-particle force through the collision handler."""
+    from biblbm_amd import capi, workloads
+    from biblbm_amd.potentials import MorsePotential
+    W = workloads.suspension("f32", interactions=True)
+    dt = time_coupled(W, steps, warmup)
+    u = MorsePotential(1.0, 1.0, 1e-3)
+    W.ibm.compute_collision_forces(u)
+    capi.call("bl_sync")
+    t0 = time.perf_counter()
+    for _ in range(20):
+        W.ibm.compute_collision_forces(u)
+    capi.call("bl_sync")
+    pass_ms = (time.perf_counter() - t0) / 20 * 1e3
+    out = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
+               collision_pass_ms=round(pass_ms, 4), vertices=W.vertices,
+               workload="suspension 512x128x128 FP32 + Morse wall (2, 2, 1e-4) + Morse pair force (1, 1, 1e-3)")
+    if f32_ms:
+        out["overhead_vs_f32_suspension"] = round(dt / steps * 1e3 / f32_ms - 1.0, 4)
+    W.ibm.close()
+    W.lattice.close()
+    return out
+
+
 def ensemble_leg(steps, warmup, n_boxes=64):
     """64 independent shear boxes (sides from {32..64}^3, one 642-vertex cell
     each) advanced by one hipGraph launch per two steps, against the same
@@ -254,6 +283,8 @@ def main():
         W2.guo.close()
         W2.lattice.close()
         line["suspension"] = suspension_leg(max(50, args.steps // 10), 20)
+        line["interactions"] = interactions_leg(max(50, args.steps // 10), 20,
+                                                line["suspension"]["f32"]["ms_per_step"])
         line["ensemble"] = ensemble_leg(max(50, args.steps // 10), 10)
     if not args.no_cpu_baseline:
         cb, cpu_pos = cpu_baseline(args.cpu_steps, args.cpu_budget)

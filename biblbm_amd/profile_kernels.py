@@ -12,10 +12,12 @@ and summarise with
     python3 -m biblbm_amd.profile_kernels stats OUT
     python3 -m biblbm_amd.profile_kernels traffic OUT_F OUT_W 268435456 k_collide_stream
 
-`collide W S N` builds workload W (rbc_in_pipe, suspension, taylor_green or
-capsule_in_shear) with storage S, runs two coupled steps so that the force
+`collide W S N` builds workload W (rbc_in_pipe, suspension,
+suspension_interactions, taylor_green or capsule_in_shear) with storage S, runs two coupled steps so that the force
 tiles are those of a running simulation, then N collide-and-stream launches.
-`coupled W S N` runs N eager coupled steps instead.  `copy B N` runs N
+`coupled W S N` runs N eager coupled steps instead; `collisions W S N` runs
+N collision-handler passes (Morse r_min 1, beta 1, strength 1e-3) on the
+vertices of workload W after two coupled steps.  `copy B N` runs N
 launches of a plain 16-byte-per-thread device copy of B bytes (the
 calibration kernel for the HBM byte counters).
 """
@@ -50,6 +52,23 @@ def run(argv):
                 capi.call("bl_sync")
                 ms = (time.perf_counter() - t0) * 1e3
             print("%s %s %s: %d launches, %.4f ms each" % (cmd, argv[i + 1], argv[i + 2], n, ms / n), flush=True)
+            W.ibm.close()
+            W.lattice.close()
+            i += 4
+        elif cmd == "collisions":
+            import time
+            from .potentials import MorsePotential
+            W = _workload(argv[i + 1], argv[i + 2])
+            n = int(argv[i + 3])
+            u = MorsePotential(1.0, 1.0, 1e-3)
+            W.ibm.coupled_steps(2, use_graph=False)
+            capi.call("bl_sync")
+            t0 = time.perf_counter()
+            for _ in range(n):
+                W.ibm.compute_collision_forces(u)
+            capi.call("bl_sync")
+            ms = (time.perf_counter() - t0) * 1e3
+            print("collisions %s %s: %d passes, %.4f ms each" % (argv[i + 1], argv[i + 2], n, ms / n), flush=True)
             W.ibm.close()
             W.lattice.close()
             i += 4
