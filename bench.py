@@ -27,6 +27,10 @@ bounding box).  Extra objects on the same JSON line:
   interactions  the suspension with the rbc_in_pipe prog's Morse wall and
                 particle-particle interactions, and one collision-handler
                 pass alone
+  precision     384^3 Taylor-Green box with 16 capsules in FP64, FP32 and
+                FP16-shifted storage from the same start: speed, and the
+                velocity, position and deformation-index errors of the
+                reduced forms against the FP64 run
 """
 import argparse
 import json
@@ -158,6 +162,42 @@ def suspension_leg(steps, warmup):
     return out
 
 
+def precision_leg(steps, warmup):
+    """384^3 periodic Taylor-Green box with 16 Skalak capsules (642
+    vertices), stepped from the same start in FP64, FP32 and FP16-shifted
+    storage; errors of the reduced forms against the FP64 run: velocity
+    field rel-L2, vertex positions rel-L2 and the largest per-capsule change
+    of the Taylor deformation index."""
+    import numpy as np
+    from biblbm_amd import workloads
+    from biblbm_amd.ibm import taylor_deformation
+    out = {}
+    ref = None
+    for storage in ("f64", "f32", "f16s"):
+        W = workloads.taylor_green(storage)
+        dt = time_coupled(W, steps, warmup)
+        _, u = W.lattice.moments()
+        pos = W.ibm.positions()
+        D = np.array([taylor_deformation(W.ibm.observables(p)["inertia"]) for p in range(16)])
+        row = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4))
+        if ref is None:
+            ref = (u, pos, D)
+            row["deformation_index"] = dict(min=float(D.min()), max=float(D.max()))
+        else:
+            row["velocity_rel_l2_vs_f64"] = float(np.linalg.norm(u - ref[0]) / np.linalg.norm(ref[0]))
+            row["position_rel_l2_vs_f64"] = float(np.linalg.norm(pos - ref[1]) / np.linalg.norm(ref[1]))
+            row["deformation_index_max_abs_diff_vs_f64"] = float(np.abs(D - ref[2]).max())
+        out[storage] = row
+        del u
+        W.ibm.close()
+        W.lattice.close()
+        del W
+    out["fp16s_over_fp32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
+    out["workload"] = ("taylor_green 384^3 periodic, u0 0.03, tau 0.8, 16 Skalak capsules (642 vertices, "
+                       "radius 8, Ca 0.1), Peskin, steps %d after %d warmup from the same start" % (steps, warmup))
+    return out
+
+
 def interactions_leg(steps, warmup, f32_ms=None):
     """The 400-cell suspension (FP32 storage) with the rbc_in_pipe prog's
     interactions: Morse wall decorator on the channel walls, Morse
@@ -286,6 +326,7 @@ def main():
         line["interactions"] = interactions_leg(max(50, args.steps // 10), 20,
                                                 line["suspension"]["f32"]["ms_per_step"])
         line["ensemble"] = ensemble_leg(max(50, args.steps // 10), 10)
+        line["precision"] = precision_leg(max(100, args.steps // 4), 20)
     if not args.no_cpu_baseline:
         cb, cpu_pos = cpu_baseline(args.cpu_steps, args.cpu_budget)
         # the same number of coupled steps on the GPU from the same state: parity check

@@ -199,6 +199,14 @@ def test_coupled_steps_match_oracle(device, kernel):
     _, u = L.moments()
     _, uref = O.lbm_moments(f, force)
     assert np.linalg.norm(u - uref) <= 1e-10 * np.linalg.norm(uref)
+    # Taylor deformation index of each particle from the GPU's inertia
+    # tensor against the oracle's inertia of the oracle's vertices
+    from biblbm_amd.ibm import taylor_deformation
+    for pid, p in enumerate(parts):
+        d_gpu = taylor_deformation(ib.observables(pid)["inertia"])
+        d_ref = taylor_deformation(O.moment_of_inertia(p.tri, p.pos))
+        print(kernel, "particle", pid, "deformation index", d_gpu, "oracle", d_ref)
+        assert abs(d_gpu - d_ref) <= 1e-8
 
 
 def _build_pair_ref_start():
