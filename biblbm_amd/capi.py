@@ -69,6 +69,7 @@ SIGNATURES = {
     "bl_ibm_get_phase": [_P, _P],
     "bl_ibm_stage_times": [_P, _P],
     "bl_ibm_tile_stats": [_P, _P],
+    "bl_ibm_set_fused_staging": [_P, _I],
     "bl_ensemble_create": [_I, _P, _P, _I, ctypes.POINTER(_P)],
     "bl_ensemble_destroy": [_P],
     "bl_ensemble_steps": [_P, _I, _I],

@@ -35,6 +35,7 @@ namespace bl {
 namespace {
 
 constexpr int kTileCap = 512;      // list entries per 4x4x4 tile (stencils touching it)
+constexpr int kSmallTile = 64;     // tiles with more entries are spread by a whole workgroup
 constexpr int kMaxList = kTileCap; // entries the spreading kernel sorts in LDS
 
 // ------------------------------------------------------------- helpers
@@ -198,6 +199,7 @@ struct ListReset {
     unsigned char* flags = nullptr;
     int* pairs = nullptr;
     int* arrive = nullptr;
+    int* nbig = nullptr;
 };
 
 __device__ __forceinline__ void list_reset(const ListReset& lr)
@@ -213,6 +215,7 @@ __device__ __forceinline__ void list_reset(const ListReset& lr)
         if (atomicAdd(lr.arrive, 1) == nb - 1) {
             *lr.na = 0;
             *lr.pairs = 0;
+            *lr.nbig = 0;
             *lr.arrive = 0;
         }
     }
@@ -263,7 +266,8 @@ __device__ __forceinline__ void count_tiles(const StencilTiles& st, const Grid& 
 
 __device__ __forceinline__ void fill_tiles(const StencilTiles& st, const Grid& gd, int entry, const int* keys,
                                            const int* cnt, int* rank, int* count, int* list, int cap, int* act,
-                                           int* na, unsigned char* flags, int* status, int* pairs)
+                                           int* na, unsigned char* flags, int* status, int* pairs, int* big,
+                                           int* nbig)
 {
 #pragma unroll 1
     for (int i = 0; i < st.ax[0].m; ++i)
@@ -279,6 +283,7 @@ __device__ __forceinline__ void fill_tiles(const StencilTiles& st, const Grid& g
                 } else {  // the workgroup's table was full for this tile
                     slot = atomicAdd(&count[tt], 1);
                     if (slot == 0) touch_tile(tt, act, na, flags);
+                    if (slot == kSmallTile) big[atomicAdd(nbig, 1)] = tt;
                     atomicAdd(pairs, 1);
                     if (slot >= cap) atomicOr(&status[2], 1);
                 }
@@ -308,13 +313,15 @@ __device__ __forceinline__ int axis_stencil(int kernel, int S, double h, double 
 // the tiles of its top-hat stencil).  Per workgroup the pairs are counted
 // per distinct tile in LDS, one global atomic per tile reserves the
 // workgroup's run, and the slots inside the run come from LDS.  The order
-// within a list is whatever the atomics give; spreading sorts.
+// within a list is whatever the atomics give; spreading sorts.  The add that
+// takes a tile past kSmallTile entries appends it to the big-tile list.
 __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double h, long NV,
                                              const double* __restrict__ pos, int* __restrict__ base,
                                              double* __restrict__ W, int* __restrict__ count,
                                              int* __restrict__ list, int cap, int* __restrict__ act,
                                              int* __restrict__ na, unsigned char* __restrict__ flags,
-                                             int* __restrict__ status, int* __restrict__ pairs, NearView nw)
+                                             int* __restrict__ status, int* __restrict__ pairs,
+                                             int* __restrict__ big, int* __restrict__ nbig, NearView nw)
 {
     __shared__ int keys[kHash];
     __shared__ int cnt[kHash];  // pairs per slot, then the slot's first index in the tile's list
@@ -382,6 +389,7 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
         const int old = atomicAdd(&count[t], c);
         cnt[i] = old;
         if (old == 0) fresh[atomicAdd(&n_fresh, 1)] = t;
+        if (old <= kSmallTile && old + c > kSmallTile) big[atomicAdd(nbig, 1)] = t;
         atomicAdd(&n_pairs, c);
         if (old + c > cap) atomicOr(&status[2], 1);
     }
@@ -395,8 +403,8 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
         act[fresh_base + i] = fresh[i];
         flags[fresh[i]] = 1;
     }
-    if (e0 >= 0) fill_tiles(st, gd, e0, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs);
-    if (e1 >= 0) fill_tiles(stt, gd, e1, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs);
+    if (e0 >= 0) fill_tiles(st, gd, e0, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
+    if (e1 >= 0) fill_tiles(stt, gd, e1, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
 }
 
 __global__ void k_clear_bins(int* count, long n)
@@ -704,22 +712,13 @@ __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, lon
 }
 
 // ------------------------------------------------- velocity staging
-// Staged velocity: one aligned 4-vector per cell (u, 0) so that a z-row of
-// a stencil is one contiguous run of memory.
-template <class C>
-struct alignas(4 * sizeof(C)) Vel4 {
-    C x, y, z, w;
-};
-
-// The progs' computeVelocity pass (u = j/rho + F/2 from the post-stream
-// populations and the last collision's force,
-// src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
-// bounce-back / no-dynamics cells, the wall velocity on velocity-wall
-// cells), restricted to the cells of the tiles this step's stencils touch
-// (`touched`, the step's tile flags).  `lv` carries the force field and
-// flags of the last collision.  One thread per cell in z-fastest order, so
-// population reads coalesce as in the collide kernel and the staged vectors
-// are written contiguously.
+// The progs' computeVelocity pass (staged_velocity), restricted to the
+// cells of the tiles this step's stencils touch (`touched`, the step's tile
+// flags).  `lv` carries the force field and flags of the last collision.
+// One thread per cell in z-fastest order, so population reads coalesce as
+// in the collide kernel and the staged vectors are written contiguously.
+// (With the staging carried by the collision, k_collide_stream writes the
+// same values from the populations it loads.)
 template <class S, bool SWAPPED>
 __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const unsigned char* __restrict__ touched,
                                                  Vel4<typename Pop<S>::C>* __restrict__ ut)
@@ -733,25 +732,13 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const unsigned char
     const unsigned t = ((unsigned)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
     if (!touched[t]) return;
     const unsigned char m = lv.mask ? lv.mask[c] : (unsigned char)M_FLUID;
-    C u[3] = {C(0), C(0), C(0)};
-    if (m == M_VWALL) {
-        const int pp[3] = {x, y, z};
-        const int side = pp[lv.wall_axis] == 0 ? 0 : 1;
-        for (int d = 0; d < 3; ++d) u[d] = (C)lv.wall_u[3 * side + d];
-    } else if (m == M_FLUID) {
+    C f[Q];
+    if (m == M_FLUID) {
         Nbr nb;
         if (SWAPPED) nb.init(lv, x, y, z);
-        C f[Q];
         load_post<S>(lv, nb, c, SWAPPED, f);
-        C Fc[3];
-        cell_force<FT, C>(lv, c, x, y, z, Fc);
-        const C rb = sum_rho(f);
-        C j[3];
-        sum_j(f, j);
-        const C invRho = C(1) / (rb + C(1));
-        for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
     }
-    ut[c] = Vel4<C>{u[0], u[1], u[2], C(0)};
+    ut[c] = staged_velocity<FT, C>(lv, lv.flags, lv.force, m, c, x, y, z, f);
 }
 
 // ------------------------------------------------------- interpolation
@@ -885,19 +872,22 @@ __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __r
 // Every cell of a touched 4x4x4 tile sums g + phi phi phi F_k over the
 // tile's vertices in increasing vertex order, the order of the reference's
 // vertex loop (src/fsi/ImmersedBoundaryDynamics_forces.hh:53-85), and writes
-// its force; one lane per cell.  The tile's list is sorted first.  Then,
-// kChunk entries at a time, lane e forms entry e's weights on the tile's own
-// 4 cells per axis (zero outside the stencil) and puts them with the
-// entry's force into LDS; the cells walk the chunk in order with broadcast
-// LDS reads.
+// its force; one lane per cell.  The tile's list is sorted first.  Then, 64
+// entries at a time, lane e loads entry e and forms its weights on the
+// tile's own 4 cells per axis (zero outside the stencil) and its force in
+// registers (all zero past the list's end); two halves of 32 go through LDS
+// in turn, and the cells walk each half in order, four entries per step,
+// with broadcast LDS reads.
 // One launch (k_spread) holds two kinds of workgroups:
-//   small: tiles with at most 64 entries, one wave per tile (four
-//          independent waves per workgroup), keys sorted in registers, one
-//          running sum from g in vertex order;
-//   big:   the other tiles, one workgroup of four waves per tile, keys
-//          sorted in LDS; wave w sums the w-th contiguous quarter of the
-//          sorted list and the four partial sums are added to g in wave
-//          order.
+//   small: tiles with at most kSmallTile entries (the touched-tile list,
+//          skipping the others), one wave per tile, four independent waves
+//          per workgroup, keys sorted in registers, one running sum from g
+//          in vertex order;
+//   big:   the big-tile list, one workgroup of four waves per tile; each
+//          wave sorts a run of 64 keys in registers and the runs are merged
+//          by rank in LDS (a bitonic sort in LDS past 256 entries); wave w
+//          sums the w-th contiguous quarter of the sorted list and the four
+//          partial sums are added to g in wave order.
 // Each clears the tile's count once done, so the next binning starts from
 // zero without a clearing pass.
 constexpr int kChunk = 32;
@@ -912,15 +902,16 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Lane `lane` (< kChunk) stages list entry `e` (tile-local weights, force)
-// into ew[.][lane].  The vertex's weights are loaded whole (not indexed by
-// the stencil offset) so that no load waits for the base.  A near-wall
-// vertex's kernel part carries the force times w (w < 1), its top-hat part
-// the force times 1 - w (w > 0), the top hat's two weights per axis.
-__device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int org[3], int e,
-                                            const int* __restrict__ base, const double* __restrict__ W,
-                                            const double* __restrict__ vforce, const NearView& nw,
-                                            double (*ew)[kChunk + 1], int lane)
+// Weights of list entry `e` on the tile's cells (rows 0-11: axis-major, 4
+// cells per axis) and its force (rows 12-14), into registers.  The vertex's
+// weights are loaded whole (not indexed by the stencil offset) so that no
+// load waits for the base.  A near-wall vertex's kernel part carries the
+// force times w (w < 1), its top-hat part the force times 1 - w (w > 0),
+// the top hat's two weights per axis.
+__device__ __forceinline__ void entry_weights(const Grid& gd, int Sw, const int org[3], int e,
+                                              const int* __restrict__ base, const double* __restrict__ W,
+                                              const double* __restrict__ vforce, const NearView& nw,
+                                              double ev[15])
 {
     const bool nr = (e & kNearKey) != 0;
     const long v = nr ? (long)((e & ~kNearKey) >> 1) : (long)e;
@@ -959,7 +950,7 @@ __device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int or
             if (gd.per[ax] && d < 0) d += gd.n[ax];
             const double* wa = w + 4 * ax;
             const double x = d == 0 ? wa[0] : (d == 1 ? wa[1] : (d == 2 ? wa[2] : wa[3]));
-            ew[4 * ax + l][lane] = (unsigned)d < (unsigned)sw ? x : 0.0;
+            ev[4 * ax + l] = (unsigned)d < (unsigned)sw ? x : 0.0;
         }
     }
     double F0 = vforce[3 * v], F1 = vforce[3 * v + 1], F2 = vforce[3 * v + 2];
@@ -973,19 +964,61 @@ __device__ __forceinline__ void stage_entry(const Grid& gd, int Sw, const int or
             F2 *= s;
         }
     }
-    ew[12][lane] = F0;
-    ew[13][lane] = F1;
-    ew[14][lane] = F2;
+    ev[12] = F0;
+    ev[13] = F1;
+    ev[14] = F2;
 }
 
+// Adds entries [0, cnt) of a staged half (cnt <= kChunk; the rows are zero
+// from cnt up to the next multiple of 4, so a padded entry adds nothing)
+// in entry order; the LDS reads of four entries are issued together.
 __device__ __forceinline__ void chunk_sum(const double (*ew)[kChunk + 1], int cnt, int lx, int ly, int lz,
                                           double& a0, double& a1, double& a2)
 {
-    for (int e = 0; e < cnt; ++e) {
-        const double w = (ew[lx][e] * ew[4 + ly][e]) * ew[8 + lz][e];
-        a0 += w * ew[12][e];
-        a1 += w * ew[13][e];
-        a2 += w * ew[14][e];
+    for (int e = 0; e < cnt; e += 4) {
+        double w[4], f0[4], f1[4], f2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            w[k] = (ew[lx][e + k] * ew[4 + ly][e + k]) * ew[8 + lz][e + k];
+            f0[k] = ew[12][e + k];
+            f1[k] = ew[13][e + k];
+            f2[k] = ew[14][e + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            a0 += w[k] * f0[k];
+            a1 += w[k] * f1[k];
+            a2 += w[k] * f2[k];
+        }
+    }
+}
+
+// Sums the sorted entries [lo, hi) of a tile into a0..a2, 64 at a time;
+// key(e) gives the e-th sorted entry.
+template <class KeyOf>
+__device__ __forceinline__ void sum_entries(const Grid& gd, int Sw, const int org[3], int lo, int hi, KeyOf key,
+                                            const int* __restrict__ base, const double* __restrict__ W,
+                                            const double* __restrict__ vforce, const NearView& nw,
+                                            double (*ew)[kChunk + 1], int lane, int lx, int ly, int lz,
+                                            double& a0, double& a1, double& a2)
+{
+    for (int c0 = lo; c0 < hi; c0 += 64) {
+        const int e = c0 + lane;
+        double ev[15];
+#pragma unroll
+        for (int r = 0; r < 15; ++r) ev[r] = 0.0;
+        if (e < hi) entry_weights(gd, Sw, org, key(e), base, W, vforce, nw, ev);
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int cnt = min(kChunk, hi - c0 - half * kChunk);
+            if (cnt <= 0) break;
+            if ((lane >> 5) == half)
+#pragma unroll
+                for (int r = 0; r < 15; ++r) ew[r][lane & (kChunk - 1)] = ev[r];
+            wave_sync();
+            chunk_sum(ew, cnt, lx, ly, lz, a0, a1, a2);
+            wave_sync();
+        }
     }
 }
 
@@ -1010,13 +1043,38 @@ __device__ __forceinline__ void store_cell(const LatView& lv, const Grid& gd, co
     }
 }
 
-template <class S>
+// ascending bitonic sort of one key per lane across the wave
+__device__ __forceinline__ int wave_sort(int key, int lane)
+{
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int other = __shfl_xor(key, stride);
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            key = (lower == up) ? min(key, other) : max(key, other);
+        }
+    return key;
+}
+
+// number of entries of the ascending run r[0, 64) that are below `key`
+__device__ __forceinline__ int rank_in_run(const int* r, int key)
+{
+    int lo = 0;  // count among r[0, 63), then r[63] when all of those are below
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+        if (r[lo + step - 1] < key) lo += step;
+    return lo + (r[lo] < key ? 1 : 0);
+}
+
+template <class FT>
 __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[15][kChunk + 1], const LatView& lv,
                                              const Grid& gd, int Sw, const int* __restrict__ active,
                                              const int* __restrict__ nactive, int* __restrict__ count, int cap,
                                              const int* __restrict__ list, const int* __restrict__ base,
                                              const double* __restrict__ W, const double* __restrict__ vforce,
-                                             const NearView& nw, typename Pop<S>::F* __restrict__ Fout)
+                                             const NearView& nw, FT* __restrict__ Fout)
 {
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
@@ -1026,66 +1084,50 @@ __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[1
     for (int a = blk * kSpreadWaves + wv; a < na; a += nb * kSpreadWaves) {
         const int t = active[a];
         const int n = count[t];
-        if (n > 64) continue;
-        int key = lane < n ? list[(long)t * cap + lane] : 0x7fffffff;
-#pragma unroll
-        for (int size = 2; size <= 64; size <<= 1)
-#pragma unroll
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                const int other = __shfl_xor(key, stride);
-                const bool up = (lane & size) == 0;
-                const bool lower = (lane & stride) == 0;
-                key = (lower == up) ? min(key, other) : max(key, other);
-            }
+        if (n > kSmallTile) continue;
+        const int key = wave_sort(lane < n ? list[(long)t * cap + lane] : 0x7fffffff, lane);
         int org[3];
         tile_origin(gd, t, org);
         double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
-        for (int c0 = 0; c0 < n; c0 += kChunk) {
-            const int kk = __shfl(key, (c0 + lane) & 63);
-            if (lane < kChunk && c0 + lane < n) stage_entry(gd, Sw, org, kk, base, W, vforce, nw, ew, lane);
-            wave_sync();
-            chunk_sum(ew, min(kChunk, n - c0), lx, ly, lz, acc0, acc1, acc2);
-            wave_sync();
-        }
+        sum_entries(gd, Sw, org, 0, n, [&](int) { return key; }, base, W, vforce, nw, ew, lane, lx, ly, lz, acc0,
+                    acc1, acc2);
         store_cell(lv, gd, org, lx, ly, lz, acc0, acc1, acc2, Fout);
         if (lane == 0) count[t] = 0;
     }
 }
 
-template <class S>
-__device__ __forceinline__ void spread_big(int blk, int nb, int* keys, double (*ew_all)[15][kChunk + 1],
+template <class FT>
+__device__ __forceinline__ void spread_big(int blk, int nb, int* keys, int* runs, double (*ew_all)[15][kChunk + 1],
                                            double (*part)[3][64], const LatView& lv, const Grid& gd, int Sw,
-                                           const int* __restrict__ active, const int* __restrict__ nactive,
+                                           const int* __restrict__ big, const int* __restrict__ nbig,
                                            int* __restrict__ count, int cap, const int* __restrict__ list,
                                            const int* __restrict__ base, const double* __restrict__ W,
                                            const double* __restrict__ vforce, const NearView& nw,
-                                           typename Pop<S>::F* __restrict__ Fout)
+                                           FT* __restrict__ Fout)
 {
     const int wv = threadIdx.x >> 6;
     double(*ew)[kChunk + 1] = ew_all[wv];
-    const int na = *nactive;
+    const int nt = *nbig;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
-    for (int a = blk; a < na; a += nb) {
-        const int t = active[a];
-        int n = count[t];
-        if (n <= 64) continue;
-        n = min(n, cap);  // a longer list was cut at binning and reported there
+    for (int a = blk; a < nt; a += nb) {
+        const int t = big[a];
+        const int n = min(count[t], cap);  // a longer list was cut at binning and reported there
         const long o = (long)t * cap;
-        // sorted keys: in registers of every wave for n <= 64, else in LDS
-        int key = lane < n ? list[o + lane] : 0x7fffffff;
-        const bool small = n <= 64;
-        if (small) {
+        if (n <= 64 * kSpreadWaves) {
+            // four register-sorted runs, merged by rank (keys are distinct)
+            const int e = wv * 64 + lane;
+            const int key = wave_sort(e < n ? list[o + e] : 0x7fffffff, lane);
+            runs[e] = key;
+            __syncthreads();
+            if (e < n) {
+                int r = lane;
 #pragma unroll
-            for (int size = 2; size <= 64; size <<= 1)
-#pragma unroll
-                for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                    const int other = __shfl_xor(key, stride);
-                    const bool up = (lane & size) == 0;
-                    const bool lower = (lane & stride) == 0;
-                    key = (lower == up) ? min(key, other) : max(key, other);
-                }
+                for (int w2 = 0; w2 < kSpreadWaves; ++w2)
+                    if (w2 != wv) r += rank_in_run(runs + 64 * w2, key);
+                keys[r] = key;
+            }
         } else {
             int n2 = 1;
             while (n2 < n) n2 <<= 1;
@@ -1107,20 +1149,15 @@ __device__ __forceinline__ void spread_big(int blk, int nb, int* keys, double (*
                     __syncthreads();
                 }
         }
+        __syncthreads();
         int org[3];
         tile_origin(gd, t, org);
         // wave w sums the contiguous range [lo, hi) of the sorted entries
         const int q = (n + kSpreadWaves - 1) / kSpreadWaves;
         const int lo = min(n, wv * q), hi = min(n, lo + q);
         double acc0 = 0, acc1 = 0, acc2 = 0;
-        for (int c0 = lo; c0 < hi; c0 += kChunk) {
-            const int e = c0 + lane;
-            const int kk = small ? __shfl(key, e & 63) : keys[min(e, n - 1)];
-            if (lane < kChunk && e < hi) stage_entry(gd, Sw, org, kk, base, W, vforce, nw, ew, lane);
-            wave_sync();
-            chunk_sum(ew, min(kChunk, hi - c0), lx, ly, lz, acc0, acc1, acc2);
-            wave_sync();
-        }
+        sum_entries(gd, Sw, org, lo, hi, [&](int k) { return keys[k]; }, base, W, vforce, nw, ew, lane, lx, ly, lz,
+                    acc0, acc1, acc2);
         part[wv][0][lane] = acc0;
         part[wv][1][lane] = acc1;
         part[wv][2][lane] = acc2;
@@ -1145,6 +1182,8 @@ template <class S>
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid gd, int Sw, int n_small,
                                                               const int* __restrict__ active,
                                                               const int* __restrict__ nactive,
+                                                              const int* __restrict__ big,
+                                                              const int* __restrict__ nbig,
                                                               int* __restrict__ count, int cap,
                                                               const int* __restrict__ list,
                                                               const int* __restrict__ base,
@@ -1153,15 +1192,15 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid g
                                                               typename Pop<S>::F* __restrict__ Fout)
 {
     __shared__ int keys[kMaxList];
+    __shared__ int runs[64 * kSpreadWaves];
     __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
     __shared__ double part[kSpreadWaves][3][64];
     const int b = (int)blockIdx.x;
     if (b < n_small)
-        spread_small<S>(b, n_small, ew_all, lv, gd, Sw, active, nactive, count, cap, list, base, W, vforce, nw,
-                        Fout);
+        spread_small(b, n_small, ew_all, lv, gd, Sw, active, nactive, count, cap, list, base, W, vforce, nw, Fout);
     else
-        spread_big<S>(b - n_small, (int)gridDim.x - n_small, keys, ew_all, part, lv, gd, Sw, active, nactive, count,
-                      cap, list, base, W, vforce, nw, Fout);
+        spread_big(b - n_small, (int)gridDim.x - n_small, keys, runs, ew_all, part, lv, gd, Sw, big, nbig, count,
+                   cap, list, base, W, vforce, nw, Fout);
 }
 
 // --------------------------------------------------------------- motion
@@ -1515,6 +1554,7 @@ ListReset list_reset_of(bl_ibm g)
     lr.flags = g->flags_alt;
     lr.pairs = g->n_active + 5 + g->alt;
     lr.arrive = g->n_active + 4;
+    lr.nbig = g->n_active + 8 + g->alt;
     return lr;
 }
 
@@ -1552,7 +1592,8 @@ int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
     if (g->NV)
         hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
                            (const double*)g->pos, g->v_base, g->v_w, g->tile_count, g->tile_list, kTileCap, act,
-                           g->n_active + g->alt, g->flags_alt, g->n_active, g->n_active + 5 + g->alt, near_view(g));
+                           g->n_active + g->alt, g->flags_alt, g->n_active, g->n_active + 5 + g->alt,
+                           g->active + (2L + g->alt) * L->ntiles, g->n_active + 8 + g->alt, near_view(g));
     if (done) BL_HIP(hipEventRecord(done, s));
     BL_CHECK_LAUNCH();
     g->binned = true;
@@ -1650,10 +1691,16 @@ void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s)
     const Grid gd = grid_of(L);
     const int* act = g->active + (long)g->alt * L->ntiles;
     const int* na = g->n_active + g->alt;
-    const int gs = (int)std::max<long>(1, std::min<long>((L->ntiles + kSpreadWaves - 1) / kSpreadWaves, 4096));
-    const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, 2048));
-    hipLaunchKernelGGL(k_spread<S>, dim3(gs + gb), dim3(64 * kSpreadWaves), 0, s, lv, gd, g->S, gs, act, na,
-                       g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
+    const int* big = g->active + (2L + g->alt) * L->ntiles;
+    const int* nbig = g->n_active + 8 + g->alt;
+    // about one small-tile wave per touched tile and one workgroup per big
+    // tile: the touched tiles are at most every tile and, for membranes,
+    // fewer than the vertices; big tiles fewer than a 32nd of the vertices
+    const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(g->NV, 1024)));
+    const int gs = (int)((small_waves + kSpreadWaves - 1) / kSpreadWaves);
+    const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, std::max<long>(g->NV / 32, 256)));
+    hipLaunchKernelGGL(k_spread<S>, dim3(gs + gb), dim3(64 * kSpreadWaves), 0, s, lv, gd, g->S, gs, act, na, big,
+                       nbig, g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
                        (const double*)g->v_w, (const double*)g->force, near_view(g), (FT*)g->force_alt);
 }
 
@@ -1826,11 +1873,67 @@ int step_forked(bl_ibm g)
     return BL_OK;
 }
 
+// Whether coupled steps carry the velocity staging in the collision
+// (step_fused).  Only without a phase field: the collision of a step reads
+// the phase voxelized from the moved vertices, which need the staged
+// velocity first.  By default for 65 536 vertices or more, where the
+// staging pass reads populations over most of the lattice.
+bool use_fused(bl_ibm g)
+{
+    bl_lattice L = g->lat;
+    if (g->fused_staging == 0 || g->NV == 0 || L->omega_c0 != L->omega_c1) return false;
+    return g->fused_staging == 1 || g->NV >= 65536;
+}
+
+// One coupled step with the velocity staging carried by the collision.
+// The reference's order (computeVelocity, interpolate, spread, move,
+// collide) has the collision depend only on the spread force, and the
+// interpolation only on the staged velocity, so the step runs as
+//   A: particle-particle and membrane forces (positions before the move)
+//   M: binning; then, after A, spreading; collide-and-stream, which also
+//      stages u (pre-collision populations, the last collision's force
+//      field) on the touched tiles; post-stream walls; interpolation +
+//      advection (readying the next binning's list); geometry.
+// The staged values are those of k_stage_u; one pass over the touched
+// populations is saved.
+int step_fused(bl_ibm g)
+{
+    int rc;
+    bl_lattice L = g->lat;
+    hipStream_t M = L->stream, A = g->aux[0];
+    hipEvent_t* e = g->fev;
+    BL_HIP(hipEventRecord(e[0], M));
+    BL_HIP(hipStreamWaitEvent(A, e[0], 0));
+    if ((rc = do_forces(g, A))) return rc;
+    BL_HIP(hipEventRecord(e[1], A));
+    if ((rc = do_bin(g, M))) return rc;
+    BL_HIP(hipStreamWaitEvent(M, e[1], 0));
+    if ((rc = launch_spread(g, M))) return rc;
+    // after the swap: the lattice holds this step's field and flags (the
+    // touched tiles), the boundary's spare buffers the last collision's
+    L->emit_u = g->ut;
+    L->emit_force = g->force_alt;
+    L->emit_flags = g->flags_alt;
+    rc = collide_stream_launch(L);
+    L->emit_u = nullptr;
+    L->emit_force = nullptr;
+    L->emit_flags = nullptr;
+    if (rc) return rc;
+    if ((rc = velocity_walls_launch(L))) return rc;
+    if ((rc = launch_interp(g, true, M, true))) return rc;
+    g->binned = false;
+    return update_particles(g, 1, M);
+}
+
 // n coupled steps enqueued on the lattice stream (forked IBM stages)
 int steps_forked(bl_ibm g, int n)
 {
     int rc;
     for (int k = 0; k < n; ++k) {
+        if (use_fused(g)) {
+            if ((rc = step_fused(g))) return rc;
+            continue;
+        }
         if ((rc = step_forked(g))) return rc;
         if ((rc = collide_stream_launch(g->lat))) return rc;
         if ((rc = velocity_walls_launch(g->lat))) return rc;
@@ -2071,9 +2174,9 @@ int bl_ibm_init(bl_ibm g)
     BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     // tiles
     if ((rc = alloc(g->tile_count, L->ntiles)) || (rc = alloc(g->tile_list, (size_t)L->ntiles * kTileCap)) ||
-        (rc = alloc(g->active, 2 * (size_t)L->ntiles)) || (rc = alloc(g->n_active, 8)))
+        (rc = alloc(g->active, 4 * (size_t)L->ntiles)) || (rc = alloc(g->n_active, 16)))
         return rc;
-    BL_HIP(hipMemset(g->n_active, 0, 8 * sizeof(int)));
+    BL_HIP(hipMemset(g->n_active, 0, 16 * sizeof(int)));
     if ((rc = alloc(g->flags_alt, L->ntiles))) return rc;
     BL_HIP(hipMemset(g->flags_alt, 0, L->ntiles));
     BL_HIP(hipMalloc(&g->force_alt, 3 * L->stride * L->force_bytes));
@@ -2364,6 +2467,15 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2])
     BL_HIP(hipStreamSynchronize(g->lat->stream));
     out[0] = st[g->last_list];
     out[1] = st[5 + g->last_list];
+    return BL_OK;
+}
+
+int bl_ibm_set_fused_staging(bl_ibm g, int mode)
+{
+    if (!g) return fail(BL_ERR_ARG, "null ibm handle");
+    if (mode < -1 || mode > 1) return fail(BL_ERR_ARG, "bl_ibm_set_fused_staging: mode must be -1, 0 or 1");
+    if (mode != g->fused_staging) ibm_invalidate_graphs(g);
+    g->fused_staging = mode;
     return BL_OK;
 }
 

@@ -109,9 +109,11 @@ struct bl_ibm_s {
     double* v_w = nullptr;    // 12 per vertex (3 axes x up to 4 weights)
     int* tile_count = nullptr;  // entries per tile (cleared by the spreading that consumes them)
     int* tile_list = nullptr;   // per tile a fixed-capacity list of the vertices whose stencil touches it
-    int* active = nullptr;      // two lists of touched tiles, one per force-flag buffer (ntiles each)
+    int* active = nullptr;      // two lists of touched tiles, one per force-flag buffer, then the two
+                                // lists of their tiles with more than 64 entries (ntiles each)
     int* n_active = nullptr;    // [0], [1] lengths of the two lists, [2] overflow flags, [4] arrival counter
-                                // of the list reset, [5], [6] (tile, vertex) pairs of the two lists
+                                // of the list reset, [5], [6] (tile, vertex) pairs of the two lists,
+                                // [8], [9] lengths of the two big-tile lists
     int alt = 0;                // list and flag buffer the next binning writes (flags_alt)
     int last_list = 0;          // list of the last binning
     bool alt_full = false;      // flags_alt has every tile set: the next binning clears all of it
@@ -135,6 +137,7 @@ struct bl_ibm_s {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double stage_ms[5] = {0, 0, 0, 0, 0};
     bool timing = false;
+    int fused_staging = -1;  // velocity staging carried by the collision: -1 auto (large vertex counts), 0 off, 1 on
 
     hipStream_t aux[2] = {nullptr, nullptr};  // forked stages of a step (default priority)
     hipEvent_t fev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -57,6 +57,9 @@ LatView view_of(const bl_lattice_s* L)
     v.wall_axis = L->wall_axis;
     for (int k = 0; k < 6; ++k) v.wall_u[k] = L->wall_u[k];
     v.xcd_remap = L->xcd_remap;
+    v.emit_u = L->emit_u;
+    v.emit_force = L->emit_force;
+    v.emit_flags = L->emit_flags;
     return v;
 }
 
@@ -78,7 +81,11 @@ __device__ __forceinline__ K odd_slot(const NbrOff& nb, bool inner, int sx, int 
     return valid ? (K)p_valid * (K)st + (K)(c + d) : (K)p_face * (K)st + (K)c;
 }
 
-template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K>
+// EMIT: the velocity staging of an immersed-boundary step rides on the
+// launch (bl_lattice_s::emit_u): cells of flagged tiles also write the
+// computeVelocity value of the populations they loaded, with the last
+// collision's force field.
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT = false>
 __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 {
     using P = Pop<S>;
@@ -118,6 +125,11 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
     }
 
     const unsigned char m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
+    if (EMIT) {
+        const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
+        if (v.flags[t])
+            ((Vel4<C>*)v.emit_u)[c] = staged_velocity<FT, C>(v, v.emit_flags, v.emit_force, m, c, x, y, z, f);
+    }
     if (m == M_FLUID || m == M_VWALL) {
         C F[3] = {C(0), C(0), C(0)};
         if (FMODE == 1) { F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2]; }
@@ -153,6 +165,17 @@ template <class S, bool ODD, bool MASK, int FMODE>
 void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
 {
     const bool big = ODD && (size_t)Q * (size_t)v.stride > 0xffffffffull;
+    if constexpr (FMODE == 2) {
+        if (v.emit_u) {  // the launcher passes emit_u only without a phase field
+            if (big)
+                hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, 2, false, size_t, true>), dim3(nblk), dim3(blk), 0, s,
+                                   v);
+            else
+                hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, 2, false, unsigned, true>), dim3(nblk), dim3(blk), 0,
+                                   s, v);
+            return;
+        }
+    }
     if (big) {
         if (phase)
             hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true, size_t>), dim3(nblk), dim3(blk), 0, s, v);
@@ -201,6 +224,7 @@ int collide_stream_launch(bl_lattice_s* L)
     const int fm = force_mode(L);
     if (fm != 2) v.flags = nullptr;
     const bool phase = L->phase != nullptr && L->omega_c0 != L->omega_c1;
+    if (fm != 2 || phase) v.emit_u = nullptr;
     const unsigned nblk = (unsigned)((L->ncell + L->block - 1) / L->block);
     if (L->storage == BL_STORE_F64) launch_cs<double>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
     else if (L->storage == BL_STORE_F32) launch_cs<float>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);

@@ -36,6 +36,15 @@ struct bl_lattice_s {
     bool force_field = false;  // tiles may carry a field
     bool flags_full = false;   // every tile flag was set (a full field was loaded); cleared by its consumer
 
+    // Velocity staging carried by the next collide-and-stream (set by an
+    // immersed boundary around one launch): on the cells of the tiles flagged
+    // in tile_flags, u = j/rho + F/2 of the pre-collision populations with
+    // the force of emit_force / emit_flags (the last collision's field) goes
+    // to emit_u, one aligned 4-vector of the compute type per cell.
+    void* emit_u = nullptr;
+    const void* emit_force = nullptr;
+    const unsigned char* emit_flags = nullptr;
+
     long long steps = 0;  // completed collide-and-stream steps (AA parity)
     hipStream_t stream = nullptr;
 
@@ -80,6 +89,9 @@ struct LatView {
     int wall_axis;
     double wall_u[6];
     int xcd_remap;
+    void* emit_u;  // velocity staging carried by the collision (see bl_lattice_s)
+    const void* emit_force;
+    const unsigned char* emit_flags;
 };
 
 LatView view_of(const bl_lattice_s* L);

@@ -176,20 +176,29 @@ __device__ __forceinline__ void store_post(const LatView& v, const Nbr& n, long 
     }
 }
 
-// External force of cell c.
+// External force of cell c in the field (flags, force): the field's value
+// in a flagged tile, the body force elsewhere.
 template <class FT, class C>
-__device__ __forceinline__ void cell_force(const LatView& v, long c, int x, int y, int z, C F[3])
+__device__ __forceinline__ void cell_force_in(const LatView& v, const unsigned char* flags, const void* force,
+                                              long c, int x, int y, int z, C F[3])
 {
     F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2];
-    if (v.flags) {
+    if (flags) {
         const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
-        if (v.flags[t]) {
-            const FT* Fv = (const FT*)v.force;
+        if (flags[t]) {
+            const FT* Fv = (const FT*)force;
             F[0] = (C)Fv[c];
             F[1] = (C)Fv[v.stride + c];
             F[2] = (C)Fv[2 * v.stride + c];
         }
     }
+}
+
+// External force of cell c.
+template <class FT, class C>
+__device__ __forceinline__ void cell_force(const LatView& v, long c, int x, int y, int z, C F[3])
+{
+    cell_force_in<FT, C>(v, v.flags, v.force, c, x, y, z, F);
 }
 
 // ------------------------------------------------------------- collision
@@ -284,6 +293,40 @@ __device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
         f[17] += -o12 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy - 3 * uz) + Fz * (C(-1) - 2 * uz - 3 * uy));
         f[18] += -o12 * mu * (Fx * ux + Fy * (C(-1) - 2 * uy + 3 * uz) + Fz * (C(1) - 2 * uz + 3 * uy));
     }
+}
+
+// ------------------------------------------------------ velocity staging
+// Staged velocity: one aligned 4-vector per cell (u, 0) so that a z-row of
+// a stencil is one contiguous run of memory.
+template <class C>
+struct alignas(4 * sizeof(C)) Vel4 {
+    C x, y, z, w;
+};
+
+// The progs' computeVelocity value of one cell (u = j/rho + F/2 from the
+// post-stream populations f and the last collision's force,
+// src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
+// bounce-back / no-dynamics cells, the wall velocity on velocity-wall
+// cells).  (flags, force) is the last collision's force field.
+template <class FT, class C>
+__device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsigned char* flags, const void* force,
+                                                   unsigned char m, long c, int x, int y, int z, const C f[Q])
+{
+    C u[3] = {C(0), C(0), C(0)};
+    if (m == M_VWALL) {
+        const int pp[3] = {x, y, z};
+        const int side = pp[v.wall_axis] == 0 ? 0 : 1;
+        for (int d = 0; d < 3; ++d) u[d] = (C)v.wall_u[3 * side + d];
+    } else if (m == M_FLUID) {
+        C Fc[3];
+        cell_force_in<FT, C>(v, flags, force, c, x, y, z, Fc);
+        const C rb = sum_rho(f);
+        C j[3];
+        sum_j(f, j);
+        const C invRho = C(1) / (rb + C(1));
+        for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
+    }
+    return Vel4<C>{u[0], u[1], u[2], C(0)};
 }
 
 }  // namespace bl

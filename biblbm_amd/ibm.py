@@ -132,6 +132,11 @@ class ImmersedBoundary:
         call("bl_ibm_get_phase", self.handle, ptr(out))
         return out
 
+    def set_fused_staging(self, mode):
+        """-1 auto, 0 a staging pass of its own, 1 staging carried by the
+        collide-and-stream launch (bl_ibm_set_fused_staging)."""
+        call("bl_ibm_set_fused_staging", self.handle, int(mode))
+
     def tile_stats(self):
         out = np.zeros(2, np.int64)
         call("bl_ibm_tile_stats", self.handle, ptr(out))

@@ -190,6 +190,15 @@ int bl_ibm_stage_times(bl_ibm g, double out[5]);
 /* Tiles touched by the last binning and (tile, vertex) list entries
  * (diagnostics for the traffic model; no counterpart in the reference). */
 int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
+/* Launch structure of bl_coupled_steps (no counterpart in the reference,
+ * whose order it keeps): mode 1 carries the velocity staging (the progs'
+ * computeVelocity pass, restricted to the touched tiles) in the
+ * collide-and-stream launch, which then runs before the interpolation and
+ * advection of the same step; 0 stages in a pass of its own before the
+ * spreading; -1 (default) picks 1 for 65 536 vertices or more.  Without a
+ * phase field only (omega_in == omega_out); otherwise the separate pass is
+ * used.  The staged values are the same either way. */
+int bl_ibm_set_fused_staging(bl_ibm g, int mode);
 
 /* ---------------------------------------------------- curved walls */
 /* GuoRigidWallBoundary(boundary, lattice) (src/fsi/GuoBoundaryCondition.hh:

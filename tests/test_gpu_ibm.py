@@ -316,3 +316,62 @@ def test_lattice_destroyed_before_boundary(device):
     with pytest.raises(capi.BlError):
         ib.coupled_steps(1)
     ib.close()
+
+
+@pytest.mark.parametrize("kernel,storage", [("peskin", "f64"), ("roma", "f64"), ("peskin", "f32")])
+def test_fused_staging_matches_separate_pass(device, kernel, storage):
+    """Coupled steps with the velocity staging carried by the
+    collide-and-stream launch give the trajectory of the separate staging
+    pass, and (FP64) the oracle's."""
+    mask = np.zeros(SHAPE, np.uint8)
+    mask[:, 0, :] = mask[:, -1, :] = 1
+    L1, ib1, f0, F, parts = _build_pair(kernel, storage, periodic=(1, 0, 1), mask=mask)
+    L2, ib2, *_ = _build_pair(kernel, storage, periodic=(1, 0, 1), mask=mask)
+    ib1.set_fused_staging(0)
+    ib2.set_fused_staging(1)
+    steps = 6
+    ib1.coupled_steps(steps, use_graph=False)
+    ib2.coupled_steps(steps, use_graph=False)
+    p1, p2 = ib1.positions(), ib2.positions()
+    assert np.abs(p2 - p1).max() <= 1e-13 * np.abs(p1).max()
+    _, u1 = L1.moments()
+    _, u2 = L2.moments()
+    assert np.abs(u2 - u1).max() <= 1e-13 * np.abs(u1).max()
+    if storage == "f64":
+        f, force = f0, F
+        for _ in range(steps):
+            f, force, _ = O.coupled_step(f, force, np.zeros(SHAPE), mask, (1, 0, 1), 1 / 0.9, 1 / 0.9,
+                                         (2e-6, 0, 0), parts, kernel)
+        ref = np.concatenate([p.pos for p in parts])
+        err = np.linalg.norm(p2 - ref) / np.linalg.norm(ref)
+        print(kernel, "fused staging position rel L2 vs oracle", err)
+        assert err <= 1e-10
+
+
+def test_fused_staging_graph_replay_equals_eager(device):
+    L1, ib1, *_ = _build_pair("peskin")
+    L2, ib2, *_ = _build_pair("peskin")
+    ib1.set_fused_staging(1)
+    ib2.set_fused_staging(1)
+    ib1.coupled_steps(6, use_graph=False)
+    ib2.coupled_steps(6, use_graph=True)
+    assert np.array_equal(ib1.positions(), ib2.positions())
+    assert np.array_equal(L1.get_populations(), L2.get_populations())
+    for _ in range(3):
+        ib1.coupled_steps(4, use_graph=False)
+        ib2.coupled_steps(4, use_graph=True)
+        L2.moments()
+        assert np.array_equal(ib1.positions(), ib2.positions())
+    # switching the mode between replays keeps the trajectory
+    ib1.set_fused_staging(0)
+    ib2.set_fused_staging(0)
+    ib1.coupled_steps(4, use_graph=False)
+    ib2.coupled_steps(4, use_graph=True)
+    assert np.array_equal(ib1.positions(), ib2.positions())
+
+
+def test_fused_staging_mode_is_checked(device):
+    from biblbm_amd import capi
+    L, ib, *_ = _build_pair("roma")
+    with pytest.raises(capi.BlError):
+        ib.set_fused_staging(2)
