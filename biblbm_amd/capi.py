@@ -70,6 +70,16 @@ SIGNATURES = {
     "bl_ibm_stage_times": [_P, _P],
     "bl_ibm_tile_stats": [_P, _P],
     "bl_ibm_set_fused_staging": [_P, _I],
+    "bl_shape_set_tag": [_P, _I, ctypes.c_char_p],
+    "bl_particle_pack": [_P, _I, _P, ctypes.c_longlong, _PLL],
+    "bl_particle_describe": [_P, _I, _PI, _PI, _PLL],
+    "bl_ibm_save_checkpoint": [_P, ctypes.c_char_p, ctypes.c_char_p],
+    "bl_ibm_load_checkpoint": [_P, ctypes.c_char_p, ctypes.c_char_p],
+    "bl_particle_write_vtk": [_P, _I, ctypes.c_char_p],
+    "bl_particle_write_gmsh": [_P, _I, ctypes.c_char_p],
+    "bl_ibm_write_lightweight": [_P, ctypes.c_char_p, ctypes.c_longlong],
+    "bl_lattice_save_checkpoint": [_P, ctypes.c_char_p],
+    "bl_lattice_load_checkpoint": [_P, ctypes.c_char_p],
     "bl_ensemble_create": [_I, _P, _P, _I, ctypes.POINTER(_P)],
     "bl_ensemble_destroy": [_P],
     "bl_ensemble_steps": [_P, _I, _I],

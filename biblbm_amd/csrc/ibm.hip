@@ -2065,6 +2065,7 @@ int bl_particle_add(bl_ibm g, int shape_id, int model, const double* params, int
         return fail(BL_ERR_ARG, "bl_particle_add: vol_desired must be positive");
     const int nv = g->shapes[shape_id].nv;
     p.pos.assign(pos, pos + 3 * (size_t)nv);
+    p.id = (long long)g->parts.size();
     g->parts.push_back(std::move(p));
     if (pid) *pid = (int)g->parts.size() - 1;
     return BL_OK;
@@ -2172,6 +2173,16 @@ int bl_ibm_init(bl_ibm g)
     }
     BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
     BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
+    {  // vertex velocities and forces of particles loaded from a checkpoint
+        long v0 = 0;
+        for (const auto& p : g->parts) {
+            const size_t n = p.pos.size();
+            if (p.vel.size() == n) BL_HIP(hipMemcpy(g->vel + 3 * v0, p.vel.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            if (p.force.size() == n)
+                BL_HIP(hipMemcpy(g->force + 3 * v0, p.force.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            v0 += (long)n / 3;
+        }
+    }
     // tiles
     if ((rc = alloc(g->tile_count, L->ntiles)) || (rc = alloc(g->tile_list, (size_t)L->ntiles * kTileCap)) ||
         (rc = alloc(g->active, 4 * (size_t)L->ntiles)) || (rc = alloc(g->n_active, 16)))

@@ -20,8 +20,10 @@ struct bl_ibm_s {
     std::vector<bl::ShapeHost> shapes;
     struct PartHost {
         int shape = 0, model = 0, bug = 1;
+        long long id = 0;  // the reference's particle id (the index unless loaded from a checkpoint)
         double params[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         std::vector<double> pos;
+        std::vector<double> vel, force;  // from a checkpoint (else zero at init)
     };
     std::vector<PartHost> parts;
     bool inited = false;

@@ -1,6 +1,7 @@
 // Host-side rest geometry and topology of a triangulated membrane shape.
 #pragma once
 
+#include <string>
 #include <vector>
 
 namespace bl {
@@ -17,6 +18,7 @@ struct ShapeHost {
     std::vector<int> pair;          // 4*np: (i0,i1,i2) and (i0,i2,i3) share edge (i0,i2)
     std::vector<double> pair_cs0;   // 2*np: cos, sin of the rest dihedral angle
     double area = 0, volume = 0, center[3] = {0, 0, 0}, radius = 0;
+    std::string tag;  // shape descriptor (the reference's ParticleShape tag)
 
     // Per-vertex incidence lists (CSR), in the order the reference
     // accumulates: links, then triangles, then pairs (element index order).

@@ -9,6 +9,7 @@
 // -> collideAndStream.
 //
 // usage: rbc_in_pipe mesh.msh R L steps out_prefix [second_order] [ibm_boundary] [storage] [interactions]
+//                    [checkpoint_interval] [start_from_checkpoint]
 //   R, L          pipe radius and length (lattice units)
 //   second_order  1 (default): Guo curved wall; 0: bounce-back surface
 //   ibm_boundary  1: the immersed boundary blends and reflects at the pipe
@@ -19,6 +20,13 @@
 //                 strength 1e-4) and Morse particle-particle force (r_min 1,
 //                 beta 1, strength 1e-3), the values of its input.xml
 //                 (read_parameters, :150-178, and main, :735-741); 0 (default): none
+//   checkpoint_interval   > 0: at every multiple of it, write_checkpoint
+//                 (:686-696: parallelIO::save of the lattice to
+//                 checkpoint_lbm, fsi.save_checkpoint("checkpoint_fsi"),
+//                 it.write_checkpoint("checkpoint_time")) and the lightweight
+//                 particle data, in the current folder; 0 (default): none
+//   start_from_checkpoint 1: resume from those files (setup_flow :320-322,
+//                 create_particles :348-351, create_time :677-684)
 // Writes <out_prefix>_final.txt (vertex positions) and
 // <out_prefix>_profile.txt (u_x across the pipe at mid-length).
 #include <cmath>
@@ -62,6 +70,8 @@ int main(int argc, char** argv)
         const bool ibm_boundary = argc > 7 ? std::atoi(argv[7]) != 0 : false;
         const std::string storage = argc > 8 ? argv[8] : "f64";
         const bool interactions = argc > 9 ? std::atoi(argv[9]) != 0 : false;
+        const plint checkpoint_interval = argc > 10 ? std::atol(argv[10]) : 0;
+        const bool load_checkpoint = argc > 11 ? std::atoi(argv[11]) != 0 : false;
 
         blCheck(bl_init(0), "bl_init");
         const T tau = 1.0, u_max = 0.02;
@@ -122,6 +132,7 @@ int main(int argc, char** argv)
             u = Array<T, 3>(0, 0, 0);
         });
         lattice.initialize();
+        if (load_checkpoint) parallelIO::load("checkpoint_lbm", lattice, false);
 
         ImmersedBoundaryDynamics3D<T, DESCRIPTOR> fsi(lattice, shape_library, BL_KERNEL_PESKIN);
         if (ibm_boundary) fsi.set_boundary(&pipe_boundary);
@@ -131,8 +142,11 @@ int main(int argc, char** argv)
             fsi.add_force_decorator(&wall_interaction);
             fsi.add_pp_force(&pp_interaction);
         }
-        fsi.add_particle(&rbc);
+        if (load_checkpoint) fsi.load_checkpoint("checkpoint_fsi");
+        else fsi.add_particle(&rbc);
         fsi.init();
+        Time it(0);
+        if (load_checkpoint) it.read_checkpoint("checkpoint_time");
         std::unique_ptr<MultiTensorField3D<T, 3>> velocity = generateMultiTensorField<T, 3>(domain);
 
         std::cout << "rbc_in_pipe: " << lattice.getNx() << "x" << lattice.getNy() << "x" << lattice.getNz()
@@ -142,7 +156,13 @@ int main(int argc, char** argv)
                                                " wall nodes"
                                          : std::string())
                   << ", storage " << storage << std::endl;
-        for (plint it = 0; it < Nt; ++it) {
+        for (; it < Nt; ++it) {
+            if (checkpoint_interval > 0 && it.is_multiple_of(checkpoint_interval)) {
+                parallelIO::save(lattice, "checkpoint_lbm", false);
+                fsi.save_checkpoint("checkpoint_fsi");
+                it.write_checkpoint("checkpoint_time");
+                fsi.write_lightweight_particle_data(it.to_plint());
+            }
             setExternalVector(lattice, domain, DESCRIPTOR<T>::ExternalField::forceBeginsAt, external_force);
             applyProcessingFunctional(wrap_ibm_dynamics3D(fsi), domain, lattice, *velocity);
             lattice.collideAndStream();

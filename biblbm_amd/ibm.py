@@ -132,6 +132,52 @@ class ImmersedBoundary:
         call("bl_ibm_get_phase", self.handle, ptr(out))
         return out
 
+    # --- on-disk formats ------------------------------------------------------
+    def set_shape_tag(self, sid, tag):
+        call("bl_shape_set_tag", self.handle, int(sid), tag.encode())
+
+    def describe(self, pid):
+        """(shape id, membrane model, the reference's particle id)."""
+        sid, model, pid_ref = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        call("bl_particle_describe", self.handle, int(pid), ctypes.byref(sid), ctypes.byref(model),
+             ctypes.byref(pid_ref))
+        return sid.value, model.value, pid_ref.value
+
+    def pack(self, pid):
+        """ParticleBase3D::pack bytes of one particle (bl_particle_pack)."""
+        n = ctypes.c_longlong()
+        call("bl_particle_pack", self.handle, int(pid), None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value)
+        call("bl_particle_pack", self.handle, int(pid), buf, n.value, ctypes.byref(n))
+        return buf.raw[:n.value]
+
+    def save_checkpoint(self, folder, name="checkpoint_fsi"):
+        """save_checkpoint (src/fsi/ImmersedBoundaryDynamics_io.hh:17-73);
+        creates <folder>/<name>/ for the rank file."""
+        import os
+        os.makedirs(os.path.join(folder, name), exist_ok=True)
+        call("bl_ibm_save_checkpoint", self.handle, folder.encode(), name.encode())
+
+    def load_checkpoint(self, folder, name="checkpoint_fsi"):
+        """load_checkpoint (src/fsi/ImmersedBoundaryDynamics_io.hh:75-158):
+        replaces the particles; call before init()."""
+        call("bl_ibm_load_checkpoint", self.handle, folder.encode(), name.encode())
+        n = ctypes.c_int()
+        call("bl_ibm_count", self.handle, ctypes.byref(n), None)
+        self.particles = []
+        for p in range(n.value):
+            sid = self.describe(p)[0]
+            self.particles.append((sid, self.shapes[sid][0]))
+
+    def write_vtk(self, pid, path):
+        call("bl_particle_write_vtk", self.handle, int(pid), str(path).encode())
+
+    def write_gmsh(self, pid, path):
+        call("bl_particle_write_gmsh", self.handle, int(pid), str(path).encode())
+
+    def write_lightweight(self, path, iteration):
+        call("bl_ibm_write_lightweight", self.handle, str(path).encode(), int(iteration))
+
     def set_fused_staging(self, mode):
         """-1 auto, 0 a staging pass of its own, 1 staging carried by the
         collide-and-stream launch (bl_ibm_set_fused_staging)."""

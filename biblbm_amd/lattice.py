@@ -110,6 +110,13 @@ class Lattice:
         call("bl_lattice_time_collide", self.handle, int(n), ctypes.byref(ms))
         return ms.value
 
+    def save_checkpoint(self, path):
+        """Populations as stored, step count and force field (bl_lattice_save_checkpoint)."""
+        call("bl_lattice_save_checkpoint", self.handle, str(path).encode())
+
+    def load_checkpoint(self, path):
+        call("bl_lattice_load_checkpoint", self.handle, str(path).encode())
+
     def close(self):
         if getattr(self, "handle", None):
             capi.lib.bl_lattice_destroy(self.handle)

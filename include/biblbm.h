@@ -200,6 +200,52 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
  * used.  The staged values are the same either way. */
 int bl_ibm_set_fused_staging(bl_ibm g, int mode);
 
+/* ---------------------------------------------------- on-disk formats */
+/* ParticleShape tag (src/fsi/ParticleShape.h:64,118): the shape descriptor
+ * the lightweight output names. */
+int bl_shape_set_tag(bl_ibm g, int shape_id, const char *tag);
+/* ParticleBase3D::pack of one particle (src/fsi/ParticleBase.hh:218-224,
+ * src/fsi/DeformableParticle3D.hh:277-287, src/fsi/DeformableCapsule.hh:
+ * 217-223, src/fsi/RBCParticle.hh:346-350): type id (capsule 0, RBC 1, the
+ * reference build's registration order), shape id, particle id, should-
+ * voxelize, pos/vel/force per vertex, then the model's parameters.  A model
+ * the reference types cannot hold (3, or a capsule without the c-term quirk)
+ * gets type id 100 and model, flag and 8 parameters.  buf may be null to ask
+ * for the length. */
+int bl_particle_pack(bl_ibm g, int pid, char *buf, long long cap, long long *len);
+/* Shape, membrane model and the reference's particle id (get_id,
+ * src/fsi/ParticleBase.h) of particle index pid: the index for particles
+ * added with bl_particle_add, the stored id for particles loaded from a
+ * checkpoint (kept in increasing id order). */
+int bl_particle_describe(bl_ibm g, int pid, int *shape_id, int *model, long long *id);
+/* ImmersedBoundaryDynamics3D::save_checkpoint / load_checkpoint
+ * (src/fsi/ImmersedBoundaryDynamics_io.hh:17-158) for one rank: header
+ * <folder>/<name>.plb (particle count, 1, "<name>/0.plb") and the rank file
+ * <folder>/<name>/0.plb (plint count, packs); the folder <folder>/<name>
+ * must exist.  Loading replaces the particles (before bl_ibm_init; the
+ * shapes must have been added in the same order) and keeps their vertex
+ * velocities and forces for the upload at init. */
+int bl_ibm_save_checkpoint(bl_ibm g, const char *folder, const char *name);
+int bl_ibm_load_checkpoint(bl_ibm g, const char *folder, const char *name);
+/* ParticleBase3D::write_vtk (src/fsi/ParticleBase.hh:270-398): VTU with
+ * appended raw Float32 points and force_tot, Int32 cells. */
+int bl_particle_write_vtk(bl_ibm g, int pid, const char *path);
+/* ParticleBase3D::write_gmsh (src/fsi/ParticleBase.hh:400-437). */
+int bl_particle_write_gmsh(bl_ibm g, int pid, const char *path);
+/* write_lightweight_particle_data (src/fsi/ImmersedBoundaryDynamics_io.hh:
+ * 178-200, src/fsi/DeformableParticle3D.hh:318-333): "iteration:<it>", then
+ * per particle tag:id:com(3):gyration(xx,xy,xz,yy,yz,zz):volume. */
+int bl_ibm_write_lightweight(bl_ibm g, const char *path, long long iteration);
+/* Lattice state for a restart in this library's own layout (the
+ * reference's parallelIO::save / load of the lattice,
+ * progs/rbc_in_pipe/rbc_in_pipe.cpp:321-322,693, writes Palabos' format,
+ * which is not restated): sizes, storage, step count, the populations as
+ * stored and the external force of every cell.  Loading needs a lattice of
+ * the same size and storage; the continuation is bitwise that of the saved
+ * run. */
+int bl_lattice_save_checkpoint(bl_lattice h, const char *path);
+int bl_lattice_load_checkpoint(bl_lattice h, const char *path);
+
 /* ---------------------------------------------------- curved walls */
 /* GuoRigidWallBoundary(boundary, lattice) (src/fsi/GuoBoundaryCondition.hh:
  * 17-22, instantiator :62-135): the wall nodes are the cells outside the

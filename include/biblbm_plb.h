@@ -31,7 +31,11 @@
  *                                     src/fsi/ForceDecorator.hh:18-35
  *   fsi::ImmersedBoundaryDynamics3D: add_particle, init, get_particle,
  *   set_boundary, add_force_decorator, add_pp_force,
- *   compute_collision_forces, compute_total_volume, write_particles_as_vtk, and the
+ *   compute_collision_forces, compute_total_volume, write_particles_as_vtk,
+ *   write_lightweight_particle_data, save_checkpoint / load_checkpoint,
+ *   fsi::Time and parallelIO::save / load of the lattice
+ *                                     src/fsi/ImmersedBoundaryDynamics_io.hh:17-200,
+ *                                     src/fsi/Time.h:15-48; and the
  *   per-step functional
  *   applyProcessingFunctional(wrap_ibm_dynamics3D(ibm), domain, lattice, u)
  *                                     src/fsi/ImmersedBoundaryDynamics.h:57-114,
@@ -65,6 +69,8 @@
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "biblbm.h"
 
 namespace plb {
@@ -75,6 +81,9 @@ inline void blCheck(int rc, const char* what)
 {
     if (rc != BL_OK) throw std::runtime_error(std::string(what) + ": " + bl_last_error());
 }
+
+// io::mkdir (one level; an existing folder is fine)
+inline void makeDir(const std::string& path) { ::mkdir(path.c_str(), 0755); }
 
 template <typename T, int n>
 struct Array {
@@ -707,6 +716,7 @@ public:
         s->vertices = v;
         s->triangles = t;
         s->tag = tag;
+        if (!shapes_.count(tag)) order_.push_back(tag);
         shapes_[tag] = std::move(s);
         return shapes_[tag].get();
     }
@@ -755,9 +765,13 @@ public:
         return store_mesh(v, t, tag);
     }
     const ParticleShape<T>* get_by_tag(const std::string& tag) const { return shapes_.at(tag).get(); }
+    // shape ids are the storage order (the reference library's get_id)
+    plint count() const { return (plint)order_.size(); }
+    const ParticleShape<T>* get_by_id(plint id) const { return shapes_.at(order_.at((size_t)id)).get(); }
 
 private:
     std::map<std::string, std::unique_ptr<ParticleShape<T>>> shapes_;
+    std::vector<std::string> order_;
 };
 
 template <class T>
@@ -928,9 +942,8 @@ class ImmersedBoundaryDynamics3D {
 public:
     ImmersedBoundaryDynamics3D(MultiBlockLattice3D<T, Descriptor>& lattice, const ParticleShapeLibrary<T>& lib,
                                int kernel = BL_KERNEL_ROMA)
-        : lattice_(lattice), kernel_(kernel)
+        : lattice_(lattice), lib_(lib), kernel_(kernel)
     {
-        (void)lib;
     }
     ~ImmersedBoundaryDynamics3D()
     {
@@ -946,22 +959,57 @@ public:
         parts_.push_back(p);
         return (plint)parts_.size() - 1;
     }
+    // load_checkpoint (src/fsi/ImmersedBoundaryDynamics_io.hh:75-158): the
+    // particles of <name>.plb (and its rank file) replace the added ones at
+    // init(); shape ids refer to the library's storage order
+    void load_checkpoint(const std::string& name)
+    {
+        if (g_) throw std::runtime_error("load_checkpoint: call before init()");
+        checkpoint_ = name;
+    }
+    // save_checkpoint (src/fsi/ImmersedBoundaryDynamics_io.hh:17-73):
+    // <name>.plb and <name>/0.plb
+    void save_checkpoint(const std::string& name)
+    {
+        if (!g_) throw std::runtime_error("save_checkpoint: call init() first");
+        lattice_.flushPendingBoundaryStep();
+        makeDir(name);
+        blCheck(bl_ibm_save_checkpoint(g_, ".", name.c_str()), "save_checkpoint");
+    }
     void init()
     {
         blCheck(bl_ibm_create(lattice_.handle(), kernel_, &g_), "ImmersedBoundaryDynamics3D");
         std::map<const ParticleShape<T>*, int> sid;
+        auto add_shape = [&](const ParticleShape<T>* s) {
+            if (sid.count(s)) return;
+            std::vector<double> v;
+            std::vector<int> t;
+            for (const auto& x : s->vertices) v.insert(v.end(), {x[0], x[1], x[2]});
+            for (const auto& x : s->triangles) t.insert(t.end(), {(int)x[0], (int)x[1], (int)x[2]});
+            int id = -1;
+            blCheck(bl_shape_add(g_, (int)s->vertices.size(), (int)s->triangles.size(), v.data(), t.data(), &id),
+                    "store_mesh");
+            blCheck(bl_shape_set_tag(g_, id, s->tag.c_str()), "store_mesh");
+            sid[s] = id;
+            shape_nv_.push_back((plint)s->vertices.size());
+        };
+        for (plint k = 0; k < lib_.count(); ++k) add_shape(lib_.get_by_id(k));
+        if (!checkpoint_.empty()) {
+            blCheck(bl_ibm_load_checkpoint(g_, ".", checkpoint_.c_str()), "load_checkpoint");
+            int n = 0;
+            blCheck(bl_ibm_count(g_, &n, nullptr), "load_checkpoint");
+            for (int p = 0; p < n; ++p) {
+                int s = -1, model = -1;
+                long long id = -1;
+                blCheck(bl_particle_describe(g_, p, &s, &model, &id), "load_checkpoint");
+                nv_.push_back(shape_nv_.at((size_t)s));
+            }
+            parts_.clear();
+        }
         for (const ParticleBase3D<T>* p : parts_) {
             const ParticleShape<T>* s = p->get_shape();
-            if (!sid.count(s)) {
-                std::vector<double> v;
-                std::vector<int> t;
-                for (const auto& x : s->vertices) v.insert(v.end(), {x[0], x[1], x[2]});
-                for (const auto& x : s->triangles) t.insert(t.end(), {(int)x[0], (int)x[1], (int)x[2]});
-                int id = -1;
-                blCheck(bl_shape_add(g_, (int)s->vertices.size(), (int)s->triangles.size(), v.data(), t.data(), &id),
-                        "store_mesh");
-                sid[s] = id;
-            }
+            add_shape(s);
+            nv_.push_back(p->count_nodes());
             std::vector<double> pos;
             for (const auto& x : p->positions()) pos.insert(pos.end(), {x[0], x[1], x[2]});
             const std::vector<double> prm = p->params();
@@ -1009,11 +1057,11 @@ public:
     // current vertex forces of particle i into a host copy
     bool get_forces(plint i, std::vector<Array<T, 3>>& f)
     {
-        if (i < 0 || i >= (plint)parts_.size()) return false;
+        if (i < 0 || i >= (plint)nv_.size()) return false;
         lattice_.flushPendingBoundaryStep();
-        std::vector<double> buf(3 * parts_[i]->count_nodes());
+        std::vector<double> buf(3 * nv_[i]);
         blCheck(bl_particle_get(g_, (int)i, 2, buf.data()), "get_forces");
-        f.resize(parts_[i]->count_nodes());
+        f.resize(nv_[i]);
         for (size_t k = 0; k < f.size(); ++k) f[k] = Array<T, 3>(buf[3 * k], buf[3 * k + 1], buf[3 * k + 2]);
         return true;
     }
@@ -1036,11 +1084,11 @@ public:
     // current vertices of particle i into a host copy (src/fsi/ImmersedBoundaryDynamics.h:95)
     bool get_particle(plint i, std::vector<Array<T, 3>>& pos)
     {
-        if (i < 0 || i >= (plint)parts_.size()) return false;
+        if (i < 0 || i >= (plint)nv_.size()) return false;
         lattice_.flushPendingBoundaryStep();
-        std::vector<double> buf(3 * parts_[i]->count_nodes());
+        std::vector<double> buf(3 * nv_[i]);
         blCheck(bl_particle_get(g_, (int)i, 0, buf.data()), "get_particle");
-        pos.resize(parts_[i]->count_nodes());
+        pos.resize(nv_[i]);
         for (size_t k = 0; k < pos.size(); ++k) pos[k] = Array<T, 3>(buf[3 * k], buf[3 * k + 1], buf[3 * k + 2]);
         return true;
     }
@@ -1061,42 +1109,46 @@ public:
         T V = 0, a;
         Array<T, 3> c;
         T I[9];
-        for (plint i = 0; i < (plint)parts_.size(); ++i) {
+        for (plint i = 0; i < (plint)nv_.size(); ++i) {
             T v;
             observables(i, a, v, c, I);
             V += v;
         }
         return V;
     }
-    // legacy-VTK polydata of every particle's surface
-    void write_particles_as_vtk(plint it, const std::string& prefix = "particles_")
+    plint count_particles() const { return (plint)nv_.size(); }
+    // write_particles_as_vtk (src/fsi/ImmersedBoundaryDynamics_io.hh:160-176):
+    // obj<id>-<it>.vtu per particle in the folder `dir` (the reference's
+    // output directory)
+    void write_particles_as_vtk(plint it, const std::string& dir = "")
     {
+        lattice_.flushPendingBoundaryStep();
+        for (plint i = 0; i < (plint)nv_.size(); ++i) {
+            int s = -1, model = -1;
+            long long id = -1;
+            blCheck(bl_particle_describe(g_, (int)i, &s, &model, &id), "write_particles_as_vtk");
+            std::ostringstream name;
+            name << dir << "obj" << id << "-" << it << ".vtu";
+            blCheck(bl_particle_write_vtk(g_, (int)i, name.str().c_str()), "write_particles_as_vtk");
+        }
+    }
+    // write_lightweight_particle_data (src/fsi/ImmersedBoundaryDynamics_io.hh:
+    // 178-200): objsP0-<it>.txt in `dir`
+    void write_lightweight_particle_data(plint it, const std::string& dir = "")
+    {
+        lattice_.flushPendingBoundaryStep();
         std::ostringstream name;
-        name << prefix << it << ".vtk";
-        std::ofstream out(name.str().c_str());
-        std::vector<std::vector<Array<T, 3>>> all(parts_.size());
-        plint nv = 0, nt = 0;
-        for (size_t i = 0; i < parts_.size(); ++i) {
-            get_particle((plint)i, all[i]);
-            nv += (plint)all[i].size();
-            nt += (plint)parts_[i]->get_shape()->triangles.size();
-        }
-        out << "# vtk DataFile Version 3.0\nparticles\nASCII\nDATASET POLYDATA\nPOINTS " << nv << " double\n";
-        for (const auto& p : all)
-            for (const auto& x : p) out << x[0] << " " << x[1] << " " << x[2] << "\n";
-        out << "POLYGONS " << nt << " " << 4 * nt << "\n";
-        plint off = 0;
-        for (size_t i = 0; i < parts_.size(); ++i) {
-            for (const auto& t : parts_[i]->get_shape()->triangles)
-                out << "3 " << off + t[0] << " " << off + t[1] << " " << off + t[2] << "\n";
-            off += (plint)all[i].size();
-        }
+        name << dir << "objsP0-" << it << ".txt";
+        blCheck(bl_ibm_write_lightweight(g_, name.str().c_str(), (long long)it), "write_lightweight_particle_data");
     }
     void schedule_step() { lattice_.setPendingBoundaryStep(g_); }
 
 private:
     MultiBlockLattice3D<T, Descriptor>& lattice_;
+    const ParticleShapeLibrary<T>& lib_;
     int kernel_;
+    std::string checkpoint_;
+    std::vector<plint> shape_nv_, nv_;  // vertices per shape id / per particle
     void push_decorator(const ForceDecorator<T>* d)
     {
         bl_boundary b;
@@ -1145,6 +1197,56 @@ void applyProcessingFunctional(fsi::ImmersedBoundaryWrapperFunctional3D<T, Descr
     std::unique_ptr<fsi::ImmersedBoundaryWrapperFunctional3D<T, Descriptor, Periodicity>> own(f);
     own->ibm.schedule_step();
 }
+
+// parallelIO::save / load of the lattice (progs/rbc_in_pipe/rbc_in_pipe.cpp:
+// 321-322,693): <name>.blc in this library's checkpoint layout (populations
+// as stored, step count, force field), not Palabos' serializer format.
+namespace parallelIO {
+template <typename T, template <typename U> class Descriptor>
+void save(MultiBlockLattice3D<T, Descriptor>& lattice, const std::string& name, bool = false)
+{
+    lattice.flushPendingBoundaryStep();
+    blCheck(bl_lattice_save_checkpoint(lattice.handle(), (name + ".blc").c_str()), "parallelIO::save");
+}
+template <typename T, template <typename U> class Descriptor>
+void load(const std::string& name, MultiBlockLattice3D<T, Descriptor>& lattice, bool = false)
+{
+    blCheck(bl_lattice_load_checkpoint(lattice.handle(), (name + ".blc").c_str()), "parallelIO::load");
+}
+}  // namespace parallelIO
+
+namespace fsi {
+// Iteration counter with its checkpoint (src/fsi/Time.h:15-48).
+class Time {
+public:
+    Time() : it_(0) {}
+    Time(plint it) : it_(it) {}
+    void write_checkpoint(const std::string& file) const
+    {
+        std::ofstream out(file.c_str());
+        out << it_;
+    }
+    void read_checkpoint(const std::string& file)
+    {
+        std::ifstream in(file.c_str());
+        if (!(in >> it_)) throw std::runtime_error("Time::read_checkpoint: cannot read " + file);
+    }
+    bool is_multiple_of(plint n) const { return (it_ % n) == 0; }
+    bool operator<(plint it) const { return it_ < it; }
+    bool operator<=(plint it) const { return it_ <= it; }
+    bool operator==(plint it) const { return it_ == it; }
+    bool operator!=(plint it) const { return it_ != it; }
+    Time& operator++()
+    {
+        ++it_;
+        return *this;
+    }
+    plint to_plint() const { return it_; }
+
+private:
+    plint it_;
+};
+}  // namespace fsi
 
 }  // namespace plb
 

@@ -144,3 +144,32 @@ def test_rbc_in_pipe_prog_with_interactions(device, tmp_path):
     ib.close()
     W.close()
     L.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16s"])
+def test_rbc_in_pipe_prog_restarts_from_checkpoint(device, tmp_path, storage):
+    """The prog's checkpoint flow (write_checkpoint every interval:
+    parallelIO::save, fsi.save_checkpoint, Time::write_checkpoint, and the
+    lightweight particle data; then a run started from those files): the
+    resumed run ends on the uninterrupted run's vertex positions and
+    velocity profile, to the bit."""
+    from biblbm_amd import meshes
+    v, t = meshes.biconcave(2, 3.5)
+    msh = tmp_path / "rbc.msh"
+    meshes.write_gmsh(str(msh), v, t)
+    exe = os.path.join(ROOT, "biblbm_amd", "examples", "rbc_in_pipe")
+    args = [exe, str(msh), "8", "16", "13"]
+    full = subprocess.run(args + ["full", "1", "1", storage, "0", "5", "0"], cwd=tmp_path, capture_output=True,
+                          text=True, timeout=120)
+    print(full.stdout, full.stderr)
+    assert full.returncode == 0
+    assert open(tmp_path / "checkpoint_time").read().strip() == "10"
+    assert (tmp_path / "checkpoint_fsi.plb").read_text().split() == ["1", "1", "checkpoint_fsi/0.plb"]
+    lw = (tmp_path / "objsP0-10.txt").read_text().split("\n")
+    assert lw[0] == "iteration:10" and lw[1].startswith("rbc:0:")
+    res = subprocess.run(args + ["resumed", "1", "1", storage, "0", "5", "1"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    for suffix in ("_final.txt", "_profile.txt"):
+        assert (tmp_path / ("full" + suffix)).read_text() == (tmp_path / ("resumed" + suffix)).read_text()
