@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbiblbm.so")
 
-SOURCES = ["errors.cpp", "lattice.hip", "ibm.hip", "mesh.cpp", "ensemble.hip", "walls.hip", "interactions.hip", "io.cpp"]
+SOURCES = ["errors.cpp", "lattice.hip", "ibm.hip", "mesh.cpp", "ensemble.hip", "walls.hip", "interactions.hip", "io.cpp", "init.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -71,6 +71,11 @@ SIGNATURES = {
     "bl_ibm_tile_stats": [_P, _P],
     "bl_ibm_set_fused_staging": [_P, _I],
     "bl_shape_set_tag": [_P, _I, ctypes.c_char_p],
+    "bl_rbc_shrink_volume": [_P, _I, _D, _I, _D, _P],
+    "bl_positions_create": [_P, _P, _D, ctypes.POINTER(_P)],
+    "bl_positions_generate": [_P, ctypes.c_longlong, _D, ctypes.c_char],
+    "bl_positions_get": [_P, ctypes.c_char, _P, ctypes.c_longlong, _PLL],
+    "bl_positions_destroy": [_P],
     "bl_particle_pack": [_P, _I, _P, ctypes.c_longlong, _PLL],
     "bl_particle_describe": [_P, _I, _PI, _PI, _PLL],
     "bl_ibm_save_checkpoint": [_P, ctypes.c_char_p, ctypes.c_char_p],

@@ -339,7 +339,23 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     StencilTiles st, stt;
     int e0 = -1, e1 = -1;  // list entries of the kernel part and the top-hat part (-1: none)
-    if (v < NV) {
+    // a vertex whose position is not finite (a membrane that blew up) or
+    // absurdly far off touches no tile: zero weights at base 0, reported
+    bool off = false;
+    if (v < NV)
+        for (int a = 0; a < 3; ++a) off = off || !(fabs(pos[3 * v + a]) < 1e9);
+    if (v < NV && off) {
+        for (int a = 0; a < 3; ++a) {
+            base[3 * v + a] = 0;
+            for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = 0.0;
+        }
+        if (nw.on) {
+            nw.near[v] = 0;
+            nw.blend[v] = 1.0;
+        }
+        atomicOr(&status[2], 4);
+    }
+    if (v < NV && !off) {
         int b[3];
         for (int a = 0; a < 3; ++a) {
             double w[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1203,6 +1219,67 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid g
                    cap, list, base, W, vforce, nw, Fout);
 }
 
+// ------------------------------------------------------- calibration
+// shrink_rbc_volume (progs/common/calibrate_rbc_shape.h:13-72) with
+// RBCParticle::relax_nodes (src/fsi/RBCParticle.hh:318-342) as its step,
+// for every selected particle at once.  Node mass: area / 3 of each
+// incident triangle (tri::triangle_area, src/fsi/TriangleUtils.h:52-55),
+// summed in triangle order (the vertex's incidence list is in element
+// order).  The step counter lives on the device so that the launches of
+// any step are the same.
+__global__ void __launch_bounds__(256) k_calib_mass(ForceView f, long NV, const int* __restrict__ active,
+                                                    double* __restrict__ mass, int* __restrict__ step)
+{
+#pragma clang fp contract(off)
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0) *step += 1;
+    if (v >= NV) return;
+    const int p = f.v_part[v];
+    if (!active[p]) return;
+    const int s = f.p_shape[p];
+    const long vo = f.p_voff[p];
+    const long sv = f.s_voff[s] + (v - vo);
+    const int* ht = f.s_vt_head + sv + s;
+    const int* Lt = f.s_vt + f.s_vtoff[s] + ht[0];
+    const int nt = ht[1] - ht[0];
+    double M = 0.0;
+    for (int k = 0; k < nt; ++k) {
+        const long st = f.s_toff[s] + (Lt[k] >> 2);
+        const D3 v0 = ld3(f.pos + 3 * (vo + f.s_tri[3 * st])), v1 = ld3(f.pos + 3 * (vo + f.s_tri[3 * st + 1])),
+                 v2 = ld3(f.pos + 3 * (vo + f.s_tri[3 * st + 2]));
+        M += (0.5 * norm(cross(v1 - v0, v2 - v0))) / 3.;
+    }
+    mass[v] = M;
+}
+
+// One relax step of the selected particles: vel += F / M - damping vel,
+// pos += vel; the first vertex of each sets vol_desired of this step (the
+// ramp from vol_start to vol_final over the first n_it steps).
+__global__ void __launch_bounds__(256) k_calib_relax(long NV, const int* __restrict__ v_part,
+                                                     const long* __restrict__ p_voff, const int* __restrict__ active,
+                                                     const double* __restrict__ damping,
+                                                     const double* __restrict__ vol_start,
+                                                     const double* __restrict__ vol_final, int n_it,
+                                                     const int* __restrict__ step, const double* __restrict__ mass,
+                                                     const double* __restrict__ force, double* __restrict__ vel,
+                                                     double* __restrict__ pos, double* __restrict__ params)
+{
+#pragma clang fp contract(off)
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NV) return;
+    const int p = v_part[v];
+    if (!active[p]) return;
+    const int it = *step - 1;
+    if (v == p_voff[p] && it <= n_it)
+        params[8 * p + 5] = vol_start[p] + ((double)it / (double)n_it) * (vol_final[p] - vol_start[p]);
+    const double M = mass[v], d = damping[p];
+    for (int k = 0; k < 3; ++k) {
+        const double u = vel[3 * v + k] + (force[3 * v + k] / M - d * vel[3 * v + k]);
+        vel[3 * v + k] = u;
+        pos[3 * v + k] = pos[3 * v + k] + u;
+    }
+}
+
 // --------------------------------------------------------------- motion
 __global__ void k_move(long NV, double* __restrict__ pos, const double* __restrict__ vel)
 {
@@ -1335,8 +1412,13 @@ __global__ void k_ray_hits(ForceView f, Grid gd, long NT, int pass, int* col_cou
     const long vo = f.p_voff[p];
     const int* tri = f.s_tri + 3 * (f.s_toff[s] + (g - f.p_toff[p]));
     const D3 a = ld3(f.pos + 3 * (vo + tri[0])), b = ld3(f.pos + 3 * (vo + tri[1])), c = ld3(f.pos + 3 * (vo + tri[2]));
+    // a triangle with a non-finite corner, or wider than the lattice, is
+    // not traced (its membrane diverged; the binning reports it)
+    const double ext = fmax(fmax(fabs(a.x), fabs(b.x)), fmax(fmax(fabs(c.x), fabs(a.y)), fmax(fabs(b.y), fabs(c.y))));
+    if (!(ext < 1e9)) return;
     const long x0 = (long)ceil(fmin(a.x, fmin(b.x, c.x))), x1 = (long)floor(fmax(a.x, fmax(b.x, c.x)));
     const long y0 = (long)ceil(fmin(a.y, fmin(b.y, c.y))), y1 = (long)floor(fmax(a.y, fmax(b.y, c.y)));
+    if (x1 - x0 > (long)gd.n[0] + 4 || y1 - y0 > (long)gd.n[1] + 4) return;
     const D3 e1 = b - a, e2 = c - a;
     const D3 dir = {0, 0, 1};
     const D3 q = cross(dir, e2);
@@ -1808,6 +1890,8 @@ int check_overflow(bl_ibm g)
         return fail(BL_ERR_CAPACITY, "immersed boundary: more than " + std::to_string(kTileCap) +
                                          " stencils touch one 4x4x4 tile");
     if (st[2] & 2) return fail(BL_ERR_CAPACITY, "voxelization: hit buffer capacity exceeded");
+    if (st[2] & 4)
+        return fail(BL_ERR_STATE, "immersed boundary: a vertex position is not finite (the membrane diverged)");
     return BL_OK;
 }
 
@@ -2392,6 +2476,133 @@ int bl_particle_set_positions(bl_ibm g, int pid, const double* pos)
     rc = update_particles(g, 0);
     if (rc) return rc;
     BL_HIP(hipStreamSynchronize(g->lat->stream));
+    return BL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Membrane forces alone (RBCParticle::compute_forces): no decorators, no
+// particle-particle forces.
+int membrane_forces(bl_ibm g, hipStream_t s)
+{
+    const ForceView f = force_view(g);
+    const long ne = g->NT + g->NP + g->NL;
+    DecoSet none;
+    none.n = 0;
+    hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair, g->ec_link,
+                       g->ec);
+    hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
+                       g->ec_link, (const double*)g->ec, g->force, none, (const double*)nullptr);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { hipFree(p); }
+    int alloc(size_t n)
+    {
+        BL_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)));
+        return BL_OK;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int bl_rbc_shrink_volume(bl_ibm g, int pid, double vol_final, int n_it, double damping, double* damping_out)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (pid < -1 || pid >= g->np) return fail(BL_ERR_ARG, "bl_rbc_shrink_volume: bad particle id");
+    if (n_it < 1 || !(damping > 0) || !(vol_final > 0))
+        return fail(BL_ERR_ARG, "bl_rbc_shrink_volume: need n_it >= 1, damping > 0, vol_final > 0");
+    const int np = g->np;
+    std::vector<int> todo(np, 0);
+    std::vector<double> damp(np, damping), v0(np, 0.0), v1(np, vol_final);
+    std::vector<long> voff(np + 1, 0);
+    for (int p = 0; p < np; ++p) {
+        const auto& P = g->parts[p];
+        voff[p + 1] = voff[p] + g->shapes[P.shape].nv;
+        v0[p] = g->shapes[P.shape].volume;  // the rest shape's volume
+        const bool rbc = P.model == BL_MODEL_RBC || P.model == 3;
+        if (pid == p && !rbc) return fail(BL_ERR_ARG, "bl_rbc_shrink_volume: the particle has no volume constraint");
+        todo[p] = (pid == -1 || pid == p) && rbc;
+    }
+    hipStream_t s = g->lat->stream;
+    const size_t nv3 = 3 * (size_t)g->NV;
+    DevBuf<int> d_active, d_step;
+    DevBuf<double> d_damp, d_v0, d_v1, d_mass, pos0, vel0, frc0, prm0;
+    if ((rc = d_active.alloc(np)) || (rc = d_step.alloc(1)) || (rc = d_damp.alloc(np)) || (rc = d_v0.alloc(np)) ||
+        (rc = d_v1.alloc(np)) || (rc = d_mass.alloc(g->NV)) || (rc = pos0.alloc(nv3)) || (rc = vel0.alloc(nv3)) ||
+        (rc = frc0.alloc(nv3)) || (rc = prm0.alloc(8 * (size_t)np)))
+        return rc;
+    BL_HIP(hipMemcpyAsync(d_v0.p, v0.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(d_v1.p, v1.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+    // the starting state: geometry and forces of the current positions with
+    // the current parameters (rbc.compute_forces() before the loop)
+    if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
+    BL_HIP(hipMemcpyAsync(pos0.p, g->pos, sizeof(double) * nv3, hipMemcpyDeviceToDevice, s));
+    BL_HIP(hipMemcpyAsync(vel0.p, g->vel, sizeof(double) * nv3, hipMemcpyDeviceToDevice, s));
+    BL_HIP(hipMemcpyAsync(frc0.p, g->force, sizeof(double) * nv3, hipMemcpyDeviceToDevice, s));
+    BL_HIP(hipMemcpyAsync(prm0.p, g->p_params, sizeof(double) * 8 * np, hipMemcpyDeviceToDevice, s));
+    const ForceView f = force_view(g);
+    for (;;) {
+        bool any = false;
+        for (int p = 0; p < np; ++p) any = any || todo[p];
+        if (!any) break;
+        BL_HIP(hipMemcpyAsync(d_active.p, todo.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+        BL_HIP(hipMemcpyAsync(d_damp.p, damp.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+        BL_HIP(hipMemsetAsync(d_step.p, 0, sizeof(int), s));
+        for (int it = 0; it <= 2 * n_it; ++it) {
+            hipLaunchKernelGGL(k_calib_mass, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, (const int*)d_active.p,
+                               d_mass.p, d_step.p);
+            hipLaunchKernelGGL(k_calib_relax, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV, (const int*)g->v_part,
+                               (const long*)g->p_voff, (const int*)d_active.p, (const double*)d_damp.p,
+                               (const double*)d_v0.p, (const double*)d_v1.p, n_it, (const int*)d_step.p,
+                               (const double*)d_mass.p, (const double*)g->force, g->vel, g->pos, g->p_params);
+            BL_CHECK_LAUNCH();
+            if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
+        }
+        // a particle whose area went NaN starts over with twice the damping
+        std::vector<double> geom(11 * (size_t)np);
+        BL_HIP(hipMemcpyAsync(geom.data(), g->p_geom, sizeof(double) * geom.size(), hipMemcpyDeviceToHost, s));
+        BL_HIP(hipStreamSynchronize(s));
+        for (int p = 0; p < np; ++p) {
+            if (!todo[p]) continue;
+            const double area = geom[11 * p];
+            if (area == area) {
+                todo[p] = 0;
+                continue;
+            }
+            damp[p] *= 2.0;
+            const size_t a = 3 * (size_t)voff[p], n = 3 * (size_t)(voff[p + 1] - voff[p]);
+            BL_HIP(hipMemcpyAsync(g->pos + a, pos0.p + a, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            BL_HIP(hipMemcpyAsync(g->vel + a, vel0.p + a, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            BL_HIP(hipMemcpyAsync(g->force + a, frc0.p + a, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            BL_HIP(hipMemcpyAsync(g->p_params + 8 * p, prm0.p + 8 * p, sizeof(double) * 8, hipMemcpyDeviceToDevice, s));
+            if (damp[p] > 2.0) {  // left in its starting state
+                if ((rc = update_particles(g, 0, s))) return rc;
+                BL_HIP(hipStreamSynchronize(s));
+                return fail(BL_ERR_STATE, "bl_rbc_shrink_volume: particle " + std::to_string(p) +
+                                              " diverges for every damping up to 2");
+            }
+        }
+        // restored particles need their geometry back before the next attempt
+        if ((rc = update_particles(g, 0, s))) return rc;
+    }
+    // the calibrated vol_desired stays with the particles (host copy too)
+    std::vector<double> prm(8 * (size_t)np);
+    BL_HIP(hipMemcpyAsync(prm.data(), g->p_params, sizeof(double) * prm.size(), hipMemcpyDeviceToHost, s));
+    BL_HIP(hipStreamSynchronize(s));
+    for (int p = 0; p < np; ++p) {
+        for (int k = 0; k < 8; ++k) g->parts[p].params[k] = prm[8 * p + k];
+        if (damping_out) damping_out[p] = damp[p];
+    }
+    g->binned = false;
+    ibm_invalidate_graphs(g);
     return BL_OK;
 }
 

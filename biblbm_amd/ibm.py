@@ -132,6 +132,15 @@ class ImmersedBoundary:
         call("bl_ibm_get_phase", self.handle, ptr(out))
         return out
 
+    # --- initial conditions ----------------------------------------------------
+    def shrink_rbc_volume(self, vol_final, n_it, damping=1e-2, pid=-1):
+        """shrink_rbc_volume (progs/common/calibrate_rbc_shape.h:13-72) on the
+        device for particle pid or every volume-constrained particle; returns
+        the damping each particle finished with."""
+        out = np.zeros(max(len(self.particles), 1))
+        call("bl_rbc_shrink_volume", self.handle, int(pid), float(vol_final), int(n_it), float(damping), ptr(out))
+        return out[:len(self.particles)]
+
     # --- on-disk formats ------------------------------------------------------
     def set_shape_tag(self, sid, tag):
         call("bl_shape_set_tag", self.handle, int(sid), tag.encode())

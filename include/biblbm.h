@@ -57,6 +57,7 @@ typedef struct bl_lattice_s *bl_lattice;
 typedef struct bl_ibm_s *bl_ibm;
 typedef struct bl_ensemble_s *bl_ensemble;
 typedef struct bl_guo_wall_s *bl_guo_wall;
+typedef struct bl_positions_s *bl_positions;
 
 /* Analytic boundaries (src/fsi/Boundary.h): PipeBoundary(y0, z0, radius,
  * length) along x (:132-186), ParallelPlatesBoundary(y0, y1) normal to y
@@ -199,6 +200,34 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
  * phase field only (omega_in == omega_out); otherwise the separate pass is
  * used.  The staged values are the same either way. */
 int bl_ibm_set_fused_staging(bl_ibm g, int mode);
+
+/* ------------------------------------------------ initial conditions */
+/* shrink_rbc_volume (progs/common/calibrate_rbc_shape.h:13-72) on the
+ * device, for particle pid or (pid = -1) every particle with a volume
+ * constraint (models 2 and 3) at once: vol_desired ramps from the rest
+ * shape's volume to vol_final over n_it steps of RBCParticle::relax_nodes
+ * (src/fsi/RBCParticle.hh:318-342: node mass = area/3 of the incident
+ * triangles, vel += F/M - damping vel, pos += vel, membrane forces), 2 n_it
+ * + 1 steps in all; a particle whose area turns NaN starts again from its
+ * starting state with twice the damping (an error once the damping would
+ * exceed 2).  The particles keep the calibrated positions, velocities,
+ * forces and vol_desired; damping_out (np entries, may be null) receives
+ * the damping each particle finished with.  Needs bl_ibm_init; the lattice
+ * is not touched. */
+int bl_rbc_shrink_volume(bl_ibm g, int pid, double vol_final, int n_it, double damping, double *damping_out);
+/* ParticlePositionInitializer (src/fsi/ParticlePositionInitializer.h:19-37,
+ * .hh:18-86), host only: spheres placed one after another in the box
+ * (x0, x1, y0, y1, z0, z1), inside the boundary (may be null) and apart from
+ * every sphere placed before, drawn with the C library's rand() as the
+ * reference draws them (seed it with srand for a given sequence).
+ * generate fails with BL_ERR_ARG when 2 radius exceeds 2 max_radius, with
+ * BL_ERR_CAPACITY when a point finds no room in 5e6 draws (the points
+ * placed so far stay); get returns the points of one tag in placement
+ * order (out may be null to ask for the count). */
+int bl_positions_create(const int box[6], const bl_boundary *boundary, double max_radius, bl_positions *out);
+int bl_positions_generate(bl_positions h, long long n, double radius, char tag);
+int bl_positions_get(bl_positions h, char tag, double *out, long long cap, long long *n);
+int bl_positions_destroy(bl_positions h);
 
 /* ---------------------------------------------------- on-disk formats */
 /* ParticleShape tag (src/fsi/ParticleShape.h:64,118): the shape descriptor

@@ -471,3 +471,118 @@ def coupled_step(f, force, phase, mask, periodic, omega, omega_in, body, particl
         p.move(v, shape, periodic, boundary)
     f = lbm_run(f, new_force, phase, mask, periodic, omega, omega_in, 1, wall_axis, wall_u, guo=guo)
     return f, new_force, vels
+
+
+# ------------------------------------------------- initial conditions
+def create_rbc_params(rest, tri, shear_modulus, K_area, K_bend):
+    """create_rbc_params (progs/common/calibrate_rbc_shape.h:75-112), in the
+    library's parameter order [shear_modulus, L0, k_area_global,
+    k_area_local, k_volume, vol_desired, k_bend, theta0]: L0 the mean link
+    length, vol_desired the rest volume, the area stiffness beyond
+    K() = 2 G (src/fsi/RBCParticle.hh:76-79) split 0.8 / 0.2 between global
+    and local, k_volume = 1000 G / radius (largest vertex distance from the
+    origin), theta0 of a regular icosahedral triangulation."""
+    rest = np.asarray(rest, float)
+    topo = build_topology(tri, rest)
+    lavg = topo["link_len"].sum() / len(topo["link_len"])
+    vol = particle_update(tri, rest)["volume"]
+    delta = K_area - 2.0 * shear_modulus
+    radius = np.sqrt((rest ** 2).sum(1)).max()
+    nv = len(rest)
+    theta0 = np.arccos((np.sqrt(3.0) * (nv - 2.0) - 5 * np.pi) / (np.sqrt(3.0) * (nv - 2.0) - 3.0 * np.pi))
+    return np.array([shear_modulus, lavg, delta * 0.8, delta * 0.2, 1000.0 * shear_modulus / radius, vol, K_bend,
+                     theta0])
+
+
+def node_masses(tri, pos):
+    """Area / 3 of every incident triangle, summed per vertex in triangle
+    order (RBCParticle::relax_nodes, src/fsi/RBCParticle.hh:322-330;
+    tri::triangle_area, src/fsi/TriangleUtils.h:52-55)."""
+    M = np.zeros(len(pos))
+    for t in np.asarray(tri):
+        a = 0.5 * np.linalg.norm(np.cross(pos[t[1]] - pos[t[0]], pos[t[2]] - pos[t[0]]))
+        for k in t:
+            M[k] += a / 3.0
+    return M
+
+
+def shrink_rbc_volume(p, vol_final, n_it, damping=1e-2, max_damping=2.0):
+    """shrink_rbc_volume (progs/common/calibrate_rbc_shape.h:13-72) on an
+    oracle Particle of model 2: from the rest-shape volume, vol_desired
+    ramps to vol_final over n_it relaxation steps, then 2 n_it + 1 steps
+    in all; each step is relax_nodes (src/fsi/RBCParticle.hh:318-342):
+    vel += F / M - damping vel, pos += vel, geometry update, new forces.
+    A step whose area is NaN restarts the whole calibration from the
+    starting state with twice the damping.  Returns the final damping; the
+    particle ends in the calibrated state, its velocities in p.vel and
+    forces in p.force.  The reference retries without end; here a damping
+    past max_damping raises (the step is unstable for any damping above 2)."""
+    vol_start = particle_update(p.tri, p.rest)["volume"]
+    p0, prm0 = p.pos.copy(), p.params.copy()
+    F0 = p.forces()
+    vel0 = np.zeros_like(p.pos)
+    while True:
+        p.pos, p.params = p0.copy(), prm0.copy()
+        g = particle_update(p.tri, p.pos)
+        p.area, p.volume = g["area"], g["volume"]
+        vel, F = vel0.copy(), F0.copy()
+        ok = True
+        for it in range(2 * n_it + 1):
+            if it <= n_it:
+                p.params[5] = vol_start + (float(it) / float(n_it)) * (vol_final - vol_start)
+            M = node_masses(p.tri, p.pos)
+            vel = vel + (F / M[:, None] - damping * vel)
+            p.pos = p.pos + vel
+            g = particle_update(p.tri, p.pos)
+            p.area, p.volume, p.com = g["area"], g["volume"], g["com"]
+            F = p.forces()
+            if p.area != p.area:
+                damping *= 2.0
+                ok = False
+                if damping > max_damping:
+                    raise FloatingPointError("shrink_rbc_volume: diverges for every damping up to %g" % max_damping)
+                break
+        if ok:
+            p.vel, p.force = vel, F
+            return damping
+
+
+def place_spheres(box, boundary, max_radius, batches, seed=None):
+    """ParticlePositionInitializer (src/fsi/ParticlePositionInitializer.hh:
+    18-86) with the C library's rand(): per batch (count, radius), each
+    point is drawn as box corner + radius + rand()/RAND_MAX * (extent -
+    2 radius) per axis (x, y, z), until it lies inside `boundary` (margin
+    = radius) and overlaps no earlier sphere (distance^2 < (r1 + r2)^2);
+    5e6 draws at most per point.  box = (x0, x1, y0, y1, z0, z1) in lattice
+    cells.  Returns one (n, 3) array per batch, or None for a batch that
+    could not be placed."""
+    libc = ctypes.CDLL(None)
+    if seed is not None:
+        libc.srand(ctypes.c_uint(seed))
+    RAND_MAX = 2147483647.0
+    pts, out = [], []
+    for count, radius in batches:
+        if 2 * radius > 2 * max_radius:
+            out.append(None)
+            continue
+        got = []
+        for _ in range(count):
+            placed = False
+            for _ in range(int(5e6)):
+                c = []
+                for a in range(3):
+                    lo, hi = box[2 * a], box[2 * a + 1]
+                    c.append(lo + radius + (libc.rand() / RAND_MAX) * ((hi - lo) - 2 * radius))
+                c = np.array(c)
+                if boundary is not None and not lib.or_boundary_contains(ctypes.byref(boundary), _p(c), radius):
+                    continue
+                if any(((q - c) ** 2).sum() < (r + radius) ** 2 for q, r in pts):
+                    continue
+                placed = True
+                break
+            if not placed:
+                break
+            pts.append((c, radius))
+            got.append(c)
+        out.append(np.array(got) if len(got) == count else None)
+    return out
