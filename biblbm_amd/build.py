@@ -61,7 +61,7 @@ def build(verbose=False, jobs=None):
     build_examples()
 
 
-EXAMPLES = ["capsule_in_shear", "rbc_in_pipe"]
+EXAMPLES = ["capsule_in_shear", "rbc_in_pipe", "initial_condition"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 

@@ -35,7 +35,11 @@
  *   write_lightweight_particle_data, save_checkpoint / load_checkpoint,
  *   fsi::Time and parallelIO::save / load of the lattice
  *                                     src/fsi/ImmersedBoundaryDynamics_io.hh:17-200,
- *                                     src/fsi/Time.h:15-48; and the
+ *                                     src/fsi/Time.h:15-48;
+ *   create_rbc_params, shrink_rbc_volume (on the device),
+ *   fsi::ParticlePositionInitializer
+ *                                     progs/common/calibrate_rbc_shape.h:13-112,
+ *                                     src/fsi/ParticlePositionInitializer.hh:18-86; and the
  *   per-step functional
  *   applyProcessingFunctional(wrap_ibm_dynamics3D(ibm), domain, lattice, u)
  *                                     src/fsi/ImmersedBoundaryDynamics.h:57-114,
@@ -76,6 +80,7 @@
 namespace plb {
 
 typedef long plint;
+typedef unsigned long pluint;
 
 inline void blCheck(int rc, const char* what)
 {
@@ -525,6 +530,42 @@ private:
 
 // Interaction potentials (src/fsi/Potentials.h:28-72): the force at a
 // distance and the cut-off; desc() is what crosses the C-ABI.
+// ParticlePositionInitializer (src/fsi/ParticlePositionInitializer.h:19-37,
+// .hh:18-86): spheres drawn with the C library's rand() inside the box and
+// the boundary, apart from every sphere placed before.
+template <class T>
+class ParticlePositionInitializer {
+public:
+    ParticlePositionInitializer(const Box3D& box, const Boundary<T>* boundary, T max_radius)
+    {
+        const int b[6] = {(int)box.x0, (int)box.x1, (int)box.y0, (int)box.y1, (int)box.z0, (int)box.z1};
+        if (boundary) {
+            const bl_boundary d = boundary->desc();
+            blCheck(bl_positions_create(b, &d, max_radius, &h_), "ParticlePositionInitializer");
+        } else {
+            blCheck(bl_positions_create(b, nullptr, max_radius, &h_), "ParticlePositionInitializer");
+        }
+    }
+    ~ParticlePositionInitializer() { bl_positions_destroy(h_); }
+    ParticlePositionInitializer(const ParticlePositionInitializer&) = delete;
+    ParticlePositionInitializer& operator=(const ParticlePositionInitializer&) = delete;
+    bool generate_points(pluint N, T radius, char tag)
+    {
+        return bl_positions_generate(h_, (long long)N, radius, tag) == BL_OK;
+    }
+    void get_points(char tag, std::vector<Array<T, 3>>& points)
+    {
+        long long n = 0;
+        blCheck(bl_positions_get(h_, tag, nullptr, 0, &n), "get_points");
+        std::vector<double> buf(3 * (size_t)n);
+        blCheck(bl_positions_get(h_, tag, buf.data(), n, &n), "get_points");
+        for (long long k = 0; k < n; ++k) points.push_back(Array<T, 3>(buf[3 * k], buf[3 * k + 1], buf[3 * k + 2]));
+    }
+
+private:
+    bl_positions h_ = nullptr;
+};
+
 template <class T>
 class MorsePotential {
 public:
@@ -932,10 +973,97 @@ public:
         return {p_.shear_modulus, p_.L0, p_.k_area_global, p_.k_area_local, p_.k_volume, p_.vol_desired, p_.k_bend,
                 p_.theta0};
     }
+    RBCParameters<T>& params_ref() { return p_; }
+    const RBCParameters<T>& params_ref() const { return p_; }
 
 private:
     RBCParameters<T> p_;
 };
+
+// create_rbc_params (progs/common/calibrate_rbc_shape.h:75-112): L0 the mean
+// link length (distinct edges in (i0, i1) order), vol_desired the rest
+// volume, the area stiffness beyond K() = 2 G split 0.8 / 0.2 between
+// global and local, k_volume = 1000 G / radius, theta0 of a regular
+// triangulation with the shape's vertex count.
+template <class T>
+RBCParameters<T> create_rbc_params(const ParticleShape<T>* shape, T shear_modulus, T K_area, T K_bend)
+{
+    std::vector<std::pair<plint, plint>> edges;
+    for (const auto& t : shape->triangles)
+        for (int k = 0; k < 3; ++k) {
+            const plint a = t[k], b = t[(k + 1) % 3];
+            edges.push_back(std::make_pair(std::min(a, b), std::max(a, b)));
+        }
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    T lsum = 0;
+    for (const auto& e : edges) {
+        const Array<T, 3>&a = shape->vertices[e.first], &b = shape->vertices[e.second];
+        lsum += std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1]) + (b[2] - a[2]) * (b[2] - a[2]));
+    }
+    T vol = 0;
+    for (const auto& t : shape->triangles) {
+        const Array<T, 3>&a = shape->vertices[t[0]], &b = shape->vertices[t[1]], &c = shape->vertices[t[2]];
+        vol += (a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                a[2] * (b[0] * c[1] - b[1] * c[0])) / 6;
+    }
+    const T nv = (T)shape->vertices.size(), s3 = std::sqrt((T)3);
+    RBCParameters<T> p;
+    p.shear_modulus = shear_modulus;
+    p.L0 = lsum / (T)edges.size();
+    p.vol_desired = vol;
+    const T delta = K_area - 2 * shear_modulus;
+    p.k_area_global = delta * 0.8;
+    p.k_area_local = delta * 0.2;
+    p.k_volume = 1000. * shear_modulus / shape->get_radius();
+    p.k_bend = K_bend;
+    p.theta0 = std::acos((s3 * (nv - 2.) - 5 * M_PI) / (s3 * (nv - 2.) - 3. * M_PI));
+    return p;
+}
+
+// shrink_rbc_volume(rbc, vol_final, N_it, m, damping)
+// (progs/common/calibrate_rbc_shape.h:13-72) on the device: the particle's
+// vertices relax while vol_desired ramps to vol_final; on return the
+// particle holds the calibrated vertices and vol_desired.  A scratch
+// lattice and immersed boundary carry the device state.
+template <class T>
+void shrink_rbc_volume(RBCParticle<T>& rbc, T vol_final, int N_it, T m = 1, T damping = 1.e-2)
+{
+    (void)m;
+    bl_lattice L = nullptr;
+    bl_ibm g = nullptr;
+    const int per[3] = {0, 0, 0};
+    auto cleanup = [&]() {
+        if (g) bl_ibm_destroy(g);
+        if (L) bl_lattice_destroy(L);
+    };
+    try {
+        blCheck(bl_lattice_create(4, 4, 4, per, BL_STORE_F64, &L), "shrink_rbc_volume");
+        blCheck(bl_ibm_create(L, BL_KERNEL_PESKIN, &g), "shrink_rbc_volume");
+        const ParticleShape<T>* s = rbc.get_shape();
+        std::vector<double> v, x;
+        std::vector<int> t;
+        for (const auto& a : s->vertices) v.insert(v.end(), {a[0], a[1], a[2]});
+        for (const auto& a : s->triangles) t.insert(t.end(), {(int)a[0], (int)a[1], (int)a[2]});
+        for (const auto& a : rbc.positions()) x.insert(x.end(), {a[0], a[1], a[2]});
+        int sid = -1, pid = -1;
+        blCheck(bl_shape_add(g, (int)s->vertices.size(), (int)s->triangles.size(), v.data(), t.data(), &sid),
+                "shrink_rbc_volume");
+        const std::vector<double> prm = rbc.params();
+        blCheck(bl_particle_add(g, sid, BL_MODEL_RBC, prm.data(), (int)prm.size(), x.data(), 1, &pid),
+                "shrink_rbc_volume");
+        blCheck(bl_ibm_init(g), "shrink_rbc_volume");
+        blCheck(bl_rbc_shrink_volume(g, pid, vol_final, N_it, damping, nullptr), "shrink_rbc_volume");
+        blCheck(bl_particle_get(g, pid, 0, x.data()), "shrink_rbc_volume");
+        for (size_t k = 0; k < rbc.positions().size(); ++k)
+            rbc.positions()[k] = Array<T, 3>(x[3 * k], x[3 * k + 1], x[3 * k + 2]);
+        rbc.params_ref().vol_desired = vol_final;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
 
 template <typename T, template <typename U> class Descriptor, class Periodicity = void>
 class ImmersedBoundaryDynamics3D {

@@ -173,3 +173,29 @@ def test_rbc_in_pipe_prog_restarts_from_checkpoint(device, tmp_path, storage):
     assert res.returncode == 0
     for suffix in ("_final.txt", "_profile.txt"):
         assert (tmp_path / ("full" + suffix)).read_text() == (tmp_path / ("resumed" + suffix)).read_text()
+
+
+def test_initial_condition_prog_matches_reference(device, tmp_path):
+    """The C++ initial_condition prog (create_rbc_params, shrink_rbc_volume,
+    ParticlePositionInitializer, save_checkpoint) on the 626-vertex sphere
+    mesh with the parameters of the reference's own calibration and
+    placement runs (tests/golden/init): calibrated vertices within 1e-10,
+    sphere centres bit-exact, and a checkpoint holding the placed cells."""
+    from biblbm_amd import meshes
+    from conftest import load_golden
+    rest = load_golden("fsi", "sphere626", "rest_pos.npy")
+    tri = load_golden("fsi", "sphere626", "tri.npy")
+    msh = tmp_path / "rbc.msh"
+    meshes.write_gmsh(str(msh), rest, tri)
+    args = load_golden("init", "shrink626", "args.npy")
+    exe = os.path.join(ROOT, "biblbm_amd", "examples", "initial_condition")
+    r = subprocess.run([exe, str(msh), "0.02", "0.4", "0.002", "0.8", str(int(args[1])), "%.17g" % args[2], "5", "40",
+                        "ic"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    shape = np.loadtxt(tmp_path / "ic_shape.txt")
+    ref = load_golden("init", "shrink626", "pos.npy")
+    assert np.abs(shape - ref).max() <= 1e-10 * np.abs(ref).max()
+    assert np.array_equal(np.loadtxt(tmp_path / "ic_R.txt"), load_golden("init", "placement", "R.npy"))
+    assert np.array_equal(np.loadtxt(tmp_path / "ic_P.txt"), load_golden("init", "placement", "P.npy"))
+    assert (tmp_path / "checkpoint_fsi.plb").read_text().split() == ["40", "1", "checkpoint_fsi/0.plb"]
