@@ -162,39 +162,56 @@ def suspension_leg(steps, warmup):
     return out
 
 
-def precision_leg(steps, warmup):
+def precision_leg(steps, checkpoints=(100, 200)):
     """384^3 periodic Taylor-Green box with 16 Skalak capsules (642
     vertices), stepped from the same start in FP64, FP32 and FP16-shifted
-    storage; errors of the reduced forms against the FP64 run: velocity
-    field rel-L2, vertex positions rel-L2 and the largest per-capsule change
-    of the Taylor deformation index."""
+    storage; errors of the reduced forms against the FP64 run after each
+    checkpoint step count and after `steps`: velocity field rel-L2, vertex
+    positions rel-L2 and the largest per-capsule change of the Taylor
+    deformation index.  Speed from the steps after the first checkpoint
+    (graph replay, warm)."""
     import numpy as np
-    from biblbm_amd import workloads
+    from biblbm_amd import capi, workloads
     from biblbm_amd.ibm import taylor_deformation
+    marks = sorted(set([c for c in checkpoints if c < steps] + [steps]))
     out = {}
-    ref = None
+    ref = {}
     for storage in ("f64", "f32", "f16s"):
         W = workloads.taylor_green(storage)
-        dt = time_coupled(W, steps, warmup)
-        _, u = W.lattice.moments()
-        pos = W.ibm.positions()
-        D = np.array([taylor_deformation(W.ibm.observables(p)["inertia"]) for p in range(16)])
-        row = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4))
-        if ref is None:
-            ref = (u, pos, D)
-            row["deformation_index"] = dict(min=float(D.min()), max=float(D.max()))
-        else:
-            row["velocity_rel_l2_vs_f64"] = float(np.linalg.norm(u - ref[0]) / np.linalg.norm(ref[0]))
-            row["position_rel_l2_vs_f64"] = float(np.linalg.norm(pos - ref[1]) / np.linalg.norm(ref[1]))
-            row["deformation_index_max_abs_diff_vs_f64"] = float(np.abs(D - ref[2]).max())
+        row = {}
+        done, t_timed, n_timed = 0, 0.0, 0
+        for m in marks:
+            capi.call("bl_sync")
+            t0 = time.perf_counter()
+            W.ibm.coupled_steps(m - done, use_graph=True)
+            capi.call("bl_sync")
+            if done > 0:
+                t_timed += time.perf_counter() - t0
+                n_timed += m - done
+            done = m
+            _, u = W.lattice.moments()
+            pos = W.ibm.positions()
+            D = np.array([taylor_deformation(W.ibm.observables(p)["inertia"]) for p in range(16)])
+            if storage == "f64":
+                ref[m] = (u, pos, D)
+                row["deformation_index_at_%d" % m] = dict(min=float(D.min()), max=float(D.max()))
+            else:
+                r = ref[m]
+                row["at_%d_steps" % m] = dict(
+                    velocity_rel_l2_vs_f64=float(np.linalg.norm(u - r[0]) / np.linalg.norm(r[0])),
+                    position_rel_l2_vs_f64=float(np.linalg.norm(pos - r[1]) / np.linalg.norm(r[1])),
+                    deformation_index_max_abs_diff_vs_f64=float(np.abs(D - r[2]).max()))
+            del u
+        row["mlups"] = round(W.cells * n_timed / t_timed / 1e6, 1)
+        row["ms_per_step"] = round(t_timed / n_timed * 1e3, 4)
         out[storage] = row
-        del u
         W.ibm.close()
         W.lattice.close()
         del W
+    ref.clear()
     out["fp16s_over_fp32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
     out["workload"] = ("taylor_green 384^3 periodic, u0 0.03, tau 0.8, 16 Skalak capsules (642 vertices, "
-                       "radius 8, Ca 0.1), Peskin, steps %d after %d warmup from the same start" % (steps, warmup))
+                       "radius 8, Ca 0.1), Peskin, %s steps from the same start" % "/".join(map(str, marks)))
     return out
 
 
@@ -264,6 +281,12 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     return out
 
 
+def progress(msg):
+    """One line on stderr per finished leg (the JSON line stays alone on stdout)."""
+    sys.stderr.write("bench: %s\n" % msg)
+    sys.stderr.flush()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,11 +345,16 @@ def main():
         W2.ibm.close()
         W2.guo.close()
         W2.lattice.close()
+        progress("rbc_in_pipe f64 / f32 / second order done")
         line["suspension"] = suspension_leg(max(50, args.steps // 10), 20)
+        progress("suspension done")
         line["interactions"] = interactions_leg(max(50, args.steps // 10), 20,
                                                 line["suspension"]["f32"]["ms_per_step"])
+        progress("interactions done")
         line["ensemble"] = ensemble_leg(max(50, args.steps // 10), 10)
-        line["precision"] = precision_leg(max(100, args.steps // 4), 20)
+        progress("ensemble done")
+        line["precision"] = precision_leg(max(300, args.steps // 4))
+        progress("precision done")
     if not args.no_cpu_baseline:
         cb, cpu_pos = cpu_baseline(args.cpu_steps, args.cpu_budget)
         # the same number of coupled steps on the GPU from the same state: parity check
