@@ -1354,18 +1354,28 @@ __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* _
                                                   double* __restrict__ pos, int remap)
 {
     __shared__ double sft[3];
+    __shared__ double part[64][11];  // a batch of chunk partials, loaded by all lanes at once
     const int p = blockIdx.x;
-    if (threadIdx.x == 0) {
-        double acc[5] = {0, 0, 0, 0, 0};
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int c = p_choff[p]; c < p_choff[p + 1]; ++c) {
-            const double* P = partial + 11 * (long)c;
-            for (int q = 0; q < 5; ++q) acc[q] += P[q];
-            for (int d = 0; d < 3; ++d) {
-                lo[d] = fmin(lo[d], P[5 + 2 * d]);
-                hi[d] = fmax(hi[d], P[6 + 2 * d]);
+    const int c0 = p_choff[p], c1 = p_choff[p + 1];
+    double acc[5] = {0, 0, 0, 0, 0};
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int b = c0; b < c1; b += 64) {
+        const int nb = min(64, c1 - b);
+        if ((int)threadIdx.x < nb)
+            for (int q = 0; q < 11; ++q) part[threadIdx.x][q] = partial[11 * (long)(b + threadIdx.x) + q];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < nb; ++k) {  // chunk order
+                const double* P = part[k];
+                for (int q = 0; q < 5; ++q) acc[q] += P[q];
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = fmin(lo[d], P[5 + 2 * d]);
+                    hi[d] = fmax(hi[d], P[6 + 2 * d]);
+                }
             }
-        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
         double* G = geom + 11 * p;
         const double area = acc[0];
         double com[3] = {acc[2] / area, acc[3] / area, acc[4] / area};
@@ -1917,23 +1927,24 @@ int step_serial(bl_ibm g)
     return BL_OK;
 }
 
-// ... or forked over the lattice stream M and the boundary's streams A and
-// B, so that independent stages overlap:
+// ... or forked over the lattice stream M and the boundary's stream A, so
+// that independent stages overlap:
 //   A: particle-particle forces (when there are any) and membrane forces
-//      (read the positions before the move)
-//   M: binning, then velocity staging (the last collision's force field
-//      and the new tile flags)
-//   A: after the binning: spreading into the second force buffer (it needs
-//      the lists and the forces, not the staging)
-//   B: after the staging and the forces: interpolation + advection (the
-//      launch also readies the next binning's list: the flags and list two
-//      steps back are free once staged), geometry, voxelization
-//   M: joins A and B (the collision reads the force field and the phase;
-//      the next binning reads the moved positions).
+//      (read the positions before the move); after the binning, spreading
+//      into the second force buffer (it needs the lists and the forces,
+//      not the staging)
+//   M: binning, velocity staging (the last collision's force field and the
+//      new tile flags), then, once the forces have read the positions,
+//      interpolation + advection (the launch also readies the next
+//      binning's list: the flags and list two steps back are free once
+//      staged), geometry, voxelization; then joins A (the collision reads
+//      the spread force field).
+// The chain that ends at the collision stays on one queue; only the start
+// of A, the binning's lists and the spread field cross between queues.
 int step_forked(bl_ibm g)
 {
     int rc;
-    hipStream_t M = g->lat->stream, A = g->aux[0], B = g->aux[1];
+    hipStream_t M = g->lat->stream, A = g->aux[0];
     hipEvent_t* e = g->fev;
     BL_HIP(hipEventRecord(e[0], M));
     BL_HIP(hipStreamWaitEvent(A, e[0], 0));
@@ -1941,19 +1952,15 @@ int step_forked(bl_ibm g)
     BL_HIP(hipEventRecord(e[1], A));
     if ((rc = do_bin(g, M, e[2]))) return rc;
     if ((rc = launch_stage(g, M, false))) return rc;
-    BL_HIP(hipEventRecord(e[3], M));
     BL_HIP(hipStreamWaitEvent(A, e[2], 0));
     if ((rc = launch_spread(g, A))) return rc;
     BL_HIP(hipEventRecord(e[4], A));
-    BL_HIP(hipStreamWaitEvent(B, e[3], 0));
-    BL_HIP(hipStreamWaitEvent(B, e[1], 0));
-    if ((rc = launch_interp(g, true, B, true))) return rc;
+    BL_HIP(hipStreamWaitEvent(M, e[1], 0));
+    if ((rc = launch_interp(g, true, M, true))) return rc;
     g->binned = false;
-    if ((rc = update_particles(g, 1, B))) return rc;
-    if ((rc = do_voxelize(g, B))) return rc;
-    BL_HIP(hipEventRecord(e[5], B));
+    if ((rc = update_particles(g, 1, M))) return rc;
+    if ((rc = do_voxelize(g, M))) return rc;
     BL_HIP(hipStreamWaitEvent(M, e[4], 0));
-    BL_HIP(hipStreamWaitEvent(M, e[5], 0));
     return BL_OK;
 }
 
