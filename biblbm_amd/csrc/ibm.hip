@@ -1097,18 +1097,30 @@ __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[1
     const int na = *nactive;
     const int lane = threadIdx.x & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
-    for (int a = blk * kSpreadWaves + wv; a < na; a += nb * kSpreadWaves) {
-        const int t = active[a];
+    const int stride = nb * kSpreadWaves;
+    int a = blk * kSpreadWaves + wv;
+    int t = a < na ? active[a] : 0;
+    while (a < na) {
+        // the list's first slots are loaded with the count (not after it),
+        // and the next tile's id while this one is summed
+        const int raw = list[(long)t * cap + lane];
         const int n = count[t];
-        if (n > kSmallTile) continue;
-        const int key = wave_sort(lane < n ? list[(long)t * cap + lane] : 0x7fffffff, lane);
-        int org[3];
-        tile_origin(gd, t, org);
-        double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
-        sum_entries(gd, Sw, org, 0, n, [&](int) { return key; }, base, W, vforce, nw, ew, lane, lx, ly, lz, acc0,
-                    acc1, acc2);
-        store_cell(lv, gd, org, lx, ly, lz, acc0, acc1, acc2, Fout);
-        if (lane == 0) count[t] = 0;
+        const int a_next = a + stride;
+        const int t_next = a_next < na ? active[a_next] : 0;
+        // a big tile is its workgroup's; a count already cleared (0) is one
+        // that workgroup has finished (a listed tile has at least one entry)
+        if (n > 0 && n <= kSmallTile) {
+            const int key = wave_sort(lane < n ? raw : 0x7fffffff, lane);
+            int org[3];
+            tile_origin(gd, t, org);
+            double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
+            sum_entries(gd, Sw, org, 0, n, [&](int) { return key; }, base, W, vforce, nw, ew, lane, lx, ly, lz, acc0,
+                        acc1, acc2);
+            store_cell(lv, gd, org, lx, ly, lz, acc0, acc1, acc2, Fout);
+            if (lane == 0) count[t] = 0;
+        }
+        a = a_next;
+        t = t_next;
     }
 }
 
@@ -1789,7 +1801,7 @@ void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s)
     // tile: the touched tiles are at most every tile and, for membranes,
     // fewer than the vertices; big tiles fewer than a 32nd of the vertices
     const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(g->NV, 1024)));
-    const int gs = (int)((small_waves + kSpreadWaves - 1) / kSpreadWaves);
+    const int gs = (int)std::min<long>((small_waves + kSpreadWaves - 1) / kSpreadWaves, 4096);
     const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, std::max<long>(g->NV / 32, 256)));
     hipLaunchKernelGGL(k_spread<S>, dim3(gs + gb), dim3(64 * kSpreadWaves), 0, s, lv, gd, g->S, gs, act, na, big,
                        nbig, g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
