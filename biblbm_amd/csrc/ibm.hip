@@ -202,14 +202,14 @@ struct ListReset {
     int* nbig = nullptr;
 };
 
-__device__ __forceinline__ void list_reset(const ListReset& lr)
+__device__ __forceinline__ void list_reset(const ListReset& lr, int bx, int nbx)
 {
-    const int nb = min((int)gridDim.x, kResetBlocks);
-    if ((int)blockIdx.x >= nb) return;
+    const int nb = min(nbx, kResetBlocks);
+    if (bx >= nb) return;
     __shared__ int n;
     if (threadIdx.x == 0) n = *lr.na;
     __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) lr.flags[lr.act[i]] = 0;
+    for (int i = bx * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) lr.flags[lr.act[i]] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         if (atomicAdd(lr.arrive, 1) == nb - 1) {
@@ -221,7 +221,7 @@ __device__ __forceinline__ void list_reset(const ListReset& lr)
     }
 }
 
-__global__ void __launch_bounds__(256) k_bin_prep(ListReset lr) { list_reset(lr); }
+__global__ void __launch_bounds__(256) k_bin_prep(ListReset lr) { list_reset(lr, blockIdx.x, gridDim.x); }
 
 // Near-wall vertices (a boundary is set and the vertex is nearer to it than
 // the kernel's half support + 1, src/fsi/ImmersedBoundaryDynamics.hh:
@@ -315,14 +315,41 @@ __device__ __forceinline__ int axis_stencil(int kernel, int S, double h, double 
 // workgroup's run, and the slots inside the run come from LDS.  The order
 // within a list is whatever the atomics give; spreading sorts.  The add that
 // takes a tile past kSmallTile entries appends it to the big-tile list.
-__global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double h, long NV,
-                                             const double* __restrict__ pos, int* __restrict__ base,
-                                             double* __restrict__ W, int* __restrict__ count,
-                                             int* __restrict__ list, int cap, int* __restrict__ act,
-                                             int* __restrict__ na, unsigned char* __restrict__ flags,
-                                             int* __restrict__ status, int* __restrict__ pairs,
-                                             int* __restrict__ big, int* __restrict__ nbig, NearView nw)
+struct BinArgs {
+    Grid gd;
+    int kernel, S;
+    double h;
+    long NV;
+    const double* pos;
+    int* base;
+    double* W;
+    int *count, *list;
+    int cap;
+    int *act, *na;
+    unsigned char* flags;
+    int *status, *pairs, *big, *nbig;
+    NearView nw;
+};
+
+__device__ __forceinline__ void bin_body(long bx, const BinArgs& A)
 {
+    const Grid& gd = A.gd;
+    const int kernel = A.kernel, S = A.S, cap = A.cap;
+    const double h = A.h;
+    const long NV = A.NV;
+    const double* __restrict__ pos = A.pos;
+    int* __restrict__ base = A.base;
+    double* __restrict__ W = A.W;
+    int* __restrict__ count = A.count;
+    int* __restrict__ list = A.list;
+    int* __restrict__ act = A.act;
+    int* __restrict__ na = A.na;
+    unsigned char* __restrict__ flags = A.flags;
+    int* __restrict__ status = A.status;
+    int* __restrict__ pairs = A.pairs;
+    int* __restrict__ big = A.big;
+    int* __restrict__ nbig = A.nbig;
+    const NearView& nw = A.nw;
     __shared__ int keys[kHash];
     __shared__ int cnt[kHash];  // pairs per slot, then the slot's first index in the tile's list
     __shared__ int rank[kHash];
@@ -336,7 +363,7 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
         n_occ = 0;
     }
     __syncthreads();
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long v = bx * blockDim.x + threadIdx.x;
     StencilTiles st, stt;
     int e0 = -1, e1 = -1;  // list entries of the kernel part and the top-hat part (-1: none)
     // a vertex whose position is not finite (a membrane that blew up) or
@@ -422,6 +449,8 @@ __global__ void __launch_bounds__(256) k_bin(Grid gd, int kernel, int S, double 
     if (e0 >= 0) fill_tiles(st, gd, e0, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
     if (e1 >= 0) fill_tiles(stt, gd, e1, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
 }
+
+__global__ void __launch_bounds__(256) k_bin(BinArgs a) { bin_body(blockIdx.x, a); }
 
 __global__ void k_clear_bins(int* count, long n)
 {
@@ -615,10 +644,18 @@ __device__ void pair_forces(const ForceView& f, int s, long vo, const double* P,
 // of all particles: the element's forces on its vertices go to its own slots
 // of `ec` (9 doubles per triangle, 12 per pair, 3 per link).  Elements that
 // the particle's model does not use are skipped.
-__global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long NP, long NL, long ec_pair,
-                                                     long ec_link, double* __restrict__ ec)
+struct ElemArgs {
+    ForceView f;
+    long NT, NP, NL, ec_pair, ec_link;
+    double* ec;
+};
+
+__device__ __forceinline__ void elem_body(long bx, const ElemArgs& A)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const ForceView& f = A.f;
+    const long NT = A.NT, NP = A.NP, NL = A.NL, ec_pair = A.ec_pair, ec_link = A.ec_link;
+    double* __restrict__ ec = A.ec;
+    const long e = bx * blockDim.x + threadIdx.x;
     if (e < NT) {
         const int p = f.t_part[e];
         D3 out[3];
@@ -640,6 +677,8 @@ __global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long 
     }
 }
 
+__global__ void __launch_bounds__(256) k_elem_forces(ElemArgs a) { elem_body(blockIdx.x, a); }
+
 // Force on every vertex: the contributions of its incident elements --
 // links (src/fsi/RBCParticle.hh:149-171), triangles
 // (src/fsi/DeformableCapsule.hh:60-150, src/fsi/RBCParticle.hh:176-209) and
@@ -654,11 +693,24 @@ __global__ void __launch_bounds__(256) k_elem_forces(ForceView f, long NT, long 
 // (src/fsi/ImmersedBoundaryDynamics_forces.hh:293-301).
 constexpr int kGatherLanes = 4;
 
-__global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, long ec_pair, long ec_link,
-                                                       const double* __restrict__ ec, double* __restrict__ force,
-                                                       DecoSet deco, const double* __restrict__ fpp)
+struct GatherArgs {
+    ForceView f;
+    long NV, ec_pair, ec_link;
+    const double* ec;
+    double* force;
+    DecoSet deco;
+    const double* fpp;
+};
+
+__device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
 {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const ForceView& f = A.f;
+    const long NV = A.NV, ec_pair = A.ec_pair, ec_link = A.ec_link;
+    const double* __restrict__ ec = A.ec;
+    double* __restrict__ force = A.force;
+    const DecoSet& deco = A.deco;
+    const double* __restrict__ fpp = A.fpp;
+    const long gid = bx * blockDim.x + threadIdx.x;
     const long v = gid / kGatherLanes;
     const int lane = (int)(gid % kGatherLanes);
     double c0 = 0, c1 = 0, c2 = 0;
@@ -727,6 +779,8 @@ __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, lon
     }
 }
 
+__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs a) { gather_body(blockIdx.x, a); }
+
 // ------------------------------------------------- velocity staging
 // The progs' computeVelocity pass (staged_velocity), restricted to the
 // cells of the tiles this step's stencils touch (`touched`, the step's tile
@@ -735,13 +789,21 @@ __global__ void __launch_bounds__(256) k_vertex_gather(ForceView f, long NV, lon
 // in the collide kernel and the staged vectors are written contiguously.
 // (With the staging carried by the collision, k_collide_stream writes the
 // same values from the populations it loads.)
+struct StageArgs {
+    LatView lv;
+    const unsigned char* touched;
+    void* ut;  // Vel4 of the storage's compute type
+};
+
 template <class S, bool SWAPPED>
-__global__ void __launch_bounds__(256) k_stage_u(LatView lv, const unsigned char* __restrict__ touched,
-                                                 Vel4<typename Pop<S>::C>* __restrict__ ut)
+__device__ __forceinline__ void stage_body(unsigned bx, const StageArgs& A)
 {
     using C = typename Pop<S>::C;
     using FT = typename Pop<S>::F;
-    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    const LatView& lv = A.lv;
+    const unsigned char* __restrict__ touched = A.touched;
+    Vel4<C>* __restrict__ ut = (Vel4<C>*)A.ut;
+    const unsigned c = bx * blockDim.x + threadIdx.x;
     if (c >= lv.ncell) return;
     int x, y, z;
     decode(c, lv, x, y, z);
@@ -756,6 +818,9 @@ __global__ void __launch_bounds__(256) k_stage_u(LatView lv, const unsigned char
     }
     ut[c] = staged_velocity<FT, C>(lv, lv.flags, lv.force, m, c, x, y, z, f);
 }
+
+template <class S, bool SWAPPED>
+__global__ void __launch_bounds__(256) k_stage_u(StageArgs a) { stage_body<S, SWAPPED>(blockIdx.x, a); }
 
 // ------------------------------------------------------- interpolation
 // U_k = sum over the stencil nodes of phi(dx) phi(dy) phi(dz) u
@@ -791,16 +856,32 @@ __device__ __forceinline__ void move_reflect(const bl_boundary& b, double* p, co
 // hat, lanes (j, k) < 2 over its 2x2x2 stencil, and takes (1 - w) u_tophat
 // + w u_kernel, or u_tophat alone for w <= 0
 // (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:182-230).
+struct InterpArgs {
+    Grid gd;
+    long NV;
+    const int* base;
+    const double* W;
+    const void* ut;  // Vel4<C>
+    double *vel, *pos;
+    ListReset lr;
+    NearView nw;
+};
+
 template <class C, int SW, bool MOVE>
-__global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __restrict__ base,
-                                                const double* __restrict__ W, const Vel4<C>* __restrict__ ut,
-                                                double* __restrict__ vel, double* __restrict__ pos, ListReset lr,
-                                                NearView nw)
+__device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& A)
 {
-    if (lr.act) list_reset(lr);  // the next binning's list, carried by this launch
+    const Grid& gd = A.gd;
+    const long NV = A.NV;
+    const int* __restrict__ base = A.base;
+    const double* __restrict__ W = A.W;
+    const Vel4<C>* __restrict__ ut = (const Vel4<C>*)A.ut;
+    double* __restrict__ vel = A.vel;
+    double* __restrict__ pos = A.pos;
+    const NearView& nw = A.nw;
+    if (A.lr.act) list_reset(A.lr, (int)bx, nbx);  // the next binning's list, carried by this launch
     constexpr int LA = SW == 2 ? 2 : 4;
     constexpr int LPV = LA * LA;
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gid = bx * blockDim.x + threadIdx.x;
     const long v = gid / LPV;
     const int r = (int)(gid % LPV);
     const int j = r / LA, k = r % LA;
@@ -883,6 +964,9 @@ __global__ void __launch_bounds__(256) k_interp(Grid gd, long NV, const int* __r
         }
     }
 }
+
+template <class C, int SW, bool MOVE>
+__global__ void __launch_bounds__(256) k_interp(InterpArgs a) { interp_body<C, SW, MOVE>(blockIdx.x, gridDim.x, a); }
 
 // ----------------------------------------------------------- spreading
 // Every cell of a touched 4x4x4 tile sums g + phi phi phi F_k over the
@@ -1206,29 +1290,72 @@ __device__ __forceinline__ void spread_big(int blk, int nb, int* keys, int* runs
 
 // Both kinds in one launch: workgroups [0, n_small) take the small tiles,
 // the others the big ones.
+struct SpreadArgs {
+    LatView lv;
+    Grid gd;
+    int Sw, n_small;
+    const int *active, *nactive, *big, *nbig;
+    int* count;
+    int cap;
+    const int *list, *base;
+    const double *W, *vforce;
+    NearView nw;
+    void* Fout;  // force field in the storage's force type
+};
+
 template <class S>
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(LatView lv, Grid gd, int Sw, int n_small,
-                                                              const int* __restrict__ active,
-                                                              const int* __restrict__ nactive,
-                                                              const int* __restrict__ big,
-                                                              const int* __restrict__ nbig,
-                                                              int* __restrict__ count, int cap,
-                                                              const int* __restrict__ list,
-                                                              const int* __restrict__ base,
-                                                              const double* __restrict__ W,
-                                                              const double* __restrict__ vforce, NearView nw,
-                                                              typename Pop<S>::F* __restrict__ Fout)
+__device__ __forceinline__ void spread_body(int bx, int nbx, const SpreadArgs& A)
 {
+    using FT = typename Pop<S>::F;
     __shared__ int keys[kMaxList];
     __shared__ int runs[64 * kSpreadWaves];
     __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
     __shared__ double part[kSpreadWaves][3][64];
-    const int b = (int)blockIdx.x;
-    if (b < n_small)
-        spread_small(b, n_small, ew_all, lv, gd, Sw, active, nactive, count, cap, list, base, W, vforce, nw, Fout);
+    if (bx < A.n_small)
+        spread_small(bx, A.n_small, ew_all, A.lv, A.gd, A.Sw, A.active, A.nactive, A.count, A.cap, A.list, A.base,
+                     A.W, A.vforce, A.nw, (FT*)A.Fout);
     else
-        spread_big(b - n_small, (int)gridDim.x - n_small, keys, runs, ew_all, part, lv, gd, Sw, big, nbig, count,
-                   cap, list, base, W, vforce, nw, Fout);
+        spread_big(bx - A.n_small, nbx - A.n_small, keys, runs, ew_all, part, A.lv, A.gd, A.Sw, A.big, A.nbig,
+                   A.count, A.cap, A.list, A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(SpreadArgs a)
+{
+    spread_body<S>(blockIdx.x, gridDim.x, a);
+}
+
+// ------------------------------------------- stages paired in one launch
+// Two stages of a coupled step that do not read each other's output run as
+// one launch: workgroups [0, n1) take the first, the others the second
+// (the choice is uniform over a workgroup, so its barriers stay whole).
+// Binning | element forces: both read the positions only.
+__global__ void __launch_bounds__(256) k_bin_elem(BinArgs b, int n1, ElemArgs e)
+{
+    const int bx = blockIdx.x;
+    if (bx < n1) bin_body(bx, b);
+    else elem_body(bx - n1, e);
+}
+
+// Vertex force gather (element forces) | velocity staging (the binning's
+// tiles, the last collision's field).
+template <class S, bool SWAPPED>
+__global__ void __launch_bounds__(256) k_gather_stage(GatherArgs ga, int n1, StageArgs sa)
+{
+    const int bx = blockIdx.x;
+    if (bx < n1) gather_body(bx, ga);
+    else stage_body<S, SWAPPED>(bx - n1, sa);
+}
+
+// Spreading (lists, weights and vertex forces) | interpolation + advection
+// (weights and staged velocity; it moves the vertices, which the spreading
+// does not read).
+template <class S, int SW>
+__global__ void __launch_bounds__(256) k_spread_interp(SpreadArgs sp, int n1, InterpArgs ia)
+{
+    const int bx = blockIdx.x;
+    if (bx < n1) spread_body<S>(bx, n1, sp);
+    else interp_body<typename Pop<S>::C, SW, true>(bx - n1, (int)gridDim.x - n1, ia);
 }
 
 // ------------------------------------------------------- calibration
@@ -1685,38 +1812,50 @@ int prep_bin(bl_ibm g, hipStream_t s)
     return BL_OK;
 }
 
-int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
+BinArgs bin_args(bl_ibm g)
 {
     bl_lattice L = g->lat;
-    const Grid gd = grid_of(L);
-    int* act = g->active + (long)g->alt * L->ntiles;
-    int rc;
-    if (!g->prepped && (rc = prep_bin(g, s))) return rc;
-    g->prepped = false;
-    if (g->NV)
-        hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, gd, g->kernel, g->S, g->h, g->NV,
-                           (const double*)g->pos, g->v_base, g->v_w, g->tile_count, g->tile_list, kTileCap, act,
-                           g->n_active + g->alt, g->flags_alt, g->n_active, g->n_active + 5 + g->alt,
-                           g->active + (2L + g->alt) * L->ntiles, g->n_active + 8 + g->alt, near_view(g));
-    if (done) BL_HIP(hipEventRecord(done, s));
-    BL_CHECK_LAUNCH();
+    BinArgs a;
+    a.gd = grid_of(L);
+    a.kernel = g->kernel;
+    a.S = g->S;
+    a.h = g->h;
+    a.NV = g->NV;
+    a.pos = g->pos;
+    a.base = g->v_base;
+    a.W = g->v_w;
+    a.count = g->tile_count;
+    a.list = g->tile_list;
+    a.cap = kTileCap;
+    a.act = g->active + (long)g->alt * L->ntiles;
+    a.na = g->n_active + g->alt;
+    a.flags = g->flags_alt;
+    a.status = g->n_active;
+    a.pairs = g->n_active + 5 + g->alt;
+    a.big = g->active + (2L + g->alt) * L->ntiles;
+    a.nbig = g->n_active + 8 + g->alt;
+    a.nw = near_view(g);
+    return a;
+}
+
+// host state after a binning has been enqueued
+void binned(bl_ibm g)
+{
     g->binned = true;
     g->bins_dirty = true;
     g->last_list = g->alt;
-    return BL_OK;
 }
 
-// u on the cells of the touched tiles
-template <class S>
-void launch_stage_s(bl_ibm g, const LatView& lv, const unsigned char* touched, bool swapped, hipStream_t s)
+int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
 {
-    bl_lattice L = g->lat;
-    using V = Vel4<typename Pop<S>::C>;
-    const dim3 grid((unsigned)((L->ncell + 255) / 256));
-    if (swapped)
-        hipLaunchKernelGGL((k_stage_u<S, true>), grid, dim3(256), 0, s, lv, touched, (V*)g->ut);
-    else
-        hipLaunchKernelGGL((k_stage_u<S, false>), grid, dim3(256), 0, s, lv, touched, (V*)g->ut);
+    int rc;
+    if (!g->prepped && (rc = prep_bin(g, s))) return rc;
+    g->prepped = false;
+    if (g->NV) hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, bin_args(g));
+    if (done) BL_HIP(hipEventRecord(done, s));
+    BL_CHECK_LAUNCH();
+    binned(g);
+    return BL_OK;
 }
 
 LatView ibm_view(bl_ibm g)
@@ -1730,40 +1869,81 @@ LatView ibm_view(bl_ibm g)
 // binning's spreading (`spread` false) the last collision's force field and
 // flags are the lattice's, and the binning's flags are in flags_alt; after
 // it (`spread` true) the two have been swapped.
+StageArgs stage_args(bl_ibm g, bool spread)
+{
+    bl_lattice L = g->lat;
+    StageArgs a;
+    a.lv = ibm_view(g);
+    a.touched = g->flags_alt;
+    a.ut = g->ut;
+    if (spread) {
+        a.lv.force = g->force_alt;
+        if (a.lv.flags) a.lv.flags = g->flags_alt;
+        a.touched = L->tile_flags;
+    }
+    return a;
+}
+
+// u on the cells of the touched tiles, one thread per cell
+unsigned stage_blocks(bl_ibm g) { return (unsigned)((g->lat->ncell + 255) / 256); }
+
+template <class S>
+void launch_stage_s(const StageArgs& a, unsigned nblocks, bool swapped, hipStream_t s)
+{
+    if (swapped)
+        hipLaunchKernelGGL((k_stage_u<S, true>), dim3(nblocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_stage_u<S, false>), dim3(nblocks), dim3(256), 0, s, a);
+}
+
 int launch_stage(bl_ibm g, hipStream_t s, bool spread)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
-    LatView lv = ibm_view(g);
-    const unsigned char* touched = g->flags_alt;
-    if (spread) {
-        lv.force = g->force_alt;
-        if (lv.flags) lv.flags = g->flags_alt;
-        touched = L->tile_flags;
-    }
+    const StageArgs a = stage_args(g, spread);
     const bool swapped = (L->steps & 1) != 0;
-    if (L->storage == BL_STORE_F64) launch_stage_s<double>(g, lv, touched, swapped, s);
-    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(g, lv, touched, swapped, s);
-    else launch_stage_s<__half>(g, lv, touched, swapped, s);
+    if (L->storage == BL_STORE_F64) launch_stage_s<double>(a, stage_blocks(g), swapped, s);
+    else if (L->storage == BL_STORE_F32) launch_stage_s<float>(a, stage_blocks(g), swapped, s);
+    else launch_stage_s<__half>(a, stage_blocks(g), swapped, s);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
+InterpArgs interp_args(bl_ibm g, const ListReset& lr)
+{
+    InterpArgs a;
+    a.gd = grid_of(g->lat);
+    a.NV = g->NV;
+    a.base = g->v_base;
+    a.W = g->v_w;
+    a.ut = g->ut;
+    a.vel = g->vel;
+    a.pos = g->pos;
+    a.lr = lr;
+    a.nw = near_view(g);
+    return a;
+}
+
+unsigned interp_blocks(bl_ibm g) { return (unsigned)nblk((g->S == 2 ? 4 : 16) * g->NV); }
+
 template <class C, bool MOVE>
 void launch_interp_c(bl_ibm g, hipStream_t s, const ListReset& lr)
 {
-    const Grid gd = grid_of(g->lat);
-    const dim3 grid(nblk((g->S == 2 ? 4 : 16) * g->NV));
-    const Vel4<C>* ut = (const Vel4<C>*)g->ut;
-    if (g->S == 4)
-        hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos, lr, near_view(g));
-    else if (g->S == 3)
-        hipLaunchKernelGGL((k_interp<C, 3, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos, lr, near_view(g));
-    else
-        hipLaunchKernelGGL((k_interp<C, 2, MOVE>), grid, dim3(256), 0, s, gd, g->NV, (const int*)g->v_base,
-                           (const double*)g->v_w, ut, g->vel, g->pos, lr, near_view(g));
+    const InterpArgs a = interp_args(g, lr);
+    const dim3 grid(interp_blocks(g));
+    if (g->S == 4) hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, a);
+    else if (g->S == 3) hipLaunchKernelGGL((k_interp<C, 3, MOVE>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_interp<C, 2, MOVE>), grid, dim3(256), 0, s, a);
+}
+
+// The list reset carried by a launch that follows this step's spreading:
+// the flags and list of two steps back are free once the staging has read
+// them.
+ListReset reset_next_of(bl_ibm g, hipStream_t s)
+{
+    prep_rare(g, s);
+    g->prepped = true;
+    return list_reset_of(g);
 }
 
 // vertex velocities from the staged u; with `move` the vertices advance.
@@ -1773,11 +1953,7 @@ int launch_interp(bl_ibm g, bool move, hipStream_t s, bool reset_next = false)
 {
     if (g->NV == 0) return BL_OK;
     ListReset lr;
-    if (reset_next) {
-        prep_rare(g, s);
-        lr = list_reset_of(g);
-        g->prepped = true;
-    }
+    if (reset_next) lr = reset_next_of(g, s);
     const bool f64 = g->lat->storage == BL_STORE_F64;
     if (f64 && move) launch_interp_c<double, true>(g, s, lr);
     else if (f64) launch_interp_c<double, false>(g, s, lr);
@@ -1787,40 +1963,44 @@ int launch_interp(bl_ibm g, bool move, hipStream_t s, bool reset_next = false)
     return BL_OK;
 }
 
-template <class S>
-void launch_spread_s(bl_ibm g, const LatView& lv, hipStream_t s)
+// Spreading of the last binning's lists into the boundary's second force
+// buffer; `blocks` receives the launch's workgroup count.
+SpreadArgs spread_args(bl_ibm g, unsigned& blocks)
 {
     bl_lattice L = g->lat;
-    using FT = typename Pop<S>::F;
-    const Grid gd = grid_of(L);
-    const int* act = g->active + (long)g->alt * L->ntiles;
-    const int* na = g->n_active + g->alt;
-    const int* big = g->active + (2L + g->alt) * L->ntiles;
-    const int* nbig = g->n_active + 8 + g->alt;
+    SpreadArgs a;
+    a.lv = ibm_view(g);
+    a.gd = grid_of(L);
+    a.Sw = g->S;
+    a.active = g->active + (long)g->alt * L->ntiles;
+    a.nactive = g->n_active + g->alt;
+    a.big = g->active + (2L + g->alt) * L->ntiles;
+    a.nbig = g->n_active + 8 + g->alt;
+    a.count = g->tile_count;
+    a.cap = kTileCap;
+    a.list = g->tile_list;
+    a.base = g->v_base;
+    a.W = g->v_w;
+    a.vforce = g->force;
+    a.nw = near_view(g);
+    a.Fout = g->force_alt;
     // about one small-tile wave per touched tile and one workgroup per big
     // tile: the touched tiles are at most every tile and, for membranes,
     // fewer than the vertices; big tiles fewer than a 32nd of the vertices
     const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(g->NV, 1024)));
     const int gs = (int)std::min<long>((small_waves + kSpreadWaves - 1) / kSpreadWaves, 4096);
     const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, std::max<long>(g->NV / 32, 256)));
-    hipLaunchKernelGGL(k_spread<S>, dim3(gs + gb), dim3(64 * kSpreadWaves), 0, s, lv, gd, g->S, gs, act, na, big,
-                       nbig, g->tile_count, kTileCap, (const int*)g->tile_list, (const int*)g->v_base,
-                       (const double*)g->v_w, (const double*)g->force, near_view(g), (FT*)g->force_alt);
+    a.n_small = gs;
+    blocks = (unsigned)(gs + gb);
+    return a;
 }
 
-// Spreading writes the force field and flags of the next collision into
-// the boundary's second buffers, which then trade places with the
-// lattice's: the last collision's field stays readable (velocity staging)
-// while the next one is formed.
-int launch_spread(bl_ibm g, hipStream_t s)
+// Host state once a spreading is enqueued: its force field and flags trade
+// places with the lattice's, so that the last collision's field stays
+// readable (velocity staging) while the next one is formed.
+void spread_swap(bl_ibm g)
 {
     bl_lattice L = g->lat;
-    if (g->NV == 0) return BL_OK;
-    const LatView lv = ibm_view(g);
-    if (L->storage == BL_STORE_F64) launch_spread_s<double>(g, lv, s);
-    else if (L->storage == BL_STORE_F32) launch_spread_s<float>(g, lv, s);
-    else launch_spread_s<__half>(g, lv, s);
-    BL_CHECK_LAUNCH();
     if (L->flags_full) {  // the buffer leaving the lattice has every tile set
         g->alt_full = true;
         L->flags_full = false;
@@ -1829,6 +2009,19 @@ int launch_spread(bl_ibm g, hipStream_t s)
     std::swap(L->force, g->force_alt);
     g->alt ^= 1;
     g->bins_dirty = false;
+}
+
+int launch_spread(bl_ibm g, hipStream_t s)
+{
+    bl_lattice L = g->lat;
+    if (g->NV == 0) return BL_OK;
+    unsigned nb;
+    const SpreadArgs a = spread_args(g, nb);
+    if (L->storage == BL_STORE_F64) hipLaunchKernelGGL(k_spread<double>, dim3(nb), dim3(256), 0, s, a);
+    else if (L->storage == BL_STORE_F32) hipLaunchKernelGGL(k_spread<float>, dim3(nb), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_spread<__half>, dim3(nb), dim3(256), 0, s, a);
+    BL_CHECK_LAUNCH();
+    spread_swap(g);
     return BL_OK;
 }
 
@@ -1842,19 +2035,44 @@ int do_interpolate(bl_ibm g)
 
 // Vertex forces on stream s: particle-particle forces into their own
 // buffer, per-element membrane forces, then the per-vertex sum.
+ElemArgs elem_args(bl_ibm g)
+{
+    ElemArgs a;
+    a.f = force_view(g);
+    a.NT = g->NT;
+    a.NP = g->NP;
+    a.NL = g->NL;
+    a.ec_pair = g->ec_pair;
+    a.ec_link = g->ec_link;
+    a.ec = g->ec;
+    return a;
+}
+
+// `walls`: with the wall-interaction decorators and the particle-particle
+// forces of this step (when there are any)
+GatherArgs gather_args(bl_ibm g, bool walls)
+{
+    GatherArgs a;
+    a.f = force_view(g);
+    a.NV = g->NV;
+    a.ec_pair = g->ec_pair;
+    a.ec_link = g->ec_link;
+    a.ec = g->ec;
+    a.force = g->force;
+    if (walls) a.deco = deco_set(g);
+    else a.deco.n = 0;
+    a.fpp = walls && !g->pp.empty() ? g->pp_force : nullptr;
+    return a;
+}
+
 int do_forces(bl_ibm g, hipStream_t s)
 {
     int rc;
-    const ForceView f = force_view(g);
-    const long ne = g->NT + g->NP + g->NL;
     const bool pp = !g->pp.empty() && g->NV;
     if (pp && (rc = interactions_launch(g, s))) return rc;
     if (g->NV) {
-        hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair,
-                           g->ec_link, g->ec);
-        hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
-                           g->ec_link, (const double*)g->ec, g->force, deco_set(g),
-                           pp ? (const double*)g->pp_force : nullptr);
+        hipLaunchKernelGGL(k_elem_forces, dim3(nblk(g->NT + g->NP + g->NL)), dim3(256), 0, s, elem_args(g));
+        hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, gather_args(g, true));
     }
     BL_CHECK_LAUNCH();
     return BL_OK;
@@ -2028,7 +2246,69 @@ int step_fused(bl_ibm g)
     return update_particles(g, 1, M);
 }
 
-// n coupled steps enqueued on the lattice stream (forked IBM stages)
+// One coupled step on the lattice stream alone, as four launches before
+// the collision, each pairing two stages that do not read each other's
+// output:
+//   binning | membrane element forces (both read the positions only)
+//   vertex force gather | velocity staging (the binning's tiles, the last
+//     collision's field and flags)
+//   spreading | interpolation + advection (the launch also readies the
+//     next binning's list)
+//   geometry (then voxelization, with a phase field)
+// The reference's order and every value are those of the other layouts;
+// no stage waits on another queue.
+template <class S>
+int step_layered_s(bl_ibm g)
+{
+    int rc;
+    bl_lattice L = g->lat;
+    hipStream_t s = L->stream;
+    if (!g->pp.empty() && (rc = interactions_launch(g, s))) return rc;
+    if (!g->prepped && (rc = prep_bin(g, s))) return rc;
+    g->prepped = false;
+    const int nb = (int)nblk(g->NV), ne = (int)nblk(g->NT + g->NP + g->NL);
+    hipLaunchKernelGGL(k_bin_elem, dim3(nb + ne), dim3(256), 0, s, bin_args(g), nb, elem_args(g));
+    binned(g);
+    const int ng = (int)nblk(kGatherLanes * g->NV);
+    const StageArgs sa = stage_args(g, false);
+    const dim3 g2(ng + stage_blocks(g));
+    if (L->steps & 1)
+        hipLaunchKernelGGL((k_gather_stage<S, true>), g2, dim3(256), 0, s, gather_args(g, true), ng, sa);
+    else
+        hipLaunchKernelGGL((k_gather_stage<S, false>), g2, dim3(256), 0, s, gather_args(g, true), ng, sa);
+    unsigned nsp;
+    const SpreadArgs sp = spread_args(g, nsp);
+    spread_swap(g);
+    const InterpArgs ia = interp_args(g, reset_next_of(g, s));
+    const dim3 g3(nsp + interp_blocks(g));
+    if (g->S == 4) hipLaunchKernelGGL((k_spread_interp<S, 4>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
+    else if (g->S == 3) hipLaunchKernelGGL((k_spread_interp<S, 3>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
+    else hipLaunchKernelGGL((k_spread_interp<S, 2>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
+    BL_CHECK_LAUNCH();
+    g->binned = false;
+    if ((rc = update_particles(g, 1, s))) return rc;
+    return do_voxelize(g, s);
+}
+
+int step_layered(bl_ibm g)
+{
+    const int st = g->lat->storage;
+    if (st == BL_STORE_F64) return step_layered_s<double>(g);
+    if (st == BL_STORE_F32) return step_layered_s<float>(g);
+    return step_layered_s<__half>(g);
+}
+
+// Whether coupled steps without fused staging run as step_layered (one
+// stream) rather than step_forked (two): by default below 65 536 vertices,
+// where the stages are a few workgroups each and a wait across queues
+// costs more than the overlap gains.
+bool use_layered(bl_ibm g)
+{
+    if (g->NV == 0 || g->layered == 0) return false;
+    return g->layered == 1 || g->NV < 65536;
+}
+
+// n coupled steps enqueued on the lattice stream
 int steps_forked(bl_ibm g, int n)
 {
     int rc;
@@ -2037,7 +2317,7 @@ int steps_forked(bl_ibm g, int n)
             if ((rc = step_fused(g))) return rc;
             continue;
         }
-        if ((rc = step_forked(g))) return rc;
+        if ((rc = use_layered(g) ? step_layered(g) : step_forked(g))) return rc;
         if ((rc = collide_stream_launch(g->lat))) return rc;
         if ((rc = velocity_walls_launch(g->lat))) return rc;
     }
@@ -2505,14 +2785,8 @@ namespace {
 // particle-particle forces.
 int membrane_forces(bl_ibm g, hipStream_t s)
 {
-    const ForceView f = force_view(g);
-    const long ne = g->NT + g->NP + g->NL;
-    DecoSet none;
-    none.n = 0;
-    hipLaunchKernelGGL(k_elem_forces, dim3(nblk(ne)), dim3(256), 0, s, f, g->NT, g->NP, g->NL, g->ec_pair, g->ec_link,
-                       g->ec);
-    hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, f, g->NV, g->ec_pair,
-                       g->ec_link, (const double*)g->ec, g->force, none, (const double*)nullptr);
+    hipLaunchKernelGGL(k_elem_forces, dim3(nblk(g->NT + g->NP + g->NL)), dim3(256), 0, s, elem_args(g));
+    hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, gather_args(g, false));
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
@@ -2717,6 +2991,15 @@ int bl_ibm_set_fused_staging(bl_ibm g, int mode)
     if (mode < -1 || mode > 1) return fail(BL_ERR_ARG, "bl_ibm_set_fused_staging: mode must be -1, 0 or 1");
     if (mode != g->fused_staging) ibm_invalidate_graphs(g);
     g->fused_staging = mode;
+    return BL_OK;
+}
+
+int bl_ibm_set_layered(bl_ibm g, int mode)
+{
+    if (!g) return fail(BL_ERR_ARG, "bl_ibm_set_layered: null handle");
+    if (mode < -1 || mode > 1) return fail(BL_ERR_ARG, "bl_ibm_set_layered: mode must be -1, 0 or 1");
+    if (mode != g->layered) ibm_invalidate_graphs(g);
+    g->layered = mode;
     return BL_OK;
 }
 

@@ -140,6 +140,7 @@ struct bl_ibm_s {
     double stage_ms[5] = {0, 0, 0, 0, 0};
     bool timing = false;
     int fused_staging = -1;  // velocity staging carried by the collision: -1 auto (large vertex counts), 0 off, 1 on
+    int layered = -1;        // stages paired on one stream: -1 auto (small vertex counts), 0 off (forked), 1 on
 
     hipStream_t aux[2] = {nullptr, nullptr};  // forked stages of a step (default priority)
     hipEvent_t fev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

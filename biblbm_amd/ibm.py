@@ -192,6 +192,11 @@ class ImmersedBoundary:
         collide-and-stream launch (bl_ibm_set_fused_staging)."""
         call("bl_ibm_set_fused_staging", self.handle, int(mode))
 
+    def set_layered(self, mode):
+        """-1 auto, 0 membrane forces and spreading on a second stream, 1
+        one stream with paired stages (bl_ibm_set_layered)."""
+        call("bl_ibm_set_layered", self.handle, int(mode))
+
     def tile_stats(self):
         out = np.zeros(2, np.int64)
         call("bl_ibm_tile_stats", self.handle, ptr(out))

@@ -200,6 +200,14 @@ int bl_ibm_tile_stats(bl_ibm g, long long out[2]);
  * phase field only (omega_in == omega_out); otherwise the separate pass is
  * used.  The staged values are the same either way. */
 int bl_ibm_set_fused_staging(bl_ibm g, int mode);
+/* Stream layout of bl_coupled_steps without fused staging (no counterpart
+ * in the reference, whose order it keeps): mode 1 runs a step on the
+ * lattice stream alone, pairing stages that do not read each other's
+ * output in one launch (binning | element forces, force gather | velocity
+ * staging, spreading | interpolation); 0 forks the membrane forces and the
+ * spreading onto a second stream; -1 (default) picks 1 below 65 536
+ * vertices.  The results are the same either way. */
+int bl_ibm_set_layered(bl_ibm g, int mode);
 
 /* ------------------------------------------------ initial conditions */
 /* shrink_rbc_volume (progs/common/calibrate_rbc_shape.h:13-72) on the

@@ -370,6 +370,29 @@ def test_fused_staging_graph_replay_equals_eager(device):
     assert np.array_equal(ib1.positions(), ib2.positions())
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s"])
+def test_layered_step_equals_forked(device, storage):
+    """A coupled step on one stream with paired stages (binning | element
+    forces, gather | staging, spreading | interpolation) gives bitwise the
+    trajectory and flow of the step forked over two streams, eagerly and
+    from graph replays."""
+    from biblbm_amd import capi
+    mask = np.zeros(SHAPE, np.uint8)
+    mask[:, 0, :] = mask[:, -1, :] = 1
+    runs = []
+    for mode in (0, 1):
+        L, ib, *_ = _build_pair("peskin", storage, periodic=(1, 0, 1), mask=mask)
+        ib.set_fused_staging(0)
+        ib.set_layered(mode)
+        ib.coupled_steps(3, use_graph=False)
+        ib.coupled_steps(6, use_graph=True)
+        runs.append((L, ib, ib.positions(), L.get_populations()))
+    assert np.array_equal(runs[0][2], runs[1][2])
+    assert np.array_equal(runs[0][3], runs[1][3])
+    with pytest.raises(capi.BlError):
+        runs[1][1].set_layered(2)
+
+
 def test_fused_staging_mode_is_checked(device):
     from biblbm_amd import capi
     L, ib, *_ = _build_pair("roma")
