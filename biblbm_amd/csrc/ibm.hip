@@ -2330,7 +2330,8 @@ int ibm_step_launch(bl_ibm g, bool forked)
 {
     int rc;
     if (forked) return steps_forked(g, 1);
-    if ((rc = step_serial(g))) return rc;
+    // on the lattice stream alone: paired stages unless stage times are taken
+    if ((rc = !g->timing && use_layered(g) ? step_layered(g) : step_serial(g))) return rc;
     if ((rc = collide_stream_launch(g->lat))) return rc;
     return velocity_walls_launch(g->lat);
 }
