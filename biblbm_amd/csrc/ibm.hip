@@ -437,17 +437,21 @@ __device__ __forceinline__ void bin_body(long bx, const BinArgs& A)
         if (old + c > cap) atomicOr(&status[2], 1);
     }
     __syncthreads();
+    // the touched-list reservation is issued before the list fill and its
+    // result read after it, so the two round trips overlap
+    int fb = 0;
     if (threadIdx.x == 0) {
-        fresh_base = n_fresh ? atomicAdd(na, n_fresh) : 0;
+        if (n_fresh) fb = atomicAdd(na, n_fresh);
         if (n_pairs) atomicAdd(pairs, n_pairs);
     }
+    if (e0 >= 0) fill_tiles(st, gd, e0, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
+    if (e1 >= 0) fill_tiles(stt, gd, e1, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
+    if (threadIdx.x == 0) fresh_base = fb;
     __syncthreads();
     for (int i = threadIdx.x; i < n_fresh; i += blockDim.x) {
         act[fresh_base + i] = fresh[i];
         flags[fresh[i]] = 1;
     }
-    if (e0 >= 0) fill_tiles(st, gd, e0, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
-    if (e1 >= 0) fill_tiles(stt, gd, e1, keys, cnt, rank, count, list, cap, act, na, flags, status, pairs, big, nbig);
 }
 
 __global__ void __launch_bounds__(256) k_bin(BinArgs a) { bin_body(blockIdx.x, a); }
