@@ -493,6 +493,8 @@ struct ForceView {
     const double* s_area0;
     const double* pos;
     const int *v_part, *t_part, *r_part, *l_part;  // particle of each vertex / triangle / pair / link
+    const int4* v_inc;  // per vertex: {link list start, links, triangle list start, triangles},
+                        // {pair list start, pairs, particle, 0}; counts of unused kinds are 0
 };
 
 // Skalak / NeoHookean in-plane forces of one triangle
@@ -726,19 +728,14 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
         c2 = w[2];
     }
     if (v < NV) {
-        const int p = f.v_part[v];
-        const int s = f.p_shape[p];
-        const int model = f.p_model[p];
-        const long sv = f.s_voff[s] + (v - f.p_voff[p]);
-        const int* hl = f.s_vl_head + sv + s;
-        const int* ht = f.s_vt_head + sv + s;
-        const int* hp = f.s_vp_head + sv + s;
-        const int nl = model == BL_MODEL_RBC ? hl[1] - hl[0] : 0;
-        const int nt = ht[1] - ht[0];
-        const int npr = (model == BL_MODEL_RBC || model == 3) ? hp[1] - hp[0] : 0;
-        const int* Ll = f.s_vl + f.s_vloff[s] + hl[0];
-        const int* Lt = f.s_vt + f.s_vtoff[s] + ht[0];
-        const int* Lp = f.s_vp + f.s_vpoff[s] + hp[0];
+        // the vertex's incidence lists in one load (built at init), so the
+        // element forces are two dependent loads away
+        const int4 ia = f.v_inc[2 * v], ib = f.v_inc[2 * v + 1];
+        const int p = ib.z;
+        const int nl = ia.y, nt = ia.w, npr = ib.y;
+        const int* Ll = f.s_vl + ia.x;
+        const int* Lt = f.s_vt + ia.z;
+        const int* Lp = f.s_vp + ib.x;
         const double* El = ec + ec_link + 3 * f.p_loff[p];
         const double* Et = ec + 9 * f.p_toff[p];
         const double* Ep = ec + ec_pair + 12 * f.p_poff[p];
@@ -1711,6 +1708,7 @@ ForceView force_view(bl_ibm g)
     f.s_area0 = g->s_area0;
     f.pos = g->pos;
     f.v_part = g->v_part; f.t_part = g->t_part; f.r_part = g->r_part; f.l_part = g->l_part;
+    f.v_inc = (const int4*)g->v_inc;
     return f;
 }
 
@@ -2393,7 +2391,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
-                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
+                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_inc, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
                     g->l_part};
     for (void* p : ptrs) hipFree(p);
     interactions_free(g);
@@ -2539,8 +2537,29 @@ int bl_ibm_init(bl_ibm g)
             rp.insert(rp.end(), poff[p + 1] - poff[p], p);
             lp.insert(lp.end(), loff[p + 1] - loff[p], p);
         }
+        // per vertex: where its link / triangle / pair incidence lists start
+        // in the concatenated lists and how many entries its particle's
+        // model uses (links for RBCs, pairs for RBCs and model 3), and the
+        // particle
+        std::vector<int> inc;
+        inc.reserve(8 * (size_t)g->NV);
+        for (int p = 0; p < g->np; ++p) {
+            const int sh = g->parts[p].shape, model = g->parts[p].model;
+            const ShapeHost& S = g->shapes[sh];
+            const bool links = model == BL_MODEL_RBC, pairs = model == BL_MODEL_RBC || model == 3;
+            for (int i = 0; i < S.nv; ++i) {
+                inc.push_back(s_vloff[sh] + S.v_link_off[i]);
+                inc.push_back(links ? S.v_link_off[i + 1] - S.v_link_off[i] : 0);
+                inc.push_back(s_vtoff[sh] + S.v_tri_off[i]);
+                inc.push_back(S.v_tri_off[i + 1] - S.v_tri_off[i]);
+                inc.push_back(s_vpoff[sh] + S.v_pair_off[i]);
+                inc.push_back(pairs ? S.v_pair_off[i + 1] - S.v_pair_off[i] : 0);
+                inc.push_back(p);
+                inc.push_back(0);
+            }
+        }
         if ((rc = upload(g->v_part, vp)) || (rc = upload(g->t_part, tp)) || (rc = upload(g->r_part, rp)) ||
-            (rc = upload(g->l_part, lp)))
+            (rc = upload(g->l_part, lp)) || (rc = upload(g->v_inc, inc)))
             return rc;
     }
     if ((rc = upload(g->p_voff, voff)) || (rc = upload(g->p_toff, toff)) || (rc = upload(g->p_loff, loff)) ||

@@ -68,6 +68,7 @@ struct bl_ibm_s {
     double* p_geom = nullptr;    // 11 per particle: area, volume, com(3), bbox(6)
     double* p_delta = nullptr;   // 3 per particle (COM remap shift)
     int* v_part = nullptr;       // particle of each vertex, triangle, pair, link
+    int* v_inc = nullptr;        // per vertex: incidence list starts and counts, particle (8 ints)
     int* t_part = nullptr;
     int* r_part = nullptr;
     int* l_part = nullptr;
