@@ -1434,23 +1434,19 @@ __global__ void k_move(long NV, double* __restrict__ pos, const double* __restri
 // into chunks of 256 (one workgroup each, fixed-order tree in LDS), the
 // chunk partials are then summed in chunk order, so the result does not
 // depend on scheduling.
-__global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int* __restrict__ ch_part,
-                                                      const int* __restrict__ ch_idx,
-                                                      const int* __restrict__ p_choff, double* __restrict__ partial)
+// Chunk c (its row of `chunks`: first triangle, triangles, vertex range
+// of its bounding-box share, all global indices) -> 11 partials.
+__global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int4* __restrict__ chunks,
+                                                      const int* __restrict__ t_vert, double* __restrict__ partial)
 {
     __shared__ double sh[11][256];
     const int c = blockIdx.x;
-    const int p = ch_part[c], k = ch_idx[c];
-    const int nch = p_choff[p + 1] - p_choff[p];
-    const int s = f.p_shape[p];
-    const long vo = f.p_voff[p], nv = f.p_voff[p + 1] - vo;
-    const long nt = f.p_toff[p + 1] - f.p_toff[p];
+    const int4 ch = chunks[c];
     double acc[5] = {0, 0, 0, 0, 0};
-    const long t = (long)k * 256 + threadIdx.x;
-    if (t < nt) {
-        const int* tri = f.s_tri + 3 * (f.s_toff[s] + t);
-        const D3 v0 = ld3(f.pos + 3 * (vo + tri[0])), v1 = ld3(f.pos + 3 * (vo + tri[1])),
-                 v2 = ld3(f.pos + 3 * (vo + tri[2]));
+    if ((int)threadIdx.x < ch.y) {
+        const int* tv = t_vert + 3 * ((long)ch.x + threadIdx.x);
+        const D3 v0 = ld3(f.pos + 3 * (long)tv[0]), v1 = ld3(f.pos + 3 * (long)tv[1]),
+                 v2 = ld3(f.pos + 3 * (long)tv[2]);
         const double dA = 0.5 * norm(cross(v1 - v0, v2 - v0));
         acc[0] = dA;
         acc[1] = dot(v0, cross(v1, v2)) / 6.;
@@ -1459,10 +1455,9 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int* __
         acc[4] = ((v0.z + v1.z + v2.z) / 3.0) * dA;
     }
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    const long v0i = nv * k / nch, v1i = nv * (k + 1) / nch;
-    for (long v = v0i + threadIdx.x; v < v1i; v += blockDim.x)
+    for (long v = ch.z + threadIdx.x; v < ch.w; v += blockDim.x)
         for (int d = 0; d < 3; ++d) {
-            const double x = f.pos[3 * (vo + v) + d];
+            const double x = f.pos[3 * v + d];
             lo[d] = fmin(lo[d], x);
             hi[d] = fmax(hi[d], x);
         }
@@ -1757,8 +1752,8 @@ int update_particles(bl_ibm g, int remap, hipStream_t s)
     if (g->np == 0) return BL_OK;
     const ForceView f = force_view(g);
     const Grid gd = grid_of(g->lat);
-    hipLaunchKernelGGL(k_geom_partial, dim3(g->nchunk), dim3(256), 0, s, f, (const int*)g->ch_part,
-                       (const int*)g->ch_idx, (const int*)g->p_choff, g->ch_partial);
+    hipLaunchKernelGGL(k_geom_partial, dim3(g->nchunk), dim3(256), 0, s, f, (const int4*)g->ch_tab,
+                       (const int*)g->t_vert, g->ch_partial);
     hipLaunchKernelGGL(k_geom_final, dim3(g->np), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
                        (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, g->pos, remap);
     BL_CHECK_LAUNCH();
@@ -2391,7 +2386,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->p_delta, g->pos, g->vel, g->force, g->ec, g->ut, g->v_base, g->v_w,
                     g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
-                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->v_inc, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
+                    g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->ch_tab, g->t_vert, g->v_inc, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
                     g->l_part};
     for (void* p : ptrs) hipFree(p);
     interactions_free(g);
@@ -2600,18 +2595,28 @@ int bl_ibm_init(bl_ibm g)
     BL_HIP(hipMalloc(&g->force_alt, 3 * L->stride * L->force_bytes));
     BL_HIP(hipMemset(g->force_alt, 0, 3 * L->stride * L->force_bytes));
     {
-        std::vector<int> chp, chi, choff{0};
+        // chunks of 256 triangles per particle; chunk k also takes the
+        // bounding box of vertices [nv k / nch, nv (k + 1) / nch)
+        std::vector<int> chp, chi, choff{0}, tab, tvert;
+        tvert.reserve(3 * (size_t)g->NT);
         for (int p = 0; p < g->np; ++p) {
-            const long nt = toff[p + 1] - toff[p];
+            const ShapeHost& S = g->shapes[g->parts[p].shape];
+            const long nt = toff[p + 1] - toff[p], nv = voff[p + 1] - voff[p];
             const int nch = (int)std::max<long>(1, (nt + 255) / 256);
             for (int k = 0; k < nch; ++k) {
                 chp.push_back(p);
                 chi.push_back(k);
+                tab.push_back((int)(toff[p] + 256L * k));
+                tab.push_back((int)std::max<long>(0, std::min<long>(256, nt - 256L * k)));
+                tab.push_back((int)(voff[p] + nv * k / nch));
+                tab.push_back((int)(voff[p] + nv * (k + 1) / nch));
             }
             choff.push_back(choff.back() + nch);
+            for (long t = 0; t < 3 * nt; ++t) tvert.push_back((int)(voff[p] + S.tri[t]));
         }
         g->nchunk = (int)chp.size();
         if ((rc = upload(g->ch_part, chp)) || (rc = upload(g->ch_idx, chi)) || (rc = upload(g->p_choff, choff)) ||
+            (rc = upload(g->ch_tab, tab)) || (rc = upload(g->t_vert, tvert)) ||
             (rc = alloc(g->ch_partial, 11 * (size_t)std::max(g->nchunk, 1))))
             return rc;
     }

@@ -127,6 +127,8 @@ struct bl_ibm_s {
     int* p_choff = nullptr;              // np+1 chunk offsets
     int nchunk = 0;
     double* ch_partial = nullptr;        // 11 per chunk
+    int* ch_tab = nullptr;               // per chunk: first triangle, triangles, bbox vertex range (4)
+    int* t_vert = nullptr;               // per triangle: its three global vertex indices
 
     // ---- voxelization ----
     int* col_count = nullptr;
