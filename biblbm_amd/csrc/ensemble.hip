@@ -22,7 +22,15 @@ struct bl_ensemble_s {
     hipEvent_t fork = nullptr, done = nullptr;
     std::vector<hipEvent_t> join;
     hipGraphExec_t graph = nullptr;
-    std::vector<long long> parity;  // each box's AA parity the graph was captured for
+    // what the graph was captured for, per box: AA parity, tile-list buffer,
+    // pending list reset, lattice and boundary generations
+    struct Key {
+        long long state;
+        unsigned long long lgen, igen;
+        bool operator==(const Key& o) const { return state == o.state && lgen == o.lgen && igen == o.igen; }
+        bool operator!=(const Key& o) const { return !(*this == o); }
+    };
+    std::vector<Key> parity;
 };
 
 namespace bl {
@@ -53,11 +61,15 @@ int order_after(bl_ensemble e)
     return BL_OK;
 }
 
-// what a captured step depends on: the AA parity of the lattice and the
-// tile-list buffer of its immersed boundary
-long long capture_key(bl_ensemble e, size_t i)
+// what a captured step depends on: the AA parity of the lattice, the
+// tile-list buffer and pending list reset of its immersed boundary, and the
+// generations that every setter of either bumps
+bl_ensemble_s::Key capture_key(bl_ensemble e, size_t i)
 {
-    return (e->lats[i]->steps & 1) | (e->ibms[i] ? (long long)e->ibms[i]->alt << 1 : 0);
+    const bl_ibm g = e->ibms[i];
+    const long long st = (e->lats[i]->steps & 1) | (g ? (long long)g->alt << 1 : 0) |
+                         (g ? (long long)g->prepped << 2 : 0);
+    return bl_ensemble_s::Key{st, e->lats[i]->gen, g ? g->gen : 0ull};
 }
 
 bool parity_matches(bl_ensemble e)
@@ -154,6 +166,8 @@ int bl_ensemble_create(int n_boxes, const bl_lattice* lats, const bl_ibm* ibms, 
             if (ibms[i]->lat != lats[i])
                 return fail(BL_ERR_ARG, "bl_ensemble_create: immersed boundary belongs to another lattice");
             if (!ibms[i]->inited) return fail(BL_ERR_STATE, "bl_ensemble_create: immersed boundary not initialised");
+            int rc = need_stencil(ibms[i]);
+            if (rc) return rc;
         }
     }
     bl_ensemble e = new bl_ensemble_s();
@@ -281,7 +295,7 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
     }
     L->block = best_block;
     L->xcd_remap = best_remap;
-    if (L->ibm) L->ibm->graph_parity[0] = L->ibm->graph_parity[1] = -1;  // captured steps hold the old launch shape
+    ++L->gen;  // captured steps hold the old launch shape
     char buf[128];
     snprintf(buf, sizeof(buf), "], \"best\": {\"block\": %d, \"xcd_remap\": %d, \"ms\": %.6f}}", best_block,
              best_remap, best);

@@ -2108,7 +2108,10 @@ int do_voxelize(bl_ibm g, hipStream_t s)
     hipLaunchKernelGGL(k_fill_columns, dim3(nblk(ncol)), dim3(256), 0, s, gd, (const int*)g->col_count,
                        (const int*)g->col_off, g->col_z, g->col_cap, g->phase);
     BL_CHECK_LAUNCH();
-    L->phase = g->phase;
+    if (L->phase != g->phase) {
+        L->phase = g->phase;
+        ++L->gen;
+    }
     return BL_OK;
 }
 
@@ -2342,9 +2345,39 @@ void ibm_detach(bl_ibm g)
     g->lat = nullptr;
 }
 
-void ibm_invalidate_graphs(bl_ibm g) { g->graph_parity[0] = g->graph_parity[1] = -1; }
+void ibm_invalidate_graphs(bl_ibm g)
+{
+    g->graph_parity[0] = g->graph_parity[1] = -1;
+    ++g->gen;
+}
 
-bool ibm_needs_settle(bl_ibm g) { return g->lat && (g->lat->flags_full || g->alt_full); }
+// Steps run eagerly while a loaded full force field is consumed, and for the
+// first step after the body force changed (its velocity staging reads the
+// previous body force; a captured step would bake that value in).
+bool ibm_needs_settle(bl_ibm g)
+{
+    const bl_lattice L = g->lat;
+    if (!L) return false;
+    const bool g_changed = L->g_last[0] != L->g[0] || L->g_last[1] != L->g[1] || L->g_last[2] != L->g[2];
+    return L->flags_full || g->alt_full || g_changed;
+}
+
+// The stages that place delta-kernel stencils on the lattice (binning,
+// interpolation, spreading) need every periodic extent to hold one stencil
+// width, so that a stencil never covers a cell twice; the membrane and
+// collision-handler forces do not.
+int need_stencil(bl_ibm g)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    const bl_lattice L = g->lat;
+    const int n3[3] = {L->nx, L->ny, L->nz};
+    for (int d = 0; d < 3; ++d)
+        if (L->periodic[d] && n3[d] < 4)
+            return fail(BL_ERR_ARG, "immersed boundary: the kernel stages need periodic extents of at least 4 "
+                                    "(one stencil width)");
+    return BL_OK;
+}
 
 }  // namespace bl
 
@@ -2357,10 +2390,6 @@ int bl_ibm_create(bl_lattice lat, int kernel, bl_ibm* out)
     if (!lat || !out) return fail(BL_ERR_ARG, "bl_ibm_create: null argument");
     if (kernel < BL_KERNEL_ROMA || kernel > BL_KERNEL_TOPHAT) return fail(BL_ERR_ARG, "bl_ibm_create: unknown kernel");
     if (lat->ibm) return fail(BL_ERR_STATE, "bl_ibm_create: lattice already has an immersed boundary");
-    const int n3[3] = {lat->nx, lat->ny, lat->nz};
-    for (int d = 0; d < 3; ++d)
-        if (lat->periodic[d] && n3[d] < 4)
-            return fail(BL_ERR_ARG, "bl_ibm_create: periodic extents must be at least 4 (one stencil width)");
     bl_ibm g = new bl_ibm_s();
     g->lat = lat;
     g->kernel = kernel;
@@ -2378,6 +2407,7 @@ int bl_ibm_destroy(bl_ibm g)
         hipStreamSynchronize(g->lat->stream);
         if (g->lat->phase == g->phase) g->lat->phase = g->lat->own_phase;
         g->lat->ibm = nullptr;
+        ++g->lat->gen;
     }
     void* ptrs[] = {g->s_voff, g->s_toff, g->s_loff, g->s_poff, g->s_vtoff, g->s_vloff, g->s_vpoff, g->s_tri,
                     g->s_tri_area0, g->s_tri_geo0, g->s_rest, g->s_link, g->s_link_len, g->s_pair, g->s_pair_cs0,
@@ -2655,7 +2685,7 @@ int bl_ibm_set_boundary(bl_ibm g, const bl_boundary* b)
 
 int bl_ibm_interpolate(bl_ibm g)
 {
-    int rc = need_init(g);
+    int rc = need_stencil(g);
     return rc ? rc : do_interpolate(g);
 }
 
@@ -2667,7 +2697,7 @@ int bl_ibm_compute_forces(bl_ibm g)
 
 int bl_ibm_spread(bl_ibm g)
 {
-    int rc = need_init(g);
+    int rc = need_stencil(g);
     if (rc) return rc;
     if ((rc = do_spread(g))) return rc;
     return check_overflow(g);
@@ -2683,7 +2713,7 @@ int bl_ibm_move(bl_ibm g)
 
 int bl_ibm_step(bl_ibm g)
 {
-    int rc = need_init(g);
+    int rc = need_stencil(g);
     if (rc) return rc;
     g->timing = true;
     rc = step_serial(g);
@@ -2702,12 +2732,15 @@ namespace {
 constexpr int kLongRun = 32;  // steps per replay of the long captured run
 
 // Captures `n` (even) coupled steps into graph slot `slot` unless the slot
-// holds a capture made in the same step parity / buffer / list-reset state.
+// holds a capture made in the same step parity / buffer / list-reset state
+// and the same lattice and boundary generations (no setter ran since).
 int ensure_graph(bl_ibm g, int slot, int n)
 {
     bl_lattice L = g->lat;
     const long long par = (L->steps & 1) | ((long long)g->alt << 1) | ((long long)g->prepped << 2);
-    if (g->graph[slot] && g->graph_parity[slot] == par) return BL_OK;
+    if (g->graph[slot] && g->graph_parity[slot] == par && g->graph_gen[slot][0] == L->gen &&
+        g->graph_gen[slot][1] == g->gen)
+        return BL_OK;
     if (g->graph[slot]) {
         hipGraphExecDestroy(g->graph[slot]);
         g->graph[slot] = nullptr;
@@ -2724,6 +2757,10 @@ int ensure_graph(bl_ibm g, int slot, int n)
     hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     g->graph_parity[slot] = par;
+    // after the capture: a capture that set state for the first time (the
+    // phase field of a voxelization) keys on the state it captured
+    g->graph_gen[slot][0] = L->gen;
+    g->graph_gen[slot][1] = g->gen;
     return BL_OK;
 }
 
@@ -2739,7 +2776,7 @@ int replay(bl_ibm g, int slot, int n, int reps)
 
 int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph)
 {
-    int rc = need_init(g);
+    int rc = need_stencil(g);
     if (rc) return rc;
     bl_lattice L = g->lat;
     if (n_steps < 0) return fail(BL_ERR_ARG, "bl_coupled_steps: negative step count");

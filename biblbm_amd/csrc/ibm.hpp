@@ -152,6 +152,10 @@ struct bl_ibm_s {
     // each with the step parity / buffer state it was captured in
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     long long graph_parity[2] = {-1, -1};
+    unsigned long long graph_gen[2][2] = {{0, 0}, {0, 0}};  // lattice and boundary generations of each capture
+    // Bumped whenever captured steps stop matching the boundary's passes
+    // (interactions, walls, launch structure, a loaded checkpoint).
+    unsigned long long gen = 0;
 };
 
 namespace bl {
@@ -166,6 +170,9 @@ int ibm_check_overflow(bl_ibm g);
 // until bl_ibm_destroy).
 void ibm_detach(bl_ibm g);
 void ibm_invalidate_graphs(bl_ibm g);  // captured steps no longer match the lattice's passes
+// BL_OK when the boundary is initialised and every periodic extent of its
+// lattice holds one stencil width (the kernel stages need it).
+int need_stencil(bl_ibm g);
 // A full force field was loaded into the lattice (every tile flagged):
 // steps must run eagerly until the flags are consistent again.
 bool ibm_needs_settle(bl_ibm g);

@@ -499,6 +499,7 @@ int bl_lattice_load_checkpoint(bl_lattice L, const char* path)
         BL_HIP(hipMemcpy((char*)L->pops + (size_t)i * L->stride * L->pop_bytes, pops.data() + i * plane, plane,
                          hipMemcpyHostToDevice));
     L->steps = meta[0];
+    ++L->gen;
     if (L->ibm) ibm_invalidate_graphs(L->ibm);
     if (meta[2]) return bl_lattice_set_force_field(L, F.data());
     return BL_OK;

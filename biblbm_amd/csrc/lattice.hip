@@ -50,6 +50,7 @@ LatView view_of(const bl_lattice_s* L)
     v.flags = L->tile_flags;
     v.force = L->force;
     for (int d = 0; d < 3; ++d) v.g[d] = L->g[d];
+    for (int d = 0; d < 3; ++d) v.g_last[d] = L->g_last[d];
     v.om0 = L->omega_c0;
     v.om1 = L->omega_c1;
     v.om_base = L->omega_out;
@@ -231,6 +232,8 @@ int collide_stream_launch(bl_lattice_s* L)
     else launch_cs<__half>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
     BL_CHECK_LAUNCH();
     L->steps += 1;
+    for (int d = 0; d < 3; ++d) L->g_last[d] = L->g[d];
+    L->collided = true;
     return BL_OK;
 }
 
@@ -617,6 +620,7 @@ int ensure_force_field(bl_lattice_s* L)
     hipLaunchKernelGGL(k_fill_u8, dim3(nb), dim3(256), 0, L->stream, L->tile_flags, L->ntiles, (unsigned char)0);
     BL_CHECK_LAUNCH();
     L->force_field = true;
+    ++L->gen;
     return BL_OK;
 }
 
@@ -745,6 +749,7 @@ int bl_lattice_set_dynamics(bl_lattice L, double omega_out, double omega_in)
     const double c0 = 0.0, c1 = 1.0;
     L->omega_c0 = omega_out * omega_in / (c0 * omega_out + (1. - c0) * omega_in);
     L->omega_c1 = omega_out * omega_in / (c1 * omega_out + (1. - c1) * omega_in);
+    ++L->gen;
     return BL_OK;
 }
 
@@ -752,6 +757,7 @@ int bl_lattice_set_mask(bl_lattice L, const unsigned char* mask)
 {
     if (!L) return fail(BL_ERR_ARG, "null lattice");
     BL_HIP(hipStreamSynchronize(L->stream));
+    ++L->gen;
     if (!mask) {
         hipFree(L->mask);
         L->mask = nullptr;
@@ -799,6 +805,7 @@ int bl_lattice_set_velocity_walls(bl_lattice L, int axis, const double u_lo[3], 
         L->wall_u[d] = u_lo ? u_lo[d] : 0.0;
         L->wall_u[3 + d] = u_hi ? u_hi[d] : 0.0;
     }
+    ++L->gen;
     return BL_OK;
 }
 
@@ -806,6 +813,9 @@ int bl_lattice_set_body_force(bl_lattice L, const double g[3])
 {
     if (!L) return fail(BL_ERR_ARG, "null lattice");
     for (int d = 0; d < 3; ++d) L->g[d] = g ? g[d] : 0.0;
+    if (!L->collided)
+        for (int d = 0; d < 3; ++d) L->g_last[d] = L->g[d];
+    ++L->gen;
     return BL_OK;
 }
 
@@ -827,6 +837,7 @@ int bl_lattice_set_force_field(bl_lattice L, const double* force)
     hipLaunchKernelGGL(k_fill_u8, dim3(nt), dim3(256), 0, L->stream, L->tile_flags, L->ntiles, (unsigned char)1);
     BL_CHECK_LAUNCH();
     L->flags_full = true;
+    ++L->gen;
     BL_HIP(hipStreamSynchronize(L->stream));
     hipFree(d);
     return BL_OK;
@@ -836,6 +847,7 @@ int bl_lattice_set_phase(bl_lattice L, const unsigned char* phase)
 {
     if (!L) return fail(BL_ERR_ARG, "null lattice");
     BL_HIP(hipStreamSynchronize(L->stream));
+    ++L->gen;
     if (!phase) {
         hipFree(L->own_phase);
         L->own_phase = nullptr;
@@ -991,6 +1003,7 @@ int bl_lattice_set_launch(bl_lattice L, int block, int xcd_remap)
     if (block < 64 || block > 256 || block % 64) return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be 64, 128, 192 or 256");
     L->block = block;
     L->xcd_remap = xcd_remap ? 1 : 0;
+    ++L->gen;
     return BL_OK;
 }
 

@@ -28,6 +28,12 @@ struct bl_lattice_s {
     // external force: body force g everywhere except in tiles whose flag is
     // set, where force[d*stride + c] holds the full value.
     double g[3] = {0, 0, 0};
+    // body force of the last collision: the value the untouched cells held
+    // when the progs' computeVelocity pass of the next step reads the field
+    // (the force reset to g comes after it); g itself until the first
+    // collision
+    double g_last[3] = {0, 0, 0};
+    bool collided = false;
     void* force = nullptr;  // 3 * stride of double (F64) or float
     size_t force_bytes = 0;
     unsigned char* tile_flags = nullptr;
@@ -46,6 +52,11 @@ struct bl_lattice_s {
     const unsigned char* emit_flags = nullptr;
 
     long long steps = 0;  // completed collide-and-stream steps (AA parity)
+    // Bumped by every change that a captured step bakes into its kernel
+    // arguments or launch choices (omega, body force, wall velocities, mask,
+    // phase and force-field buffers, launch shape, inserted walls); captured
+    // graphs keyed on an older value are captured again.
+    unsigned long long gen = 0;
     hipStream_t stream = nullptr;
 
     int block = 256;
@@ -83,6 +94,7 @@ struct LatView {
     const unsigned char* flags;
     const void* force;
     double g[3];
+    double g_last[3];  // body force of the last collision (read by the velocity staging)
     double om0, om1;
     double om_base;  // omega_out, as Dynamics::getOmega() reports it
     int ntx, nty, ntz;

@@ -177,12 +177,12 @@ __device__ __forceinline__ void store_post(const LatView& v, const Nbr& n, long 
 }
 
 // External force of cell c in the field (flags, force): the field's value
-// in a flagged tile, the body force elsewhere.
+// in a flagged tile, the body force gb elsewhere.
 template <class FT, class C>
-__device__ __forceinline__ void cell_force_in(const LatView& v, const unsigned char* flags, const void* force,
-                                              long c, int x, int y, int z, C F[3])
+__device__ __forceinline__ void cell_force_in(const LatView& v, const double* gb, const unsigned char* flags,
+                                              const void* force, long c, int x, int y, int z, C F[3])
 {
-    F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2];
+    F[0] = (C)gb[0]; F[1] = (C)gb[1]; F[2] = (C)gb[2];
     if (flags) {
         const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
         if (flags[t]) {
@@ -198,7 +198,7 @@ __device__ __forceinline__ void cell_force_in(const LatView& v, const unsigned c
 template <class FT, class C>
 __device__ __forceinline__ void cell_force(const LatView& v, long c, int x, int y, int z, C F[3])
 {
-    cell_force_in<FT, C>(v, v.flags, v.force, c, x, y, z, F);
+    cell_force_in<FT, C>(v, v.g, v.flags, v.force, c, x, y, z, F);
 }
 
 // ------------------------------------------------------------- collision
@@ -307,7 +307,8 @@ struct alignas(4 * sizeof(C)) Vel4 {
 // post-stream populations f and the last collision's force,
 // src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
 // bounce-back / no-dynamics cells, the wall velocity on velocity-wall
-// cells).  (flags, force) is the last collision's force field.
+// cells).  (flags, force) is the last collision's force field, with the
+// last collision's body force in the tiles it does not flag.
 template <class FT, class C>
 __device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsigned char* flags, const void* force,
                                                    unsigned char m, long c, int x, int y, int z, const C f[Q])
@@ -319,7 +320,7 @@ __device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsig
         for (int d = 0; d < 3; ++d) u[d] = (C)v.wall_u[3 * side + d];
     } else if (m == M_FLUID) {
         C Fc[3];
-        cell_force_in<FT, C>(v, flags, force, c, x, y, z, Fc);
+        cell_force_in<FT, C>(v, v.g_last, flags, force, c, x, y, z, Fc);
         const C rb = sum_rho(f);
         C j[3];
         sum_j(f, j);

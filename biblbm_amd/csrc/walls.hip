@@ -362,6 +362,8 @@ int bl_guo_wall_destroy(bl_guo_wall w)
         hipStreamSynchronize(w->lat->stream);
         auto& v = w->lat->guo;
         v.erase(std::remove(v.begin(), v.end(), w), v.end());
+        ++w->lat->gen;
+        if (w->lat->ibm) ibm_invalidate_graphs(w->lat->ibm);
     }
     hipFree(w->d_cell);
     hipFree(w->d_dir);
@@ -418,6 +420,7 @@ int bl_guo_wall_insert(bl_guo_wall w)
     if (w->inserted) return fail(BL_ERR_STATE, "bl_guo_wall_insert: already inserted");
     w->lat->guo.push_back(w);
     w->inserted = true;
+    ++w->lat->gen;
     if (w->lat->ibm) ibm_invalidate_graphs(w->lat->ibm);
     return BL_OK;
 }

@@ -26,6 +26,8 @@ class ImmersedBoundary:
         self.kernel = kernel
         self.shapes = []      # (nv, nt)
         self.particles = []   # (shape_id, nv)
+        self.meshes = []      # (rest_pos, tri) per shape, as added
+        self.specs = []       # (shape_id, model, params, bug_compatible) per added particle
 
     def add_shape(self, rest_pos, tri):
         rest_pos = f64(rest_pos).reshape(-1, 3)
@@ -33,6 +35,7 @@ class ImmersedBoundary:
         sid = ctypes.c_int()
         call("bl_shape_add", self.handle, len(rest_pos), len(tri), ptr(rest_pos), ptr(tri), ctypes.byref(sid))
         self.shapes.append((len(rest_pos), len(tri)))
+        self.meshes.append((rest_pos.copy(), tri.copy()))
         return sid.value
 
     def shape_info(self, sid):
@@ -49,6 +52,7 @@ class ImmersedBoundary:
         call("bl_particle_add", self.handle, int(sid), int(model), ptr(params), len(params), ptr(pos),
              int(bool(bug_compatible)), ctypes.byref(pid))
         self.particles.append((sid, len(pos)))
+        self.specs.append((int(sid), int(model), params.copy(), bool(bug_compatible)))
         return pid.value
 
     def init(self):

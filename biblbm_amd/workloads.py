@@ -239,7 +239,7 @@ def suspension(storage="f32", nx=512, ny=128, nz=128, tau=1.0, u_c=0.02, n642=30
         ib.add_pp_force(MorsePotential(1.0, 1.0, 1e-3))
     vf = (n642 * vols["rbc642"] + n2562 * vols["rbc2562"] + n162 * vols["platelet"]) / (nx * (ny - 2) * nz)
     return Workload("suspension", L, ib, nx * ny * nz, nx * (ny - 2) * nz, nv,
-                    dict(nx=nx, ny=ny, nz=nz, tau=tau, G=G, cells=len(placed), volume_fraction=vf,
+                    dict(nx=nx, ny=ny, nz=nz, tau=tau, G=G, g=g, cells=len(placed), volume_fraction=vf,
                          kernel=kernel, seed=seed))
 
 

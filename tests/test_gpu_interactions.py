@@ -59,8 +59,6 @@ def test_collision_forces_match_reference(device, name):
     from biblbm_amd.potentials import Potential
     from biblbm_amd.walls import Boundary
     c = pp_case(name)
-    if any(p and n < 4 for p, n in zip(c["periodic"], c["shape"])):
-        pytest.skip("the immersed boundary needs periodic extents of at least one stencil width (4)")
     L = Lattice(*c["shape"], periodic=tuple(c["periodic"]), omega=1.0)
     ib = ImmersedBoundary(L, "peskin")
     allpos, idx = _golden_particles(ib, c)
