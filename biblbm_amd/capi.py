@@ -44,6 +44,7 @@ SIGNATURES = {
     "bl_lattice_get_moments": [_P, _P, _P],
     "bl_lattice_get_force": [_P, _P],
     "bl_lattice_conserved": [_P, _P],
+    "bl_lattice_conserved_static": [_P, _P],
     "bl_lattice_steps": [_P, _PLL],
     "bl_lattice_set_launch": [_P, _I, _I],
     "bl_lattice_stream": [_P, ctypes.POINTER(_P)],

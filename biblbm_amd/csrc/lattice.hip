@@ -588,6 +588,39 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* partial, lon
         for (int k = 0; k < 4; ++k) out[k] = sh[k][0];
 }
 
+}  // namespace bl
+
+// ---------------------------------------- allocation-free conservation sums
+// bl_lattice_conserved_static keeps the per-block partial sums of
+// k_conserved_partial in this file-scope array instead of the lattice's own
+// buffer: 4 doubles per 256-cell block for lattices of up to kStaticSumCells
+// cells, then the 4 results (8 MiB + 32 B; every device the library runs on
+// holds its own copy until the process ends).  Host code never passes the
+// array's name to a kernel: the device address of each device's copy comes
+// from hipGetSymbolAddress (static_scratch_init) and is kept per device.
+constexpr long kStaticSumCells = 1L << 26;
+constexpr long kStaticSumBlocks = kStaticSumCells / 256;
+static __device__ __align__(256) double scratch[4 * kStaticSumBlocks + 4];
+
+namespace bl {
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` per device (null: not looked up yet)
+
+// Looks up the current device's copy of `scratch` (once per device); the
+// caller makes `device` current first.
+int static_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(BL_ERR_ARG, "static scratch: device index beyond 63");
+    if (g_scratch_dev[device]) return BL_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(BL_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = (double*)p;
+    return BL_OK;
+}
+
 // ------------------------------------------------------------ fills
 // Streaming copy (bandwidth reference): grid-stride over 16-byte elements.
 __global__ void __launch_bounds__(256) k_copy16(const int4* __restrict__ a, int4* __restrict__ b, long n)
@@ -660,6 +693,8 @@ int bl_init(int device)
     BL_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(BL_ERR_ARG, "bl_init: no such device");
     BL_HIP(hipSetDevice(device));
+    int rc = static_scratch_init(device);
+    if (rc) return rc;
     g_device = device;
     g_inited = true;
     return BL_OK;
@@ -696,7 +731,10 @@ int bl_lattice_create(int nx, int ny, int nz, const int periodic[3], int storage
         int rc = bl_init(0);
         if (rc) return rc;
     }
+    int dev = 0;
+    BL_HIP(hipGetDevice(&dev));
     bl_lattice L = new bl_lattice_s();
+    L->device = dev;
     L->nx = nx; L->ny = ny; L->nz = nz;
     L->ncell = ncell;
     L->stride = (ncell + 127) / 128 * 128;
@@ -986,6 +1024,37 @@ int bl_lattice_conserved(bl_lattice L, double out[4])
     BL_CHECK_LAUNCH();
     BL_HIP(hipMemcpyAsync(out, L->d_partials + 4 * nb, sizeof(double) * 4, hipMemcpyDeviceToHost, L->stream));
     BL_HIP(hipStreamSynchronize(L->stream));
+    return BL_OK;
+}
+
+int bl_lattice_conserved_static(bl_lattice L, double* out)
+{
+    if (!L || !out) return fail(BL_ERR_ARG, "bl_lattice_conserved_static: null argument");
+    if (L->ncell > kStaticSumCells)
+        return fail(BL_ERR_CAPACITY, "bl_lattice_conserved_static: at most 67108864 cells (the static scratch "
+                                     "holds 4 partial sums per 256 cells)");
+    if (L->device < 0 || L->device >= kMaxDevices || !g_scratch_dev[L->device]) {
+        // first call on a device that bl_init did not set up: look its copy
+        // up with that device current
+        int cur = -1;
+        BL_HIP(hipGetDevice(&cur));
+        if (cur != L->device)
+            return fail(BL_ERR_STATE, "bl_lattice_conserved_static: make the lattice's device current (bl_init) first");
+        int rc = static_scratch_init(L->device);
+        if (rc) return rc;
+    }
+    double* scr = g_scratch_dev[L->device];
+    const LatView v = view_of(L);
+    const long nb = (L->ncell + 255) / 256;
+    const bool swapped = (L->steps & 1) != 0;
+    if (L->storage == BL_STORE_F64)
+        hipLaunchKernelGGL(k_conserved_partial<double>, dim3(nb), dim3(256), 0, L->stream, v, swapped, scr);
+    else if (L->storage == BL_STORE_F32)
+        hipLaunchKernelGGL(k_conserved_partial<float>, dim3(nb), dim3(256), 0, L->stream, v, swapped, scr);
+    else
+        hipLaunchKernelGGL(k_conserved_partial<__half>, dim3(nb), dim3(256), 0, L->stream, v, swapped, scr);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, L->stream, (const double*)scr, nb, out);
+    BL_CHECK_LAUNCH();
     return BL_OK;
 }
 

@@ -52,6 +52,7 @@ struct bl_lattice_s {
     const unsigned char* emit_flags = nullptr;
 
     long long steps = 0;  // completed collide-and-stream steps (AA parity)
+    int device = 0;       // the device current at creation (holds every buffer)
     // Bumped by every change that a captured step bakes into its kernel
     // arguments or launch choices (omega, body force, wall velocities, mask,
     // phase and force-field buffers, launch shape, inserted walls); captured

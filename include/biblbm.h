@@ -121,6 +121,18 @@ int bl_lattice_get_moments(bl_lattice lat, double *rho, double *u);
 int bl_lattice_get_force(bl_lattice lat, double *force);
 /* FP64 device sums over non-bounce-back cells: mass, jx, jy, jz. */
 int bl_lattice_conserved(bl_lattice lat, double out[4]);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  The sums of bl_lattice_conserved, bitwise equal,
+ * written to `out` (4 doubles of device memory, or host memory the device
+ * can write) in the order of the lattice stream, without host
+ * synchronisation and without allocating anything: the per-block partial
+ * sums live in a static device array of the library (8 MiB + 32 B per
+ * device, held until the process ends), which limits the lattice to
+ * 2^26 = 67 108 864 cells (BL_ERR_CAPACITY beyond).  Every call on one
+ * device shares that array, so calls that overlap on one device (lattices
+ * stepping on different streams) race. */
+int bl_lattice_conserved_static(bl_lattice lat, double *out);
 int bl_lattice_steps(bl_lattice lat, long long *steps);
 /* Launch shape of the collide-stream kernel: threads per block (64, 128,
  * 192 or 256) and XCD-aware block remapping on/off. */
