@@ -1,36 +1,38 @@
 #!/usr/bin/env python3
 """Benchmark: coupled D3Q19 + immersed-boundary time steps on one MI355X.
 
-Default workload: rbc_in_pipe (pipe radius 32, length 257, bbox
-257 x 67 x 67 = 1 153 673 lattice cells, bounce-back wall, body force for a
-centreline velocity of 0.02, tau 0.9, one 2562-vertex Evans-Fung red cell
-with Skalak + bending + area + volume forces, Peskin 4-point kernel),
+Headline workload (the largest single-GPU configuration): the precision
+configuration, a 384^3 periodic box seeded with a Taylor-Green field
+(u0 0.03, tau 0.8) holding 16 Skalak capsules (642-vertex icospheres, radius
+8, Ca 0.1) at seeded non-overlapping positions, Peskin 4-point kernel,
 populations stored in FP64.  A step is one full coupled step (stencil
 binning, velocity staging, interpolation, membrane forces, spreading,
-advection, collide-and-stream), replayed from a captured hipGraph; inputs are
-resident in HBM before the timed region.
+advection, collide-and-stream), replayed from captured hipGraphs that are
+instantiated before the timed region; inputs are resident in HBM before it
+starts.
 
-value = lattice cells x steps / seconds / 1e6 (MLUPS over all cells of the
-bounding box).  Extra objects on the same JSON line:
-  roofline      collide-stream kernel (the dominant kernel): algorithmic bytes
-                per launch / average launch time from HIP events on the
-                lattice stream, against 8 TB/s
-  cpu_baseline  the FP64 C restatement (oracle) running the same coupled step
-                on this host's cores, on a bounded number of steps
-  fp32          the same workload with FP32 storage
-  suspension    the 400-cell channel: FP16-shifted vs FP32 storage speed and
-                velocity error against the FP64 run
+value = lattice cells x steps / seconds / 1e6 (MLUPS).  Extra objects on the
+same JSON line:
+  roofline      collide-stream kernel (the dominant kernel) on the headline
+                lattice: algorithmic bytes per launch / average launch time
+                from HIP events on the lattice stream, against 8 TB/s; with
+                the per-parity times, the tuned launch shapes and the
+                measured device-copy bandwidth at 4 GiB
+  cpu_baseline  the FP64 C restatement (oracle) running the same coupled
+                step on this host's cores, a bounded number of steps, and
+                the GPU's vertex positions after the same steps against it
+  precision     the headline workload in FP32 and FP16-shifted storage from
+                the same start: speed and velocity / position / deformation
+                index errors against the FP64 run
+  capsule_in_shear  the 64^3 shear box with one capsule (FP64)
+  rbc_in_pipe   pipe R 32 x L 257, one red cell, FP64 and FP32, and with the
+                reference prog's default Guo curved wall (second_order);
+                its populations (175 MB FP64) fit the 256 MB last-level
+                cache, so its roofline figure is cache-assisted
+  suspension    the 400-cell channel: FP64 / FP32 / FP16-shifted speed,
+                velocity error against the FP64 run, per-stage IBM times
+  interactions  the suspension with Morse wall and particle-particle forces
   ensemble      64 shear boxes of uneven sizes advanced by one hipGraph
-                launch, against the same boxes stepped one after another
-  second_order  rbc_in_pipe with the reference prog's default wall: the Guo
-                curved wall on the pipe surface, no dynamics beyond it
-  interactions  the suspension with the rbc_in_pipe prog's Morse wall and
-                particle-particle interactions, and one collision-handler
-                pass alone
-  precision     384^3 Taylor-Green box with 16 capsules in FP64, FP32 and
-                FP16-shifted storage from the same start: speed, and the
-                velocity, position and deformation-index errors of the
-                reduced forms against the FP64 run
 """
 import argparse
 import json
@@ -47,7 +49,16 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {"f64": 8, "f32": 4, "f16s": 2}
 
 
+def progress(msg):
+    """One line on stderr per finished leg (the JSON line stays alone on stdout)."""
+    sys.stderr.write("bench: %s\n" % msg)
+    sys.stderr.flush()
+
+
 def collide_bytes_per_launch(cells, storage, has_mask, active_tiles, ntiles):
+    """Algorithmic bytes of one collide-and-stream launch: 2 x 19 populations
+    per cell, the mask byte if the lattice has one, the force field of the
+    IBM-touched tiles (3 values per cell) and one flag byte per tile."""
     s = BYTES[storage]
     sF = 8 if storage == "f64" else 4
     b = cells * 2 * 19 * s
@@ -68,9 +79,19 @@ def read_traffic(workload, storage):
     return float(m.group(1)) if m else None
 
 
+def ntiles_of(L):
+    nx, ny, nz = L.shape
+    return ((nx + 3) // 4) * ((ny + 3) // 4) * ((nz + 3) // 4)
+
+
 def time_coupled(W, steps, warmup):
+    """warmup steps, then the graphs of the timed run are captured
+    (bl_coupled_prepare) and `steps` coupled steps are timed between two
+    device synchronisations."""
     from biblbm_amd import capi
     W.ibm.coupled_steps(warmup, use_graph=True)
+    capi.call("bl_sync")
+    W.ibm.prepare(steps)
     capi.call("bl_sync")
     t0 = time.perf_counter()
     W.ibm.coupled_steps(steps, use_graph=True)
@@ -78,98 +99,97 @@ def time_coupled(W, steps, warmup):
     return time.perf_counter() - t0
 
 
-def roofline(W, storage, n=40):
+def roofline(W, storage, has_mask, n=20):
+    """The collide kernel alone on the workload's lattice (the force field
+    and flags of the last step in place): n launches over both parities
+    timed with HIP events on the lattice stream, then each parity alone."""
     L = W.lattice
     st = W.ibm.tile_stats()
     ms = L.time_collide(n) / n
-    nb = collide_bytes_per_launch(W.cells, storage, True, st["active_tiles"], L_ntiles(L))
+    par = [0.0, 0.0]
+    reps = 4
+    for _ in range(reps):
+        for _ in range(2):
+            par[L.steps & 1] += L.time_collide(1) / reps
+    nb = collide_bytes_per_launch(W.cells, storage, has_mask, st["active_tiles"], ntiles_of(L))
     achieved = nb / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=read_traffic(W.name, storage),
                 kernel="bl::k_collide_stream", launch_ms=round(ms, 5), algorithmic_bytes=nb,
-                active_tiles=st["active_tiles"])
+                bytes_per_cell=2 * 19 * BYTES[storage] + (1 if has_mask else 0),
+                active_tiles=st["active_tiles"],
+                by_parity={p: dict(launch_ms=round(par[k], 5), frac=round(nb / (par[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                   **L.launch(k)) for k, p in ((0, "even"), (1, "odd"))})
 
 
-def L_ntiles(L):
-    nx, ny, nz = L.shape
-    return ((nx + 3) // 4) * ((ny + 3) // 4) * ((nz + 3) // 4)
+def copy_bandwidth(nbytes=1 << 32, n_iter=10):
+    """Device-to-device copy of nbytes (16-byte elements), read + write bytes
+    per second: the bandwidth a streaming kernel can reach on this device."""
+    import ctypes
+    from biblbm_amd import capi
+    ms = ctypes.c_double()
+    capi.call("bl_device_copy", nbytes, n_iter, ctypes.byref(ms))
+    return round(2.0 * nbytes * n_iter / (ms.value * 1e-3) / 1e9, 1)
+
+
+def oracle_particles(ibm):
+    from oracle import oracle_py as O
+    pos = ibm.positions()
+    out, k = [], 0
+    for sid, model, params, bug in ibm.specs:
+        rest, tri = ibm.meshes[sid]
+        out.append(O.Particle(rest, tri, model, params, pos[k:k + len(rest)], bug))
+        k += len(rest)
+    return out
 
 
 def cpu_baseline(steps, budget_s):
-    """The oracle (FP64 C restatement, OpenMP in the lattice step) on the
-    rbc_in_pipe workload: same lattice, mesh, parameters."""
+    """The oracle (FP64 C restatement, OpenMP in the lattice passes, IBM
+    serial) on the headline workload from its start: same lattice, fields,
+    meshes, parameters and positions; then the GPU runs the same number of
+    steps from a fresh start and its vertex positions are compared."""
     import numpy as np
-    from biblbm_amd import meshes, workloads
+    from biblbm_amd import workloads
     from oracle import oracle_py as O
-    R, length, u_c, tau = 32, 257, 0.02, 0.9
-    mask, r2, c = workloads.pipe_mask(R, length)
-    shape = mask.shape
-    nu = (tau - 0.5) / 3.0
-    g = 4.0 * nu * u_c / (R * R)
-    prof = np.where(r2 <= R * R, u_c * (1.0 - r2 / (R * R)), 0.0)
-    u = np.zeros(shape + (3,))
-    u[..., 0] = prof[None]
-    f = O.lbm_init_equilibrium(np.ones(shape), u)
-    del u
-    force = np.broadcast_to(np.array([g, 0, 0]), shape + (3,)).copy()
-    v, t = meshes.biconcave(4, 12.0)
-    vol = meshes.signed_volume(v, t)
-    G = nu * (u_c / R) * 12.0 / 0.1
-    k_bend = 2.0 * 0.02 * G * 144.0 / np.sqrt(3.0)
-    rot = meshes.rotation_matrix([0, 1, 0], np.pi / 2)
-    pos = v @ rot.T + [length / 2.0, c, c]
-    part = O.Particle(v, t, 3, workloads.skalak_rbc_params(G, 10.0, 12.0, vol, k_bend), pos)
-    phase = np.zeros(shape)
+    G = workloads.taylor_green("f64")
+    n = G.params["n"]
+    rho, u = workloads.taylor_green_fields(n, G.params["u0"])
+    f = O.lbm_init_equilibrium(rho, u)
+    del rho, u
+    parts = oracle_particles(G.ibm)
+    shape = (n, n, n)
+    force = np.zeros(shape + (3,))
+    zero = np.zeros(shape)
+    mask = np.zeros(shape, np.uint8)
+    om = 1 / G.params["tau"]
     done = 0
     t0 = time.perf_counter()
     while done < steps:
-        f, force, _ = O.coupled_step(f, force, phase, mask, (1, 0, 0), 1 / tau, 1 / tau, (g, 0, 0), [part],
-                                     "peskin")
+        f, force, _ = O.coupled_step(f, force, zero, mask, (1, 1, 1), om, om, (0, 0, 0), parts, "peskin")
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    del f, force
+    G.ibm.coupled_steps(done, use_graph=False)
+    gpos = G.ibm.positions()
+    ref = np.concatenate([p.pos for p in parts])
+    G.ibm.close()
+    G.lattice.close()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=round(mask.size * done / dt / 1e6, 3), unit="MLUPS", cores=cores, kind="port",
-                sample="%d coupled steps of rbc_in_pipe (1 153 673 cells, 2562 vertices), FP64 oracle, "
-                       "OpenMP in the lattice step" % done), part.pos
-
-
-def suspension_leg(steps, warmup):
-    import numpy as np
-    from biblbm_amd import workloads
-    out = {}
-    u_ref = None
-    for storage in ("f64", "f32", "f16s"):
-        W = workloads.suspension(storage)
-        dt = time_coupled(W, steps, warmup)
-        _, u = W.lattice.moments()
-        if storage == "f64":
-            u_ref = u
-            err = 0.0
-        else:
-            err = float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))
-        out[storage] = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
-                            velocity_rel_l2_vs_f64=err)
-        cells, nv, vf = W.cells, W.vertices, W.params["volume_fraction"]
-        W.ibm.close()
-        W.lattice.close()
-        del W
-    out["fp16s_over_fp32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
-    out["workload"] = ("suspension 512x128x128 channel, 400 cells (300x642, 80x2562, 20x162 vertices), "
-                       "%.1f%% volume fraction, %d vertices, steps %d after %d warmup" %
-                       (100 * vf, nv, steps, warmup))
-    return out
+    return dict(value=round(n ** 3 * done / dt / 1e6, 3), unit="MLUPS", cores=cores, kind="port",
+                sample="%d coupled steps of the 384^3 Taylor-Green box with 16 capsules (56 623 104 cells, "
+                       "10 272 vertices), FP64 oracle, OpenMP in the lattice passes" % done,
+                gpu_vs_cpu_position_rel_l2=float(np.linalg.norm(gpos - ref) / np.linalg.norm(ref)))
 
 
 def precision_leg(steps, checkpoints=(100, 200)):
-    """384^3 periodic Taylor-Green box with 16 Skalak capsules (642
-    vertices), stepped from the same start in FP64, FP32 and FP16-shifted
-    storage; errors of the reduced forms against the FP64 run after each
-    checkpoint step count and after `steps`: velocity field rel-L2, vertex
-    positions rel-L2 and the largest per-capsule change of the Taylor
-    deformation index.  Speed from the steps after the first checkpoint
-    (graph replay, warm)."""
+    """The headline workload in FP32 and FP16-shifted storage (FP32
+    collision, FP64 rest geometry and conservation sums) from the same start
+    as the FP64 run: errors against it after each checkpoint step count and
+    after `steps` (velocity field rel-L2, vertex positions rel-L2, largest
+    per-capsule change of the Taylor deformation index), speed from the
+    steps after the first checkpoint."""
     import numpy as np
     from biblbm_amd import capi, workloads
     from biblbm_amd.ibm import taylor_deformation
@@ -181,6 +201,7 @@ def precision_leg(steps, checkpoints=(100, 200)):
         row = {}
         done, t_timed, n_timed = 0, 0.0, 0
         for m in marks:
+            W.ibm.prepare(m - done)
             capi.call("bl_sync")
             t0 = time.perf_counter()
             W.ibm.coupled_steps(m - done, use_graph=True)
@@ -215,6 +236,35 @@ def precision_leg(steps, checkpoints=(100, 200)):
     return out
 
 
+def suspension_leg(steps, warmup):
+    import numpy as np
+    from biblbm_amd import workloads
+    out = {}
+    u_ref = None
+    for storage in ("f64", "f32", "f16s"):
+        W = workloads.suspension(storage)
+        dt = time_coupled(W, steps, warmup)
+        _, u = W.lattice.moments()
+        W.ibm.step()
+        stages = {k: round(v, 5) for k, v in W.ibm.stage_times().items()}
+        if storage == "f64":
+            u_ref = u
+            err = 0.0
+        else:
+            err = float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))
+        out[storage] = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
+                            velocity_rel_l2_vs_f64=err, ibm_stage_ms=stages)
+        nv, vf = W.vertices, W.params["volume_fraction"]
+        W.ibm.close()
+        W.lattice.close()
+        del W
+    out["fp16s_over_fp32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
+    out["workload"] = ("suspension 512x128x128 channel, 400 cells (300x642, 80x2562, 20x162 vertices), "
+                       "%.1f%% volume fraction, %d vertices, %d steps after %d warmup" %
+                       (100 * vf, nv, steps, warmup))
+    return out
+
+
 def interactions_leg(steps, warmup, f32_ms=None):
     """The 400-cell suspension (FP32 storage) with the rbc_in_pipe prog's
     interactions: Morse wall decorator on the channel walls, Morse
@@ -246,20 +296,20 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     each) advanced by one hipGraph launch per two steps, against the same
     boxes stepped one after another (each with its own captured graph)."""
     import numpy as np
-    from biblbm_amd import workloads
-    from biblbm_amd.ensemble import Ensemble, tune_collide
+    from biblbm_amd import capi, tuning, workloads
+    from biblbm_amd.ensemble import Ensemble
     from biblbm_amd.ibm import taylor_deformation
     ws = workloads.shear_ensemble("f64", n_boxes=n_boxes)
     cells = sum(w.cells for w in ws)
-    tuned = tune_collide(ws[0].lattice)["best"]
+    best = tuning.tune_collide(ws[0].lattice)["best"]
     for w in ws:
-        w.lattice.set_launch(tuned["block"], tuned["xcd_remap"])
+        tuning.apply_collide(w.lattice, best)
     ens = Ensemble([(w.lattice, w.ibm) for w in ws])
     ens.steps(warmup)
     ms = ens.time_steps(steps)
-    from biblbm_amd import capi
     for w in ws:
         w.ibm.coupled_steps(warmup, use_graph=True)
+        w.ibm.prepare(steps)
     capi.call("bl_sync")
     t0 = time.perf_counter()
     for w in ws:
@@ -270,7 +320,7 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     out = dict(mlups=round(cells * steps / (ms * 1e-3) / 1e6, 1), ms_per_step=round(ms / steps, 4),
                per_box_loop_mlups=round(cells * steps / loop_s / 1e6, 1),
                deformation_index=dict(min=float(D.min()), median=float(np.median(D)), max=float(D.max())),
-               tuned_launch=tuned,
+               tuned_launch=best,
                workload="%d shear boxes, %d cells, sides %s, one 642-vertex biconcave cell each, FP64, "
                         "steps %d after %d warmup" % (n_boxes, cells, sorted({w.params["s"] for w in ws}), steps,
                                                       warmup))
@@ -281,19 +331,51 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     return out
 
 
-def progress(msg):
-    """One line on stderr per finished leg (the JSON line stays alone on stdout)."""
-    sys.stderr.write("bench: %s\n" % msg)
-    sys.stderr.flush()
+def pipe_leg(steps, warmup):
+    from biblbm_amd import workloads
+    out = {}
+    for storage in ("f64", "f32"):
+        W = workloads.rbc_in_pipe(storage)
+        dt = time_coupled(W, steps, warmup)
+        rl = roofline(W, storage, True)
+        rl["note"] = "populations of this lattice fit the 256 MB last-level cache: cache-assisted figure"
+        out[storage] = dict(mlups=round(W.cells * steps / dt / 1e6, 2), ms_per_step=round(dt / steps * 1e3, 5),
+                            fluid_mlups=round(W.fluid_cells * steps / dt / 1e6, 2), roofline=rl)
+        W.ibm.close()
+        W.lattice.close()
+    W2 = workloads.rbc_in_pipe("f64", second_order=True)
+    n2 = max(2, steps // 2)
+    dt2 = time_coupled(W2, n2, max(2, warmup // 2))
+    out["second_order"] = dict(mlups=round(W2.cells * n2 / dt2 / 1e6, 2), ms_per_step=round(dt2 / n2 * 1e3, 5),
+                               wall_nodes=W2.params["wall_nodes"], fluid_cells=W2.fluid_cells)
+    W2.ibm.close()
+    W2.guo.close()
+    W2.lattice.close()
+    out["workload"] = ("rbc_in_pipe: pipe R=32 L=257 (257x67x67 cells, bounce-back; second_order: Guo curved "
+                       "wall, no dynamics beyond it), tau 0.9, body force for u_c 0.02, one 2562-vertex "
+                       "Evans-Fung red cell R=12 (Skalak + bending + area + volume), Peskin")
+    return out
+
+
+def capsule_leg(steps, warmup):
+    from biblbm_amd import workloads
+    W = workloads.capsule_in_shear("f64")
+    dt = time_coupled(W, steps, warmup)
+    out = dict(mlups=round(W.cells * steps / dt / 1e6, 2), ms_per_step=round(dt / steps * 1e3, 5),
+               workload="capsule_in_shear 64^3, velocity walls +-0.01, tau 1, Skalak capsule 642 vertices R=8, "
+                        "Ca 0.1, Peskin, FP64")
+    W.ibm.close()
+    W.lattice.close()
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f64", choices=["f64", "f32", "f16s"])
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -303,67 +385,54 @@ def main():
         sys.exit("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design, "
                  "not an implementation); run with --gpus 1")
 
-    import numpy as np
-    from biblbm_amd import capi, workloads
+    from biblbm_amd import capi, tuning, workloads
 
     capi.call("bl_init", 0)
-    W = workloads.rbc_in_pipe(args.storage)
+    tuning.load()
+    W = workloads.taylor_green(args.storage)
+    tuned = tuning.tune_collide(W.lattice)["best"]
+    progress("headline workload built, collide launch shapes tuned")
     dt = time_coupled(W, args.steps, args.warmup)
     value = W.cells * args.steps / dt / 1e6
     W.ibm.step()
     stages = {k: round(v, 5) for k, v in W.ibm.stage_times().items()}
-    rl = roofline(W, args.storage)
+    rl = roofline(W, args.storage, False)
+    rl["tuned"] = tuned
+    rl["device_copy_gbs_4gib"] = copy_bandwidth()
+    rl["copy_frac"] = round(rl["device_copy_gbs_4gib"] / HBM_PEAK_GBS, 4)
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.storage, "data": "synthetic",
-        "config": {"workload": "rbc_in_pipe: pipe R=32 L=257 (257x67x67 cells, bounce-back), tau 0.9, body "
-                               "force for u_c 0.02, one 2562-vertex Evans-Fung red cell R=12 (Skalak + bending "
-                               "+ area + volume), Peskin kernel, hipGraph-replayed coupled steps",
-                   "cells": W.cells, "fluid_cells": W.fluid_cells, "vertices": W.vertices,
-                   "storage": args.storage, "parallelism": "single GPU"},
+        "config": {"workload": "precision: 384^3 periodic Taylor-Green box (u0 0.03, tau 0.8) with 16 Skalak "
+                               "capsules (642-vertex icospheres, radius 8, Ca 0.1) at seeded positions, Peskin "
+                               "kernel, hipGraph-replayed coupled steps",
+                   "cells": W.cells, "vertices": W.vertices, "storage": args.storage, "parallelism": "single GPU"},
         "roofline": rl,
         "ibm_stage_ms": stages,
-        "fluid_mlups": round(W.fluid_cells * args.steps / dt / 1e6, 2),
         "vertex_updates_per_s": round(W.vertices * args.steps / dt, 1),
     }
     W.ibm.close()
     W.lattice.close()
+    del W
+    progress("headline done: %.1f MLUPS" % value)
     if not args.no_extras:
-        W32 = workloads.rbc_in_pipe("f32")
-        dt32 = time_coupled(W32, args.steps, args.warmup)
-        line["fp32"] = dict(value=round(W32.cells * args.steps / dt32 / 1e6, 2),
-                            ms_per_step=round(dt32 / args.steps * 1e3, 5),
-                            roofline=roofline(W32, "f32"))
-        W32.ibm.close()
-        W32.lattice.close()
-        W2 = workloads.rbc_in_pipe("f64", second_order=True)
-        dt2 = time_coupled(W2, args.steps // 2, args.warmup // 2)
-        line["second_order"] = dict(value=round(W2.cells * (args.steps // 2) / dt2 / 1e6, 2),
-                                    ms_per_step=round(dt2 / (args.steps // 2) * 1e3, 5),
-                                    wall_nodes=W2.params["wall_nodes"], fluid_cells=W2.fluid_cells)
-        W2.ibm.close()
-        W2.guo.close()
-        W2.lattice.close()
-        progress("rbc_in_pipe f64 / f32 / second order done")
-        line["suspension"] = suspension_leg(max(50, args.steps // 10), 20)
+        line["capsule_in_shear"] = capsule_leg(max(50, args.steps), 10)
+        line["rbc_in_pipe"] = pipe_leg(max(100, args.steps), 20)
+        progress("capsule_in_shear / rbc_in_pipe done")
+        line["suspension"] = suspension_leg(max(50, args.steps // 4), 10)
         progress("suspension done")
-        line["interactions"] = interactions_leg(max(50, args.steps // 10), 20,
+        line["interactions"] = interactions_leg(max(20, args.steps // 10), 10,
                                                 line["suspension"]["f32"]["ms_per_step"])
         progress("interactions done")
-        line["ensemble"] = ensemble_leg(max(50, args.steps // 10), 10)
+        line["ensemble"] = ensemble_leg(max(50, args.steps // 4), 10)
         progress("ensemble done")
-        line["precision"] = precision_leg(max(300, args.steps // 4))
+        line["precision"] = precision_leg(max(300, args.steps))
         progress("precision done")
     if not args.no_cpu_baseline:
-        cb, cpu_pos = cpu_baseline(args.cpu_steps, args.cpu_budget)
-        # the same number of coupled steps on the GPU from the same state: parity check
-        n = int(cb["sample"].split()[0])
-        G = workloads.rbc_in_pipe("f64")
-        G.ibm.coupled_steps(n, use_graph=False)
-        gpos = G.ibm.positions()
-        cb["gpu_vs_cpu_position_rel_l2"] = float(np.linalg.norm(gpos - cpu_pos) / np.linalg.norm(cpu_pos))
-        line["cpu_baseline"] = cb
+        line["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_budget)
+        progress("cpu baseline done")
+    tuning.save()
     print(json.dumps(line), flush=True)
 
 

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbiblbm.so")
 
-SOURCES = ["errors.cpp", "lattice.hip", "ibm.hip", "mesh.cpp", "ensemble.hip", "walls.hip", "interactions.hip", "io.cpp", "init.cpp"]
+SOURCES = ["errors.cpp", "lattice.hip", "ibm.hip", "mesh.cpp", "ensemble.hip", "walls.hip", "interactions.hip", "io.cpp",
+           "init.cpp", "tune.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

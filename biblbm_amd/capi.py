@@ -47,6 +47,8 @@ SIGNATURES = {
     "bl_lattice_conserved_static": [_P, _P],
     "bl_lattice_steps": [_P, _PLL],
     "bl_lattice_set_launch": [_P, _I, _I],
+    "bl_lattice_set_launch_parity": [_P, _I, _I, _I, _I],
+    "bl_lattice_get_launch": [_P, _I, _P],
     "bl_lattice_stream": [_P, ctypes.POINTER(_P)],
     "bl_lattice_time_collide": [_P, _I, _PD],
     "bl_sync": [],
@@ -63,6 +65,7 @@ SIGNATURES = {
     "bl_ibm_move": [_P],
     "bl_ibm_step": [_P],
     "bl_coupled_steps": [_P, _I, _I],
+    "bl_coupled_prepare": [_P, _I],
     "bl_ibm_count": [_P, _PI, _PI],
     "bl_particle_get": [_P, _I, _I, _P],
     "bl_particle_set_positions": [_P, _I, _P],
@@ -92,6 +95,10 @@ SIGNATURES = {
     "bl_ensemble_steps": [_P, _I, _I],
     "bl_ensemble_time_steps": [_P, _I, _I, _PD],
     "bl_tune_collide": [_P, ctypes.c_char_p, _I],
+    "bl_tune_coupled": [_P, _I, ctypes.c_char_p, _I],
+    "bl_tune_apply_coupled": [_P],
+    "bl_tune_cache_save": [ctypes.c_char_p],
+    "bl_tune_cache_load": [ctypes.c_char_p],
     "bl_guo_wall_create": [_P, _P, ctypes.POINTER(_P)],
     "bl_guo_wall_destroy": [_P],
     "bl_guo_wall_count": [_P, _PLL],

@@ -1,5 +1,4 @@
-// Ensembles of independent boxes advanced together, and the launch-shape
-// tuner of the collide-and-stream kernel.
+// Ensembles of independent boxes advanced together.
 //
 // An ensemble groups lattices (each optionally with its immersed boundary)
 // created through the ordinary entry points.  One hipGraph holds two
@@ -233,78 +232,6 @@ int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double* ms
     hipEventDestroy(b);
     for (bl_ibm g : e->ibms)
         if (g && (rc = ibm_check_overflow(g))) return rc;
-    return BL_OK;
-}
-
-int bl_tune_collide(bl_lattice L, char* report, int cap)
-{
-    if (!L) return fail(BL_ERR_ARG, "bl_tune_collide: null lattice");
-    static const int blocks[] = {64, 128, 192, 256};
-    const size_t sz = L->storage == BL_STORE_F64 ? 8 : (L->storage == BL_STORE_F32 ? 4 : 2);
-    const size_t bytes = (size_t)Q * L->stride * sz;
-    void* saved = nullptr;
-    BL_HIP(hipMalloc(&saved, bytes));
-    BL_HIP(hipMemcpyAsync(saved, L->pops, bytes, hipMemcpyDeviceToDevice, L->stream));
-    const long long steps0 = L->steps;
-    const int block0 = L->block, remap0 = L->xcd_remap;
-    hipEvent_t a, b;
-    BL_HIP(hipEventCreate(&a));
-    BL_HIP(hipEventCreate(&b));
-    std::string rep = "{\"trials\": [";
-    double best = 1e30;
-    int best_block = block0, best_remap = remap0;
-    int rc = BL_OK;
-    bool first = true;
-    for (int remap = 0; remap < 2 && rc == BL_OK; ++remap)
-        for (int bi = 0; bi < 4 && rc == BL_OK; ++bi) {
-            L->block = blocks[bi];
-            L->xcd_remap = remap;
-            for (int k = 0; k < 2 && rc == BL_OK; ++k) rc = collide_stream_launch(L);
-            if (rc) break;
-            const int n = 10;
-            hipEventRecord(a, L->stream);
-            for (int k = 0; k < n && rc == BL_OK; ++k) rc = collide_stream_launch(L);
-            hipEventRecord(b, L->stream);
-            if (rc) break;
-            hipEventSynchronize(b);
-            float t = 0.f;
-            hipEventElapsedTime(&t, a, b);
-            const double ms = (double)t / n;
-            char buf[128];
-            snprintf(buf, sizeof(buf), "%s{\"block\": %d, \"xcd_remap\": %d, \"ms\": %.6f}", first ? "" : ", ",
-                     blocks[bi], remap, ms);
-            rep += buf;
-            first = false;
-            if (ms < best) {
-                best = ms;
-                best_block = blocks[bi];
-                best_remap = remap;
-            }
-        }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    // the trials advanced the state: put the populations and the step count back
-    hipMemcpyAsync(L->pops, saved, bytes, hipMemcpyDeviceToDevice, L->stream);
-    hipStreamSynchronize(L->stream);
-    hipFree(saved);
-    L->steps = steps0;
-    if (rc) {
-        L->block = block0;
-        L->xcd_remap = remap0;
-        return rc;
-    }
-    L->block = best_block;
-    L->xcd_remap = best_remap;
-    ++L->gen;  // captured steps hold the old launch shape
-    char buf[128];
-    snprintf(buf, sizeof(buf), "], \"best\": {\"block\": %d, \"xcd_remap\": %d, \"ms\": %.6f}}", best_block,
-             best_remap, best);
-    rep += buf;
-    if (report && cap > 0) {
-        const size_t n = std::min<size_t>(rep.size(), (size_t)cap - 1);
-        rep.copy(report, n);
-        report[n] = '\0';
-    }
     return BL_OK;
 }
 

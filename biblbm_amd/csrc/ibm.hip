@@ -2805,6 +2805,21 @@ int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph)
     return check_overflow(g);
 }
 
+int bl_coupled_prepare(bl_ibm g, int n_steps)
+{
+    int rc = need_stencil(g);
+    if (rc) return rc;
+    if (n_steps < 0) return fail(BL_ERR_ARG, "bl_coupled_prepare: negative step count");
+    if (ibm_needs_settle(g))
+        return fail(BL_ERR_STATE, "bl_coupled_prepare: the next step runs eagerly (a loaded force field or a changed "
+                                  "body force); run it first");
+    // the graphs bl_coupled_steps(g, n_steps, 1) replays from this state: runs
+    // of kLongRun steps keep the step parity, so both are captured for it
+    if (n_steps >= kLongRun && (rc = ensure_graph(g, 1, kLongRun))) return rc;
+    if (n_steps % kLongRun >= 2 && (rc = ensure_graph(g, 0, 2))) return rc;
+    return BL_OK;
+}
+
 int bl_ibm_count(bl_ibm g, int* particles, int* vertices)
 {
     if (!g) return fail(BL_ERR_ARG, "null ibm handle");

@@ -57,7 +57,7 @@ LatView view_of(const bl_lattice_s* L)
     v.ntx = L->ntx; v.nty = L->nty; v.ntz = L->ntz;
     v.wall_axis = L->wall_axis;
     for (int k = 0; k < 6; ++k) v.wall_u[k] = L->wall_u[k];
-    v.xcd_remap = L->xcd_remap;
+    v.xcd_remap = L->xcd_remap[0];
     v.emit_u = L->emit_u;
     v.emit_force = L->emit_force;
     v.emit_flags = L->emit_flags;
@@ -85,15 +85,15 @@ __device__ __forceinline__ K odd_slot(const NbrOff& nb, bool inner, int sx, int 
 // EMIT: the velocity staging of an immersed-boundary step rides on the
 // launch (bl_lattice_s::emit_u): cells of flagged tiles also write the
 // computeVelocity value of the populations they loaded, with the last
-// collision's force field.
-template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT = false>
+// collision's force field.  NT: the even step's population loads and stores
+// carry the nontemporal hint.
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT = false, bool NT = false>
 __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 {
     using P = Pop<S>;
     using C = typename P::C;
     using FT = typename P::F;
-    unsigned bid = blockIdx.x;
-    if (v.xcd_remap) bid = xcd_block(bid, gridDim.x);
+    const unsigned bid = collide_block(blockIdx.x, gridDim.x, v.xcd_remap);
     const unsigned c = bid * blockDim.x + threadIdx.x;
     if (c >= v.ncell) return;
     int x, y, z;
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = P::ld(A + i * st, c, i);
+        for (int i = 0; i < Q; ++i) f[i] = P::template ld<unsigned, NT>(A + i * st, c, i);
     }
 
     const unsigned char m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
@@ -158,13 +158,39 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) P::st(A + opp(i) * st, c, i, f[i]);
+        for (int i = 0; i < Q; ++i) P::template st<unsigned, NT>(A + opp(i) * st, c, i, f[i]);
     }
 }
 
-template <class S, bool ODD, bool MASK, int FMODE>
-void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
+// the even step with nontemporal hints (the odd step's misaligned z +-1
+// accesses rely on the L2 merging partial lines, so it keeps the default)
+template <class S, bool MASK, int FMODE>
+void launch_cs_nt(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
 {
+    if constexpr (FMODE == 2) {
+        if (v.emit_u) {
+            hipLaunchKernelGGL((k_collide_stream<S, false, MASK, 2, false, unsigned, true, true>), dim3(nblk), dim3(blk),
+                               0, s, v);
+            return;
+        }
+    }
+    if (phase)
+        hipLaunchKernelGGL((k_collide_stream<S, false, MASK, FMODE, true, unsigned, false, true>), dim3(nblk), dim3(blk),
+                           0, s, v);
+    else
+        hipLaunchKernelGGL((k_collide_stream<S, false, MASK, FMODE, false, unsigned, false, true>), dim3(nblk), dim3(blk),
+                           0, s, v);
+}
+
+template <class S, bool ODD, bool MASK, int FMODE>
+void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s, bool nt)
+{
+    if constexpr (!ODD) {
+        if (nt) {
+            launch_cs_nt<S, MASK, FMODE>(v, phase, nblk, blk, s);
+            return;
+        }
+    }
     const bool big = ODD && (size_t)Q * (size_t)v.stride > 0xffffffffull;
     if constexpr (FMODE == 2) {
         if (v.emit_u) {  // the launcher passes emit_u only without a phase field
@@ -191,23 +217,23 @@ void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_
 }
 
 template <class S, bool ODD, bool MASK>
-void launch_cs2(const LatView& v, int fmode, bool phase, unsigned nblk, int blk, hipStream_t s)
+void launch_cs2(const LatView& v, int fmode, bool phase, unsigned nblk, int blk, hipStream_t s, bool nt)
 {
-    if (fmode == 0) launch_cs3<S, ODD, MASK, 0>(v, phase, nblk, blk, s);
-    else if (fmode == 1) launch_cs3<S, ODD, MASK, 1>(v, phase, nblk, blk, s);
-    else launch_cs3<S, ODD, MASK, 2>(v, phase, nblk, blk, s);
+    if (fmode == 0) launch_cs3<S, ODD, MASK, 0>(v, phase, nblk, blk, s, nt);
+    else if (fmode == 1) launch_cs3<S, ODD, MASK, 1>(v, phase, nblk, blk, s, nt);
+    else launch_cs3<S, ODD, MASK, 2>(v, phase, nblk, blk, s, nt);
 }
 
 template <class S>
 void launch_cs(const LatView& v, bool odd, bool mask, int fmode, bool phase, unsigned nblk, int blk,
-               hipStream_t s)
+               hipStream_t s, bool nt)
 {
     if (odd) {
-        if (mask) launch_cs2<S, true, true>(v, fmode, phase, nblk, blk, s);
-        else launch_cs2<S, true, false>(v, fmode, phase, nblk, blk, s);
+        if (mask) launch_cs2<S, true, true>(v, fmode, phase, nblk, blk, s, nt);
+        else launch_cs2<S, true, false>(v, fmode, phase, nblk, blk, s, nt);
     } else {
-        if (mask) launch_cs2<S, false, true>(v, fmode, phase, nblk, blk, s);
-        else launch_cs2<S, false, false>(v, fmode, phase, nblk, blk, s);
+        if (mask) launch_cs2<S, false, true>(v, fmode, phase, nblk, blk, s, nt);
+        else launch_cs2<S, false, false>(v, fmode, phase, nblk, blk, s, nt);
     }
 }
 
@@ -226,10 +252,14 @@ int collide_stream_launch(bl_lattice_s* L)
     if (fm != 2) v.flags = nullptr;
     const bool phase = L->phase != nullptr && L->omega_c0 != L->omega_c1;
     if (fm != 2 || phase) v.emit_u = nullptr;
-    const unsigned nblk = (unsigned)((L->ncell + L->block - 1) / L->block);
-    if (L->storage == BL_STORE_F64) launch_cs<double>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
-    else if (L->storage == BL_STORE_F32) launch_cs<float>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
-    else launch_cs<__half>(v, odd, L->mask != nullptr, fm, phase, nblk, L->block, L->stream);
+    const int par = odd ? 1 : 0;
+    const int blk = L->block[par];
+    v.xcd_remap = L->xcd_remap[par];
+    const bool nt = L->nontemporal[par] != 0;
+    const unsigned nblk = (unsigned)((L->ncell + blk - 1) / blk);
+    if (L->storage == BL_STORE_F64) launch_cs<double>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
+    else if (L->storage == BL_STORE_F32) launch_cs<float>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
+    else launch_cs<__half>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
     BL_CHECK_LAUNCH();
     L->steps += 1;
     for (int d = 0; d < 3; ++d) L->g_last[d] = L->g[d];
@@ -752,6 +782,7 @@ int bl_lattice_create(int nx, int ny, int nz, const int periodic[3], int storage
         delete L;
         return fail(BL_ERR_HIP, std::string("bl_lattice_create: ") + hipGetErrorString(e));
     }
+    tune_cache_apply(L);
     *out = L;
     return bl_lattice_init_equilibrium(L, nullptr, nullptr);
 }
@@ -1067,12 +1098,34 @@ int bl_lattice_steps(bl_lattice L, long long* steps)
 
 int bl_lattice_set_launch(bl_lattice L, int block, int xcd_remap)
 {
+    int rc = bl_lattice_set_launch_parity(L, 0, block, xcd_remap ? 1 : 0, 0);
+    return rc ? rc : bl_lattice_set_launch_parity(L, 1, block, xcd_remap ? 1 : 0, 0);
+}
+
+int bl_lattice_set_launch_parity(bl_lattice L, int parity, int block, int xcd_remap, int nontemporal)
+{
     if (!L) return fail(BL_ERR_ARG, "null lattice");
+    if (parity != 0 && parity != 1) return fail(BL_ERR_ARG, "bl_lattice_set_launch_parity: parity must be 0 or 1");
     // the collide-stream kernels are compiled for at most 256 threads per block
-    if (block < 64 || block > 256 || block % 64) return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be 64, 128, 192 or 256");
-    L->block = block;
-    L->xcd_remap = xcd_remap ? 1 : 0;
+    if (block < 64 || block > 256 || block % 64)
+        return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be 64, 128, 192 or 256");
+    if (nontemporal && parity == 1)
+        return fail(BL_ERR_ARG, "bl_lattice_set_launch_parity: nontemporal hints are for the even step only");
+    if (xcd_remap < 0 || xcd_remap > (1 << 20) || (xcd_remap > 1 && (xcd_remap & (xcd_remap - 1))))
+        return fail(BL_ERR_ARG, "bl_lattice_set_launch: xcd_remap must be 0, 1 or a power of two up to 2^20");
+    L->block[parity] = block;
+    L->xcd_remap[parity] = xcd_remap;
+    L->nontemporal[parity] = nontemporal ? 1 : 0;
     ++L->gen;
+    return BL_OK;
+}
+
+int bl_lattice_get_launch(bl_lattice L, int parity, int out[3])
+{
+    if (!L || !out || (parity != 0 && parity != 1)) return fail(BL_ERR_ARG, "bl_lattice_get_launch: bad argument");
+    out[0] = L->block[parity];
+    out[1] = L->xcd_remap[parity];
+    out[2] = L->nontemporal[parity];
     return BL_OK;
 }
 

@@ -60,8 +60,13 @@ struct bl_lattice_s {
     unsigned long long gen = 0;
     hipStream_t stream = nullptr;
 
-    int block = 256;
-    int xcd_remap = 1;
+    // launch shape of the collide-and-stream kernel per step parity (index 0
+    // even, 1 odd): threads per block, XCD-aware block order, nontemporal
+    // population loads and stores (even step only; set by the tuner or a
+    // cached tuning result)
+    int block[2] = {256, 256};
+    int xcd_remap[2] = {1, 1};
+    int nontemporal[2] = {0, 0};
 
     // scratch
     double* d_stage = nullptr;  // ncell * 19 doubles, allocated on demand
@@ -112,6 +117,9 @@ int ensure_force_field(bl_lattice_s* L);
 int collide_stream_launch(bl_lattice_s* L);
 int velocity_walls_launch(bl_lattice_s* L);  // post-stream passes: velocity walls, then Guo walls
 int guo_walls_launch(bl_lattice_s* L);
+// Launch shapes of an earlier tuning of a lattice of this size on this
+// device (tune.cpp), if the tuning table holds one.
+void tune_cache_apply(bl_lattice_s* L);
 void guo_walls_detach(bl_lattice_s* L);
 
 }  // namespace bl

@@ -8,39 +8,66 @@
 namespace bl {
 
 // ------------------------------------------------------------ storage I/O
+// Nontemporal loads and stores (streaming hint: the line is not expected to
+// be reused); __half goes through its 16-bit pattern.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+template <>
+__device__ __forceinline__ __half ld_nt(const __half* p)
+{
+    return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
+}
+template <>
+__device__ __forceinline__ void st_nt(__half* p, __half v)
+{
+    __builtin_nontemporal_store(__half_as_ushort(v), reinterpret_cast<unsigned short*>(p));
+}
+
 template <class S>
 struct Pop;
 template <>
 struct Pop<double> {
     using C = double;
     using F = double;
-    template <class K>
-    __device__ static C ld(const double* A, K k, int) { return A[k]; }
-    template <class K>
-    __device__ static void st(double* A, K k, int, C v) { A[k] = v; }
+    template <class K, bool NT = false>
+    __device__ static C ld(const double* A, K k, int) { return NT ? ld_nt(A + k) : A[k]; }
+    template <class K, bool NT = false>
+    __device__ static void st(double* A, K k, int, C v)
+    {
+        if (NT) st_nt(A + k, v);
+        else A[k] = v;
+    }
 };
 template <>
 struct Pop<float> {
     using C = float;
     using F = float;
-    template <class K>
-    __device__ static C ld(const float* A, K k, int) { return A[k]; }
-    template <class K>
-    __device__ static void st(float* A, K k, int, C v) { A[k] = v; }
+    template <class K, bool NT = false>
+    __device__ static C ld(const float* A, K k, int) { return NT ? ld_nt(A + k) : A[k]; }
+    template <class K, bool NT = false>
+    __device__ static void st(float* A, K k, int, C v)
+    {
+        if (NT) st_nt(A + k, v);
+        else A[k] = v;
+    }
 };
 template <>
 struct Pop<__half> {
     using C = float;
     using F = float;
-    template <class K>
+    template <class K, bool NT = false>
     __device__ static C ld(const __half* A, K k, int i)
     {
-        return __half2float(A[k]) * (float)(tw(i) / kF16Scale);
+        return __half2float(NT ? ld_nt(A + k) : A[k]) * (float)(tw(i) / kF16Scale);
     }
-    template <class K>
+    template <class K, bool NT = false>
     __device__ static void st(__half* A, K k, int i, C v)
     {
-        A[k] = __float2half_rn(v * (float)(kF16Scale / tw(i)));
+        const __half h = __float2half_rn(v * (float)(kF16Scale / tw(i)));
+        if (NT) st_nt(A + k, h);
+        else A[k] = h;
     }
 };
 
@@ -59,6 +86,28 @@ __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nb)
     const unsigned per = nb >> 3, rem = nb & 7u;
     const unsigned xcd = bid & 7u, k = bid >> 3;
     return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+// Between the two: chunks of 2^lc consecutive blocks dealt round-robin over
+// the XCDs, so each XCD's L2 sees runs of 2^lc blocks (neighbouring blocks
+// share partial cache lines there) while all XCDs stream from nearby
+// addresses.  Blocks past the last whole round keep their own index.
+__device__ __forceinline__ unsigned xcd_chunk_block(unsigned bid, unsigned nb, unsigned lc)
+{
+    const unsigned span = 8u << lc;
+    const unsigned full = nb / span * span;
+    if (bid >= full) return bid;
+    const unsigned x = bid & 7u, k = bid >> 3;
+    return ((((k >> lc) << 3) + x) << lc) + (k & ((1u << lc) - 1u));
+}
+
+// The block order of the collide launch: mode 0 the hardware's, 1 one
+// contiguous range per XCD, 2^k (k >= 1) chunks of 2^k blocks.
+__device__ __forceinline__ unsigned collide_block(unsigned bid, unsigned nb, int mode)
+{
+    if (mode == 0) return bid;
+    if (mode == 1) return xcd_block(bid, nb);
+    return xcd_chunk_block(bid, nb, (unsigned)(31 - __builtin_clz((unsigned)mode)));
 }
 
 struct Nbr {

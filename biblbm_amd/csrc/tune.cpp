@@ -1,0 +1,376 @@
+// The project's autotuner: launch shapes of the collide-and-stream kernel
+// per AA step parity, and the launch structure of coupled steps; results
+// kept in a per-process table that can be written to and read from a file
+// (one JSON object per line) and that a new lattice of a tuned size picks up
+// at creation.  The reference has no tuner: its block decomposition is fixed
+// (src/multiBlock/defaultMultiBlockPolicy3D.h:119-129).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "ibm.hpp"
+#include "lattice.hpp"
+
+namespace bl {
+namespace {
+
+struct Launch {
+    int block = 256, xcd = 1, nt = 0;
+    double ms = 0.0;
+};
+
+struct CollideEntry {
+    std::string device;
+    int nx = 0, ny = 0, nz = 0, storage = 0;
+    Launch par[2];
+};
+
+struct CoupledEntry {
+    std::string device;
+    int nx = 0, ny = 0, nz = 0, storage = 0;
+    long vertices = 0;
+    int fused = -1, layered = -1;
+    double ms = 0.0;
+};
+
+std::vector<CollideEntry> g_collide;
+std::vector<CoupledEntry> g_coupled;
+
+std::string device_arch(int device)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
+    return std::string(p.gcnArchName);
+}
+
+template <class E>
+E* find_entry(std::vector<E>& v, const std::string& dev, const bl_lattice_s* L)
+{
+    for (E& e : v)
+        if (e.device == dev && e.nx == L->nx && e.ny == L->ny && e.nz == L->nz && e.storage == L->storage) return &e;
+    return nullptr;
+}
+
+void put_report(const std::string& rep, char* report, int cap)
+{
+    if (report && cap > 0) {
+        const size_t n = std::min<size_t>(rep.size(), (size_t)cap - 1);
+        rep.copy(report, n);
+        report[n] = '\0';
+    }
+}
+
+}  // namespace
+
+// A lattice of a tuned size on a tuned device starts with the tuned shapes.
+void tune_cache_apply(bl_lattice_s* L)
+{
+    const std::string dev = device_arch(L->device);
+    const CollideEntry* e = find_entry(g_collide, dev, L);
+    if (!e) return;
+    for (int p = 0; p < 2; ++p) {
+        L->block[p] = e->par[p].block;
+        L->xcd_remap[p] = e->par[p].xcd;
+        L->nontemporal[p] = p == 0 ? e->par[p].nt : 0;
+    }
+    ++L->gen;
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" {
+
+int bl_tune_collide(bl_lattice L, char* report, int cap)
+{
+    if (!L) return fail(BL_ERR_ARG, "bl_tune_collide: null lattice");
+    // candidates: threads per block x block order x nontemporal hints (even
+    // step only); every even candidate is timed once, every odd one over
+    // the same number of launches
+    static const int blocks[] = {64, 128, 192, 256};
+    static const int orders[] = {0, 1, 4, 16, 64};  // block orders (see bl_lattice_set_launch_parity)
+    std::vector<Launch> cand[2];
+    for (int xcd : orders)
+        for (int nt = 0; nt < 2; ++nt)
+            for (int b : blocks) {
+                Launch l;
+                l.block = b;
+                l.xcd = xcd;
+                l.nt = nt;
+                cand[0].push_back(l);
+                if (nt == 0) cand[1].push_back(l);
+            }
+    const size_t sz = L->pop_bytes;
+    const size_t bytes = (size_t)Q * L->stride * sz;
+    void* saved = nullptr;
+    BL_HIP(hipMalloc(&saved, bytes));
+    BL_HIP(hipMemcpyAsync(saved, L->pops, bytes, hipMemcpyDeviceToDevice, L->stream));
+    const long long steps0 = L->steps;
+    int keep_block[2] = {L->block[0], L->block[1]}, keep_xcd[2] = {L->xcd_remap[0], L->xcd_remap[1]},
+        keep_nt[2] = {L->nontemporal[0], L->nontemporal[1]};
+    hipEvent_t ev[3];
+    for (auto& e : ev) BL_HIP(hipEventCreate(&e));
+    int rc = BL_OK;
+    if (L->steps & 1) rc = collide_stream_launch(L);  // trials start on an even step
+    // trial k: even candidate k, odd candidate k mod (odd candidates); every
+    // candidate of a parity is timed over the same number of launches
+    const size_t ntrial = std::max(cand[0].size(), cand[1].size());
+    const int reps = 4;
+    for (size_t k = 0; k < ntrial && rc == BL_OK; ++k) {
+        const Launch& e = cand[0][k % cand[0].size()];
+        const Launch& o = cand[1][k % cand[1].size()];
+        L->block[0] = e.block; L->xcd_remap[0] = e.xcd; L->nontemporal[0] = e.nt;
+        L->block[1] = o.block; L->xcd_remap[1] = o.xcd; L->nontemporal[1] = 0;
+        for (int w = 0; w < 2 && rc == BL_OK; ++w) rc = collide_stream_launch(L);
+        double te = 0, to = 0;
+        for (int r = 0; r < reps && rc == BL_OK; ++r) {
+            hipEventRecord(ev[0], L->stream);
+            rc = collide_stream_launch(L);
+            hipEventRecord(ev[1], L->stream);
+            if (rc == BL_OK) rc = collide_stream_launch(L);
+            hipEventRecord(ev[2], L->stream);
+            if (rc) break;
+            hipEventSynchronize(ev[2]);
+            float a = 0.f, b = 0.f;
+            hipEventElapsedTime(&a, ev[0], ev[1]);
+            hipEventElapsedTime(&b, ev[1], ev[2]);
+            te += a;
+            to += b;
+        }
+        cand[0][k % cand[0].size()].ms += te / reps;
+        cand[1][k % cand[1].size()].ms += to / reps;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    // the trials advanced the state: put the populations and the step count back
+    hipMemcpyAsync(L->pops, saved, bytes, hipMemcpyDeviceToDevice, L->stream);
+    hipStreamSynchronize(L->stream);
+    hipFree(saved);
+    L->steps = steps0;
+    if (rc) {
+        for (int p = 0; p < 2; ++p) {
+            L->block[p] = keep_block[p];
+            L->xcd_remap[p] = keep_xcd[p];
+            L->nontemporal[p] = keep_nt[p];
+        }
+        return rc;
+    }
+    Launch best[2];
+    std::string rep = "{";
+    for (int p = 0; p < 2; ++p) {
+        const double per = (double)ntrial / (double)cand[p].size();  // trials each candidate took part in
+        best[p].ms = 1e30;
+        rep += p == 0 ? "\"even\": [" : "], \"odd\": [";
+        for (size_t k = 0; k < cand[p].size(); ++k) {
+            Launch c = cand[p][k];
+            c.ms /= per;
+            char buf[128];
+            snprintf(buf, sizeof(buf), "%s{\"block\": %d, \"xcd_remap\": %d, \"nontemporal\": %d, \"ms\": %.6f}",
+                     k ? ", " : "", c.block, c.xcd, c.nt, c.ms);
+            rep += buf;
+            if (c.ms < best[p].ms) best[p] = c;
+        }
+        L->block[p] = best[p].block;
+        L->xcd_remap[p] = best[p].xcd;
+        L->nontemporal[p] = best[p].nt;
+    }
+    ++L->gen;  // captured steps hold the old launch shapes
+    char buf[256];
+    snprintf(buf, sizeof(buf),
+             "], \"best\": {\"even\": {\"block\": %d, \"xcd_remap\": %d, \"nontemporal\": %d, \"ms\": %.6f}, "
+             "\"odd\": {\"block\": %d, \"xcd_remap\": %d, \"nontemporal\": %d, \"ms\": %.6f}}}",
+             best[0].block, best[0].xcd, best[0].nt, best[0].ms, best[1].block, best[1].xcd, best[1].nt, best[1].ms);
+    rep += buf;
+    // remember the result for lattices of this size
+    const std::string dev = device_arch(L->device);
+    CollideEntry* e = find_entry(g_collide, dev, L);
+    if (!e) {
+        g_collide.emplace_back();
+        e = &g_collide.back();
+        e->device = dev;
+        e->nx = L->nx; e->ny = L->ny; e->nz = L->nz; e->storage = L->storage;
+    }
+    e->par[0] = best[0];
+    e->par[1] = best[1];
+    put_report(rep, report, cap);
+    return BL_OK;
+}
+
+int bl_tune_coupled(bl_ibm g, int n_steps, char* report, int cap)
+{
+    if (!g || n_steps < 2 || n_steps % 2) return fail(BL_ERR_ARG, "bl_tune_coupled: n_steps must be even and >= 2");
+    int rc = need_stencil(g);
+    if (rc) return rc;
+    bl_lattice L = g->lat;
+    const int keep_fused = g->fused_staging, keep_layered = g->layered;
+    const bool phase = L->phase != nullptr && L->omega_c0 != L->omega_c1;
+    struct Trial {
+        int fused, layered;
+        double ms;
+    };
+    std::vector<Trial> trials;
+    for (int f = 0; f < (phase ? 1 : 2); ++f)
+        for (int l = 0; l < 2; ++l) trials.push_back(Trial{f, l, 0.0});
+    hipEvent_t a, b;
+    BL_HIP(hipEventCreate(&a));
+    BL_HIP(hipEventCreate(&b));
+    std::string rep = "{\"trials\": [";
+    Trial best{keep_fused, keep_layered, 1e30};
+    for (size_t k = 0; k < trials.size() && rc == BL_OK; ++k) {
+        Trial& t = trials[k];
+        if ((rc = bl_ibm_set_fused_staging(g, t.fused)) || (rc = bl_ibm_set_layered(g, t.layered))) break;
+        // a first run captures this layout's graphs (and settles pending state)
+        if ((rc = bl_coupled_steps(g, n_steps, 1))) break;
+        hipEventRecord(a, L->stream);
+        if ((rc = bl_coupled_steps(g, n_steps, 1))) break;
+        hipEventRecord(b, L->stream);
+        hipEventSynchronize(b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        t.ms = (double)ms / n_steps;
+        char buf[128];
+        snprintf(buf, sizeof(buf), "%s{\"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f}", k ? ", " : "",
+                 t.fused, t.layered, t.ms);
+        rep += buf;
+        if (t.ms < best.ms) best = t;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) {
+        bl_ibm_set_fused_staging(g, keep_fused);
+        bl_ibm_set_layered(g, keep_layered);
+        return rc;
+    }
+    if ((rc = bl_ibm_set_fused_staging(g, best.fused)) || (rc = bl_ibm_set_layered(g, best.layered))) return rc;
+    char buf[160];
+    snprintf(buf, sizeof(buf), "], \"steps_run\": %d, \"best\": {\"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f}}",
+             (int)(2 * trials.size()) * n_steps, best.fused, best.layered, best.ms);
+    rep += buf;
+    int nv = 0;
+    bl_ibm_count(g, nullptr, &nv);
+    const std::string dev = device_arch(L->device);
+    CoupledEntry* e = nullptr;
+    for (CoupledEntry& c : g_coupled)
+        if (c.device == dev && c.nx == L->nx && c.ny == L->ny && c.nz == L->nz && c.storage == L->storage &&
+            c.vertices == nv)
+            e = &c;
+    if (!e) {
+        g_coupled.emplace_back();
+        e = &g_coupled.back();
+        e->device = dev;
+        e->nx = L->nx; e->ny = L->ny; e->nz = L->nz; e->storage = L->storage;
+        e->vertices = nv;
+    }
+    e->fused = best.fused;
+    e->layered = best.layered;
+    e->ms = best.ms;
+    put_report(rep, report, cap);
+    return BL_OK;
+}
+
+int bl_tune_apply_coupled(bl_ibm g)
+{
+    if (!g || !g->lat) return fail(BL_ERR_ARG, "bl_tune_apply_coupled: no lattice");
+    const bl_lattice L = g->lat;
+    int nv = 0;
+    bl_ibm_count(g, nullptr, &nv);
+    const std::string dev = device_arch(L->device);
+    for (const CoupledEntry& c : g_coupled)
+        if (c.device == dev && c.nx == L->nx && c.ny == L->ny && c.nz == L->nz && c.storage == L->storage &&
+            c.vertices == nv) {
+            int rc = bl_ibm_set_fused_staging(g, c.fused);
+            return rc ? rc : bl_ibm_set_layered(g, c.layered);
+        }
+    return BL_OK;
+}
+
+int bl_tune_cache_save(const char* path)
+{
+    if (!path) return fail(BL_ERR_ARG, "bl_tune_cache_save: null path");
+    std::ofstream out(path);
+    if (!out) return fail(BL_ERR_ARG, std::string("bl_tune_cache_save: cannot write ") + path);
+    for (const CollideEntry& e : g_collide) {
+        char buf[512];
+        snprintf(buf, sizeof(buf),
+                 "{\"kind\": \"collide\", \"device\": \"%s\", \"nx\": %d, \"ny\": %d, \"nz\": %d, \"storage\": %d, "
+                 "\"even\": [%d, %d, %d, %.6f], \"odd\": [%d, %d, %d, %.6f]}",
+                 e.device.c_str(), e.nx, e.ny, e.nz, e.storage, e.par[0].block, e.par[0].xcd, e.par[0].nt, e.par[0].ms,
+                 e.par[1].block, e.par[1].xcd, e.par[1].nt, e.par[1].ms);
+        out << buf << "\n";
+    }
+    for (const CoupledEntry& e : g_coupled) {
+        char buf[512];
+        snprintf(buf, sizeof(buf),
+                 "{\"kind\": \"coupled\", \"device\": \"%s\", \"nx\": %d, \"ny\": %d, \"nz\": %d, \"storage\": %d, "
+                 "\"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f}",
+                 e.device.c_str(), e.nx, e.ny, e.nz, e.storage, e.vertices, e.fused, e.layered, e.ms);
+        out << buf << "\n";
+    }
+    return out ? BL_OK : fail(BL_ERR_ARG, std::string("bl_tune_cache_save: write failed: ") + path);
+}
+
+int bl_tune_cache_load(const char* path)
+{
+    if (!path) return fail(BL_ERR_ARG, "bl_tune_cache_load: null path");
+    std::ifstream in(path);
+    if (!in) return fail(BL_ERR_ARG, std::string("bl_tune_cache_load: cannot read ") + path);
+    std::string line;
+    int bad = 0;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        char dev[128] = {0};
+        CollideEntry c;
+        CoupledEntry d;
+        if (sscanf(line.c_str(),
+                   "{\"kind\": \"collide\", \"device\": \"%127[^\"]\", \"nx\": %d, \"ny\": %d, \"nz\": %d, "
+                   "\"storage\": %d, \"even\": [%d, %d, %d, %lf], \"odd\": [%d, %d, %d, %lf]}",
+                   dev, &c.nx, &c.ny, &c.nz, &c.storage, &c.par[0].block, &c.par[0].xcd, &c.par[0].nt, &c.par[0].ms,
+                   &c.par[1].block, &c.par[1].xcd, &c.par[1].nt, &c.par[1].ms) == 13) {
+            bool ok = true;
+            for (int p = 0; p < 2; ++p)
+                ok = ok && c.par[p].block >= 64 && c.par[p].block <= 256 && c.par[p].block % 64 == 0 &&
+                     c.par[p].xcd >= 0 && c.par[p].xcd <= (1 << 20) &&
+                     (c.par[p].xcd <= 1 || (c.par[p].xcd & (c.par[p].xcd - 1)) == 0);
+            if (!ok) {
+                ++bad;
+                continue;
+            }
+            c.par[1].nt = 0;
+            c.device = dev;
+            bool found = false;
+            for (CollideEntry& e : g_collide)
+                if (e.device == c.device && e.nx == c.nx && e.ny == c.ny && e.nz == c.nz && e.storage == c.storage) {
+                    e = c;
+                    found = true;
+                }
+            if (!found) g_collide.push_back(c);
+        } else if (sscanf(line.c_str(),
+                          "{\"kind\": \"coupled\", \"device\": \"%127[^\"]\", \"nx\": %d, \"ny\": %d, \"nz\": %d, "
+                          "\"storage\": %d, \"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, "
+                          "\"ms_per_step\": %lf}",
+                          dev, &d.nx, &d.ny, &d.nz, &d.storage, &d.vertices, &d.fused, &d.layered, &d.ms) == 9) {
+            if (d.fused < -1 || d.fused > 1 || d.layered < -1 || d.layered > 1) {
+                ++bad;
+                continue;
+            }
+            d.device = dev;
+            bool found = false;
+            for (CoupledEntry& e : g_coupled)
+                if (e.device == d.device && e.nx == d.nx && e.ny == d.ny && e.nz == d.nz && e.storage == d.storage &&
+                    e.vertices == d.vertices) {
+                    e = d;
+                    found = true;
+                }
+            if (!found) g_coupled.push_back(d);
+        } else {
+            ++bad;
+        }
+    }
+    if (bad) return fail(BL_ERR_ARG, std::string("bl_tune_cache_load: ") + std::to_string(bad) + " unreadable line(s) in " + path);
+    return BL_OK;
+}
+
+}  // extern "C"

@@ -46,8 +46,8 @@ class Ensemble:
 
 
 def tune_collide(lattice):
-    """Times every launch shape of the lattice's collide kernel, keeps the
-    fastest (the lattice state is left as it was); returns the report."""
-    buf = ctypes.create_string_buffer(4096)
-    call("bl_tune_collide", lattice.handle, buf, len(buf))
-    return json.loads(buf.value.decode())
+    """Times every launch shape of the lattice's collide kernel per step
+    parity, keeps the fastest (the lattice state is left as it was);
+    returns the report (biblbm_amd.tuning.tune_collide)."""
+    from .tuning import tune_collide as _t
+    return _t(lattice)

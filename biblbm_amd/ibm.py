@@ -102,6 +102,19 @@ class ImmersedBoundary:
     def coupled_steps(self, n, use_graph=True):
         call("bl_coupled_steps", self.handle, int(n), int(bool(use_graph)))
 
+    def prepare(self, n):
+        """Capture the graphs coupled_steps(n) replays from the current
+        state without running a step (bl_coupled_prepare)."""
+        call("bl_coupled_prepare", self.handle, int(n))
+
+    def tune(self, n_steps=8):
+        """bl_tune_coupled: time each launch structure over real steps,
+        keep the fastest (results do not depend on it)."""
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        call("bl_tune_coupled", self.handle, int(n_steps), buf, len(buf))
+        return json.loads(buf.value.decode())
+
     # --- data ----------------------------------------------------------------
     def _nv(self, pid):
         return sum(n for _, n in self.particles) if pid < 0 else self.particles[pid][1]

@@ -60,6 +60,15 @@ class Lattice:
     def set_launch(self, block=256, xcd_remap=True):
         call("bl_lattice_set_launch", self.handle, int(block), int(bool(xcd_remap)))
 
+    def set_launch_parity(self, parity, block=256, xcd_remap=True, nontemporal=False):
+        call("bl_lattice_set_launch_parity", self.handle, int(parity), int(block), int(bool(xcd_remap)),
+             int(bool(nontemporal)))
+
+    def launch(self, parity):
+        out = np.zeros(3, np.int32)
+        call("bl_lattice_get_launch", self.handle, int(parity), ptr(out))
+        return dict(block=int(out[0]), xcd_remap=int(out[1]), nontemporal=int(out[2]))
+
     # --- state -------------------------------------------------------------
     def init_equilibrium(self, rho=None, u=None):
         r = None if rho is None else f64(rho).reshape(-1)

@@ -249,9 +249,9 @@ def suspension_interactions(storage="f32", **kw):
     return suspension(storage, interactions=True, **kw)
 
 
-def taylor_green(storage="f64", n=384, u0=0.03, tau=0.8, n_caps=16, radius=8.0, level=3, Ca=0.1, C=10.0,
-                 kernel="peskin", seed=11):
-    L = Lattice(n, n, n, storage=storage, omega=1 / tau)
+def taylor_green_fields(n, u0):
+    """Density and velocity of the Taylor-Green start on an n^3 periodic box
+    (k = 2 pi / n; rho from the Taylor-Green pressure, c_s^2 = 1/3)."""
     k = 2 * np.pi / n
     x = np.arange(n)
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij", sparse=True)
@@ -259,8 +259,16 @@ def taylor_green(storage="f64", n=384, u0=0.03, tau=0.8, n_caps=16, radius=8.0, 
     u[..., 0] = u0 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
     u[..., 1] = -u0 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
     rho = 1.0 + 3.0 * u0 ** 2 / 16.0 * (np.cos(2 * k * X) + np.cos(2 * k * Y)) * (np.cos(2 * k * Z) + 2)
-    L.init_equilibrium(np.broadcast_to(rho, (n, n, n)), u)
-    del u
+    return np.broadcast_to(rho, (n, n, n)), u
+
+
+def taylor_green(storage="f64", n=384, u0=0.03, tau=0.8, n_caps=16, radius=8.0, level=3, Ca=0.1, C=10.0,
+                 kernel="peskin", seed=11):
+    L = Lattice(n, n, n, storage=storage, omega=1 / tau)
+    k = 2 * np.pi / n
+    rho, u = taylor_green_fields(n, u0)
+    L.init_equilibrium(rho, u)
+    del u, rho
     nu = (tau - 0.5) / 3.0
     G = nu * (u0 * k) * radius / Ca
     v, t = meshes.icosphere(level, radius)

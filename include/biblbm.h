@@ -135,8 +135,19 @@ int bl_lattice_conserved(bl_lattice lat, double out[4]);
 int bl_lattice_conserved_static(bl_lattice lat, double *out);
 int bl_lattice_steps(bl_lattice lat, long long *steps);
 /* Launch shape of the collide-stream kernel: threads per block (64, 128,
- * 192 or 256) and XCD-aware block remapping on/off. */
+ * 192 or 256) and XCD-aware block remapping on/off, for both AA step
+ * parities, without nontemporal hints.  No counterpart in the reference
+ * (its block decomposition is fixed, src/multiBlock/defaultMultiBlockPolicy3D.h:119-129). */
 int bl_lattice_set_launch(bl_lattice lat, int block, int xcd_remap);
+/* The same for one parity (0: even steps, which address each cell's own
+ * slots; 1: odd steps, which address the neighbours'), with nontemporal
+ * population loads and stores (even steps only).  xcd_remap: 0 keeps the
+ * hardware's block order (round-robin over the 8 XCDs), 1 gives each XCD
+ * one contiguous range of cells, a power of two k >= 2 deals chunks of k
+ * consecutive blocks round-robin over the XCDs.  Results do not change. */
+int bl_lattice_set_launch_parity(bl_lattice lat, int parity, int block, int xcd_remap, int nontemporal);
+/* out = {block, xcd_remap, nontemporal} of one parity. */
+int bl_lattice_get_launch(bl_lattice lat, int parity, int out[3]);
 /* HIP stream the lattice works on (hipStream_t), for event timing. */
 int bl_lattice_stream(bl_lattice lat, void **stream);
 /* Runs n collide-stream launches (no boundary pass) and returns their total
@@ -187,6 +198,11 @@ int bl_ibm_step(bl_ibm g);
  * progs/capsule_in_shear/capsule_in_shear.cpp:187-227).  use_graph captures
  * one step in a hipGraph and replays it. */
 int bl_coupled_steps(bl_ibm g, int n_steps, int use_graph);
+/* Captures the graphs bl_coupled_steps(g, n_steps, 1) replays from the
+ * current state, without running a step, so that a timed run replays
+ * only (no counterpart in the reference).  BL_ERR_STATE while the next step
+ * must run eagerly (after a loaded force field or a body-force change). */
+int bl_coupled_prepare(bl_ibm g, int n_steps);
 int bl_ibm_count(bl_ibm g, int *particles, int *vertices);
 /* Vertex arrays of one particle (pid) or of all (pid = -1), nv*3 doubles. */
 int bl_particle_get(bl_ibm g, int pid, int what, double *out); /* what: 0 pos, 1 vel, 2 force */
@@ -388,11 +404,30 @@ int bl_ensemble_steps(bl_ensemble e, int n_steps, int use_graph);
 int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms);
 
 /* ------------------------------------------------------------- tuning */
-/* No counterpart in the reference.  Times the collide-and-stream kernel of
- * this lattice for every launch shape bl_lattice_set_launch accepts, keeps
- * the fastest, and restores the populations and step count the trials
- * advanced.  report (cap bytes, may be NULL) receives a JSON summary. */
+/* No counterpart in the reference (its block decomposition is fixed,
+ * src/multiBlock/defaultMultiBlockPolicy3D.h:119-129).  Times the
+ * collide-and-stream kernel of this lattice for every launch shape per AA
+ * step parity (block 64..256, XCD-aware order on/off, nontemporal hints on
+ * the even step on/off), keeps the fastest for each parity, restores the
+ * populations and step count the trials advanced, and records the result in
+ * the process's tuning table (a lattice of the same size, storage and
+ * device created later starts with it).  report (cap bytes, may be NULL)
+ * receives a JSON summary. */
 int bl_tune_collide(bl_lattice lat, char *report, int cap);
+/* Times n_steps (even) coupled steps of each launch structure
+ * (bl_ibm_set_fused_staging 0/1 without a phase field, bl_ibm_set_layered
+ * 0/1), keeps the fastest and records it in the tuning table (key: device,
+ * lattice size, storage, vertex count).  Every layout gives the same
+ * results; the trials are real steps: the simulation advances by
+ * 2 n_steps per layout tried (the report says how many). */
+int bl_tune_coupled(bl_ibm g, int n_steps, char *report, int cap);
+/* Applies the tuning table's launch structure for this boundary, if any. */
+int bl_tune_apply_coupled(bl_ibm g);
+/* The tuning table to / from a file, one JSON object per line; loading
+ * merges (an entry of the same key replaces the one held) and refuses
+ * unreadable lines. */
+int bl_tune_cache_save(const char *path);
+int bl_tune_cache_load(const char *path);
 
 #ifdef __cplusplus
 }

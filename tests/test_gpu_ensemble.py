@@ -1,5 +1,4 @@
-"""Ensembles of shear boxes (bl_ensemble_*) and the collide launch-shape
-tuner (bl_tune_collide), through the C-ABI."""
+"""Ensembles of shear boxes (bl_ensemble_*), through the C-ABI."""
 import numpy as np
 import pytest
 
@@ -108,24 +107,3 @@ def test_ensemble_rejects_mismatched_pairs(device):
     with pytest.raises(capi.BlError):
         Ensemble([(ws[0].lattice, ws[0].ibm), (ws[0].lattice, None)])
     _close(ws)
-
-
-def test_tuner_keeps_state_and_results(device):
-    """The tuner's trials leave populations and step count as they were, and
-    the tuned launch shape gives bitwise the same steps."""
-    from biblbm_amd.ensemble import tune_collide
-    A, B = _boxes(sides=(32,)), _boxes(sides=(32,))
-    a, b = A[0], B[0]
-    a.ibm.coupled_steps(3, use_graph=False)
-    b.ibm.coupled_steps(3, use_graph=False)
-    before = a.lattice.get_populations()
-    rep = tune_collide(a.lattice)
-    assert len(rep["trials"]) == 8
-    assert rep["best"]["block"] in (64, 128, 192, 256)
-    assert a.lattice.steps == 3
-    assert np.array_equal(a.lattice.get_populations(), before)
-    a.ibm.coupled_steps(6, use_graph=True)
-    b.ibm.coupled_steps(6, use_graph=True)
-    assert np.array_equal(a.lattice.get_populations(), b.lattice.get_populations())
-    assert np.array_equal(a.ibm.positions(), b.ibm.positions())
-    _close(A + B)

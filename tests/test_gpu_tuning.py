@@ -1,0 +1,123 @@
+"""The autotuner (bl_tune_*), through the C-ABI: the collide tuner's trials
+leave the lattice as it was and the tuned per-parity shapes step bitwise
+like the default; the coupled-step tuner's layouts give bitwise the same
+trajectory; the tuning table survives a save / load and a lattice of the
+tuned size starts with the tuned shapes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(s=32, seed=0):
+    from biblbm_amd import workloads
+    return workloads.shear_box("f64", s, 0.9, 0.3, G=0.004, seed=seed, radius=0.2 * s)
+
+
+def test_collide_tuner_keeps_state_and_results(device):
+    from biblbm_amd import tuning
+    a, b = _box(), _box()
+    a.ibm.coupled_steps(3, use_graph=False)
+    b.ibm.coupled_steps(3, use_graph=False)
+    before = a.lattice.get_populations()
+    rep = tuning.tune_collide(a.lattice)
+    assert len(rep["even"]) == 40 and len(rep["odd"]) == 20
+    for par in ("even", "odd"):
+        assert rep["best"][par]["block"] in (64, 128, 192, 256)
+        assert rep["best"][par]["xcd_remap"] in (0, 1, 4, 16, 64)
+    assert rep["best"]["odd"]["nontemporal"] == 0
+    assert a.lattice.steps == 3
+    assert np.array_equal(a.lattice.get_populations(), before)
+    got = [a.lattice.launch(p) for p in (0, 1)]
+    assert got[0] == {k: rep["best"]["even"][k] for k in got[0]}
+    assert got[1] == {k: rep["best"]["odd"][k] for k in got[1]}
+    a.ibm.coupled_steps(6, use_graph=True)
+    b.ibm.coupled_steps(6, use_graph=True)
+    assert np.array_equal(a.lattice.get_populations(), b.lattice.get_populations())
+    assert np.array_equal(a.ibm.positions(), b.ibm.positions())
+    for w in (a, b):
+        w.ibm.close()
+        w.lattice.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_nontemporal_even_step_is_bitwise(device, storage):
+    from biblbm_amd.lattice import Lattice
+    rng = np.random.default_rng(4)
+    shape = (24, 20, 28)
+    f0 = None
+    Ls = []
+    for nt in (False, True):
+        L = Lattice(*shape, periodic=(1, 0, 1), storage=storage, omega=1 / 0.8)
+        mask = np.zeros(shape, np.uint8)
+        mask[:, 0, :] = mask[:, -1, :] = 1
+        L.set_mask(mask)
+        L.set_body_force((1e-5, 0, 0))
+        if f0 is None:
+            L.init_equilibrium(rng.uniform(0.99, 1.01, shape), rng.uniform(-0.02, 0.02, shape + (3,)))
+            f0 = L.get_populations()
+        else:
+            L.put_populations(f0)
+        L.set_launch_parity(0, 128, False, nt)
+        L.collide_and_stream(5)
+        Ls.append(L)
+    assert np.array_equal(Ls[0].get_populations(), Ls[1].get_populations())
+
+
+def test_coupled_tuner_layouts_give_the_same_trajectory(device):
+    from biblbm_amd import tuning
+    a, b = _box(seed=3), _box(seed=3)
+    rep = tuning.tune_coupled(a.ibm, 4)
+    n = rep["steps_run"]
+    assert n == 2 * 4 * len(rep["trials"])
+    b.ibm.coupled_steps(n, use_graph=False)
+    assert np.array_equal(a.ibm.positions(), b.ibm.positions())
+    assert np.array_equal(a.lattice.get_populations(), b.lattice.get_populations())
+    for w in (a, b):
+        w.ibm.close()
+        w.lattice.close()
+
+
+def test_tuning_table_save_load(device, tmp_path):
+    from biblbm_amd import tuning
+    from biblbm_amd.lattice import Lattice
+    L = Lattice(40, 36, 44, periodic=(1, 1, 1), storage="f32", omega=1.0)
+    rep = tuning.tune_collide(L)
+    path = tmp_path / "tune.jsonl"
+    tuning.save(str(path))
+    lines = path.read_text().strip().splitlines()
+    assert any('"nx": 40, "ny": 36, "nz": 44, "storage": 1' in ln for ln in lines)
+    assert tuning.load(str(path))
+    L2 = Lattice(40, 36, 44, periodic=(1, 1, 1), storage="f32", omega=1.0)
+    assert L2.launch(0) == {k: rep["best"]["even"][k] for k in ("block", "xcd_remap", "nontemporal")}
+    assert L2.launch(1)["block"] == rep["best"]["odd"]["block"]
+    bad = tmp_path / "bad.jsonl"
+    bad.write_text('{"kind": "collide", "device": "x", "nx": 1}\n')
+    from biblbm_amd import capi
+    with pytest.raises(capi.BlError):
+        tuning.load(str(bad))
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 4, 16, 64])
+def test_block_orders_are_bitwise(device, order):
+    """Every block order (hardware order, one range per XCD, chunks of k
+    blocks over the XCDs) gives the same populations, on a lattice whose
+    block count is not a multiple of the chunk round (tail blocks)."""
+    from biblbm_amd.lattice import Lattice
+    rng = np.random.default_rng(7)
+    shape = (30, 26, 22)  # 17160 cells: 67.03 blocks of 256
+    ref = None
+    f0 = O_start = None
+    L0 = Lattice(*shape, periodic=(1, 1, 0), storage="f64", omega=1 / 0.7)
+    L0.init_equilibrium(rng.uniform(0.99, 1.01, shape), rng.uniform(-0.02, 0.02, shape + (3,)))
+    f0 = L0.get_populations()
+    L0.set_body_force((1e-5, 2e-6, 0))
+    L0.collide_and_stream(4)
+    ref = L0.get_populations()
+    L = Lattice(*shape, periodic=(1, 1, 0), storage="f64", omega=1 / 0.7)
+    L.put_populations(f0)
+    L.set_body_force((1e-5, 2e-6, 0))
+    for p in (0, 1):
+        L.set_launch_parity(p, 64 if p else 128, order, False)
+    L.collide_and_stream(4)
+    assert np.array_equal(L.get_populations(), ref)
