@@ -69,6 +69,10 @@ SIGNATURES = {
     "bl_ibm_count": [_P, _PI, _PI],
     "bl_particle_get": [_P, _I, _I, _P],
     "bl_particle_set_positions": [_P, _I, _P],
+    "bl_particle_set_state": [_P, _I, _I, _P],
+    "bl_particle_set_params": [_P, _I, _P, _I],
+    "bl_rbc_relax_nodes": [_P, _I, _D],
+    "bl_particle_membrane_forces": [_P],
     "bl_particle_observables": [_P, _I, _P],
     "bl_ibm_get_phase": [_P, _P],
     "bl_ibm_stage_times": [_P, _P],

@@ -2980,6 +2980,87 @@ int bl_rbc_shrink_volume(bl_ibm g, int pid, double vol_final, int n_it, double d
     return BL_OK;
 }
 
+int bl_particle_set_state(bl_ibm g, int pid, int what, const double* data)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!data || what < 0 || what > 2 || pid < -1 || pid >= g->np)
+        return fail(BL_ERR_ARG, "bl_particle_set_state: bad argument");
+    if (what == 0) return bl_particle_set_positions(g, pid, data);
+    long v0 = 0, v1 = 0;
+    for (int p = 0; p < g->np; ++p) {
+        const long n = g->shapes[g->parts[p].shape].nv;
+        if (p < pid) v0 += n;
+        if (pid < 0 || p <= pid) v1 += n;
+    }
+    if (pid < 0) v0 = 0;
+    double* dst = what == 1 ? g->vel : g->force;
+    BL_HIP(hipMemcpyAsync(dst + 3 * v0, data, sizeof(double) * 3 * (v1 - v0), hipMemcpyHostToDevice, g->lat->stream));
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    return BL_OK;
+}
+
+int bl_particle_set_params(bl_ibm g, int pid, const double* params, int nparams)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (!params || pid < 0 || pid >= g->np || nparams < 1 || nparams > 8)
+        return fail(BL_ERR_ARG, "bl_particle_set_params: bad argument");
+    double* h = g->parts[pid].params;
+    for (int k = 0; k < nparams; ++k) h[k] = params[k];
+    BL_HIP(hipMemcpyAsync(g->p_params + 8 * (size_t)pid, h, sizeof(double) * 8, hipMemcpyHostToDevice, g->lat->stream));
+    BL_HIP(hipStreamSynchronize(g->lat->stream));
+    ibm_invalidate_graphs(g);
+    return BL_OK;
+}
+
+int bl_rbc_relax_nodes(bl_ibm g, int pid, double damping)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    if (pid < 0 || pid >= g->np || !(damping >= 0)) return fail(BL_ERR_ARG, "bl_rbc_relax_nodes: bad argument");
+    const int model = g->parts[pid].model;
+    if (model != BL_MODEL_RBC && model != 3) return fail(BL_ERR_ARG, "bl_rbc_relax_nodes: not a red-cell model");
+    hipStream_t s = g->lat->stream;
+    const int np = g->np;
+    std::vector<int> act(np, 0);
+    act[pid] = 1;
+    std::vector<double> damp(np, damping);
+    DevBuf<int> d_active, d_step;
+    DevBuf<double> d_damp, d_mass, d_v;
+    if ((rc = d_active.alloc(np)) || (rc = d_step.alloc(1)) || (rc = d_damp.alloc(np)) || (rc = d_mass.alloc(g->NV)) ||
+        (rc = d_v.alloc(np)))
+        return rc;
+    BL_HIP(hipMemcpyAsync(d_active.p, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(d_damp.p, damp.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemsetAsync(d_step.p, 0, sizeof(int), s));
+    const ForceView f = force_view(g);
+    // the vertex forces in place are those of the current positions
+    // (compute_forces before the first step); n_it = -1: vol_desired stays
+    // as the parameters hold it
+    hipLaunchKernelGGL(k_calib_mass, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, (const int*)d_active.p, d_mass.p,
+                       d_step.p);
+    hipLaunchKernelGGL(k_calib_relax, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV, (const int*)g->v_part,
+                       (const long*)g->p_voff, (const int*)d_active.p, (const double*)d_damp.p, (const double*)d_v.p,
+                       (const double*)d_v.p, -1, (const int*)d_step.p, (const double*)d_mass.p,
+                       (const double*)g->force, g->vel, g->pos, g->p_params);
+    BL_CHECK_LAUNCH();
+    if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
+    BL_HIP(hipStreamSynchronize(s));
+    g->binned = false;
+    return BL_OK;
+}
+
+int bl_particle_membrane_forces(bl_ibm g)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    hipStream_t s = g->lat->stream;
+    if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
+    BL_HIP(hipStreamSynchronize(s));
+    return BL_OK;
+}
+
 int bl_particle_observables(bl_ibm g, int pid, double out[14])
 {
     int rc = need_init(g);

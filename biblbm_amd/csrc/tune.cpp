@@ -92,7 +92,7 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
     // step only); every even candidate is timed once, every odd one over
     // the same number of launches
     static const int blocks[] = {64, 128, 192, 256};
-    static const int orders[] = {0, 1, 4, 16, 64};  // block orders (see bl_lattice_set_launch_parity)
+    static const int orders[] = {0, 1, 4, 16, 64, 256, 1024};  // block orders (see bl_lattice_set_launch_parity)
     std::vector<Launch> cand[2];
     for (int xcd : orders)
         for (int nt = 0; nt < 2; ++nt)

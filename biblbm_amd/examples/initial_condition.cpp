@@ -22,7 +22,7 @@
 #include <iostream>
 #include <string>
 
-#include "biblbm_plb.h"
+#include "biblbm_calibrate.h"
 
 using namespace plb;
 using namespace plb::fsi;

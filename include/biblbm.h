@@ -207,6 +207,25 @@ int bl_ibm_count(bl_ibm g, int *particles, int *vertices);
 /* Vertex arrays of one particle (pid) or of all (pid = -1), nv*3 doubles. */
 int bl_particle_get(bl_ibm g, int pid, int what, double *out); /* what: 0 pos, 1 vel, 2 force */
 int bl_particle_set_positions(bl_ibm g, int pid, const double *pos);
+/* Vertex state of one particle (pid) or of all (pid = -1): what 0
+ * positions (as bl_particle_set_positions), 1 velocities, 2 forces; nv*3
+ * doubles.  The vertex arrays of ParticleBase3D (src/fsi/ParticleBase.h:30-36). */
+int bl_particle_set_state(bl_ibm g, int pid, int what, const double *data);
+/* The first nparams (<= 8) model parameters of particle pid, in the order
+ * of bl_particle_add (e.g. vol_desired of an RBC, RBCParticle::params(),
+ * src/fsi/RBCParticle.h:70-71). */
+int bl_particle_set_params(bl_ibm g, int pid, const double *params, int nparams);
+/* RBCParticle::relax_nodes(damping) (src/fsi/RBCParticle.hh:318-342) for
+ * particle pid (models 2 and 3): node mass = area/3 of the incident
+ * triangles, vel += F / M - damping vel, pos += vel, geometry update, new
+ * membrane forces; the forces in place on entry are those of the current
+ * positions (bl_particle_membrane_forces). */
+int bl_rbc_relax_nodes(bl_ibm g, int pid, double damping);
+/* compute_forces of every particle's membrane alone (no decorators, no
+ * particle-particle forces) after a geometry update: the particles'
+ * compute_forces() (src/fsi/RBCParticle.hh:140-247,
+ * src/fsi/DeformableCapsule.hh:60-150). */
+int bl_particle_membrane_forces(bl_ibm g);
 /* area, volume, centre of mass (3), inertia tensor (9) of the enclosed
  * volume (src/fsi/DeformableParticle3D.hh:54-82,173-272). */
 int bl_particle_observables(bl_ibm g, int pid, double out[14]);

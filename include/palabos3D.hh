@@ -1,0 +1,9 @@
+/*
+ * palabos3D.hh -- a header name the reference's progs (and the progs/common helpers
+ * they include) use; the declarations are those of biblbm_plb.h, the
+ * device-backed C++ layer of biblbm_amd.
+ */
+#ifndef BIBLBM_SHIM_PALABOS3D_HH
+#define BIBLBM_SHIM_PALABOS3D_HH
+#include "biblbm_plb.h"
+#endif

@@ -1,0 +1,9 @@
+/*
+ * parallelism/mpiManager.h -- a header name the reference's progs (and the progs/common helpers
+ * they include) use; the declarations are those of biblbm_plb.h, the
+ * device-backed C++ layer of biblbm_amd.
+ */
+#ifndef BIBLBM_SHIM_PARALLELISM_MPIMANAGER_H
+#define BIBLBM_SHIM_PARALLELISM_MPIMANAGER_H
+#include "../biblbm_plb.h"
+#endif

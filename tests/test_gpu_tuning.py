@@ -21,10 +21,10 @@ def test_collide_tuner_keeps_state_and_results(device):
     b.ibm.coupled_steps(3, use_graph=False)
     before = a.lattice.get_populations()
     rep = tuning.tune_collide(a.lattice)
-    assert len(rep["even"]) == 40 and len(rep["odd"]) == 20
+    assert len(rep["even"]) == 56 and len(rep["odd"]) == 28
     for par in ("even", "odd"):
         assert rep["best"][par]["block"] in (64, 128, 192, 256)
-        assert rep["best"][par]["xcd_remap"] in (0, 1, 4, 16, 64)
+        assert rep["best"][par]["xcd_remap"] in (0, 1, 4, 16, 64, 256, 1024)
     assert rep["best"]["odd"]["nontemporal"] == 0
     assert a.lattice.steps == 3
     assert np.array_equal(a.lattice.get_populations(), before)
@@ -98,7 +98,7 @@ def test_tuning_table_save_load(device, tmp_path):
         tuning.load(str(bad))
 
 
-@pytest.mark.parametrize("order", [0, 1, 2, 4, 16, 64])
+@pytest.mark.parametrize("order", [0, 1, 2, 4, 16, 64, 1024])
 def test_block_orders_are_bitwise(device, order):
     """Every block order (hardware order, one range per XCD, chunks of k
     blocks over the XCDs) gives the same populations, on a lattice whose
