@@ -81,5 +81,38 @@ def build_examples():
     return LIB
 
 
+# The reference's own progs, compiled unchanged against the Palabos-named
+# headers in include/ (palabos3D.h, fsi/headers.h, ...) and linked against
+# the library, when the reference checkout is present.  Outputs go to
+# examples/ref_progs/ (not versioned).
+REF_PROGS = ["capsule_in_shear/capsule_in_shear.cpp", "rbc_in_pipe/rbc_in_pipe.cpp", "rbc_in_shear/rbc_in_shear.cpp",
+             "rbcs_in_shear/rbcs_in_shear.cpp"]
+REF_PROG_FLAGS = ["-std=gnu++14", "-fpermissive", "-w", "-O2"]
+
+
+def ref_prog_command(src, out):
+    rocm_lib = "/opt/rocm/lib"
+    return (["g++"] + REF_PROG_FLAGS + ["-I" + INCLUDE, src, "-L" + HERE, "-lbiblbm", "-L" + rocm_lib,
+                                        "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + rocm_lib, "-o", out])
+
+
+def build_ref_progs(ref="/root/reference", jobs=4):
+    progs = os.path.join(ref, "progs")
+    if not os.path.isdir(progs):
+        return []
+    outdir = os.path.join(HERE, "examples", "ref_progs")
+    os.makedirs(outdir, exist_ok=True)
+
+    def one(rel):
+        out = os.path.join(outdir, os.path.basename(os.path.dirname(rel)))
+        r = subprocess.run(ref_prog_command(os.path.join(progs, rel), out), capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("reference prog %s does not build against include/: %s" % (rel, r.stderr[-4000:]))
+        return out
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(one, REF_PROGS))
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -73,6 +73,7 @@ SIGNATURES = {
     "bl_particle_set_params": [_P, _I, _P, _I],
     "bl_rbc_relax_nodes": [_P, _I, _D],
     "bl_particle_membrane_forces": [_P],
+    "bl_ibm_set_vertex_forces": [_P, _P],
     "bl_particle_observables": [_P, _I, _P],
     "bl_ibm_get_phase": [_P, _P],
     "bl_ibm_stage_times": [_P, _P],

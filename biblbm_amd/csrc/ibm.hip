@@ -706,6 +706,7 @@ struct GatherArgs {
     double* force;
     DecoSet deco;
     const double* fpp;
+    const double* ext;  // host-given vertex forces, added after the decorators (or null)
 };
 
 __device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
@@ -726,6 +727,11 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
         c0 = w[0];
         c1 = w[1];
         c2 = w[2];
+    }
+    if (A.ext && lane == 0 && v < NV) {
+        c0 += A.ext[3 * v];
+        c1 += A.ext[3 * v + 1];
+        c2 += A.ext[3 * v + 2];
     }
     if (v < NV) {
         // the vertex's incidence lists in one load (built at init), so the
@@ -2059,6 +2065,7 @@ GatherArgs gather_args(bl_ibm g, bool walls)
     if (walls) a.deco = deco_set(g);
     else a.deco.n = 0;
     a.fpp = walls && !g->pp.empty() ? g->pp_force : nullptr;
+    a.ext = walls && g->has_ext ? g->ext_force : nullptr;
     return a;
 }
 
@@ -2417,7 +2424,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
                     g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->ch_tab, g->t_vert, g->v_inc, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
-                    g->l_part};
+                    g->l_part, g->ext_force};
     for (void* p : ptrs) hipFree(p);
     interactions_free(g);
     for (auto& e : g->ev)
@@ -3048,6 +3055,24 @@ int bl_rbc_relax_nodes(bl_ibm g, int pid, double damping)
     if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
     BL_HIP(hipStreamSynchronize(s));
     g->binned = false;
+    return BL_OK;
+}
+
+int bl_ibm_set_vertex_forces(bl_ibm g, const double* force)
+{
+    int rc = need_init(g);
+    if (rc) return rc;
+    hipStream_t s = g->lat->stream;
+    if (!force) {
+        if (g->has_ext) ibm_invalidate_graphs(g);
+        g->has_ext = false;
+        return BL_OK;
+    }
+    if (!g->ext_force) BL_HIP(hipMalloc(&g->ext_force, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
+    BL_HIP(hipMemcpyAsync(g->ext_force, force, sizeof(double) * 3 * g->NV, hipMemcpyHostToDevice, s));
+    BL_HIP(hipStreamSynchronize(s));
+    if (!g->has_ext) ibm_invalidate_graphs(g);
+    g->has_ext = true;
     return BL_OK;
 }
 

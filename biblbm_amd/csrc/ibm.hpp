@@ -105,6 +105,8 @@ struct bl_ibm_s {
     int* pp_cell = nullptr;   // per vertex: collision cell, -1 if the vertex does not take part
     bl::PPRec* pp_rec = nullptr;  // per vertex: position, owner, successor in its cell (ascending id)
     double* pp_force = nullptr;   // per vertex: the step's particle-particle forces (3)
+    double* ext_force = nullptr;  // per vertex: forces given by the host (bl_ibm_set_vertex_forces), 3
+    bool has_ext = false;
     long pp_ncell = 0;
 
     // ---- stencil / tiles ----

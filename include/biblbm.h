@@ -226,6 +226,13 @@ int bl_rbc_relax_nodes(bl_ibm g, int pid, double damping);
  * compute_forces() (src/fsi/RBCParticle.hh:140-247,
  * src/fsi/DeformableCapsule.hh:60-150). */
 int bl_particle_membrane_forces(bl_ibm g);
+/* Vertex forces (nv*3 over all particles, in particle order) that every
+ * later coupled step adds to each vertex's force after the wall
+ * decorators and before the membrane forces, until replaced; NULL removes
+ * them.  The force decorators of a prog that have no device form
+ * (ForceDecorator::apply_force, src/fsi/ForceDecorator.h:12-15) are
+ * evaluated on the host and arrive here. */
+int bl_ibm_set_vertex_forces(bl_ibm g, const double *force);
 /* area, volume, centre of mass (3), inertia tensor (9) of the enclosed
  * volume (src/fsi/DeformableParticle3D.hh:54-82,173-272). */
 int bl_particle_observables(bl_ibm g, int pid, double out[14]);

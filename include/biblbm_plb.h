@@ -71,6 +71,7 @@
 #include <cstdlib>
 #include <ctime>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <limits>
 #include <map>
@@ -568,6 +569,7 @@ public:
         if (pending_ibm_) {
             bl_ibm g = pending_ibm_;
             pending_ibm_ = nullptr;
+            if (pre_step_) pre_step_();
             blCheck(bl_coupled_steps(g, 1, 0), "collideAndStream (with the immersed boundary)");
         } else {
             blCheck(bl_collide_stream(handle(), 1), "collideAndStream");
@@ -656,12 +658,15 @@ public:
     void setPendingBoundaryStep(bl_ibm g) { pending_ibm_ = g; }
     // the immersed boundary of this lattice (its phase field)
     void setBoundaryHandle(bl_ibm g) { ibm_ = g; }
+    // runs before each scheduled immersed-boundary step (host decorators)
+    void setPreStepHook(std::function<void()> f) { pre_step_ = std::move(f); }
     bl_ibm boundaryHandle() const { return ibm_; }
     void flushPendingBoundaryStep()
     {
         if (pending_ibm_) {
             bl_ibm g = pending_ibm_;
             pending_ibm_ = nullptr;
+            if (pre_step_) pre_step_();
             blCheck(bl_ibm_step(g), "immersed boundary step");
         }
     }
@@ -688,6 +693,7 @@ private:
     bl_lattice h_ = nullptr;
     bl_ibm pending_ibm_ = nullptr;
     bl_ibm ibm_ = nullptr;
+    std::function<void()> pre_step_;
 };
 
 template <typename T, template <typename U> class Descriptor>
@@ -994,12 +1000,18 @@ private:
     Interaction potential_;
 };
 
+template <class T>
+class ParticleBase3D;
+
 // Force decorators (src/fsi/ForceDecorator.h:12-27): applied to every
 // particle before its membrane forces.  The device runs the wall
-// interaction; a decorator without a device form is refused when added.
+// interaction; a prog's own decorator (apply_force on the particle) runs on
+// the host before each coupled step, its vertex forces handed to the device
+// (bl_ibm_set_vertex_forces), after the device decorators.
 template <class T>
 struct ForceDecorator {
     virtual ~ForceDecorator() {}
+    virtual void apply_force(ParticleBase3D<T>*) {}
     virtual bool device_form(bl_boundary& b, bl_potential& u) const
     {
         (void)b;
@@ -1780,6 +1792,14 @@ public:
         blCheck(bl_particle_membrane_forces(d.handle()), "RBCParticle::compute_forces");
         d.download(this->vertices_);
     }
+    // the reference prints the membrane energy terms; the device computes
+    // forces, not energies, so this prints the constrained quantities
+    template <class Stream>
+    void print_energies(Stream& out) const
+    {
+        out << "area " << this->area() << " (rest " << this->shape()->get_area() << "), volume " << this->volume()
+            << " (desired " << p_.vol_desired << ")" << std::endl;
+    }
     // one damped relaxation step (src/fsi/RBCParticle.hh:318-342)
     void relax_nodes(T damping)
     {
@@ -1976,7 +1996,7 @@ public:
         }
         blCheck(bl_ibm_init(g_), "init");
         if (boundary_) set_boundary(boundary_);
-        for (const ForceDecorator<T>* d : decorators_) push_decorator(d);
+        for (ForceDecorator<T>* d : decorators_) push_decorator(d);
         for (const PPInteractionForce<T>* f : pp_forces_) push_pp_force(f);
     }
     bl_ibm handle() const { return g_; }
@@ -2080,6 +2100,13 @@ public:
         p = it->second;
         return true;
     }
+    bool get_particle(plint id, const ParticleBase3D<T>*& p)
+    {
+        ParticleBase3D<T>* q = nullptr;
+        if (!get_particle(id, q)) return false;
+        p = q;
+        return true;
+    }
     // current vertices of particle index i into a host copy
     bool get_particle(plint i, std::vector<Array<T, 3>>& pos)
     {
@@ -2156,14 +2183,40 @@ private:
     std::string checkpoint_;
     std::vector<plint> shape_nv_, nv_;  // vertices per shape id / per particle
     std::vector<const ParticleShape<T>*> shape_of_;
-    void push_decorator(const ForceDecorator<T>* d)
+    void push_decorator(ForceDecorator<T>* d)
     {
         bl_boundary b;
         bl_potential u;
-        if (!d->device_form(b, u))
-            throw std::runtime_error("add_force_decorator: only wall-interaction decorators run on the device");
+        if (!d->device_form(b, u)) {
+            host_decorators_.push_back(d);
+            lattice_.setPreStepHook([this]() { host_decorator_forces(); });
+            return;
+        }
         lattice_.flushPendingBoundaryStep();
         blCheck(bl_ibm_add_wall_interaction(g_, &b, &u), "add_force_decorator");
+    }
+    // the host decorators' vertex forces of the current state, in particle
+    // order, for the next step (each decorator adds to forces that start at
+    // zero, in the order the decorators were added)
+    void host_decorator_forces()
+    {
+        int nv = 0;
+        blCheck(bl_ibm_count(g_, nullptr, &nv), "force decorator");
+        std::vector<double> pos(3 * (size_t)nv), out;
+        blCheck(bl_particle_get(g_, -1, 0, pos.data()), "force decorator");
+        out.reserve(pos.size());
+        size_t k = 0;
+        for (auto& kv : objs_) {
+            ParticleBase3D<T>* p = kv.second;
+            for (auto it = p->begin(); it != p->end(); ++it, ++k) {
+                it->pos = Array<T, 3>(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]);
+                it->force.resetToZero();
+            }
+            p->update();
+            for (ForceDecorator<T>* d : host_decorators_) d->apply_force(p);
+            for (auto it = p->begin(); it != p->end(); ++it) out.insert(out.end(), {it->force[0], it->force[1], it->force[2]});
+        }
+        blCheck(bl_ibm_set_vertex_forces(g_, out.data()), "force decorator");
     }
     void push_pp_force(const PPInteractionForce<T>* f)
     {
@@ -2176,7 +2229,7 @@ private:
     ObjMapType objs_;
     std::map<plint, int> index_;  // particle id -> device index
     plint next_id_ = 0;
-    std::vector<const ForceDecorator<T>*> decorators_;
+    std::vector<ForceDecorator<T>*> decorators_, host_decorators_;
     std::vector<const PPInteractionForce<T>*> pp_forces_;
     const Boundary<T>* boundary_ = nullptr;
     bl_ibm g_ = nullptr;
