@@ -379,6 +379,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the launch shapes the lattice starts with (from the loaded tuning table, if it has "
+                         "this size) instead of tuning them here")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -390,8 +393,11 @@ def main():
     capi.call("bl_init", 0)
     tuning.load()
     W = workloads.taylor_green(args.storage)
-    tuned = tuning.tune_collide(W.lattice)["best"]
-    progress("headline workload built, collide launch shapes tuned")
+    if args.no_tune:
+        tuned = {"even": W.lattice.launch(0), "odd": W.lattice.launch(1), "source": "as created"}
+    else:
+        tuned = tuning.tune_collide(W.lattice)["best"]
+    progress("headline workload built, collide launch shapes %s" % ("kept" if args.no_tune else "tuned"))
     dt = time_coupled(W, args.steps, args.warmup)
     value = W.cells * args.steps / dt / 1e6
     W.ibm.step()

@@ -12,6 +12,10 @@ and summarise with
     python3 -m biblbm_amd.profile_kernels stats OUT
     python3 -m biblbm_amd.profile_kernels traffic OUT_F OUT_W 268435456 k_collide_stream
 
+Launch shapes come from the saved tuning table ($BIBLBM_TUNE_CACHE) when it
+has the lattice's size, so that a bench run followed by a profile with the
+same table profiles the shapes the bench timed.
+
 `collide W S N` builds workload W (rbc_in_pipe, suspension,
 suspension_interactions, taylor_green or capsule_in_shear) with storage S, runs two coupled steps so that the force
 tiles are those of a running simulation, then N collide-and-stream launches.
@@ -33,8 +37,11 @@ def _workload(name, storage):
 
 
 def run(argv):
-    from . import capi
+    from . import capi, tuning
     capi.call("bl_init", 0)
+    # lattices take the tuned launch shapes of their size from the saved
+    # tuning table, when there is one ($BIBLBM_TUNE_CACHE)
+    tuning.load()
     i = 0
     while i < len(argv):
         cmd = argv[i]

@@ -580,7 +580,11 @@ public:
     bl_lattice handle()
     {
         if (h_) return h_;
-        const int per[3] = {periodic_[0], periodic_[1], periodic_[2]};
+        int per[3] = {periodic_[0], periodic_[1], periodic_[2]};
+        // velocity boundaries on the two faces of an axis bound that axis
+        // (the progs' shear setup leaves y at Palabos's default,
+        // non-periodic, and marks its faces as velocity boundaries)
+        if (wall_axis_ >= 0) per[wall_axis_] = 0;
         blCheck(bl_lattice_create((int)nx_, (int)ny_, (int)nz_, per, storage_, &h_), "MultiBlockLattice3D");
         blCheck(bl_lattice_set_dynamics(h_, omega_, omega_in_), "GuoExternalForceVOFDynamics");
         pushMask();

@@ -199,3 +199,81 @@ def test_initial_condition_prog_matches_reference(device, tmp_path):
     assert np.array_equal(np.loadtxt(tmp_path / "ic_R.txt"), load_golden("init", "placement", "R.npy"))
     assert np.array_equal(np.loadtxt(tmp_path / "ic_P.txt"), load_golden("init", "placement", "P.npy"))
     assert (tmp_path / "checkpoint_fsi.plb").read_text().split() == ["40", "1", "checkpoint_fsi/0.plb"]
+
+
+def _read_vti_vectors(path):
+    """First appended Float32 array of a .vti written by VtkImageOutput3D, as
+    an (nx, ny, nz, ncomp) array."""
+    import re
+    raw = open(path, "rb").read()
+    head = raw[:raw.index(b"<AppendedData")].decode()
+    ext = [int(s) for s in re.search(r'WholeExtent="([^"]+)"', head).group(1).split()]
+    ncomp = int(re.search(r'NumberOfComponents="(\d+)"', head).group(1))
+    nx, ny, nz = ext[1] - ext[0] + 1, ext[3] - ext[2] + 1, ext[5] - ext[4] + 1
+    start = raw.index(b"_", raw.index(b"<AppendedData")) + 1
+    nbytes = int(np.frombuffer(raw[start:start + 8], dtype="<u8")[0])
+    assert nbytes == nx * ny * nz * ncomp * 4
+    a = np.frombuffer(raw[start + 8:start + 8 + nbytes], dtype="<f4").reshape(nz, ny, nx, ncomp)
+    return a.transpose(2, 1, 0, 3)
+
+
+REF_PROG_DIR = os.path.join(ROOT, "biblbm_amd", "examples", "ref_progs")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_PROG_DIR, "capsule_in_shear")),
+                    reason="the reference's capsule_in_shear prog was not built (needs the reference checkout)")
+def test_reference_capsule_in_shear_prog_runs(device, tmp_path):
+    """The reference's progs/capsule_in_shear/capsule_in_shear.cpp, compiled
+    unchanged against include/, run to its end (20 shear times, 401 velocity
+    outputs) on a 61^3 box with a radius-6 capsule: Couette walls at -uMax and
+    +uMax, a linear profile away from the capsule, the capsule volume held and
+    its Taylor deformation settled to a steady value."""
+    from biblbm_amd import meshes
+    N, Re, Ca, tau = 60, 1.0, 0.15, 0.8
+    v, t = meshes.icosphere(3, 6.0)
+    meshes.write_gmsh(str(tmp_path / "sphere_r6_TRImesh_1046n_2088el.msh"), v, t)
+    r = subprocess.run([os.path.join(REF_PROG_DIR, "capsule_in_shear"), "-N", str(N), "-Re", str(Re), "-Ca", str(Ca)],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    import re
+    # the prog's own figures: uMax = gamma N / 2 and Nt = ceil(20 / gamma)
+    u_max = float(re.search(r"uMax:([0-9.eE+-]+)", r.stdout).group(1))
+    Nt = int(re.search(r"Iteration 0 / (\d+)", r.stdout).group(1))
+    gamma = 2 * u_max / N
+    nu = (tau - 0.5) / 3.0
+    a = np.sqrt(Re * nu / gamma)
+    assert abs(a - 6.0) < 0.1
+    assert abs(Nt - 20 / gamma) <= 1.0 + 1e-3 * Nt
+    assert "Iteration %d / %d" % (Nt // 50 * 50, Nt) in r.stdout
+    out = tmp_path / "spheroid_N60_Ca0_15_tau0_8"
+    d = np.loadtxt(out / "deformation_p0.txt")
+    assert d.shape == (Nt // 10 + 1, 11) and np.all(np.isfinite(d))
+    np.testing.assert_allclose(d[:, 0], gamma * 10 * np.arange(d.shape[0]), rtol=1e-5)
+    vol = d[:, 1]
+    assert np.max(np.abs(vol / vol[0] - 1)) < 0.02
+    # Taylor deformation (L - B) / (L + B) from the inertia tensor of an
+    # ellipsoid: the squared semi-axes go as trace/2 - eigenvalue
+    D = []
+    for row in d[:, 2:]:
+        lam = np.linalg.eigvalsh(row.reshape(3, 3))
+        ax = np.sqrt(lam.sum() / 2 - lam)
+        D.append((ax.max() - ax.min()) / (ax.max() + ax.min()))
+    D = np.array(D)
+    assert D[0] < 0.02
+    assert 0.1 < D[-1] < 0.7
+    tail = D[int(0.75 * len(D)):]
+    assert tail.max() - tail.min() < 0.03
+    # the last velocity output: the walls move at -uMax and +uMax, and the
+    # profile is linear across the gap at a column far from the capsule
+    files = sorted(out.glob("vtk*.vti"))
+    its = np.array([int(f.stem[3:]) for f in files])
+    k = its[1] - its[0]
+    assert abs(k - Nt / 400.0) <= 1.0
+    np.testing.assert_array_equal(its, k * np.arange(len(its)))
+    assert its[-1] <= Nt < its[-1] + k
+    u = _read_vti_vectors(str(files[-1]))
+    np.testing.assert_allclose(u[:, 0, :, 0], -u_max, rtol=1e-5)
+    np.testing.assert_allclose(u[:, N, :, 0], u_max, rtol=1e-5)
+    y = np.arange(N + 1)
+    np.testing.assert_allclose(u[0, :, 0, 0], u_max * (2 * y / N - 1), atol=0.05 * u_max)
