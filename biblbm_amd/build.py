@@ -23,6 +23,10 @@ HIPCC_FLAGS = [
     # packed FP32 instructions stay off (the compiler may otherwise form
     # v_pk_* FP32 ops from scalar or float2 code)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b+c is fused within one source expression only: the rounding of a
+    # cell's arithmetic then does not depend on the kernel it is inlined in
+    # (every launch shape and block order gives bitwise the same results)
+    "-ffp-contract=on",
     "-Wno-unused-command-line-argument", "-Wno-unused-result",
 ]
 

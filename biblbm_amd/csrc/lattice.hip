@@ -58,6 +58,7 @@ LatView view_of(const bl_lattice_s* L)
     v.wall_axis = L->wall_axis;
     for (int k = 0; k < 6; ++k) v.wall_u[k] = L->wall_u[k];
     v.xcd_remap = L->xcd_remap[0];
+    v.row_segs = 1;
     v.emit_u = L->emit_u;
     v.emit_force = L->emit_force;
     v.emit_flags = L->emit_flags;
@@ -82,22 +83,18 @@ __device__ __forceinline__ K odd_slot(const NbrOff& nb, bool inner, int sx, int 
     return valid ? (K)p_valid * (K)st + (K)(c + d) : (K)p_face * (K)st + (K)c;
 }
 
-// EMIT: the velocity staging of an immersed-boundary step rides on the
-// launch (bl_lattice_s::emit_u): cells of flagged tiles also write the
-// computeVelocity value of the populations they loaded, with the last
-// collision's force field.  NT: the even step's population loads and stores
-// carry the nontemporal hint.
-template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT = false, bool NT = false>
-__global__ void __launch_bounds__(256) k_collide_stream(LatView v)
+// One cell of the collide-and-stream pass.  EMIT: the velocity staging of an
+// immersed-boundary step rides on the launch (bl_lattice_s::emit_u): cells
+// of flagged tiles also write the computeVelocity value of the populations
+// they loaded, with the last collision's force field.  NT: nontemporal hints
+// on the population loads and stores, 1 on all of them, 2 on the planes the
+// odd step addresses z-aligned only (on even steps 2 is 1).
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT, int NT>
+__device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int y, int z)
 {
     using P = Pop<S>;
     using C = typename P::C;
     using FT = typename P::F;
-    const unsigned bid = collide_block(blockIdx.x, gridDim.x, v.xcd_remap);
-    const unsigned c = bid * blockDim.x + threadIdx.x;
-    if (c >= v.ncell) return;
-    int x, y, z;
-    decode(c, v, x, y, z);
     S* __restrict__ A = (S*)v.pops;
     const long st = v.stride;
 
@@ -111,18 +108,14 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 
     C f[Q];
     if (ODD) {
-        if (inner) {
 #pragma unroll
-            for (int i = 0; i < Q; ++i)
-                f[i] = P::ld(A, odd_slot<K>(nb, true, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i), i);
-        } else {
-#pragma unroll
-            for (int i = 0; i < Q; ++i)
-                f[i] = P::ld(A, odd_slot<K>(nb, false, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i), i);
+        for (int i = 0; i < Q; ++i) {
+            const K k = odd_slot<K>(nb, inner, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i);
+            f[i] = (NT == 1 || (NT == 2 && cz(i) == 0)) ? P::template ld<K, true>(A, k, i) : P::template ld<K, false>(A, k, i);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = P::template ld<unsigned, NT>(A + i * st, c, i);
+        for (int i = 0; i < Q; ++i) f[i] = P::template ld<unsigned, NT != 0>(A + i * st, c, i);
     }
 
     const unsigned char m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
@@ -147,77 +140,69 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
     }
 
     if (ODD) {
-        if (inner) {
 #pragma unroll
-            for (int i = 0; i < Q; ++i)
-                P::st(A, odd_slot<K>(nb, true, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i)), i, f[i]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < Q; ++i)
-                P::st(A, odd_slot<K>(nb, false, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i)), i, f[i]);
+        for (int i = 0; i < Q; ++i) {
+            const K k = odd_slot<K>(nb, inner, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i));
+            if (NT == 1 || (NT == 2 && cz(i) == 0)) P::template st<K, true>(A, k, i, f[i]);
+            else P::template st<K, false>(A, k, i, f[i]);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) P::template st<unsigned, NT>(A + opp(i) * st, c, i, f[i]);
+        for (int i = 0; i < Q; ++i) P::template st<unsigned, NT != 0>(A + opp(i) * st, c, i, f[i]);
     }
 }
 
-// the even step with nontemporal hints (the odd step's misaligned z +-1
-// accesses rely on the L2 merging partial lines, so it keeps the default)
-template <class S, bool MASK, int FMODE>
-void launch_cs_nt(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT = false, int NT = 0>
+__global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 {
-    if constexpr (FMODE == 2) {
-        if (v.emit_u) {
-            hipLaunchKernelGGL((k_collide_stream<S, false, MASK, 2, false, unsigned, true, true>), dim3(nblk), dim3(blk),
-                               0, s, v);
-            return;
-        }
+    unsigned c;
+    int x, y, z;
+    if (v.xcd_remap == BL_ORDER_ROWS) {
+        if (!row_cell(v, c, x, y, z)) return;
+    } else {
+        c = collide_block(blockIdx.x, gridDim.x, v.xcd_remap) * blockDim.x + threadIdx.x;
+        if (c >= v.ncell) return;
+        decode(c, v, x, y, z);
     }
-    if (phase)
-        hipLaunchKernelGGL((k_collide_stream<S, false, MASK, FMODE, true, unsigned, false, true>), dim3(nblk), dim3(blk),
-                           0, s, v);
-    else
-        hipLaunchKernelGGL((k_collide_stream<S, false, MASK, FMODE, false, unsigned, false, true>), dim3(nblk), dim3(blk),
-                           0, s, v);
+    cs_cell<S, ODD, MASK, FMODE, PHASE, K, EMIT, NT>(v, c, x, y, z);
 }
 
-template <class S, bool ODD, bool MASK, int FMODE>
-void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s, bool nt)
+template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT, int NT>
+void launch_one(const LatView& v, unsigned nblk, int blk, hipStream_t s)
 {
-    if constexpr (!ODD) {
-        if (nt) {
-            launch_cs_nt<S, MASK, FMODE>(v, phase, nblk, blk, s);
-            return;
-        }
-    }
+    hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, PHASE, K, EMIT, NT>), dim3(nblk), dim3(blk), 0, s, v);
+}
+
+template <class S, bool ODD, bool MASK, int FMODE, int NT>
+void launch_cs4(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s)
+{
     const bool big = ODD && (size_t)Q * (size_t)v.stride > 0xffffffffull;
     if constexpr (FMODE == 2) {
         if (v.emit_u) {  // the launcher passes emit_u only without a phase field
-            if (big)
-                hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, 2, false, size_t, true>), dim3(nblk), dim3(blk), 0, s,
-                                   v);
-            else
-                hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, 2, false, unsigned, true>), dim3(nblk), dim3(blk), 0,
-                                   s, v);
+            if (big) launch_one<S, ODD, MASK, 2, false, size_t, true, NT>(v, nblk, blk, s);
+            else launch_one<S, ODD, MASK, 2, false, unsigned, true, NT>(v, nblk, blk, s);
             return;
         }
     }
     if (big) {
-        if (phase)
-            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true, size_t>), dim3(nblk), dim3(blk), 0, s, v);
-        else
-            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, false, size_t>), dim3(nblk), dim3(blk), 0, s, v);
+        if (phase) launch_one<S, ODD, MASK, FMODE, true, size_t, false, NT>(v, nblk, blk, s);
+        else launch_one<S, ODD, MASK, FMODE, false, size_t, false, NT>(v, nblk, blk, s);
     } else {
-        if (phase)
-            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, true, unsigned>), dim3(nblk), dim3(blk), 0, s, v);
-        else
-            hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, false, unsigned>), dim3(nblk), dim3(blk), 0, s, v);
+        if (phase) launch_one<S, ODD, MASK, FMODE, true, unsigned, false, NT>(v, nblk, blk, s);
+        else launch_one<S, ODD, MASK, FMODE, false, unsigned, false, NT>(v, nblk, blk, s);
     }
 }
 
+template <class S, bool ODD, bool MASK, int FMODE>
+void launch_cs3(const LatView& v, bool phase, unsigned nblk, int blk, hipStream_t s, int nt)
+{
+    if (nt == 0) launch_cs4<S, ODD, MASK, FMODE, 0>(v, phase, nblk, blk, s);
+    else if (nt == 1 || !ODD) launch_cs4<S, ODD, MASK, FMODE, 1>(v, phase, nblk, blk, s);
+    else launch_cs4<S, ODD, MASK, FMODE, 2>(v, phase, nblk, blk, s);
+}
+
 template <class S, bool ODD, bool MASK>
-void launch_cs2(const LatView& v, int fmode, bool phase, unsigned nblk, int blk, hipStream_t s, bool nt)
+void launch_cs2(const LatView& v, int fmode, bool phase, unsigned nblk, int blk, hipStream_t s, int nt)
 {
     if (fmode == 0) launch_cs3<S, ODD, MASK, 0>(v, phase, nblk, blk, s, nt);
     else if (fmode == 1) launch_cs3<S, ODD, MASK, 1>(v, phase, nblk, blk, s, nt);
@@ -226,7 +211,7 @@ void launch_cs2(const LatView& v, int fmode, bool phase, unsigned nblk, int blk,
 
 template <class S>
 void launch_cs(const LatView& v, bool odd, bool mask, int fmode, bool phase, unsigned nblk, int blk,
-               hipStream_t s, bool nt)
+               hipStream_t s, int nt)
 {
     if (odd) {
         if (mask) launch_cs2<S, true, true>(v, fmode, phase, nblk, blk, s, nt);
@@ -255,8 +240,12 @@ int collide_stream_launch(bl_lattice_s* L)
     const int par = odd ? 1 : 0;
     const int blk = L->block[par];
     v.xcd_remap = L->xcd_remap[par];
-    const bool nt = L->nontemporal[par] != 0;
-    const unsigned nblk = (unsigned)((L->ncell + blk - 1) / blk);
+    const int nt = L->nontemporal[par];
+    unsigned nblk = (unsigned)((L->ncell + blk - 1) / blk);
+    if (v.xcd_remap == BL_ORDER_ROWS) {
+        v.row_segs = (L->nz + blk - 1) / blk;
+        nblk = (unsigned)(((long)L->nx * L->ny + 7) / 8 * 8 * v.row_segs);
+    }
     if (L->storage == BL_STORE_F64) launch_cs<double>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
     else if (L->storage == BL_STORE_F32) launch_cs<float>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
     else launch_cs<__half>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
@@ -1109,13 +1098,15 @@ int bl_lattice_set_launch_parity(bl_lattice L, int parity, int block, int xcd_re
     // the collide-stream kernels are compiled for at most 256 threads per block
     if (block < 64 || block > 256 || block % 64)
         return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be 64, 128, 192 or 256");
-    if (nontemporal && parity == 1)
-        return fail(BL_ERR_ARG, "bl_lattice_set_launch_parity: nontemporal hints are for the even step only");
-    if (xcd_remap < 0 || xcd_remap > (1 << 20) || (xcd_remap > 1 && (xcd_remap & (xcd_remap - 1))))
-        return fail(BL_ERR_ARG, "bl_lattice_set_launch: xcd_remap must be 0, 1 or a power of two up to 2^20");
+    if (xcd_remap < 0 || xcd_remap > (1 << 20) ||
+        (xcd_remap > 1 && xcd_remap != BL_ORDER_ROWS && (xcd_remap & (xcd_remap - 1))))
+        return fail(BL_ERR_ARG,
+                    "bl_lattice_set_launch: xcd_remap must be 0, 1, BL_ORDER_ROWS (3) or a power of two up to 2^20");
+    if (nontemporal < 0 || nontemporal > 2)
+        return fail(BL_ERR_ARG, "bl_lattice_set_launch_parity: nontemporal must be 0, 1 or 2");
     L->block[parity] = block;
     L->xcd_remap[parity] = xcd_remap;
-    L->nontemporal[parity] = nontemporal ? 1 : 0;
+    L->nontemporal[parity] = parity == 0 && nontemporal ? 1 : nontemporal;
     ++L->gen;
     return BL_OK;
 }

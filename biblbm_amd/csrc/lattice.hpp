@@ -107,6 +107,7 @@ struct LatView {
     int wall_axis;
     double wall_u[6];
     int xcd_remap;
+    int row_segs;  // blocks per z-row under BL_ORDER_ROWS
     void* emit_u;  // velocity staging carried by the collision (see bl_lattice_s)
     const void* emit_force;
     const unsigned char* emit_flags;

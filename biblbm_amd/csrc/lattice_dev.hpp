@@ -33,6 +33,8 @@ struct Pop<double> {
     using F = double;
     template <class K, bool NT = false>
     __device__ static C ld(const double* A, K k, int) { return NT ? ld_nt(A + k) : A[k]; }
+    __device__ static double enc(C v, int) { return v; }
+    __device__ static C dec(double s, int) { return s; }
     template <class K, bool NT = false>
     __device__ static void st(double* A, K k, int, C v)
     {
@@ -46,6 +48,8 @@ struct Pop<float> {
     using F = float;
     template <class K, bool NT = false>
     __device__ static C ld(const float* A, K k, int) { return NT ? ld_nt(A + k) : A[k]; }
+    __device__ static float enc(C v, int) { return v; }
+    __device__ static C dec(float s, int) { return s; }
     template <class K, bool NT = false>
     __device__ static void st(float* A, K k, int, C v)
     {
@@ -62,10 +66,12 @@ struct Pop<__half> {
     {
         return __half2float(NT ? ld_nt(A + k) : A[k]) * (float)(tw(i) / kF16Scale);
     }
+    __device__ static __half enc(C v, int i) { return __float2half_rn(v * (float)(kF16Scale / tw(i))); }
+    __device__ static C dec(__half s, int i) { return __half2float(s) * (float)(tw(i) / kF16Scale); }
     template <class K, bool NT = false>
     __device__ static void st(__half* A, K k, int i, C v)
     {
-        const __half h = __float2half_rn(v * (float)(kF16Scale / tw(i)));
+        const __half h = enc(v, i);
         if (NT) st_nt(A + k, h);
         else A[k] = h;
     }
@@ -168,6 +174,28 @@ __device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f)
     if (f.l == 0) return n;
     const unsigned t = __umulhi(f.m, n);
     return (t + ((n - t) >> 1)) >> (f.l - 1);
+}
+
+// BL_ORDER_ROWS: hardware block b runs on XCD b % 8; its k = b / 8-th block
+// there covers segment k % s of z-row 8 (k / s) + b % 8, s = row_segs blocks
+// per row.  Every cache line that a row's misaligned z +-1 accesses share
+// (with its own wrap-around) then stays in one XCD's L2, while the eight
+// XCDs work on adjacent rows.  False for a thread past the last row or
+// past the row's end.
+__device__ __forceinline__ bool row_cell(const LatView& v, unsigned& c, int& x, int& y, int& z)
+{
+    const unsigned b = blockIdx.x, k = b >> 3;
+    const unsigned s = (unsigned)v.row_segs;
+    const unsigned kr = k / s;
+    const unsigned row = (kr << 3) + (b & 7u);
+    const unsigned zz = (k - kr * s) * blockDim.x + threadIdx.x;
+    if (row >= (unsigned)v.nx * (unsigned)v.ny || zz >= (unsigned)v.nz) return false;
+    const unsigned q = fdiv(row, v.dy);
+    x = (int)q;
+    y = (int)(row - q * (unsigned)v.ny);
+    z = (int)zz;
+    c = row * (unsigned)v.nz + zz;
+    return true;
 }
 
 __device__ __forceinline__ void decode(unsigned c, const LatView& v, int& x, int& y, int& z)

@@ -20,6 +20,7 @@ namespace {
 struct Launch {
     int block = 256, xcd = 1, nt = 0;
     double ms = 0.0;
+    int trials = 0;  // tuning trials that timed it
 };
 
 struct CollideEntry {
@@ -74,7 +75,7 @@ void tune_cache_apply(bl_lattice_s* L)
     for (int p = 0; p < 2; ++p) {
         L->block[p] = e->par[p].block;
         L->xcd_remap[p] = e->par[p].xcd;
-        L->nontemporal[p] = p == 0 ? e->par[p].nt : 0;
+        L->nontemporal[p] = e->par[p].nt;
     }
     ++L->gen;
 }
@@ -88,21 +89,20 @@ extern "C" {
 int bl_tune_collide(bl_lattice L, char* report, int cap)
 {
     if (!L) return fail(BL_ERR_ARG, "bl_tune_collide: null lattice");
-    // candidates: threads per block x block order x nontemporal hints (even
-    // step only); every even candidate is timed once, every odd one over
-    // the same number of launches
+    // candidates, per parity: threads per block x block order x nontemporal
+    // hints
     static const int blocks[] = {64, 128, 192, 256};
-    static const int orders[] = {0, 1, 4, 16, 64, 256, 1024};  // block orders (see bl_lattice_set_launch_parity)
+    static const int orders[] = {0, 1, BL_ORDER_ROWS, 4, 16, 64, 256, 1024};  // see bl_lattice_set_launch_parity
     std::vector<Launch> cand[2];
     for (int xcd : orders)
-        for (int nt = 0; nt < 2; ++nt)
+        for (int nt = 0; nt < 3; ++nt)  // 2: the odd step's z-aligned planes only
             for (int b : blocks) {
                 Launch l;
                 l.block = b;
                 l.xcd = xcd;
                 l.nt = nt;
-                cand[0].push_back(l);
-                if (nt == 0) cand[1].push_back(l);
+                if (nt < 2) cand[0].push_back(l);
+                cand[1].push_back(l);
             }
     const size_t sz = L->pop_bytes;
     const size_t bytes = (size_t)Q * L->stride * sz;
@@ -116,15 +116,15 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
     for (auto& e : ev) BL_HIP(hipEventCreate(&e));
     int rc = BL_OK;
     if (L->steps & 1) rc = collide_stream_launch(L);  // trials start on an even step
-    // trial k: even candidate k, odd candidate k mod (odd candidates); every
-    // candidate of a parity is timed over the same number of launches
+    // trial k: even candidate k mod (even candidates), odd candidate k mod
+    // (odd candidates); each candidate's time is the mean over its trials
     const size_t ntrial = std::max(cand[0].size(), cand[1].size());
     const int reps = 4;
     for (size_t k = 0; k < ntrial && rc == BL_OK; ++k) {
         const Launch& e = cand[0][k % cand[0].size()];
         const Launch& o = cand[1][k % cand[1].size()];
         L->block[0] = e.block; L->xcd_remap[0] = e.xcd; L->nontemporal[0] = e.nt;
-        L->block[1] = o.block; L->xcd_remap[1] = o.xcd; L->nontemporal[1] = 0;
+        L->block[1] = o.block; L->xcd_remap[1] = o.xcd; L->nontemporal[1] = o.nt;
         for (int w = 0; w < 2 && rc == BL_OK; ++w) rc = collide_stream_launch(L);
         double te = 0, to = 0;
         for (int r = 0; r < reps && rc == BL_OK; ++r) {
@@ -141,8 +141,12 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
             te += a;
             to += b;
         }
-        cand[0][k % cand[0].size()].ms += te / reps;
-        cand[1][k % cand[1].size()].ms += to / reps;
+        Launch& ce = cand[0][k % cand[0].size()];
+        Launch& co = cand[1][k % cand[1].size()];
+        ce.ms += te / reps;
+        co.ms += to / reps;
+        ++ce.trials;
+        ++co.trials;
     }
     for (auto& e : ev) hipEventDestroy(e);
     // the trials advanced the state: put the populations and the step count back
@@ -161,12 +165,11 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
     Launch best[2];
     std::string rep = "{";
     for (int p = 0; p < 2; ++p) {
-        const double per = (double)ntrial / (double)cand[p].size();  // trials each candidate took part in
         best[p].ms = 1e30;
         rep += p == 0 ? "\"even\": [" : "], \"odd\": [";
         for (size_t k = 0; k < cand[p].size(); ++k) {
             Launch c = cand[p][k];
-            c.ms /= per;
+            c.ms /= c.trials > 0 ? c.trials : 1;
             char buf[128];
             snprintf(buf, sizeof(buf), "%s{\"block\": %d, \"xcd_remap\": %d, \"nontemporal\": %d, \"ms\": %.6f}",
                      k ? ", " : "", c.block, c.xcd, c.nt, c.ms);
@@ -333,12 +336,12 @@ int bl_tune_cache_load(const char* path)
             for (int p = 0; p < 2; ++p)
                 ok = ok && c.par[p].block >= 64 && c.par[p].block <= 256 && c.par[p].block % 64 == 0 &&
                      c.par[p].xcd >= 0 && c.par[p].xcd <= (1 << 20) &&
-                     (c.par[p].xcd <= 1 || (c.par[p].xcd & (c.par[p].xcd - 1)) == 0);
+                     (c.par[p].xcd <= 1 || c.par[p].xcd == BL_ORDER_ROWS || (c.par[p].xcd & (c.par[p].xcd - 1)) == 0) &&
+                     c.par[p].nt >= 0 && c.par[p].nt <= 2;
             if (!ok) {
                 ++bad;
                 continue;
             }
-            c.par[1].nt = 0;
             c.device = dev;
             bool found = false;
             for (CollideEntry& e : g_collide)

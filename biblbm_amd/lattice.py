@@ -60,9 +60,10 @@ class Lattice:
     def set_launch(self, block=256, xcd_remap=True):
         call("bl_lattice_set_launch", self.handle, int(block), int(bool(xcd_remap)))
 
-    def set_launch_parity(self, parity, block=256, xcd_remap=True, nontemporal=False):
-        call("bl_lattice_set_launch_parity", self.handle, int(parity), int(block), int(bool(xcd_remap)),
-             int(bool(nontemporal)))
+    def set_launch_parity(self, parity, block=256, xcd_remap=True, nontemporal=0):
+        """nontemporal: 0 / False none, 1 / True every population, 2 the
+        odd step's z-aligned planes only."""
+        call("bl_lattice_set_launch_parity", self.handle, int(parity), int(block), int(xcd_remap), int(nontemporal))
 
     def launch(self, parity):
         out = np.zeros(3, np.int32)

@@ -25,7 +25,7 @@ DEFAULT_PATH = os.environ.get("BIBLBM_TUNE_CACHE",
 
 def tune_collide(lattice):
     """bl_tune_collide: returns the JSON report (trials per parity, best)."""
-    buf = ctypes.create_string_buffer(16384)
+    buf = ctypes.create_string_buffer(65536)
     call("bl_tune_collide", lattice.handle, buf, len(buf))
     return json.loads(buf.value.decode())
 

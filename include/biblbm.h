@@ -38,6 +38,8 @@ extern "C" {
 #define BL_STORE_F16S 2  /* FP16 of fBar_i / t_i * 2^10, FP32 collision   */
 
 /* immersed-boundary delta kernels */
+#define BL_ORDER_ROWS 3  /* collide launch order: whole z-rows per XCD (bl_lattice_set_launch_parity) */
+
 #define BL_KERNEL_ROMA 0    /* 3-point, src/fsi/Dirac.h:74-101   */
 #define BL_KERNEL_PESKIN 1  /* 4-point, src/fsi/Dirac.h:23-44    */
 #define BL_KERNEL_TOPHAT 2  /* 2-point, src/fsi/Dirac.h:51-66    */
@@ -140,11 +142,15 @@ int bl_lattice_steps(bl_lattice lat, long long *steps);
  * (its block decomposition is fixed, src/multiBlock/defaultMultiBlockPolicy3D.h:119-129). */
 int bl_lattice_set_launch(bl_lattice lat, int block, int xcd_remap);
 /* The same for one parity (0: even steps, which address each cell's own
- * slots; 1: odd steps, which address the neighbours'), with nontemporal
- * population loads and stores (even steps only).  xcd_remap: 0 keeps the
+ * slots; 1: odd steps, which address the neighbours').  nontemporal: 0 no
+ * hints, 1 nontemporal loads and stores of every population, 2 only of the
+ * planes the odd step addresses z-aligned (its z +-1 accesses are
+ * misaligned; on even steps 2 is 1).  xcd_remap: 0 keeps the
  * hardware's block order (round-robin over the 8 XCDs), 1 gives each XCD
  * one contiguous range of cells, a power of two k >= 2 deals chunks of k
- * consecutive blocks round-robin over the XCDs.  Results do not change. */
+ * consecutive blocks round-robin over the XCDs, BL_ORDER_ROWS (3) gives
+ * each z-row of cells (split into ceil(nz / block) blocks) to one XCD, the
+ * eight XCDs taking adjacent rows.  Results do not change. */
 int bl_lattice_set_launch_parity(bl_lattice lat, int parity, int block, int xcd_remap, int nontemporal);
 /* out = {block, xcd_remap, nontemporal} of one parity. */
 int bl_lattice_get_launch(bl_lattice lat, int parity, int out[3]);
@@ -433,8 +439,8 @@ int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms
 /* No counterpart in the reference (its block decomposition is fixed,
  * src/multiBlock/defaultMultiBlockPolicy3D.h:119-129).  Times the
  * collide-and-stream kernel of this lattice for every launch shape per AA
- * step parity (block 64..256, XCD-aware order on/off, nontemporal hints on
- * the even step on/off), keeps the fastest for each parity, restores the
+ * step parity (block 64..256, the block orders and nontemporal levels of
+ * bl_lattice_set_launch_parity), keeps the fastest for each parity, restores the
  * populations and step count the trials advanced, and records the result in
  * the process's tuning table (a lattice of the same size, storage and
  * device created later starts with it).  report (cap bytes, may be NULL)

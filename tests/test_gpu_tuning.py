@@ -21,11 +21,11 @@ def test_collide_tuner_keeps_state_and_results(device):
     b.ibm.coupled_steps(3, use_graph=False)
     before = a.lattice.get_populations()
     rep = tuning.tune_collide(a.lattice)
-    assert len(rep["even"]) == 56 and len(rep["odd"]) == 28
+    assert len(rep["even"]) == 64 and len(rep["odd"]) == 96
     for par in ("even", "odd"):
         assert rep["best"][par]["block"] in (64, 128, 192, 256)
-        assert rep["best"][par]["xcd_remap"] in (0, 1, 4, 16, 64, 256, 1024)
-    assert rep["best"]["odd"]["nontemporal"] == 0
+        assert rep["best"][par]["xcd_remap"] in (0, 1, 3, 4, 16, 64, 256, 1024)
+    assert rep["best"]["even"]["nontemporal"] in (0, 1) and rep["best"]["odd"]["nontemporal"] in (0, 1, 2)
     assert a.lattice.steps == 3
     assert np.array_equal(a.lattice.get_populations(), before)
     got = [a.lattice.launch(p) for p in (0, 1)]
@@ -40,14 +40,17 @@ def test_collide_tuner_keeps_state_and_results(device):
         w.lattice.close()
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16s"])
-def test_nontemporal_even_step_is_bitwise(device, storage):
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s"])
+def test_nontemporal_hints_are_bitwise(device, storage):
+    """Nontemporal population loads and stores on both parities (level 1: the
+    odd step's misaligned z +-1 accesses included; level 2: its z-aligned
+    planes only) leave the results unchanged."""
     from biblbm_amd.lattice import Lattice
     rng = np.random.default_rng(4)
     shape = (24, 20, 28)
     f0 = None
     Ls = []
-    for nt in (False, True):
+    for nt in (0, 1, 2):
         L = Lattice(*shape, periodic=(1, 0, 1), storage=storage, omega=1 / 0.8)
         mask = np.zeros(shape, np.uint8)
         mask[:, 0, :] = mask[:, -1, :] = 1
@@ -59,9 +62,12 @@ def test_nontemporal_even_step_is_bitwise(device, storage):
         else:
             L.put_populations(f0)
         L.set_launch_parity(0, 128, False, nt)
+        L.set_launch_parity(1, 192, 16, nt)
+        assert L.launch(1)["nontemporal"] == nt and L.launch(0)["nontemporal"] == min(nt, 1)
         L.collide_and_stream(5)
         Ls.append(L)
-    assert np.array_equal(Ls[0].get_populations(), Ls[1].get_populations())
+    for L in Ls[1:]:
+        assert np.array_equal(Ls[0].get_populations(), L.get_populations())
 
 
 def test_coupled_tuner_layouts_give_the_same_trajectory(device):
@@ -98,11 +104,12 @@ def test_tuning_table_save_load(device, tmp_path):
         tuning.load(str(bad))
 
 
-@pytest.mark.parametrize("order", [0, 1, 2, 4, 16, 64, 1024])
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4, 16, 64, 1024])
 def test_block_orders_are_bitwise(device, order):
     """Every block order (hardware order, one range per XCD, chunks of k
-    blocks over the XCDs) gives the same populations, on a lattice whose
-    block count is not a multiple of the chunk round (tail blocks)."""
+    blocks over the XCDs, whole z-rows per XCD) gives the same populations,
+    on a lattice whose block count is not a multiple of the chunk round
+    (tail blocks) and whose row count is not a multiple of 8."""
     from biblbm_amd.lattice import Lattice
     rng = np.random.default_rng(7)
     shape = (30, 26, 22)  # 17160 cells: 67.03 blocks of 256
@@ -121,3 +128,33 @@ def test_block_orders_are_bitwise(device, order):
         L.set_launch_parity(p, 64 if p else 128, order, False)
     L.collide_and_stream(4)
     assert np.array_equal(L.get_populations(), ref)
+
+
+@pytest.mark.parametrize("shape,periodic,block,storage", [
+    ((9, 5, 150), (1, 1, 1), 64, "f64"), ((9, 5, 150), (1, 0, 0), 128, "f64"), ((7, 3, 256), (0, 1, 1), 256, "f64"),
+    ((4, 6, 64), (1, 1, 1), 192, "f64"), ((5, 4, 70), (0, 0, 0), 64, "f32"), ((6, 5, 90), (1, 1, 0), 128, "f16s")])
+def test_row_orders_are_bitwise(device, shape, periodic, block, storage):
+    """BL_ORDER_ROWS (rows longer than a block: several segments per row, the
+    last one partial; a row count that is not a multiple of 8; reflection at
+    bounded x, y and z faces), both parities, with every nontemporal level,
+    against the hardware order."""
+    from biblbm_amd.lattice import Lattice
+    n = int(np.prod(shape))
+    rho = 1 + 0.01 * np.sin(np.arange(n)).reshape(shape)
+    u = 0.02 * np.cos(np.arange(3 * n)).reshape(shape + (3,))
+    mask = np.zeros(shape, np.uint8)
+    mask[1, 1, 1] = 1  # one bounce-back cell
+    runs = []
+    for order, nt in ((0, 0), (3, 0), (3, 1), (3, 2), (16, 2)):
+        L = Lattice(*shape, periodic=periodic, storage=storage, omega=1 / 0.8)
+        L.set_mask(mask)
+        L.init_equilibrium(rho, u)
+        L.set_body_force((1e-5, -2e-6, 3e-6))
+        for p in (0, 1):
+            L.set_launch_parity(p, block, order, nt)
+            assert L.launch(p) == dict(block=block, xcd_remap=order, nontemporal=min(nt, 1) if p == 0 else nt)
+        L.collide_and_stream(5)
+        runs.append(L.get_populations())
+        L.close()
+    for r in runs[1:]:
+        assert np.array_equal(runs[0], r)
