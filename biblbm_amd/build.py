@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbiblbm.so")
 
 SOURCES = ["errors.cpp", "lattice.hip", "ibm.hip", "mesh.cpp", "ensemble.hip", "walls.hip", "interactions.hip", "io.cpp",
-           "init.cpp", "tune.cpp"]
+           "init.cpp", "tune.cpp", "rigid.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

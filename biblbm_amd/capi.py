@@ -117,6 +117,11 @@ SIGNATURES = {
     "bl_ibm_add_wall_interaction": [_P, _P, _P],
     "bl_ibm_add_pp_force": [_P, _P],
     "bl_ibm_compute_collision_forces": [_P, _P, _P],
+    "bl_rigid_add": [_P, _I, _P, _P],
+    "bl_rigid_move": [_P],
+    "bl_rigid_set_scale": [_P, _I, _D],
+    "bl_rigid_get": [_P, _I, _P],
+    "bl_rigid_grow": [_P, _I, _D, _I, _P, _P],
 }
 
 

@@ -1492,7 +1492,7 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int4* _
 __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* __restrict__ p_choff,
                                                   const double* __restrict__ partial, const long* __restrict__ p_voff,
                                                   double* __restrict__ geom, double* __restrict__ delta,
-                                                  double* __restrict__ pos, int remap)
+                                                  double* __restrict__ pos, int remap, const int* __restrict__ p_model)
 {
     __shared__ double sft[3];
     __shared__ double part[64][11];  // a batch of chunk partials, loaded by all lanes at once
@@ -1522,7 +1522,8 @@ __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* _
         double com[3] = {acc[2] / area, acc[3] / area, acc[4] / area};
         for (int d = 0; d < 3; ++d) {
             sft[d] = 0.0;
-            if (!remap || !gd.per[d]) continue;
+            // a rigid particle maps its own centre of mass (rigid.hip)
+            if (!remap || !gd.per[d] || p_model[p] == BL_MODEL_RIGID) continue;
             const double L = (double)gd.n[d];
             const double x0 = -0.5;
             const double r = com[d] - L * floor((com[d] - x0) * (1. / L));
@@ -1761,7 +1762,8 @@ int update_particles(bl_ibm g, int remap, hipStream_t s)
     hipLaunchKernelGGL(k_geom_partial, dim3(g->nchunk), dim3(256), 0, s, f, (const int4*)g->ch_tab,
                        (const int*)g->t_vert, g->ch_partial);
     hipLaunchKernelGGL(k_geom_final, dim3(g->np), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
-                       (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, g->pos, remap);
+                       (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, g->pos, remap,
+                       (const int*)g->p_model);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
@@ -2377,6 +2379,7 @@ int need_stencil(bl_ibm g)
 {
     int rc = need_init(g);
     if (rc) return rc;
+    if ((rc = rigid_refuse_coupling(g))) return rc;
     const bl_lattice L = g->lat;
     const int n3[3] = {L->nx, L->ny, L->nz};
     for (int d = 0; d < 3; ++d)
@@ -2424,7 +2427,7 @@ int bl_ibm_destroy(bl_ibm g)
                     g->tile_count, g->tile_list, g->active, g->n_active,
                     g->col_count, g->col_off, g->col_fill, g->col_z, g->n_hits, g->phase,
                     g->flags_alt, g->force_alt, g->ch_part, g->ch_idx, g->p_choff, g->ch_partial, g->ch_tab, g->t_vert, g->v_inc, g->v_near, g->v_blend, g->v_base_t, g->v_w_t, g->v_part, g->t_part, g->r_part,
-                    g->l_part, g->ext_force};
+                    g->l_part, g->ext_force, g->r_state};
     for (void* p : ptrs) hipFree(p);
     interactions_free(g);
     for (auto& e : g->ev)
@@ -2672,8 +2675,10 @@ int bl_ibm_init(bl_ibm g)
     if ((rc = alloc_near(g))) return rc;
     if (!g->pp.empty() && (rc = interactions_alloc(g))) return rc;
     // update() of every particle and the centre-of-mass mapping of add_particle
+    // (rigid particles map their own centre, rigid_init)
     rc = update_particles(g, 1);
     if (rc) return rc;
+    if ((rc = rigid_init(g))) return rc;
     BL_HIP(hipStreamSynchronize(L->stream));
     return BL_OK;
 }
@@ -2699,6 +2704,7 @@ int bl_ibm_interpolate(bl_ibm g)
 int bl_ibm_compute_forces(bl_ibm g)
 {
     int rc = need_init(g);
+    if (!rc) rc = rigid_refuse_coupling(g);
     return rc ? rc : do_forces(g, g->lat->stream);
 }
 
@@ -2713,6 +2719,7 @@ int bl_ibm_spread(bl_ibm g)
 int bl_ibm_move(bl_ibm g)
 {
     int rc = need_init(g);
+    if (!rc) rc = rigid_refuse_coupling(g);
     if (rc) return rc;
     if ((rc = do_move(g))) return rc;
     return do_voxelize(g, g->lat->stream);

@@ -24,6 +24,7 @@ struct bl_ibm_s {
         double params[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         std::vector<double> pos;
         std::vector<double> vel, force;  // from a checkpoint (else zero at init)
+        double rigid[32] = {};  // BL_MODEL_RIGID: state, then the rest-shape inertia and volume (rigid.hip)
     };
     std::vector<PartHost> parts;
     bool inited = false;
@@ -91,6 +92,10 @@ struct bl_ibm_s {
     double* v_blend = nullptr;        // per vertex: distance - half support
     int* v_base_t = nullptr;          // per vertex: top-hat stencil base (3)
     double* v_w_t = nullptr;          // per vertex: top-hat weights (2 per axis)
+
+    // ---- rigid particles (BL_MODEL_RIGID, rigid.hip) ----
+    int n_rigid = 0;
+    double* r_state = nullptr;  // 32 per particle (rigid ones only filled)
 
     // ---- interaction forces ----
     struct WallDeco {
@@ -175,6 +180,10 @@ void ibm_invalidate_graphs(bl_ibm g);  // captured steps no longer match the lat
 // BL_OK when the boundary is initialised and every periodic extent of its
 // lattice holds one stencil width (the kernel stages need it).
 int need_stencil(bl_ibm g);
+// Rigid particles: state upload at bl_ibm_init (rigid.hip); BL_ERR_STATE
+// from the fluid-coupling stages while the boundary holds rigid particles.
+int rigid_init(bl_ibm g);
+int rigid_refuse_coupling(bl_ibm g);
 // A full force field was loaded into the lattice (every tile flagged):
 // steps must run eagerly until the flags are consistent again.
 bool ibm_needs_settle(bl_ibm g);

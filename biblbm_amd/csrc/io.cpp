@@ -88,6 +88,8 @@ int vertex_state(bl_ibm g, int pid, std::vector<double>& pos, std::vector<double
 int pack_particle(bl_ibm g, int pid, std::vector<char>& b)
 {
     const auto& p = g->parts[pid];
+    if (p.model == BL_MODEL_RIGID)  // RigidParticle3D::pack (src/fsi/RigidParticle.hh:114-130) is not restated
+        return fail(BL_ERR_STATE, "checkpoint: rigid particles have no packed form here");
     std::vector<double> pos, vel, frc;
     int rc = vertex_state(g, pid, pos, vel, frc);
     if (rc) return rc;

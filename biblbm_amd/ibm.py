@@ -58,6 +58,37 @@ class ImmersedBoundary:
     def init(self):
         call("bl_ibm_init", self.handle)
 
+    # --- rigid particles (growth packing) ------------------------------------
+    def add_rigid(self, sid, com, quat=(1.0, 0.0, 0.0, 0.0), scale=1.0, density=1.0, damping=0.0, relaxation=0.45):
+        """RigidParticle3D (src/fsi/RigidParticle.hh:13-39) at centre `com`
+        with orientation quaternion (w, x, y, z) and the given scale."""
+        st = f64(list(com) + list(quat) + [scale, density, damping, relaxation])
+        pid = ctypes.c_int()
+        call("bl_rigid_add", self.handle, int(sid), ptr(st), ctypes.byref(pid))
+        nv = self.shapes[sid][0]
+        self.particles.append((sid, nv))
+        self.specs.append((int(sid), 4, st.copy(), False))
+        return pid.value
+
+    def rigid_move(self):
+        """move_vertices() of every rigid particle under the current forces."""
+        call("bl_rigid_move", self.handle)
+
+    def rigid_set_scale(self, scale, pid=-1):
+        call("bl_rigid_set_scale", self.handle, int(pid), float(scale))
+
+    def rigid_state(self, pid):
+        """dict of com, vel, angvel, quat, scale, density, damping, relaxation."""
+        out = np.empty(17)
+        call("bl_rigid_get", self.handle, int(pid), ptr(out))
+        return dict(com=out[0:3], vel=out[3:6], angvel=out[6:9], quat=out[9:13], scale=out[13], density=out[14],
+                    damping=out[15], relaxation=out[16])
+
+    def rigid_grow(self, n_iter, initial_scale, potential, boundary=None, every=10):
+        """The growth loop of progs/common/create_particles.h:129-155."""
+        call("bl_rigid_grow", self.handle, int(n_iter), float(initial_scale), int(every), ctypes.byref(potential),
+             None if boundary is None else ctypes.byref(boundary))
+
     def set_boundary(self, boundary):
         """set_boundary (src/fsi/ImmersedBoundaryDynamics.h:93): near-wall
         kernel blending and vertex reflection; None removes it."""

@@ -54,6 +54,8 @@ extern "C" {
  * "Skalak + bending + area + volume" membrane of the red-cell workloads.
  * params: {G, C, k_area_global, k_area_local, k_volume, vol_desired, k_bend} */
 #define BL_MODEL_SKALAK_RBC 3
+/* RigidParticle3D (src/fsi/RigidParticle.hh:13-98), added with bl_rigid_add */
+#define BL_MODEL_RIGID 4
 
 typedef struct bl_lattice_s *bl_lattice;
 typedef struct bl_ibm_s *bl_ibm;
@@ -419,6 +421,39 @@ int bl_ibm_add_pp_force(bl_ibm g, const bl_potential *u);
  * none), its wall forces (src/fsi/CollisionForces.h:167-187) on vertices
  * in unit cells that the boundary, widened by the cut-off, crosses. */
 int bl_ibm_compute_collision_forces(bl_ibm g, const bl_potential *u, const bl_boundary *b);
+
+/* ------------------------------------------ rigid particles, packing */
+/* A RigidParticle3D (src/fsi/RigidParticle.hh:13-39) of shape shape_id:
+ * state = {com x, y, z, quaternion w, x, y, z, scale, density, damping,
+ * relaxation factor}; velocity and angular velocity start at zero, the
+ * vertices at com + R(q) (scale v_rest).  The rest-shape inertia is
+ * ParticleShape's (src/fsi/ParticleShape.hh:249-360, about the mesh
+ * origin).  Like bl_particle_add, before bl_ibm_init, whose periodic mapping
+ * moves the centre of mass (and the vertices) onto the grid.  A boundary
+ * holding rigid particles refuses the fluid-coupling stages (bl_ibm_step,
+ * bl_coupled_steps, ...; BL_ERR_STATE): RigidParticle3D::compute_forces is
+ * not implemented. */
+int bl_rigid_add(bl_ibm g, int shape_id, const double state[11], int *pid);
+/* move_vertices() of every rigid particle (src/fsi/ImmersedBoundaryDynamics.hh:
+ * 454-493 with RigidParticle3D::move_vertices, src/fsi/RigidParticle.hh:57-98)
+ * under the current vertex forces: force and torque summed over the
+ * vertices in order, velocity, angular velocity, centre of mass and
+ * orientation advanced, vertices regenerated, centre of mass mapped onto the
+ * periodic grid with the vertices. */
+int bl_rigid_move(bl_ibm g);
+/* p->scale() = scale; p->update() for rigid particle pid (-1: all). */
+int bl_rigid_set_scale(bl_ibm g, int pid, double scale);
+/* out = {com (3), velocity (3), angular velocity (3), quaternion (4),
+ * scale, density, damping, relaxation factor} of rigid particle pid. */
+int bl_rigid_get(bl_ibm g, int pid, double out[17]);
+/* The growth loop of create_particles (progs/common/create_particles.h:
+ * 129-155) on the device: for it = 0..n_iter, every `every` iterations the
+ * scale of all rigid particles becomes initial_scale + (1 - initial_scale)
+ * it / n_iter (vertices regenerated); then bl_ibm_compute_collision_forces(u,
+ * b) and bl_rigid_move.  The boundary's particles must all be rigid for the
+ * packing to mean the reference's. */
+int bl_rigid_grow(bl_ibm g, int n_iter, double initial_scale, int every, const bl_potential *u,
+                  const bl_boundary *b);
 
 /* ----------------------------------------------------------- ensembles */
 /* No counterpart in the reference, which runs independent boxes as separate

@@ -199,6 +199,19 @@ int or_boundary_intersects_rect(const or_boundary *b, const double r[6], double 
 long or_collision_forces(int nx, int ny, int nz, const int periodic[3], const or_potential *u, long nv,
                          const double *pos, const int *obj, const or_boundary *wall, double *force);
 
+/* ---- rigid bodies (growth packing) ------------------------------------ */
+/* Rest-shape moment of inertia about the mesh origin
+ * (src/fsi/ParticleShape.hh:249-360), row-major. */
+void or_shape_inertia(long nv, long nt, const int *tri, const double *rest, double I[9]);
+/* state (16): com (3), velocity (3), angular velocity (3), quaternion
+ * (w, x, y, z), scale, density, damping.  update: pos = com + R (scale v)
+ * (src/fsi/RigidParticle.hh:30-39); move: RigidParticle3D::move_vertices
+ * (src/fsi/RigidParticle.hh:57-98) with the vertex forces `force`, then the
+ * centre of mass mapped onto the periodic grid with the vertices. */
+void or_rigid_update(long nv, const double *rest, const double *state, double *pos);
+void or_rigid_move(long nv, const double *rest, const double I[9], double volume, const int periodic[3],
+                   const int n[3], double *state, double *pos, const double *force);
+
 #ifdef __cplusplus
 }
 #endif

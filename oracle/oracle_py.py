@@ -586,3 +586,77 @@ def place_spheres(box, boundary, max_radius, batches, seed=None):
             got.append(c)
         out.append(np.array(got) if len(got) == count else None)
     return out
+
+
+# ---- rigid bodies (growth packing) -------------------------------------
+lib.or_shape_inertia.restype = None
+lib.or_rigid_update.restype = None
+lib.or_rigid_move.restype = None
+lib.or_rigid_move.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+lib.or_rigid_update.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+lib.or_shape_inertia.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+
+
+def shape_inertia(rest, tri):
+    """Rest-shape moment of inertia about the mesh origin (3x3)."""
+    rest = _c(rest, np.float64)
+    tri = _c(tri, np.int32)
+    out = np.empty(9)
+    lib.or_shape_inertia(len(rest), len(tri), _p(tri), _p(rest), _p(out))
+    return out.reshape(3, 3)
+
+
+def rigid_state(com, quat, scale, density=1.0, damping=0.0, vel=(0, 0, 0), angvel=(0, 0, 0)):
+    """The 16-double rigid state: com, velocity, angular velocity,
+    quaternion (w, x, y, z), scale, density, damping."""
+    return np.concatenate([np.asarray(com, float), np.asarray(vel, float), np.asarray(angvel, float),
+                           np.asarray(quat, float), [scale, density, damping]])
+
+
+def rigid_update(rest, state):
+    rest = _c(rest, np.float64)
+    out = np.empty_like(rest)
+    lib.or_rigid_update(len(rest), _p(rest), _p(_c(state, np.float64)), _p(out))
+    return out
+
+
+def rigid_move(rest, inertia, volume, periodic, shape, state, pos, force):
+    """One RigidParticle3D::move_vertices + periodic centre-of-mass remap;
+    returns (state, pos) updated copies."""
+    rest = _c(rest, np.float64)
+    st = _c(state, np.float64).copy()
+    p = _c(pos, np.float64).copy()
+    lib.or_rigid_move(len(rest), _p(rest), _p(_c(inertia, np.float64).reshape(-1)), float(volume),
+                      _p(_c(periodic, np.int32)), _p(_c(shape, np.int32)), _p(st), _p(p),
+                      _p(_c(force, np.float64)))
+    return st, p
+
+
+def rigid_grow(rest, inertia, volume, shape, periodic, states, u, boundary, n_iter, initial_scale, every=10,
+               record=()):
+    """The growth loop of progs/common/create_particles.h:129-155 over rigid
+    particles (states: (n, 16)): every `every` iterations the scale moves
+    linearly from initial_scale to 1 and the vertices are regenerated; each
+    iteration the collision handler's pair and wall forces, then every
+    particle moves in order.  Returns (states, positions) after the last
+    iteration and {k: (states, positions)} after each iteration in
+    `record`."""
+    states = np.array(states, dtype=np.float64)
+    n = len(states)
+    nv = len(rest)
+    obj = np.repeat(np.arange(n, dtype=np.int32), nv)
+    pos = np.concatenate([rigid_update(rest, s) for s in states])
+    out = {}
+    for it in range(n_iter + 1):
+        if it % every == 0:
+            scale = initial_scale + (1.0 - initial_scale) * (it / n_iter)
+            states[:, 13] = scale
+            pos = np.concatenate([rigid_update(rest, s) for s in states])
+        force, _ = collision_forces(shape, periodic, u, pos, obj, wall=boundary)
+        for k in range(n):
+            sl = slice(k * nv, (k + 1) * nv)
+            states[k], pos[sl] = rigid_move(rest, inertia, volume, periodic, shape, states[k], pos[sl], force[sl])
+        if it in record:
+            out[it] = (states.copy(), pos.copy())
+    return states, pos, out
