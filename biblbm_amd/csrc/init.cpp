@@ -62,15 +62,20 @@ int bl_positions_create(const int box[6], const bl_boundary* boundary, double ma
 {
     if (!box || !out) return fail(BL_ERR_ARG, "bl_positions_create: null argument");
     if (!(max_radius > 0)) return fail(BL_ERR_ARG, "bl_positions_create: max_radius must be positive");
+    // the search grid has cells of at least 2 max_radius: at most 2^24 of them
+    double cells = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        const double ext = (double)box[2 * a + 1] - (double)box[2 * a];
+        if (!(ext > 0)) return fail(BL_ERR_ARG, "bl_positions_create: empty box");
+        cells *= std::max(1.0, std::floor(ext / (2 * max_radius)));
+    }
+    if (!(cells <= (double)(1 << 24)))
+        return fail(BL_ERR_CAPACITY, "bl_positions_create: the box holds more than 2^24 cells of 2 max_radius");
     bl_positions h = new bl_positions_s();
     for (int a = 0; a < 3; ++a) {
         h->lo[a] = box[2 * a];
-        h->ext[a] = (double)(box[2 * a + 1] - box[2 * a]);
-        if (h->ext[a] <= 0) {
-            delete h;
-            return fail(BL_ERR_ARG, "bl_positions_create: empty box");
-        }
-        h->nc[a] = std::max(1, (int)std::floor(h->ext[a] / (2 * max_radius)));
+        h->ext[a] = (double)box[2 * a + 1] - (double)box[2 * a];
+        h->nc[a] = (int)std::max(1.0, std::floor(h->ext[a] / (2 * max_radius)));
         h->cell[a] = h->ext[a] / h->nc[a];
     }
     h->grid.resize((size_t)h->nc[0] * h->nc[1] * h->nc[2]);

@@ -37,6 +37,8 @@
 // type can hold (the Skalak + bending + area + volume membrane, or a capsule
 // without the reference's c term) is written with type id kTypeOwn and its
 // model, flag and eight parameters after the vertex block.
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -188,6 +190,15 @@ int unpack_particle(bl_ibm g, const char*& it, const char* end, long long* id)
     return BL_OK;
 }
 
+// A regular file (not a directory, device or missing path): reading a
+// directory through a filebuf throws, so every path a checkpoint names is
+// checked before it is opened.
+bool regular_file(const std::string& path)
+{
+    struct stat st;
+    return ::stat(path.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
 std::string join(const std::string& folder, const std::string& file)
 {
     if (folder.empty()) return file;
@@ -272,11 +283,31 @@ int bl_ibm_save_checkpoint(bl_ibm g, const char* folder, const char* name)
     return out ? BL_OK : fail(BL_ERR_ARG, "bl_ibm_save_checkpoint: write failed");
 }
 
+namespace {
+int load_checkpoint(bl_ibm g, const char* folder, const char* name);
+}
+
 int bl_ibm_load_checkpoint(bl_ibm g, const char* folder, const char* name)
 {
     if (!g || !folder || !name) return fail(BL_ERR_ARG, "bl_ibm_load_checkpoint: null argument");
     if (g->inited) return fail(BL_ERR_STATE, "bl_ibm_load_checkpoint: particles must be loaded before bl_ibm_init");
+    std::vector<bl_ibm_s::PartHost> keep = g->parts;
+    try {
+        return load_checkpoint(g, folder, name);
+    } catch (const std::exception& e) {  // no exception leaves the C ABI
+        g->parts.swap(keep);
+        return fail(BL_ERR_ARG, std::string("bl_ibm_load_checkpoint: ") + e.what());
+    }
+}
+
+}  // extern "C"
+
+namespace {
+int load_checkpoint(bl_ibm g, const char* folder, const char* name)
+{
     const std::string base(folder), nm(name);
+    if (!regular_file(join(base, nm + ".plb")))
+        return fail(BL_ERR_ARG, "bl_ibm_load_checkpoint: cannot open " + join(base, nm + ".plb"));
     std::ifstream h(join(base, nm + ".plb"));
     if (!h) return fail(BL_ERR_ARG, "bl_ibm_load_checkpoint: cannot open " + join(base, nm + ".plb"));
     long long total = -1, nfiles = -1;
@@ -289,8 +320,9 @@ int bl_ibm_load_checkpoint(bl_ibm g, const char* folder, const char* name)
     std::vector<long long> ids;
     for (long long f = 0; f < nfiles; ++f) {
         std::getline(h, line);
-        std::ifstream in(join(base, line), std::ios::binary);
-        if (!in) {
+        std::ifstream in;
+        if (regular_file(join(base, line))) in.open(join(base, line), std::ios::binary);
+        if (!in.is_open() || !in) {
             g->parts.swap(keep);
             return fail(BL_ERR_ARG, "bl_ibm_load_checkpoint: cannot open " + join(base, line));
         }
@@ -332,6 +364,9 @@ int bl_ibm_load_checkpoint(bl_ibm g, const char* folder, const char* name)
     g->parts.swap(sorted);
     return BL_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int bl_particle_write_vtk(bl_ibm g, int pid, const char* path)
 {
@@ -475,6 +510,7 @@ int bl_lattice_save_checkpoint(bl_lattice L, const char* path)
 int bl_lattice_load_checkpoint(bl_lattice L, const char* path)
 {
     if (!L || !path) return fail(BL_ERR_ARG, "bl_lattice_load_checkpoint: null argument");
+    if (!regular_file(path)) return fail(BL_ERR_ARG, std::string("bl_lattice_load_checkpoint: cannot open ") + path);
     std::ifstream in(path, std::ios::binary);
     if (!in) return fail(BL_ERR_ARG, std::string("bl_lattice_load_checkpoint: cannot open ") + path);
     char magic[sizeof(kLatticeMagic) - 1];
