@@ -236,32 +236,51 @@ def precision_leg(steps, checkpoints=(100, 200)):
     return out
 
 
-def suspension_leg(steps, warmup):
+def suspension_leg(steps, warmup, horizons=(500, 1000, 2000)):
+    """The 400-cell channel in FP64, FP32 and FP16-shifted storage from the
+    same start (the immersed boundary in FP64 with FP64 populations, FP32
+    otherwise): speed over `steps` after `warmup`, then the runs continue to
+    each horizon (steps from the start) and the velocity field and vertex
+    positions are compared with the FP64 run's there."""
     import numpy as np
     from biblbm_amd import workloads
     out = {}
-    u_ref = None
+    ref = {}
     for storage in ("f64", "f32", "f16s"):
         W = workloads.suspension(storage)
         dt = time_coupled(W, steps, warmup)
-        _, u = W.lattice.moments()
+        done = warmup + steps
+        errs = {}
+        for h in [done] + [h for h in horizons if h > done]:
+            if h > done:
+                W.ibm.coupled_steps(h - done, use_graph=True)
+                done = h
+            _, u = W.lattice.moments()
+            x = W.ibm.positions()
+            if storage == "f64":
+                ref[h] = (u, x)
+            else:
+                errs["at_%d_steps" % h] = dict(
+                    velocity_rel_l2_vs_f64=float(np.linalg.norm(u - ref[h][0]) / np.linalg.norm(ref[h][0])),
+                    position_rel_l2_vs_f64=float(np.linalg.norm(x - ref[h][1]) / np.linalg.norm(ref[h][1])))
+            del u, x
         W.ibm.step()
         stages = {k: round(v, 5) for k, v in W.ibm.stage_times().items()}
-        if storage == "f64":
-            u_ref = u
-            err = 0.0
-        else:
-            err = float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))
-        out[storage] = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
-                            velocity_rel_l2_vs_f64=err, ibm_stage_ms=stages)
+        row = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
+                   ibm_stage_ms=stages)
+        if storage != "f64":
+            row["velocity_rel_l2_vs_f64"] = errs["at_%d_steps" % (warmup + steps)]["velocity_rel_l2_vs_f64"]
+            row["errors"] = errs
+        out[storage] = row
         nv, vf = W.vertices, W.params["volume_fraction"]
         W.ibm.close()
         W.lattice.close()
         del W
+    ref.clear()
     out["fp16s_over_fp32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
     out["workload"] = ("suspension 512x128x128 channel, 400 cells (300x642, 80x2562, 20x162 vertices), "
-                       "%.1f%% volume fraction, %d vertices, %d steps after %d warmup" %
-                       (100 * vf, nv, steps, warmup))
+                       "%.1f%% volume fraction, %d vertices, %d steps timed after %d warmup, errors against the "
+                       "FP64 run up to %d steps" % (100 * vf, nv, steps, warmup, max(horizons)))
     return out
 
 

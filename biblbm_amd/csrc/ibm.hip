@@ -39,17 +39,24 @@ constexpr int kSmallTile = 64;     // tiles with more entries are spread by a wh
 constexpr int kMaxList = kTileCap; // entries the spreading kernel sorts in LDS
 
 // ------------------------------------------------------------- helpers
+// Vertex data (positions, velocities, forces, element forces, stencil
+// weights, spreading sums) is held and computed in the real type R: double
+// with FP64 populations, float with FP32 and FP16-shifted ones.  Rest
+// geometry, per-particle area / volume / centroid reductions and the
+// conservation sums stay FP64 in both.
+
 // Delta kernels (src/fsi/Dirac.h:23-66,74-101), 1-D factor.
-__device__ __forceinline__ double dirac(int kernel, double x)
+template <class R>
+__device__ __forceinline__ R dirac(int kernel, R x)
 {
-    const double dx = fabs(x);
+    const R dx = fabs(x);
     if (kernel == BL_KERNEL_ROMA)
-        return (1.0 / 6.0) * (dx > 0.5 ? (5.0 - 3 * dx - sqrt(-3 * (1 - dx) * (1 - dx) + 1))
-                                       : 2.0 * (1 + sqrt(1 - 3 * dx * dx)));
+        return (R(1.0) / R(6.0)) * (dx > R(0.5) ? (R(5.0) - R(3) * dx - sqrt(-R(3) * (R(1) - dx) * (R(1) - dx) + R(1)))
+                                                : R(2.0) * (R(1) + sqrt(R(1) - R(3) * dx * dx)));
     if (kernel == BL_KERNEL_PESKIN)
-        return 0.125 * (dx <= 1 ? (3 - 2 * dx + sqrt(1 + 4 * dx * (1 - dx)))
-                                : (5 - 2 * dx - sqrt(-7 + 4 * dx * (3 - dx))));
-    return 1. - fabs(x);
+        return R(0.125) * (dx <= R(1) ? (R(3) - R(2) * dx + sqrt(R(1) + R(4) * dx * (R(1) - dx)))
+                                      : (R(5) - R(2) * dx - sqrt(-R(7) + R(4) * dx * (R(3) - dx))));
+    return R(1.) - fabs(x);
 }
 
 __device__ __forceinline__ int wrap_node(int i, int n, int per)
@@ -61,21 +68,45 @@ __device__ __forceinline__ int wrap_node(int i, int n, int per)
     return (i < 0 || i >= n) ? -1 : i;
 }
 
-struct D3 {
-    double x, y, z;
+template <class R>
+struct V3 {
+    R x, y, z;
 };
-__device__ __forceinline__ D3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
-__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ D3 operator/(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
-__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ D3 cross(D3 a, D3 b)
+using D3 = V3<double>;
+template <class R>
+__device__ __forceinline__ V3<R> ld3(const R* p) { return {p[0], p[1], p[2]}; }
+// a vertex in FP64 (geometry reductions, ray tracing)
+template <class R>
+__device__ __forceinline__ D3 ld3d(const R* p) { return {(double)p[0], (double)p[1], (double)p[2]}; }
+template <class R>
+__device__ __forceinline__ V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R>
+__device__ __forceinline__ V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R>
+__device__ __forceinline__ V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R>
+__device__ __forceinline__ V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R>
+__device__ __forceinline__ R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R>
+__device__ __forceinline__ V3<R> cross(V3<R> a, V3<R> b)
 {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ double norm(D3 a) { return sqrt(dot(a, a)); }
-__device__ __forceinline__ void st3(double* p, D3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
+template <class R>
+__device__ __forceinline__ R norm(V3<R> a) { return sqrt(dot(a, a)); }
+template <class R>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using T = double2;
+};
+template <>
+struct Vec2<float> {
+    using T = float2;
+};
+template <class R>
+__device__ __forceinline__ void st3(R* p, V3<R> a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
 
 struct Grid {
     int n[3];
@@ -234,13 +265,14 @@ __global__ void __launch_bounds__(256) k_bin_prep(ListReset lr) { list_reset(lr,
 // reference's summation order.
 constexpr int kNearKey = 1 << 30;
 
+template <class R>
 struct NearView {
     bl_boundary b;
     int on = 0;                      // a boundary is set
     unsigned char* near = nullptr;   // per vertex: near-wall
-    double* blend = nullptr;         // per vertex: distance - half support
+    R* blend = nullptr;              // per vertex: distance - half support
     int* base_t = nullptr;           // per vertex: top-hat stencil base (3)
-    double* W_t = nullptr;           // per vertex: top-hat weights (2 per axis)
+    R* W_t = nullptr;                // per vertex: top-hat weights (2 per axis)
 };
 
 // The first pair that reaches a tile (outside a workgroup's table) lists it
@@ -292,12 +324,13 @@ __device__ __forceinline__ void fill_tiles(const StencilTiles& st, const Grid& g
 }
 
 // Support-box base (wrapped on periodic axes) and the S weights of one axis.
-__device__ __forceinline__ int axis_stencil(int kernel, int S, double h, double x, int n, int per, double* w)
+template <class R>
+__device__ __forceinline__ int axis_stencil(int kernel, int S, R h, R x, int n, int per, R* w)
 {
     const int raw = (int)ceil(x - h);
 #pragma unroll 1
     for (int k = 0; k < 4; ++k)
-        if (k < S) w[k] = dirac(kernel, (double)(raw + k) - x);
+        if (k < S) w[k] = dirac<R>(kernel, (R)(raw + k) - x);
     int bw = raw;
     if (per) {
         bw = raw % n;
@@ -315,31 +348,33 @@ __device__ __forceinline__ int axis_stencil(int kernel, int S, double h, double 
 // workgroup's run, and the slots inside the run come from LDS.  The order
 // within a list is whatever the atomics give; spreading sorts.  The add that
 // takes a tile past kSmallTile entries appends it to the big-tile list.
+template <class R>
 struct BinArgs {
     Grid gd;
     int kernel, S;
     double h;
     long NV;
-    const double* pos;
+    const R* pos;
     int* base;
-    double* W;
+    R* W;
     int *count, *list;
     int cap;
     int *act, *na;
     unsigned char* flags;
     int *status, *pairs, *big, *nbig;
-    NearView nw;
+    NearView<R> nw;
 };
 
-__device__ __forceinline__ void bin_body(long bx, const BinArgs& A)
+template <class R>
+__device__ __forceinline__ void bin_body(long bx, const BinArgs<R>& A)
 {
     const Grid& gd = A.gd;
     const int kernel = A.kernel, S = A.S, cap = A.cap;
-    const double h = A.h;
+    const R h = (R)A.h;
     const long NV = A.NV;
-    const double* __restrict__ pos = A.pos;
+    const R* __restrict__ pos = A.pos;
     int* __restrict__ base = A.base;
-    double* __restrict__ W = A.W;
+    R* __restrict__ W = A.W;
     int* __restrict__ count = A.count;
     int* __restrict__ list = A.list;
     int* __restrict__ act = A.act;
@@ -349,7 +384,7 @@ __device__ __forceinline__ void bin_body(long bx, const BinArgs& A)
     int* __restrict__ pairs = A.pairs;
     int* __restrict__ big = A.big;
     int* __restrict__ nbig = A.nbig;
-    const NearView& nw = A.nw;
+    const NearView<R>& nw = A.nw;
     __shared__ int keys[kHash];
     __shared__ int cnt[kHash];  // pairs per slot, then the slot's first index in the tile's list
     __shared__ int rank[kHash];
@@ -370,41 +405,42 @@ __device__ __forceinline__ void bin_body(long bx, const BinArgs& A)
     // absurdly far off touches no tile: zero weights at base 0, reported
     bool off = false;
     if (v < NV)
-        for (int a = 0; a < 3; ++a) off = off || !(fabs(pos[3 * v + a]) < 1e9);
+        for (int a = 0; a < 3; ++a) off = off || !(fabs(pos[3 * v + a]) < R(1e9));
     if (v < NV && off) {
         for (int a = 0; a < 3; ++a) {
             base[3 * v + a] = 0;
-            for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = 0.0;
+            for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = R(0.0);
         }
         if (nw.on) {
             nw.near[v] = 0;
-            nw.blend[v] = 1.0;
+            nw.blend[v] = R(1.0);
         }
         atomicOr(&status[2], 4);
     }
     if (v < NV && !off) {
         int b[3];
         for (int a = 0; a < 3; ++a) {
-            double w[4] = {0.0, 0.0, 0.0, 0.0};
-            b[a] = axis_stencil(kernel, S, h, pos[3 * v + a], gd.n[a], gd.per[a], w);
+            R w[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+            b[a] = axis_stencil<R>(kernel, S, h, pos[3 * v + a], gd.n[a], gd.per[a], w);
             base[3 * v + a] = b[a];
             for (int k = 0; k < 4; ++k) W[12 * v + 4 * a + k] = w[k];
         }
         st.init(gd, S, b);
         e0 = (int)v;
         if (nw.on) {
-            const double p[3] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]};
-            const bool nr = bnd_less_than(nw.b, p, h + 1.);
-            const double wb = bnd_distance(nw.b, p) - h;
+            // the wall distance in FP64 (the boundary's own arithmetic)
+            const double p[3] = {(double)pos[3 * v], (double)pos[3 * v + 1], (double)pos[3 * v + 2]};
+            const bool nr = bnd_less_than(nw.b, p, A.h + 1.);
+            const R wb = (R)(bnd_distance(nw.b, p) - A.h);
             nw.near[v] = nr;
             nw.blend[v] = wb;
             if (nr) {
-                e0 = wb > 0 ? (kNearKey | (int)(2 * v)) : -1;
-                if (wb < 1) {
+                e0 = wb > R(0) ? (kNearKey | (int)(2 * v)) : -1;
+                if (wb < R(1)) {
                     int bt[3];
                     for (int a = 0; a < 3; ++a) {
-                        double w[4];
-                        bt[a] = axis_stencil(BL_KERNEL_TOPHAT, 2, 1.0, p[a], gd.n[a], gd.per[a], w);
+                        R w[4];
+                        bt[a] = axis_stencil<R>(BL_KERNEL_TOPHAT, 2, R(1.0), pos[3 * v + a], gd.n[a], gd.per[a], w);
                         nw.base_t[3 * v + a] = bt[a];
                         nw.W_t[6 * v + 2 * a] = w[0];
                         nw.W_t[6 * v + 2 * a + 1] = w[1];
@@ -454,7 +490,8 @@ __device__ __forceinline__ void bin_body(long bx, const BinArgs& A)
     }
 }
 
-__global__ void __launch_bounds__(256) k_bin(BinArgs a) { bin_body(blockIdx.x, a); }
+template <class R>
+__global__ void __launch_bounds__(256) k_bin(BinArgs<R> a) { bin_body<R>(blockIdx.x, a); }
 
 __global__ void k_clear_bins(int* count, long n)
 {
@@ -476,6 +513,7 @@ __global__ void k_fill_i32(int* p, long n, int v)
 
 
 // ------------------------------------------------------------ membrane
+template <class R>
 struct ForceView {
     int np;
     const long *p_voff, *p_toff, *p_loff, *p_poff;
@@ -491,7 +529,7 @@ struct ForceView {
     const double* s_pair_cs0;
     const int *s_vt_head, *s_vt, *s_vl_head, *s_vl, *s_vp_head, *s_vp;
     const double* s_area0;
-    const double* pos;
+    const R* pos;
     const int *v_part, *t_part, *r_part, *l_part;  // particle of each vertex / triangle / pair / link
     const int4* v_inc;  // per vertex: {link list start, links, triangle list start, triangles},
                         // {pair list start, pairs, particle, 0}; counts of unused kinds are 0
@@ -500,69 +538,75 @@ struct ForceView {
 // Skalak / NeoHookean in-plane forces of one triangle
 // (src/fsi/DeformableCapsule.hh:60-150), with the reference's
 // c = l' sin(phi) / (l' sin(phi0)) when `bug` is set.
-__device__ void capsule_triangle(D3 p0, D3 p1, D3 p2, const double* geo0, double area0, double G, double Cc,
-                                 bool skalak, bool bug, int only, D3 out[3])
+template <class R>
+__device__ void capsule_triangle(V3<R> p0, V3<R> p1, V3<R> p2, const double* geo0, double area0_, double G_,
+                                 double Cc_, bool skalak, bool bug, int only, V3<R> out[3])
 {
-    const D3 e0 = p1 - p0, e1 = p2 - p0;
-    const double l = norm(e0), lp = norm(e1);
-    const double l0 = geo0[0], l0p = geo0[1], cos_phi0 = geo0[2], sin_phi0 = geo0[3];
-    const double e0e1 = dot(e0, e1);
-    const double sin_phi = norm(cross(e0, e1)) / (l * lp);
-    const double cos_phi = e0e1 / (l * lp);
-    const double a = l / l0;
-    const double b = (lp * cos_phi / l0p - l * cos_phi0 / l0) / sin_phi0;
-    const double c = bug ? lp * sin_phi / (lp * sin_phi0) : lp * sin_phi / (l0p * sin_phi0);
-    const double D11 = (a * a - 1.) / 2.;
-    const double D22 = (b * b + c * c - 1) / 2.;
-    const double D12 = a * b / 2.;
-    const double I1 = 2. * (D11 + D22);
-    const double I2 = I1 + 4 * (D11 * D22 - D12 * D12);
-    D3 gl[3], glp[3], gc[3];
+    using V = V3<R>;
+    const R area0 = (R)area0_, G = (R)G_, Cc = (R)Cc_;
+    const V e0 = p1 - p0, e1 = p2 - p0;
+    const R l = norm(e0), lp = norm(e1);
+    const R l0 = (R)geo0[0], l0p = (R)geo0[1], cos_phi0 = (R)geo0[2], sin_phi0 = (R)geo0[3];
+    const R e0e1 = dot(e0, e1);
+    const R sin_phi = norm(cross(e0, e1)) / (l * lp);
+    const R cos_phi = e0e1 / (l * lp);
+    const R a = l / l0;
+    const R b = (lp * cos_phi / l0p - l * cos_phi0 / l0) / sin_phi0;
+    const R c = bug ? lp * sin_phi / (lp * sin_phi0) : lp * sin_phi / (l0p * sin_phi0);
+    const R D11 = (a * a - R(1.)) / R(2.);
+    const R D22 = (b * b + c * c - R(1)) / R(2.);
+    const R D12 = a * b / R(2.);
+    const R I1 = R(2.) * (D11 + D22);
+    const R I2 = I1 + R(4) * (D11 * D22 - D12 * D12);
+    V gl[3], glp[3], gc[3];
     gl[0] = e0 / (-l);
     gl[1] = e0 / l;
-    gl[2] = {0, 0, 0};
+    gl[2] = {R(0), R(0), R(0)};
     glp[0] = e1 / (-lp);
-    glp[1] = {0, 0, 0};
+    glp[1] = {R(0), R(0), R(0)};
     glp[2] = e1 / lp;
-    gc[0] = (-1. / (l * lp) + e0e1 / (l * l * l * lp)) * e0 + (-1. / (l * lp) + e0e1 / (l * lp * lp * lp)) * e1;
-    gc[1] = (1. / (l * lp)) * e1 - (e0e1 / (l * l * l * lp)) * e0;
-    gc[2] = (1. / (l * lp)) * e0 - (e0e1 / (l * lp * lp * lp)) * e1;
-    const double f1 = l / (l0 * l0);
-    const double f2a = lp / (l0p * l0p) - l * cos_phi * cos_phi0 / (l0 * l0p);
-    const double f2b = l * (cos_phi0 * cos_phi0) / (l0 * l0) - lp * cos_phi * cos_phi0 / (l0p * l0);
-    const double f2c = l * lp * cos_phi0 / (l0 * l0p);
-    const double f2 = 1. / (sin_phi0 * sin_phi0);
-    const double f3 = 1. / (2. * l0p * sin_phi0);
-    const double f3a = lp * cos_phi / l0p - 2. * l * cos_phi0 / l0;
-    const double f3b = l * cos_phi / l0p;
-    const double f3c = l * lp / l0p;
+    gc[0] = (-R(1.) / (l * lp) + e0e1 / (l * l * l * lp)) * e0 + (-R(1.) / (l * lp) + e0e1 / (l * lp * lp * lp)) * e1;
+    gc[1] = (R(1.) / (l * lp)) * e1 - (e0e1 / (l * l * l * lp)) * e0;
+    gc[2] = (R(1.) / (l * lp)) * e0 - (e0e1 / (l * lp * lp * lp)) * e1;
+    const R f1 = l / (l0 * l0);
+    const R f2a = lp / (l0p * l0p) - l * cos_phi * cos_phi0 / (l0 * l0p);
+    const R f2b = l * (cos_phi0 * cos_phi0) / (l0 * l0) - lp * cos_phi * cos_phi0 / (l0p * l0);
+    const R f2c = l * lp * cos_phi0 / (l0 * l0p);
+    const R f2 = R(1.) / (sin_phi0 * sin_phi0);
+    const R f3 = R(1.) / (R(2.) * l0p * sin_phi0);
+    const R f3a = lp * cos_phi / l0p - R(2.) * l * cos_phi0 / l0;
+    const R f3b = l * cos_phi / l0p;
+    const R f3c = l * lp / l0p;
     for (int i = 0; i < 3; ++i) {
         if (only >= 0 && i != only) continue;
-        const D3 gD11 = f1 * gl[i];
-        const D3 gD22 = f2 * (f2a * glp[i] + f2b * gl[i] - f2c * gc[i]);
-        const D3 gD12 = f3 * (f3a * gl[i] + f3b * glp[i] + f3c * gc[i]);
-        const D3 gI1 = 2. * (gD11 + gD22);
-        const D3 gI2 = (2. + 4. * D22) * gD11 + (2. + 4. * D11) * gD22 - (8. * D12) * gD12;
+        const V gD11 = f1 * gl[i];
+        const V gD22 = f2 * (f2a * glp[i] + f2b * gl[i] - f2c * gc[i]);
+        const V gD12 = f3 * (f3a * gl[i] + f3b * glp[i] + f3c * gc[i]);
+        const V gI1 = R(2.) * (gD11 + gD22);
+        const V gI2 = (R(2.) + R(4.) * D22) * gD11 + (R(2.) + R(4.) * D11) * gD22 - (R(8.) * D12) * gD12;
         if (skalak)
-            out[i] = (-G / 2. * area0) * ((I1 + 1) * gI1 + (Cc * I2 - 1) * gI2);
+            out[i] = (-G / R(2.) * area0) * ((I1 + R(1)) * gI1 + (Cc * I2 - R(1)) * gI2);
         else
-            out[i] = (-G / 2. * area0) * (gI1 - (1. / ((1 + I2) * (1 + I2))) * gI2);
+            out[i] = (-G / R(2.) * area0) * (gI1 - (R(1.) / ((R(1) + I2) * (R(1) + I2))) * gI2);
     }
 }
 
 // Area (local + global) and volume constraint forces of one triangle
 // (src/fsi/RBCParticle.hh:176-209, src/fsi/TriangleUtils.h:60-129).
-__device__ void area_volume_triangle(D3 v0, D3 v1, D3 v2, double area0, double A0tot, double A, double V,
-                                     double k_ag, double k_al, double k_v, double V0, D3 out[3])
+template <class R>
+__device__ void area_volume_triangle(V3<R> v0, V3<R> v1, V3<R> v2, double area0_, double A0tot_, double A_, double V_,
+                                     double k_ag_, double k_al_, double k_v_, double V0_, V3<R> out[3])
 {
-    const double v01v11 = v0.y - v1.y, v02v22 = v0.z - v2.z, v02v12 = v0.z - v1.z;
-    const double v01v21 = v0.y - v2.y, v00v20 = v0.x - v2.x, v00v10 = v0.x - v1.x;
-    const double a = v01v11 * v02v22 - v02v12 * v01v21;
-    const double b = v02v12 * v00v20 - (v0.x - v1.x) * v02v22;
-    const double c = v00v10 * v01v21 - v01v11 * v00v20;
-    const double la = 0.5 * sqrt(a * a + b * b + c * c);
-    const double denom = 1. / (4. * la);
-    D3 g0, g1, g2;
+    const R area0 = (R)area0_, A0tot = (R)A0tot_, A = (R)A_, V = (R)V_, k_ag = (R)k_ag_, k_al = (R)k_al_,
+            k_v = (R)k_v_, V0 = (R)V0_;
+    const R v01v11 = v0.y - v1.y, v02v22 = v0.z - v2.z, v02v12 = v0.z - v1.z;
+    const R v01v21 = v0.y - v2.y, v00v20 = v0.x - v2.x, v00v10 = v0.x - v1.x;
+    const R a = v01v11 * v02v22 - v02v12 * v01v21;
+    const R b = v02v12 * v00v20 - (v0.x - v1.x) * v02v22;
+    const R c = v00v10 * v01v21 - v01v11 * v00v20;
+    const R la = R(0.5) * sqrt(a * a + b * b + c * c);
+    const R denom = R(1.) / (R(4.) * la);
+    V3<R> g0, g1, g2;
     g0.x = (b * (v2.z - v1.z) + c * (v1.y - v2.y)) * denom;
     g0.y = (a * (v1.z - v2.z) + c * (v2.x - v1.x)) * denom;
     g0.z = (a * (v2.y - v1.y) + b * (v1.x - v2.x)) * denom;
@@ -572,74 +616,82 @@ __device__ void area_volume_triangle(D3 v0, D3 v1, D3 v2, double area0, double A
     g2.x = (b * (v1.z - v0.z) + c * v01v11) * denom;
     g2.y = (a * (v02v12) + c * (v1.x - v0.x)) * denom;
     g2.z = (a * (v1.y - v0.y) + b * v00v10) * denom;
-    double df_area = 0.;
+    R df_area = R(0.);
     df_area += k_ag * (A0tot - A) / A0tot;
     df_area += k_al * (area0 - la) / area0;
-    const double df_vol = k_v * (V0 - V) / V0;
+    const R df_vol = k_v * (V0 - V) / V0;
     out[0] = out[0] + df_area * g0;
     out[1] = out[1] + df_area * g1;
     out[2] = out[2] + df_area * g2;
-    out[0] = out[0] + df_vol * (cross(v1, v2) / 6.);
-    out[1] = out[1] + df_vol * (cross(v2, v0) / 6.);
-    out[2] = out[2] + df_vol * (cross(v0, v1) / 6.);
+    out[0] = out[0] + df_vol * (cross(v1, v2) / R(6.));
+    out[1] = out[1] + df_vol * (cross(v2, v0) / R(6.));
+    out[2] = out[2] + df_vol * (cross(v0, v1) / R(6.));
 }
 
 // Force of one link on its first vertex (the second gets the negative):
 // worm-like chain + power-law repulsion (src/fsi/RBCParticle.hh:149-171).
-__device__ D3 link_force(const ForceView& f, int s, long vo, const double* P, long link)
+// The link's coefficients depend on the rest length and the modulus only
+// and are formed in FP64.
+template <class R>
+__device__ V3<R> link_force(const ForceView<R>& f, int s, long vo, const double* P, long link)
 {
     const double m = 2;
     const double x0 = 1. / 2.2;
     const double ks_coeff1 = 0.25 * (6. * x0 + (3 * x0 * x0 - 2 * x0 * x0 * x0) / ((1 - x0) * (1 - x0)));
     const double ks_coeff2 = x0 / (2. * (1 - x0) * (1 - x0) * (1 - x0)) + 0.25 * (1. - 1. / ((1 - x0) * (1 - x0)));
     const long sl = f.s_loff[s] + link;
-    const double l0 = f.s_link_len[sl];
-    const double kp_coeff1 = -1. / pow(l0, m);
-    const double kp_coeff2 = (m + 1) / pow(l0, m);
-    const double G_lhs = 4. * l0 * P[0] / sqrt(3.0);
-    const double ks = -kp_coeff1 * G_lhs / (ks_coeff1 * kp_coeff2 - ks_coeff2 * kp_coeff1);
-    const double kp = ks_coeff1 * G_lhs / (ks_coeff1 * kp_coeff2 - ks_coeff2 * kp_coeff1);
+    const double l0d = f.s_link_len[sl];
+    const double kp_coeff1 = -1. / pow(l0d, m);
+    const double kp_coeff2 = (m + 1) / pow(l0d, m);
+    const double G_lhs = 4. * l0d * P[0] / sqrt(3.0);
+    const R ks = (R)(-kp_coeff1 * G_lhs / (ks_coeff1 * kp_coeff2 - ks_coeff2 * kp_coeff1));
+    const R kp = (R)(ks_coeff1 * G_lhs / (ks_coeff1 * kp_coeff2 - ks_coeff2 * kp_coeff1));
+    const R l0 = (R)l0d;
     const int a = f.s_link[2 * sl], b = f.s_link[2 * sl + 1];
-    const D3 dv = ld3(f.pos + 3 * (vo + a)) - ld3(f.pos + 3 * (vo + b));
-    const double L = norm(dv);
-    const double x = L / (l0 * 2.2);
-    const double df = -0.25 * ks * (6. * x + (3. * x * x - 2. * x * x * x) / ((1 - x) * (1 - x))) + kp / pow(L, m);
+    const V3<R> dv = ld3(f.pos + 3 * (vo + a)) - ld3(f.pos + 3 * (vo + b));
+    const R L = norm(dv);
+    const R x = L / (l0 * R(2.2));
+    const R df = -R(0.25) * ks * (R(6.) * x + (R(3.) * x * x - R(2.) * x * x * x) / ((R(1) - x) * (R(1) - x))) +
+                 kp / pow(L, (R)m);
     return (df / L) * dv;
 }
 
 // Forces of one triangle on its three vertices.
-__device__ void tri_forces(const ForceView& f, int p, int s, int model, long vo, const double* P, long tri,
-                           D3 out[3])
+template <class R>
+__device__ void tri_forces(const ForceView<R>& f, int p, int s, int model, long vo, const double* P, long tri,
+                           V3<R> out[3])
 {
     const long st = f.s_toff[s] + tri;
     const int i0 = f.s_tri[3 * st], i1 = f.s_tri[3 * st + 1], i2 = f.s_tri[3 * st + 2];
-    const D3 v0 = ld3(f.pos + 3 * (vo + i0)), v1 = ld3(f.pos + 3 * (vo + i1)), v2 = ld3(f.pos + 3 * (vo + i2));
-    out[0] = out[1] = out[2] = D3{0, 0, 0};
+    const V3<R> v0 = ld3(f.pos + 3 * (vo + i0)), v1 = ld3(f.pos + 3 * (vo + i1)), v2 = ld3(f.pos + 3 * (vo + i2));
+    out[0] = out[1] = out[2] = V3<R>{R(0), R(0), R(0)};
     const double area0 = f.s_tri_area0[st];
     if (model == BL_MODEL_CAPSULE_SKALAK || model == BL_MODEL_CAPSULE_NEOHOOKEAN || model == 3)
-        capsule_triangle(v0, v1, v2, f.s_tri_geo0 + 4 * st, area0, P[0], P[1], model != BL_MODEL_CAPSULE_NEOHOOKEAN,
-                         f.p_bug[p] != 0, -1, out);
+        capsule_triangle<R>(v0, v1, v2, f.s_tri_geo0 + 4 * st, area0, P[0], P[1],
+                            model != BL_MODEL_CAPSULE_NEOHOOKEAN, f.p_bug[p] != 0, -1, out);
     if (model == BL_MODEL_RBC || model == 3)
-        area_volume_triangle(v0, v1, v2, area0, f.s_area0[s], f.p_geom[11 * p], f.p_geom[11 * p + 1], P[2], P[3],
-                             P[4], P[5], out);
+        area_volume_triangle<R>(v0, v1, v2, area0, f.s_area0[s], f.p_geom[11 * p], f.p_geom[11 * p + 1], P[2], P[3],
+                                P[4], P[5], out);
 }
 
 // Bending forces of one dihedral pair on its four vertices.
-__device__ void pair_forces(const ForceView& f, int s, long vo, const double* P, long pair, D3 out[4])
+template <class R>
+__device__ void pair_forces(const ForceView<R>& f, int s, long vo, const double* P, long pair, V3<R> out[4])
 {
+    using V = V3<R>;
     const long sp = f.s_poff[s] + pair;
     const int* q = f.s_pair + 4 * sp;
-    const D3 v0 = ld3(f.pos + 3 * (vo + q[0])), v1 = ld3(f.pos + 3 * (vo + q[1]));
-    const D3 v2 = ld3(f.pos + 3 * (vo + q[2])), v3 = ld3(f.pos + 3 * (vo + q[3]));
-    const D3 e0 = v1 - v0, e1 = v2 - v0, e2 = v3 - v0;
-    const double e1n = norm(e1);
-    const D3 e1u = e1 / e1n;
-    const D3 n1 = cross(e0, e1), n2 = cross(e1, e2);
-    const double n1n = norm(n1), n2n = norm(n2);
-    const D3 n1u = n1 / n1n, n2u = n2 / n2n;
-    const double cs = dot(n1u, n2u);
-    const double sn = ((dot(n1 - n2, v1 - v3) > 0) ? 1.0 : -1.0) * norm(cross(n1u, n2u));
-    const double pref = P[6] * (-f.s_pair_cs0[2 * sp] * sn + f.s_pair_cs0[2 * sp + 1] * cs);
+    const V v0 = ld3(f.pos + 3 * (vo + q[0])), v1 = ld3(f.pos + 3 * (vo + q[1]));
+    const V v2 = ld3(f.pos + 3 * (vo + q[2])), v3 = ld3(f.pos + 3 * (vo + q[3]));
+    const V e0 = v1 - v0, e1 = v2 - v0, e2 = v3 - v0;
+    const R e1n = norm(e1);
+    const V e1u = e1 / e1n;
+    const V n1 = cross(e0, e1), n2 = cross(e1, e2);
+    const R n1n = norm(n1), n2n = norm(n2);
+    const V n1u = n1 / n1n, n2u = n2 / n2n;
+    const R cs = dot(n1u, n2u);
+    const R sn = ((dot(n1 - n2, v1 - v3) > R(0)) ? R(1.0) : -R(1.0)) * norm(cross(n1u, n2u));
+    const R pref = (R)P[6] * (-(R)f.s_pair_cs0[2 * sp] * sn + (R)f.s_pair_cs0[2 * sp + 1] * cs);
     out[0] = pref * ((dot(e1u, e1 - e2) / n2n) * n2u + (dot(e1u, e1 - e0) / n1n) * n1u);
     out[1] = pref * ((-e1n / n1n) * n1u);
     out[2] = pref * ((dot(e1u, e2) / n2n) * n2u + (dot(e1u, e0) / n1n) * n1u);
@@ -648,42 +700,45 @@ __device__ void pair_forces(const ForceView& f, int s, long vo, const double* P,
 
 // One thread per element, over the concatenation [triangles | pairs | links]
 // of all particles: the element's forces on its vertices go to its own slots
-// of `ec` (9 doubles per triangle, 12 per pair, 3 per link).  Elements that
+// of `ec` (9 values per triangle, 12 per pair, 3 per link).  Elements that
 // the particle's model does not use are skipped.
+template <class R>
 struct ElemArgs {
-    ForceView f;
+    ForceView<R> f;
     long NT, NP, NL, ec_pair, ec_link;
-    double* ec;
+    R* ec;
 };
 
-__device__ __forceinline__ void elem_body(long bx, const ElemArgs& A)
+template <class R>
+__device__ __forceinline__ void elem_body(long bx, const ElemArgs<R>& A)
 {
-    const ForceView& f = A.f;
+    const ForceView<R>& f = A.f;
     const long NT = A.NT, NP = A.NP, NL = A.NL, ec_pair = A.ec_pair, ec_link = A.ec_link;
-    double* __restrict__ ec = A.ec;
+    R* __restrict__ ec = A.ec;
     const long e = bx * blockDim.x + threadIdx.x;
     if (e < NT) {
         const int p = f.t_part[e];
-        D3 out[3];
-        tri_forces(f, p, f.p_shape[p], f.p_model[p], f.p_voff[p], f.p_params + 8 * p, e - f.p_toff[p], out);
+        V3<R> out[3];
+        tri_forces<R>(f, p, f.p_shape[p], f.p_model[p], f.p_voff[p], f.p_params + 8 * p, e - f.p_toff[p], out);
         for (int k = 0; k < 3; ++k) st3(ec + 9 * e + 3 * k, out[k]);
     } else if (e < NT + NP) {
         const long g = e - NT;
         const int p = f.r_part[g];
         const int model = f.p_model[p];
         if (model != BL_MODEL_RBC && model != 3) return;
-        D3 out[4];
-        pair_forces(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_poff[p], out);
+        V3<R> out[4];
+        pair_forces<R>(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_poff[p], out);
         for (int k = 0; k < 4; ++k) st3(ec + ec_pair + 12 * g + 3 * k, out[k]);
     } else if (e < NT + NP + NL) {
         const long g = e - NT - NP;
         const int p = f.l_part[g];
         if (f.p_model[p] != BL_MODEL_RBC) return;
-        st3(ec + ec_link + 3 * g, link_force(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_loff[p]));
+        st3(ec + ec_link + 3 * g, link_force<R>(f, f.p_shape[p], f.p_voff[p], f.p_params + 8 * p, g - f.p_loff[p]));
     }
 }
 
-__global__ void __launch_bounds__(256) k_elem_forces(ElemArgs a) { elem_body(blockIdx.x, a); }
+template <class R>
+__global__ void __launch_bounds__(256) k_elem_forces(ElemArgs<R> a) { elem_body<R>(blockIdx.x, a); }
 
 // Force on every vertex: the contributions of its incident elements --
 // links (src/fsi/RBCParticle.hh:149-171), triangles
@@ -699,34 +754,38 @@ __global__ void __launch_bounds__(256) k_elem_forces(ElemArgs a) { elem_body(blo
 // (src/fsi/ImmersedBoundaryDynamics_forces.hh:293-301).
 constexpr int kGatherLanes = 4;
 
+template <class R>
 struct GatherArgs {
-    ForceView f;
+    ForceView<R> f;
     long NV, ec_pair, ec_link;
-    const double* ec;
-    double* force;
+    const R* ec;
+    R* force;
     DecoSet deco;
-    const double* fpp;
-    const double* ext;  // host-given vertex forces, added after the decorators (or null)
+    const R* fpp;
+    const R* ext;  // host-given vertex forces, added after the decorators (or null)
 };
 
-__device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
+template <class R>
+__device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
 {
-    const ForceView& f = A.f;
+    const ForceView<R>& f = A.f;
     const long NV = A.NV, ec_pair = A.ec_pair, ec_link = A.ec_link;
-    const double* __restrict__ ec = A.ec;
-    double* __restrict__ force = A.force;
+    const R* __restrict__ ec = A.ec;
+    R* __restrict__ force = A.force;
     const DecoSet& deco = A.deco;
-    const double* __restrict__ fpp = A.fpp;
+    const R* __restrict__ fpp = A.fpp;
     const long gid = bx * blockDim.x + threadIdx.x;
     const long v = gid / kGatherLanes;
     const int lane = (int)(gid % kGatherLanes);
-    double c0 = 0, c1 = 0, c2 = 0;
+    R c0 = R(0), c1 = R(0), c2 = R(0);
     if (deco.n && lane == 0 && v < NV) {
+        // the decorators' potentials and wall geometry in FP64
+        const double p[3] = {(double)f.pos[3 * v], (double)f.pos[3 * v + 1], (double)f.pos[3 * v + 2]};
         double w[3] = {0, 0, 0};
-        deco_apply(deco, f.pos + 3 * v, w);
-        c0 = w[0];
-        c1 = w[1];
-        c2 = w[2];
+        deco_apply(deco, p, w);
+        c0 = (R)w[0];
+        c1 = (R)w[1];
+        c2 = (R)w[2];
     }
     if (A.ext && lane == 0 && v < NV) {
         c0 += A.ext[3 * v];
@@ -742,26 +801,26 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
         const int* Ll = f.s_vl + ia.x;
         const int* Lt = f.s_vt + ia.z;
         const int* Lp = f.s_vp + ib.x;
-        const double* El = ec + ec_link + 3 * f.p_loff[p];
-        const double* Et = ec + 9 * f.p_toff[p];
-        const double* Ep = ec + ec_pair + 12 * f.p_poff[p];
+        const R* El = ec + ec_link + 3 * f.p_loff[p];
+        const R* Et = ec + 9 * f.p_toff[p];
+        const R* Ep = ec + ec_pair + 12 * f.p_poff[p];
         for (int e = lane; e < nl + nt + npr; e += kGatherLanes) {
             if (e < nl) {
                 const int code = Ll[e];
-                const double* x = El + 3 * (code >> 2);
-                const double sg = (code & 3) == 0 ? 1.0 : -1.0;
+                const R* x = El + 3 * (code >> 2);
+                const R sg = (code & 3) == 0 ? R(1.0) : -R(1.0);
                 c0 += sg * x[0];
                 c1 += sg * x[1];
                 c2 += sg * x[2];
             } else if (e < nl + nt) {
                 const int code = Lt[e - nl];
-                const double* x = Et + 9 * (code >> 2) + 3 * (code & 3);
+                const R* x = Et + 9 * (code >> 2) + 3 * (code & 3);
                 c0 += x[0];
                 c1 += x[1];
                 c2 += x[2];
             } else {
                 const int code = Lp[e - nl - nt];
-                const double* x = Ep + 12 * (code >> 2) + 3 * (code & 3);
+                const R* x = Ep + 12 * (code >> 2) + 3 * (code & 3);
                 c0 += x[0];
                 c1 += x[1];
                 c2 += x[2];
@@ -786,7 +845,8 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs& A)
     }
 }
 
-__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs a) { gather_body(blockIdx.x, a); }
+template <class R>
+__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs<R> a) { gather_body<R>(blockIdx.x, a); }
 
 // ------------------------------------------------- velocity staging
 // The progs' computeVelocity pass (staged_velocity), restricted to the
@@ -839,52 +899,56 @@ __global__ void __launch_bounds__(256) k_stage_u(StageArgs a) { stage_body<S, SW
 // z-rows.  With MOVE the vertex is then advanced, x += U (explicit Euler,
 // src/fsi/DeformableParticle3D.hh:26-36).
 // Explicit Euler step that reflects off the boundary
-// (DeformableParticle3D::move_vertices, src/fsi/DeformableParticle3D.hh:26-50).
-__device__ __forceinline__ void move_reflect(const bl_boundary& b, double* p, const double* u)
+// (DeformableParticle3D::move_vertices, src/fsi/DeformableParticle3D.hh:26-50),
+// in FP64 (the boundary's own arithmetic) for either vertex type.
+template <class R>
+__device__ __forceinline__ void move_reflect(const bl_boundary& b, R* pr, const R* ur)
 {
 #pragma clang fp contract(off)
-    const double old[3] = {p[0], p[1], p[2]};
+    const double u[3] = {(double)ur[0], (double)ur[1], (double)ur[2]};
+    const double old[3] = {(double)pr[0], (double)pr[1], (double)pr[2]};
     const double nw[3] = {old[0] + u[0], old[1] + u[1], old[2] + u[2]};
     double t;
     if (!bnd_trace_ray(b, old, nw, t)) {
-        p[0] = nw[0];
-        p[1] = nw[1];
-        p[2] = nw[2];
+        pr[0] = (R)nw[0];
+        pr[1] = (R)nw[1];
+        pr[2] = (R)nw[2];
         return;
     }
     double q[3] = {old[0] + t * u[0], old[1] + t * u[1], old[2] + t * u[2]};
     double n[3];
     bnd_normal(b, q, n);
     const double vn = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
-    for (int a = 0; a < 3; ++a) p[a] = q[a] + (u[a] - 2. * vn * n[a]) * (1. - t);
+    for (int a = 0; a < 3; ++a) pr[a] = (R)(q[a] + (u[a] - 2. * vn * n[a]) * (1. - t));
 }
 
 // A near-wall vertex (blend weight w < 1) also interpolates with the top
 // hat, lanes (j, k) < 2 over its 2x2x2 stencil, and takes (1 - w) u_tophat
 // + w u_kernel, or u_tophat alone for w <= 0
 // (src/fsi/ImmersedBoundaryDynamics_interpolation.hh:182-230).
+template <class R>
 struct InterpArgs {
     Grid gd;
     long NV;
     const int* base;
-    const double* W;
+    const R* W;
     const void* ut;  // Vel4<C>
-    double *vel, *pos;
+    R *vel, *pos;
     ListReset lr;
-    NearView nw;
+    NearView<R> nw;
 };
 
-template <class C, int SW, bool MOVE>
-__device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& A)
+template <class C, int SW, bool MOVE, class R>
+__device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs<R>& A)
 {
     const Grid& gd = A.gd;
     const long NV = A.NV;
     const int* __restrict__ base = A.base;
-    const double* __restrict__ W = A.W;
+    const R* __restrict__ W = A.W;
     const Vel4<C>* __restrict__ ut = (const Vel4<C>*)A.ut;
-    double* __restrict__ vel = A.vel;
-    double* __restrict__ pos = A.pos;
-    const NearView& nw = A.nw;
+    R* __restrict__ vel = A.vel;
+    R* __restrict__ pos = A.pos;
+    const NearView<R>& nw = A.nw;
     if (A.lr.act) list_reset(A.lr, (int)bx, nbx);  // the next binning's list, carried by this launch
     constexpr int LA = SW == 2 ? 2 : 4;
     constexpr int LPV = LA * LA;
@@ -892,23 +956,23 @@ __device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& 
     const long v = gid / LPV;
     const int r = (int)(gid % LPV);
     const int j = r / LA, k = r % LA;
-    double a0 = 0, a1 = 0, a2 = 0;
+    R a0 = R(0), a1 = R(0), a2 = R(0);
     if (v < NV && j < SW && k < SW) {
         const int yj = node_at(base[3 * v + 1], j, gd.n[1], gd.per[1]);
         const int zk = node_at(base[3 * v + 2], k, gd.n[2], gd.per[2]);
         if (yj >= 0 && zk >= 0) {
-            const double* Wv = W + 12 * v;
-            const double wy = Wv[4 + j], wz = Wv[8 + k];
+            const R* Wv = W + 12 * v;
+            const R wy = Wv[4 + j], wz = Wv[8 + k];
             const int bx = base[3 * v];
 #pragma unroll
             for (int i = 0; i < SW; ++i) {
                 const int xi = node_at(bx, i, gd.n[0], gd.per[0]);
                 if (xi < 0) continue;
-                const double w = (Wv[i] * wy) * wz;
+                const R w = (Wv[i] * wy) * wz;
                 const Vel4<C> u = ut[((size_t)xi * gd.n[1] + yj) * gd.n[2] + zk];
-                a0 += w * (double)u.x;
-                a1 += w * (double)u.y;
-                a2 += w * (double)u.z;
+                a0 += w * (R)u.x;
+                a1 += w * (R)u.y;
+                a2 += w * (R)u.z;
             }
         }
     }
@@ -918,24 +982,24 @@ __device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& 
         a1 += __shfl_xor(a1, o);
         a2 += __shfl_xor(a2, o);
     }
-    if (nw.on && v < NV && nw.near[v] && nw.blend[v] < 1) {  // the same for all lanes of the vertex
-        const double wb = nw.blend[v];
-        double t0 = 0, t1 = 0, t2 = 0;
+    if (nw.on && v < NV && nw.near[v] && nw.blend[v] < R(1)) {  // the same for all lanes of the vertex
+        const R wb = nw.blend[v];
+        R t0 = R(0), t1 = R(0), t2 = R(0);
         if (j < 2 && k < 2) {
             const int* bt = nw.base_t + 3 * v;
-            const double* Wt = nw.W_t + 6 * v;
+            const R* Wt = nw.W_t + 6 * v;
             const int yj = node_at(bt[1], j, gd.n[1], gd.per[1]);
             const int zk = node_at(bt[2], k, gd.n[2], gd.per[2]);
             if (yj >= 0 && zk >= 0) {
-                const double wy = Wt[2 + j], wz = Wt[4 + k];
+                const R wy = Wt[2 + j], wz = Wt[4 + k];
                 for (int i = 0; i < 2; ++i) {
                     const int xi = node_at(bt[0], i, gd.n[0], gd.per[0]);
                     if (xi < 0) continue;
-                    const double w = (Wt[i] * wy) * wz;
+                    const R w = (Wt[i] * wy) * wz;
                     const Vel4<C> u = ut[((size_t)xi * gd.n[1] + yj) * gd.n[2] + zk];
-                    t0 += w * (double)u.x;
-                    t1 += w * (double)u.y;
-                    t2 += w * (double)u.z;
+                    t0 += w * (R)u.x;
+                    t1 += w * (R)u.y;
+                    t2 += w * (R)u.z;
                 }
             }
         }
@@ -945,14 +1009,14 @@ __device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& 
             t1 += __shfl_xor(t1, o);
             t2 += __shfl_xor(t2, o);
         }
-        if (wb <= 0) {
+        if (wb <= R(0)) {
             a0 = t0;
             a1 = t1;
             a2 = t2;
         } else {
-            a0 = (1. - wb) * t0 + wb * a0;
-            a1 = (1. - wb) * t1 + wb * a1;
-            a2 = (1. - wb) * t2 + wb * a2;
+            a0 = (R(1.) - wb) * t0 + wb * a0;
+            a1 = (R(1.) - wb) * t1 + wb * a1;
+            a2 = (R(1.) - wb) * t2 + wb * a2;
         }
     }
     if (r == 0 && v < NV) {
@@ -961,8 +1025,8 @@ __device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& 
         vel[3 * v + 2] = a2;
         if (MOVE) {
             if (nw.on) {
-                const double u[3] = {a0, a1, a2};
-                move_reflect(nw.b, pos + 3 * v, u);
+                const R u[3] = {a0, a1, a2};
+                move_reflect<R>(nw.b, pos + 3 * v, u);
             } else {
                 pos[3 * v] += a0;
                 pos[3 * v + 1] += a1;
@@ -972,8 +1036,11 @@ __device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs& 
     }
 }
 
-template <class C, int SW, bool MOVE>
-__global__ void __launch_bounds__(256) k_interp(InterpArgs a) { interp_body<C, SW, MOVE>(blockIdx.x, gridDim.x, a); }
+template <class C, int SW, bool MOVE, class R>
+__global__ void __launch_bounds__(256) k_interp(InterpArgs<R> a)
+{
+    interp_body<C, SW, MOVE, R>(blockIdx.x, gridDim.x, a);
+}
 
 // ----------------------------------------------------------- spreading
 // Every cell of a touched 4x4x4 tile sums g + phi phi phi F_k over the
@@ -1015,33 +1082,34 @@ __device__ __forceinline__ void wave_sync()
 // load waits for the base.  A near-wall vertex's kernel part carries the
 // force times w (w < 1), its top-hat part the force times 1 - w (w > 0),
 // the top hat's two weights per axis.
+template <class R>
 __device__ __forceinline__ void entry_weights(const Grid& gd, int Sw, const int org[3], int e,
-                                              const int* __restrict__ base, const double* __restrict__ W,
-                                              const double* __restrict__ vforce, const NearView& nw,
-                                              double ev[15])
+                                              const int* __restrict__ base, const R* __restrict__ W,
+                                              const R* __restrict__ vforce, const NearView<R>& nw, R ev[15])
 {
+    using R2 = typename Vec2<R>::T;
     const bool nr = (e & kNearKey) != 0;
     const long v = nr ? (long)((e & ~kNearKey) >> 1) : (long)e;
     const bool top = nr && (e & 1);
-    double w[12];
+    R w[12];
     const int* bv;
     int sw;
     if (top) {
-        const double2* W2 = (const double2*)(nw.W_t + 6 * v);
+        const R2* W2 = (const R2*)(nw.W_t + 6 * v);
         for (int k = 0; k < 3; ++k) {
-            const double2 p = W2[k];
+            const R2 p = W2[k];
             w[4 * k] = p.x;
             w[4 * k + 1] = p.y;
-            w[4 * k + 2] = 0.0;
-            w[4 * k + 3] = 0.0;
+            w[4 * k + 2] = R(0.0);
+            w[4 * k + 3] = R(0.0);
         }
         bv = nw.base_t + 3 * v;
         sw = 2;
     } else {
-        const double2* W2 = (const double2*)(W + 12 * v);
+        const R2* W2 = (const R2*)(W + 12 * v);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const double2 p = W2[k];
+            const R2 p = W2[k];
             w[2 * k] = p.x;
             w[2 * k + 1] = p.y;
         }
@@ -1055,16 +1123,16 @@ __device__ __forceinline__ void entry_weights(const Grid& gd, int Sw, const int 
         for (int l = 0; l < 4; ++l) {
             int d = org[ax] + l - b;
             if (gd.per[ax] && d < 0) d += gd.n[ax];
-            const double* wa = w + 4 * ax;
-            const double x = d == 0 ? wa[0] : (d == 1 ? wa[1] : (d == 2 ? wa[2] : wa[3]));
-            ev[4 * ax + l] = (unsigned)d < (unsigned)sw ? x : 0.0;
+            const R* wa = w + 4 * ax;
+            const R x = d == 0 ? wa[0] : (d == 1 ? wa[1] : (d == 2 ? wa[2] : wa[3]));
+            ev[4 * ax + l] = (unsigned)d < (unsigned)sw ? x : R(0.0);
         }
     }
-    double F0 = vforce[3 * v], F1 = vforce[3 * v + 1], F2 = vforce[3 * v + 2];
+    R F0 = vforce[3 * v], F1 = vforce[3 * v + 1], F2 = vforce[3 * v + 2];
     if (nr) {
-        const double wb = nw.blend[v];
-        const bool scale = top ? wb > 0 : wb < 1;
-        const double s = top ? 1. - wb : wb;
+        const R wb = nw.blend[v];
+        const bool scale = top ? wb > R(0) : wb < R(1);
+        const R s = top ? R(1.) - wb : wb;
         if (scale) {
             F0 *= s;
             F1 *= s;
@@ -1079,11 +1147,12 @@ __device__ __forceinline__ void entry_weights(const Grid& gd, int Sw, const int 
 // Adds entries [0, cnt) of a staged half (cnt <= kChunk; the rows are zero
 // from cnt up to the next multiple of 4, so a padded entry adds nothing)
 // in entry order; the LDS reads of four entries are issued together.
-__device__ __forceinline__ void chunk_sum(const double (*ew)[kChunk + 1], int cnt, int lx, int ly, int lz,
-                                          double& a0, double& a1, double& a2)
+template <class R>
+__device__ __forceinline__ void chunk_sum(const R (*ew)[kChunk + 1], int cnt, int lx, int ly, int lz, R& a0, R& a1,
+                                          R& a2)
 {
     for (int e = 0; e < cnt; e += 4) {
-        double w[4], f0[4], f1[4], f2[4];
+        R w[4], f0[4], f1[4], f2[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             w[k] = (ew[lx][e + k] * ew[4 + ly][e + k]) * ew[8 + lz][e + k];
@@ -1102,18 +1171,18 @@ __device__ __forceinline__ void chunk_sum(const double (*ew)[kChunk + 1], int cn
 
 // Sums the sorted entries [lo, hi) of a tile into a0..a2, 64 at a time;
 // key(e) gives the e-th sorted entry.
-template <class KeyOf>
+template <class R, class KeyOf>
 __device__ __forceinline__ void sum_entries(const Grid& gd, int Sw, const int org[3], int lo, int hi, KeyOf key,
-                                            const int* __restrict__ base, const double* __restrict__ W,
-                                            const double* __restrict__ vforce, const NearView& nw,
-                                            double (*ew)[kChunk + 1], int lane, int lx, int ly, int lz,
-                                            double& a0, double& a1, double& a2)
+                                            const int* __restrict__ base, const R* __restrict__ W,
+                                            const R* __restrict__ vforce, const NearView<R>& nw,
+                                            R (*ew)[kChunk + 1], int lane, int lx, int ly, int lz, R& a0, R& a1,
+                                            R& a2)
 {
     for (int c0 = lo; c0 < hi; c0 += 64) {
         const int e = c0 + lane;
-        double ev[15];
+        R ev[15];
 #pragma unroll
-        for (int r = 0; r < 15; ++r) ev[r] = 0.0;
+        for (int r = 0; r < 15; ++r) ev[r] = R(0.0);
         if (e < hi) entry_weights(gd, Sw, org, key(e), base, W, vforce, nw, ev);
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
@@ -1137,9 +1206,9 @@ __device__ __forceinline__ void tile_origin(const Grid& gd, int t, int org[3])
     org[2] = tz * kTile;
 }
 
-template <class FT>
+template <class FT, class R>
 __device__ __forceinline__ void store_cell(const LatView& lv, const Grid& gd, const int org[3], int lx, int ly,
-                                           int lz, double a0, double a1, double a2, FT* __restrict__ Fout)
+                                           int lz, R a0, R a1, R a2, FT* __restrict__ Fout)
 {
     const int x = org[0] + lx, y = org[1] + ly, z = org[2] + lz;
     if (x < gd.n[0] && y < gd.n[1] && z < gd.n[2]) {
@@ -1175,16 +1244,16 @@ __device__ __forceinline__ int rank_in_run(const int* r, int key)
     return lo + (r[lo] < key ? 1 : 0);
 }
 
-template <class FT>
-__device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[15][kChunk + 1], const LatView& lv,
+template <class FT, class R>
+__device__ __forceinline__ void spread_small(int blk, int nb, R (*ew_all)[15][kChunk + 1], const LatView& lv,
                                              const Grid& gd, int Sw, const int* __restrict__ active,
                                              const int* __restrict__ nactive, int* __restrict__ count, int cap,
                                              const int* __restrict__ list, const int* __restrict__ base,
-                                             const double* __restrict__ W, const double* __restrict__ vforce,
-                                             const NearView& nw, FT* __restrict__ Fout)
+                                             const R* __restrict__ W, const R* __restrict__ vforce,
+                                             const NearView<R>& nw, FT* __restrict__ Fout)
 {
     const int wv = threadIdx.x >> 6;
-    double(*ew)[kChunk + 1] = ew_all[wv];
+    R(*ew)[kChunk + 1] = ew_all[wv];
     const int na = *nactive;
     const int lane = threadIdx.x & 63;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
@@ -1204,9 +1273,9 @@ __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[1
             const int key = wave_sort(lane < n ? raw : 0x7fffffff, lane);
             int org[3];
             tile_origin(gd, t, org);
-            double acc0 = lv.g[0], acc1 = lv.g[1], acc2 = lv.g[2];
-            sum_entries(gd, Sw, org, 0, n, [&](int) { return key; }, base, W, vforce, nw, ew, lane, lx, ly, lz, acc0,
-                        acc1, acc2);
+            R acc0 = (R)lv.g[0], acc1 = (R)lv.g[1], acc2 = (R)lv.g[2];
+            sum_entries<R>(gd, Sw, org, 0, n, [&](int) { return key; }, base, W, vforce, nw, ew, lane, lx, ly, lz,
+                           acc0, acc1, acc2);
             store_cell(lv, gd, org, lx, ly, lz, acc0, acc1, acc2, Fout);
             if (lane == 0) count[t] = 0;
         }
@@ -1215,17 +1284,17 @@ __device__ __forceinline__ void spread_small(int blk, int nb, double (*ew_all)[1
     }
 }
 
-template <class FT>
-__device__ __forceinline__ void spread_big(int blk, int nb, int* keys, int* runs, double (*ew_all)[15][kChunk + 1],
-                                           double (*part)[3][64], const LatView& lv, const Grid& gd, int Sw,
+template <class FT, class R>
+__device__ __forceinline__ void spread_big(int blk, int nb, int* keys, int* runs, R (*ew_all)[15][kChunk + 1],
+                                           R (*part)[3][64], const LatView& lv, const Grid& gd, int Sw,
                                            const int* __restrict__ big, const int* __restrict__ nbig,
                                            int* __restrict__ count, int cap, const int* __restrict__ list,
-                                           const int* __restrict__ base, const double* __restrict__ W,
-                                           const double* __restrict__ vforce, const NearView& nw,
+                                           const int* __restrict__ base, const R* __restrict__ W,
+                                           const R* __restrict__ vforce, const NearView<R>& nw,
                                            FT* __restrict__ Fout)
 {
     const int wv = threadIdx.x >> 6;
-    double(*ew)[kChunk + 1] = ew_all[wv];
+    R(*ew)[kChunk + 1] = ew_all[wv];
     const int nt = *nbig;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -1274,15 +1343,15 @@ __device__ __forceinline__ void spread_big(int blk, int nb, int* keys, int* runs
         // wave w sums the contiguous range [lo, hi) of the sorted entries
         const int q = (n + kSpreadWaves - 1) / kSpreadWaves;
         const int lo = min(n, wv * q), hi = min(n, lo + q);
-        double acc0 = 0, acc1 = 0, acc2 = 0;
-        sum_entries(gd, Sw, org, lo, hi, [&](int k) { return keys[k]; }, base, W, vforce, nw, ew, lane, lx, ly, lz,
-                    acc0, acc1, acc2);
+        R acc0 = R(0), acc1 = R(0), acc2 = R(0);
+        sum_entries<R>(gd, Sw, org, lo, hi, [&](int k) { return keys[k]; }, base, W, vforce, nw, ew, lane, lx, ly,
+                       lz, acc0, acc1, acc2);
         part[wv][0][lane] = acc0;
         part[wv][1][lane] = acc1;
         part[wv][2][lane] = acc2;
         __syncthreads();
         if (wv == 0) {
-            double s0 = lv.g[0], s1 = lv.g[1], s2 = lv.g[2];
+            R s0 = (R)lv.g[0], s1 = (R)lv.g[1], s2 = (R)lv.g[2];
             for (int w = 0; w < kSpreadWaves; ++w) {
                 s0 += part[w][0][lane];
                 s1 += part[w][1][lane];
@@ -1297,6 +1366,7 @@ __device__ __forceinline__ void spread_big(int blk, int nb, int* keys, int* runs
 
 // Both kinds in one launch: workgroups [0, n_small) take the small tiles,
 // the others the big ones.
+template <class R>
 struct SpreadArgs {
     LatView lv;
     Grid gd;
@@ -1305,31 +1375,31 @@ struct SpreadArgs {
     int* count;
     int cap;
     const int *list, *base;
-    const double *W, *vforce;
-    NearView nw;
+    const R *W, *vforce;
+    NearView<R> nw;
     void* Fout;  // force field in the storage's force type
 };
 
-template <class S>
-__device__ __forceinline__ void spread_body(int bx, int nbx, const SpreadArgs& A)
+template <class S, class R>
+__device__ __forceinline__ void spread_body(int bx, int nbx, const SpreadArgs<R>& A)
 {
     using FT = typename Pop<S>::F;
     __shared__ int keys[kMaxList];
     __shared__ int runs[64 * kSpreadWaves];
-    __shared__ double ew_all[kSpreadWaves][15][kChunk + 1];
-    __shared__ double part[kSpreadWaves][3][64];
+    __shared__ R ew_all[kSpreadWaves][15][kChunk + 1];
+    __shared__ R part[kSpreadWaves][3][64];
     if (bx < A.n_small)
-        spread_small(bx, A.n_small, ew_all, A.lv, A.gd, A.Sw, A.active, A.nactive, A.count, A.cap, A.list, A.base,
-                     A.W, A.vforce, A.nw, (FT*)A.Fout);
+        spread_small<FT, R>(bx, A.n_small, ew_all, A.lv, A.gd, A.Sw, A.active, A.nactive, A.count, A.cap, A.list,
+                            A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
     else
-        spread_big(bx - A.n_small, nbx - A.n_small, keys, runs, ew_all, part, A.lv, A.gd, A.Sw, A.big, A.nbig,
-                   A.count, A.cap, A.list, A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
+        spread_big<FT, R>(bx - A.n_small, nbx - A.n_small, keys, runs, ew_all, part, A.lv, A.gd, A.Sw, A.big, A.nbig,
+                          A.count, A.cap, A.list, A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
 }
 
-template <class S>
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(SpreadArgs a)
+template <class S, class R>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(SpreadArgs<R> a)
 {
-    spread_body<S>(blockIdx.x, gridDim.x, a);
+    spread_body<S, R>(blockIdx.x, gridDim.x, a);
 }
 
 // ------------------------------------------- stages paired in one launch
@@ -1337,32 +1407,33 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread(SpreadArgs a)
 // one launch: workgroups [0, n1) take the first, the others the second
 // (the choice is uniform over a workgroup, so its barriers stay whole).
 // Binning | element forces: both read the positions only.
-__global__ void __launch_bounds__(256) k_bin_elem(BinArgs b, int n1, ElemArgs e)
+template <class R>
+__global__ void __launch_bounds__(256) k_bin_elem(BinArgs<R> b, int n1, ElemArgs<R> e)
 {
     const int bx = blockIdx.x;
-    if (bx < n1) bin_body(bx, b);
-    else elem_body(bx - n1, e);
+    if (bx < n1) bin_body<R>(bx, b);
+    else elem_body<R>(bx - n1, e);
 }
 
 // Vertex force gather (element forces) | velocity staging (the binning's
 // tiles, the last collision's field).
-template <class S, bool SWAPPED>
-__global__ void __launch_bounds__(256) k_gather_stage(GatherArgs ga, int n1, StageArgs sa)
+template <class S, bool SWAPPED, class R>
+__global__ void __launch_bounds__(256) k_gather_stage(GatherArgs<R> ga, int n1, StageArgs sa)
 {
     const int bx = blockIdx.x;
-    if (bx < n1) gather_body(bx, ga);
+    if (bx < n1) gather_body<R>(bx, ga);
     else stage_body<S, SWAPPED>(bx - n1, sa);
 }
 
 // Spreading (lists, weights and vertex forces) | interpolation + advection
 // (weights and staged velocity; it moves the vertices, which the spreading
 // does not read).
-template <class S, int SW>
-__global__ void __launch_bounds__(256) k_spread_interp(SpreadArgs sp, int n1, InterpArgs ia)
+template <class S, int SW, class R>
+__global__ void __launch_bounds__(256) k_spread_interp(SpreadArgs<R> sp, int n1, InterpArgs<R> ia)
 {
     const int bx = blockIdx.x;
-    if (bx < n1) spread_body<S>(bx, n1, sp);
-    else interp_body<typename Pop<S>::C, SW, true>(bx - n1, (int)gridDim.x - n1, ia);
+    if (bx < n1) spread_body<S, R>(bx, n1, sp);
+    else interp_body<typename Pop<S>::C, SW, true, R>(bx - n1, (int)gridDim.x - n1, ia);
 }
 
 // ------------------------------------------------------- calibration
@@ -1373,7 +1444,8 @@ __global__ void __launch_bounds__(256) k_spread_interp(SpreadArgs sp, int n1, In
 // summed in triangle order (the vertex's incidence list is in element
 // order).  The step counter lives on the device so that the launches of
 // any step are the same.
-__global__ void __launch_bounds__(256) k_calib_mass(ForceView f, long NV, const int* __restrict__ active,
+template <class R>
+__global__ void __launch_bounds__(256) k_calib_mass(ForceView<R> f, long NV, const int* __restrict__ active,
                                                     double* __restrict__ mass, int* __restrict__ step)
 {
 #pragma clang fp contract(off)
@@ -1391,8 +1463,8 @@ __global__ void __launch_bounds__(256) k_calib_mass(ForceView f, long NV, const 
     double M = 0.0;
     for (int k = 0; k < nt; ++k) {
         const long st = f.s_toff[s] + (Lt[k] >> 2);
-        const D3 v0 = ld3(f.pos + 3 * (vo + f.s_tri[3 * st])), v1 = ld3(f.pos + 3 * (vo + f.s_tri[3 * st + 1])),
-                 v2 = ld3(f.pos + 3 * (vo + f.s_tri[3 * st + 2]));
+        const D3 v0 = ld3d(f.pos + 3 * (vo + f.s_tri[3 * st])), v1 = ld3d(f.pos + 3 * (vo + f.s_tri[3 * st + 1])),
+                 v2 = ld3d(f.pos + 3 * (vo + f.s_tri[3 * st + 2]));
         M += (0.5 * norm(cross(v1 - v0, v2 - v0))) / 3.;
     }
     mass[v] = M;
@@ -1400,15 +1472,17 @@ __global__ void __launch_bounds__(256) k_calib_mass(ForceView f, long NV, const 
 
 // One relax step of the selected particles: vel += F / M - damping vel,
 // pos += vel; the first vertex of each sets vol_desired of this step (the
-// ramp from vol_start to vol_final over the first n_it steps).
+// ramp from vol_start to vol_final over the first n_it steps).  The step's
+// arithmetic is FP64; the vertex state keeps the boundary's type.
+template <class R>
 __global__ void __launch_bounds__(256) k_calib_relax(long NV, const int* __restrict__ v_part,
                                                      const long* __restrict__ p_voff, const int* __restrict__ active,
                                                      const double* __restrict__ damping,
                                                      const double* __restrict__ vol_start,
                                                      const double* __restrict__ vol_final, int n_it,
                                                      const int* __restrict__ step, const double* __restrict__ mass,
-                                                     const double* __restrict__ force, double* __restrict__ vel,
-                                                     double* __restrict__ pos, double* __restrict__ params)
+                                                     const R* __restrict__ force, R* __restrict__ vel,
+                                                     R* __restrict__ pos, double* __restrict__ params)
 {
 #pragma clang fp contract(off)
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1420,14 +1494,16 @@ __global__ void __launch_bounds__(256) k_calib_relax(long NV, const int* __restr
         params[8 * p + 5] = vol_start[p] + ((double)it / (double)n_it) * (vol_final[p] - vol_start[p]);
     const double M = mass[v], d = damping[p];
     for (int k = 0; k < 3; ++k) {
-        const double u = vel[3 * v + k] + (force[3 * v + k] / M - d * vel[3 * v + k]);
-        vel[3 * v + k] = u;
-        pos[3 * v + k] = pos[3 * v + k] + u;
+        const double u0 = (double)vel[3 * v + k];
+        const double u = u0 + ((double)force[3 * v + k] / M - d * u0);
+        vel[3 * v + k] = (R)u;
+        pos[3 * v + k] = (R)((double)pos[3 * v + k] + u);
     }
 }
 
 // --------------------------------------------------------------- motion
-__global__ void k_move(long NV, double* __restrict__ pos, const double* __restrict__ vel)
+template <class R>
+__global__ void k_move(long NV, R* __restrict__ pos, const R* __restrict__ vel)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 3 * NV) return;
@@ -1442,7 +1518,8 @@ __global__ void k_move(long NV, double* __restrict__ pos, const double* __restri
 // depend on scheduling.
 // Chunk c (its row of `chunks`: first triangle, triangles, vertex range
 // of its bounding-box share, all global indices) -> 11 partials.
-__global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int4* __restrict__ chunks,
+template <class R>
+__global__ void __launch_bounds__(256) k_geom_partial(ForceView<R> f, const int4* __restrict__ chunks,
                                                       const int* __restrict__ t_vert, double* __restrict__ partial)
 {
     __shared__ double sh[11][256];
@@ -1451,8 +1528,8 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int4* _
     double acc[5] = {0, 0, 0, 0, 0};
     if ((int)threadIdx.x < ch.y) {
         const int* tv = t_vert + 3 * ((long)ch.x + threadIdx.x);
-        const D3 v0 = ld3(f.pos + 3 * (long)tv[0]), v1 = ld3(f.pos + 3 * (long)tv[1]),
-                 v2 = ld3(f.pos + 3 * (long)tv[2]);
+        const D3 v0 = ld3d(f.pos + 3 * (long)tv[0]), v1 = ld3d(f.pos + 3 * (long)tv[1]),
+                 v2 = ld3d(f.pos + 3 * (long)tv[2]);
         const double dA = 0.5 * norm(cross(v1 - v0, v2 - v0));
         acc[0] = dA;
         acc[1] = dot(v0, cross(v1, v2)) / 6.;
@@ -1463,7 +1540,7 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int4* _
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (long v = ch.z + threadIdx.x; v < ch.w; v += blockDim.x)
         for (int d = 0; d < 3; ++d) {
-            const double x = f.pos[3 * v + d];
+            const double x = (double)f.pos[3 * v + d];
             lo[d] = fmin(lo[d], x);
             hi[d] = fmax(hi[d], x);
         }
@@ -1489,10 +1566,11 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView f, const int4* _
 // Chunk partials -> geometry; the centre of mass is then mapped onto the
 // periodic grid (src/fsi/ParticleBase.hh:113-136, src/fsi/Periodicity.h:83-87)
 // and the whole particle shifted by the same amount.
+template <class R>
 __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* __restrict__ p_choff,
                                                   const double* __restrict__ partial, const long* __restrict__ p_voff,
                                                   double* __restrict__ geom, double* __restrict__ delta,
-                                                  double* __restrict__ pos, int remap, const int* __restrict__ p_model)
+                                                  R* __restrict__ pos, int remap, const int* __restrict__ p_model)
 {
     __shared__ double sft[3];
     __shared__ double part[64][11];  // a batch of chunk partials, loaded by all lanes at once
@@ -1543,9 +1621,9 @@ __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* _
     const double s0 = sft[0], s1 = sft[1], s2 = sft[2];
     if (s0 == 0.0 && s1 == 0.0 && s2 == 0.0) return;
     for (long v = p_voff[p] + threadIdx.x; v < p_voff[p + 1]; v += blockDim.x) {
-        pos[3 * v] += s0;
-        pos[3 * v + 1] += s1;
-        pos[3 * v + 2] += s2;
+        pos[3 * v] = (R)((double)pos[3 * v] + s0);
+        pos[3 * v + 1] = (R)((double)pos[3 * v + 1] + s1);
+        pos[3 * v + 2] = (R)((double)pos[3 * v + 2] + s2);
     }
 }
 
@@ -1554,7 +1632,8 @@ __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* _
 // src/fsi/RayTracer.hh:19-39,80-108).  Pass 0 counts hits per column,
 // pass 1 writes them (z mapped into [-0.5, nz - 0.5) on a periodic z axis,
 // as ZBuffer::put does, src/fsi/ZBuffer.h:39-49).
-__global__ void k_ray_hits(ForceView f, Grid gd, long NT, int pass, int* col_count, const int* col_off,
+template <class R>
+__global__ void k_ray_hits(ForceView<R> f, Grid gd, long NT, int pass, int* col_count, const int* col_off,
                            int* col_fill, double* col_z, long cap, int* status)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1563,7 +1642,8 @@ __global__ void k_ray_hits(ForceView f, Grid gd, long NT, int pass, int* col_cou
     const int s = f.p_shape[p];
     const long vo = f.p_voff[p];
     const int* tri = f.s_tri + 3 * (f.s_toff[s] + (g - f.p_toff[p]));
-    const D3 a = ld3(f.pos + 3 * (vo + tri[0])), b = ld3(f.pos + 3 * (vo + tri[1])), c = ld3(f.pos + 3 * (vo + tri[2]));
+    const D3 a = ld3d(f.pos + 3 * (vo + tri[0])), b = ld3d(f.pos + 3 * (vo + tri[1])),
+             c = ld3d(f.pos + 3 * (vo + tri[2]));
     // a triangle with a non-finite corner, or wider than the lattice, is
     // not traced (its membrane diverged; the binning reports it)
     const double ext = fmax(fmax(fabs(a.x), fabs(b.x)), fmax(fmax(fabs(c.x), fabs(a.y)), fmax(fabs(b.y), fabs(c.y))));
@@ -1692,9 +1772,47 @@ int alloc(T*& dst, size_t n)
     return BL_OK;
 }
 
-ForceView force_view(bl_ibm g)
+int alloc_bytes(void*& dst, size_t n)
 {
-    ForceView f;
+    hipFree(dst);
+    dst = nullptr;
+    BL_HIP(hipMalloc(&dst, std::max<size_t>(n, 1)));
+    return BL_OK;
+}
+
+// Runs fn(R{}) with R the boundary's vertex real type.
+template <class Fn>
+auto with_vreal(bl_ibm g, Fn&& fn)
+{
+    if (g->vreal == 4) return fn(float{});
+    return fn(double{});
+}
+
+// The vertex real type that goes with a population storage type.
+template <class S>
+struct VRof {
+    using T = float;
+};
+template <>
+struct VRof<double> {
+    using T = double;
+};
+
+// The kernels templated on the storage use its vertex type; a boundary
+// whose vertex type differs (rigid particles on a reduced-precision
+// lattice) never reaches them (rigid_refuse_coupling).
+int vreal_matches_storage(bl_ibm g)
+{
+    const int want = g->lat->storage == BL_STORE_F64 ? 8 : 4;
+    if (g->vreal != want)
+        return fail(BL_ERR_STATE, "immersed boundary: vertex precision does not match the lattice storage");
+    return BL_OK;
+}
+
+template <class R>
+ForceView<R> force_view(bl_ibm g)
+{
+    ForceView<R> f;
     f.np = g->np;
     f.p_voff = g->p_voff; f.p_toff = g->p_toff; f.p_loff = g->p_loff; f.p_poff = g->p_poff;
     f.p_shape = g->p_shape; f.p_model = g->p_model; f.p_bug = g->p_bug;
@@ -1708,21 +1826,22 @@ ForceView force_view(bl_ibm g)
     f.s_vl_head = g->s_vl_head; f.s_vl = g->s_vl;
     f.s_vp_head = g->s_vp_head; f.s_vp = g->s_vp;
     f.s_area0 = g->s_area0;
-    f.pos = g->pos;
+    f.pos = (const R*)g->pos;
     f.v_part = g->v_part; f.t_part = g->t_part; f.r_part = g->r_part; f.l_part = g->l_part;
     f.v_inc = (const int4*)g->v_inc;
     return f;
 }
 
-NearView near_view(bl_ibm g)
+template <class R>
+NearView<R> near_view(bl_ibm g)
 {
-    NearView nw;
+    NearView<R> nw;
     nw.b = g->boundary;
     nw.on = g->has_boundary && g->v_near ? 1 : 0;
     nw.near = g->v_near;
-    nw.blend = g->v_blend;
+    nw.blend = (R*)g->v_blend;
     nw.base_t = g->v_base_t;
-    nw.W_t = g->v_w_t;
+    nw.W_t = (R*)g->v_w_t;
     return nw;
 }
 
@@ -1732,9 +1851,9 @@ int alloc_near(bl_ibm g)
     if (!g->has_boundary || !g->inited || g->v_near) return BL_OK;
     const size_t nv = (size_t)std::max<long>(g->NV, 1);
     hipError_t e = hipMalloc(&g->v_near, nv);
-    if (e == hipSuccess) e = hipMalloc(&g->v_blend, sizeof(double) * nv);
+    if (e == hipSuccess) e = hipMalloc(&g->v_blend, (size_t)g->vreal * nv);
     if (e == hipSuccess) e = hipMalloc(&g->v_base_t, sizeof(int) * 3 * nv);
-    if (e == hipSuccess) e = hipMalloc(&g->v_w_t, sizeof(double) * 6 * nv);
+    if (e == hipSuccess) e = hipMalloc(&g->v_w_t, (size_t)g->vreal * 6 * nv);
     if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("near-wall buffers: ") + hipGetErrorString(e));
     return BL_OK;
 }
@@ -1757,13 +1876,16 @@ void mark(bl_ibm g, int k)
 int update_particles(bl_ibm g, int remap, hipStream_t s)
 {
     if (g->np == 0) return BL_OK;
-    const ForceView f = force_view(g);
     const Grid gd = grid_of(g->lat);
-    hipLaunchKernelGGL(k_geom_partial, dim3(g->nchunk), dim3(256), 0, s, f, (const int4*)g->ch_tab,
-                       (const int*)g->t_vert, g->ch_partial);
-    hipLaunchKernelGGL(k_geom_final, dim3(g->np), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
-                       (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, g->pos, remap,
-                       (const int*)g->p_model);
+    with_vreal(g, [&](auto tag) {
+        using R = decltype(tag);
+        hipLaunchKernelGGL(k_geom_partial<R>, dim3(g->nchunk), dim3(256), 0, s, force_view<R>(g),
+                           (const int4*)g->ch_tab, (const int*)g->t_vert, g->ch_partial);
+        hipLaunchKernelGGL(k_geom_final<R>, dim3(g->np), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
+                           (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, (R*)g->pos,
+                           remap, (const int*)g->p_model);
+        return 0;
+    });
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
@@ -1817,18 +1939,19 @@ int prep_bin(bl_ibm g, hipStream_t s)
     return BL_OK;
 }
 
-BinArgs bin_args(bl_ibm g)
+template <class R>
+BinArgs<R> bin_args(bl_ibm g)
 {
     bl_lattice L = g->lat;
-    BinArgs a;
+    BinArgs<R> a;
     a.gd = grid_of(L);
     a.kernel = g->kernel;
     a.S = g->S;
     a.h = g->h;
     a.NV = g->NV;
-    a.pos = g->pos;
+    a.pos = (const R*)g->pos;
     a.base = g->v_base;
-    a.W = g->v_w;
+    a.W = (R*)g->v_w;
     a.count = g->tile_count;
     a.list = g->tile_list;
     a.cap = kTileCap;
@@ -1839,7 +1962,7 @@ BinArgs bin_args(bl_ibm g)
     a.pairs = g->n_active + 5 + g->alt;
     a.big = g->active + (2L + g->alt) * L->ntiles;
     a.nbig = g->n_active + 8 + g->alt;
-    a.nw = near_view(g);
+    a.nw = near_view<R>(g);
     return a;
 }
 
@@ -1856,7 +1979,12 @@ int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
     int rc;
     if (!g->prepped && (rc = prep_bin(g, s))) return rc;
     g->prepped = false;
-    if (g->NV) hipLaunchKernelGGL(k_bin, dim3(nblk(g->NV)), dim3(256), 0, s, bin_args(g));
+    if (g->NV)
+        with_vreal(g, [&](auto tag) {
+            using R = decltype(tag);
+            hipLaunchKernelGGL(k_bin<R>, dim3(nblk(g->NV)), dim3(256), 0, s, bin_args<R>(g));
+            return 0;
+        });
     if (done) BL_HIP(hipEventRecord(done, s));
     BL_CHECK_LAUNCH();
     binned(g);
@@ -1914,18 +2042,19 @@ int launch_stage(bl_ibm g, hipStream_t s, bool spread)
     return BL_OK;
 }
 
-InterpArgs interp_args(bl_ibm g, const ListReset& lr)
+template <class R>
+InterpArgs<R> interp_args(bl_ibm g, const ListReset& lr)
 {
-    InterpArgs a;
+    InterpArgs<R> a;
     a.gd = grid_of(g->lat);
     a.NV = g->NV;
     a.base = g->v_base;
-    a.W = g->v_w;
+    a.W = (const R*)g->v_w;
     a.ut = g->ut;
-    a.vel = g->vel;
-    a.pos = g->pos;
+    a.vel = (R*)g->vel;
+    a.pos = (R*)g->pos;
     a.lr = lr;
-    a.nw = near_view(g);
+    a.nw = near_view<R>(g);
     return a;
 }
 
@@ -1934,11 +2063,12 @@ unsigned interp_blocks(bl_ibm g) { return (unsigned)nblk((g->S == 2 ? 4 : 16) * 
 template <class C, bool MOVE>
 void launch_interp_c(bl_ibm g, hipStream_t s, const ListReset& lr)
 {
-    const InterpArgs a = interp_args(g, lr);
+    using R = C;  // the staged velocity's type is the vertex type
+    const InterpArgs<R> a = interp_args<R>(g, lr);
     const dim3 grid(interp_blocks(g));
-    if (g->S == 4) hipLaunchKernelGGL((k_interp<C, 4, MOVE>), grid, dim3(256), 0, s, a);
-    else if (g->S == 3) hipLaunchKernelGGL((k_interp<C, 3, MOVE>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_interp<C, 2, MOVE>), grid, dim3(256), 0, s, a);
+    if (g->S == 4) hipLaunchKernelGGL((k_interp<C, 4, MOVE, R>), grid, dim3(256), 0, s, a);
+    else if (g->S == 3) hipLaunchKernelGGL((k_interp<C, 3, MOVE, R>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_interp<C, 2, MOVE, R>), grid, dim3(256), 0, s, a);
 }
 
 // The list reset carried by a launch that follows this step's spreading:
@@ -1957,6 +2087,8 @@ ListReset reset_next_of(bl_ibm g, hipStream_t s)
 int launch_interp(bl_ibm g, bool move, hipStream_t s, bool reset_next = false)
 {
     if (g->NV == 0) return BL_OK;
+    int rc = vreal_matches_storage(g);
+    if (rc) return rc;
     ListReset lr;
     if (reset_next) lr = reset_next_of(g, s);
     const bool f64 = g->lat->storage == BL_STORE_F64;
@@ -1970,10 +2102,11 @@ int launch_interp(bl_ibm g, bool move, hipStream_t s, bool reset_next = false)
 
 // Spreading of the last binning's lists into the boundary's second force
 // buffer; `blocks` receives the launch's workgroup count.
-SpreadArgs spread_args(bl_ibm g, unsigned& blocks)
+template <class R>
+SpreadArgs<R> spread_args(bl_ibm g, unsigned& blocks)
 {
     bl_lattice L = g->lat;
-    SpreadArgs a;
+    SpreadArgs<R> a;
     a.lv = ibm_view(g);
     a.gd = grid_of(L);
     a.Sw = g->S;
@@ -1985,9 +2118,9 @@ SpreadArgs spread_args(bl_ibm g, unsigned& blocks)
     a.cap = kTileCap;
     a.list = g->tile_list;
     a.base = g->v_base;
-    a.W = g->v_w;
-    a.vforce = g->force;
-    a.nw = near_view(g);
+    a.W = (const R*)g->v_w;
+    a.vforce = (const R*)g->force;
+    a.nw = near_view<R>(g);
     a.Fout = g->force_alt;
     // about one small-tile wave per touched tile and one workgroup per big
     // tile: the touched tiles are at most every tile and, for membranes,
@@ -2020,11 +2153,17 @@ int launch_spread(bl_ibm g, hipStream_t s)
 {
     bl_lattice L = g->lat;
     if (g->NV == 0) return BL_OK;
+    int rc = vreal_matches_storage(g);
+    if (rc) return rc;
     unsigned nb;
-    const SpreadArgs a = spread_args(g, nb);
-    if (L->storage == BL_STORE_F64) hipLaunchKernelGGL(k_spread<double>, dim3(nb), dim3(256), 0, s, a);
-    else if (L->storage == BL_STORE_F32) hipLaunchKernelGGL(k_spread<float>, dim3(nb), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_spread<__half>, dim3(nb), dim3(256), 0, s, a);
+    if (L->storage == BL_STORE_F64) {
+        const SpreadArgs<double> a = spread_args<double>(g, nb);
+        hipLaunchKernelGGL((k_spread<double, double>), dim3(nb), dim3(256), 0, s, a);
+    } else {
+        const SpreadArgs<float> a = spread_args<float>(g, nb);
+        if (L->storage == BL_STORE_F32) hipLaunchKernelGGL((k_spread<float, float>), dim3(nb), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_spread<__half, float>), dim3(nb), dim3(256), 0, s, a);
+    }
     BL_CHECK_LAUNCH();
     spread_swap(g);
     return BL_OK;
@@ -2040,35 +2179,50 @@ int do_interpolate(bl_ibm g)
 
 // Vertex forces on stream s: particle-particle forces into their own
 // buffer, per-element membrane forces, then the per-vertex sum.
-ElemArgs elem_args(bl_ibm g)
+template <class R>
+ElemArgs<R> elem_args(bl_ibm g)
 {
-    ElemArgs a;
-    a.f = force_view(g);
+    ElemArgs<R> a;
+    a.f = force_view<R>(g);
     a.NT = g->NT;
     a.NP = g->NP;
     a.NL = g->NL;
     a.ec_pair = g->ec_pair;
     a.ec_link = g->ec_link;
-    a.ec = g->ec;
+    a.ec = (R*)g->ec;
     return a;
 }
 
 // `walls`: with the wall-interaction decorators and the particle-particle
 // forces of this step (when there are any)
-GatherArgs gather_args(bl_ibm g, bool walls)
+template <class R>
+GatherArgs<R> gather_args(bl_ibm g, bool walls)
 {
-    GatherArgs a;
-    a.f = force_view(g);
+    GatherArgs<R> a;
+    a.f = force_view<R>(g);
     a.NV = g->NV;
     a.ec_pair = g->ec_pair;
     a.ec_link = g->ec_link;
-    a.ec = g->ec;
-    a.force = g->force;
+    a.ec = (const R*)g->ec;
+    a.force = (R*)g->force;
     if (walls) a.deco = deco_set(g);
     else a.deco.n = 0;
-    a.fpp = walls && !g->pp.empty() ? g->pp_force : nullptr;
-    a.ext = walls && g->has_ext ? g->ext_force : nullptr;
+    a.fpp = walls && !g->pp.empty() ? (const R*)g->pp_force : nullptr;
+    a.ext = walls && g->has_ext ? (const R*)g->ext_force : nullptr;
     return a;
+}
+
+// per-element membrane forces, then the per-vertex sum (`walls`: with the
+// decorators, particle-particle and host-given forces)
+void launch_membrane(bl_ibm g, hipStream_t s, bool walls)
+{
+    with_vreal(g, [&](auto tag) {
+        using R = decltype(tag);
+        hipLaunchKernelGGL(k_elem_forces<R>, dim3(nblk(g->NT + g->NP + g->NL)), dim3(256), 0, s, elem_args<R>(g));
+        hipLaunchKernelGGL(k_vertex_gather<R>, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s,
+                           gather_args<R>(g, walls));
+        return 0;
+    });
 }
 
 int do_forces(bl_ibm g, hipStream_t s)
@@ -2076,10 +2230,7 @@ int do_forces(bl_ibm g, hipStream_t s)
     int rc;
     const bool pp = !g->pp.empty() && g->NV;
     if (pp && (rc = interactions_launch(g, s))) return rc;
-    if (g->NV) {
-        hipLaunchKernelGGL(k_elem_forces, dim3(nblk(g->NT + g->NP + g->NL)), dim3(256), 0, s, elem_args(g));
-        hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, gather_args(g, true));
-    }
+    if (g->NV) launch_membrane(g, s, true);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
@@ -2094,7 +2245,13 @@ int do_spread(bl_ibm g)
 int do_move(bl_ibm g)
 {
     hipStream_t s = g->lat->stream;
-    if (g->NV) hipLaunchKernelGGL(k_move, dim3(nblk(3 * g->NV)), dim3(256), 0, s, g->NV, g->pos, (const double*)g->vel);
+    if (g->NV)
+        with_vreal(g, [&](auto tag) {
+            using R = decltype(tag);
+            hipLaunchKernelGGL(k_move<R>, dim3(nblk(3 * g->NV)), dim3(256), 0, s, g->NV, (R*)g->pos,
+                               (const R*)g->vel);
+            return 0;
+        });
     BL_CHECK_LAUNCH();
     g->binned = false;
     return update_particles(g, 1);
@@ -2106,14 +2263,19 @@ int do_voxelize(bl_ibm g, hipStream_t s)
     if (L->omega_c0 == L->omega_c1 || g->NT == 0) return BL_OK;
     const Grid gd = grid_of(L);
     const long ncol = (long)L->nx * L->ny;
-    const ForceView f = force_view(g);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ncol)), dim3(256), 0, s, g->col_count, ncol, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ncol)), dim3(256), 0, s, g->col_fill, ncol, 0);
-    hipLaunchKernelGGL(k_ray_hits, dim3(nblk(g->NT)), dim3(256), 0, s, f, gd, g->NT, 0, g->col_count,
-                       (const int*)g->col_off, g->col_fill, g->col_z, g->col_cap, g->n_active);
-    hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(256), 0, s, (const int*)g->col_count, ncol, g->col_off, g->n_hits);
-    hipLaunchKernelGGL(k_ray_hits, dim3(nblk(g->NT)), dim3(256), 0, s, f, gd, g->NT, 1, g->col_count,
-                       (const int*)g->col_off, g->col_fill, g->col_z, g->col_cap, g->n_active);
+    with_vreal(g, [&](auto tag) {
+        using R = decltype(tag);
+        const ForceView<R> f = force_view<R>(g);
+        hipLaunchKernelGGL(k_ray_hits<R>, dim3(nblk(g->NT)), dim3(256), 0, s, f, gd, g->NT, 0, g->col_count,
+                           (const int*)g->col_off, g->col_fill, g->col_z, g->col_cap, g->n_active);
+        hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(256), 0, s, (const int*)g->col_count, ncol, g->col_off,
+                           g->n_hits);
+        hipLaunchKernelGGL(k_ray_hits<R>, dim3(nblk(g->NT)), dim3(256), 0, s, f, gd, g->NT, 1, g->col_count,
+                           (const int*)g->col_off, g->col_fill, g->col_z, g->col_cap, g->n_active);
+        return 0;
+    });
     hipLaunchKernelGGL(k_fill_columns, dim3(nblk(ncol)), dim3(256), 0, s, gd, (const int*)g->col_count,
                        (const int*)g->col_off, g->col_z, g->col_cap, g->phase);
     BL_CHECK_LAUNCH();
@@ -2269,30 +2431,32 @@ int step_fused(bl_ibm g)
 template <class S>
 int step_layered_s(bl_ibm g)
 {
+    using R = typename VRof<S>::T;
     int rc;
     bl_lattice L = g->lat;
+    if ((rc = vreal_matches_storage(g))) return rc;
     hipStream_t s = L->stream;
     if (!g->pp.empty() && (rc = interactions_launch(g, s))) return rc;
     if (!g->prepped && (rc = prep_bin(g, s))) return rc;
     g->prepped = false;
     const int nb = (int)nblk(g->NV), ne = (int)nblk(g->NT + g->NP + g->NL);
-    hipLaunchKernelGGL(k_bin_elem, dim3(nb + ne), dim3(256), 0, s, bin_args(g), nb, elem_args(g));
+    hipLaunchKernelGGL(k_bin_elem<R>, dim3(nb + ne), dim3(256), 0, s, bin_args<R>(g), nb, elem_args<R>(g));
     binned(g);
     const int ng = (int)nblk(kGatherLanes * g->NV);
     const StageArgs sa = stage_args(g, false);
     const dim3 g2(ng + stage_blocks(g));
     if (L->steps & 1)
-        hipLaunchKernelGGL((k_gather_stage<S, true>), g2, dim3(256), 0, s, gather_args(g, true), ng, sa);
+        hipLaunchKernelGGL((k_gather_stage<S, true, R>), g2, dim3(256), 0, s, gather_args<R>(g, true), ng, sa);
     else
-        hipLaunchKernelGGL((k_gather_stage<S, false>), g2, dim3(256), 0, s, gather_args(g, true), ng, sa);
+        hipLaunchKernelGGL((k_gather_stage<S, false, R>), g2, dim3(256), 0, s, gather_args<R>(g, true), ng, sa);
     unsigned nsp;
-    const SpreadArgs sp = spread_args(g, nsp);
+    const SpreadArgs<R> sp = spread_args<R>(g, nsp);
     spread_swap(g);
-    const InterpArgs ia = interp_args(g, reset_next_of(g, s));
+    const InterpArgs<R> ia = interp_args<R>(g, reset_next_of(g, s));
     const dim3 g3(nsp + interp_blocks(g));
-    if (g->S == 4) hipLaunchKernelGGL((k_spread_interp<S, 4>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
-    else if (g->S == 3) hipLaunchKernelGGL((k_spread_interp<S, 3>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
-    else hipLaunchKernelGGL((k_spread_interp<S, 2>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
+    if (g->S == 4) hipLaunchKernelGGL((k_spread_interp<S, 4, R>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
+    else if (g->S == 3) hipLaunchKernelGGL((k_spread_interp<S, 3, R>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
+    else hipLaunchKernelGGL((k_spread_interp<S, 2, R>), g3, dim3(256), 0, s, sp, (int)nsp, ia);
     BL_CHECK_LAUNCH();
     g->binned = false;
     if ((rc = update_particles(g, 1, s))) return rc;
@@ -2352,6 +2516,35 @@ void ibm_detach(bl_ibm g)
     for (auto& st : g->aux)
         if (st) hipStreamSynchronize(st);
     g->lat = nullptr;
+}
+
+int vreal_upload(bl_ibm g, void* dst, long off, const double* src, long n, hipStream_t s)
+{
+    if (n <= 0) return BL_OK;
+    if (g->vreal == 8) {
+        BL_HIP(hipMemcpyAsync((double*)dst + off, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        BL_HIP(hipStreamSynchronize(s));
+        return BL_OK;
+    }
+    std::vector<float> tmp(src, src + n);
+    BL_HIP(hipMemcpyAsync((float*)dst + off, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+    BL_HIP(hipStreamSynchronize(s));
+    return BL_OK;
+}
+
+int vreal_download(bl_ibm g, double* dst, const void* src, long off, long n, hipStream_t s)
+{
+    if (n <= 0) return BL_OK;
+    if (g->vreal == 8) {
+        BL_HIP(hipMemcpyAsync(dst, (const double*)src + off, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        BL_HIP(hipStreamSynchronize(s));
+        return BL_OK;
+    }
+    std::vector<float> tmp(n);
+    BL_HIP(hipMemcpyAsync(tmp.data(), (const float*)src + off, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    BL_HIP(hipStreamSynchronize(s));
+    for (long i = 0; i < n; ++i) dst[i] = (double)tmp[i];
+    return BL_OK;
 }
 
 void ibm_invalidate_graphs(bl_ibm g)
@@ -2597,14 +2790,22 @@ int bl_ibm_init(bl_ibm g)
             (rc = upload(g->l_part, lp)) || (rc = upload(g->v_inc, inc)))
             return rc;
     }
+    // vertex reals: FP32 with reduced-precision populations; rigid particles
+    // (their own FP64 dynamics, no fluid coupling) keep FP64
+    bool any_rigid = false;
+    for (const auto& p : g->parts) any_rigid = any_rigid || p.model == BL_MODEL_RIGID;
+    g->vreal = (L->storage == BL_STORE_F64 || any_rigid) ? 8 : 4;
+    const size_t vr = (size_t)g->vreal;
+    const size_t nec = 9 * (size_t)g->NT + 12 * (size_t)g->NP + 3 * (size_t)g->NL;
     if ((rc = upload(g->p_voff, voff)) || (rc = upload(g->p_toff, toff)) || (rc = upload(g->p_loff, loff)) ||
         (rc = upload(g->p_poff, poff)) || (rc = upload(g->p_shape, pshape)) || (rc = upload(g->p_model, pmodel)) ||
-        (rc = upload(g->p_bug, pbug)) || (rc = upload(g->p_params, params)) || (rc = upload(g->pos, pos)) ||
-        (rc = alloc(g->vel, 3 * g->NV)) || (rc = alloc(g->force, 3 * g->NV)) || (rc = alloc(g->p_geom, 11 * (size_t)g->np)) ||
-        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) ||
-        (rc = alloc(g->ec, 9 * (size_t)g->NT + 12 * (size_t)g->NP + 3 * (size_t)g->NL)) ||
-        (rc = alloc(g->v_base, 3 * g->NV)) || (rc = alloc(g->v_w, 12 * g->NV)))
+        (rc = upload(g->p_bug, pbug)) || (rc = upload(g->p_params, params)) ||
+        (rc = alloc_bytes(g->pos, vr * 3 * g->NV)) || (rc = alloc_bytes(g->vel, vr * 3 * g->NV)) ||
+        (rc = alloc_bytes(g->force, vr * 3 * g->NV)) || (rc = alloc(g->p_geom, 11 * (size_t)g->np)) ||
+        (rc = alloc(g->p_delta, 3 * (size_t)g->np)) || (rc = alloc_bytes(g->ec, vr * nec)) ||
+        (rc = alloc(g->v_base, 3 * g->NV)) || (rc = alloc_bytes(g->v_w, vr * 12 * g->NV)))
         return rc;
+    if ((rc = vreal_upload(g, g->pos, 0, pos.data(), (long)pos.size(), L->stream))) return rc;
     g->ec_pair = 9 * g->NT;
     g->ec_link = g->ec_pair + 12 * g->NP;
     {
@@ -2613,16 +2814,17 @@ int bl_ibm_init(bl_ibm g)
         BL_HIP(hipMalloc(&g->ut, csz * (size_t)L->ncell));
         BL_HIP(hipMemset(g->ut, 0, csz * (size_t)L->ncell));
     }
-    BL_HIP(hipMemset(g->vel, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
-    BL_HIP(hipMemset(g->force, 0, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
+    BL_HIP(hipMemset(g->vel, 0, vr * 3 * std::max<long>(g->NV, 1)));
+    BL_HIP(hipMemset(g->force, 0, vr * 3 * std::max<long>(g->NV, 1)));
     {  // vertex velocities and forces of particles loaded from a checkpoint
         long v0 = 0;
         for (const auto& p : g->parts) {
-            const size_t n = p.pos.size();
-            if (p.vel.size() == n) BL_HIP(hipMemcpy(g->vel + 3 * v0, p.vel.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-            if (p.force.size() == n)
-                BL_HIP(hipMemcpy(g->force + 3 * v0, p.force.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-            v0 += (long)n / 3;
+            const long n = (long)p.pos.size();
+            if ((long)p.vel.size() == n && (rc = vreal_upload(g, g->vel, 3 * v0, p.vel.data(), n, L->stream)))
+                return rc;
+            if ((long)p.force.size() == n && (rc = vreal_upload(g, g->force, 3 * v0, p.force.data(), n, L->stream)))
+                return rc;
+            v0 += n / 3;
         }
     }
     // tiles
@@ -2852,10 +3054,8 @@ int bl_particle_get(bl_ibm g, int pid, int what, double* out)
     std::vector<long> voff{0};
     for (const auto& p : g->parts) voff.push_back(voff.back() + g->shapes[p.shape].nv);
     const long v0 = pid < 0 ? 0 : voff[pid], v1 = pid < 0 ? voff.back() : voff[pid + 1];
-    const double* src = what == 0 ? g->pos : (what == 1 ? g->vel : g->force);
-    BL_HIP(hipMemcpyAsync(out, src + 3 * v0, sizeof(double) * 3 * (v1 - v0), hipMemcpyDeviceToHost, g->lat->stream));
-    BL_HIP(hipStreamSynchronize(g->lat->stream));
-    return BL_OK;
+    const void* src = what == 0 ? g->pos : (what == 1 ? g->vel : g->force);
+    return vreal_download(g, out, src, 3 * v0, 3 * (v1 - v0), g->lat->stream);
 }
 
 int bl_particle_set_positions(bl_ibm g, int pid, const double* pos)
@@ -2866,7 +3066,7 @@ int bl_particle_set_positions(bl_ibm g, int pid, const double* pos)
     std::vector<long> voff{0};
     for (const auto& p : g->parts) voff.push_back(voff.back() + g->shapes[p.shape].nv);
     const long v0 = pid < 0 ? 0 : voff[pid], v1 = pid < 0 ? voff.back() : voff[pid + 1];
-    BL_HIP(hipMemcpyAsync(g->pos + 3 * v0, pos, sizeof(double) * 3 * (v1 - v0), hipMemcpyHostToDevice, g->lat->stream));
+    if ((rc = vreal_upload(g, g->pos, 3 * v0, pos, 3 * (v1 - v0), g->lat->stream))) return rc;
     rc = update_particles(g, 0);
     if (rc) return rc;
     BL_HIP(hipStreamSynchronize(g->lat->stream));
@@ -2880,8 +3080,25 @@ namespace {
 // particle-particle forces.
 int membrane_forces(bl_ibm g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_elem_forces, dim3(nblk(g->NT + g->NP + g->NL)), dim3(256), 0, s, elem_args(g));
-    hipLaunchKernelGGL(k_vertex_gather, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s, gather_args(g, false));
+    launch_membrane(g, s, false);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
+}
+
+// One calibration step (node masses, then the relax update) of the
+// selected particles.
+int calib_step(bl_ibm g, hipStream_t s, const int* active, const double* damp, const double* v0, const double* v1,
+               int n_it, double* mass, int* step)
+{
+    with_vreal(g, [&](auto tag) {
+        using R = decltype(tag);
+        hipLaunchKernelGGL(k_calib_mass<R>, dim3(nblk(g->NV)), dim3(256), 0, s, force_view<R>(g), g->NV, active, mass,
+                           step);
+        hipLaunchKernelGGL(k_calib_relax<R>, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV, (const int*)g->v_part,
+                           (const long*)g->p_voff, active, damp, v0, v1, n_it, (const int*)step, (const double*)mass,
+                           (const R*)g->force, (R*)g->vel, (R*)g->pos, g->p_params);
+        return 0;
+    });
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
@@ -2921,22 +3138,23 @@ int bl_rbc_shrink_volume(bl_ibm g, int pid, double vol_final, int n_it, double d
     }
     hipStream_t s = g->lat->stream;
     const size_t nv3 = 3 * (size_t)g->NV;
+    const size_t vr = (size_t)g->vreal;
     DevBuf<int> d_active, d_step;
-    DevBuf<double> d_damp, d_v0, d_v1, d_mass, pos0, vel0, frc0, prm0;
+    DevBuf<double> d_damp, d_v0, d_v1, d_mass, prm0;
+    DevBuf<char> pos0, vel0, frc0;  // the vertex state, in the boundary's reals
     if ((rc = d_active.alloc(np)) || (rc = d_step.alloc(1)) || (rc = d_damp.alloc(np)) || (rc = d_v0.alloc(np)) ||
-        (rc = d_v1.alloc(np)) || (rc = d_mass.alloc(g->NV)) || (rc = pos0.alloc(nv3)) || (rc = vel0.alloc(nv3)) ||
-        (rc = frc0.alloc(nv3)) || (rc = prm0.alloc(8 * (size_t)np)))
+        (rc = d_v1.alloc(np)) || (rc = d_mass.alloc(g->NV)) || (rc = pos0.alloc(vr * nv3)) ||
+        (rc = vel0.alloc(vr * nv3)) || (rc = frc0.alloc(vr * nv3)) || (rc = prm0.alloc(8 * (size_t)np)))
         return rc;
     BL_HIP(hipMemcpyAsync(d_v0.p, v0.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
     BL_HIP(hipMemcpyAsync(d_v1.p, v1.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
     // the starting state: geometry and forces of the current positions with
     // the current parameters (rbc.compute_forces() before the loop)
     if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
-    BL_HIP(hipMemcpyAsync(pos0.p, g->pos, sizeof(double) * nv3, hipMemcpyDeviceToDevice, s));
-    BL_HIP(hipMemcpyAsync(vel0.p, g->vel, sizeof(double) * nv3, hipMemcpyDeviceToDevice, s));
-    BL_HIP(hipMemcpyAsync(frc0.p, g->force, sizeof(double) * nv3, hipMemcpyDeviceToDevice, s));
+    BL_HIP(hipMemcpyAsync(pos0.p, g->pos, vr * nv3, hipMemcpyDeviceToDevice, s));
+    BL_HIP(hipMemcpyAsync(vel0.p, g->vel, vr * nv3, hipMemcpyDeviceToDevice, s));
+    BL_HIP(hipMemcpyAsync(frc0.p, g->force, vr * nv3, hipMemcpyDeviceToDevice, s));
     BL_HIP(hipMemcpyAsync(prm0.p, g->p_params, sizeof(double) * 8 * np, hipMemcpyDeviceToDevice, s));
-    const ForceView f = force_view(g);
     for (;;) {
         bool any = false;
         for (int p = 0; p < np; ++p) any = any || todo[p];
@@ -2945,13 +3163,7 @@ int bl_rbc_shrink_volume(bl_ibm g, int pid, double vol_final, int n_it, double d
         BL_HIP(hipMemcpyAsync(d_damp.p, damp.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
         BL_HIP(hipMemsetAsync(d_step.p, 0, sizeof(int), s));
         for (int it = 0; it <= 2 * n_it; ++it) {
-            hipLaunchKernelGGL(k_calib_mass, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, (const int*)d_active.p,
-                               d_mass.p, d_step.p);
-            hipLaunchKernelGGL(k_calib_relax, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV, (const int*)g->v_part,
-                               (const long*)g->p_voff, (const int*)d_active.p, (const double*)d_damp.p,
-                               (const double*)d_v0.p, (const double*)d_v1.p, n_it, (const int*)d_step.p,
-                               (const double*)d_mass.p, (const double*)g->force, g->vel, g->pos, g->p_params);
-            BL_CHECK_LAUNCH();
+            if ((rc = calib_step(g, s, d_active.p, d_damp.p, d_v0.p, d_v1.p, n_it, d_mass.p, d_step.p))) return rc;
             if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
         }
         // a particle whose area went NaN starts over with twice the damping
@@ -2966,10 +3178,10 @@ int bl_rbc_shrink_volume(bl_ibm g, int pid, double vol_final, int n_it, double d
                 continue;
             }
             damp[p] *= 2.0;
-            const size_t a = 3 * (size_t)voff[p], n = 3 * (size_t)(voff[p + 1] - voff[p]);
-            BL_HIP(hipMemcpyAsync(g->pos + a, pos0.p + a, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            BL_HIP(hipMemcpyAsync(g->vel + a, vel0.p + a, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            BL_HIP(hipMemcpyAsync(g->force + a, frc0.p + a, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            const size_t a = vr * 3 * (size_t)voff[p], n = vr * 3 * (size_t)(voff[p + 1] - voff[p]);
+            BL_HIP(hipMemcpyAsync((char*)g->pos + a, pos0.p + a, n, hipMemcpyDeviceToDevice, s));
+            BL_HIP(hipMemcpyAsync((char*)g->vel + a, vel0.p + a, n, hipMemcpyDeviceToDevice, s));
+            BL_HIP(hipMemcpyAsync((char*)g->force + a, frc0.p + a, n, hipMemcpyDeviceToDevice, s));
             BL_HIP(hipMemcpyAsync(g->p_params + 8 * p, prm0.p + 8 * p, sizeof(double) * 8, hipMemcpyDeviceToDevice, s));
             if (damp[p] > 2.0) {  // left in its starting state
                 if ((rc = update_particles(g, 0, s))) return rc;
@@ -3008,10 +3220,8 @@ int bl_particle_set_state(bl_ibm g, int pid, int what, const double* data)
         if (pid < 0 || p <= pid) v1 += n;
     }
     if (pid < 0) v0 = 0;
-    double* dst = what == 1 ? g->vel : g->force;
-    BL_HIP(hipMemcpyAsync(dst + 3 * v0, data, sizeof(double) * 3 * (v1 - v0), hipMemcpyHostToDevice, g->lat->stream));
-    BL_HIP(hipStreamSynchronize(g->lat->stream));
-    return BL_OK;
+    void* dst = what == 1 ? g->vel : g->force;
+    return vreal_upload(g, dst, 3 * v0, data, 3 * (v1 - v0), g->lat->stream);
 }
 
 int bl_particle_set_params(bl_ibm g, int pid, const double* params, int nparams)
@@ -3048,17 +3258,10 @@ int bl_rbc_relax_nodes(bl_ibm g, int pid, double damping)
     BL_HIP(hipMemcpyAsync(d_active.p, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
     BL_HIP(hipMemcpyAsync(d_damp.p, damp.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
     BL_HIP(hipMemsetAsync(d_step.p, 0, sizeof(int), s));
-    const ForceView f = force_view(g);
     // the vertex forces in place are those of the current positions
     // (compute_forces before the first step); n_it = -1: vol_desired stays
     // as the parameters hold it
-    hipLaunchKernelGGL(k_calib_mass, dim3(nblk(g->NV)), dim3(256), 0, s, f, g->NV, (const int*)d_active.p, d_mass.p,
-                       d_step.p);
-    hipLaunchKernelGGL(k_calib_relax, dim3(nblk(g->NV)), dim3(256), 0, s, g->NV, (const int*)g->v_part,
-                       (const long*)g->p_voff, (const int*)d_active.p, (const double*)d_damp.p, (const double*)d_v.p,
-                       (const double*)d_v.p, -1, (const int*)d_step.p, (const double*)d_mass.p,
-                       (const double*)g->force, g->vel, g->pos, g->p_params);
-    BL_CHECK_LAUNCH();
+    if ((rc = calib_step(g, s, d_active.p, d_damp.p, d_v.p, d_v.p, -1, d_mass.p, d_step.p))) return rc;
     if ((rc = update_particles(g, 0, s)) || (rc = membrane_forces(g, s))) return rc;
     BL_HIP(hipStreamSynchronize(s));
     g->binned = false;
@@ -3075,9 +3278,8 @@ int bl_ibm_set_vertex_forces(bl_ibm g, const double* force)
         g->has_ext = false;
         return BL_OK;
     }
-    if (!g->ext_force) BL_HIP(hipMalloc(&g->ext_force, sizeof(double) * 3 * std::max<long>(g->NV, 1)));
-    BL_HIP(hipMemcpyAsync(g->ext_force, force, sizeof(double) * 3 * g->NV, hipMemcpyHostToDevice, s));
-    BL_HIP(hipStreamSynchronize(s));
+    if (!g->ext_force) BL_HIP(hipMalloc(&g->ext_force, (size_t)g->vreal * 3 * std::max<long>(g->NV, 1)));
+    if ((rc = vreal_upload(g, g->ext_force, 0, force, 3 * g->NV, s))) return rc;
     if (!g->has_ext) ibm_invalidate_graphs(g);
     g->has_ext = true;
     return BL_OK;

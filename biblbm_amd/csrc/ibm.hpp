@@ -75,10 +75,14 @@ struct bl_ibm_s {
     int* l_part = nullptr;
 
     // ---- vertices ----
-    double* pos = nullptr;
-    double* vel = nullptr;
-    double* force = nullptr;
-    double* ec = nullptr;  // per-element force contributions: 9 per triangle, 12 per pair, 3 per link
+    // Vertex state in `vreal`-byte reals: 8 (double) with FP64 populations
+    // or rigid particles, 4 (float) with FP32 and FP16-shifted populations
+    // (fixed at bl_ibm_init).  Host copies and the C-ABI stay FP64.
+    int vreal = 8;
+    void* pos = nullptr;
+    void* vel = nullptr;
+    void* force = nullptr;
+    void* ec = nullptr;  // per-element force contributions: 9 per triangle, 12 per pair, 3 per link
     long ec_pair = 0, ec_link = 0;  // offsets of the pair and link sections in ec
     void* ut = nullptr;    // staged velocity, (u, 0) per cell (FP64 for FP64 storage, else FP32)
     bool binned = false;   // tile lists match the current positions
@@ -89,9 +93,9 @@ struct bl_ibm_s {
     bool has_boundary = false;
     bl_boundary boundary{};
     unsigned char* v_near = nullptr;  // per vertex: nearer than half support + 1
-    double* v_blend = nullptr;        // per vertex: distance - half support
+    void* v_blend = nullptr;          // per vertex: distance - half support (vreal)
     int* v_base_t = nullptr;          // per vertex: top-hat stencil base (3)
-    double* v_w_t = nullptr;          // per vertex: top-hat weights (2 per axis)
+    void* v_w_t = nullptr;            // per vertex: top-hat weights (2 per axis, vreal)
 
     // ---- rigid particles (BL_MODEL_RIGID, rigid.hip) ----
     int n_rigid = 0;
@@ -109,14 +113,14 @@ struct bl_ibm_s {
     int* pp_next = nullptr;   // per vertex: next vertex of its cell's list
     int* pp_cell = nullptr;   // per vertex: collision cell, -1 if the vertex does not take part
     bl::PPRec* pp_rec = nullptr;  // per vertex: position, owner, successor in its cell (ascending id)
-    double* pp_force = nullptr;   // per vertex: the step's particle-particle forces (3)
-    double* ext_force = nullptr;  // per vertex: forces given by the host (bl_ibm_set_vertex_forces), 3
+    void* pp_force = nullptr;     // per vertex: the step's particle-particle forces (3, vreal)
+    void* ext_force = nullptr;    // per vertex: forces given by the host (bl_ibm_set_vertex_forces), 3 (vreal)
     bool has_ext = false;
     long pp_ncell = 0;
 
     // ---- stencil / tiles ----
     int* v_base = nullptr;    // 3 per vertex
-    double* v_w = nullptr;    // 12 per vertex (3 axes x up to 4 weights)
+    void* v_w = nullptr;      // 12 per vertex (3 axes x up to 4 weights, vreal)
     int* tile_count = nullptr;  // entries per tile (cleared by the spreading that consumes them)
     int* tile_list = nullptr;   // per tile a fixed-capacity list of the vertices whose stencil touches it
     int* active = nullptr;      // two lists of touched tiles, one per force-flag buffer, then the two
@@ -183,6 +187,10 @@ int need_stencil(bl_ibm g);
 // Rigid particles: state upload at bl_ibm_init (rigid.hip); BL_ERR_STATE
 // from the fluid-coupling stages while the boundary holds rigid particles.
 int rigid_init(bl_ibm g);
+// FP64 vertex data <-> the boundary's vertex reals (n values, stream order
+// on s; the host side is synchronised before return).
+int vreal_upload(bl_ibm g, void* dst, long off, const double* src, long n, hipStream_t s);
+int vreal_download(bl_ibm g, double* dst, const void* src, long off, long n, hipStream_t s);
 int rigid_refuse_coupling(bl_ibm g);
 // A full force field was loaded into the lattice (every tile flagged):
 // steps must run eagerly until the flags are consistent again.

@@ -37,22 +37,27 @@ namespace {
 
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-__global__ void __launch_bounds__(256) k_pp_bin(long NV, const double* __restrict__ pos, CollGrid g,
+// Vertex positions and forces are in the boundary's vertex reals R; the
+// handler's geometry and potentials are evaluated in FP64.
+template <class R>
+__global__ void __launch_bounds__(256) k_pp_bin(long NV, const R* __restrict__ pos, CollGrid g,
                                                 int* __restrict__ head, int* __restrict__ next,
                                                 int* __restrict__ cell)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= NV) return;
-    const int c = coll_cell(g, pos + 3 * v);
+    const double p[3] = {(double)pos[3 * v], (double)pos[3 * v + 1], (double)pos[3 * v + 2]};
+    const int c = coll_cell(g, p);
     cell[v] = c;
     if (c >= 0) next[v] = atomicExch(head + c, (int)v);
 }
 
 // Every listed vertex: its successor in its cell (the smallest larger id)
 // and, for the smallest id, the cell's first vertex.
+template <class R>
 __global__ void __launch_bounds__(256) k_pp_link(long NV, const int* __restrict__ cell,
                                                  const int* __restrict__ head, const int* __restrict__ next,
-                                                 const double* __restrict__ pos, const int* __restrict__ vpart,
+                                                 const R* __restrict__ pos, const int* __restrict__ vpart,
                                                  PPRec* __restrict__ rec, int* __restrict__ first)
 {
     const long vl = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,9 +72,9 @@ __global__ void __launch_bounds__(256) k_pp_link(long NV, const int* __restrict_
         if (w < v) smallest = false;
     }
     PPRec r;
-    r.x = pos[3 * vl];
-    r.y = pos[3 * vl + 1];
-    r.z = pos[3 * vl + 2];
+    r.x = (double)pos[3 * vl];
+    r.y = (double)pos[3 * vl + 1];
+    r.z = (double)pos[3 * vl + 2];
     r.obj = vpart[v];
     r.succ = succ;
     rec[v] = r;
@@ -155,10 +160,10 @@ constexpr int kPPLanes = 4;
 
 // out[v] = in[v] (0 without `in`) + the vertex's pair forces, for every
 // vertex (those outside the handler get in[v]).
+template <class R>
 __global__ void __launch_bounds__(256) k_pp_force(long NV, CollGrid g, bl_potential u, int has_wall, bl_boundary wall,
                                                   const int* __restrict__ cell, const int* __restrict__ first,
-                                                  const PPRec* __restrict__ rec, const double* in,
-                                                  double* out)
+                                                  const PPRec* __restrict__ rec, const R* in, R* out)
 {
 #pragma clang fp contract(off)
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,9 +172,9 @@ __global__ void __launch_bounds__(256) k_pp_force(long NV, CollGrid g, bl_potent
     const int c = vl < NV ? cell[vl] : -1;
     double f[3] = {0, 0, 0};
     if (in && lane == 0 && vl < NV) {
-        f[0] = in[3 * vl];
-        f[1] = in[3 * vl + 1];
-        f[2] = in[3 * vl + 2];
+        f[0] = (double)in[3 * vl];
+        f[1] = (double)in[3 * vl + 1];
+        f[2] = (double)in[3 * vl + 2];
     }
     double pv[3] = {0, 0, 0};
     const int NZ = g.num[2] + 2, NY = g.num[1] + 2;
@@ -234,9 +239,9 @@ __global__ void __launch_bounds__(256) k_pp_force(long NV, CollGrid g, bl_potent
             }
         }
     }
-    out[3 * vl] = f[0];
-    out[3 * vl + 1] = f[1];
-    out[3 * vl + 2] = f[2];
+    out[3 * vl] = (R)f[0];
+    out[3 * vl + 1] = (R)f[1];
+    out[3 * vl + 2] = (R)f[2];
 }
 
 __global__ void __launch_bounds__(256) k_pp_clear(long NV, const int* __restrict__ cell, int* __restrict__ head,
@@ -280,19 +285,29 @@ int check_potential(bl_ibm g, const bl_potential* u, const char* who, bool pp)
 
 // One collision-handler pass: out := in (0 without `in`) + pair forces (+
 // the handler's wall forces) of every vertex.
-int pp_pass(bl_ibm g, const bl_potential& u, const bl_boundary* wall, const double* in, double* out, hipStream_t s)
+template <class R>
+void pp_launch(bl_ibm g, const CollGrid& cg, const bl_potential& u, const bl_boundary* wall, const void* in,
+               void* out, hipStream_t s)
+{
+    const long NV = g->NV;
+    const bl_boundary w = wall ? *wall : bl_boundary{};
+    hipLaunchKernelGGL(k_pp_bin<R>, dim3(nblk(NV)), dim3(256), 0, s, NV, (const R*)g->pos, cg, g->pp_head,
+                       g->pp_next, g->pp_cell);
+    hipLaunchKernelGGL(k_pp_link<R>, dim3(nblk(NV)), dim3(256), 0, s, NV, (const int*)g->pp_cell,
+                       (const int*)g->pp_head, (const int*)g->pp_next, (const R*)g->pos, (const int*)g->v_part,
+                       g->pp_rec, g->pp_first);
+    hipLaunchKernelGGL(k_pp_force<R>, dim3(nblk(kPPLanes * NV)), dim3(256), 0, s, NV, cg, u, wall ? 1 : 0, w,
+                       (const int*)g->pp_cell, (const int*)g->pp_first, (const PPRec*)g->pp_rec, (const R*)in,
+                       (R*)out);
+}
+
+int pp_pass(bl_ibm g, const bl_potential& u, const bl_boundary* wall, const void* in, void* out, hipStream_t s)
 {
     const long NV = g->NV;
     if (NV == 0) return BL_OK;
     const CollGrid cg = grid_for(g, u);
-    const bl_boundary w = wall ? *wall : bl_boundary{};
-    hipLaunchKernelGGL(k_pp_bin, dim3(nblk(NV)), dim3(256), 0, s, NV, (const double*)g->pos, cg, g->pp_head,
-                       g->pp_next, g->pp_cell);
-    hipLaunchKernelGGL(k_pp_link, dim3(nblk(NV)), dim3(256), 0, s, NV, (const int*)g->pp_cell,
-                       (const int*)g->pp_head, (const int*)g->pp_next, (const double*)g->pos,
-                       (const int*)g->v_part, g->pp_rec, g->pp_first);
-    hipLaunchKernelGGL(k_pp_force, dim3(nblk(kPPLanes * NV)), dim3(256), 0, s, NV, cg, u, wall ? 1 : 0, w,
-                       (const int*)g->pp_cell, (const int*)g->pp_first, (const PPRec*)g->pp_rec, in, out);
+    if (g->vreal == 4) pp_launch<float>(g, cg, u, wall, in, out, s);
+    else pp_launch<double>(g, cg, u, wall, in, out, s);
     hipLaunchKernelGGL(k_pp_clear, dim3(nblk(NV)), dim3(256), 0, s, NV, (const int*)g->pp_cell, g->pp_head,
                        g->pp_first);
     BL_CHECK_LAUNCH();
@@ -327,7 +342,7 @@ int interactions_alloc(bl_ibm g)
     if (e == hipSuccess) e = hipMalloc(&g->pp_next, sizeof(int) * nv);
     if (e == hipSuccess) e = hipMalloc(&g->pp_cell, sizeof(int) * nv);
     if (e == hipSuccess) e = hipMalloc(&g->pp_rec, sizeof(PPRec) * nv);
-    if (e == hipSuccess) e = hipMalloc(&g->pp_force, sizeof(double) * 3 * nv);
+    if (e == hipSuccess) e = hipMalloc(&g->pp_force, (size_t)g->vreal * 3 * nv);
     if (e != hipSuccess) return fail(BL_ERR_HIP, std::string("collision grid: ") + hipGetErrorString(e));
     g->pp_ncell = ncell;
     hipStream_t s = g->lat->stream;

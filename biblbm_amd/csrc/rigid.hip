@@ -292,7 +292,7 @@ int rigid_move_launch(bl_ibm g, hipStream_t s)
                        (const long*)g->p_voff, (const double*)g->pos, (const double*)g->force, g->r_state, gd);
     hipLaunchKernelGGL(k_rigid_place, dim3(g->np), dim3(256), 0, s, (const int*)g->p_model, (const int*)g->p_shape,
                        (const long*)g->p_voff, (const int*)g->s_voff, (const double*)g->s_rest,
-                       (const double*)g->r_state, g->pos, 1, 1, -1);
+                       (const double*)g->r_state, (double*)g->pos, 1, 1, -1);
     BL_CHECK_LAUNCH();
     g->binned = false;
     return BL_OK;
@@ -304,7 +304,7 @@ int rigid_scale_launch(bl_ibm g, int pid, double scale, hipStream_t s)
                        g->r_state, pid, scale);
     hipLaunchKernelGGL(k_rigid_place, dim3(pid >= 0 ? 1 : g->np), dim3(256), 0, s, (const int*)g->p_model,
                        (const int*)g->p_shape, (const long*)g->p_voff, (const int*)g->s_voff,
-                       (const double*)g->s_rest, (const double*)g->r_state, g->pos, 1, 0, pid);
+                       (const double*)g->s_rest, (const double*)g->r_state, (double*)g->pos, 1, 0, pid);
     BL_CHECK_LAUNCH();
     g->binned = false;
     return BL_OK;
@@ -329,7 +329,7 @@ int rigid_init(bl_ibm g)
                        grid3_of(g->lat));
     hipLaunchKernelGGL(k_rigid_place, dim3(g->np), dim3(256), 0, s, (const int*)g->p_model, (const int*)g->p_shape,
                        (const long*)g->p_voff, (const int*)g->s_voff, (const double*)g->s_rest,
-                       (const double*)g->r_state, g->pos, 0, 1, -1);
+                       (const double*)g->r_state, (double*)g->pos, 0, 1, -1);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }
