@@ -236,20 +236,24 @@ def precision_leg(steps, checkpoints=(100, 200)):
     return out
 
 
-def suspension_leg(steps, warmup, horizons=(500, 1000, 2000)):
+def suspension_leg(steps, warmup, horizons=(500, 1000, 2000), tune_steps=4):
     """The 400-cell channel in FP64, FP32 and FP16-shifted storage from the
     same start (the immersed boundary in FP64 with FP64 populations, FP32
-    otherwise): speed over `steps` after `warmup`, then the runs continue to
-    each horizon (steps from the start) and the velocity field and vertex
+    otherwise): the immersed-boundary launch parameters are tuned first
+    (bl_tune_ibm, the same number of real steps in every storage), then the
+    speed over `steps` after `warmup`, then the runs continue to each
+    horizon (steps from the start) and the velocity field and vertex
     positions are compared with the FP64 run's there."""
     import numpy as np
-    from biblbm_amd import workloads
+    from biblbm_amd import tuning, workloads
     out = {}
     ref = {}
     for storage in ("f64", "f32", "f16s"):
         W = workloads.suspension(storage)
+        tuned = tuning.tune_ibm(W.ibm, tune_steps)["best"]
         dt = time_coupled(W, steps, warmup)
-        done = warmup + steps
+        done = W.lattice.steps
+        first = "at_%d_steps" % done
         errs = {}
         for h in [done] + [h for h in horizons if h > done]:
             if h > done:
@@ -267,9 +271,9 @@ def suspension_leg(steps, warmup, horizons=(500, 1000, 2000)):
         W.ibm.step()
         stages = {k: round(v, 5) for k, v in W.ibm.stage_times().items()}
         row = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
-                   ibm_stage_ms=stages)
+                   ibm_stage_ms=stages, ibm_launch=tuned)
         if storage != "f64":
-            row["velocity_rel_l2_vs_f64"] = errs["at_%d_steps" % (warmup + steps)]["velocity_rel_l2_vs_f64"]
+            row["velocity_rel_l2_vs_f64"] = errs[first]["velocity_rel_l2_vs_f64"]
             row["errors"] = errs
         out[storage] = row
         nv, vf = W.vertices, W.params["volume_fraction"]
@@ -279,8 +283,9 @@ def suspension_leg(steps, warmup, horizons=(500, 1000, 2000)):
     ref.clear()
     out["fp16s_over_fp32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
     out["workload"] = ("suspension 512x128x128 channel, 400 cells (300x642, 80x2562, 20x162 vertices), "
-                       "%.1f%% volume fraction, %d vertices, %d steps timed after %d warmup, errors against the "
-                       "FP64 run up to %d steps" % (100 * vf, nv, steps, warmup, max(horizons)))
+                       "%.1f%% volume fraction, %d vertices, %d steps timed after the launch tuner's steps and %d "
+                       "warmup, errors against the FP64 run up to %d steps" % (100 * vf, nv, steps, warmup,
+                                                                             max(horizons)))
     return out
 
 

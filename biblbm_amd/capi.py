@@ -79,6 +79,9 @@ SIGNATURES = {
     "bl_ibm_stage_times": [_P, _P],
     "bl_ibm_tile_stats": [_P, _P],
     "bl_ibm_set_fused_staging": [_P, _I],
+    "bl_ibm_set_launch": [_P, _P],
+    "bl_ibm_get_launch": [_P, _P],
+    "bl_tune_ibm": [_P, _I, ctypes.c_char_p, _I],
     "bl_ibm_set_layered": [_P, _I],
     "bl_shape_set_tag": [_P, _I, ctypes.c_char_p],
     "bl_rbc_shrink_volume": [_P, _I, _D, _I, _D, _P],

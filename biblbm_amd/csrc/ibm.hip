@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -752,7 +753,10 @@ __global__ void __launch_bounds__(256) k_elem_forces(ElemArgs<R> a) { elem_body<
 // reference applies before the membrane forces, and the sum ends with the
 // particle-particle forces (`fpp`), which it adds after them
 // (src/fsi/ImmersedBoundaryDynamics_forces.hh:293-301).
-constexpr int kGatherLanes = 4;
+// GL (2, 4 or 8) lanes per vertex, a launch parameter (bl_ibm_set_launch):
+// the lane count fixes the butterfly, so results agree to rounding across
+// lane counts and are bitwise repeatable for each.
+constexpr int kGatherLanes = 4;  // the default
 
 template <class R>
 struct GatherArgs {
@@ -765,7 +769,7 @@ struct GatherArgs {
     const R* ext;  // host-given vertex forces, added after the decorators (or null)
 };
 
-template <class R>
+template <class R, int GL>
 __device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
 {
     const ForceView<R>& f = A.f;
@@ -775,8 +779,8 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
     const DecoSet& deco = A.deco;
     const R* __restrict__ fpp = A.fpp;
     const long gid = bx * blockDim.x + threadIdx.x;
-    const long v = gid / kGatherLanes;
-    const int lane = (int)(gid % kGatherLanes);
+    const long v = gid / GL;
+    const int lane = (int)(gid % GL);
     R c0 = R(0), c1 = R(0), c2 = R(0);
     if (deco.n && lane == 0 && v < NV) {
         // the decorators' potentials and wall geometry in FP64
@@ -804,7 +808,7 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
         const R* El = ec + ec_link + 3 * f.p_loff[p];
         const R* Et = ec + 9 * f.p_toff[p];
         const R* Ep = ec + ec_pair + 12 * f.p_poff[p];
-        for (int e = lane; e < nl + nt + npr; e += kGatherLanes) {
+        for (int e = lane; e < nl + nt + npr; e += GL) {
             if (e < nl) {
                 const int code = Ll[e];
                 const R* x = El + 3 * (code >> 2);
@@ -828,7 +832,7 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
         }
     }
 #pragma unroll
-    for (int o = 1; o < kGatherLanes; o <<= 1) {
+    for (int o = 1; o < GL; o <<= 1) {
         c0 += __shfl_xor(c0, o);
         c1 += __shfl_xor(c1, o);
         c2 += __shfl_xor(c2, o);
@@ -845,8 +849,8 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
     }
 }
 
-template <class R>
-__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs<R> a) { gather_body<R>(blockIdx.x, a); }
+template <class R, int GL>
+__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs<R> a) { gather_body<R, GL>(blockIdx.x, a); }
 
 // ------------------------------------------------- velocity staging
 // The progs' computeVelocity pass (staged_velocity), restricted to the
@@ -1417,11 +1421,11 @@ __global__ void __launch_bounds__(256) k_bin_elem(BinArgs<R> b, int n1, ElemArgs
 
 // Vertex force gather (element forces) | velocity staging (the binning's
 // tiles, the last collision's field).
-template <class S, bool SWAPPED, class R>
+template <class S, bool SWAPPED, class R, int GL>
 __global__ void __launch_bounds__(256) k_gather_stage(GatherArgs<R> ga, int n1, StageArgs sa)
 {
     const int bx = blockIdx.x;
-    if (bx < n1) gather_body<R>(bx, ga);
+    if (bx < n1) gather_body<R, GL>(bx, ga);
     else stage_body<S, SWAPPED>(bx - n1, sa);
 }
 
@@ -1982,7 +1986,8 @@ int do_bin(bl_ibm g, hipStream_t s, hipEvent_t done = nullptr)
     if (g->NV)
         with_vreal(g, [&](auto tag) {
             using R = decltype(tag);
-            hipLaunchKernelGGL(k_bin<R>, dim3(nblk(g->NV)), dim3(256), 0, s, bin_args<R>(g));
+            const int bb = g->lp[BL_IBM_BIN_BLOCK];
+            hipLaunchKernelGGL(k_bin<R>, dim3(nblk(g->NV, bb)), dim3(bb), 0, s, bin_args<R>(g));
             return 0;
         });
     if (done) BL_HIP(hipEventRecord(done, s));
@@ -2058,17 +2063,18 @@ InterpArgs<R> interp_args(bl_ibm g, const ListReset& lr)
     return a;
 }
 
-unsigned interp_blocks(bl_ibm g) { return (unsigned)nblk((g->S == 2 ? 4 : 16) * g->NV); }
+unsigned interp_blocks(bl_ibm g, int blk = 256) { return (unsigned)nblk((g->S == 2 ? 4 : 16) * g->NV, blk); }
 
 template <class C, bool MOVE>
 void launch_interp_c(bl_ibm g, hipStream_t s, const ListReset& lr)
 {
     using R = C;  // the staged velocity's type is the vertex type
     const InterpArgs<R> a = interp_args<R>(g, lr);
-    const dim3 grid(interp_blocks(g));
-    if (g->S == 4) hipLaunchKernelGGL((k_interp<C, 4, MOVE, R>), grid, dim3(256), 0, s, a);
-    else if (g->S == 3) hipLaunchKernelGGL((k_interp<C, 3, MOVE, R>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_interp<C, 2, MOVE, R>), grid, dim3(256), 0, s, a);
+    const int ib = g->lp[BL_IBM_INTERP_BLOCK];
+    const dim3 grid(interp_blocks(g, ib));
+    if (g->S == 4) hipLaunchKernelGGL((k_interp<C, 4, MOVE, R>), grid, dim3(ib), 0, s, a);
+    else if (g->S == 3) hipLaunchKernelGGL((k_interp<C, 3, MOVE, R>), grid, dim3(ib), 0, s, a);
+    else hipLaunchKernelGGL((k_interp<C, 2, MOVE, R>), grid, dim3(ib), 0, s, a);
 }
 
 // The list reset carried by a launch that follows this step's spreading:
@@ -2124,10 +2130,15 @@ SpreadArgs<R> spread_args(bl_ibm g, unsigned& blocks)
     a.Fout = g->force_alt;
     // about one small-tile wave per touched tile and one workgroup per big
     // tile: the touched tiles are at most every tile and, for membranes,
-    // fewer than the vertices; big tiles fewer than a 32nd of the vertices
-    const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(g->NV, 1024)));
+    // fewer than the vertices (by default a wave per vertex); big tiles
+    // fewer than a 32nd of the vertices (by default a workgroup per 32
+    // vertices).  Both are launch parameters: the grid does not change any
+    // sum.
+    const long sw = (long)g->NV * g->lp[BL_IBM_SPREAD_SMALL] / 4;
+    const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(sw, 1024)));
     const int gs = (int)std::min<long>((small_waves + kSpreadWaves - 1) / kSpreadWaves, 4096);
-    const int gb = (int)std::max<long>(1, std::min<long>(L->ntiles, std::max<long>(g->NV / 32, 256)));
+    const int gb = (int)std::max<long>(
+        1, std::min<long>(L->ntiles, std::max<long>(g->NV / g->lp[BL_IBM_SPREAD_BIG], 256)));
     a.n_small = gs;
     blocks = (unsigned)(gs + gb);
     return a;
@@ -2214,13 +2225,22 @@ GatherArgs<R> gather_args(bl_ibm g, bool walls)
 
 // per-element membrane forces, then the per-vertex sum (`walls`: with the
 // decorators, particle-particle and host-given forces)
+template <class R, int GL>
+void launch_gather(bl_ibm g, hipStream_t s, bool walls)
+{
+    hipLaunchKernelGGL((k_vertex_gather<R, GL>), dim3(nblk(GL * g->NV)), dim3(256), 0, s, gather_args<R>(g, walls));
+}
+
 void launch_membrane(bl_ibm g, hipStream_t s, bool walls)
 {
+    const int eb = g->lp[BL_IBM_ELEM_BLOCK];
     with_vreal(g, [&](auto tag) {
         using R = decltype(tag);
-        hipLaunchKernelGGL(k_elem_forces<R>, dim3(nblk(g->NT + g->NP + g->NL)), dim3(256), 0, s, elem_args<R>(g));
-        hipLaunchKernelGGL(k_vertex_gather<R>, dim3(nblk(kGatherLanes * g->NV)), dim3(256), 0, s,
-                           gather_args<R>(g, walls));
+        hipLaunchKernelGGL(k_elem_forces<R>, dim3(nblk(g->NT + g->NP + g->NL, eb)), dim3(eb), 0, s, elem_args<R>(g));
+        const int gl = g->lp[BL_IBM_GATHER_LANES];
+        if (gl == 2) launch_gather<R, 2>(g, s, walls);
+        else if (gl == 8) launch_gather<R, 8>(g, s, walls);
+        else launch_gather<R, 4>(g, s, walls);
         return 0;
     });
 }
@@ -2428,6 +2448,19 @@ int step_fused(bl_ibm g)
 //   geometry (then voxelization, with a phase field)
 // The reference's order and every value are those of the other layouts;
 // no stage waits on another queue.
+// force gather (GL lanes per vertex) | velocity staging, one launch
+template <class S, class R, int GL>
+void launch_gather_stage(bl_ibm g, hipStream_t s)
+{
+    const int ng = (int)nblk(GL * g->NV);
+    const StageArgs sa = stage_args(g, false);
+    const dim3 g2(ng + stage_blocks(g));
+    if (g->lat->steps & 1)
+        hipLaunchKernelGGL((k_gather_stage<S, true, R, GL>), g2, dim3(256), 0, s, gather_args<R>(g, true), ng, sa);
+    else
+        hipLaunchKernelGGL((k_gather_stage<S, false, R, GL>), g2, dim3(256), 0, s, gather_args<R>(g, true), ng, sa);
+}
+
 template <class S>
 int step_layered_s(bl_ibm g)
 {
@@ -2439,16 +2472,14 @@ int step_layered_s(bl_ibm g)
     if (!g->pp.empty() && (rc = interactions_launch(g, s))) return rc;
     if (!g->prepped && (rc = prep_bin(g, s))) return rc;
     g->prepped = false;
-    const int nb = (int)nblk(g->NV), ne = (int)nblk(g->NT + g->NP + g->NL);
-    hipLaunchKernelGGL(k_bin_elem<R>, dim3(nb + ne), dim3(256), 0, s, bin_args<R>(g), nb, elem_args<R>(g));
+    const int bb = g->lp[BL_IBM_BIN_BLOCK];  // both halves of the paired launch
+    const int nb = (int)nblk(g->NV, bb), ne = (int)nblk(g->NT + g->NP + g->NL, bb);
+    hipLaunchKernelGGL(k_bin_elem<R>, dim3(nb + ne), dim3(bb), 0, s, bin_args<R>(g), nb, elem_args<R>(g));
     binned(g);
-    const int ng = (int)nblk(kGatherLanes * g->NV);
-    const StageArgs sa = stage_args(g, false);
-    const dim3 g2(ng + stage_blocks(g));
-    if (L->steps & 1)
-        hipLaunchKernelGGL((k_gather_stage<S, true, R>), g2, dim3(256), 0, s, gather_args<R>(g, true), ng, sa);
-    else
-        hipLaunchKernelGGL((k_gather_stage<S, false, R>), g2, dim3(256), 0, s, gather_args<R>(g, true), ng, sa);
+    const int gl = g->lp[BL_IBM_GATHER_LANES];
+    if (gl == 2) launch_gather_stage<S, R, 2>(g, s);
+    else if (gl == 8) launch_gather_stage<S, R, 8>(g, s);
+    else launch_gather_stage<S, R, 4>(g, s);
     unsigned nsp;
     const SpreadArgs<R> sp = spread_args<R>(g, nsp);
     spread_swap(g);
@@ -3396,6 +3427,31 @@ int bl_ibm_set_layered(bl_ibm g, int mode)
     if (mode < -1 || mode > 1) return fail(BL_ERR_ARG, "bl_ibm_set_layered: mode must be -1, 0 or 1");
     if (mode != g->layered) ibm_invalidate_graphs(g);
     g->layered = mode;
+    return BL_OK;
+}
+
+int bl_ibm_set_launch(bl_ibm g, const int params[BL_IBM_NLAUNCH])
+{
+    if (!g || !params) return fail(BL_ERR_ARG, "bl_ibm_set_launch: null argument");
+    auto in = [](int v, std::initializer_list<int> ok) { return std::find(ok.begin(), ok.end(), v) != ok.end(); };
+    const int* p = params;
+    if (!in(p[BL_IBM_BIN_BLOCK], {64, 128, 256}) || !in(p[BL_IBM_ELEM_BLOCK], {64, 128, 256}) ||
+        !in(p[BL_IBM_GATHER_LANES], {2, 4, 8}) || !in(p[BL_IBM_INTERP_BLOCK], {64, 128, 256}) ||
+        !in(p[BL_IBM_SPREAD_SMALL], {1, 2, 4, 8}) || !in(p[BL_IBM_SPREAD_BIG], {16, 32, 64, 128}))
+        return fail(BL_ERR_ARG, "bl_ibm_set_launch: a launch parameter is outside its allowed values");
+    bool same = true;
+    for (int k = 0; k < BL_IBM_NLAUNCH; ++k) same = same && g->lp[k] == p[k];
+    if (same) return BL_OK;
+    if (g->lat) BL_HIP(hipStreamSynchronize(g->lat->stream));
+    for (int k = 0; k < BL_IBM_NLAUNCH; ++k) g->lp[k] = p[k];
+    ibm_invalidate_graphs(g);
+    return BL_OK;
+}
+
+int bl_ibm_get_launch(bl_ibm g, int params[BL_IBM_NLAUNCH])
+{
+    if (!g || !params) return fail(BL_ERR_ARG, "bl_ibm_get_launch: null argument");
+    for (int k = 0; k < BL_IBM_NLAUNCH; ++k) params[k] = g->lp[k];
     return BL_OK;
 }
 

@@ -153,6 +153,12 @@ struct bl_ibm_s {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double stage_ms[5] = {0, 0, 0, 0, 0};
     bool timing = false;
+    // launch parameters of the immersed-boundary kernels (bl_ibm_set_launch,
+    // chosen by bl_tune_ibm), indexed by BL_IBM_*: threads per workgroup of
+    // the binning, element forces and interpolation, lanes per vertex of the
+    // force gather, small-tile spreading waves per 4 vertices, vertices per
+    // big-tile spreading workgroup
+    int lp[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32};
     int fused_staging = -1;  // velocity staging carried by the collision: -1 auto (large vertex counts), 0 off, 1 on
     int layered = -1;        // stages paired on one stream: -1 auto (small vertex counts), 0 off (forked), 1 on
 

@@ -1,5 +1,6 @@
 // The project's autotuner: launch shapes of the collide-and-stream kernel
-// per AA step parity, and the launch structure of coupled steps; results
+// per AA step parity, the launch structure of coupled steps and the
+// immersed-boundary kernels' launch parameters; results
 // kept in a per-process table that can be written to and read from a file
 // (one JSON object per line) and that a new lattice of a tuned size picks up
 // at creation.  The reference has no tuner: its block decomposition is fixed
@@ -8,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -35,6 +37,8 @@ struct CoupledEntry {
     long vertices = 0;
     int fused = -1, layered = -1;
     double ms = 0.0;
+    int lp[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32};  // bl_ibm_set_launch
+    bool has_lp = false;
 };
 
 std::vector<CollideEntry> g_collide;
@@ -62,6 +66,28 @@ void put_report(const std::string& rep, char* report, int cap)
         rep.copy(report, n);
         report[n] = '\0';
     }
+}
+
+// The table entry of a boundary (device, lattice size, storage, vertex
+// count), created with the boundary's current settings if absent.
+CoupledEntry& coupled_entry(bl_ibm g)
+{
+    const bl_lattice L = g->lat;
+    int nv = 0;
+    bl_ibm_count(g, nullptr, &nv);
+    const std::string dev = device_arch(L->device);
+    for (CoupledEntry& c : g_coupled)
+        if (c.device == dev && c.nx == L->nx && c.ny == L->ny && c.nz == L->nz && c.storage == L->storage &&
+            c.vertices == nv)
+            return c;
+    g_coupled.emplace_back();
+    CoupledEntry& e = g_coupled.back();
+    e.device = dev;
+    e.nx = L->nx; e.ny = L->ny; e.nz = L->nz; e.storage = L->storage;
+    e.vertices = nv;
+    e.fused = g->fused_staging;
+    e.layered = g->layered;
+    return e;
 }
 
 }  // namespace
@@ -252,24 +278,86 @@ int bl_tune_coupled(bl_ibm g, int n_steps, char* report, int cap)
     snprintf(buf, sizeof(buf), "], \"steps_run\": %d, \"best\": {\"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f}}",
              (int)(2 * trials.size()) * n_steps, best.fused, best.layered, best.ms);
     rep += buf;
-    int nv = 0;
-    bl_ibm_count(g, nullptr, &nv);
-    const std::string dev = device_arch(L->device);
-    CoupledEntry* e = nullptr;
-    for (CoupledEntry& c : g_coupled)
-        if (c.device == dev && c.nx == L->nx && c.ny == L->ny && c.nz == L->nz && c.storage == L->storage &&
-            c.vertices == nv)
-            e = &c;
-    if (!e) {
-        g_coupled.emplace_back();
-        e = &g_coupled.back();
-        e->device = dev;
-        e->nx = L->nx; e->ny = L->ny; e->nz = L->nz; e->storage = L->storage;
-        e->vertices = nv;
+    CoupledEntry& e = coupled_entry(g);
+    e.fused = best.fused;
+    e.layered = best.layered;
+    e.ms = best.ms;
+    put_report(rep, report, cap);
+    return BL_OK;
+}
+
+int bl_tune_ibm(bl_ibm g, int n_steps, char* report, int cap)
+{
+    if (!g || n_steps < 2 || n_steps % 2) return fail(BL_ERR_ARG, "bl_tune_ibm: n_steps must be even and >= 2");
+    int rc = need_stencil(g);
+    if (rc) return rc;
+    bl_lattice L = g->lat;
+    static const std::vector<int> values[BL_IBM_NLAUNCH] = {
+        {64, 128, 256}, {64, 128, 256}, {2, 4, 8}, {64, 128, 256}, {1, 2, 4, 8}, {16, 32, 64, 128}};
+    static const char* names[BL_IBM_NLAUNCH] = {"bin_block", "elem_block", "gather_lanes", "interp_block",
+                                                "spread_small", "spread_big"};
+    int keep[BL_IBM_NLAUNCH], best[BL_IBM_NLAUNCH];
+    if ((rc = bl_ibm_get_launch(g, keep))) return rc;
+    std::copy(keep, keep + BL_IBM_NLAUNCH, best);
+    hipEvent_t a, b;
+    BL_HIP(hipEventCreate(&a));
+    BL_HIP(hipEventCreate(&b));
+    // time of n_steps with parameters p (a first run captures the graphs)
+    auto time_of = [&](const int* p, double& ms) -> int {
+        int r;
+        if ((r = bl_ibm_set_launch(g, p)) || (r = bl_coupled_steps(g, n_steps, 1))) return r;
+        hipEventRecord(a, L->stream);
+        if ((r = bl_coupled_steps(g, n_steps, 1))) return r;
+        hipEventRecord(b, L->stream);
+        hipEventSynchronize(b);
+        float t = 0.f;
+        hipEventElapsedTime(&t, a, b);
+        ms = (double)t / n_steps;
+        return BL_OK;
+    };
+    double best_ms = 0.0;
+    int runs = 1;
+    rc = time_of(best, best_ms);
+    std::string rep = "{\"trials\": [";
+    bool first = true;
+    for (int pass = 0; pass < 2 && rc == BL_OK; ++pass)
+        for (int k = 0; k < BL_IBM_NLAUNCH && rc == BL_OK; ++k)
+            for (int v : values[k]) {
+                if (v == best[k]) continue;
+                int p[BL_IBM_NLAUNCH];
+                std::copy(best, best + BL_IBM_NLAUNCH, p);
+                p[k] = v;
+                double ms = 0.0;
+                if ((rc = time_of(p, ms))) break;
+                ++runs;
+                char buf[128];
+                snprintf(buf, sizeof(buf), "%s{\"%s\": %d, \"ms_per_step\": %.6f}", first ? "" : ", ", names[k], v, ms);
+                rep += buf;
+                first = false;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best[k] = v;
+                }
+            }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) {
+        bl_ibm_set_launch(g, keep);
+        return rc;
     }
-    e->fused = best.fused;
-    e->layered = best.layered;
-    e->ms = best.ms;
+    if ((rc = bl_ibm_set_launch(g, best))) return rc;
+    rep += "], \"steps_run\": " + std::to_string(2 * runs * n_steps) + ", \"best\": {";
+    for (int k = 0; k < BL_IBM_NLAUNCH; ++k) {
+        char buf[64];
+        snprintf(buf, sizeof(buf), "\"%s\": %d, ", names[k], best[k]);
+        rep += buf;
+    }
+    char buf[64];
+    snprintf(buf, sizeof(buf), "\"ms_per_step\": %.6f}}", best_ms);
+    rep += buf;
+    CoupledEntry& e = coupled_entry(g);
+    std::copy(best, best + BL_IBM_NLAUNCH, e.lp);
+    e.has_lp = true;
     put_report(rep, report, cap);
     return BL_OK;
 }
@@ -285,7 +373,9 @@ int bl_tune_apply_coupled(bl_ibm g)
         if (c.device == dev && c.nx == L->nx && c.ny == L->ny && c.nz == L->nz && c.storage == L->storage &&
             c.vertices == nv) {
             int rc = bl_ibm_set_fused_staging(g, c.fused);
-            return rc ? rc : bl_ibm_set_layered(g, c.layered);
+            if (!rc) rc = bl_ibm_set_layered(g, c.layered);
+            if (!rc && c.has_lp) rc = bl_ibm_set_launch(g, c.lp);
+            return rc;
         }
     return BL_OK;
 }
@@ -308,8 +398,10 @@ int bl_tune_cache_save(const char* path)
         char buf[512];
         snprintf(buf, sizeof(buf),
                  "{\"kind\": \"coupled\", \"device\": \"%s\", \"nx\": %d, \"ny\": %d, \"nz\": %d, \"storage\": %d, "
-                 "\"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f}",
-                 e.device.c_str(), e.nx, e.ny, e.nz, e.storage, e.vertices, e.fused, e.layered, e.ms);
+                 "\"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f, "
+                 "\"ibm_launch\": [%d, %d, %d, %d, %d, %d]}",
+                 e.device.c_str(), e.nx, e.ny, e.nz, e.storage, e.vertices, e.fused, e.layered, e.ms,
+                 e.has_lp ? e.lp[0] : 0, e.lp[1], e.lp[2], e.lp[3], e.lp[4], e.lp[5]);
         out << buf << "\n";
     }
     return out ? BL_OK : fail(BL_ERR_ARG, std::string("bl_tune_cache_save: write failed: ") + path);
@@ -353,11 +445,27 @@ int bl_tune_cache_load(const char* path)
         } else if (sscanf(line.c_str(),
                           "{\"kind\": \"coupled\", \"device\": \"%127[^\"]\", \"nx\": %d, \"ny\": %d, \"nz\": %d, "
                           "\"storage\": %d, \"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, "
-                          "\"ms_per_step\": %lf}",
-                          dev, &d.nx, &d.ny, &d.nz, &d.storage, &d.vertices, &d.fused, &d.layered, &d.ms) == 9) {
+                          "\"ms_per_step\": %lf, \"ibm_launch\": [%d, %d, %d, %d, %d, %d]}",
+                          dev, &d.nx, &d.ny, &d.nz, &d.storage, &d.vertices, &d.fused, &d.layered, &d.ms, &d.lp[0],
+                          &d.lp[1], &d.lp[2], &d.lp[3], &d.lp[4], &d.lp[5]) == 15) {
             if (d.fused < -1 || d.fused > 1 || d.layered < -1 || d.layered > 1) {
                 ++bad;
                 continue;
+            }
+            // ibm_launch [0, ...]: no launch parameters tuned for this entry
+            d.has_lp = d.lp[0] != 0;
+            auto in = [](int v, std::initializer_list<int> ok) {
+                return std::find(ok.begin(), ok.end(), v) != ok.end();
+            };
+            if (d.has_lp && !(in(d.lp[0], {64, 128, 256}) && in(d.lp[1], {64, 128, 256}) && in(d.lp[2], {2, 4, 8}) &&
+                              in(d.lp[3], {64, 128, 256}) && in(d.lp[4], {1, 2, 4, 8}) &&
+                              in(d.lp[5], {16, 32, 64, 128}))) {
+                ++bad;
+                continue;
+            }
+            if (!d.has_lp) {
+                const int dflt[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32};
+                std::copy(dflt, dflt + BL_IBM_NLAUNCH, d.lp);
             }
             d.device = dev;
             bool found = false;

@@ -245,6 +245,32 @@ class ImmersedBoundary:
         one stream with paired stages (bl_ibm_set_layered)."""
         call("bl_ibm_set_layered", self.handle, int(mode))
 
+    LAUNCH_KEYS = ("bin_block", "elem_block", "gather_lanes", "interp_block", "spread_small", "spread_big")
+
+    def set_launch(self, **params):
+        """Immersed-boundary launch parameters (bl_ibm_set_launch), by name;
+        the ones not given keep their values."""
+        cur = self.launch()
+        for k, v in params.items():
+            if k not in cur:
+                raise KeyError(k)
+            cur[k] = int(v)
+        arr = np.array([cur[k] for k in self.LAUNCH_KEYS], np.int32)
+        call("bl_ibm_set_launch", self.handle, ptr(arr))
+
+    def launch(self):
+        out = np.zeros(len(self.LAUNCH_KEYS), np.int32)
+        call("bl_ibm_get_launch", self.handle, ptr(out))
+        return dict(zip(self.LAUNCH_KEYS, (int(x) for x in out)))
+
+    def tune_launch(self, n_steps=8):
+        """bl_tune_ibm: coordinate descent over the launch parameters with
+        real steps; keeps and records the fastest."""
+        import json
+        buf = ctypes.create_string_buffer(16384)
+        call("bl_tune_ibm", self.handle, int(n_steps), buf, len(buf))
+        return json.loads(buf.value.decode())
+
     def tile_stats(self):
         out = np.zeros(2, np.int64)
         call("bl_ibm_tile_stats", self.handle, ptr(out))

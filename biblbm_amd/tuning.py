@@ -7,6 +7,9 @@ decomposition is fixed, src/multiBlock/defaultMultiBlockPolicy3D.h:119-129).
 - tune_coupled: the launch structure of coupled steps (velocity staging
   carried by the collision or not, stages paired on one stream or forked);
   every structure gives the same results, the trials are real steps.
+- tune_ibm: the immersed-boundary kernels' launch parameters (workgroup
+  sizes of binning, element forces and interpolation, gather lanes per
+  vertex, spreading grid), coordinate descent over real steps.
 - save / load: the process's tuning table as JSON lines (key: device,
   lattice size, storage, and for coupled steps the vertex count); a lattice
   created after a load starts with the tuned shapes of its size, and
@@ -39,6 +42,10 @@ def apply_collide(lattice, best):
 
 def tune_coupled(ibm, n_steps=8):
     return ibm.tune(n_steps)
+
+
+def tune_ibm(ibm, n_steps=4):
+    return ibm.tune_launch(n_steps)
 
 
 def apply_coupled(ibm):

@@ -270,6 +270,28 @@ int bl_ibm_set_fused_staging(bl_ibm g, int mode);
  * spreading onto a second stream; -1 (default) picks 1 below 65 536
  * vertices.  The results are the same either way. */
 int bl_ibm_set_layered(bl_ibm g, int mode);
+/* Launch parameters of the immersed-boundary kernels (no counterpart in
+ * the reference, whose loops are serial), indexed by BL_IBM_*:
+ *   BL_IBM_BIN_BLOCK     threads per binning workgroup (64, 128, 256;
+ *                        also the paired binning | element-force launch)
+ *   BL_IBM_ELEM_BLOCK    threads per element-force workgroup (64..256)
+ *   BL_IBM_GATHER_LANES  lanes per vertex of the force gather (2, 4, 8)
+ *   BL_IBM_INTERP_BLOCK  threads per interpolation workgroup (64..256)
+ *   BL_IBM_SPREAD_SMALL  small-tile spreading waves per 4 vertices (1..8)
+ *   BL_IBM_SPREAD_BIG    vertices per big-tile spreading workgroup (16..128)
+ * Defaults {256, 256, 4, 256, 4, 32}.  Only the gather lane count changes
+ * an order of summation (its butterfly): results agree to rounding across
+ * lane counts; every other parameter leaves them bitwise the same.
+ * bl_ibm_set_launch refuses values outside those sets (BL_ERR_ARG). */
+#define BL_IBM_BIN_BLOCK 0
+#define BL_IBM_ELEM_BLOCK 1
+#define BL_IBM_GATHER_LANES 2
+#define BL_IBM_INTERP_BLOCK 3
+#define BL_IBM_SPREAD_SMALL 4
+#define BL_IBM_SPREAD_BIG 5
+#define BL_IBM_NLAUNCH 6
+int bl_ibm_set_launch(bl_ibm g, const int params[BL_IBM_NLAUNCH]);
+int bl_ibm_get_launch(bl_ibm g, int params[BL_IBM_NLAUNCH]);
 
 /* ------------------------------------------------ initial conditions */
 /* shrink_rbc_volume (progs/common/calibrate_rbc_shape.h:13-72) on the
@@ -488,7 +510,15 @@ int bl_tune_collide(bl_lattice lat, char *report, int cap);
  * results; the trials are real steps: the simulation advances by
  * 2 n_steps per layout tried (the report says how many). */
 int bl_tune_coupled(bl_ibm g, int n_steps, char *report, int cap);
-/* Applies the tuning table's launch structure for this boundary, if any. */
+/* Times n_steps (even) coupled steps for the immersed-boundary launch
+ * parameters of bl_ibm_set_launch, one parameter at a time over its
+ * allowed values with the best of the others kept (coordinate descent,
+ * two passes), keeps the fastest and records it in the tuning table entry
+ * of bl_tune_coupled's key.  The trials are real steps (the report says how
+ * many); with the gather lane count the trajectory changes at rounding. */
+int bl_tune_ibm(bl_ibm g, int n_steps, char *report, int cap);
+/* Applies the tuning table's launch structure and immersed-boundary launch
+ * parameters for this boundary, if any. */
 int bl_tune_apply_coupled(bl_ibm g);
 /* The tuning table to / from a file, one JSON object per line; loading
  * merges (an entry of the same key replaces the one held) and refuses

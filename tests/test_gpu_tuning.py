@@ -1,7 +1,9 @@
 """The autotuner (bl_tune_*), through the C-ABI: the collide tuner's trials
 leave the lattice as it was and the tuned per-parity shapes step bitwise
 like the default; the coupled-step tuner's layouts give bitwise the same
-trajectory; the tuning table survives a save / load and a lattice of the
+trajectory; the immersed-boundary launch parameters keep the trajectory
+bitwise (the gather lane count to rounding) and their tuner's choice is
+persisted; the tuning table survives a save / load and a lattice of the
 tuned size starts with the tuned shapes."""
 import numpy as np
 import pytest
@@ -158,3 +160,95 @@ def test_row_orders_are_bitwise(device, shape, periodic, block, storage):
         L.close()
     for r in runs[1:]:
         assert np.array_equal(runs[0], r)
+
+
+def _small_suspension(storage="f64", seed=3):
+    from biblbm_amd import workloads
+    return workloads.suspension(storage, nx=64, ny=40, nz=48, n642=2, n2562=1, n162=2, cell_radius=8.0, seed=seed)
+
+
+ORDER_KEEPING = [dict(bin_block=64), dict(bin_block=128), dict(elem_block=64), dict(elem_block=128),
+                 dict(interp_block=64), dict(interp_block=128), dict(spread_small=1), dict(spread_small=8),
+                 dict(spread_big=16), dict(spread_big=128)]
+
+
+@pytest.mark.parametrize("fused,layered", [(0, 1), (1, 0), (0, 0)])
+def test_ibm_launch_parameters_are_bitwise(device, fused, layered):
+    """Every immersed-boundary launch parameter except the gather lane count
+    leaves the trajectory bitwise the same, for each step layout (paired
+    stages on one stream, staging carried by the collision, forked)."""
+    ref = _small_suspension()
+    ref.ibm.set_fused_staging(fused)
+    ref.ibm.set_layered(layered)
+    ref.ibm.coupled_steps(6, use_graph=True)
+    x_ref, f_ref = ref.ibm.positions(), ref.lattice.get_populations()
+    ref.ibm.close()
+    ref.lattice.close()
+    for params in ORDER_KEEPING:
+        W = _small_suspension()
+        W.ibm.set_fused_staging(fused)
+        W.ibm.set_layered(layered)
+        W.ibm.set_launch(**params)
+        W.ibm.coupled_steps(2, use_graph=False)
+        W.ibm.coupled_steps(4, use_graph=True)
+        assert np.array_equal(W.ibm.positions(), x_ref), params
+        assert np.array_equal(W.lattice.get_populations(), f_ref), params
+        W.ibm.close()
+        W.lattice.close()
+
+
+@pytest.mark.parametrize("lanes", [2, 8])
+def test_gather_lane_counts_agree_to_rounding(device, lanes):
+    """2 or 8 lanes per vertex in the force gather change only the order of
+    the per-vertex sums: FP64 trajectories agree with the 4-lane default to
+    1e-12 relative, and each lane count is bitwise repeatable."""
+    runs = []
+    for gl in (4, lanes, lanes):
+        W = _small_suspension()
+        W.ibm.set_launch(gather_lanes=gl)
+        p0 = W.ibm.positions()
+        W.ibm.coupled_steps(6, use_graph=True)
+        runs.append((W.ibm.positions(), p0))
+        W.ibm.close()
+        W.lattice.close()
+    (x4, p0), (xa, _), (xb, _) = runs
+    assert np.array_equal(xa, xb)
+    assert np.linalg.norm(xa - x4) <= 1e-12 * np.linalg.norm(x4 - p0)
+
+
+def test_ibm_launch_rejects_values_outside_the_sets(device):
+    from biblbm_amd import capi
+    W = _small_suspension()
+    before = W.ibm.launch()
+    for bad in (dict(bin_block=512), dict(gather_lanes=3), dict(spread_small=16), dict(spread_big=8),
+                dict(interp_block=32), dict(elem_block=0)):
+        with pytest.raises(capi.BlError):
+            W.ibm.set_launch(**bad)
+        assert W.ibm.launch() == before
+    W.ibm.close()
+    W.lattice.close()
+
+
+def test_ibm_tuner_persists_and_applies(device, tmp_path):
+    """bl_tune_ibm picks launch parameters from its trials (the report
+    counts the steps it ran), records them under the boundary's key, and a
+    saved table applied to a fresh boundary of the same workload sets them."""
+    from biblbm_amd import tuning
+    W = _small_suspension("f32")
+    s0 = W.lattice.steps
+    rep = W.ibm.tune_launch(2)
+    best = rep["best"]
+    assert rep["steps_run"] == 2 * 2 * (1 + len(rep["trials"]))
+    assert W.ibm.launch() == {k: best[k] for k in W.ibm.LAUNCH_KEYS}
+    assert W.lattice.steps - s0 == rep["steps_run"]
+    path = tmp_path / "tune.jsonl"
+    tuning.save(str(path))
+    assert '"ibm_launch": [' in path.read_text()
+    W.ibm.close()
+    W.lattice.close()
+    assert tuning.load(str(path))
+    W2 = _small_suspension("f32")
+    tuning.apply_coupled(W2.ibm)
+    assert W2.ibm.launch() == {k: best[k] for k in W2.ibm.LAUNCH_KEYS}
+    W2.ibm.close()
+    W2.lattice.close()
