@@ -54,9 +54,9 @@ namespace {
 // levels and with the snapshot described at bl_guo_wall_s, which give the
 // result of the reference's node-by-node pass.  Arithmetic in FP64
 // whatever the storage.
-__device__ __forceinline__ double feq_d(int i, double rhoBar, const double* j, double jSqr)
+// invRho = 1 / (rhoBar + 1), formed once per cell by the caller
+__device__ __forceinline__ double feq_d(int i, double rhoBar, double invRho, const double* j, double jSqr)
 {
-    const double invRho = 1.0 / (rhoBar + 1.0);
     const double c_j = cx(i) * j[0] + cy(i) * j[1] + cz(i) * j[2];
     return tw(i) * (rhoBar + 3. * c_j + invRho * (4.5 * c_j * c_j - 1.5 * jSqr));
 }
@@ -117,11 +117,12 @@ __device__ void guo_node(const LatView& v, const GuoNodes& g, long k, int x, int
     const double jsw = jw[0] * jw[0] + jw[1] * jw[1] + jw[2] * jw[2];
     const unsigned char m = v.mask ? v.mask[c] : (unsigned char)M_FLUID;
     const bool has_eq = m == M_FLUID || m == M_VWALL;
+    const double inv1 = 1.0 / (rb1 + 1.0), inv2 = 1.0 / (rb2 + 1.0);
     C fw[Q];
     for (int p = 0; p < Q; ++p) {
-        const double fneq1 = f1[p] - feq_d(p, rb1, j1, j1sqr);
-        const double fneqw = dl < 0.75 ? dl * fneq1 + (1. - dl) * (f2[p] - feq_d(p, rb2, j2, j2sqr)) : fneq1;
-        fw[p] = (C)((has_eq ? feq_d(p, rb1, jw, jsw) : 0.0) + fneqw);
+        const double fneq1 = f1[p] - feq_d(p, rb1, inv1, j1, j1sqr);
+        const double fneqw = dl < 0.75 ? dl * fneq1 + (1. - dl) * (f2[p] - feq_d(p, rb2, inv2, j2, j2sqr)) : fneq1;
+        fw[p] = (C)((has_eq ? feq_d(p, rb1, inv1, jw, jsw) : 0.0) + fneqw);
     }
     Nbr nb;
     nb.init(v, x, y, z);

@@ -144,11 +144,54 @@ def traffic(fetch_dir, write_dir, copy_bytes, kernel_substr):
     return total
 
 
+def traffic_table(fetch_dir, write_dir, copy_bytes, trace_dir, top=20):
+    """Per-kernel HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE scaled as
+    in `traffic`, medians over dispatches) with the average duration of the
+    same kernel from a --kernel-trace --stats run of the same command, and
+    the resulting bandwidth against 8 TB/s."""
+    cb = float(copy_bytes)
+    per = {}
+    for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _csv_rows(d, "counter_collection.csv")
+        disp = {}
+        for r in rows:
+            if r["Counter_Name"] != name:
+                continue
+            key = (r["Kernel_Name"], r["Dispatch_Id"])
+            disp[key] = disp.get(key, 0.0) + float(r["Counter_Value"])
+        byk = {}
+        for (k, _), v in disp.items():
+            byk.setdefault(k, []).append(v)
+        copy = [v for k, vs in byk.items() if "k_copy16" in k for v in vs]
+        if not copy:
+            raise SystemExit("no %s samples for k_copy16 in %s" % (name, d))
+        factor = cb / (sorted(copy)[len(copy) // 2] * 1024.0)
+        for k, vs in byk.items():
+            if "k_copy16" in k:
+                continue
+            per.setdefault(k, {})[name] = sorted(vs)[len(vs) // 2] * 1024.0 * factor
+    dur = {r["Name"]: float(r["AverageNs"]) for r in _csv_rows(trace_dir, "kernel_stats.csv")}
+    rows = []
+    for k, v in per.items():
+        f, w = v.get("FETCH_SIZE", 0.0), v.get("WRITE_SIZE", 0.0)
+        ns = dur.get(k)
+        gbs = (f + w) / ns if ns else float("nan")
+        rows.append((k, f, w, ns, gbs))
+    rows.sort(key=lambda r: -(r[3] or 0))
+    print("| kernel | fetch MB | write MB | avg us | GB/s | % of 8 TB/s |")
+    print("|---|---|---|---|---|---|")
+    for k, f, w, ns, gbs in rows[:top]:
+        print("| %s | %.2f | %.2f | %s | %.0f | %.1f |" % (
+            k[:90].replace("|", "/"), f / 1e6, w / 1e6, "%.2f" % (ns / 1e3) if ns else "-", gbs, 100 * gbs / 8000.0))
+
+
 def main(argv):
     if argv and argv[0] == "stats":
         stats(argv[1], int(argv[2]) if len(argv) > 2 else 25)
     elif argv and argv[0] == "traffic":
         traffic(argv[1], argv[2], argv[3], argv[4])
+    elif argv and argv[0] == "traffic_table":
+        traffic_table(argv[1], argv[2], argv[3], argv[4], int(argv[5]) if len(argv) > 5 else 20)
     else:
         run(argv)
 
