@@ -315,22 +315,67 @@ def interactions_leg(steps, warmup, f32_ms=None):
     return out
 
 
+def ensemble_oracle_check(boxes=(0, 1), steps=20):
+    """Boxes of the ensemble configuration (the first two draws), stepped as
+    a batched ensemble of their own, against the FP64 oracle's composed
+    steps from the same start: vertex positions rel-L2 and the Taylor
+    deformation index of each box."""
+    import numpy as np
+    from biblbm_amd import meshes, workloads
+    from biblbm_amd.ensemble import Ensemble
+    from biblbm_amd.ibm import taylor_deformation
+    from oracle import oracle_py as O
+    params = workloads.shear_ensemble_params()
+    ws = [workloads.shear_box("f64", s, tau, Ca, seed=bs) for s, tau, Ca, bs in (params[k] for k in boxes)]
+    starts = [(w.lattice.get_populations(), w.ibm.positions()) for w in ws]
+    ens = Ensemble([(w.lattice, w.ibm) for w in ws])
+    ens.steps(steps)
+    out = []
+    for w, (f, pos0) in zip(ws, starts):
+        shape = w.lattice.shape
+        mask = np.zeros(shape, np.uint8)
+        mask[:, 0, :] = mask[:, -1, :] = 3
+        u_w = w.params["wall_u"]
+        wall_u = np.array([-u_w, 0, 0, u_w, 0, 0])
+        R, G = w.params["radius"], w.params["G"]
+        v, t = meshes.biconcave(3, R)
+        p = O.Particle(v, t, 3, workloads.skalak_rbc_params(G, 10.0, R, meshes.signed_volume(v, t), 0.02 * G * R * R),
+                       pos0)
+        force = np.zeros(shape + (3,))
+        om = 1 / w.params["tau"]
+        for _ in range(steps):
+            f, force, _ = O.coupled_step(f, force, np.zeros(shape), mask, (1, 0, 1), om, om, (0, 0, 0), [p], "peskin",
+                                         1, wall_u)
+        gpos = w.ibm.positions()
+        D_gpu = taylor_deformation(w.ibm.observables(0)["inertia"])
+        D_ref = taylor_deformation(O.moment_of_inertia(t, p.pos))
+        out.append(dict(side=w.params["s"], steps=steps,
+                        position_rel_l2=float(np.linalg.norm(gpos - p.pos) / np.linalg.norm(p.pos - pos0)),
+                        deformation_index=D_gpu, deformation_index_abs_diff_vs_oracle=float(abs(D_gpu - D_ref))))
+    ens.close()
+    for w in ws:
+        w.ibm.close()
+        w.lattice.close()
+    return out
+
+
 def ensemble_leg(steps, warmup, n_boxes=64):
     """64 independent shear boxes (sides from {32..64}^3, one 642-vertex cell
-    each) advanced by one hipGraph launch per two steps, against the same
-    boxes stepped one after another (each with its own captured graph)."""
+    each): every stage of a step one launch over all boxes (batched), two
+    steps per hipGraph launch, the batched collide's workgroup order from
+    the ensemble tuner; against the same boxes stepped one after another
+    (each with its own captured graph), and two boxes against the oracle."""
     import numpy as np
-    from biblbm_amd import capi, tuning, workloads
+    from biblbm_amd import capi, workloads
     from biblbm_amd.ensemble import Ensemble
     from biblbm_amd.ibm import taylor_deformation
     ws = workloads.shear_ensemble("f64", n_boxes=n_boxes)
     cells = sum(w.cells for w in ws)
-    best = tuning.tune_collide(ws[0].lattice)["best"]
-    for w in ws:
-        tuning.apply_collide(w.lattice, best)
     ens = Ensemble([(w.lattice, w.ibm) for w in ws])
     ens.steps(warmup)
+    tuned = ens.tune(4)
     ms = ens.time_steps(steps)
+    batched = ens.batched
     for w in ws:
         w.ibm.coupled_steps(warmup, use_graph=True)
         w.ibm.prepare(steps)
@@ -342,16 +387,17 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     loop_s = time.perf_counter() - t0
     D = np.array([taylor_deformation(w.ibm.observables(0)["inertia"]) for w in ws])
     out = dict(mlups=round(cells * steps / (ms * 1e-3) / 1e6, 1), ms_per_step=round(ms / steps, 4),
+               batched=batched, tuned_collide_order=tuned["best"],
                per_box_loop_mlups=round(cells * steps / loop_s / 1e6, 1),
                deformation_index=dict(min=float(D.min()), median=float(np.median(D)), max=float(D.max())),
-               tuned_launch=best,
                workload="%d shear boxes, %d cells, sides %s, one 642-vertex biconcave cell each, FP64, "
-                        "steps %d after %d warmup" % (n_boxes, cells, sorted({w.params["s"] for w in ws}), steps,
-                                                      warmup))
+                        "steps %d after %d warmup and the tuner's steps" %
+                        (n_boxes, cells, sorted({w.params["s"] for w in ws}), steps, warmup))
     ens.close()
     for w in ws:
         w.ibm.close()
         w.lattice.close()
+    out["vs_oracle"] = ensemble_oracle_check()
     return out
 
 

@@ -80,6 +80,9 @@ SIGNATURES = {
     "bl_ibm_tile_stats": [_P, _P],
     "bl_ibm_set_fused_staging": [_P, _I],
     "bl_ibm_set_launch": [_P, _P],
+    "bl_ensemble_set_batched": [_P, _I],
+    "bl_ensemble_is_batched": [_P, _PI],
+    "bl_ensemble_tune": [_P, _I, ctypes.c_char_p, _I],
     "bl_ibm_get_launch": [_P, _P],
     "bl_tune_ibm": [_P, _I, ctypes.c_char_p, _I],
     "bl_ibm_set_layered": [_P, _I],

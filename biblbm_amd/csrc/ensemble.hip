@@ -2,10 +2,13 @@
 //
 // An ensemble groups lattices (each optionally with its immersed boundary)
 // created through the ordinary entry points.  One hipGraph holds two
-// coupled steps (both AA parities) of every box, each box a parallel branch
-// forked from and joined into the ensemble's stream, so the small kernels of
-// different boxes run side by side and the whole ensemble step is one graph
-// launch.  The reference runs such boxes as separate programs.
+// coupled steps (both AA parities) of every box.  Batched (the default
+// whenever the boxes allow it: same storage and parity, paired-stage IBM
+// steps, no Guo walls or phase fields): every stage of a step is one launch
+// over all boxes, each box's workgroups running with its own arguments
+// (ensemble_batch_build, ibm.hip).  Otherwise each box is a parallel branch
+// forked from and joined into the ensemble's stream.  The reference runs
+// such boxes as separate programs.
 #include <algorithm>
 #include <cstdio>
 #include <string>
@@ -30,6 +33,9 @@ struct bl_ensemble_s {
         bool operator!=(const Key& o) const { return !(*this == o); }
     };
     std::vector<Key> parity;
+    bl::EnsembleBatch* batch = nullptr;  // the tables of the captured batched steps
+    int mode = -1;                       // -1 batched when possible, 0 branches, 1 batched only
+    int order[2] = {0, 0};               // batched collide workgroup order per AA parity
 };
 
 namespace bl {
@@ -79,11 +85,41 @@ bool parity_matches(bl_ensemble e)
     return true;
 }
 
+int capture_batched(bl_ensemble e)
+{
+    BL_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = ensemble_batch_launch(e->batch, e->stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(e->stream, &graph);
+    if (rc != BL_OK || ee != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return rc ? rc : fail(BL_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(ee));
+    }
+    const hipError_t ie = hipGraphInstantiate(&e->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(BL_ERR_HIP, std::string("ensemble graph: ") + hipGetErrorString(ie));
+    e->parity.resize(e->lats.size());
+    for (size_t i = 0; i < e->lats.size(); ++i) e->parity[i] = capture_key(e, i);
+    return BL_OK;
+}
+
 int capture(bl_ensemble e)
 {
+    // a replay still running may read the old tables
+    if (e->graph || e->batch) BL_HIP(hipStreamSynchronize(e->stream));
     if (e->graph) {
         hipGraphExecDestroy(e->graph);
         e->graph = nullptr;
+    }
+    ensemble_batch_free(e->batch);
+    e->batch = nullptr;
+    if (e->mode != 0) {
+        int rc = ensemble_batch_build(e->lats, e->ibms, e->order, &e->batch);
+        if (rc) return rc;
+        if (e->batch) return capture_batched(e);
+        if (e->mode == 1)
+            return fail(BL_ERR_STATE, "ensemble: the boxes cannot run batched (storage, parity, step layout, "
+                                      "walls or pending state differ)");
     }
     std::vector<long long> s0(e->lats.size());
     for (size_t i = 0; i < e->lats.size(); ++i) s0[i] = e->lats[i]->steps;
@@ -192,12 +228,101 @@ int bl_ensemble_destroy(bl_ensemble e)
     if (!e) return BL_OK;
     if (e->stream) hipStreamSynchronize(e->stream);
     if (e->graph) hipGraphExecDestroy(e->graph);
+    ensemble_batch_free(e->batch);
     for (hipEvent_t ev : e->join)
         if (ev) hipEventDestroy(ev);
     if (e->fork) hipEventDestroy(e->fork);
     if (e->done) hipEventDestroy(e->done);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
+    return BL_OK;
+}
+
+int bl_ensemble_set_batched(bl_ensemble e, int mode)
+{
+    if (!e || mode < -1 || mode > 1) return fail(BL_ERR_ARG, "bl_ensemble_set_batched: mode must be -1, 0 or 1");
+    if (mode != e->mode && e->graph) {
+        if (e->stream) hipStreamSynchronize(e->stream);
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+        ensemble_batch_free(e->batch);
+        e->batch = nullptr;
+    }
+    e->mode = mode;
+    return BL_OK;
+}
+
+int bl_ensemble_tune(bl_ensemble e, int n_steps, char* report, int cap)
+{
+    if (!e || n_steps < 2 || n_steps % 2) return fail(BL_ERR_ARG, "bl_ensemble_tune: n_steps must be even and >= 2");
+    // the batched collide's workgroup order, per AA parity in turn (the
+    // other held), over real steps
+    static const int orders[] = {0, 1, 4, 16, 64, 256};
+    const int keep[2] = {e->order[0], e->order[1]};
+    int best[2] = {keep[0], keep[1]};
+    double best_ms = 1e30;
+    int steps_run = 0, rc = BL_OK;
+    std::string rep = "{\"trials\": [";
+    bool first = true;
+    auto trial = [&](int even, int odd, double& ms) -> int {
+        e->order[0] = even;
+        e->order[1] = odd;
+        if (e->graph) {  // recapture with the new order
+            BL_HIP(hipStreamSynchronize(e->stream));
+            hipGraphExecDestroy(e->graph);
+            e->graph = nullptr;
+        }
+        int r;
+        if ((r = bl_ensemble_steps(e, n_steps, 1))) return r;
+        if ((r = bl_ensemble_time_steps(e, n_steps, 1, &ms))) return r;
+        steps_run += 2 * n_steps;
+        ms /= n_steps;
+        return BL_OK;
+    };
+    for (int p = 0; p < 2 && rc == BL_OK; ++p)
+        for (int o : orders) {
+            double ms = 0;
+            const int ev = p == 0 ? o : best[0], od = p == 1 ? o : best[1];
+            if ((rc = trial(ev, od, ms))) break;
+            char buf[128];
+            snprintf(buf, sizeof(buf), "%s{\"even\": %d, \"odd\": %d, \"ms_per_step\": %.6f}", first ? "" : ", ", ev,
+                     od, ms);
+            rep += buf;
+            first = false;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best[0] = ev;
+                best[1] = od;
+            }
+        }
+    if (rc) {
+        e->order[0] = keep[0];
+        e->order[1] = keep[1];
+        return rc;
+    }
+    if (e->graph && (best[0] != e->order[0] || best[1] != e->order[1])) {
+        hipStreamSynchronize(e->stream);
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    e->order[0] = best[0];
+    e->order[1] = best[1];
+    char buf[256];
+    snprintf(buf, sizeof(buf), "], \"steps_run\": %d, \"batched\": %d, \"best\": {\"even\": %d, \"odd\": %d, "
+             "\"ms_per_step\": %.6f}}", steps_run, e->batch ? 1 : 0, best[0], best[1], best_ms);
+    rep += buf;
+    if (report && cap > 0) {
+        const size_t n = std::min<size_t>(rep.size(), (size_t)cap - 1);
+        rep.copy(report, n);
+        report[n] = '\0';
+    }
+    return BL_OK;
+}
+
+int bl_ensemble_is_batched(bl_ensemble e, int* batched)
+{
+    if (!e || !batched) return fail(BL_ERR_ARG, "bl_ensemble_is_batched: null argument");
+    *batched = e->graph && e->batch ? 1 : 0;
     return BL_OK;
 }
 

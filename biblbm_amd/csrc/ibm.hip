@@ -24,6 +24,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ibm.hpp"
@@ -1523,11 +1524,29 @@ __global__ void k_move(long NV, R* __restrict__ pos, const R* __restrict__ vel)
 // Chunk c (its row of `chunks`: first triangle, triangles, vertex range
 // of its bounding-box share, all global indices) -> 11 partials.
 template <class R>
-__global__ void __launch_bounds__(256) k_geom_partial(ForceView<R> f, const int4* __restrict__ chunks,
-                                                      const int* __restrict__ t_vert, double* __restrict__ partial)
+struct GeomArgs {
+    ForceView<R> f;
+    const int4* chunks;
+    const int* t_vert;
+    double* partial;
+    int nchunk, np;
+    Grid gd;
+    const int* p_choff;
+    const long* p_voff;
+    double *geom, *delta;
+    R* pos;
+    int remap;
+    const int* p_model;
+};
+
+template <class R>
+__device__ __forceinline__ void geom_partial_body(int c, const GeomArgs<R>& A)
 {
+    const ForceView<R>& f = A.f;
+    const int4* __restrict__ chunks = A.chunks;
+    const int* __restrict__ t_vert = A.t_vert;
+    double* __restrict__ partial = A.partial;
     __shared__ double sh[11][256];
-    const int c = blockIdx.x;
     const int4 ch = chunks[c];
     double acc[5] = {0, 0, 0, 0, 0};
     if ((int)threadIdx.x < ch.y) {
@@ -1567,18 +1586,26 @@ __global__ void __launch_bounds__(256) k_geom_partial(ForceView<R> f, const int4
     if (threadIdx.x < 11) partial[11 * (long)c + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+template <class R>
+__global__ void __launch_bounds__(256) k_geom_partial(GeomArgs<R> a) { geom_partial_body<R>(blockIdx.x, a); }
+
 // Chunk partials -> geometry; the centre of mass is then mapped onto the
 // periodic grid (src/fsi/ParticleBase.hh:113-136, src/fsi/Periodicity.h:83-87)
 // and the whole particle shifted by the same amount.
 template <class R>
-__global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* __restrict__ p_choff,
-                                                  const double* __restrict__ partial, const long* __restrict__ p_voff,
-                                                  double* __restrict__ geom, double* __restrict__ delta,
-                                                  R* __restrict__ pos, int remap, const int* __restrict__ p_model)
+__device__ __forceinline__ void geom_final_body(int p, const GeomArgs<R>& A)
 {
+    const Grid& gd = A.gd;
+    const int* __restrict__ p_choff = A.p_choff;
+    const double* __restrict__ partial = A.partial;
+    const long* __restrict__ p_voff = A.p_voff;
+    double* __restrict__ geom = A.geom;
+    double* __restrict__ delta = A.delta;
+    R* __restrict__ pos = A.pos;
+    const int remap = A.remap;
+    const int* __restrict__ p_model = A.p_model;
     __shared__ double sft[3];
     __shared__ double part[64][11];  // a batch of chunk partials, loaded by all lanes at once
-    const int p = blockIdx.x;
     const int c0 = p_choff[p], c1 = p_choff[p + 1];
     double acc[5] = {0, 0, 0, 0, 0};
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -1630,6 +1657,9 @@ __global__ void __launch_bounds__(64) k_geom_final(int np, Grid gd, const int* _
         pos[3 * v + 2] = (R)((double)pos[3 * v + 2] + s2);
     }
 }
+
+template <class R>
+__global__ void __launch_bounds__(64) k_geom_final(GeomArgs<R> a) { geom_final_body<R>(blockIdx.x, a); }
 
 // --------------------------------------------------------- voxelization
 // z-rays through integer (x, y) columns (RayTracer::find_intersections_z,
@@ -1877,17 +1907,35 @@ void mark(bl_ibm g, int k)
     if (g->timing) hipEventRecord(g->ev[k], g->lat->stream);
 }
 
+template <class R>
+GeomArgs<R> geom_args(bl_ibm g, int remap)
+{
+    GeomArgs<R> a;
+    a.f = force_view<R>(g);
+    a.chunks = (const int4*)g->ch_tab;
+    a.t_vert = g->t_vert;
+    a.partial = g->ch_partial;
+    a.nchunk = g->nchunk;
+    a.np = g->np;
+    a.gd = grid_of(g->lat);
+    a.p_choff = g->p_choff;
+    a.p_voff = g->p_voff;
+    a.geom = g->p_geom;
+    a.delta = g->p_delta;
+    a.pos = (R*)g->pos;
+    a.remap = remap;
+    a.p_model = g->p_model;
+    return a;
+}
+
 int update_particles(bl_ibm g, int remap, hipStream_t s)
 {
     if (g->np == 0) return BL_OK;
-    const Grid gd = grid_of(g->lat);
     with_vreal(g, [&](auto tag) {
         using R = decltype(tag);
-        hipLaunchKernelGGL(k_geom_partial<R>, dim3(g->nchunk), dim3(256), 0, s, force_view<R>(g),
-                           (const int4*)g->ch_tab, (const int*)g->t_vert, g->ch_partial);
-        hipLaunchKernelGGL(k_geom_final<R>, dim3(g->np), dim3(64), 0, s, g->np, gd, (const int*)g->p_choff,
-                           (const double*)g->ch_partial, (const long*)g->p_voff, g->p_geom, g->p_delta, (R*)g->pos,
-                           remap, (const int*)g->p_model);
+        const GeomArgs<R> a = geom_args<R>(g, remap);
+        hipLaunchKernelGGL(k_geom_partial<R>, dim3(g->nchunk), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_geom_final<R>, dim3(g->np), dim3(64), 0, s, a);
         return 0;
     });
     BL_CHECK_LAUNCH();
@@ -2108,8 +2156,10 @@ int launch_interp(bl_ibm g, bool move, hipStream_t s, bool reset_next = false)
 
 // Spreading of the last binning's lists into the boundary's second force
 // buffer; `blocks` receives the launch's workgroup count.
+// `share`: boxes whose spreading shares one launch (a batched ensemble);
+// the grid's lower bounds are divided among them.
 template <class R>
-SpreadArgs<R> spread_args(bl_ibm g, unsigned& blocks)
+SpreadArgs<R> spread_args(bl_ibm g, unsigned& blocks, int share = 1)
 {
     bl_lattice L = g->lat;
     SpreadArgs<R> a;
@@ -2135,10 +2185,11 @@ SpreadArgs<R> spread_args(bl_ibm g, unsigned& blocks)
     // vertices).  Both are launch parameters: the grid does not change any
     // sum.
     const long sw = (long)g->NV * g->lp[BL_IBM_SPREAD_SMALL] / 4;
-    const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(sw, 1024)));
+    const long min_small = std::max<long>(64, 1024 / share), min_big = std::max<long>(4, 256 / share);
+    const long small_waves = std::max<long>(64, std::min<long>(L->ntiles, std::max<long>(sw, min_small)));
     const int gs = (int)std::min<long>((small_waves + kSpreadWaves - 1) / kSpreadWaves, 4096);
     const int gb = (int)std::max<long>(
-        1, std::min<long>(L->ntiles, std::max<long>(g->NV / g->lp[BL_IBM_SPREAD_BIG], 256)));
+        1, std::min<long>(L->ntiles, std::max<long>(g->NV / g->lp[BL_IBM_SPREAD_BIG], min_big)));
     a.n_small = gs;
     blocks = (unsigned)(gs + gb);
     return a;
@@ -2527,6 +2578,350 @@ int steps_forked(bl_ibm g, int n)
     }
     return BL_OK;
 }
+
+// ------------------------------------------------ batches of boxes
+// An ensemble of independent boxes whose steps all take the paired-stage
+// layout (step_layered) runs each stage as one launch over every box:
+// LayeredEntry holds one box's arguments of one step, as step_layered
+// forms them; a batched kernel finds its box with batch_box over the
+// launch's per-box workgroup offsets and runs the box-local workgroup with
+// that box's arguments.  plan_layered advances the boundary's host state
+// exactly as step_layered does, without launching.
+template <class R>
+struct LayeredEntry {
+    ListReset reset;
+    int nr;  // workgroups of the binning's list reset (0: the list is ready)
+    BinArgs<R> bin;
+    ElemArgs<R> elem;
+    int nb, ne;
+    GatherArgs<R> gather;
+    StageArgs stage;
+    int ng, nst;
+    SpreadArgs<R> spread;
+    InterpArgs<R> interp;
+    int nsp, nip;
+    GeomArgs<R> geom;
+};
+
+// Whether a boundary's coupled step can join a batch: the paired-stage
+// layout, no particle-particle forces or phase field, no pending clear of
+// its tile buffers, and the vertex type of its storage.
+bool layered_batchable(bl_ibm g)
+{
+    const bl_lattice L = g->lat;
+    return L && g->inited && g->NV > 0 && use_layered(g) && !use_fused(g) && g->pp.empty() &&
+           L->omega_c0 == L->omega_c1 && !g->bins_dirty && !g->alt_full && !ibm_needs_settle(g) && !g->timing &&
+           g->n_rigid == 0 && g->vreal == (L->storage == BL_STORE_F64 ? 8 : 4);
+}
+
+// One step's entry; BL_ERR_STATE when a clear of the tile buffers would be
+// needed in the middle of the step (the batch is then not used).
+template <class R>
+int plan_layered(bl_ibm g, LayeredEntry<R>& E, int share)
+{
+    bl_lattice L = g->lat;
+    E.nr = 0;
+    if (!g->prepped) {
+        if (g->bins_dirty || g->alt_full) return BL_ERR_STATE;
+        E.reset = list_reset_of(g);
+        E.nr = (int)std::min<long>(nblk(L->ntiles), kResetBlocks);
+        g->prepped = true;
+    }
+    g->prepped = false;
+    E.bin = bin_args<R>(g);
+    E.elem = elem_args<R>(g);
+    E.nb = (int)nblk(g->NV);
+    E.ne = (int)nblk(g->NT + g->NP + g->NL);
+    binned(g);
+    E.gather = gather_args<R>(g, true);
+    E.stage = stage_args(g, false);
+    E.ng = (int)nblk((long)g->lp[BL_IBM_GATHER_LANES] * g->NV);
+    E.nst = (int)stage_blocks(g);
+    unsigned nsp;
+    E.spread = spread_args<R>(g, nsp, share);
+    E.nsp = (int)nsp;
+    spread_swap(g);
+    if (g->bins_dirty || g->alt_full) return BL_ERR_STATE;
+    g->prepped = true;
+    E.interp = interp_args<R>(g, list_reset_of(g));
+    E.nip = (int)interp_blocks(g);
+    g->binned = false;
+    E.geom = geom_args<R>(g, 1);
+    return BL_OK;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_reset_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                     const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    const LayeredEntry<R>& E = tab[box];
+    list_reset(E.reset, (int)blockIdx.x - base[box], E.nr);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_bin_elem_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                        const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    const LayeredEntry<R>& E = tab[box];
+    const int bx = (int)blockIdx.x - base[box];
+    if (bx < E.nb) bin_body<R>(bx, E.bin);
+    else elem_body<R>(bx - E.nb, E.elem);
+}
+
+template <class S, bool SWAPPED, class R, int GL>
+__global__ void __launch_bounds__(256) k_gather_stage_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                            const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    const LayeredEntry<R>& E = tab[box];
+    const int bx = (int)blockIdx.x - base[box];
+    if (bx < E.ng) gather_body<R, GL>(bx, E.gather);
+    else stage_body<S, SWAPPED>(bx - E.ng, E.stage);
+}
+
+template <class S, int SW, class R>
+__global__ void __launch_bounds__(256) k_spread_interp_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                             const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    const LayeredEntry<R>& E = tab[box];
+    const int bx = (int)blockIdx.x - base[box];
+    if (bx < E.nsp) spread_body<S, R>(bx, E.nsp, E.spread);
+    else interp_body<typename Pop<S>::C, SW, true, R>(bx - E.nsp, E.nip, E.interp);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_geom_partial_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                            const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    geom_partial_body<R>((int)blockIdx.x - base[box], tab[box].geom);
+}
+
+template <class R>
+__global__ void __launch_bounds__(64) k_geom_final_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                         const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    geom_final_body<R>((int)blockIdx.x - base[box], tab[box].geom);
+}
+
+}  // namespace
+
+// Two coupled steps of every box of an ensemble as batched launches: the
+// entries and workgroup offsets of every launch in one device buffer.
+struct EnsembleBatch {
+    int storage = 0, sw = 4, gl = 4;
+    int order[2] = {0, 0};  // collide workgroup order per AA parity (collide_block modes)
+    bool any_ibm = false, any_vwall = false;
+    void* dev = nullptr;  // entries and offsets of both steps
+    struct Launch {
+        size_t tab = 0, base = 0;  // byte offsets into dev
+        int nbox = 0;
+        unsigned nblk = 0;
+    };
+    struct Step {
+        bool swapped = false;  // the IBM stages' AA parity (before the collision)
+        Launch reset, bin_elem, gather_stage, spread_interp, geom_partial, geom_final, collide, vwall, vwall_u;
+        CollidePlan c0;        // template choices of the collide launch
+        bool vw_swapped = false;
+    } step[2];
+};
+
+namespace {
+
+// Appends n objects to a host byte image (8-byte aligned), returning their offset.
+template <class T>
+size_t put(std::vector<char>& img, const T* p, size_t n)
+{
+    size_t off = (img.size() + 15) & ~(size_t)15;
+    img.resize(off + sizeof(T) * n);
+    if (n) memcpy(img.data() + off, p, sizeof(T) * n);
+    return off;
+}
+
+// the workgroup offsets of a launch over boxes with the given counts
+EnsembleBatch::Launch put_launch(std::vector<char>& img, size_t tab, const std::vector<int>& counts)
+{
+    EnsembleBatch::Launch l;
+    std::vector<int> base(counts.size() + 1, 0);
+    for (size_t k = 0; k < counts.size(); ++k) base[k + 1] = base[k] + counts[k];
+    l.tab = tab;
+    l.base = put(img, base.data(), base.size());
+    l.nbox = (int)counts.size();
+    l.nblk = (unsigned)base.back();
+    return l;
+}
+
+template <class R>
+int batch_plan_ibm(const std::vector<bl_ibm>& gs, std::vector<char>& img, EnsembleBatch::Step& st)
+{
+    std::vector<LayeredEntry<R>> E(gs.size());
+    for (size_t k = 0; k < gs.size(); ++k) {
+        const int rc = plan_layered<R>(gs[k], E[k], (int)gs.size());
+        if (rc) return rc;
+    }
+    const size_t tab = put(img, E.data(), E.size());
+    std::vector<int> nr, nbe, ngs, nsi, nch, npa;
+    for (size_t k = 0; k < gs.size(); ++k) {
+        nr.push_back(E[k].nr);
+        nbe.push_back(E[k].nb + E[k].ne);
+        ngs.push_back(E[k].ng + E[k].nst);
+        nsi.push_back(E[k].nsp + E[k].nip);
+        nch.push_back(gs[k]->nchunk);
+        npa.push_back(gs[k]->np);
+    }
+    st.reset = put_launch(img, tab, nr);
+    st.bin_elem = put_launch(img, tab, nbe);
+    st.gather_stage = put_launch(img, tab, ngs);
+    st.spread_interp = put_launch(img, tab, nsi);
+    st.geom_partial = put_launch(img, tab, nch);
+    st.geom_final = put_launch(img, tab, npa);
+    return BL_OK;
+}
+
+template <class S>
+void batch_launch_ibm(const EnsembleBatch& B, const EnsembleBatch::Step& st, hipStream_t s)
+{
+    using R = typename VRof<S>::T;
+    const char* d = (const char*)B.dev;
+    auto tab = [&](const EnsembleBatch::Launch& l) { return (const LayeredEntry<R>*)(d + l.tab); };
+    auto base = [&](const EnsembleBatch::Launch& l) { return (const int*)(d + l.base); };
+    auto go = [&](auto kern, const EnsembleBatch::Launch& l, int blk) {
+        if (l.nblk) hipLaunchKernelGGL(kern, dim3(l.nblk), dim3(blk), 0, s, tab(l), base(l), l.nbox);
+    };
+    go(k_reset_batch<R>, st.reset, 256);
+    go(k_bin_elem_batch<R>, st.bin_elem, 256);
+    auto gs = [&](auto sw_tag) {
+        constexpr bool SWP = decltype(sw_tag)::value;
+        if (B.gl == 2) go(k_gather_stage_batch<S, SWP, R, 2>, st.gather_stage, 256);
+        else if (B.gl == 8) go(k_gather_stage_batch<S, SWP, R, 8>, st.gather_stage, 256);
+        else go(k_gather_stage_batch<S, SWP, R, 4>, st.gather_stage, 256);
+    };
+    if (st.swapped) gs(std::true_type{});
+    else gs(std::false_type{});
+    if (B.sw == 4) go(k_spread_interp_batch<S, 4, R>, st.spread_interp, 256);
+    else if (B.sw == 3) go(k_spread_interp_batch<S, 3, R>, st.spread_interp, 256);
+    else go(k_spread_interp_batch<S, 2, R>, st.spread_interp, 256);
+    go(k_geom_partial_batch<R>, st.geom_partial, 256);
+    go(k_geom_final_batch<R>, st.geom_final, 64);
+}
+
+}  // namespace
+
+int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<bl_ibm>& ibms, const int order[2],
+                         EnsembleBatch** out)
+{
+    *out = nullptr;
+    const size_t n = lats.size();
+    if (n < 2) return BL_OK;
+    std::vector<bl_ibm> gs;
+    int sw = 0, gl = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const bl_lattice L = lats[k];
+        if (!L->guo.empty() || L->storage != lats[0]->storage || (L->steps & 1) != (lats[0]->steps & 1)) return BL_OK;
+        if (!collide_batchable(collide_plan(L), collide_plan(lats[0]))) return BL_OK;
+        const bl_ibm g = ibms[k];
+        if (!g) continue;
+        if (!layered_batchable(g)) return BL_OK;
+        if ((sw && g->S != sw) || (gl && g->lp[BL_IBM_GATHER_LANES] != gl)) return BL_OK;
+        sw = g->S;
+        gl = g->lp[BL_IBM_GATHER_LANES];
+        gs.push_back(g);
+    }
+    // plan both steps; the host state of every box advances two steps and
+    // its step count is put back afterwards (nothing has run)
+    std::vector<long long> s0(n);
+    for (size_t k = 0; k < n; ++k) s0[k] = lats[k]->steps;
+    EnsembleBatch* B = new EnsembleBatch();
+    B->storage = lats[0]->storage;
+    B->sw = sw ? sw : 4;
+    B->gl = gl ? gl : 4;
+    B->any_ibm = !gs.empty();
+    B->order[0] = order[0];
+    B->order[1] = order[1];
+    std::vector<char> img;
+    int rc = BL_OK;
+    for (int k = 0; k < 2 && rc == BL_OK; ++k) {
+        EnsembleBatch::Step& st = B->step[k];
+        st.swapped = (lats[0]->steps & 1) != 0;
+        if (!gs.empty()) {
+            if (B->storage == BL_STORE_F64) rc = batch_plan_ibm<double>(gs, img, st);
+            else rc = batch_plan_ibm<float>(gs, img, st);
+            if (rc) break;
+        }
+        std::vector<LatView> views;
+        std::vector<int> cnt;
+        st.c0 = collide_plan(lats[0]);
+        for (bl_lattice L : lats) {
+            const CollidePlan p = collide_plan(L);
+            views.push_back(p.v);
+            cnt.push_back((int)((L->ncell + 255) / 256));
+            collide_commit(L);
+        }
+        st.collide = put_launch(img, put(img, views.data(), views.size()), cnt);
+        std::vector<VwallPlan> vw;
+        std::vector<int> vcnt, ucnt;
+        st.vw_swapped = (lats[0]->steps & 1) != 0;
+        for (bl_lattice L : lats)
+            if (L->has_vwall && L->mask) {
+                vw.push_back(VwallPlan{view_of(L), L->nwall, L->d_wall});
+                vcnt.push_back((int)((L->nwall + 255) / 256));
+                ucnt.push_back((int)((2 * L->nwall + 255) / 256));
+            }
+        B->any_vwall = !vw.empty();
+        const size_t vtab = put(img, vw.data(), vw.size());
+        st.vwall = put_launch(img, vtab, vcnt);
+        st.vwall_u = put_launch(img, vtab, ucnt);
+    }
+    for (size_t k = 0; k < n; ++k) lats[k]->steps = s0[k];
+    if (rc) {  // a clear was due mid-step: the ensemble uses per-box branches
+        delete B;
+        return rc == BL_ERR_STATE ? BL_OK : rc;
+    }
+    if (hipMalloc(&B->dev, img.size()) != hipSuccess ||
+        hipMemcpy(B->dev, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(B->dev);
+        delete B;
+        return fail(BL_ERR_HIP, "ensemble batch: device tables");
+    }
+    *out = B;
+    return BL_OK;
+}
+
+int ensemble_batch_launch(const EnsembleBatch* B, hipStream_t s)
+{
+    const char* d = (const char*)B->dev;
+    for (int k = 0; k < 2; ++k) {
+        const EnsembleBatch::Step& st = B->step[k];
+        if (B->any_ibm) {
+            if (B->storage == BL_STORE_F64) batch_launch_ibm<double>(*B, st, s);
+            else if (B->storage == BL_STORE_F32) batch_launch_ibm<float>(*B, st, s);
+            else batch_launch_ibm<__half>(*B, st, s);
+            BL_CHECK_LAUNCH();
+        }
+        int rc = collide_batch_launch(st.c0, (const LatView*)(d + st.collide.tab), (const int*)(d + st.collide.base),
+                                      st.collide.nbox, st.collide.nblk, B->order[st.c0.odd ? 1 : 0], s);
+        if (rc) return rc;
+        if (st.vwall.nblk &&
+            (rc = vwall_batch_launch(B->storage, st.vw_swapped, (const VwallPlan*)(d + st.vwall.tab),
+                                     (const int*)(d + st.vwall.base), (const int*)(d + st.vwall_u.base),
+                                     st.vwall.nbox, st.vwall.nblk, st.vwall_u.nblk, s)))
+            return rc;
+    }
+    return BL_OK;
+}
+
+void ensemble_batch_free(EnsembleBatch* B)
+{
+    if (!B) return;
+    hipFree(B->dev);
+    delete B;
+}
+
+namespace {
 
 }  // namespace
 

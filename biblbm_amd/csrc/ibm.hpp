@@ -181,6 +181,17 @@ namespace bl {
 // `forked` lets independent IBM stages run on the boundary's own extra
 // streams; otherwise everything stays on the lattice stream.
 int ibm_step_launch(bl_ibm g, bool forked);
+// Ensembles of boxes whose coupled steps take the paired-stage layout:
+// two coupled steps of every box as one launch per stage over all boxes
+// (ensemble_batch_launch enqueues them on s; the boxes' host state is as
+// after ensemble_batch_build, which plans them).  *out stays null when the
+// boxes cannot be batched (differing storage, parity, stage layout, ...).
+struct EnsembleBatch;
+// order: the batched collide's workgroup order per AA parity
+int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<bl_ibm>& ibms, const int order[2],
+                         EnsembleBatch** out);
+int ensemble_batch_launch(const EnsembleBatch* b, hipStream_t s);
+void ensemble_batch_free(EnsembleBatch* b);
 // Reads the overflow flags of the last steps (synchronises the stream).
 int ibm_check_overflow(bl_ibm g);
 // The lattice is going away: forget it (the boundary's buffers stay valid

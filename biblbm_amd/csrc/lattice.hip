@@ -229,30 +229,111 @@ static int force_mode(const bl_lattice_s* L)
     return 0;
 }
 
-int collide_stream_launch(bl_lattice_s* L)
+CollidePlan collide_plan(const bl_lattice_s* L)
 {
-    LatView v = view_of(L);
-    const bool odd = (L->steps & 1) != 0;
-    const int fm = force_mode(L);
-    if (fm != 2) v.flags = nullptr;
-    const bool phase = L->phase != nullptr && L->omega_c0 != L->omega_c1;
-    if (fm != 2 || phase) v.emit_u = nullptr;
-    const int par = odd ? 1 : 0;
-    const int blk = L->block[par];
+    CollidePlan p;
+    LatView& v = p.v;
+    v = view_of(L);
+    p.odd = (L->steps & 1) != 0;
+    p.fm = force_mode(L);
+    if (p.fm != 2) v.flags = nullptr;
+    p.phase = L->phase != nullptr && L->omega_c0 != L->omega_c1;
+    if (p.fm != 2 || p.phase) v.emit_u = nullptr;
+    p.mask = L->mask != nullptr;
+    p.storage = L->storage;
+    const int par = p.odd ? 1 : 0;
+    p.blk = L->block[par];
     v.xcd_remap = L->xcd_remap[par];
-    const int nt = L->nontemporal[par];
-    unsigned nblk = (unsigned)((L->ncell + blk - 1) / blk);
+    p.nt = L->nontemporal[par];
+    p.nblk = (unsigned)((L->ncell + p.blk - 1) / p.blk);
     if (v.xcd_remap == BL_ORDER_ROWS) {
-        v.row_segs = (L->nz + blk - 1) / blk;
-        nblk = (unsigned)(((long)L->nx * L->ny + 7) / 8 * 8 * v.row_segs);
+        v.row_segs = (L->nz + p.blk - 1) / p.blk;
+        p.nblk = (unsigned)(((long)L->nx * L->ny + 7) / 8 * 8 * v.row_segs);
     }
-    if (L->storage == BL_STORE_F64) launch_cs<double>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
-    else if (L->storage == BL_STORE_F32) launch_cs<float>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
-    else launch_cs<__half>(v, odd, L->mask != nullptr, fm, phase, nblk, blk, L->stream, nt);
-    BL_CHECK_LAUNCH();
+    return p;
+}
+
+void collide_commit(bl_lattice_s* L)
+{
     L->steps += 1;
     for (int d = 0; d < 3; ++d) L->g_last[d] = L->g[d];
     L->collided = true;
+}
+
+int collide_stream_launch(bl_lattice_s* L)
+{
+    const CollidePlan p = collide_plan(L);
+    if (p.storage == BL_STORE_F64) launch_cs<double>(p.v, p.odd, p.mask, p.fm, p.phase, p.nblk, p.blk, L->stream, p.nt);
+    else if (p.storage == BL_STORE_F32) launch_cs<float>(p.v, p.odd, p.mask, p.fm, p.phase, p.nblk, p.blk, L->stream, p.nt);
+    else launch_cs<__half>(p.v, p.odd, p.mask, p.fm, p.phase, p.nblk, p.blk, L->stream, p.nt);
+    BL_CHECK_LAUNCH();
+    collide_commit(L);
+    return BL_OK;
+}
+
+// ------------------------------------------- batches of independent boxes
+// One collide-and-stream launch over the lattices of an ensemble: workgroup
+// b belongs to box k with base[k] <= b < base[k + 1] (a binary search over
+// the box offsets, uniform over the workgroup) and runs local block
+// b - base[k] of that box with its own view.  Every box shares the template
+// choices (storage, parity, mask, force mode); the launch shape is 256
+// threads in hardware order, which gives the same values as any other.
+template <class S, bool ODD, bool MASK, int FMODE>
+__global__ void __launch_bounds__(256) k_collide_batch(const LatView* __restrict__ views, const int* __restrict__ base,
+                                                       int nbox, int order)
+{
+    // `order` deals the batch's workgroups over the XCDs (collide_block)
+    const int b = (int)collide_block(blockIdx.x, gridDim.x, order);
+    const int box = batch_box(base, nbox, b);
+    const LatView v = views[box];
+    const unsigned c = (unsigned)(b - base[box]) * blockDim.x + threadIdx.x;
+    if (c >= v.ncell) return;
+    int x, y, z;
+    decode(c, v, x, y, z);
+    cs_cell<S, ODD, MASK, FMODE, false, unsigned, false, 0>(v, c, x, y, z);
+}
+
+template <class S, bool ODD, bool MASK>
+void launch_collide_batch2(int fm, const LatView* views, const int* base, int nbox, unsigned nblk, int order,
+                           hipStream_t s)
+{
+    if (fm == 0)
+        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 0>), dim3(nblk), dim3(256), 0, s, views, base, nbox, order);
+    else if (fm == 1)
+        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 1>), dim3(nblk), dim3(256), 0, s, views, base, nbox, order);
+    else
+        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 2>), dim3(nblk), dim3(256), 0, s, views, base, nbox, order);
+}
+
+template <class S>
+void launch_collide_batch1(bool odd, bool mask, int fm, const LatView* views, const int* base, int nbox, unsigned nblk,
+                           int order, hipStream_t s)
+{
+    if (odd) {
+        if (mask) launch_collide_batch2<S, true, true>(fm, views, base, nbox, nblk, order, s);
+        else launch_collide_batch2<S, true, false>(fm, views, base, nbox, nblk, order, s);
+    } else {
+        if (mask) launch_collide_batch2<S, false, true>(fm, views, base, nbox, nblk, order, s);
+        else launch_collide_batch2<S, false, false>(fm, views, base, nbox, nblk, order, s);
+    }
+}
+
+bool collide_batchable(const CollidePlan& a, const CollidePlan& b)
+{
+    return a.storage == b.storage && a.odd == b.odd && a.mask == b.mask && a.fm == b.fm && !a.phase && !b.phase &&
+           !a.v.emit_u && !b.v.emit_u && (size_t)Q * a.v.stride <= 0xffffffffull &&
+           (size_t)Q * b.v.stride <= 0xffffffffull;
+}
+
+int collide_batch_launch(const CollidePlan& p0, const LatView* views, const int* base, int nbox, unsigned nblk,
+                         int order, hipStream_t s)
+{
+    if (p0.storage == BL_STORE_F64)
+        launch_collide_batch1<double>(p0.odd, p0.mask, p0.fm, views, base, nbox, nblk, order, s);
+    else if (p0.storage == BL_STORE_F32)
+        launch_collide_batch1<float>(p0.odd, p0.mask, p0.fm, views, base, nbox, nblk, order, s);
+    else launch_collide_batch1<__half>(p0.odd, p0.mask, p0.fm, views, base, nbox, nblk, order, s);
+    BL_CHECK_LAUNCH();
     return BL_OK;
 }
 
@@ -262,9 +343,11 @@ int collide_stream_launch(bl_lattice_s* L)
 // src/boundaryCondition/finiteDifferenceBoundaryProcessor3D.hh:68-115) with
 // the density rule of VelocityDirichletBoundaryDynamics::computeRhoBar
 // (src/boundaryCondition/boundaryDynamics.hh:631-660).  Two launches: the
-// first computes every wall cell's new populations from the post-stream
-// state, the second writes them, so no wall cell reads a value another
-// already replaced.
+// first computes, from the post-stream state, the velocities that the
+// finite differences read (wall cells and the interior cells next to
+// them); the second rewrites every wall cell from its own post-stream
+// populations and those velocities.  A cell's population slots are its
+// own in either AA layout, so no wall cell reads a value another replaced.
 template <class S>
 __device__ void cell_velocity_post(const LatView& v, long c, int x, int y, int z, bool swapped, double u[3])
 {
@@ -309,12 +392,54 @@ __device__ __forceinline__ void wall_cell_coords(const LatView& v, long w, int& 
     x = p[0]; y = p[1]; z = p[2];
 }
 
+// Wall-plane index of a cell on plane `side` of the wall axis.
+__device__ __forceinline__ long wall_index(const LatView& v, int side, const int p[3])
+{
+    const int a = v.wall_axis;
+    const int n[3] = {v.nx, v.ny, v.nz};
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    return (long)side * n[t1] * n[t2] + (long)p[t1] * n[t2] + p[t2];
+}
+
+// The velocities the finite differences of the wall cells read, computed
+// once per cell: cells w < nwall are the wall-plane cells themselves
+// (tangential neighbours), cells nwall + w the first interior cell next to
+// wall cell w (normal neighbour).  Three per cell in `out`.
 template <class S>
-__global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out)
+__device__ __forceinline__ void vwall_u_cell(const LatView& v, long nwall, bool swapped, double* out, long w)
+{
+    if (w >= 2 * nwall) return;
+    const bool inner = w >= nwall;
+    int x, y, z;
+    wall_cell_coords(v, inner ? w - nwall : w, x, y, z);
+    if (inner) {
+        int q[3] = {x, y, z};
+        const int a = v.wall_axis;
+        q[a] += q[a] == 0 ? 1 : -1;
+        x = q[0];
+        y = q[1];
+        z = q[2];
+    }
+    double u[3];
+    cell_velocity_post<S>(v, ((long)x * v.ny + y) * v.nz + z, x, y, z, swapped, u);
+    double* o = out + 3 * w;
+    o[0] = u[0];
+    o[1] = u[1];
+    o[2] = u[2];
+}
+
+template <class S>
+__global__ void k_vwall_u(LatView v, long nwall, bool swapped, double* out)
+{
+    vwall_u_cell<S>(v, nwall, swapped, out, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class S>
+__device__ __forceinline__ void vwall_compute_cell(const LatView& v, long nwall, bool swapped, const double* un,
+                                                   long w)
 {
     using C = typename Pop<S>::C;
     using FT = typename Pop<S>::F;
-    const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwall) return;
     int x, y, z;
     wall_cell_coords(v, w, x, y, z);
@@ -333,8 +458,10 @@ __global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out
     C Fc[3];
     cell_force<FT, C>(v, c, x, y, z, Fc);
     double onWall = 0.0, normal = 0.0;
+    #pragma unroll
     for (int i = 0; i < Q; ++i)
         if (ci(i, a) == 0) onWall += (double)fc[i];
+    #pragma unroll
     for (int i = 0; i < Q; ++i)
         if (ci(i, a) == orient) normal += (double)fc[i];
     const double velNormal = (double)orient * (uw[a] - 0.5 * (double)Fc[a]);
@@ -345,11 +472,7 @@ __global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out
     const int n3[3] = {v.nx, v.ny, v.nz};
     for (int d = 0; d < 3; ++d) {
         if (d == a) {
-            int q[3] = {x, y, z};
-            q[d] -= orient;
-            const long c1 = ((long)q[0] * v.ny + q[1]) * v.nz + q[2];
-            double u1[3];
-            cell_velocity_post<S>(v, c1, q[0], q[1], q[2], swapped, u1);
+            const double* u1 = un + 3 * (nwall + w);
             for (int k = 0; k < 3; ++k) du[d][k] = -orient * (-uw[k] + u1[k]);
         } else {
             int qp[3] = {x, y, z}, qm[3] = {x, y, z};
@@ -362,11 +485,8 @@ __global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out
                 if (qp[d] >= n3[d]) qp[d] = n3[d] - 1;
                 if (qm[d] < 0) qm[d] = 0;
             }
-            const long cp = ((long)qp[0] * v.ny + qp[1]) * v.nz + qp[2];
-            const long cm = ((long)qm[0] * v.ny + qm[1]) * v.nz + qm[2];
-            double up[3], um[3];
-            cell_velocity_post<S>(v, cp, qp[0], qp[1], qp[2], swapped, up);
-            cell_velocity_post<S>(v, cm, qm[0], qm[1], qm[2], swapped, um);
+            const double* up = un + 3 * wall_index(v, side, qp);
+            const double* um = un + 3 * wall_index(v, side, qm);
             for (int k = 0; k < 3; ++k) du[d][k] = (up[k] - um[k]) / 2.0;
         }
     }
@@ -381,6 +501,7 @@ __global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out
     const double jv[3] = {rho * uw[0], rho * uw[1], rho * uw[2]};
     const double jSqr = jv[0] * jv[0] + jv[1] * jv[1] + jv[2] * jv[2];
     const double invRho = 1.0 / (rb + 1.0);
+    #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const double c_j = cx(i) * jv[0] + cy(i) * jv[1] + cz(i) * jv[2];
         const double feq = tw(i) * (rb + 3. * c_j + invRho * (4.5 * c_j * c_j - 1.5 * jSqr));
@@ -396,25 +517,48 @@ __global__ void k_vwall_compute(LatView v, long nwall, bool swapped, double* out
             }
         }
         fneq *= tw(i) * 3.0 * 3.0 / 2.0;
-        out[w * Q + i] = feq + fneq;
+        fc[i] = (C)(feq + fneq);
     }
+    store_post<S>(v, n, c, swapped, fc);
 }
 
 template <class S>
-__global__ void k_vwall_store(LatView v, long nwall, bool swapped, const double* in)
+__global__ void k_vwall_compute(LatView v, long nwall, bool swapped, const double* un)
 {
-    using C = typename Pop<S>::C;
-    const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwall) return;
-    int x, y, z;
-    wall_cell_coords(v, w, x, y, z);
-    const long c = ((long)x * v.ny + y) * v.nz + z;
-    if (!v.mask || v.mask[c] != M_VWALL) return;
-    Nbr n;
-    n.init(v, x, y, z);
-    C f[Q];
-    for (int i = 0; i < Q; ++i) f[i] = (C)in[w * Q + i];
-    store_post<S>(v, n, c, swapped, f);
+    vwall_compute_cell<S>(v, nwall, swapped, un, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The velocity-wall passes of a batch of boxes (each with velocity walls
+// and no Guo wall): the same two launches, each over every box's wall
+// cells.
+// PHASE 0 neighbour velocities (base counts the workgroups over 2 nwall
+// cells), 1 the wall cells.
+template <class S, int PHASE>
+__global__ void k_vwall_batch(const VwallPlan* __restrict__ tab, const int* __restrict__ base, int nbox, bool swapped)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    const VwallPlan w = tab[box];
+    const long i = (long)((int)blockIdx.x - base[box]) * blockDim.x + threadIdx.x;
+    if (PHASE == 0) vwall_u_cell<S>(w.v, w.nwall, swapped, w.buf, i);
+    else vwall_compute_cell<S>(w.v, w.nwall, swapped, w.buf, i);
+}
+
+template <class S>
+void vwall_batch_s(bool swapped, const VwallPlan* tab, const int* base, const int* base_u, int nbox, unsigned nblk,
+                   unsigned nblk_u, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_vwall_batch<S, 0>), dim3(nblk_u), dim3(256), 0, s, tab, base_u, nbox, swapped);
+    hipLaunchKernelGGL((k_vwall_batch<S, 1>), dim3(nblk), dim3(256), 0, s, tab, base, nbox, swapped);
+}
+
+int vwall_batch_launch(int storage, bool swapped, const VwallPlan* tab, const int* base, const int* base_u, int nbox,
+                       unsigned nblk, unsigned nblk_u, hipStream_t s)
+{
+    if (storage == BL_STORE_F64) vwall_batch_s<double>(swapped, tab, base, base_u, nbox, nblk, nblk_u, s);
+    else if (storage == BL_STORE_F32) vwall_batch_s<float>(swapped, tab, base, base_u, nbox, nblk, nblk_u, s);
+    else vwall_batch_s<__half>(swapped, tab, base, base_u, nbox, nblk, nblk_u, s);
+    BL_CHECK_LAUNCH();
+    return BL_OK;
 }
 
 int velocity_walls_launch(bl_lattice_s* L)
@@ -422,16 +566,19 @@ int velocity_walls_launch(bl_lattice_s* L)
     if (!L->has_vwall || !L->mask) return guo_walls_launch(L);
     const LatView v = view_of(L);
     const bool swapped = (L->steps & 1) != 0;
-    const unsigned nb = (unsigned)((L->nwall + 255) / 256);
+    const unsigned nb = (unsigned)((L->nwall + 255) / 256), nbu = (unsigned)((2 * L->nwall + 255) / 256);
     if (L->storage == BL_STORE_F64) {
-        hipLaunchKernelGGL(k_vwall_compute<double>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
-        hipLaunchKernelGGL(k_vwall_store<double>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
+        hipLaunchKernelGGL(k_vwall_u<double>, dim3(nbu), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
+        hipLaunchKernelGGL(k_vwall_compute<double>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped,
+                           (const double*)L->d_wall);
     } else if (L->storage == BL_STORE_F32) {
-        hipLaunchKernelGGL(k_vwall_compute<float>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
-        hipLaunchKernelGGL(k_vwall_store<float>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
+        hipLaunchKernelGGL(k_vwall_u<float>, dim3(nbu), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
+        hipLaunchKernelGGL(k_vwall_compute<float>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped,
+                           (const double*)L->d_wall);
     } else {
-        hipLaunchKernelGGL(k_vwall_compute<__half>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
-        hipLaunchKernelGGL(k_vwall_store<__half>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped, (const double*)L->d_wall);
+        hipLaunchKernelGGL(k_vwall_u<__half>, dim3(nbu), dim3(256), 0, L->stream, v, L->nwall, swapped, L->d_wall);
+        hipLaunchKernelGGL(k_vwall_compute<__half>, dim3(nb), dim3(256), 0, L->stream, v, L->nwall, swapped,
+                           (const double*)L->d_wall);
     }
     BL_CHECK_LAUNCH();
     return guo_walls_launch(L);
@@ -846,7 +993,9 @@ int bl_lattice_set_mask(bl_lattice L, const unsigned char* mask)
         const long nwall = 2L * n[t1] * n[t2];
         if (nwall > L->nwall) {
             hipFree(L->d_wall);
-            BL_HIP(hipMalloc(&L->d_wall, sizeof(double) * Q * nwall));
+            // three neighbour velocities per wall cell and per interior
+            // cell next to it
+            BL_HIP(hipMalloc(&L->d_wall, sizeof(double) * 6 * nwall));
         }
         L->nwall = nwall;
     }

@@ -71,7 +71,7 @@ struct bl_lattice_s {
     // scratch
     double* d_stage = nullptr;  // ncell * 19 doubles, allocated on demand
     double* d_partials = nullptr;
-    double* d_wall = nullptr;   // velocity-wall staging
+    double* d_wall = nullptr;   // velocities read by the velocity-wall finite differences
     long nwall = 0;
     bl_ibm ibm = nullptr;
     std::vector<bl_guo_wall> guo;  // inserted Guo curved walls (not owned)
@@ -114,6 +114,37 @@ struct LatView {
 };
 
 LatView view_of(const bl_lattice_s* L);
+
+// One collide-and-stream launch of a lattice, as collide_stream_launch
+// would issue it (collide_plan reads the lattice's state, collide_commit
+// advances it: step count, last body force).
+struct CollidePlan {
+    LatView v;
+    bool odd = false, mask = false, phase = false;
+    int fm = 0, nt = 0, blk = 256, storage = 0;
+    unsigned nblk = 0;
+};
+CollidePlan collide_plan(const bl_lattice_s* L);
+void collide_commit(bl_lattice_s* L);
+// Batches of boxes (ensembles): one launch over the boxes' views, box k
+// taking workgroups [base[k], base[k + 1]) (device arrays); the plans must
+// agree on everything collide_batchable checks.
+bool collide_batchable(const CollidePlan& a, const CollidePlan& b);
+// `order`: how the batch's workgroups are dealt over the XCDs (the modes
+// of bl_lattice_set_launch_parity except BL_ORDER_ROWS)
+int collide_batch_launch(const CollidePlan& p0, const LatView* views, const int* base, int nbox, unsigned nblk,
+                         int order, hipStream_t s);
+// velocity-wall cells of one box: its view, wall-cell count and the buffer
+// of the velocities its finite differences read
+struct VwallPlan {
+    LatView v;
+    long nwall;
+    double* buf;
+};
+// base / nblk: workgroups over the wall cells, base_u / nblk_u over twice
+// as many (the neighbour-velocity pass)
+int vwall_batch_launch(int storage, bool swapped, const VwallPlan* tab, const int* base, const int* base_u, int nbox,
+                       unsigned nblk, unsigned nblk_u, hipStream_t s);
 int ensure_force_field(bl_lattice_s* L);
 int collide_stream_launch(bl_lattice_s* L);
 int velocity_walls_launch(bl_lattice_s* L);  // post-stream passes: velocity walls, then Guo walls

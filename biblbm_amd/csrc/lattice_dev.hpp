@@ -116,6 +116,20 @@ __device__ __forceinline__ unsigned collide_block(unsigned bid, unsigned nb, int
     return xcd_chunk_block(bid, nb, (unsigned)(31 - __builtin_clz((unsigned)mode)));
 }
 
+// Batched launches over boxes: the box of workgroup b, the k with
+// base[k] <= b < base[k + 1] (base ascending, nbox boxes; uniform over the
+// workgroup, so the search runs on scalar registers).
+__device__ __forceinline__ int batch_box(const int* __restrict__ base, int nbox, int b)
+{
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 struct Nbr {
     int X[3], Y[3], Z[3];
     __device__ __forceinline__ void init(const LatView& v, int x, int y, int z)

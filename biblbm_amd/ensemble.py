@@ -28,6 +28,24 @@ class Ensemble:
     def steps(self, n, use_graph=True):
         call("bl_ensemble_steps", self.handle, int(n), int(bool(use_graph)))
 
+    def set_batched(self, mode):
+        """-1 batched when the boxes allow it (default), 0 one graph branch
+        per box, 1 batched only (bl_ensemble_set_batched)."""
+        call("bl_ensemble_set_batched", self.handle, int(mode))
+
+    @property
+    def batched(self):
+        out = ctypes.c_int()
+        call("bl_ensemble_is_batched", self.handle, ctypes.byref(out))
+        return bool(out.value)
+
+    def tune(self, n_steps=4):
+        """bl_ensemble_tune: the batched collide's workgroup order per AA
+        parity, over real steps; returns the report."""
+        buf = ctypes.create_string_buffer(8192)
+        call("bl_ensemble_tune", self.handle, int(n_steps), buf, len(buf))
+        return json.loads(buf.value.decode())
+
     def time_steps(self, n, use_graph=True):
         ms = ctypes.c_double()
         call("bl_ensemble_time_steps", self.handle, int(n), int(bool(use_graph)), ctypes.byref(ms))

@@ -317,13 +317,20 @@ def shear_box(storage, s, tau, Ca, G=0.002, level=3, C=10.0, kernel="peskin", ra
                     dict(s=s, tau=tau, Ca=Ca, G=G, radius=R, wall_u=wall_u, shear_rate=gamma))
 
 
-def shear_ensemble(storage="f64", n_boxes=64, seed=11, sides=(32, 40, 48, 56, 64), **kw):
-    """The boxes of the batched-ensemble configuration (seeded draws)."""
+def shear_ensemble_params(n_boxes=64, seed=11, sides=(32, 40, 48, 56, 64)):
+    """The seeded draws of the batched-ensemble configuration: per box
+    (side, tau, capillary number, box seed)."""
     rng = np.random.default_rng(seed)
     out = []
     for b in range(n_boxes):
         s = int(rng.choice(sides))
         tau = float(rng.uniform(0.7, 1.2))
         Ca = float(np.exp(rng.uniform(np.log(0.05), np.log(0.5))))
-        out.append(shear_box(storage, s, tau, Ca, seed=seed * 1000 + b, **kw))
+        out.append((s, tau, Ca, seed * 1000 + b))
     return out
+
+
+def shear_ensemble(storage="f64", n_boxes=64, seed=11, sides=(32, 40, 48, 56, 64), **kw):
+    """The boxes of the batched-ensemble configuration (seeded draws)."""
+    return [shear_box(storage, s, tau, Ca, seed=bs, **kw)
+            for s, tau, Ca, bs in shear_ensemble_params(n_boxes, seed, sides)]

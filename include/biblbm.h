@@ -489,6 +489,23 @@ int bl_ensemble_destroy(bl_ensemble e);
 /* n coupled steps of every box.  use_graph: two steps of all boxes are
  * captured once in one hipGraph (each box a parallel branch) and replayed. */
 int bl_ensemble_steps(bl_ensemble e, int n_steps, int use_graph);
+/* How bl_ensemble_steps runs graphs (no counterpart in the reference):
+ * mode 1 batched, every stage of a step one launch over all boxes (each
+ * box's workgroups with its own arguments), an error if the boxes do not
+ * allow it; 0 one graph branch per box; -1 (default) batched when the
+ * boxes allow it (same storage and AA parity, paired-stage immersed
+ * boundary steps, no Guo walls, phase fields or particle-particle forces),
+ * else branches.  Both give the values of the boxes stepped alone. */
+int bl_ensemble_set_batched(bl_ensemble e, int mode);
+/* 1 if the ensemble's captured steps are the batched ones. */
+int bl_ensemble_is_batched(bl_ensemble e, int *batched);
+/* The project's tuner for batched ensembles (no counterpart in the
+ * reference): times n_steps (even) graph-replayed steps for each workgroup
+ * order of the batched collide-and-stream launch (the modes of
+ * bl_lattice_set_launch_parity except BL_ORDER_ROWS), per AA parity in
+ * turn, and keeps the fastest.  The trials are real steps (the report says
+ * how many); every order gives the same values. */
+int bl_ensemble_tune(bl_ensemble e, int n_steps, char *report, int cap);
 /* The same, returning the device time of the n steps in milliseconds. */
 int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms);
 

@@ -107,3 +107,77 @@ def test_ensemble_rejects_mismatched_pairs(device):
     with pytest.raises(capi.BlError):
         Ensemble([(ws[0].lattice, ws[0].ibm), (ws[0].lattice, None)])
     _close(ws)
+
+
+@pytest.mark.parametrize("storage,steps", [("f64", 6), ("f64", 5), ("f32", 6), ("f16s", 4)])
+def test_batched_ensemble_equals_branches_and_boxes_alone(device, storage, steps):
+    """Every stage of a step as one launch over all boxes (batched) gives
+    bitwise the state of the per-box graph branches and of each box stepped
+    on its own, over several graph replays and an odd step count (the last
+    step eager)."""
+    from biblbm_amd.ensemble import Ensemble
+    A, B, C = _boxes(storage=storage), _boxes(storage=storage), _boxes(storage=storage)
+    ea = Ensemble([(w.lattice, w.ibm) for w in A])
+    ea.set_batched(1)
+    eb = Ensemble([(w.lattice, w.ibm) for w in B])
+    eb.set_batched(0)
+    for n in (steps, 4):
+        ea.steps(n, use_graph=True)
+        eb.steps(n, use_graph=True)
+        for w in C:
+            w.ibm.coupled_steps(n, use_graph=False)
+        assert ea.batched and not eb.batched
+        for a, b, c in zip(A, B, C):
+            assert a.lattice.steps == b.lattice.steps == c.lattice.steps
+            assert np.array_equal(a.ibm.positions(), c.ibm.positions())
+            assert np.array_equal(b.ibm.positions(), c.ibm.positions())
+            assert np.array_equal(a.lattice.get_populations(), c.lattice.get_populations())
+            assert np.array_equal(b.lattice.get_populations(), c.lattice.get_populations())
+    ea.close()
+    eb.close()
+    _close(A + B + C)
+
+
+def test_batched_mode_refuses_boxes_that_differ(device):
+    """Batched-only mode reports boxes it cannot batch (here a box without
+    immersed boundary, whose collision has no force field); the default
+    mode falls back to graph branches and steps them correctly."""
+    from biblbm_amd import capi
+    from biblbm_amd.ensemble import Ensemble
+    from biblbm_amd.lattice import Lattice
+    ws = _boxes(sides=(24,))
+    L = Lattice(16, 16, 16, periodic=(1, 1, 1), storage="f64", omega=1 / 0.8)
+    L.init_equilibrium()
+    ens = Ensemble([(ws[0].lattice, ws[0].ibm), (L, None)])
+    ens.set_batched(1)
+    with pytest.raises(capi.BlError):
+        ens.steps(2)
+    ens.set_batched(-1)
+    ens.steps(2)
+    assert not ens.batched
+    with pytest.raises(capi.BlError):
+        ens.set_batched(2)
+    ens.close()
+    L.close()
+    _close(ws)
+
+
+def test_ensemble_tuner_keeps_the_trajectory(device):
+    """The ensemble tuner's trials (batched collide workgroup orders) are
+    real steps that give the values of the boxes stepped alone."""
+    from biblbm_amd.ensemble import Ensemble
+    A, B = _boxes(), _boxes()
+    ens = Ensemble([(w.lattice, w.ibm) for w in A])
+    rep = ens.tune(2)
+    assert rep["batched"] == 1
+    assert rep["best"]["even"] in (0, 1, 4, 16, 64, 256) and rep["best"]["odd"] in (0, 1, 4, 16, 64, 256)
+    n = rep["steps_run"]
+    assert A[0].lattice.steps == n
+    ens.steps(4)
+    for w in B:
+        w.ibm.coupled_steps(n + 4, use_graph=False)
+    for a, b in zip(A, B):
+        assert np.array_equal(a.ibm.positions(), b.ibm.positions())
+        assert np.array_equal(a.lattice.get_populations(), b.lattice.get_populations())
+    ens.close()
+    _close(A + B)
