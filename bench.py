@@ -18,9 +18,12 @@ same JSON line:
                 from HIP events on the lattice stream, against 8 TB/s; with
                 the per-parity times, the tuned launch shapes and the
                 measured device-copy bandwidth at 4 GiB
-  cpu_baseline  the FP64 C restatement (oracle) running the same coupled
-                step on this host's cores, a bounded number of steps, and
-                the GPU's vertex positions after the same steps against it
+  cpu_baseline  the reference's own lattice passes of a coupled step
+                (compiled from its sources, serial) on the headline lattice,
+                two steps; beside it (port_coupled_step) the FP64 C
+                restatement (oracle) running the whole coupled step on this
+                host's cores, and the GPU's vertex positions after the same
+                steps against it
   precision     the headline workload in FP32 and FP16-shifted storage from
                 the same start: speed and velocity / position / deformation
                 index errors against the FP64 run
@@ -143,6 +146,25 @@ def oracle_particles(ibm):
     return out
 
 
+REF_TIME = os.path.join(ROOT, "oracle", "_ref", "ref_time")
+
+
+def reference_baseline(steps=2, timeout_s=240):
+    """The reference's own lattice passes of one coupled step, compiled from
+    its sources (oracle/_ref/ref_time: computeVelocity, setExternalVector,
+    collideAndStream with GuoExternalForceVOFDynamics; serial, one core) on
+    the headline lattice (384^3 Taylor-Green, tau 0.8), `steps` steps.
+    None when the binary is absent."""
+    import subprocess
+    if not os.path.exists(REF_TIME):
+        return None
+    r = subprocess.run([REF_TIME, "384", str(steps), "0.8", "0.03"], capture_output=True, text=True,
+                       timeout=timeout_s)
+    if r.returncode != 0:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def cpu_baseline(steps, budget_s):
     """The oracle (FP64 C restatement, OpenMP in the lattice passes, IBM
     serial) on the headline workload from its start: same lattice, fields,
@@ -253,9 +275,10 @@ def suspension_leg(steps, warmup, horizons=(500, 1000, 2000), tune_steps=4):
         tuned = tuning.tune_ibm(W.ibm, tune_steps)["best"]
         dt = time_coupled(W, steps, warmup)
         done = W.lattice.steps
-        first = "at_%d_steps" % done
         errs = {}
-        for h in [done] + [h for h in horizons if h > done]:
+        for h in horizons:
+            if h < done:
+                raise RuntimeError("suspension leg: horizon %d is before the timed steps end (%d)" % (h, done))
             if h > done:
                 W.ibm.coupled_steps(h - done, use_graph=True)
                 done = h
@@ -273,7 +296,7 @@ def suspension_leg(steps, warmup, horizons=(500, 1000, 2000), tune_steps=4):
         row = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
                    ibm_stage_ms=stages, ibm_launch=tuned)
         if storage != "f64":
-            row["velocity_rel_l2_vs_f64"] = errs[first]["velocity_rel_l2_vs_f64"]
+            row["velocity_rel_l2_vs_f64"] = errs["at_%d_steps" % horizons[0]]["velocity_rel_l2_vs_f64"]
             row["errors"] = errs
         out[storage] = row
         nv, vf = W.vertices, W.params["volume_fraction"]
@@ -506,7 +529,19 @@ def main():
         line["precision"] = precision_leg(max(300, args.steps))
         progress("precision done")
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_budget)
+        port = cpu_baseline(args.cpu_steps, args.cpu_budget)
+        ref = reference_baseline()
+        if ref is not None:
+            line["cpu_baseline"] = dict(
+                value=round(ref["mlups"], 4), unit="MLUPS", cores=1, kind="reference",
+                sample="%d steps of the reference's own lattice passes of a coupled step (computeVelocity, "
+                       "setExternalVector, collideAndStream with GuoExternalForceVOFDynamics; Palabos serial build "
+                       "from the reference sources) on the 384^3 Taylor-Green lattice; its immersed-boundary "
+                       "class needs MPI and is not built, so the 16 capsules' stages are not in this figure"
+                       % ref["steps"],
+                seconds=round(ref["seconds"], 3), setup_seconds=ref["setup_seconds"], port_coupled_step=port)
+        else:
+            line["cpu_baseline"] = port
         progress("cpu baseline done")
     tuning.save()
     print(json.dumps(line), flush=True)

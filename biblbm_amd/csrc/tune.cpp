@@ -321,9 +321,12 @@ int bl_tune_ibm(bl_ibm g, int n_steps, char* report, int cap)
     std::string rep = "{\"trials\": [";
     bool first = true;
     for (int pass = 0; pass < 2 && rc == BL_OK; ++pass)
-        for (int k = 0; k < BL_IBM_NLAUNCH && rc == BL_OK; ++k)
+        for (int k = 0; k < BL_IBM_NLAUNCH && rc == BL_OK; ++k) {
+            // every other value of the parameter once per pass, whichever
+            // wins on the way (the number of trials is always the same)
+            const int cur = best[k];
             for (int v : values[k]) {
-                if (v == best[k]) continue;
+                if (v == cur) continue;
                 int p[BL_IBM_NLAUNCH];
                 std::copy(best, best + BL_IBM_NLAUNCH, p);
                 p[k] = v;
@@ -339,6 +342,7 @@ int bl_tune_ibm(bl_ibm g, int n_steps, char* report, int cap)
                     best[k] = v;
                 }
             }
+        }
     hipEventDestroy(a);
     hipEventDestroy(b);
     if (rc) {

@@ -530,9 +530,11 @@ int bl_tune_coupled(bl_ibm g, int n_steps, char *report, int cap);
 /* Times n_steps (even) coupled steps for the immersed-boundary launch
  * parameters of bl_ibm_set_launch, one parameter at a time over its
  * allowed values with the best of the others kept (coordinate descent,
- * two passes), keeps the fastest and records it in the tuning table entry
- * of bl_tune_coupled's key.  The trials are real steps (the report says how
- * many); with the gather lane count the trajectory changes at rounding. */
+ * two passes, each value other than the parameter's value at the start of
+ * its sweep tried once: always 29 runs of 2 n_steps), keeps the fastest and
+ * records it in the tuning table entry of bl_tune_coupled's key.  The
+ * trials are real steps (the report says how many); with the gather lane
+ * count the trajectory changes at rounding. */
 int bl_tune_ibm(bl_ibm g, int n_steps, char *report, int cap);
 /* Applies the tuning table's launch structure and immersed-boundary launch
  * parameters for this boundary, if any. */

@@ -238,6 +238,7 @@ def test_ibm_tuner_persists_and_applies(device, tmp_path):
     s0 = W.lattice.steps
     rep = W.ibm.tune_launch(2)
     best = rep["best"]
+    assert len(rep["trials"]) == 28  # two passes over every other value of each parameter
     assert rep["steps_run"] == 2 * 2 * (1 + len(rep["trials"]))
     assert W.ibm.launch() == {k: best[k] for k in W.ibm.LAUNCH_KEYS}
     assert W.lattice.steps - s0 == rep["steps_run"]
