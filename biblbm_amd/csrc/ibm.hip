@@ -125,6 +125,15 @@ Grid grid_of(const bl_lattice_s* L)
     return g;
 }
 
+// Workgroup order of the immersed-boundary launches over the XCDs: chunks
+// of 2^lc consecutive workgroups on one XCD, so that neighbouring vertices
+// (which share stencil nodes, element records and tiles) meet in one L2;
+// lc = 0 is the hardware's order.  Every order gives the same values.
+__device__ __forceinline__ long ibm_block(long bx, long nbx, int lc)
+{
+    return lc > 0 ? (long)xcd_chunk_block((unsigned)bx, (unsigned)nbx, (unsigned)lc) : bx;
+}
+
 // ------------------------------------------------------------- binning
 // Stencil bases are stored wrapped into [0, n) on periodic axes (raw on the
 // others), so node k of a stencil is base + k minus n at most once.
@@ -182,7 +191,7 @@ struct StencilTiles {
 // (open addressing, linear probing) and then issues one global atomic per
 // distinct tile.  A pair whose tile finds the table full goes straight to
 // the global counter; counts stay exact either way.
-constexpr int kHashBits = 11;
+constexpr int kHashBits = 9;  // 512 slots: a workgroup's 256 vertices touch a few dozen tiles
 constexpr int kHash = 1 << kHashBits;
 
 __device__ __forceinline__ int hash_insert(int* keys, int t)
@@ -365,6 +374,7 @@ struct BinArgs {
     unsigned char* flags;
     int *status, *pairs, *big, *nbig;
     NearView<R> nw;
+    int xcd;  // workgroup order (ibm_block)
 };
 
 template <class R>
@@ -493,7 +503,10 @@ __device__ __forceinline__ void bin_body(long bx, const BinArgs<R>& A)
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) k_bin(BinArgs<R> a) { bin_body<R>(blockIdx.x, a); }
+__global__ void __launch_bounds__(256) k_bin(BinArgs<R> a)
+{
+    bin_body<R>(ibm_block(blockIdx.x, gridDim.x, a.xcd), a);
+}
 
 __global__ void k_clear_bins(int* count, long n)
 {
@@ -709,6 +722,7 @@ struct ElemArgs {
     ForceView<R> f;
     long NT, NP, NL, ec_pair, ec_link;
     R* ec;
+    int xcd;  // workgroup order (ibm_block)
 };
 
 template <class R>
@@ -740,7 +754,10 @@ __device__ __forceinline__ void elem_body(long bx, const ElemArgs<R>& A)
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) k_elem_forces(ElemArgs<R> a) { elem_body<R>(blockIdx.x, a); }
+__global__ void __launch_bounds__(256) k_elem_forces(ElemArgs<R> a)
+{
+    elem_body<R>(ibm_block(blockIdx.x, gridDim.x, a.xcd), a);
+}
 
 // Force on every vertex: the contributions of its incident elements --
 // links (src/fsi/RBCParticle.hh:149-171), triangles
@@ -768,6 +785,7 @@ struct GatherArgs {
     DecoSet deco;
     const R* fpp;
     const R* ext;  // host-given vertex forces, added after the decorators (or null)
+    int xcd;       // workgroup order (ibm_block)
 };
 
 template <class R, int GL>
@@ -851,7 +869,10 @@ __device__ __forceinline__ void gather_body(long bx, const GatherArgs<R>& A)
 }
 
 template <class R, int GL>
-__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs<R> a) { gather_body<R, GL>(blockIdx.x, a); }
+__global__ void __launch_bounds__(256) k_vertex_gather(GatherArgs<R> a)
+{
+    gather_body<R, GL>(ibm_block(blockIdx.x, gridDim.x, a.xcd), a);
+}
 
 // ------------------------------------------------- velocity staging
 // The progs' computeVelocity pass (staged_velocity), restricted to the
@@ -941,6 +962,7 @@ struct InterpArgs {
     R *vel, *pos;
     ListReset lr;
     NearView<R> nw;
+    int xcd;  // workgroup order of the vertices (ibm_block; the list reset keeps the hardware's)
 };
 
 template <class C, int SW, bool MOVE, class R>
@@ -957,7 +979,7 @@ __device__ __forceinline__ void interp_body(long bx, int nbx, const InterpArgs<R
     if (A.lr.act) list_reset(A.lr, (int)bx, nbx);  // the next binning's list, carried by this launch
     constexpr int LA = SW == 2 ? 2 : 4;
     constexpr int LPV = LA * LA;
-    const long gid = bx * blockDim.x + threadIdx.x;
+    const long gid = ibm_block(bx, nbx, A.xcd) * blockDim.x + threadIdx.x;
     const long v = gid / LPV;
     const int r = (int)(gid % LPV);
     const int j = r / LA, k = r % LA;
@@ -1383,6 +1405,7 @@ struct SpreadArgs {
     const R *W, *vforce;
     NearView<R> nw;
     void* Fout;  // force field in the storage's force type
+    int xcd;     // order of the small-tile workgroups over the XCDs (ibm_block)
 };
 
 template <class S, class R>
@@ -1394,8 +1417,8 @@ __device__ __forceinline__ void spread_body(int bx, int nbx, const SpreadArgs<R>
     __shared__ R ew_all[kSpreadWaves][15][kChunk + 1];
     __shared__ R part[kSpreadWaves][3][64];
     if (bx < A.n_small)
-        spread_small<FT, R>(bx, A.n_small, ew_all, A.lv, A.gd, A.Sw, A.active, A.nactive, A.count, A.cap, A.list,
-                            A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
+        spread_small<FT, R>((int)ibm_block(bx, A.n_small, A.xcd), A.n_small, ew_all, A.lv, A.gd, A.Sw, A.active,
+                            A.nactive, A.count, A.cap, A.list, A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
     else
         spread_big<FT, R>(bx - A.n_small, nbx - A.n_small, keys, runs, ew_all, part, A.lv, A.gd, A.Sw, A.big, A.nbig,
                           A.count, A.cap, A.list, A.base, A.W, A.vforce, A.nw, (FT*)A.Fout);
@@ -1416,8 +1439,8 @@ template <class R>
 __global__ void __launch_bounds__(256) k_bin_elem(BinArgs<R> b, int n1, ElemArgs<R> e)
 {
     const int bx = blockIdx.x;
-    if (bx < n1) bin_body<R>(bx, b);
-    else elem_body<R>(bx - n1, e);
+    if (bx < n1) bin_body<R>(ibm_block(bx, n1, b.xcd), b);
+    else elem_body<R>(ibm_block(bx - n1, (int)gridDim.x - n1, e.xcd), e);
 }
 
 // Vertex force gather (element forces) | velocity staging (the binning's
@@ -1426,7 +1449,7 @@ template <class S, bool SWAPPED, class R, int GL>
 __global__ void __launch_bounds__(256) k_gather_stage(GatherArgs<R> ga, int n1, StageArgs sa)
 {
     const int bx = blockIdx.x;
-    if (bx < n1) gather_body<R, GL>(bx, ga);
+    if (bx < n1) gather_body<R, GL>(ibm_block(bx, n1, ga.xcd), ga);
     else stage_body<S, SWAPPED>(bx - n1, sa);
 }
 
@@ -2015,6 +2038,7 @@ BinArgs<R> bin_args(bl_ibm g)
     a.big = g->active + (2L + g->alt) * L->ntiles;
     a.nbig = g->n_active + 8 + g->alt;
     a.nw = near_view<R>(g);
+    a.xcd = g->lp[BL_IBM_XCD];
     return a;
 }
 
@@ -2108,6 +2132,7 @@ InterpArgs<R> interp_args(bl_ibm g, const ListReset& lr)
     a.pos = (R*)g->pos;
     a.lr = lr;
     a.nw = near_view<R>(g);
+    a.xcd = g->lp[BL_IBM_XCD];
     return a;
 }
 
@@ -2178,6 +2203,7 @@ SpreadArgs<R> spread_args(bl_ibm g, unsigned& blocks, int share = 1)
     a.vforce = (const R*)g->force;
     a.nw = near_view<R>(g);
     a.Fout = g->force_alt;
+    a.xcd = g->lp[BL_IBM_XCD];
     // about one small-tile wave per touched tile and one workgroup per big
     // tile: the touched tiles are at most every tile and, for membranes,
     // fewer than the vertices (by default a wave per vertex); big tiles
@@ -2252,6 +2278,7 @@ ElemArgs<R> elem_args(bl_ibm g)
     a.ec_pair = g->ec_pair;
     a.ec_link = g->ec_link;
     a.ec = (R*)g->ec;
+    a.xcd = g->lp[BL_IBM_XCD];
     return a;
 }
 
@@ -2271,6 +2298,7 @@ GatherArgs<R> gather_args(bl_ibm g, bool walls)
     else a.deco.n = 0;
     a.fpp = walls && !g->pp.empty() ? (const R*)g->pp_force : nullptr;
     a.ext = walls && g->has_ext ? (const R*)g->ext_force : nullptr;
+    a.xcd = g->lp[BL_IBM_XCD];
     return a;
 }
 
@@ -2666,8 +2694,8 @@ __global__ void __launch_bounds__(256) k_bin_elem_batch(const LayeredEntry<R>* _
     const int box = batch_box(base, nbox, (int)blockIdx.x);
     const LayeredEntry<R>& E = tab[box];
     const int bx = (int)blockIdx.x - base[box];
-    if (bx < E.nb) bin_body<R>(bx, E.bin);
-    else elem_body<R>(bx - E.nb, E.elem);
+    if (bx < E.nb) bin_body<R>(ibm_block(bx, E.nb, E.bin.xcd), E.bin);
+    else elem_body<R>(ibm_block(bx - E.nb, E.ne, E.elem.xcd), E.elem);
 }
 
 template <class S, bool SWAPPED, class R, int GL>
@@ -2677,7 +2705,7 @@ __global__ void __launch_bounds__(256) k_gather_stage_batch(const LayeredEntry<R
     const int box = batch_box(base, nbox, (int)blockIdx.x);
     const LayeredEntry<R>& E = tab[box];
     const int bx = (int)blockIdx.x - base[box];
-    if (bx < E.ng) gather_body<R, GL>(bx, E.gather);
+    if (bx < E.ng) gather_body<R, GL>(ibm_block(bx, E.ng, E.gather.xcd), E.gather);
     else stage_body<S, SWAPPED>(bx - E.ng, E.stage);
 }
 
@@ -3832,7 +3860,8 @@ int bl_ibm_set_launch(bl_ibm g, const int params[BL_IBM_NLAUNCH])
     const int* p = params;
     if (!in(p[BL_IBM_BIN_BLOCK], {64, 128, 256}) || !in(p[BL_IBM_ELEM_BLOCK], {64, 128, 256}) ||
         !in(p[BL_IBM_GATHER_LANES], {2, 4, 8}) || !in(p[BL_IBM_INTERP_BLOCK], {64, 128, 256}) ||
-        !in(p[BL_IBM_SPREAD_SMALL], {1, 2, 4, 8}) || !in(p[BL_IBM_SPREAD_BIG], {16, 32, 64, 128}))
+        !in(p[BL_IBM_SPREAD_SMALL], {1, 2, 4, 8}) || !in(p[BL_IBM_SPREAD_BIG], {16, 32, 64, 128}) ||
+        !in(p[BL_IBM_XCD], {0, 2, 4, 6}))
         return fail(BL_ERR_ARG, "bl_ibm_set_launch: a launch parameter is outside its allowed values");
     bool same = true;
     for (int k = 0; k < BL_IBM_NLAUNCH; ++k) same = same && g->lp[k] == p[k];

@@ -157,8 +157,8 @@ struct bl_ibm_s {
     // chosen by bl_tune_ibm), indexed by BL_IBM_*: threads per workgroup of
     // the binning, element forces and interpolation, lanes per vertex of the
     // force gather, small-tile spreading waves per 4 vertices, vertices per
-    // big-tile spreading workgroup
-    int lp[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32};
+    // big-tile spreading workgroup, workgroup order over the XCDs
+    int lp[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32, 0};
     int fused_staging = -1;  // velocity staging carried by the collision: -1 auto (large vertex counts), 0 off, 1 on
     int layered = -1;        // stages paired on one stream: -1 auto (small vertex counts), 0 off (forked), 1 on
 

@@ -37,7 +37,7 @@ struct CoupledEntry {
     long vertices = 0;
     int fused = -1, layered = -1;
     double ms = 0.0;
-    int lp[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32};  // bl_ibm_set_launch
+    int lp[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32, 0};  // bl_ibm_set_launch
     bool has_lp = false;
 };
 
@@ -293,9 +293,9 @@ int bl_tune_ibm(bl_ibm g, int n_steps, char* report, int cap)
     if (rc) return rc;
     bl_lattice L = g->lat;
     static const std::vector<int> values[BL_IBM_NLAUNCH] = {
-        {64, 128, 256}, {64, 128, 256}, {2, 4, 8}, {64, 128, 256}, {1, 2, 4, 8}, {16, 32, 64, 128}};
+        {64, 128, 256}, {64, 128, 256}, {2, 4, 8}, {64, 128, 256}, {1, 2, 4, 8}, {16, 32, 64, 128}, {0, 2, 4, 6}};
     static const char* names[BL_IBM_NLAUNCH] = {"bin_block", "elem_block", "gather_lanes", "interp_block",
-                                                "spread_small", "spread_big"};
+                                                "spread_small", "spread_big", "xcd"};
     int keep[BL_IBM_NLAUNCH], best[BL_IBM_NLAUNCH];
     if ((rc = bl_ibm_get_launch(g, keep))) return rc;
     std::copy(keep, keep + BL_IBM_NLAUNCH, best);
@@ -403,9 +403,9 @@ int bl_tune_cache_save(const char* path)
         snprintf(buf, sizeof(buf),
                  "{\"kind\": \"coupled\", \"device\": \"%s\", \"nx\": %d, \"ny\": %d, \"nz\": %d, \"storage\": %d, "
                  "\"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, \"ms_per_step\": %.6f, "
-                 "\"ibm_launch\": [%d, %d, %d, %d, %d, %d]}",
+                 "\"ibm_launch\": [%d, %d, %d, %d, %d, %d, %d]}",
                  e.device.c_str(), e.nx, e.ny, e.nz, e.storage, e.vertices, e.fused, e.layered, e.ms,
-                 e.has_lp ? e.lp[0] : 0, e.lp[1], e.lp[2], e.lp[3], e.lp[4], e.lp[5]);
+                 e.has_lp ? e.lp[0] : 0, e.lp[1], e.lp[2], e.lp[3], e.lp[4], e.lp[5], e.lp[6]);
         out << buf << "\n";
     }
     return out ? BL_OK : fail(BL_ERR_ARG, std::string("bl_tune_cache_save: write failed: ") + path);
@@ -449,9 +449,9 @@ int bl_tune_cache_load(const char* path)
         } else if (sscanf(line.c_str(),
                           "{\"kind\": \"coupled\", \"device\": \"%127[^\"]\", \"nx\": %d, \"ny\": %d, \"nz\": %d, "
                           "\"storage\": %d, \"vertices\": %ld, \"fused_staging\": %d, \"layered\": %d, "
-                          "\"ms_per_step\": %lf, \"ibm_launch\": [%d, %d, %d, %d, %d, %d]}",
+                          "\"ms_per_step\": %lf, \"ibm_launch\": [%d, %d, %d, %d, %d, %d, %d]}",
                           dev, &d.nx, &d.ny, &d.nz, &d.storage, &d.vertices, &d.fused, &d.layered, &d.ms, &d.lp[0],
-                          &d.lp[1], &d.lp[2], &d.lp[3], &d.lp[4], &d.lp[5]) == 15) {
+                          &d.lp[1], &d.lp[2], &d.lp[3], &d.lp[4], &d.lp[5], &d.lp[6]) == 16) {
             if (d.fused < -1 || d.fused > 1 || d.layered < -1 || d.layered > 1) {
                 ++bad;
                 continue;
@@ -463,12 +463,12 @@ int bl_tune_cache_load(const char* path)
             };
             if (d.has_lp && !(in(d.lp[0], {64, 128, 256}) && in(d.lp[1], {64, 128, 256}) && in(d.lp[2], {2, 4, 8}) &&
                               in(d.lp[3], {64, 128, 256}) && in(d.lp[4], {1, 2, 4, 8}) &&
-                              in(d.lp[5], {16, 32, 64, 128}))) {
+                              in(d.lp[5], {16, 32, 64, 128}) && in(d.lp[6], {0, 2, 4, 6}))) {
                 ++bad;
                 continue;
             }
             if (!d.has_lp) {
-                const int dflt[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32};
+                const int dflt[BL_IBM_NLAUNCH] = {256, 256, 4, 256, 4, 32, 0};
                 std::copy(dflt, dflt + BL_IBM_NLAUNCH, d.lp);
             }
             d.device = dev;

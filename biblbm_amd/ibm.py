@@ -245,7 +245,7 @@ class ImmersedBoundary:
         one stream with paired stages (bl_ibm_set_layered)."""
         call("bl_ibm_set_layered", self.handle, int(mode))
 
-    LAUNCH_KEYS = ("bin_block", "elem_block", "gather_lanes", "interp_block", "spread_small", "spread_big")
+    LAUNCH_KEYS = ("bin_block", "elem_block", "gather_lanes", "interp_block", "spread_small", "spread_big", "xcd")
 
     def set_launch(self, **params):
         """Immersed-boundary launch parameters (bl_ibm_set_launch), by name;

@@ -279,7 +279,11 @@ int bl_ibm_set_layered(bl_ibm g, int mode);
  *   BL_IBM_INTERP_BLOCK  threads per interpolation workgroup (64..256)
  *   BL_IBM_SPREAD_SMALL  small-tile spreading waves per 4 vertices (1..8)
  *   BL_IBM_SPREAD_BIG    vertices per big-tile spreading workgroup (16..128)
- * Defaults {256, 256, 4, 256, 4, 32}.  Only the gather lane count changes
+ *   BL_IBM_XCD           workgroup order over the XCDs of the vertex,
+ *                        element and small-tile launches: chunks of 2^k
+ *                        consecutive workgroups on one XCD (k = 0, 2, 4, 6;
+ *                        0 the hardware's order)
+ * Defaults {256, 256, 4, 256, 4, 32, 0}.  Only the gather lane count changes
  * an order of summation (its butterfly): results agree to rounding across
  * lane counts; every other parameter leaves them bitwise the same.
  * bl_ibm_set_launch refuses values outside those sets (BL_ERR_ARG). */
@@ -289,7 +293,8 @@ int bl_ibm_set_layered(bl_ibm g, int mode);
 #define BL_IBM_INTERP_BLOCK 3
 #define BL_IBM_SPREAD_SMALL 4
 #define BL_IBM_SPREAD_BIG 5
-#define BL_IBM_NLAUNCH 6
+#define BL_IBM_XCD 6
+#define BL_IBM_NLAUNCH 7
 int bl_ibm_set_launch(bl_ibm g, const int params[BL_IBM_NLAUNCH]);
 int bl_ibm_get_launch(bl_ibm g, int params[BL_IBM_NLAUNCH]);
 
@@ -531,7 +536,7 @@ int bl_tune_coupled(bl_ibm g, int n_steps, char *report, int cap);
  * parameters of bl_ibm_set_launch, one parameter at a time over its
  * allowed values with the best of the others kept (coordinate descent,
  * two passes, each value other than the parameter's value at the start of
- * its sweep tried once: always 29 runs of 2 n_steps), keeps the fastest and
+ * its sweep tried once: always 35 runs of 2 n_steps), keeps the fastest and
  * records it in the tuning table entry of bl_tune_coupled's key.  The
  * trials are real steps (the report says how many); with the gather lane
  * count the trajectory changes at rounding. */

@@ -169,7 +169,7 @@ def _small_suspension(storage="f64", seed=3):
 
 ORDER_KEEPING = [dict(bin_block=64), dict(bin_block=128), dict(elem_block=64), dict(elem_block=128),
                  dict(interp_block=64), dict(interp_block=128), dict(spread_small=1), dict(spread_small=8),
-                 dict(spread_big=16), dict(spread_big=128)]
+                 dict(spread_big=16), dict(spread_big=128), dict(xcd=2), dict(xcd=4), dict(xcd=6)]
 
 
 @pytest.mark.parametrize("fused,layered", [(0, 1), (1, 0), (0, 0)])
@@ -221,7 +221,7 @@ def test_ibm_launch_rejects_values_outside_the_sets(device):
     W = _small_suspension()
     before = W.ibm.launch()
     for bad in (dict(bin_block=512), dict(gather_lanes=3), dict(spread_small=16), dict(spread_big=8),
-                dict(interp_block=32), dict(elem_block=0)):
+                dict(interp_block=32), dict(elem_block=0), dict(xcd=3), dict(xcd=8)):
         with pytest.raises(capi.BlError):
             W.ibm.set_launch(**bad)
         assert W.ibm.launch() == before
@@ -238,7 +238,7 @@ def test_ibm_tuner_persists_and_applies(device, tmp_path):
     s0 = W.lattice.steps
     rep = W.ibm.tune_launch(2)
     best = rep["best"]
-    assert len(rep["trials"]) == 28  # two passes over every other value of each parameter
+    assert len(rep["trials"]) == 34  # two passes over every other value of each parameter
     assert rep["steps_run"] == 2 * 2 * (1 + len(rep["trials"]))
     assert W.ibm.launch() == {k: best[k] for k in W.ibm.LAUNCH_KEYS}
     assert W.lattice.steps - s0 == rep["steps_run"]
