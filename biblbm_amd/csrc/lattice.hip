@@ -119,17 +119,24 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
     }
 
     const unsigned char m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
+    // the moments of the loaded populations, shared by the staging (EMIT)
+    // and the collision: the same expressions either would form
+    const C rhoBar = sum_rho(f);
+    C jm[3];
+    sum_j(f, jm);
+    const C invRho = C(1) / (rhoBar + C(1));
     if (EMIT) {
         const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
         if (v.flags[t])
-            ((Vel4<C>*)v.emit_u)[c] = staged_velocity<FT, C>(v, v.emit_flags, v.emit_force, m, c, x, y, z, f);
+            ((Vel4<C>*)v.emit_u)[c] =
+                staged_velocity_m<FT, C>(v, v.emit_flags, v.emit_force, m, c, x, y, z, jm, invRho);
     }
     if (m == M_FLUID || m == M_VWALL) {
         C F[3] = {C(0), C(0), C(0)};
         if (FMODE == 1) { F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2]; }
         if (FMODE == 2) cell_force<FT, C>(v, c, x, y, z, F);
         const C omega = PHASE ? (v.phase[c] ? (C)v.om1 : (C)v.om0) : (C)v.om0;
-        collide<C, FMODE != 0>(f, F, omega);
+        collide_m<C, FMODE != 0>(f, F, omega, rhoBar, jm, invRho);
     } else if (m == M_BB) {
 #pragma unroll
         for (int i = 1; i <= 9; ++i) {

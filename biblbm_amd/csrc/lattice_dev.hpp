@@ -314,13 +314,13 @@ __device__ __forceinline__ void sum_j(const C f[Q], C j[3])
     j[2] = zp - zm;
 }
 
+// The collision with the moments of f already formed (rhoBar = sum_rho(f),
+// j = sum_j(f), invRho = 1 / (rhoBar + 1)): a caller that also needs them
+// (the velocity staging carried by the collision) forms them once.
 template <class C, bool FORCE>
-__device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
+__device__ __forceinline__ void collide_m(C f[Q], const C F[3], C omega, C rhoBar, const C jm[3], C invRho)
 {
-    const C rhoBar = sum_rho(f);
-    C j[3];
-    sum_j(f, j);
-    const C invRho = C(1) / (rhoBar + C(1));
+    const C j[3] = {jm[0], jm[1], jm[2]};
     C u[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) u[d] = FORCE ? j[d] * invRho + F[d] / C(2) : j[d] * invRho;
@@ -386,6 +386,16 @@ __device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
     }
 }
 
+template <class C, bool FORCE>
+__device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
+{
+    const C rhoBar = sum_rho(f);
+    C j[3];
+    sum_j(f, j);
+    const C invRho = C(1) / (rhoBar + C(1));
+    collide_m<C, FORCE>(f, F, omega, rhoBar, j, invRho);
+}
+
 // ------------------------------------------------------ velocity staging
 // Staged velocity: one aligned 4-vector per cell (u, 0) so that a z-row of
 // a stencil is one contiguous run of memory.
@@ -400,9 +410,12 @@ struct alignas(4 * sizeof(C)) Vel4 {
 // bounce-back / no-dynamics cells, the wall velocity on velocity-wall
 // cells).  (flags, force) is the last collision's force field, with the
 // last collision's body force in the tiles it does not flag.
+// The same from the cell's moments (j = sum_j(f), invRho = 1 / (sum_rho(f)
+// + 1)), which the collision that carries the staging forms anyway.
 template <class FT, class C>
-__device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsigned char* flags, const void* force,
-                                                   unsigned char m, long c, int x, int y, int z, const C f[Q])
+__device__ __forceinline__ Vel4<C> staged_velocity_m(const LatView& v, const unsigned char* flags, const void* force,
+                                                     unsigned char m, long c, int x, int y, int z, const C j[3],
+                                                     C invRho)
 {
     C u[3] = {C(0), C(0), C(0)};
     if (m == M_VWALL) {
@@ -412,13 +425,20 @@ __device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsig
     } else if (m == M_FLUID) {
         C Fc[3];
         cell_force_in<FT, C>(v, v.g_last, flags, force, c, x, y, z, Fc);
-        const C rb = sum_rho(f);
-        C j[3];
-        sum_j(f, j);
-        const C invRho = C(1) / (rb + C(1));
         for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
     }
     return Vel4<C>{u[0], u[1], u[2], C(0)};
+}
+
+template <class FT, class C>
+__device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsigned char* flags, const void* force,
+                                                   unsigned char m, long c, int x, int y, int z, const C f[Q])
+{
+    const C rb = sum_rho(f);
+    C j[3];
+    sum_j(f, j);
+    const C invRho = C(1) / (rb + C(1));
+    return staged_velocity_m<FT, C>(v, flags, force, m, c, x, y, z, j, invRho);
 }
 
 }  // namespace bl
