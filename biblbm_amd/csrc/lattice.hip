@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace bl {
@@ -83,6 +84,81 @@ __device__ __forceinline__ K odd_slot(const NbrOff& nb, bool inner, int sx, int 
     return valid ? (K)p_valid * (K)st + (K)(c + d) : (K)p_face * (K)st + (K)c;
 }
 
+// What one cell's update reads besides its populations: the mask code, the
+// collision force and relaxation rate, and for the velocity staging (EMIT)
+// whether the cell's tile is flagged and the last collision's force at the
+// cell.  Gathered in one place so that the z-row kernel can issue these
+// loads before its first barrier.
+template <class C>
+struct CellIn {
+    unsigned char m;
+    bool emit;
+    C F[3], omega, Fe[3];
+};
+
+// The tile's flags (the force field's, v.flags, and the last collision's,
+// v.emit_flags) load first, since the force loads wait for them.
+struct TileFlags {
+    bool cur, emit;
+};
+
+template <int FMODE, bool EMIT>
+__device__ __forceinline__ TileFlags cs_flags(const LatView& v, int x, int y, int z)
+{
+    TileFlags tf{false, false};
+    if (FMODE == 2 || EMIT) {
+        const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
+        if (FMODE == 2) tf.cur = v.flags[t] != 0;
+        if (EMIT && v.emit_flags) tf.emit = v.emit_flags[t] != 0;
+    }
+    return tf;
+}
+
+template <class S, bool MASK, int FMODE, bool PHASE, bool EMIT>
+__device__ __forceinline__ CellIn<typename Pop<S>::C> cs_inputs(const LatView& v, unsigned c, const TileFlags& tf)
+{
+    using C = typename Pop<S>::C;
+    using FT = typename Pop<S>::F;
+    CellIn<C> in;
+    in.m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
+    in.F[0] = in.F[1] = in.F[2] = C(0);
+    if (FMODE == 1) { in.F[0] = (C)v.g[0]; in.F[1] = (C)v.g[1]; in.F[2] = (C)v.g[2]; }
+    if (FMODE == 2) cell_force_fl<FT, C>(v, v.g, tf.cur, v.force, c, in.F);
+    in.omega = PHASE ? (v.phase[c] ? (C)v.om1 : (C)v.om0) : (C)v.om0;
+    in.emit = EMIT && tf.cur;
+    in.Fe[0] = in.Fe[1] = in.Fe[2] = C(0);
+    if (EMIT && in.emit && in.m == M_FLUID) cell_force_fl<FT, C>(v, v.g_last, tf.emit, v.emit_force, c, in.Fe);
+    return in;
+}
+
+// The update of one cell from its loaded populations f (in the collide
+// order) and its other inputs: moments, the velocity staging (EMIT), then
+// the collision of fluid and velocity-wall cells or the swap of a
+// bounce-back cell.
+template <class S, bool FORCE, bool EMIT>
+__device__ __forceinline__ void cs_update(const LatView& v, unsigned c, int x, int y, int z, typename Pop<S>::C f[Q],
+                                          const CellIn<typename Pop<S>::C>& in)
+{
+    using C = typename Pop<S>::C;
+    // the moments of the loaded populations, shared by the staging (EMIT)
+    // and the collision: the same expressions either would form
+    const C rhoBar = sum_rho(f);
+    C jm[3];
+    sum_j(f, jm);
+    const C invRho = C(1) / (rhoBar + C(1));
+    if (EMIT && in.emit) ((Vel4<C>*)v.emit_u)[c] = staged_velocity_f<C>(v, in.m, x, y, z, jm, invRho, in.Fe);
+    if (in.m == M_FLUID || in.m == M_VWALL) {
+        collide_m<C, FORCE>(f, in.F, in.omega, rhoBar, jm, invRho);
+    } else if (in.m == M_BB) {
+#pragma unroll
+        for (int i = 1; i <= 9; ++i) {
+            const C t = f[i];
+            f[i] = f[i + 9];
+            f[i + 9] = t;
+        }
+    }
+}
+
 // One cell of the collide-and-stream pass.  EMIT: the velocity staging of an
 // immersed-boundary step rides on the launch (bl_lattice_s::emit_u): cells
 // of flagged tiles also write the computeVelocity value of the populations
@@ -94,7 +170,6 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
 {
     using P = Pop<S>;
     using C = typename P::C;
-    using FT = typename P::F;
     S* __restrict__ A = (S*)v.pops;
     const long st = v.stride;
 
@@ -118,33 +193,8 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
         for (int i = 0; i < Q; ++i) f[i] = P::template ld<unsigned, NT != 0>(A + i * st, c, i);
     }
 
-    const unsigned char m = MASK ? v.mask[c] : (unsigned char)M_FLUID;
-    // the moments of the loaded populations, shared by the staging (EMIT)
-    // and the collision: the same expressions either would form
-    const C rhoBar = sum_rho(f);
-    C jm[3];
-    sum_j(f, jm);
-    const C invRho = C(1) / (rhoBar + C(1));
-    if (EMIT) {
-        const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
-        if (v.flags[t])
-            ((Vel4<C>*)v.emit_u)[c] =
-                staged_velocity_m<FT, C>(v, v.emit_flags, v.emit_force, m, c, x, y, z, jm, invRho);
-    }
-    if (m == M_FLUID || m == M_VWALL) {
-        C F[3] = {C(0), C(0), C(0)};
-        if (FMODE == 1) { F[0] = (C)v.g[0]; F[1] = (C)v.g[1]; F[2] = (C)v.g[2]; }
-        if (FMODE == 2) cell_force<FT, C>(v, c, x, y, z, F);
-        const C omega = PHASE ? (v.phase[c] ? (C)v.om1 : (C)v.om0) : (C)v.om0;
-        collide_m<C, FMODE != 0>(f, F, omega, rhoBar, jm, invRho);
-    } else if (m == M_BB) {
-#pragma unroll
-        for (int i = 1; i <= 9; ++i) {
-            const C t = f[i];
-            f[i] = f[i + 9];
-            f[i + 9] = t;
-        }
-    }
+    cs_update<S, FMODE != 0, EMIT>(v, c, x, y, z, f,
+                                   cs_inputs<S, MASK, FMODE, PHASE, EMIT>(v, c, cs_flags<FMODE, EMIT>(v, x, y, z)));
 
     if (ODD) {
 #pragma unroll
@@ -156,6 +206,121 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) P::template st<unsigned, NT != 0>(A + opp(i) * st, c, i, f[i]);
+    }
+}
+
+// Slot of population i in the LDS rows of the z-segment kernel: the ten
+// populations with c_z != 0, in index order (-1 for the others).
+__host__ __device__ constexpr int zslot(int i)
+{
+    int k = 0;
+    for (int j = 0; j < i; ++j) k += cz(j) != 0;
+    return cz(i) != 0 ? k : -1;
+}
+static_assert(zslot(3) == 0 && zslot(7) == 2 && zslot(18) == 9, "ten populations move along z");
+
+// The odd step over z-segments (launch block -B: one workgroup of B lanes
+// per segment of B cells of a z-row, nz a multiple of B, z periodic): lane
+// l reads and writes slot zb + l of every row it addresses, so the
+// population accesses are z-aligned.  The z +-1 shift of the ten
+// populations with c_z != 0 goes through LDS: after the loads lane l takes
+// the value that lane l - c_z loaded (the first and last lanes also load
+// the one cell before and after the segment), before the stores it hands
+// its value to lane l + c_z, and the value leaving the segment is stored
+// by its own lane.  A row past a non-periodic x or y face reflects, as in
+// cs_cell: its populations come from, and go to, the cell's own slots of
+// the opposite plane, with no shift.  Row offsets are uniform over the
+// workgroup, so addresses are a scalar base plus the lane's z.
+// v.row_segs holds nz / B; v.xcd_remap deals the segments over the XCDs
+// as collide_block deals blocks.
+template <class S, bool MASK, int FMODE, bool PHASE, bool EMIT, bool NT>
+__global__ void __launch_bounds__(256) k_collide_zseg(LatView v)
+{
+    using P = Pop<S>;
+    using C = typename P::C;
+    extern __shared__ double lds_rows[];
+    const int B = blockDim.x, W = B + 2;
+    C* rot = (C*)lds_rows;  // [10][B + 2]: slot l + 1 the segment's cell l, 0 and B + 1 its neighbours
+    const int nz = v.nz;
+    const unsigned nseg = (unsigned)v.row_segs;
+    const unsigned bid = collide_block(blockIdx.x, gridDim.x, v.xcd_remap);
+    const unsigned row = bid / nseg;
+    const int zb = (int)(bid - row * nseg) * B;
+    const int l = threadIdx.x, z = zb + l;
+    const unsigned q = fdiv(row, v.dy);
+    const int x = (int)q, y = (int)(row - q * (unsigned)v.ny);
+    const unsigned c = row * (unsigned)nz + (unsigned)z;
+    const unsigned own = row * (unsigned)nz;
+    S* __restrict__ A = (S*)v.pops;
+    const unsigned st = (unsigned)v.stride;  // Q * stride < 2^32 (host checks)
+    // start of row (x + a - 1, y + b - 1), or -1 past a non-periodic face
+    int R[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int xs = wrapc(x + a - 1, v.nx, v.per[0]), ys = wrapc(y + b - 1, v.ny, v.per[1]);
+            R[a][b] = (xs | ys) < 0 ? -1 : (xs * v.ny + ys) * nz;
+        }
+    const unsigned zlo = zb == 0 ? nz - 1 : zb - 1;    // the cell before the segment
+    const unsigned zhi = zb + B == nz ? 0 : zb + B;    // and after it
+
+    // the tile flags first: the force loads wait for them, and so they
+    // arrive while the populations load
+    const TileFlags tf = cs_flags<FMODE, EMIT>(v, x, y, z);
+    C f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int rb = R[1 - cx(i)][1 - cy(i)];
+        const unsigned base = rb >= 0 ? (unsigned)opp(i) * st + (unsigned)rb : (unsigned)i * st + own;
+        f[i] = P::template ld<unsigned, NT>(A + base, (unsigned)z, i);
+    }
+    // the cell's other inputs load beside the populations, not after the
+    // barrier below, which no load crosses
+    const CellIn<C> in = cs_inputs<S, MASK, FMODE, PHASE, EMIT>(v, c, tf);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (cz(i) == 0) continue;
+        const int rb = R[1 - cx(i)][1 - cy(i)];
+        const unsigned base = rb >= 0 ? (unsigned)opp(i) * st + (unsigned)rb : (unsigned)i * st + own;
+        C* r = rot + zslot(i) * W;
+        r[l + 1] = f[i];
+        if (cz(i) > 0 && l == 0) r[0] = P::template ld<unsigned, NT>(A + base, zlo, i);
+        if (cz(i) < 0 && l == B - 1) r[B + 1] = P::template ld<unsigned, NT>(A + base, zhi, i);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (cz(i) == 0) continue;
+        const C w = rot[zslot(i) * W + l + 1 - cz(i)];
+        f[i] = R[1 - cx(i)][1 - cy(i)] >= 0 ? w : f[i];
+    }
+
+    cs_update<S, FMODE != 0, EMIT>(v, c, x, y, z, f, in);
+
+    __syncthreads();  // every lane has read its shifted values
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int rb = R[1 + cx(i)][1 + cy(i)];
+        const unsigned base = rb >= 0 ? (unsigned)i * st + (unsigned)rb : (unsigned)opp(i) * st + own;
+        if (cz(i) == 0) {
+            P::template st<unsigned, NT>(A + base, (unsigned)z, i, f[i]);
+            continue;
+        }
+        rot[zslot(i) * W + l + 1] = f[i];
+        // the value leaving the segment
+        if (rb >= 0 && cz(i) > 0 && l == B - 1) P::template st<unsigned, NT>(A + base, zhi, i, f[i]);
+        if (rb >= 0 && cz(i) < 0 && l == 0) P::template st<unsigned, NT>(A + base, zlo, i, f[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (cz(i) == 0) continue;
+        const int rb = R[1 + cx(i)][1 + cy(i)];
+        const unsigned base = rb >= 0 ? (unsigned)i * st + (unsigned)rb : (unsigned)opp(i) * st + own;
+        const int src = l - cz(i);  // the lane whose value lands on slot z
+        const C w = rot[zslot(i) * W + src + 1];
+        if (rb < 0 || (src >= 0 && src < B)) P::template st<unsigned, NT>(A + base, (unsigned)z, i, rb >= 0 ? w : f[i]);
     }
 }
 
@@ -177,6 +342,13 @@ __global__ void __launch_bounds__(256) k_collide_stream(LatView v)
 template <class S, bool ODD, bool MASK, int FMODE, bool PHASE, class K, bool EMIT, int NT>
 void launch_one(const LatView& v, unsigned nblk, int blk, hipStream_t s)
 {
+    if constexpr (ODD && std::is_same<K, unsigned>::value) {
+        if (blk < 0) {  // collide_plan checked zseg_ok: nblk segments of -blk lanes
+            const size_t lds = 10 * (size_t)(2 - blk) * sizeof(typename Pop<S>::C);
+            hipLaunchKernelGGL((k_collide_zseg<S, MASK, FMODE, PHASE, EMIT, NT != 0>), dim3(nblk), dim3(-blk), lds, s, v);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_collide_stream<S, ODD, MASK, FMODE, PHASE, K, EMIT, NT>), dim3(nblk), dim3(blk), 0, s, v);
 }
 
@@ -229,6 +401,14 @@ void launch_cs(const LatView& v, bool odd, bool mask, int fmode, bool phase, uns
     }
 }
 
+// The z-segment kernel's conditions for segments of B lanes: z periodic,
+// nz a multiple of B, 32-bit slot offsets.
+bool zseg_ok(const bl_lattice_s* L, int B)
+{
+    return L->periodic[2] && B >= 64 && B <= 256 && B % 64 == 0 && L->nz % B == 0 &&
+           (size_t)Q * (size_t)L->stride <= 0xffffffffull;
+}
+
 static int force_mode(const bl_lattice_s* L)
 {
     if (L->tile_flags && L->force_field) return 2;
@@ -252,6 +432,14 @@ CollidePlan collide_plan(const bl_lattice_s* L)
     p.blk = L->block[par];
     v.xcd_remap = L->xcd_remap[par];
     p.nt = L->nontemporal[par];
+    if (p.blk < 0) {
+        if (p.odd && zseg_ok(L, -p.blk)) {
+            v.row_segs = L->nz / -p.blk;
+            p.nblk = (unsigned)((long)L->nx * L->ny * v.row_segs);
+            return p;
+        }
+        p.blk = -p.blk;  // even steps, and lattices the z-segment kernel cannot take
+    }
     p.nblk = (unsigned)((L->ncell + p.blk - 1) / p.blk);
     if (v.xcd_remap == BL_ORDER_ROWS) {
         v.row_segs = (L->nz + p.blk - 1) / p.blk;
@@ -1251,9 +1439,13 @@ int bl_lattice_set_launch_parity(bl_lattice L, int parity, int block, int xcd_re
 {
     if (!L) return fail(BL_ERR_ARG, "null lattice");
     if (parity != 0 && parity != 1) return fail(BL_ERR_ARG, "bl_lattice_set_launch_parity: parity must be 0 or 1");
-    // the collide-stream kernels are compiled for at most 256 threads per block
-    if (block < 64 || block > 256 || block % 64)
-        return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be 64, 128, 192 or 256");
+    // the collide-stream kernels are compiled for at most 256 threads per
+    // block; a negative block asks for the odd step's z-segment kernel
+    const int lanes = block < 0 ? -block : block;
+    if (lanes < 64 || lanes > 256 || lanes % 64)
+        return fail(BL_ERR_ARG, "bl_lattice_set_launch: block must be +-64, +-128, +-192 or +-256");
+    if (block < 0 && xcd_remap == BL_ORDER_ROWS)
+        return fail(BL_ERR_ARG, "bl_lattice_set_launch: z-segment blocks take xcd_remap 0, 1 or a power of two");
     if (xcd_remap < 0 || xcd_remap > (1 << 20) ||
         (xcd_remap > 1 && xcd_remap != BL_ORDER_ROWS && (xcd_remap & (xcd_remap - 1))))
         return fail(BL_ERR_ARG,

@@ -125,6 +125,9 @@ struct CollidePlan {
     unsigned nblk = 0;
 };
 CollidePlan collide_plan(const bl_lattice_s* L);
+// Whether the lattice meets the conditions of the z-segment kernel with
+// segments of B lanes (a launch block of -B).
+bool zseg_ok(const bl_lattice_s* L, int B);
 void collide_commit(bl_lattice_s* L);
 // Batches of boxes (ensembles): one launch over the boxes' views, box k
 // taking workgroups [base[k], base[k + 1]) (device arrays); the plans must

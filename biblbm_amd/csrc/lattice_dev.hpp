@@ -268,21 +268,31 @@ __device__ __forceinline__ void store_post(const LatView& v, const Nbr& n, long 
 }
 
 // External force of cell c in the field (flags, force): the field's value
-// in a flagged tile, the body force gb elsewhere.
+// in a flagged tile, the body force gb elsewhere; cell_force_fl with the
+// tile's flag already loaded.
+template <class FT, class C>
+__device__ __forceinline__ void cell_force_fl(const LatView& v, const double* gb, bool flagged, const void* force,
+                                              long c, C F[3])
+{
+    F[0] = (C)gb[0]; F[1] = (C)gb[1]; F[2] = (C)gb[2];
+    if (flagged) {
+        const FT* Fv = (const FT*)force;
+        F[0] = (C)Fv[c];
+        F[1] = (C)Fv[v.stride + c];
+        F[2] = (C)Fv[2 * v.stride + c];
+    }
+}
+
 template <class FT, class C>
 __device__ __forceinline__ void cell_force_in(const LatView& v, const double* gb, const unsigned char* flags,
                                               const void* force, long c, int x, int y, int z, C F[3])
 {
-    F[0] = (C)gb[0]; F[1] = (C)gb[1]; F[2] = (C)gb[2];
+    bool fl = false;
     if (flags) {
         const long t = ((long)(x >> 2) * v.nty + (y >> 2)) * v.ntz + (z >> 2);
-        if (flags[t]) {
-            const FT* Fv = (const FT*)force;
-            F[0] = (C)Fv[c];
-            F[1] = (C)Fv[v.stride + c];
-            F[2] = (C)Fv[2 * v.stride + c];
-        }
+        fl = flags[t] != 0;
     }
+    cell_force_fl<FT, C>(v, gb, fl, force, c, F);
 }
 
 // External force of cell c.
@@ -411,11 +421,11 @@ struct alignas(4 * sizeof(C)) Vel4 {
 // cells).  (flags, force) is the last collision's force field, with the
 // last collision's body force in the tiles it does not flag.
 // The same from the cell's moments (j = sum_j(f), invRho = 1 / (sum_rho(f)
-// + 1)), which the collision that carries the staging forms anyway.
-template <class FT, class C>
-__device__ __forceinline__ Vel4<C> staged_velocity_m(const LatView& v, const unsigned char* flags, const void* force,
-                                                     unsigned char m, long c, int x, int y, int z, const C j[3],
-                                                     C invRho)
+// + 1)), which the collision that carries the staging forms anyway, and
+// (staged_velocity_f) from the force at the cell as well.
+template <class C>
+__device__ __forceinline__ Vel4<C> staged_velocity_f(const LatView& v, unsigned char m, int x, int y, int z,
+                                                     const C j[3], C invRho, const C Fc[3])
 {
     C u[3] = {C(0), C(0), C(0)};
     if (m == M_VWALL) {
@@ -423,11 +433,19 @@ __device__ __forceinline__ Vel4<C> staged_velocity_m(const LatView& v, const uns
         const int side = pp[v.wall_axis] == 0 ? 0 : 1;
         for (int d = 0; d < 3; ++d) u[d] = (C)v.wall_u[3 * side + d];
     } else if (m == M_FLUID) {
-        C Fc[3];
-        cell_force_in<FT, C>(v, v.g_last, flags, force, c, x, y, z, Fc);
         for (int d = 0; d < 3; ++d) u[d] = j[d] * invRho + Fc[d] / C(2);
     }
     return Vel4<C>{u[0], u[1], u[2], C(0)};
+}
+
+template <class FT, class C>
+__device__ __forceinline__ Vel4<C> staged_velocity_m(const LatView& v, const unsigned char* flags, const void* force,
+                                                     unsigned char m, long c, int x, int y, int z, const C j[3],
+                                                     C invRho)
+{
+    C Fc[3] = {C(0), C(0), C(0)};
+    if (m == M_FLUID) cell_force_in<FT, C>(v, v.g_last, flags, force, c, x, y, z, Fc);
+    return staged_velocity_f<C>(v, m, x, y, z, j, invRho, Fc);
 }
 
 template <class FT, class C>

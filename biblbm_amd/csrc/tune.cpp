@@ -130,6 +130,19 @@ int bl_tune_collide(bl_lattice L, char* report, int cap)
                 if (nt < 2) cand[0].push_back(l);
                 cand[1].push_back(l);
             }
+    // the odd step's z-segment kernel, for the segment lengths the lattice
+    // takes
+    static const int seg_orders[] = {0, 16, 64};
+    for (int b : blocks)
+        if (zseg_ok(L, b))
+            for (int xcd : seg_orders)
+                for (int nt = 0; nt < 2; ++nt) {
+                    Launch l;
+                    l.block = -b;
+                    l.xcd = xcd;
+                    l.nt = nt;
+                    cand[1].push_back(l);
+                }
     const size_t sz = L->pop_bytes;
     const size_t bytes = (size_t)Q * L->stride * sz;
     void* saved = nullptr;
@@ -429,11 +442,14 @@ int bl_tune_cache_load(const char* path)
                    dev, &c.nx, &c.ny, &c.nz, &c.storage, &c.par[0].block, &c.par[0].xcd, &c.par[0].nt, &c.par[0].ms,
                    &c.par[1].block, &c.par[1].xcd, &c.par[1].nt, &c.par[1].ms) == 13) {
             bool ok = true;
-            for (int p = 0; p < 2; ++p)
-                ok = ok && c.par[p].block >= 64 && c.par[p].block <= 256 && c.par[p].block % 64 == 0 &&
+            for (int p = 0; p < 2; ++p) {
+                const int lanes = c.par[p].block < 0 ? -c.par[p].block : c.par[p].block;
+                ok = ok && lanes >= 64 && lanes <= 256 && lanes % 64 == 0 &&
+                     (c.par[p].block > 0 || c.par[p].xcd != BL_ORDER_ROWS) &&
                      c.par[p].xcd >= 0 && c.par[p].xcd <= (1 << 20) &&
                      (c.par[p].xcd <= 1 || c.par[p].xcd == BL_ORDER_ROWS || (c.par[p].xcd & (c.par[p].xcd - 1)) == 0) &&
                      c.par[p].nt >= 0 && c.par[p].nt <= 2;
+            }
             if (!ok) {
                 ++bad;
                 continue;

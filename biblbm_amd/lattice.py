@@ -62,7 +62,9 @@ class Lattice:
 
     def set_launch_parity(self, parity, block=256, xcd_remap=True, nontemporal=0):
         """nontemporal: 0 / False none, 1 / True every population, 2 the
-        odd step's z-aligned planes only."""
+        odd step's z-aligned planes only.  A negative block -B: the odd
+        step over z-segments of B cells, every access z-aligned (z
+        periodic, nz a multiple of B; elsewhere, and on even steps, B)."""
         call("bl_lattice_set_launch_parity", self.handle, int(parity), int(block), int(xcd_remap), int(nontemporal))
 
     def launch(self, parity):

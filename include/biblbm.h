@@ -139,7 +139,8 @@ int bl_lattice_conserved(bl_lattice lat, double out[4]);
 int bl_lattice_conserved_static(bl_lattice lat, double *out);
 int bl_lattice_steps(bl_lattice lat, long long *steps);
 /* Launch shape of the collide-stream kernel: threads per block (64, 128,
- * 192 or 256) and XCD-aware block remapping on/off, for both AA step
+ * 192 or 256, or the negative of one: see bl_lattice_set_launch_parity)
+ * and XCD-aware block remapping on/off, for both AA step
  * parities, without nontemporal hints.  No counterpart in the reference
  * (its block decomposition is fixed, src/multiBlock/defaultMultiBlockPolicy3D.h:119-129). */
 int bl_lattice_set_launch(bl_lattice lat, int block, int xcd_remap);
@@ -152,7 +153,14 @@ int bl_lattice_set_launch(bl_lattice lat, int block, int xcd_remap);
  * one contiguous range of cells, a power of two k >= 2 deals chunks of k
  * consecutive blocks round-robin over the XCDs, BL_ORDER_ROWS (3) gives
  * each z-row of cells (split into ceil(nz / block) blocks) to one XCD, the
- * eight XCDs taking adjacent rows.  Results do not change. */
+ * eight XCDs taking adjacent rows.  A negative block -B runs the odd step
+ * over z-segments: one workgroup of B lanes per B consecutive cells of a
+ * z-row, every population access z-aligned (the z +-1 shift goes through
+ * LDS; nontemporal 2 is then 1).  It applies where z is periodic, nz is a
+ * multiple of B and the lattice holds fewer than 2^32 population values;
+ * elsewhere, and on even steps, -B acts as B.  It takes xcd_remap 0, 1 or
+ * a power of two (dealing segments as blocks), not BL_ORDER_ROWS.  Results
+ * do not change. */
 int bl_lattice_set_launch_parity(bl_lattice lat, int parity, int block, int xcd_remap, int nontemporal);
 /* out = {block, xcd_remap, nontemporal} of one parity. */
 int bl_lattice_get_launch(bl_lattice lat, int parity, int out[3]);
@@ -519,7 +527,10 @@ int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms
  * src/multiBlock/defaultMultiBlockPolicy3D.h:119-129).  Times the
  * collide-and-stream kernel of this lattice for every launch shape per AA
  * step parity (block 64..256, the block orders and nontemporal levels of
- * bl_lattice_set_launch_parity), keeps the fastest for each parity, restores the
+ * bl_lattice_set_launch_parity: 96 odd candidates, and on odd steps the
+ * z-segment kernel for every segment length the lattice takes, in hardware
+ * order and chunks of 16 and 64, with and without nontemporal hints),
+ * keeps the fastest for each parity, restores the
  * populations and step count the trials advanced, and records the result in
  * the process's tuning table (a lattice of the same size, storage and
  * device created later starts with it).  report (cap bytes, may be NULL)

@@ -162,6 +162,88 @@ def test_row_orders_are_bitwise(device, shape, periodic, block, storage):
         assert np.array_equal(runs[0], r)
 
 
+@pytest.mark.parametrize("shape,periodic,storage,lanes,order,nt", [
+    ((6, 5, 64), (1, 1, 1), "f64", 64, 0, 1), ((7, 4, 128), (0, 1, 1), "f64", 64, 64, 0),
+    ((4, 4, 384), (1, 1, 1), "f64", 128, 64, 1), ((5, 6, 192), (1, 0, 1), "f32", 64, 1, 1),
+    ((4, 5, 64), (0, 0, 1), "f16s", 64, 4, 0), ((3, 4, 512), (1, 1, 1), "f32", 256, 16, 2),
+    ((5, 4, 384), (0, 1, 1), "f16s", 192, 0, 1), ((4, 3, 256), (1, 1, 1), "f64", 256, 0, 0)])
+def test_z_segment_kernel_is_bitwise(device, shape, periodic, storage, lanes, order, nt):
+    """The odd step's z-segment kernel (block -B: one workgroup per B cells
+    of a z-row, every access z-aligned, the z shift through LDS; one or
+    several segments per row, dealt in several orders) against the per-cell
+    kernel: reflection at bounded x and y faces, bounce-back cells, a body
+    force, both parities over five steps."""
+    from biblbm_amd.lattice import Lattice
+    n = int(np.prod(shape))
+    rho = 1 + 0.01 * np.sin(np.arange(n)).reshape(shape)
+    u = 0.02 * np.cos(np.arange(3 * n)).reshape(shape + (3,))
+    mask = np.zeros(shape, np.uint8)
+    mask[1, 1, 1] = 1
+    mask[0, shape[1] - 1, shape[2] - 1] = 1
+    mask[2, 0, lanes - 1] = 1
+    runs = []
+    for block in (lanes, -lanes):
+        L = Lattice(*shape, periodic=periodic, storage=storage, omega=1 / 0.8)
+        L.set_mask(mask)
+        L.init_equilibrium(rho, u)
+        L.set_body_force((1e-5, -2e-6, 3e-6))
+        L.set_launch_parity(1, block, order, nt)
+        assert L.launch(1) == dict(block=block, xcd_remap=order, nontemporal=nt)
+        L.collide_and_stream(5)
+        runs.append(L.get_populations())
+        L.close()
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_z_segment_kernel_falls_back_where_it_does_not_apply(device):
+    """A negative block on a lattice the z-segment kernel cannot take (nz
+    not a multiple of the segment; z bounded) steps as the positive block
+    does; BL_ORDER_ROWS is refused with it."""
+    from biblbm_amd import capi
+    from biblbm_amd.lattice import Lattice
+    for shape, periodic in (((6, 5, 40), (1, 1, 1)), ((6, 5, 64), (1, 1, 0)), ((4, 4, 96), (1, 1, 1))):
+        n = int(np.prod(shape))
+        rho = 1 + 0.01 * np.sin(np.arange(n)).reshape(shape)
+        runs = []
+        for block in (64 if shape[2] != 96 else 192, -64 if shape[2] != 96 else -192):
+            L = Lattice(*shape, periodic=periodic, storage="f64", omega=1 / 0.8)
+            L.init_equilibrium(rho, None)
+            L.set_body_force((1e-5, 0, 0))
+            for p in (0, 1):
+                L.set_launch_parity(p, block, 0, 0)
+            L.collide_and_stream(4)
+            runs.append(L.get_populations())
+            if block < 0:
+                with pytest.raises(capi.BlError):
+                    L.set_launch_parity(1, block, 3, 0)
+            L.close()
+        assert np.array_equal(runs[0], runs[1])
+
+
+@pytest.mark.parametrize("storage,side", [("f64", 64), ("f16s", 128)])
+def test_z_segment_kernel_in_coupled_steps(device, storage, side):
+    """Coupled steps of a shear box (velocity walls, tile-gated force field;
+    one and two segments per row) with the z-segment odd step, the velocity
+    staging carried by the collision and by a pass of its own, eagerly and
+    from captured graphs, against the per-cell odd step."""
+    from biblbm_amd import workloads
+    ws = []
+    for block in (192, -64):
+        for fused in (1, 0):
+            w = workloads.shear_box(storage, side, 0.9, 0.3, G=0.004, seed=5, radius=9.0)
+            w.lattice.set_launch_parity(1, block, 64, 1)
+            w.ibm.set_fused_staging(fused)
+            w.ibm.coupled_steps(3, use_graph=False)
+            w.ibm.coupled_steps(6, use_graph=True)
+            ws.append(w)
+    for w in ws[1:]:
+        assert np.array_equal(ws[0].lattice.get_populations(), w.lattice.get_populations())
+        assert np.array_equal(ws[0].ibm.positions(), w.ibm.positions())
+    for w in ws:
+        w.ibm.close()
+        w.lattice.close()
+
+
 def _small_suspension(storage="f64", seed=3):
     from biblbm_amd import workloads
     return workloads.suspension(storage, nx=64, ny=40, nz=48, n642=2, n2562=1, n162=2, cell_radius=8.0, seed=seed)
