@@ -118,7 +118,9 @@ def roofline(W, storage, has_mask, n=20):
     achieved = nb / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=read_traffic(W.name, storage),
-                kernel="bl::k_collide_stream", launch_ms=round(ms, 5), algorithmic_bytes=nb,
+                kernel="bl::k_collide_stream" + (" (even steps) / bl::k_collide_zseg (odd steps)"
+                                                 if L.launch(1)["block"] < 0 else ""),
+                launch_ms=round(ms, 5), algorithmic_bytes=nb,
                 bytes_per_cell=2 * 19 * BYTES[storage] + (1 if has_mask else 0),
                 active_tiles=st["active_tiles"],
                 by_parity={p: dict(launch_ms=round(par[k], 5), frac=round(nb / (par[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

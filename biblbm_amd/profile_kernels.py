@@ -124,21 +124,32 @@ def _counter_per_dispatch(outdir, counter, kernel_substr):
 def traffic(fetch_dir, write_dir, copy_bytes, kernel_substr):
     """HBM bytes per launch of `kernel_substr` from FETCH_SIZE and WRITE_SIZE
     (KiB per dispatch), each scaled by what the same counter reads for the
-    plain copy kernel of known size."""
+    plain copy kernel of known size.  Several kernels, comma-separated (the
+    two AA parities' kernels, launched equally often): the mean of their
+    medians."""
     cb = float(copy_bytes)
     out = {}
+    subs = kernel_substr.split(",")
     for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         copy = _counter_per_dispatch(d, name, "k_copy16")
-        kern = _counter_per_dispatch(d, name, kernel_substr)
-        if not copy or not kern:
-            raise SystemExit("no %s samples for k_copy16 / %s in %s" % (name, kernel_substr, d))
+        if not copy:
+            raise SystemExit("no %s samples for k_copy16 in %s" % (name, d))
+        meds = []
+        for sub in subs:
+            kern = _counter_per_dispatch(d, name, sub)
+            if not kern:
+                raise SystemExit("no %s samples for %s in %s" % (name, sub, d))
+            meds.append(sorted(kern)[len(kern) // 2])
+            if len(subs) > 1:
+                print("%s: %s %.1f KiB/dispatch (median)" % (name, sub, meds[-1]))
         copy_kib = sorted(copy)[len(copy) // 2]
         factor = cb / (copy_kib * 1024.0)
-        kern_kib = sorted(kern)[len(kern) // 2]
+        kern_kib = sum(meds) / len(meds)
         out[name] = (kern_kib, copy_kib, factor, kern_kib * 1024.0 * factor)
     for name, (k, c, f, b) in out.items():
-        print("%s: kernel %.1f KiB/dispatch (median of dispatches), copy kernel %.1f KiB for %d bytes, "
-              "factor %.4f -> %.4e bytes" % (name, k, c, int(cb), f, b))
+        print("%s: kernel %.1f KiB/dispatch (median of dispatches%s), copy kernel %.1f KiB for %d bytes, "
+              "factor %.4f -> %.4e bytes" % (name, k, ", mean over the kernels" if len(subs) > 1 else "", c, int(cb),
+                                             f, b))
     total = out["FETCH_SIZE"][3] + out["WRITE_SIZE"][3]
     print("traffic_bytes_per_launch: %.6e" % total)
     return total
