@@ -213,16 +213,18 @@ def precision_leg(steps, checkpoints=(100, 200)):
     as the FP64 run: errors against it after each checkpoint step count and
     after `steps` (velocity field rel-L2, vertex positions rel-L2, largest
     per-capsule change of the Taylor deformation index), speed from the
-    steps after the first checkpoint."""
+    steps after the first checkpoint; each storage's collide launch shapes
+    are tuned first (the tuner restores the state)."""
     import numpy as np
     from biblbm_amd import capi, workloads
     from biblbm_amd.ibm import taylor_deformation
     marks = sorted(set([c for c in checkpoints if c < steps] + [steps]))
     out = {}
     ref = {}
+    from biblbm_amd import tuning
     for storage in ("f64", "f32", "f16s"):
         W = workloads.taylor_green(storage)
-        row = {}
+        row = {"collide_launch": tuning.tune_collide(W.lattice)["best"]}
         done, t_timed, n_timed = 0, 0.0, 0
         for m in marks:
             W.ibm.prepare(m - done)
@@ -263,7 +265,8 @@ def precision_leg(steps, checkpoints=(100, 200)):
 def suspension_leg(steps, warmup, horizons=(500, 1000, 2000), tune_steps=4):
     """The 400-cell channel in FP64, FP32 and FP16-shifted storage from the
     same start (the immersed boundary in FP64 with FP64 populations, FP32
-    otherwise): the immersed-boundary launch parameters are tuned first
+    otherwise): the collide launch shapes (bl_tune_collide, which restores
+    the state) and the immersed-boundary launch parameters are tuned first
     (bl_tune_ibm, the same number of real steps in every storage), then the
     speed over `steps` after `warmup`, then the runs continue to each
     horizon (steps from the start) and the velocity field and vertex
@@ -274,6 +277,7 @@ def suspension_leg(steps, warmup, horizons=(500, 1000, 2000), tune_steps=4):
     ref = {}
     for storage in ("f64", "f32", "f16s"):
         W = workloads.suspension(storage)
+        collide_tuned = tuning.tune_collide(W.lattice)["best"]
         tuned = tuning.tune_ibm(W.ibm, tune_steps)["best"]
         dt = time_coupled(W, steps, warmup)
         done = W.lattice.steps
@@ -296,7 +300,7 @@ def suspension_leg(steps, warmup, horizons=(500, 1000, 2000), tune_steps=4):
         W.ibm.step()
         stages = {k: round(v, 5) for k, v in W.ibm.stage_times().items()}
         row = dict(mlups=round(W.cells * steps / dt / 1e6, 1), ms_per_step=round(dt / steps * 1e3, 4),
-                   ibm_stage_ms=stages, ibm_launch=tuned)
+                   ibm_stage_ms=stages, ibm_launch=tuned, collide_launch=collide_tuned)
         if storage != "f64":
             row["velocity_rel_l2_vs_f64"] = errs["at_%d_steps" % horizons[0]]["velocity_rel_l2_vs_f64"]
             row["errors"] = errs

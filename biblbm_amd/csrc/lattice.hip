@@ -732,7 +732,9 @@ template <class S, int PHASE>
 __global__ void k_vwall_batch(const VwallPlan* __restrict__ tab, const int* __restrict__ base, int nbox, bool swapped)
 {
     const int box = batch_box(base, nbox, (int)blockIdx.x);
-    const VwallPlan w = tab[box];
+    // the box's plan is read in place (a copy of it, which the wall
+    // velocity's indexed read would put in scratch memory, is not made)
+    const VwallPlan& w = tab[box];
     const long i = (long)((int)blockIdx.x - base[box]) * blockDim.x + threadIdx.x;
     if (PHASE == 0) vwall_u_cell<S>(w.v, w.nwall, swapped, w.buf, i);
     else vwall_compute_cell<S>(w.v, w.nwall, swapped, w.buf, i);
