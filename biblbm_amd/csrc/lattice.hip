@@ -181,6 +181,9 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
         inner = __all(x > 0 && x < v.nx - 1 && y > 0 && y < v.ny - 1) != 0;
     }
 
+    // the tile flags first: the force loads wait for them, and so they
+    // arrive while the populations load
+    const TileFlags tf = cs_flags<FMODE, EMIT>(v, x, y, z);
     C f[Q];
     if (ODD) {
 #pragma unroll
@@ -193,8 +196,7 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
         for (int i = 0; i < Q; ++i) f[i] = P::template ld<unsigned, NT != 0>(A + i * st, c, i);
     }
 
-    cs_update<S, FMODE != 0, EMIT>(v, c, x, y, z, f,
-                                   cs_inputs<S, MASK, FMODE, PHASE, EMIT>(v, c, cs_flags<FMODE, EMIT>(v, x, y, z)));
+    cs_update<S, FMODE != 0, EMIT>(v, c, x, y, z, f, cs_inputs<S, MASK, FMODE, PHASE, EMIT>(v, c, tf));
 
     if (ODD) {
 #pragma unroll
