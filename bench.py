@@ -438,8 +438,16 @@ def pipe_leg(steps, warmup):
         dt = time_coupled(W, steps, warmup)
         rl = roofline(W, storage, True)
         rl["note"] = "populations of this lattice fit the 256 MB last-level cache: cache-assisted figure"
+        # the same run continued with the inert bounce-back cells beyond the
+        # wall left alone (bl_lattice_set_skip_inert; fluid results bitwise)
+        n_inert = W.lattice.set_skip_inert(1)
+        dts = time_coupled(W, steps, warmup)
+        W.lattice.set_skip_inert(0)
         out[storage] = dict(mlups=round(W.cells * steps / dt / 1e6, 2), ms_per_step=round(dt / steps * 1e3, 5),
-                            fluid_mlups=round(W.fluid_cells * steps / dt / 1e6, 2), roofline=rl)
+                            fluid_mlups=round(W.fluid_cells * steps / dt / 1e6, 2), roofline=rl,
+                            skip_inert=dict(inert_cells=n_inert, ms_per_step=round(dts / steps * 1e3, 5),
+                                            mlups=round(W.cells * steps / dts / 1e6, 2),
+                                            fluid_mlups=round(W.fluid_cells * steps / dts / 1e6, 2)))
         W.ibm.close()
         W.lattice.close()
     W2 = workloads.rbc_in_pipe("f64", second_order=True)

@@ -32,6 +32,8 @@ SIGNATURES = {
     "bl_lattice_destroy": [_P],
     "bl_lattice_set_dynamics": [_P, _D, _D],
     "bl_lattice_set_mask": [_P, _P],
+    "bl_lattice_set_skip_inert": [_P, _I],
+    "bl_lattice_inert_cells": [_P, _PLL],
     "bl_lattice_set_velocity_walls": [_P, _I, _P, _P],
     "bl_lattice_set_body_force": [_P, _P],
     "bl_lattice_set_force_field": [_P, _P],

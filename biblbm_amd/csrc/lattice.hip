@@ -47,6 +47,7 @@ LatView view_of(const bl_lattice_s* L)
     for (int d = 0; d < 3; ++d) v.per[d] = L->periodic[d];
     v.pops = L->pops;
     v.mask = L->mask;
+    v.inert = L->skip_inert ? L->inert : nullptr;
     v.phase = L->phase;
     v.flags = L->tile_flags;
     v.force = L->force;
@@ -172,6 +173,10 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
     using C = typename P::C;
     S* __restrict__ A = (S*)v.pops;
     const long st = v.stride;
+    // an inert bounce-back cell: what it holds never reaches a fluid cell
+    // (a bounce-back cell hands each neighbour back what that neighbour
+    // sent, and every neighbour of an inert cell is bounce-back)
+    if (MASK && v.inert && ((v.inert[c >> 6] >> (c & 63u)) & 1ull)) return;
 
     NbrOff nb;
     bool inner = false;
@@ -435,7 +440,9 @@ CollidePlan collide_plan(const bl_lattice_s* L)
     v.xcd_remap = L->xcd_remap[par];
     p.nt = L->nontemporal[par];
     if (p.blk < 0) {
-        if (p.odd && zseg_ok(L, -p.blk)) {
+        // the z-segment kernel steps every cell (its workgroups exchange
+        // values through LDS), so it does not run with inert-cell skipping
+        if (p.odd && zseg_ok(L, -p.blk) && !v.inert) {
             v.row_segs = L->nz / -p.blk;
             p.nblk = (unsigned)((long)L->nx * L->ny * v.row_segs);
             return p;
@@ -1022,6 +1029,55 @@ int ensure_force_field(bl_lattice_s* L)
     return BL_OK;
 }
 
+// Inert cells: bounce-back cells whose 18 neighbours are all bounce-back or
+// past a non-periodic face; one thread per word of 64 cells.
+__global__ void k_mark_inert(LatView v, unsigned long long* bits)
+{
+    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (v.ncell + 63u) / 64u) return;
+    unsigned long long b = 0;
+    for (unsigned l = 0; l < 64u; ++l) {
+        const unsigned c = w * 64u + l;
+        if (c >= v.ncell) break;
+        if (v.mask[c] != M_BB) continue;
+        int x, y, z;
+        decode(c, v, x, y, z);
+        Nbr n;
+        n.init(v, x, y, z);
+        bool inert = true;
+        for (int i = 1; i < Q; ++i) {
+            const int d = n.at(v, cx(i), cy(i), cz(i));
+            if (d >= 0 && v.mask[d] != M_BB) inert = false;
+        }
+        if (inert) b |= 1ull << l;
+    }
+    bits[w] = b;
+}
+
+// (Re)builds the inert-cell bits from the mask when skipping is on (none
+// without a mask), frees them otherwise.
+int mark_inert(bl_lattice_s* L)
+{
+    if (!L->skip_inert || !L->mask) {
+        hipFree(L->inert);
+        L->inert = nullptr;
+        L->n_inert = 0;
+        return BL_OK;
+    }
+    const long nw = (L->ncell + 63) / 64;
+    if (!L->inert) BL_HIP(hipMalloc(&L->inert, sizeof(unsigned long long) * nw));
+    LatView v = view_of(L);
+    hipLaunchKernelGGL(k_mark_inert, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, L->stream, v, L->inert);
+    BL_CHECK_LAUNCH();
+    std::vector<unsigned long long> h(nw);
+    BL_HIP(hipMemcpyAsync(h.data(), L->inert, sizeof(unsigned long long) * nw, hipMemcpyDeviceToHost, L->stream));
+    BL_HIP(hipStreamSynchronize(L->stream));
+    long n = 0;
+    for (unsigned long long b : h) n += __builtin_popcountll(b);
+    L->n_inert = n;
+    return BL_OK;
+}
+
 }  // namespace bl
 
 // =================================================================== C-ABI
@@ -1130,6 +1186,7 @@ int bl_lattice_destroy(bl_lattice L)
     bl::guo_walls_detach(L);
     hipFree(L->pops);
     hipFree(L->mask);
+    hipFree(L->inert);
     hipFree(L->own_phase);
     hipFree(L->force);
     hipFree(L->tile_flags);
@@ -1157,7 +1214,32 @@ int bl_lattice_set_dynamics(bl_lattice L, double omega_out, double omega_in)
     return BL_OK;
 }
 
+static int set_mask_cells(bl_lattice L, const unsigned char* mask);
+
 int bl_lattice_set_mask(bl_lattice L, const unsigned char* mask)
+{
+    int rc = set_mask_cells(L, mask);
+    return rc ? rc : bl::mark_inert(L);
+}
+
+int bl_lattice_set_skip_inert(bl_lattice L, int on)
+{
+    if (!L) return fail(BL_ERR_ARG, "null lattice");
+    if (on != 0 && on != 1) return fail(BL_ERR_ARG, "bl_lattice_set_skip_inert: on must be 0 or 1");
+    BL_HIP(hipStreamSynchronize(L->stream));
+    L->skip_inert = on != 0;
+    ++L->gen;
+    return bl::mark_inert(L);
+}
+
+int bl_lattice_inert_cells(bl_lattice L, long long* n)
+{
+    if (!L || !n) return fail(BL_ERR_ARG, "bl_lattice_inert_cells: bad argument");
+    *n = L->skip_inert ? (long long)L->n_inert : 0;
+    return BL_OK;
+}
+
+static int set_mask_cells(bl_lattice L, const unsigned char* mask)
 {
     if (!L) return fail(BL_ERR_ARG, "null lattice");
     BL_HIP(hipStreamSynchronize(L->stream));

@@ -17,6 +17,13 @@ struct bl_lattice_s {
 
     unsigned char* mask = nullptr;   // ncell, or null (all fluid)
     bool has_vwall = false;
+    // Inert cells (bl_lattice_set_skip_inert): bounce-back cells whose 18
+    // neighbours are all bounce-back (or past a non-periodic face).  With
+    // skip_inert set, one bit per cell (bit c & 63 of word c >> 6), built
+    // from the mask, and the collide-and-stream leaves those cells alone.
+    bool skip_inert = false;
+    unsigned long long* inert = nullptr;
+    long n_inert = 0;
     int wall_axis = 1;
     double wall_u[6] = {0, 0, 0, 0, 0, 0};
 
@@ -96,6 +103,7 @@ struct LatView {
     int per[3];
     void* pops;
     const unsigned char* mask;
+    const unsigned long long* inert;  // inert-cell bits, or null (every cell steps)
     const unsigned char* phase;
     const unsigned char* flags;
     const void* force;

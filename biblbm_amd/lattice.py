@@ -41,6 +41,15 @@ class Lattice:
             raise ValueError("mask must have nx*ny*nz entries")
         call("bl_lattice_set_mask", self.handle, ptr(m))
 
+    def set_skip_inert(self, on):
+        """Leave inert bounce-back cells (every neighbour bounce-back) alone
+        in the collide-and-stream (bl_lattice_set_skip_inert); returns the
+        number of inert cells."""
+        call("bl_lattice_set_skip_inert", self.handle, int(bool(on)))
+        n = ctypes.c_longlong()
+        call("bl_lattice_inert_cells", self.handle, ctypes.byref(n))
+        return n.value
+
     def set_velocity_walls(self, axis, u_lo, u_hi):
         call("bl_lattice_set_velocity_walls", self.handle, int(axis), ptr(f64(u_lo)), ptr(f64(u_hi)))
 

@@ -94,6 +94,19 @@ int bl_lattice_set_dynamics(bl_lattice lat, double omega_out, double omega_in);
  * set (src/dataProcessors/dataInitializerWrapper3D.h, src/core/dynamics.h:559,670,
  * src/boundaryCondition/boundaryCondition3D.hh:1053-1066).  NULL = all fluid. */
 int bl_lattice_set_mask(bl_lattice lat, const unsigned char *mask);
+/* No counterpart in the reference (its BounceBack cells all stream): on = 1
+ * leaves inert cells alone in the collide-and-stream, bounce-back cells
+ * whose 18 neighbours are all bounce-back (or past a non-periodic face),
+ * for example the solid beyond a pipe wall.  What such a cell holds never
+ * reaches a fluid cell (a bounce-back cell hands each neighbour back what
+ * that neighbour sent), so every fluid, velocity-wall and no-dynamics cell,
+ * every bounce-back cell next to one, the moments and the conservation sums
+ * step bitwise as with on = 0; the inert cells keep stale populations.  The
+ * set is rebuilt from each new mask; the odd step's z-segment kernel then
+ * runs as the per-cell kernel.  Default 0. */
+int bl_lattice_set_skip_inert(bl_lattice lat, int on);
+/* n = inert cells skipped (0 with skipping off). */
+int bl_lattice_inert_cells(bl_lattice lat, long long *n);
 /* setBoundaryVelocity on the two planes normal to `axis` that carry mask 3
  * (index 0: u_lo, index n-1: u_hi). */
 int bl_lattice_set_velocity_walls(bl_lattice lat, int axis, const double u_lo[3],
