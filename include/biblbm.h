@@ -542,8 +542,9 @@ int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms
  * step parity (block 64..256, the block orders and nontemporal levels of
  * bl_lattice_set_launch_parity: 96 odd candidates, and on odd steps the
  * z-segment kernel for every segment length the lattice takes, in hardware
- * order and chunks of 16 and 64, with and without nontemporal hints),
- * keeps the fastest for each parity, restores the
+ * order and chunks of 4, 16 and 64, with and without nontemporal hints),
+ * times the three fastest of each parity again (8 launch pairs each, the
+ * other parity at its fastest), keeps the fastest of those, restores the
  * populations and step count the trials advanced, and records the result in
  * the process's tuning table (a lattice of the same size, storage and
  * device created later starts with it).  report (cap bytes, may be NULL)

@@ -28,6 +28,10 @@ def test_collide_tuner_keeps_state_and_results(device):
         assert rep["best"][par]["block"] in (64, 128, 192, 256)
         assert rep["best"][par]["xcd_remap"] in (0, 1, 3, 4, 16, 64, 256, 1024)
     assert rep["best"]["even"]["nontemporal"] in (0, 1) and rep["best"]["odd"]["nontemporal"] in (0, 1, 2)
+    # the pick is the fastest of the three sweep leaders timed again
+    for par in ("even", "odd"):
+        ref = rep["refined"][par]
+        assert len(ref) == 3 and rep["best"][par] == min(ref, key=lambda r: r["ms"])
     assert a.lattice.steps == 3
     assert np.array_equal(a.lattice.get_populations(), before)
     got = [a.lattice.launch(p) for p in (0, 1)]
