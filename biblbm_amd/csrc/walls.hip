@@ -21,10 +21,7 @@ struct bl_guo_wall_s {
     std::vector<long long> cell;
     std::vector<int> dir;
     std::vector<double> delta, wall_pos, vel;
-    unsigned* d_cell = nullptr;
-    int* d_dir = nullptr;
-    double* d_delta = nullptr;
-    double* d_vel = nullptr;
+    double* d_vel = nullptr;  // 3 per node, in node order
     // The reference runs the nodes one after the other in place, so a node
     // whose link reads another wall node (lattice nodes on the boundary
     // surface) sees it updated if it comes earlier in node order and not
@@ -35,7 +32,11 @@ struct bl_guo_wall_s {
     long n_snap = 0;
     std::vector<long> level_off;    // entries of level l: [level_off[l], level_off[l+1])
     int* d_order = nullptr;         // node indices by level, ascending within a level
-    int* d_src = nullptr;           // per entry and neighbour: -1 the lattice, else a snapshot slot
+    // per entry (level order): the node's cell, its link direction and, per
+    // neighbour along the link, -1 (the lattice) or a snapshot slot; and the
+    // node's link fraction
+    int4* d_ent = nullptr;
+    double* d_tdelta = nullptr;
     unsigned* d_snap_cell = nullptr;
     double* d_snap = nullptr;       // 19 per snapshot slot
     bool inserted = false;
@@ -63,14 +64,6 @@ __device__ __forceinline__ double feq_d(int i, double rhoBar, double invRho, con
 
 __device__ __forceinline__ int wrap_axis(int q, int n, int per) { return per ? (q < 0 ? q + n : (q >= n ? q - n : q)) : q; }
 
-struct GuoNodes {
-    long n;
-    const unsigned* cell;
-    const int* dir;
-    const double* delta;
-    const double* vel;
-};
-
 __device__ __forceinline__ long nbr_cell(const LatView& v, int x, int y, int z, int i, int m, int& qx, int& qy,
                                          int& qz)
 {
@@ -91,21 +84,19 @@ __device__ __forceinline__ void load_cell(const LatView& v, long q, int qx, int 
     for (int p = 0; p < Q; ++p) f[p] = (double)g[p];
 }
 
-// The extrapolation for node k from its two neighbours' populations f1, f2.
+// The extrapolation for the node at cell c (mask code m, link fraction dl,
+// wall velocity u) from its two neighbours' populations f1, f2.
 template <class S>
-__device__ void guo_node(const LatView& v, const GuoNodes& g, long k, int x, int y, int z, const double* f1,
-                         const double* f2, bool swapped)
+__device__ void guo_node(const LatView& v, unsigned c, unsigned char m, double dl, const double* u, int x, int y,
+                         int z, const double* f1, const double* f2, bool swapped)
 {
     using C = typename Pop<S>::C;
-    const unsigned c = g.cell[k];
     double j1[3], j2[3];
     const double rb1 = sum_rho(f1), rb2 = sum_rho(f2);
     sum_j(f1, j1);
     sum_j(f2, j2);
     const double j1sqr = j1[0] * j1[0] + j1[1] * j1[1] + j1[2] * j1[2];
     const double j2sqr = j2[0] * j2[0] + j2[1] * j2[1] + j2[2] * j2[2];
-    const double dl = g.delta[k];
-    const double* u = g.vel + 3 * k;
     const double wj[3] = {(rb1 + 1.0) * u[0], (rb1 + 1.0) * u[1], (rb1 + 1.0) * u[2]};
     double jw[3];
     if (dl < 0.75) {
@@ -115,7 +106,6 @@ __device__ void guo_node(const LatView& v, const GuoNodes& g, long k, int x, int
         for (int a = 0; a < 3; ++a) jw[a] = 1. / dl * (wj[a] + (dl - 1.) * j1[a]);
     }
     const double jsw = jw[0] * jw[0] + jw[1] * jw[1] + jw[2] * jw[2];
-    const unsigned char m = v.mask ? v.mask[c] : (unsigned char)M_FLUID;
     const bool has_eq = m == M_FLUID || m == M_VWALL;
     const double inv1 = 1.0 / (rb1 + 1.0), inv2 = 1.0 / (rb2 + 1.0);
     C fw[Q];
@@ -129,29 +119,38 @@ __device__ void guo_node(const LatView& v, const GuoNodes& g, long k, int x, int
     store_post<S>(v, nb, c, swapped, fw);
 }
 
-// one level of nodes, one thread each
+// One level of nodes, one thread each.  Every input loads with no branch
+// in front of it (the entry and node data, then both neighbours' 38
+// populations from the lattice at once); a neighbour that is a wall node of
+// a later entry then takes its populations from the snapshot instead (the
+// lattice value read for it is dropped).
 template <class S>
-__global__ void __launch_bounds__(64) k_guo_wall(LatView v, GuoNodes g, long n, const int* __restrict__ order,
-                                                 const int* __restrict__ src, const double* __restrict__ snap,
-                                                 bool swapped)
+__global__ void __launch_bounds__(64) k_guo_wall(LatView v, const int4* __restrict__ ent,
+                                                 const int* __restrict__ order, const double* __restrict__ tdelta,
+                                                 const double* __restrict__ vel, long n,
+                                                 const double* __restrict__ snap, bool swapped)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    const int4 e = ent[t];
     const long k = order[t];
+    const double dl = tdelta[t];
+    const unsigned c = (unsigned)e.x;
     int x, y, z;
-    decode(g.cell[k], v.ny, v.nz, x, y, z);
+    decode(c, v.ny, v.nz, x, y, z);
+    const unsigned char m = v.mask ? v.mask[c] : (unsigned char)M_FLUID;
     double f[2][Q];
-    for (int m = 0; m < 2; ++m) {
-        const int sl = src[2 * t + m];
-        if (sl >= 0) {
-            for (int p = 0; p < Q; ++p) f[m][p] = snap[(long)Q * sl + p];
-        } else {
-            int qx, qy, qz;
-            const long q = nbr_cell(v, x, y, z, g.dir[k], m + 1, qx, qy, qz);
-            load_cell<S>(v, q, qx, qy, qz, swapped, f[m]);
-        }
+#pragma unroll
+    for (int mm = 0; mm < 2; ++mm) {
+        int qx, qy, qz;
+        const long q = nbr_cell(v, x, y, z, e.y, mm + 1, qx, qy, qz);
+        load_cell<S>(v, q, qx, qy, qz, swapped, f[mm]);
     }
-    guo_node<S>(v, g, k, x, y, z, f[0], f[1], swapped);
+    if (e.z >= 0)
+        for (int p = 0; p < Q; ++p) f[0][p] = snap[(long)Q * e.z + p];
+    if (e.w >= 0)
+        for (int p = 0; p < Q; ++p) f[1][p] = snap[(long)Q * e.w + p];
+    guo_node<S>(v, c, m, dl, vel + 3 * k, x, y, z, f[0], f[1], swapped);
 }
 
 // populations of the wall nodes that earlier nodes read, taken before the
@@ -170,15 +169,14 @@ __global__ void __launch_bounds__(64) k_guo_snapshot(LatView v, long n, const un
 template <class S>
 void launch_guo(bl_guo_wall w, const LatView& v, bool swapped, hipStream_t s)
 {
-    const GuoNodes g{w->n, w->d_cell, w->d_dir, w->d_delta, w->d_vel};
     if (w->n_snap)
         hipLaunchKernelGGL(k_guo_snapshot<S>, dim3((unsigned)((w->n_snap + 63) / 64)), dim3(64), 0, s, v, w->n_snap,
                            (const unsigned*)w->d_snap_cell, w->d_snap, swapped);
     for (size_t l = 0; l + 1 < w->level_off.size(); ++l) {
         const long a = w->level_off[l], n = w->level_off[l + 1] - a;
-        hipLaunchKernelGGL(k_guo_wall<S>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, v, g, n,
-                           (const int*)w->d_order + a, (const int*)w->d_src + 2 * a, (const double*)w->d_snap,
-                           swapped);
+        hipLaunchKernelGGL(k_guo_wall<S>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, v,
+                           (const int4*)w->d_ent + a, (const int*)w->d_order + a, (const double*)w->d_tdelta + a,
+                           (const double*)w->d_vel, n, (const double*)w->d_snap, swapped);
     }
 }
 
@@ -316,35 +314,34 @@ int bl_guo_wall_create(bl_lattice lat, const bl_boundary* b, bl_guo_wall* out)
         }
     }
     const int nlev = w->n ? *std::max_element(level.begin(), level.end()) + 1 : 0;
-    std::vector<int> order, osrc;
+    std::vector<int> order;
+    std::vector<int4> ent;
+    std::vector<double> tdelta;
     w->level_off.assign(1, 0);
     for (int l = 0; l < nlev; ++l) {
         for (long k = 0; k < w->n; ++k)
             if (level[k] == l) {
                 order.push_back((int)k);
-                osrc.push_back(src[2 * k]);
-                osrc.push_back(src[2 * k + 1]);
+                ent.push_back(make_int4((int)w->cell[k], w->dir[k], src[2 * k], src[2 * k + 1]));
+                tdelta.push_back(w->delta[k]);
             }
         w->level_off.push_back((long)order.size());
     }
     w->n_snap = (long)snap_node.size();
     if (w->n) {
-        std::vector<unsigned> cu(w->cell.begin(), w->cell.end()), snap_cell;
+        std::vector<unsigned> snap_cell;
         for (int j : snap_node) snap_cell.push_back((unsigned)w->cell[j]);
-        hipError_t e = hipMalloc(&w->d_cell, sizeof(unsigned) * w->n);
-        if (e == hipSuccess) e = hipMalloc(&w->d_dir, sizeof(int) * w->n);
-        if (e == hipSuccess) e = hipMalloc(&w->d_delta, sizeof(double) * w->n);
-        if (e == hipSuccess) e = hipMalloc(&w->d_vel, sizeof(double) * 3 * w->n);
+        hipError_t e = hipMalloc(&w->d_vel, sizeof(double) * 3 * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_order, sizeof(int) * w->n);
-        if (e == hipSuccess) e = hipMalloc(&w->d_src, sizeof(int) * 2 * w->n);
+        if (e == hipSuccess) e = hipMalloc(&w->d_ent, sizeof(int4) * w->n);
+        if (e == hipSuccess) e = hipMalloc(&w->d_tdelta, sizeof(double) * w->n);
         if (e == hipSuccess) e = hipMalloc(&w->d_snap_cell, sizeof(unsigned) * std::max<size_t>(snap_cell.size(), 1));
         if (e == hipSuccess) e = hipMalloc(&w->d_snap, sizeof(double) * Q * std::max<size_t>(snap_cell.size(), 1));
-        if (e == hipSuccess) e = hipMemcpy(w->d_cell, cu.data(), sizeof(unsigned) * w->n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(w->d_dir, w->dir.data(), sizeof(int) * w->n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(w->d_delta, w->delta.data(), sizeof(double) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(w->d_vel, 0, sizeof(double) * 3 * w->n);
         if (e == hipSuccess) e = hipMemcpy(w->d_order, order.data(), sizeof(int) * w->n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(w->d_src, osrc.data(), sizeof(int) * 2 * w->n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w->d_ent, ent.data(), sizeof(int4) * w->n, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(w->d_tdelta, tdelta.data(), sizeof(double) * w->n, hipMemcpyHostToDevice);
         if (e == hipSuccess && !snap_cell.empty())
             e = hipMemcpy(w->d_snap_cell, snap_cell.data(), sizeof(unsigned) * snap_cell.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -366,12 +363,10 @@ int bl_guo_wall_destroy(bl_guo_wall w)
         ++w->lat->gen;
         if (w->lat->ibm) ibm_invalidate_graphs(w->lat->ibm);
     }
-    hipFree(w->d_cell);
-    hipFree(w->d_dir);
-    hipFree(w->d_delta);
     hipFree(w->d_vel);
     hipFree(w->d_order);
-    hipFree(w->d_src);
+    hipFree(w->d_ent);
+    hipFree(w->d_tdelta);
     hipFree(w->d_snap_cell);
     hipFree(w->d_snap);
     delete w;
