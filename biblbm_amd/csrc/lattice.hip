@@ -175,8 +175,12 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
     const long st = v.stride;
     // an inert bounce-back cell: what it holds never reaches a fluid cell
     // (a bounce-back cell hands each neighbour back what that neighbour
-    // sent, and every neighbour of an inert cell is bounce-back)
-    if (MASK && v.inert && ((v.inert[c >> 6] >> (c & 63u)) & 1ull)) return;
+    // sent, and every neighbour of an inert cell is bounce-back); only its
+    // staged velocity, zero on a bounce-back cell, is still written
+    if (MASK && v.inert && ((v.inert[c >> 6] >> (c & 63u)) & 1ull)) {
+        if (EMIT && cs_flags<FMODE, EMIT>(v, x, y, z).cur) ((Vel4<C>*)v.emit_u)[c] = Vel4<C>{C(0), C(0), C(0), C(0)};
+        return;
+    }
 
     NbrOff nb;
     bool inner = false;

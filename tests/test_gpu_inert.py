@@ -116,3 +116,46 @@ def test_skip_inert_in_coupled_pipe_steps(device, storage):
     for w in ws:
         w.ibm.close()
         w.lattice.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_skip_inert_with_a_capsule_at_the_wall(device, storage):
+    """A capsule next to the pipe wall whose 4-point stencils reach inert
+    cells two layers into the bounce-back wall, with the velocity staging
+    carried by the collision (an inert cell still stages its zero
+    velocity): positions and flow bitwise with and without skipping."""
+    from biblbm_amd import meshes, workloads
+    from biblbm_amd.ibm import ImmersedBoundary, CAPSULE_SKALAK
+    from biblbm_amd.lattice import Lattice
+    R, length = 10, 24
+    mask, _, c = workloads.pipe_mask(R, length)
+    inert = inert_cells(mask, (1, 0, 0))
+    v, t = meshes.icosphere(2, 3.0)
+    pos0 = v + [length / 2, c + R - 3.4, c]
+    hits = 0
+    for p in pos0:
+        b = np.floor(p).astype(int) - 1
+        for i, j, k in np.ndindex(4, 4, 4):
+            q = ((b[0] + i) % length, b[1] + j, b[2] + k)
+            hits += bool(inert[q])
+    assert hits > 0  # the case under test: stencil nodes on inert cells
+    out = []
+    for skip in (0, 1):
+        L = Lattice(length, mask.shape[1], mask.shape[2], periodic=(1, 0, 0), storage=storage, omega=1 / 0.9)
+        L.set_mask(mask)
+        L.set_body_force([1e-5, 0, 0])
+        L.init_equilibrium()
+        ib = ImmersedBoundary(L, "peskin")
+        sid = ib.add_shape(v, t)
+        ib.add_particle(sid, CAPSULE_SKALAK, [0.01, 10.0], pos0)
+        ib.init()
+        ib.set_fused_staging(1)
+        n = L.set_skip_inert(skip)
+        ib.coupled_steps(3, use_graph=False)
+        ib.coupled_steps(4, use_graph=True)
+        out.append((ib.positions(), L.moments()[1], n))
+        ib.close()
+        L.close()
+    assert out[0][2] == 0 and out[1][2] == int(inert.sum())
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
