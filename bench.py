@@ -393,7 +393,9 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     each): every stage of a step one launch over all boxes (batched), two
     steps per hipGraph launch, the batched collide's workgroup order from
     the ensemble tuner; against the same boxes stepped one after another
-    (each with its own captured graph), and two boxes against the oracle."""
+    (each with its own captured graph), against the collide-and-stream rate
+    of one lattice of the same total cells, and two boxes against the
+    oracle."""
     import numpy as np
     from biblbm_amd import capi, workloads
     from biblbm_amd.ensemble import Ensemble
@@ -426,6 +428,21 @@ def ensemble_leg(steps, warmup, n_boxes=64):
     for w in ws:
         w.ibm.close()
         w.lattice.close()
+    # the single-lattice collide rate at equal total cells: one shear box of
+    # that many cells (the same walls and storage), its collide launch
+    # shapes tuned, the collide-and-stream alone over both parities
+    from biblbm_amd import tuning
+    side = int(round(cells ** (1.0 / 3.0)))
+    W1 = workloads.shear_box("f64", side, 0.9, 0.3, G=0.004, seed=0, radius=0.2 * side)
+    W1.ibm.coupled_steps(2, use_graph=False)
+    tuning.tune_collide(W1.lattice)
+    n1 = 20
+    ms1 = W1.lattice.time_collide(n1) / n1
+    single = W1.cells / (ms1 * 1e-3) / 1e6
+    out["single_lattice_collide"] = dict(side=side, cells=W1.cells, ms=round(ms1, 4), mlups=round(single, 1))
+    out["vs_single_lattice_collide"] = round(out["mlups"] / single, 3)
+    W1.ibm.close()
+    W1.lattice.close()
     out["vs_oracle"] = ensemble_oracle_check()
     return out
 
