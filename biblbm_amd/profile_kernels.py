@@ -19,7 +19,8 @@ same table profiles the shapes the bench timed.
 `collide W S N` builds workload W (rbc_in_pipe, suspension,
 suspension_interactions, taylor_green or capsule_in_shear) with storage S, runs two coupled steps so that the force
 tiles are those of a running simulation, then N collide-and-stream launches.
-`coupled W S N` runs N eager coupled steps instead; `collisions W S N` runs
+`coupled W S N` runs N eager coupled steps instead; `ensemble B N` steps
+the bench's ensemble of B shear boxes N times, batched; `collisions W S N` runs
 N collision-handler passes (Morse r_min 1, beta 1, strength 1e-3) on the
 vertices of workload W after two coupled steps.  `copy B N` runs N
 launches of a plain 16-byte-per-thread device copy of B bytes (the
@@ -79,6 +80,23 @@ def run(argv):
             W.ibm.close()
             W.lattice.close()
             i += 4
+        elif cmd == "ensemble":
+            # `ensemble N S`: the bench's ensemble of N shear boxes, batched,
+            # four settle steps, then S timed steps (two per graph launch)
+            from . import workloads
+            from .ensemble import Ensemble
+            n_boxes, n = int(argv[i + 1]), int(argv[i + 2])
+            ws = workloads.shear_ensemble("f64", n_boxes=n_boxes)
+            ens = Ensemble([(w.lattice, w.ibm) for w in ws])
+            ens.steps(4)
+            capi.call("bl_sync")
+            ms = ens.time_steps(n)
+            print("ensemble %d boxes: batched %s, %.4f ms per step" % (n_boxes, ens.batched, ms / n), flush=True)
+            ens.close()
+            for w in ws:
+                w.ibm.close()
+                w.lattice.close()
+            i += 3
         elif cmd == "copy":
             import ctypes
             nbytes, n = int(argv[i + 1]), int(argv[i + 2])
