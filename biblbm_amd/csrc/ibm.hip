@@ -2720,6 +2720,17 @@ __global__ void __launch_bounds__(256) k_spread_interp_batch(const LayeredEntry<
     else interp_body<typename Pop<S>::C, SW, true, R>(bx - E.nsp, E.nip, E.interp);
 }
 
+// The interpolation with advection alone (batches whose velocity staging
+// rides on the collision: it runs after the collide-and-stream).
+template <class S, int SW, class R>
+__global__ void __launch_bounds__(256) k_interp_batch(const LayeredEntry<R>* __restrict__ tab,
+                                                      const int* __restrict__ base, int nbox)
+{
+    const int box = batch_box(base, nbox, (int)blockIdx.x);
+    const LayeredEntry<R>& E = tab[box];
+    interp_body<typename Pop<S>::C, SW, true, R>((int)blockIdx.x - base[box], E.nip, E.interp);
+}
+
 template <class R>
 __global__ void __launch_bounds__(256) k_geom_partial_batch(const LayeredEntry<R>* __restrict__ tab,
                                                             const int* __restrict__ base, int nbox)
@@ -2744,6 +2755,10 @@ struct EnsembleBatch {
     int storage = 0, sw = 4, gl = 4;
     int order[2] = {0, 0};  // collide workgroup order per AA parity (collide_block modes)
     bool any_ibm = false, any_vwall = false;
+    // velocity staging carried by the batched collision (every box has an
+    // immersed boundary): no staging pass, the interpolation after the
+    // collide-and-stream and the walls, as a single box's fused step
+    bool fused = false;
     void* dev = nullptr;  // entries and offsets of both steps
     struct Launch {
         size_t tab = 0, base = 0;  // byte offsets into dev
@@ -2752,7 +2767,7 @@ struct EnsembleBatch {
     };
     struct Step {
         bool swapped = false;  // the IBM stages' AA parity (before the collision)
-        Launch reset, bin_elem, gather_stage, spread_interp, geom_partial, geom_final, collide, vwall, vwall_u;
+        Launch reset, bin_elem, gather_stage, spread_interp, interp, geom_partial, geom_final, collide, vwall, vwall_u;
         CollidePlan c0;        // template choices of the collide launch
         bool vw_swapped = false;
     } step[2];
@@ -2784,20 +2799,32 @@ EnsembleBatch::Launch put_launch(std::vector<char>& img, size_t tab, const std::
 }
 
 template <class R>
-int batch_plan_ibm(const std::vector<bl_ibm>& gs, std::vector<char>& img, EnsembleBatch::Step& st)
+int batch_plan_ibm(const std::vector<bl_ibm>& gs, std::vector<char>& img, EnsembleBatch::Step& st, bool fused,
+                   std::vector<LatView>* emit)
 {
     std::vector<LayeredEntry<R>> E(gs.size());
     for (size_t k = 0; k < gs.size(); ++k) {
         const int rc = plan_layered<R>(gs[k], E[k], (int)gs.size());
         if (rc) return rc;
+        if (fused) {
+            // after the step's spread swap: the boundary's spare field and
+            // flags are the last collision's (what the staging reads)
+            LatView& v = (*emit)[k];
+            v.emit_u = gs[k]->ut;
+            v.emit_force = gs[k]->force_alt;
+            v.emit_flags = gs[k]->flags_alt;
+        }
     }
     const size_t tab = put(img, E.data(), E.size());
-    std::vector<int> nr, nbe, ngs, nsi, nch, npa;
+    std::vector<int> nr, nbe, ngs, nsi, nip, nch, npa;
     for (size_t k = 0; k < gs.size(); ++k) {
         nr.push_back(E[k].nr);
         nbe.push_back(E[k].nb + E[k].ne);
-        ngs.push_back(E[k].ng + E[k].nst);
-        nsi.push_back(E[k].nsp + E[k].nip);
+        // fused: the gather alone and the spreading alone (every workgroup
+        // index stays below the first stage's count)
+        ngs.push_back(E[k].ng + (fused ? 0 : E[k].nst));
+        nsi.push_back(E[k].nsp + (fused ? 0 : E[k].nip));
+        nip.push_back(fused ? E[k].nip : 0);
         nch.push_back(gs[k]->nchunk);
         npa.push_back(gs[k]->np);
     }
@@ -2805,13 +2832,19 @@ int batch_plan_ibm(const std::vector<bl_ibm>& gs, std::vector<char>& img, Ensemb
     st.bin_elem = put_launch(img, tab, nbe);
     st.gather_stage = put_launch(img, tab, ngs);
     st.spread_interp = put_launch(img, tab, nsi);
+    st.interp = put_launch(img, tab, nip);
     st.geom_partial = put_launch(img, tab, nch);
     st.geom_final = put_launch(img, tab, npa);
     return BL_OK;
 }
 
+// The immersed-boundary launches of one batched step: before the collision
+// (after = false) the binning with element forces, the gather (with the
+// staging unless fused) and the spreading (with the interpolation unless
+// fused), then the geometry unless fused; after the collision and the
+// walls (after = true, fused only) the interpolation and the geometry.
 template <class S>
-void batch_launch_ibm(const EnsembleBatch& B, const EnsembleBatch::Step& st, hipStream_t s)
+void batch_launch_ibm(const EnsembleBatch& B, const EnsembleBatch::Step& st, bool after, hipStream_t s)
 {
     using R = typename VRof<S>::T;
     const char* d = (const char*)B.dev;
@@ -2820,19 +2853,26 @@ void batch_launch_ibm(const EnsembleBatch& B, const EnsembleBatch::Step& st, hip
     auto go = [&](auto kern, const EnsembleBatch::Launch& l, int blk) {
         if (l.nblk) hipLaunchKernelGGL(kern, dim3(l.nblk), dim3(blk), 0, s, tab(l), base(l), l.nbox);
     };
-    go(k_reset_batch<R>, st.reset, 256);
-    go(k_bin_elem_batch<R>, st.bin_elem, 256);
-    auto gs = [&](auto sw_tag) {
-        constexpr bool SWP = decltype(sw_tag)::value;
-        if (B.gl == 2) go(k_gather_stage_batch<S, SWP, R, 2>, st.gather_stage, 256);
-        else if (B.gl == 8) go(k_gather_stage_batch<S, SWP, R, 8>, st.gather_stage, 256);
-        else go(k_gather_stage_batch<S, SWP, R, 4>, st.gather_stage, 256);
-    };
-    if (st.swapped) gs(std::true_type{});
-    else gs(std::false_type{});
-    if (B.sw == 4) go(k_spread_interp_batch<S, 4, R>, st.spread_interp, 256);
-    else if (B.sw == 3) go(k_spread_interp_batch<S, 3, R>, st.spread_interp, 256);
-    else go(k_spread_interp_batch<S, 2, R>, st.spread_interp, 256);
+    if (!after) {
+        go(k_reset_batch<R>, st.reset, 256);
+        go(k_bin_elem_batch<R>, st.bin_elem, 256);
+        auto gs = [&](auto sw_tag) {
+            constexpr bool SWP = decltype(sw_tag)::value;
+            if (B.gl == 2) go(k_gather_stage_batch<S, SWP, R, 2>, st.gather_stage, 256);
+            else if (B.gl == 8) go(k_gather_stage_batch<S, SWP, R, 8>, st.gather_stage, 256);
+            else go(k_gather_stage_batch<S, SWP, R, 4>, st.gather_stage, 256);
+        };
+        if (st.swapped) gs(std::true_type{});
+        else gs(std::false_type{});
+        if (B.sw == 4) go(k_spread_interp_batch<S, 4, R>, st.spread_interp, 256);
+        else if (B.sw == 3) go(k_spread_interp_batch<S, 3, R>, st.spread_interp, 256);
+        else go(k_spread_interp_batch<S, 2, R>, st.spread_interp, 256);
+        if (B.fused) return;
+    } else {
+        if (B.sw == 4) go(k_interp_batch<S, 4, R>, st.interp, 256);
+        else if (B.sw == 3) go(k_interp_batch<S, 3, R>, st.interp, 256);
+        else go(k_interp_batch<S, 2, R>, st.interp, 256);
+    }
     go(k_geom_partial_batch<R>, st.geom_partial, 256);
     go(k_geom_final_batch<R>, st.geom_final, 64);
 }
@@ -2868,6 +2908,7 @@ int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<
     B->sw = sw ? sw : 4;
     B->gl = gl ? gl : 4;
     B->any_ibm = !gs.empty();
+    B->fused = gs.size() == n && collide_plan(lats[0]).fm == 2;
     B->order[0] = order[0];
     B->order[1] = order[1];
     std::vector<char> img;
@@ -2875,17 +2916,27 @@ int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<
     for (int k = 0; k < 2 && rc == BL_OK; ++k) {
         EnsembleBatch::Step& st = B->step[k];
         st.swapped = (lats[0]->steps & 1) != 0;
+        // the collide views after the boundaries' plans (their spread swap
+        // hands each lattice this step's force field), with the staging
+        // pointers the plans recorded when fused
+        std::vector<LatView> emit(n);
         if (!gs.empty()) {
-            if (B->storage == BL_STORE_F64) rc = batch_plan_ibm<double>(gs, img, st);
-            else rc = batch_plan_ibm<float>(gs, img, st);
+            if (B->storage == BL_STORE_F64) rc = batch_plan_ibm<double>(gs, img, st, B->fused, &emit);
+            else rc = batch_plan_ibm<float>(gs, img, st, B->fused, &emit);
             if (rc) break;
         }
         std::vector<LatView> views;
         std::vector<int> cnt;
         st.c0 = collide_plan(lats[0]);
-        for (bl_lattice L : lats) {
-            const CollidePlan p = collide_plan(L);
-            views.push_back(p.v);
+        for (size_t k = 0; k < n; ++k) {
+            const bl_lattice L = lats[k];
+            LatView v = collide_plan(L).v;
+            if (B->fused) {
+                v.emit_u = emit[k].emit_u;
+                v.emit_force = emit[k].emit_force;
+                v.emit_flags = emit[k].emit_flags;
+            }
+            views.push_back(v);
             cnt.push_back((int)((L->ncell + 255) / 256));
             collide_commit(L);
         }
@@ -2924,20 +2975,28 @@ int ensemble_batch_launch(const EnsembleBatch* B, hipStream_t s)
     const char* d = (const char*)B->dev;
     for (int k = 0; k < 2; ++k) {
         const EnsembleBatch::Step& st = B->step[k];
+        auto ibm = [&](bool after) {
+            if (B->storage == BL_STORE_F64) batch_launch_ibm<double>(*B, st, after, s);
+            else if (B->storage == BL_STORE_F32) batch_launch_ibm<float>(*B, st, after, s);
+            else batch_launch_ibm<__half>(*B, st, after, s);
+        };
         if (B->any_ibm) {
-            if (B->storage == BL_STORE_F64) batch_launch_ibm<double>(*B, st, s);
-            else if (B->storage == BL_STORE_F32) batch_launch_ibm<float>(*B, st, s);
-            else batch_launch_ibm<__half>(*B, st, s);
+            ibm(false);
             BL_CHECK_LAUNCH();
         }
-        int rc = collide_batch_launch(st.c0, (const LatView*)(d + st.collide.tab), (const int*)(d + st.collide.base),
-                                      st.collide.nbox, st.collide.nblk, B->order[st.c0.odd ? 1 : 0], s);
+        int rc = collide_batch_launch(st.c0, B->fused, (const LatView*)(d + st.collide.tab),
+                                      (const int*)(d + st.collide.base), st.collide.nbox, st.collide.nblk,
+                                      B->order[st.c0.odd ? 1 : 0], s);
         if (rc) return rc;
         if (st.vwall.nblk &&
             (rc = vwall_batch_launch(B->storage, st.vw_swapped, (const VwallPlan*)(d + st.vwall.tab),
                                      (const int*)(d + st.vwall.base), (const int*)(d + st.vwall_u.base),
                                      st.vwall.nbox, st.vwall.nblk, st.vwall_u.nblk, s)))
             return rc;
+        if (B->fused) {
+            ibm(true);
+            BL_CHECK_LAUNCH();
+        }
     }
     return BL_OK;
 }

@@ -142,9 +142,10 @@ void collide_commit(bl_lattice_s* L);
 // agree on everything collide_batchable checks.
 bool collide_batchable(const CollidePlan& a, const CollidePlan& b);
 // `order`: how the batch's workgroups are dealt over the XCDs (the modes
-// of bl_lattice_set_launch_parity except BL_ORDER_ROWS)
-int collide_batch_launch(const CollidePlan& p0, const LatView* views, const int* base, int nbox, unsigned nblk,
-                         int order, hipStream_t s);
+// of bl_lattice_set_launch_parity except BL_ORDER_ROWS); `emit`: the views
+// carry velocity staging (force-field plans only)
+int collide_batch_launch(const CollidePlan& p0, bool emit, const LatView* views, const int* base, int nbox,
+                         unsigned nblk, int order, hipStream_t s);
 // velocity-wall cells of one box: its view, wall-cell count and the buffer
 // of the velocities its finite differences read
 struct VwallPlan {
