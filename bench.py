@@ -557,7 +557,7 @@ def main():
         progress("interactions done")
         line["ensemble"] = ensemble_leg(max(50, args.steps // 4), 10)
         progress("ensemble done")
-        line["precision"] = precision_leg(max(300, args.steps))
+        line["precision"] = precision_leg(max(500, args.steps), checkpoints=(100, 200, 300))
         progress("precision done")
     if not args.no_cpu_baseline:
         port = cpu_baseline(args.cpu_steps, args.cpu_budget)
