@@ -903,13 +903,18 @@ __device__ __forceinline__ void stage_body(unsigned bx, const StageArgs& A)
     const unsigned t = ((unsigned)(x >> 2) * lv.nty + (y >> 2)) * lv.ntz + (z >> 2);
     if (!touched[t]) return;
     const unsigned char m = lv.mask ? lv.mask[c] : (unsigned char)M_FLUID;
-    C f[Q];
+    // the moments as the collision forms them (working form of the storage),
+    // so that a staging carried by the collision gives the same values
+    C rb = C(0), j[3] = {C(0), C(0), C(0)};
     if (m == M_FLUID) {
         Nbr nb;
         if (SWAPPED) nb.init(lv, x, y, z);
-        load_post<S>(lv, nb, c, SWAPPED, f);
+        C f[Q];
+        load_post_w<S>(lv, nb, c, SWAPPED, f);
+        wmoments<S>(f, rb, j);
     }
-    ut[c] = staged_velocity<FT, C>(lv, lv.flags, lv.force, m, c, x, y, z, f);
+    const C invRho = C(1) / (rb + C(1));
+    ut[c] = staged_velocity_m<FT, C>(lv, lv.flags, lv.force, m, c, x, y, z, j, invRho);
 }
 
 template <class S, bool SWAPPED>

@@ -143,13 +143,12 @@ __device__ __forceinline__ void cs_update(const LatView& v, unsigned c, int x, i
     using C = typename Pop<S>::C;
     // the moments of the loaded populations, shared by the staging (EMIT)
     // and the collision: the same expressions either would form
-    const C rhoBar = sum_rho(f);
-    C jm[3];
-    sum_j(f, jm);
+    C rhoBar, jm[3];
+    wmoments<S>(f, rhoBar, jm);
     const C invRho = C(1) / (rhoBar + C(1));
     if (EMIT && in.emit) ((Vel4<C>*)v.emit_u)[c] = staged_velocity_f<C>(v, in.m, x, y, z, jm, invRho, in.Fe);
     if (in.m == M_FLUID || in.m == M_VWALL) {
-        collide_m<C, FORCE>(f, in.F, in.omega, rhoBar, jm, invRho);
+        wcollide<S, FORCE>(f, in.F, in.omega, rhoBar, jm, invRho);
     } else if (in.m == M_BB) {
 #pragma unroll
         for (int i = 1; i <= 9; ++i) {
@@ -198,11 +197,11 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const K k = odd_slot<K>(nb, inner, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i);
-            f[i] = (NT == 1 || (NT == 2 && cz(i) == 0)) ? P::template ld<K, true>(A, k, i) : P::template ld<K, false>(A, k, i);
+            f[i] = (NT == 1 || (NT == 2 && cz(i) == 0)) ? P::template ldw<K, true>(A, k, i) : P::template ldw<K, false>(A, k, i);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = P::template ld<unsigned, NT != 0>(A + i * st, c, i);
+        for (int i = 0; i < Q; ++i) f[i] = P::template ldw<unsigned, NT != 0>(A + i * st, c, i);
     }
 
     cs_update<S, FMODE != 0, EMIT>(v, c, x, y, z, f, cs_inputs<S, MASK, FMODE, PHASE, EMIT>(v, c, tf));
@@ -211,12 +210,12 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const K k = odd_slot<K>(nb, inner, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i));
-            if (NT == 1 || (NT == 2 && cz(i) == 0)) P::template st<K, true>(A, k, i, f[i]);
-            else P::template st<K, false>(A, k, i, f[i]);
+            if (NT == 1 || (NT == 2 && cz(i) == 0)) P::template stw<K, true>(A, k, i, f[i]);
+            else P::template stw<K, false>(A, k, i, f[i]);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) P::template st<unsigned, NT != 0>(A + opp(i) * st, c, i, f[i]);
+        for (int i = 0; i < Q; ++i) P::template stw<unsigned, NT != 0>(A + opp(i) * st, c, i, f[i]);
     }
 }
 
@@ -284,7 +283,7 @@ __global__ void __launch_bounds__(256) k_collide_zseg(LatView v)
     for (int i = 0; i < Q; ++i) {
         const int rb = R[1 - cx(i)][1 - cy(i)];
         const unsigned base = rb >= 0 ? (unsigned)opp(i) * st + (unsigned)rb : (unsigned)i * st + own;
-        f[i] = P::template ld<unsigned, NT>(A + base, (unsigned)z, i);
+        f[i] = P::template ldw<unsigned, NT>(A + base, (unsigned)z, i);
     }
     // the cell's other inputs load beside the populations, not after the
     // barrier below, which no load crosses
@@ -296,8 +295,8 @@ __global__ void __launch_bounds__(256) k_collide_zseg(LatView v)
         const unsigned base = rb >= 0 ? (unsigned)opp(i) * st + (unsigned)rb : (unsigned)i * st + own;
         C* r = rot + zslot(i) * W;
         r[l + 1] = f[i];
-        if (cz(i) > 0 && l == 0) r[0] = P::template ld<unsigned, NT>(A + base, zlo, i);
-        if (cz(i) < 0 && l == B - 1) r[B + 1] = P::template ld<unsigned, NT>(A + base, zhi, i);
+        if (cz(i) > 0 && l == 0) r[0] = P::template ldw<unsigned, NT>(A + base, zlo, i);
+        if (cz(i) < 0 && l == B - 1) r[B + 1] = P::template ldw<unsigned, NT>(A + base, zhi, i);
     }
     __syncthreads();
 #pragma unroll
@@ -315,13 +314,13 @@ __global__ void __launch_bounds__(256) k_collide_zseg(LatView v)
         const int rb = R[1 + cx(i)][1 + cy(i)];
         const unsigned base = rb >= 0 ? (unsigned)i * st + (unsigned)rb : (unsigned)opp(i) * st + own;
         if (cz(i) == 0) {
-            P::template st<unsigned, NT>(A + base, (unsigned)z, i, f[i]);
+            P::template stw<unsigned, NT>(A + base, (unsigned)z, i, f[i]);
             continue;
         }
         rot[zslot(i) * W + l + 1] = f[i];
         // the value leaving the segment
-        if (rb >= 0 && cz(i) > 0 && l == B - 1) P::template st<unsigned, NT>(A + base, zhi, i, f[i]);
-        if (rb >= 0 && cz(i) < 0 && l == 0) P::template st<unsigned, NT>(A + base, zlo, i, f[i]);
+        if (rb >= 0 && cz(i) > 0 && l == B - 1) P::template stw<unsigned, NT>(A + base, zhi, i, f[i]);
+        if (rb >= 0 && cz(i) < 0 && l == 0) P::template stw<unsigned, NT>(A + base, zlo, i, f[i]);
     }
     __syncthreads();
 #pragma unroll
@@ -331,7 +330,7 @@ __global__ void __launch_bounds__(256) k_collide_zseg(LatView v)
         const unsigned base = rb >= 0 ? (unsigned)i * st + (unsigned)rb : (unsigned)opp(i) * st + own;
         const int src = l - cz(i);  // the lane whose value lands on slot z
         const C w = rot[zslot(i) * W + src + 1];
-        if (rb < 0 || (src >= 0 && src < B)) P::template st<unsigned, NT>(A + base, (unsigned)z, i, rb >= 0 ? w : f[i]);
+        if (rb < 0 || (src >= 0 && src < B)) P::template stw<unsigned, NT>(A + base, (unsigned)z, i, rb >= 0 ? w : f[i]);
     }
 }
 

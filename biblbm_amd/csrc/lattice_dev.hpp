@@ -27,10 +27,17 @@ __device__ __forceinline__ void st_nt(__half* p, __half v)
 
 template <class S>
 struct Pop;
+// Each storage also loads and stores its populations in the collision's
+// working form (ldw / stw): fBar_i itself for FP64 and FP32, and for the
+// FP16-shifted storage the stored value g_i = fBar_i / t_i * 2^10 as a
+// float (kRaw): the weights t_i and the scale then fold into the moments
+// and the collision's constants (wmoments, wcollide) instead of costing
+// two multiplies per population.
 template <>
 struct Pop<double> {
     using C = double;
     using F = double;
+    static constexpr bool kRaw = false;
     template <class K, bool NT = false>
     __device__ static C ld(const double* A, K k, int) { return NT ? ld_nt(A + k) : A[k]; }
     __device__ static double enc(C v, int) { return v; }
@@ -41,11 +48,16 @@ struct Pop<double> {
         if (NT) st_nt(A + k, v);
         else A[k] = v;
     }
+    template <class K, bool NT = false>
+    __device__ static C ldw(const double* A, K k, int i) { return ld<K, NT>(A, k, i); }
+    template <class K, bool NT = false>
+    __device__ static void stw(double* A, K k, int i, C v) { st<K, NT>(A, k, i, v); }
 };
 template <>
 struct Pop<float> {
     using C = float;
     using F = float;
+    static constexpr bool kRaw = false;
     template <class K, bool NT = false>
     __device__ static C ld(const float* A, K k, int) { return NT ? ld_nt(A + k) : A[k]; }
     __device__ static float enc(C v, int) { return v; }
@@ -56,11 +68,16 @@ struct Pop<float> {
         if (NT) st_nt(A + k, v);
         else A[k] = v;
     }
+    template <class K, bool NT = false>
+    __device__ static C ldw(const float* A, K k, int i) { return ld<K, NT>(A, k, i); }
+    template <class K, bool NT = false>
+    __device__ static void stw(float* A, K k, int i, C v) { st<K, NT>(A, k, i, v); }
 };
 template <>
 struct Pop<__half> {
     using C = float;
     using F = float;
+    static constexpr bool kRaw = true;
     template <class K, bool NT = false>
     __device__ static C ld(const __half* A, K k, int i)
     {
@@ -72,6 +89,18 @@ struct Pop<__half> {
     __device__ static void st(__half* A, K k, int i, C v)
     {
         const __half h = enc(v, i);
+        if (NT) st_nt(A + k, h);
+        else A[k] = h;
+    }
+    template <class K, bool NT = false>
+    __device__ static C ldw(const __half* A, K k, int)
+    {
+        return __half2float(NT ? ld_nt(A + k) : A[k]);
+    }
+    template <class K, bool NT = false>
+    __device__ static void stw(__half* A, K k, int, C v)
+    {
+        const __half h = __float2half_rn(v);
         if (NT) st_nt(A + k, h);
         else A[k] = h;
     }
@@ -254,6 +283,20 @@ __device__ __forceinline__ void load_post(const LatView& v, const Nbr& n, long c
     }
 }
 
+// The same in the collision's working form (Pop<S>::ldw).
+template <class S>
+__device__ __forceinline__ void load_post_w(const LatView& v, const Nbr& n, long c, bool swapped,
+                                            typename Pop<S>::C f[Q])
+{
+    const S* A = (const S*)v.pops;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int pl;
+        const long k = post_slot(v, n, c, i, swapped, pl);
+        f[i] = Pop<S>::template ldw<unsigned>(A + pl * v.stride, (unsigned)k, i);
+    }
+}
+
 template <class S>
 __device__ __forceinline__ void store_post(const LatView& v, const Nbr& n, long c, bool swapped,
                                            const typename Pop<S>::C f[Q])
@@ -396,14 +439,111 @@ __device__ __forceinline__ void collide_m(C f[Q], const C F[3], C omega, C rhoBa
     }
 }
 
-template <class C, bool FORCE>
-__device__ __forceinline__ void collide(C f[Q], const C F[3], C omega)
+// The collision in the FP16-shifted storage's working form g_i = fBar_i /
+// t_i * 2^10 (Pop<__half>::ldw): the same BGK and Guo source as collide_m,
+// with every term's weight t_i cancelled against the 1 / t_i of the form
+// (the equilibrium's t_i omega becomes 2^10 omega; the Guo source's 1, 1/6
+// and 1/12, which are 3 t_i, become 3 * 2^10 (1 - omega/2)).
+template <bool FORCE>
+__device__ __forceinline__ void collide_raw(float g[Q], const float F[3], float omega, float rhoBar,
+                                            const float jm[3], float invRho)
 {
-    const C rhoBar = sum_rho(f);
-    C j[3];
-    sum_j(f, j);
-    const C invRho = C(1) / (rhoBar + C(1));
-    collide_m<C, FORCE>(f, F, omega, rhoBar, j, invRho);
+    using C = float;
+    const C s = C(kF16Scale);
+    const C j[3] = {jm[0], jm[1], jm[2]};
+    C u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = FORCE ? j[d] * invRho + F[d] / C(2) : j[d] * invRho;
+    const C rho = rhoBar + C(1);
+    const C jx = rho * u[0], jy = rho * u[1], jz = rho * u[2];
+
+    const C one_m_omega = C(1) - omega;
+    const C ws = s * omega;
+    const C jSqr = jx * jx + jy * jy + jz * jz;
+    const C kx = C(3) * jx, ky = C(3) * jy, kz = C(3) * jz;
+    const C kxSqr_ = invRho / C(2) * kx * kx;
+    const C kySqr_ = invRho / C(2) * ky * ky;
+    const C kzSqr_ = invRho / C(2) * kz * kz;
+    const C kxky_ = invRho * kx * ky;
+    const C kxkz_ = invRho * kx * kz;
+    const C kykz_ = invRho * ky * kz;
+    const C C1 = rhoBar + invRho * C(3) * jSqr;
+    C C2, C3;
+    C3 = -kxSqr_ - kySqr_ - kzSqr_;
+    g[0] = g[0] * one_m_omega + ws * (C1 + C3);
+#define BL_PAIR(a, b, c2, c3)                          \
+    C2 = (c2);                                         \
+    C3 = (c3);                                         \
+    g[a] = g[a] * one_m_omega + ws * (C1 + C2 + C3);   \
+    g[b] = g[b] * one_m_omega + ws * (C1 - C2 + C3);
+    BL_PAIR(1, 10, -kx, -kySqr_ - kzSqr_)
+    BL_PAIR(2, 11, -ky, -kxSqr_ - kzSqr_)
+    BL_PAIR(3, 12, -kz, -kxSqr_ - kySqr_)
+    BL_PAIR(4, 13, -kx - ky, kxky_ - kzSqr_)
+    BL_PAIR(5, 14, -kx + ky, -kxky_ - kzSqr_)
+    BL_PAIR(6, 15, -kx - kz, kxkz_ - kySqr_)
+    BL_PAIR(7, 16, -kx + kz, -kxkz_ - kySqr_)
+    BL_PAIR(8, 17, -ky - kz, kykz_ - kxSqr_)
+    BL_PAIR(9, 18, -ky + kz, -kykz_ - kxSqr_)
+#undef BL_PAIR
+
+    if (FORCE) {
+        const C K = C(3) * s * (C(1) - omega / C(2));
+        const C Fx = F[0], Fy = F[1], Fz = F[2];
+        const C ux = u[0], uy = u[1], uz = u[2];
+        g[0] += -K * (Fx * ux + Fy * uy + Fz * uz);
+        g[1] += K * (Fx * (C(-1) + 2 * ux) - Fy * uy - Fz * uz);
+        g[2] += -K * (Fx * ux + Fy * (C(1) - 2 * uy) + Fz * uz);
+        g[3] += -K * (Fx * ux + Fy * uy + Fz * (C(1) - 2 * uz));
+        g[4] += K * (Fx * (C(-1) + 2 * ux + 3 * uy) + Fy * (C(-1) + 2 * uy + 3 * ux) - Fz * uz);
+        g[5] += K * (Fx * (C(-1) + 2 * ux - 3 * uy) + Fy * (C(1) + 2 * uy - 3 * ux) - Fz * uz);
+        g[6] += K * (Fx * (C(-1) + 2 * ux + 3 * uz) - Fy * uy + Fz * (C(-1) + 2 * uz + 3 * ux));
+        g[7] += K * (Fx * (C(-1) + 2 * ux - 3 * uz) - Fy * uy + Fz * (C(1) + 2 * uz - 3 * ux));
+        g[8] += -K * (Fx * ux + Fy * (C(1) - 2 * uy - 3 * uz) + Fz * (C(1) - 2 * uz - 3 * uy));
+        g[9] += -K * (Fx * ux + Fy * (C(1) - 2 * uy + 3 * uz) + Fz * (C(-1) - 2 * uz + 3 * uy));
+        g[10] += K * (Fx * (C(1) + 2 * ux) - Fy * uy - Fz * uz);
+        g[11] += -K * (Fx * ux + Fy * (C(-1) - 2 * uy) + Fz * uz);
+        g[12] += -K * (Fx * ux + Fy * uy + Fz * (C(-1) - 2 * uz));
+        g[13] += K * (Fx * (C(1) + 2 * ux + 3 * uy) + Fy * (C(1) + 2 * uy + 3 * ux) - Fz * uz);
+        g[14] += K * (Fx * (C(1) + 2 * ux - 3 * uy) + Fy * (C(-1) + 2 * uy - 3 * ux) - Fz * uz);
+        g[15] += K * (Fx * (C(1) + 2 * ux + 3 * uz) - Fy * uy + Fz * (C(1) + 2 * uz + 3 * ux));
+        g[16] += K * (Fx * (C(1) + 2 * ux - 3 * uz) - Fy * uy + Fz * (C(-1) + 2 * uz - 3 * ux));
+        g[17] += -K * (Fx * ux + Fy * (C(-1) - 2 * uy - 3 * uz) + Fz * (C(-1) - 2 * uz - 3 * uy));
+        g[18] += -K * (Fx * ux + Fy * (C(-1) - 2 * uy + 3 * uz) + Fz * (C(1) - 2 * uz + 3 * uy));
+    }
+}
+
+// The moments rhoBar = sum fBar_i and j = sum c_i fBar_i of populations
+// in a storage's working form: plain sums, or for the FP16-shifted form
+// the axis and diagonal sums weighted by t_i / 2^10.
+template <class S>
+__device__ __forceinline__ void wmoments(const typename Pop<S>::C f[Q], typename Pop<S>::C& rhoBar,
+                                         typename Pop<S>::C j[3])
+{
+    using C = typename Pop<S>::C;
+    if constexpr (Pop<S>::kRaw) {
+        const C inv = C(1.0 / kF16Scale);
+        const C t0 = C(1.0 / 3.0), t1 = C(1.0 / 18.0), t4 = C(1.0 / 36.0);
+        const C ax = f[1] + f[2] + f[3] + f[10] + f[11] + f[12];
+        const C dg = f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+        rhoBar = (t0 * f[0] + t1 * ax + t4 * dg) * inv;
+        j[0] = (t1 * (f[10] - f[1]) + t4 * ((f[13] + f[14] + f[15] + f[16]) - (f[4] + f[5] + f[6] + f[7]))) * inv;
+        j[1] = (t1 * (f[11] - f[2]) + t4 * ((f[5] + f[13] + f[17] + f[18]) - (f[4] + f[8] + f[9] + f[14]))) * inv;
+        j[2] = (t1 * (f[12] - f[3]) + t4 * ((f[7] + f[9] + f[15] + f[17]) - (f[6] + f[8] + f[16] + f[18]))) * inv;
+    } else {
+        rhoBar = sum_rho(f);
+        sum_j(f, j);
+    }
+}
+
+// The collision of populations in a storage's working form.
+template <class S, bool FORCE>
+__device__ __forceinline__ void wcollide(typename Pop<S>::C f[Q], const typename Pop<S>::C F[3],
+                                         typename Pop<S>::C omega, typename Pop<S>::C rhoBar,
+                                         const typename Pop<S>::C j[3], typename Pop<S>::C invRho)
+{
+    if constexpr (Pop<S>::kRaw) collide_raw<FORCE>(f, F, omega, rhoBar, j, invRho);
+    else collide_m<typename Pop<S>::C, FORCE>(f, F, omega, rhoBar, j, invRho);
 }
 
 // ------------------------------------------------------ velocity staging
@@ -415,14 +555,14 @@ struct alignas(4 * sizeof(C)) Vel4 {
 };
 
 // The progs' computeVelocity value of one cell (u = j/rho + F/2 from the
-// post-stream populations f and the last collision's force,
+// post-stream populations' moments and the last collision's force,
 // src/dataProcessors/dataAnalysisFunctional3D.hh:772-784; zero on
 // bounce-back / no-dynamics cells, the wall velocity on velocity-wall
-// cells).  (flags, force) is the last collision's force field, with the
-// last collision's body force in the tiles it does not flag.
-// The same from the cell's moments (j = sum_j(f), invRho = 1 / (sum_rho(f)
-// + 1)), which the collision that carries the staging forms anyway, and
-// (staged_velocity_f) from the force at the cell as well.
+// cells), from the cell's moments j and invRho = 1 / (rhoBar + 1) as
+// wmoments forms them (the collision that carries the staging forms them
+// anyway): staged_velocity_f with the force at the cell, staged_velocity_m
+// reading it from (flags, force), the last collision's force field, with
+// the last collision's body force in the tiles it does not flag.
 template <class C>
 __device__ __forceinline__ Vel4<C> staged_velocity_f(const LatView& v, unsigned char m, int x, int y, int z,
                                                      const C j[3], C invRho, const C Fc[3])
@@ -446,17 +586,6 @@ __device__ __forceinline__ Vel4<C> staged_velocity_m(const LatView& v, const uns
     C Fc[3] = {C(0), C(0), C(0)};
     if (m == M_FLUID) cell_force_in<FT, C>(v, v.g_last, flags, force, c, x, y, z, Fc);
     return staged_velocity_f<C>(v, m, x, y, z, j, invRho, Fc);
-}
-
-template <class FT, class C>
-__device__ __forceinline__ Vel4<C> staged_velocity(const LatView& v, const unsigned char* flags, const void* force,
-                                                   unsigned char m, long c, int x, int y, int z, const C f[Q])
-{
-    const C rb = sum_rho(f);
-    C j[3];
-    sum_j(f, j);
-    const C invRho = C(1) / (rb + C(1));
-    return staged_velocity_m<FT, C>(v, flags, force, m, c, x, y, z, j, invRho);
 }
 
 }  // namespace bl
