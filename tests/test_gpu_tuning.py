@@ -170,7 +170,8 @@ def test_row_orders_are_bitwise(device, shape, periodic, block, storage):
     ((6, 5, 64), (1, 1, 1), "f64", 64, 0, 1), ((7, 4, 128), (0, 1, 1), "f64", 64, 64, 0),
     ((4, 4, 384), (1, 1, 1), "f64", 128, 64, 1), ((5, 6, 192), (1, 0, 1), "f32", 64, 1, 1),
     ((4, 5, 64), (0, 0, 1), "f16s", 64, 4, 0), ((3, 4, 512), (1, 1, 1), "f32", 256, 16, 2),
-    ((5, 4, 384), (0, 1, 1), "f16s", 192, 0, 1), ((4, 3, 256), (1, 1, 1), "f64", 256, 0, 0)])
+    ((5, 4, 384), (0, 1, 1), "f16s", 192, 0, 1), ((4, 3, 256), (1, 1, 1), "f64", 256, 0, 0),
+    ((3, 3, 64), (1, 0, 1), "f16s", 64, 0, 0), ((3, 5, 128), (0, 1, 1), "f32", 128, 4, 1)])
 def test_z_segment_kernel_is_bitwise(device, shape, periodic, storage, lanes, order, nt):
     """The odd step's z-segment kernel (block -B: one workgroup per B cells
     of a z-row, every access z-aligned, the z shift through LDS; one or
