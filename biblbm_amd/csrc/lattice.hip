@@ -193,7 +193,19 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
     // arrive while the populations load
     const TileFlags tf = cs_flags<FMODE, EMIT>(v, x, y, z);
     C f[Q];
-    if (ODD) {
+    // inner waves of a lattice periodic along z: every neighbour exists (x
+    // and y one cell or more from their faces, z wraps), so no address needs
+    // the reflection's select (a uniform branch; the other waves take the
+    // general form)
+    const bool all_valid = ODD && inner && v.per[2];
+    if (ODD && all_valid) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            bool valid;
+            const K k = (K)opp(i) * (K)st + (K)(c + nb.off_inner(-cx(i), -cy(i), -cz(i), sx, sy, valid));
+            f[i] = (NT == 1 || (NT == 2 && cz(i) == 0)) ? P::template ldw<K, true>(A, k, i) : P::template ldw<K, false>(A, k, i);
+        }
+    } else if (ODD) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const K k = odd_slot<K>(nb, inner, sx, sy, c, st, -cx(i), -cy(i), -cz(i), opp(i), i);
@@ -206,7 +218,15 @@ __device__ __forceinline__ void cs_cell(const LatView& v, unsigned c, int x, int
 
     cs_update<S, FMODE != 0, EMIT>(v, c, x, y, z, f, cs_inputs<S, MASK, FMODE, PHASE, EMIT>(v, c, tf));
 
-    if (ODD) {
+    if (ODD && all_valid) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            bool valid;
+            const K k = (K)i * (K)st + (K)(c + nb.off_inner(cx(i), cy(i), cz(i), sx, sy, valid));
+            if (NT == 1 || (NT == 2 && cz(i) == 0)) P::template stw<K, true>(A, k, i, f[i]);
+            else P::template stw<K, false>(A, k, i, f[i]);
+        }
+    } else if (ODD) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const K k = odd_slot<K>(nb, inner, sx, sy, c, st, cx(i), cy(i), cz(i), i, opp(i));
