@@ -85,6 +85,7 @@ SIGNATURES = {
     "bl_ensemble_set_batched": [_P, _I],
     "bl_ensemble_is_batched": [_P, _PI],
     "bl_ensemble_tune": [_P, _I, ctypes.c_char_p, _I],
+    "bl_ensemble_set_launch": [_P, _I, _I, _I],
     "bl_ibm_get_launch": [_P, _P],
     "bl_tune_ibm": [_P, _I, ctypes.c_char_p, _I],
     "bl_ibm_set_layered": [_P, _I],

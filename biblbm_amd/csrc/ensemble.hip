@@ -36,6 +36,7 @@ struct bl_ensemble_s {
     bl::EnsembleBatch* batch = nullptr;  // the tables of the captured batched steps
     int mode = -1;                       // -1 batched when possible, 0 branches, 1 batched only
     int order[2] = {0, 0};               // batched collide workgroup order per AA parity
+    int nt[2] = {0, 0};                  // and nontemporal level
 };
 
 namespace bl {
@@ -114,7 +115,7 @@ int capture(bl_ensemble e)
     ensemble_batch_free(e->batch);
     e->batch = nullptr;
     if (e->mode != 0) {
-        int rc = ensemble_batch_build(e->lats, e->ibms, e->order, &e->batch);
+        int rc = ensemble_batch_build(e->lats, e->ibms, e->order, e->nt, &e->batch);
         if (rc) return rc;
         if (e->batch) return capture_batched(e);
         if (e->mode == 1)
@@ -255,67 +256,99 @@ int bl_ensemble_set_batched(bl_ensemble e, int mode)
 int bl_ensemble_tune(bl_ensemble e, int n_steps, char* report, int cap)
 {
     if (!e || n_steps < 2 || n_steps % 2) return fail(BL_ERR_ARG, "bl_ensemble_tune: n_steps must be even and >= 2");
-    // the batched collide's workgroup order, per AA parity in turn (the
-    // other held), over real steps
+    // the batched collide's workgroup order, then its nontemporal level, per
+    // AA parity in turn (the other held), over real steps
     static const int orders[] = {0, 1, 4, 16, 64, 256};
-    const int keep[2] = {e->order[0], e->order[1]};
-    int best[2] = {keep[0], keep[1]};
+    const int keep[4] = {e->order[0], e->order[1], e->nt[0], e->nt[1]};
+    int best[4] = {keep[0], keep[1], keep[2], keep[3]};  // order even, odd; nt even, odd
     double best_ms = 1e30;
     int steps_run = 0, rc = BL_OK;
     std::string rep = "{\"trials\": [";
     bool first = true;
-    auto trial = [&](int even, int odd, double& ms) -> int {
-        e->order[0] = even;
-        e->order[1] = odd;
-        if (e->graph) {  // recapture with the new order
+    auto set = [&](const int t[4]) -> int {
+        const bool same = e->order[0] == t[0] && e->order[1] == t[1] && e->nt[0] == t[2] && e->nt[1] == t[3];
+        e->order[0] = t[0];
+        e->order[1] = t[1];
+        e->nt[0] = t[2];
+        e->nt[1] = t[3];
+        if (e->graph && !same) {  // recapture with the new launch shape
             BL_HIP(hipStreamSynchronize(e->stream));
             hipGraphExecDestroy(e->graph);
             e->graph = nullptr;
         }
+        return BL_OK;
+    };
+    auto trial = [&](const int t[4], double& ms) -> int {
         int r;
+        if ((r = set(t))) return r;
         if ((r = bl_ensemble_steps(e, n_steps, 1))) return r;
         if ((r = bl_ensemble_time_steps(e, n_steps, 1, &ms))) return r;
         steps_run += 2 * n_steps;
         ms /= n_steps;
         return BL_OK;
     };
+    auto run = [&](const int t[4]) -> int {
+        double ms = 0;
+        const int r = trial(t, ms);
+        if (r) return r;
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s{\"even\": %d, \"odd\": %d, \"nt_even\": %d, \"nt_odd\": %d, "
+                 "\"ms_per_step\": %.6f}", first ? "" : ", ", t[0], t[1], t[2], t[3], ms);
+        rep += buf;
+        first = false;
+        if (ms < best_ms) {
+            best_ms = ms;
+            for (int k = 0; k < 4; ++k) best[k] = t[k];
+        }
+        return BL_OK;
+    };
     for (int p = 0; p < 2 && rc == BL_OK; ++p)
         for (int o : orders) {
-            double ms = 0;
-            const int ev = p == 0 ? o : best[0], od = p == 1 ? o : best[1];
-            if ((rc = trial(ev, od, ms))) break;
-            char buf[128];
-            snprintf(buf, sizeof(buf), "%s{\"even\": %d, \"odd\": %d, \"ms_per_step\": %.6f}", first ? "" : ", ", ev,
-                     od, ms);
-            rep += buf;
-            first = false;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best[0] = ev;
-                best[1] = od;
-            }
+            int t[4] = {best[0], best[1], best[2], best[3]};
+            t[p] = o;
+            if ((rc = run(t))) break;
+        }
+    for (int p = 0; p < 2 && rc == BL_OK; ++p)
+        for (int nt = 0; nt <= 2; ++nt) {
+            int t[4] = {best[0], best[1], best[2], best[3]};
+            if (t[2 + p] == nt) continue;  // timed already
+            t[2 + p] = nt;
+            if ((rc = run(t))) break;
         }
     if (rc) {
-        e->order[0] = keep[0];
-        e->order[1] = keep[1];
+        set(keep);
         return rc;
     }
-    if (e->graph && (best[0] != e->order[0] || best[1] != e->order[1])) {
-        hipStreamSynchronize(e->stream);
-        hipGraphExecDestroy(e->graph);
-        e->graph = nullptr;
-    }
-    e->order[0] = best[0];
-    e->order[1] = best[1];
+    if ((rc = set(best))) return rc;
     char buf[256];
     snprintf(buf, sizeof(buf), "], \"steps_run\": %d, \"batched\": %d, \"best\": {\"even\": %d, \"odd\": %d, "
-             "\"ms_per_step\": %.6f}}", steps_run, e->batch ? 1 : 0, best[0], best[1], best_ms);
+             "\"nt_even\": %d, \"nt_odd\": %d, \"ms_per_step\": %.6f}}", steps_run, e->batch ? 1 : 0, best[0],
+             best[1], best[2], best[3], best_ms);
     rep += buf;
     if (report && cap > 0) {
         const size_t n = std::min<size_t>(rep.size(), (size_t)cap - 1);
         rep.copy(report, n);
         report[n] = '\0';
     }
+    return BL_OK;
+}
+
+int bl_ensemble_set_launch(bl_ensemble e, int parity, int order, int nontemporal)
+{
+    if (!e) return fail(BL_ERR_ARG, "null ensemble");
+    if (parity != 0 && parity != 1) return fail(BL_ERR_ARG, "bl_ensemble_set_launch: parity must be 0 or 1");
+    if (order < 0 || order > (1 << 20) || order == BL_ORDER_ROWS || (order > 1 && (order & (order - 1))))
+        return fail(BL_ERR_ARG, "bl_ensemble_set_launch: order must be 0, 1 or a power of two up to 2^20");
+    if (nontemporal < 0 || nontemporal > 2)
+        return fail(BL_ERR_ARG, "bl_ensemble_set_launch: nontemporal must be 0, 1 or 2");
+    if (e->order[parity] == order && e->nt[parity] == nontemporal) return BL_OK;
+    if (e->graph) {  // the next steps capture with the new launch
+        BL_HIP(hipStreamSynchronize(e->stream));
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    e->order[parity] = order;
+    e->nt[parity] = nontemporal;
     return BL_OK;
 }
 

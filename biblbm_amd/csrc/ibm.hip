@@ -2759,6 +2759,7 @@ __global__ void __launch_bounds__(64) k_geom_final_batch(const LayeredEntry<R>* 
 struct EnsembleBatch {
     int storage = 0, sw = 4, gl = 4;
     int order[2] = {0, 0};  // collide workgroup order per AA parity (collide_block modes)
+    int nt[2] = {0, 0};     // and nontemporal level
     bool any_ibm = false, any_vwall = false;
     // velocity staging carried by the batched collision (every box has an
     // immersed boundary): no staging pass, the interpolation after the
@@ -2885,7 +2886,7 @@ void batch_launch_ibm(const EnsembleBatch& B, const EnsembleBatch::Step& st, boo
 }  // namespace
 
 int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<bl_ibm>& ibms, const int order[2],
-                         EnsembleBatch** out)
+                         const int nt[2], EnsembleBatch** out)
 {
     *out = nullptr;
     const size_t n = lats.size();
@@ -2916,6 +2917,8 @@ int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<
     B->fused = gs.size() == n && collide_plan(lats[0]).fm == 2;
     B->order[0] = order[0];
     B->order[1] = order[1];
+    B->nt[0] = nt[0];
+    B->nt[1] = nt[1];
     std::vector<char> img;
     int rc = BL_OK;
     for (int k = 0; k < 2 && rc == BL_OK; ++k) {
@@ -2991,7 +2994,7 @@ int ensemble_batch_launch(const EnsembleBatch* B, hipStream_t s)
         }
         int rc = collide_batch_launch(st.c0, B->fused, (const LatView*)(d + st.collide.tab),
                                       (const int*)(d + st.collide.base), st.collide.nbox, st.collide.nblk,
-                                      B->order[st.c0.odd ? 1 : 0], s);
+                                      B->order[st.c0.odd ? 1 : 0], B->nt[st.c0.odd ? 1 : 0], s);
         if (rc) return rc;
         if (st.vwall.nblk &&
             (rc = vwall_batch_launch(B->storage, st.vw_swapped, (const VwallPlan*)(d + st.vwall.tab),

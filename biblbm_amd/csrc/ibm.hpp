@@ -187,9 +187,10 @@ int ibm_step_launch(bl_ibm g, bool forked);
 // after ensemble_batch_build, which plans them).  *out stays null when the
 // boxes cannot be batched (differing storage, parity, stage layout, ...).
 struct EnsembleBatch;
-// order: the batched collide's workgroup order per AA parity
+// order, nt: the batched collide's workgroup order and nontemporal level
+// per AA parity
 int ensemble_batch_build(const std::vector<bl_lattice>& lats, const std::vector<bl_ibm>& ibms, const int order[2],
-                         EnsembleBatch** out);
+                         const int nt[2], EnsembleBatch** out);
 int ensemble_batch_launch(const EnsembleBatch* b, hipStream_t s);
 void ensemble_batch_free(EnsembleBatch* b);
 // Reads the overflow flags of the last steps (synchronises the stream).

@@ -504,10 +504,11 @@ int collide_stream_launch(bl_lattice_s* L)
 // the box offsets, uniform over the workgroup) and runs local block
 // b - base[k] of that box with its own view.  Every box shares the template
 // choices (storage, parity, mask, force mode); the launch shape is 256
-// threads in hardware order, which gives the same values as any other.
+// threads, in the order `order` deals them over the XCDs (any order gives
+// the same values), with the nontemporal level NT of cs_cell.
 // EMIT: every box's view carries its velocity staging (emit_u and the last
 // collision's field), as a single lattice's step with fused staging.
-template <class S, bool ODD, bool MASK, int FMODE, bool EMIT = false>
+template <class S, bool ODD, bool MASK, int FMODE, bool EMIT = false, int NT = 0>
 __global__ void __launch_bounds__(256) k_collide_batch(const LatView* __restrict__ views, const int* __restrict__ base,
                                                        int nbox, int order)
 {
@@ -519,34 +520,44 @@ __global__ void __launch_bounds__(256) k_collide_batch(const LatView* __restrict
     if (c >= v.ncell) return;
     int x, y, z;
     decode(c, v, x, y, z);
-    cs_cell<S, ODD, MASK, FMODE, false, unsigned, EMIT, 0>(v, c, x, y, z);
+    cs_cell<S, ODD, MASK, FMODE, false, unsigned, EMIT, NT>(v, c, x, y, z);
+}
+
+template <class S, bool ODD, bool MASK, int FM, bool EMIT>
+void launch_collide_batch3(int nt, const LatView* views, const int* base, int nbox, unsigned nblk, int order,
+                           hipStream_t s)
+{
+    if (nt == 1)
+        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, FM, EMIT, 1>), dim3(nblk), dim3(256), 0, s, views, base,
+                           nbox, order);
+    else if (nt == 2)
+        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, FM, EMIT, 2>), dim3(nblk), dim3(256), 0, s, views, base,
+                           nbox, order);
+    else
+        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, FM, EMIT, 0>), dim3(nblk), dim3(256), 0, s, views, base,
+                           nbox, order);
 }
 
 template <class S, bool ODD, bool MASK>
 void launch_collide_batch2(int fm, bool emit, const LatView* views, const int* base, int nbox, unsigned nblk, int order,
-                           hipStream_t s)
+                           int nt, hipStream_t s)
 {
-    if (fm == 0)
-        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 0>), dim3(nblk), dim3(256), 0, s, views, base, nbox, order);
-    else if (fm == 1)
-        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 1>), dim3(nblk), dim3(256), 0, s, views, base, nbox, order);
-    else if (emit)
-        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 2, true>), dim3(nblk), dim3(256), 0, s, views, base, nbox,
-                           order);
-    else
-        hipLaunchKernelGGL((k_collide_batch<S, ODD, MASK, 2>), dim3(nblk), dim3(256), 0, s, views, base, nbox, order);
+    if (fm == 0) launch_collide_batch3<S, ODD, MASK, 0, false>(nt, views, base, nbox, nblk, order, s);
+    else if (fm == 1) launch_collide_batch3<S, ODD, MASK, 1, false>(nt, views, base, nbox, nblk, order, s);
+    else if (emit) launch_collide_batch3<S, ODD, MASK, 2, true>(nt, views, base, nbox, nblk, order, s);
+    else launch_collide_batch3<S, ODD, MASK, 2, false>(nt, views, base, nbox, nblk, order, s);
 }
 
 template <class S>
 void launch_collide_batch1(bool odd, bool mask, int fm, bool emit, const LatView* views, const int* base, int nbox,
-                           unsigned nblk, int order, hipStream_t s)
+                           unsigned nblk, int order, int nt, hipStream_t s)
 {
     if (odd) {
-        if (mask) launch_collide_batch2<S, true, true>(fm, emit, views, base, nbox, nblk, order, s);
-        else launch_collide_batch2<S, true, false>(fm, emit, views, base, nbox, nblk, order, s);
+        if (mask) launch_collide_batch2<S, true, true>(fm, emit, views, base, nbox, nblk, order, nt, s);
+        else launch_collide_batch2<S, true, false>(fm, emit, views, base, nbox, nblk, order, nt, s);
     } else {
-        if (mask) launch_collide_batch2<S, false, true>(fm, emit, views, base, nbox, nblk, order, s);
-        else launch_collide_batch2<S, false, false>(fm, emit, views, base, nbox, nblk, order, s);
+        if (mask) launch_collide_batch2<S, false, true>(fm, emit, views, base, nbox, nblk, order, nt, s);
+        else launch_collide_batch2<S, false, false>(fm, emit, views, base, nbox, nblk, order, nt, s);
     }
 }
 
@@ -558,14 +569,15 @@ bool collide_batchable(const CollidePlan& a, const CollidePlan& b)
 }
 
 int collide_batch_launch(const CollidePlan& p0, bool emit, const LatView* views, const int* base, int nbox,
-                         unsigned nblk, int order, hipStream_t s)
+                         unsigned nblk, int order, int nt, hipStream_t s)
 {
     if (emit && p0.fm != 2) return fail(BL_ERR_STATE, "batched collide: velocity staging needs the force field");
+    if (nt < 0 || nt > 2) return fail(BL_ERR_ARG, "batched collide: nontemporal level must be 0, 1 or 2");
     if (p0.storage == BL_STORE_F64)
-        launch_collide_batch1<double>(p0.odd, p0.mask, p0.fm, emit, views, base, nbox, nblk, order, s);
+        launch_collide_batch1<double>(p0.odd, p0.mask, p0.fm, emit, views, base, nbox, nblk, order, nt, s);
     else if (p0.storage == BL_STORE_F32)
-        launch_collide_batch1<float>(p0.odd, p0.mask, p0.fm, emit, views, base, nbox, nblk, order, s);
-    else launch_collide_batch1<__half>(p0.odd, p0.mask, p0.fm, emit, views, base, nbox, nblk, order, s);
+        launch_collide_batch1<float>(p0.odd, p0.mask, p0.fm, emit, views, base, nbox, nblk, order, nt, s);
+    else launch_collide_batch1<__half>(p0.odd, p0.mask, p0.fm, emit, views, base, nbox, nblk, order, nt, s);
     BL_CHECK_LAUNCH();
     return BL_OK;
 }

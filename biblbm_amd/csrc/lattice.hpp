@@ -143,9 +143,10 @@ void collide_commit(bl_lattice_s* L);
 bool collide_batchable(const CollidePlan& a, const CollidePlan& b);
 // `order`: how the batch's workgroups are dealt over the XCDs (the modes
 // of bl_lattice_set_launch_parity except BL_ORDER_ROWS); `emit`: the views
-// carry velocity staging (force-field plans only)
+// carry velocity staging (force-field plans only); `nt`: the nontemporal
+// level (0, 1, 2) of bl_lattice_set_launch_parity
 int collide_batch_launch(const CollidePlan& p0, bool emit, const LatView* views, const int* base, int nbox,
-                         unsigned nblk, int order, hipStream_t s);
+                         unsigned nblk, int order, int nt, hipStream_t s);
 // velocity-wall cells of one box: its view, wall-cell count and the buffer
 // of the velocities its finite differences read
 struct VwallPlan {

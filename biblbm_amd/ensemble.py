@@ -40,11 +40,17 @@ class Ensemble:
         return bool(out.value)
 
     def tune(self, n_steps=4):
-        """bl_ensemble_tune: the batched collide's workgroup order per AA
-        parity, over real steps; returns the report."""
+        """bl_ensemble_tune: the batched collide's workgroup order and
+        nontemporal level per AA parity, over real steps; returns the
+        report."""
         buf = ctypes.create_string_buffer(8192)
         call("bl_ensemble_tune", self.handle, int(n_steps), buf, len(buf))
         return json.loads(buf.value.decode())
+
+    def set_launch(self, parity, order, nontemporal=0):
+        """bl_ensemble_set_launch: the batched collide's workgroup order and
+        nontemporal level for steps of one AA parity, without tuning."""
+        call("bl_ensemble_set_launch", self.handle, int(parity), int(order), int(nontemporal))
 
     def time_steps(self, n, use_graph=True):
         ms = ctypes.c_double()

@@ -19,8 +19,9 @@ same table profiles the shapes the bench timed.
 `collide W S N` builds workload W (rbc_in_pipe, suspension,
 suspension_interactions, taylor_green or capsule_in_shear) with storage S, runs two coupled steps so that the force
 tiles are those of a running simulation, then N collide-and-stream launches.
-`coupled W S N` runs N eager coupled steps instead; `ensemble B N` steps
-the bench's ensemble of B shear boxes N times, batched; `collisions W S N` runs
+`coupled W S N` runs N eager coupled steps instead; `ensemble B N [EO OO EN ON]`
+steps the bench's ensemble of B shear boxes N times, batched, with the
+given collide orders and nontemporal levels; `collisions W S N` runs
 N collision-handler passes (Morse r_min 1, beta 1, strength 1e-3) on the
 vertices of workload W after two coupled steps.  `copy B N` runs N
 launches of a plain 16-byte-per-thread device copy of B bytes (the
@@ -81,13 +82,19 @@ def run(argv):
             W.lattice.close()
             i += 4
         elif cmd == "ensemble":
-            # `ensemble N S`: the bench's ensemble of N shear boxes, batched,
-            # four settle steps, then S timed steps (two per graph launch)
+            # `ensemble N S [EO OO EN ON]`: the bench's ensemble of N shear
+            # boxes, batched (collide orders EO / OO and nontemporal levels
+            # EN / ON for even / odd steps, default 0), four settle steps,
+            # then S timed steps (two per graph launch)
             from . import workloads
             from .ensemble import Ensemble
             n_boxes, n = int(argv[i + 1]), int(argv[i + 2])
             ws = workloads.shear_ensemble("f64", n_boxes=n_boxes)
             ens = Ensemble([(w.lattice, w.ibm) for w in ws])
+            shape = [int(a) for a in argv[i + 3:i + 7]] if len(argv) >= i + 7 and argv[i + 3].isdigit() else None
+            if shape:
+                ens.set_launch(0, shape[0], shape[2])
+                ens.set_launch(1, shape[1], shape[3])
             ens.steps(4)
             capi.call("bl_sync")
             ms = ens.time_steps(n)
@@ -96,7 +103,7 @@ def run(argv):
             for w in ws:
                 w.ibm.close()
                 w.lattice.close()
-            i += 3
+            i += 7 if shape else 3
         elif cmd == "copy":
             import ctypes
             nbytes, n = int(argv[i + 1]), int(argv[i + 2])

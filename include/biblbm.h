@@ -529,9 +529,15 @@ int bl_ensemble_is_batched(bl_ensemble e, int *batched);
  * reference): times n_steps (even) graph-replayed steps for each workgroup
  * order of the batched collide-and-stream launch (the modes of
  * bl_lattice_set_launch_parity except BL_ORDER_ROWS), per AA parity in
- * turn, and keeps the fastest.  The trials are real steps (the report says
- * how many); every order gives the same values. */
+ * turn, then for each nontemporal level (0, 1, 2) of that launch with the
+ * orders kept, and keeps the fastest.  The trials are real steps (the
+ * report says how many); every order and level gives the same values. */
 int bl_ensemble_tune(bl_ensemble e, int n_steps, char *report, int cap);
+/* Sets the batched collide's workgroup order (0, 1 or a power of two, as
+ * bl_lattice_set_launch_parity) and nontemporal level (0, 1, 2) for steps
+ * of AA parity `parity` without tuning (no counterpart in the reference);
+ * the next steps capture with it.  Every setting gives the same values. */
+int bl_ensemble_set_launch(bl_ensemble e, int parity, int order, int nontemporal);
 /* The same, returning the device time of the n steps in milliseconds. */
 int bl_ensemble_time_steps(bl_ensemble e, int n_steps, int use_graph, double *ms);
 

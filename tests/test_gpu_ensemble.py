@@ -163,19 +163,49 @@ def test_batched_mode_refuses_boxes_that_differ(device):
 
 
 def test_ensemble_tuner_keeps_the_trajectory(device):
-    """The ensemble tuner's trials (batched collide workgroup orders) are
-    real steps that give the values of the boxes stepped alone."""
+    """The ensemble tuner's trials (batched collide workgroup orders and
+    nontemporal levels) are real steps that give the values of the boxes
+    stepped alone."""
     from biblbm_amd.ensemble import Ensemble
     A, B = _boxes(), _boxes()
     ens = Ensemble([(w.lattice, w.ibm) for w in A])
     rep = ens.tune(2)
     assert rep["batched"] == 1
     assert rep["best"]["even"] in (0, 1, 4, 16, 64, 256) and rep["best"]["odd"] in (0, 1, 4, 16, 64, 256)
+    assert rep["best"]["nt_even"] in (0, 1, 2) and rep["best"]["nt_odd"] in (0, 1, 2)
+    assert len(rep["trials"]) == 2 * 6 + 2 * 2  # orders, then the two other levels, per parity
     n = rep["steps_run"]
     assert A[0].lattice.steps == n
     ens.steps(4)
     for w in B:
         w.ibm.coupled_steps(n + 4, use_graph=False)
+    for a, b in zip(A, B):
+        assert np.array_equal(a.ibm.positions(), b.ibm.positions())
+        assert np.array_equal(a.lattice.get_populations(), b.lattice.get_populations())
+    ens.close()
+    _close(A + B)
+
+
+@pytest.mark.parametrize("even,odd", [((256, 2), (0, 1)), ((16, 1), (64, 2))])
+def test_ensemble_set_launch_keeps_the_trajectory(device, even, odd):
+    """Fixed batched collide orders and nontemporal levels
+    (bl_ensemble_set_launch), changed between graph runs, give bitwise the
+    boxes stepped alone; out-of-range settings are refused."""
+    from biblbm_amd import capi
+    from biblbm_amd.ensemble import Ensemble
+    A, B = _boxes(), _boxes()
+    ens = Ensemble([(w.lattice, w.ibm) for w in A])
+    ens.set_launch(0, *even)
+    ens.set_launch(1, *odd)
+    ens.steps(4)
+    assert ens.batched
+    ens.set_launch(0, 0, 0)
+    ens.steps(2)
+    for bad in ((2, 0, 0), (0, 3, 0), (0, 6, 0), (1, 0, 3), (0, -1, 0)):
+        with pytest.raises(capi.BlError):
+            ens.set_launch(*bad)
+    for w in B:
+        w.ibm.coupled_steps(6, use_graph=False)
     for a, b in zip(A, B):
         assert np.array_equal(a.ibm.positions(), b.ibm.positions())
         assert np.array_equal(a.lattice.get_populations(), b.lattice.get_populations())
