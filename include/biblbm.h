@@ -534,8 +534,9 @@ int bl_ensemble_is_batched(bl_ensemble e, int *batched);
  * report says how many); every order and level gives the same values. */
 int bl_ensemble_tune(bl_ensemble e, int n_steps, char *report, int cap);
 /* Sets the batched collide's workgroup order (0, 1 or a power of two, as
- * bl_lattice_set_launch_parity) and nontemporal level (0, 1, 2) for steps
- * of AA parity `parity` without tuning (no counterpart in the reference);
+ * bl_lattice_set_launch_parity) and nontemporal level (0, 1, 2; on even
+ * steps, which move no population along z, 2 acts as 1) for steps of AA
+ * parity `parity` without tuning (no counterpart in the reference);
  * the next steps capture with it.  Every setting gives the same values. */
 int bl_ensemble_set_launch(bl_ensemble e, int parity, int order, int nontemporal);
 /* The same, returning the device time of the n steps in milliseconds. */
